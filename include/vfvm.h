@@ -1,0 +1,261 @@
+/*
+ * vfvm.h — C ABI of the MI355X-native vfvm3d hot path.
+ *
+ * Scope: the per-Delaunay-edge Voronoi-face assembly of the scalar transport
+ * operators into a fixed-pattern CSR matrix, and the restarted GMRES that solves
+ * it, with the steady/transient Picard drivers that call them.  Every entry point
+ * returns an int status (VFVM_OK = 0); the message of the last failure on the
+ * calling thread is returned by vfvm_last_error().
+ *
+ * Plain pointers and sizes only.  Host arrays passed in are copied; the caller
+ * keeps ownership.  A context owns one HIP stream on one device and is not
+ * thread-safe: use one context per host thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   Tetrahedron::update_tet            src/base/geom_tet.rs:58      -> vfvm_mesh_from_tets / vfvm_mesh_lattice
+ *   Variables::add_scldom_* / add_sclbnd_* / add_sclflw / add_vecdom_*
+ *                                      src/base/vars.rs:80-183      -> vfvm_scldom_add / vfvm_sclbnd_add / vfvm_vecdom_add / vfvm_sclflw_add
+ *   OperatorBase::apply / apply_time   src/operator/operator_base.rs:5-7,
+ *                                      src/operator/core/opscl_dom_time.rs:35 -> vfvm_op descriptors
+ *   LinearBase::update_sparse          src/linear/linear_base.rs:7  -> vfvm_pattern_build
+ *   SteadyBase::assemble_matrix + LinearBase::build_matrix
+ *                                      src/steady/steady_base.rs:61-66, src/linear/linear_gmres.rs:66-74 -> vfvm_assemble
+ *   LinearBase::solve_matrix           src/linear/linear_gmres.rs:76-84 -> vfvm_solve
+ *   SteadyBase::solve                  src/steady/steady_base.rs:16-135 -> vfvm_steady_solve
+ *   TransientBase::solve               src/transient/transient_base.rs:16-160 -> vfvm_transient_solve
+ */
+#ifndef VFVM_H
+#define VFVM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mirror the VFVChemError variants on this path, src/base/error.rs:5) ---- */
+enum vfvm_status {
+    VFVM_OK = 0,
+    VFVM_ERR_INVALID_ARGUMENT = 1,          /* null pointer, size or id out of range            */
+    VFVM_ERR_HIP = 2,                       /* HIP runtime error (message holds hipGetErrorString) */
+    VFVM_ERR_NO_EXTENSION = 3,              /* device code not loadable                          */
+    VFVM_ERR_TETRA_DEGENERATE_INPUT = 10,   /* TetraDegenerateInput   geom_tet.rs:88              */
+    VFVM_ERR_TETRA_POINTS_ON_BOUNDARY = 11, /* TetraPointsOnBoundary  geom_tet.rs:120             */
+    VFVM_ERR_TETRA_SINGULAR_GRADIENT = 12,  /* TetraSingularGradient  geom_tet.rs:284             */
+    VFVM_ERR_OPERATOR_NON_UNKNOWN = 20,     /* OperatorNonUnknownInLinearSystem operator_base.rs:15 */
+    VFVM_ERR_OPERATOR_USE_APPLY_TIME = 21,  /* OperatorUseApplyTime   opscl_dom_time.rs:66         */
+    VFVM_ERR_OPERATOR_UNSUPPORTED = 22,     /* operator combination outside the built path        */
+    VFVM_ERR_PATTERN_MISMATCH = 23,         /* assembly ops do not match the built pattern        */
+    VFVM_ERR_LINEAR_NON_POSITIVE_TOLERANCE = 30, /* LinearGmresNonPositiveTolerance linear_gmres.rs:29 */
+    VFVM_ERR_LINEAR_NON_POSITIVE_RESTART = 31,   /* LinearGmresNonPositiveRestart   linear_gmres.rs:32 */
+    VFVM_ERR_LINEAR_NON_POSITIVE_MAX_OUTER = 32, /* LinearGmresNonPositiveMaxOuter  linear_gmres.rs:35 */
+    VFVM_ERR_LINEAR_DID_NOT_CONVERGE = 33,       /* LinearGmresDidNotConverge       linear_gmres.rs:81 */
+    VFVM_ERR_LINEAR_NON_FINITE = 34,             /* breakdown to NaN/Inf inside the Krylov solve      */
+    VFVM_ERR_SOLVER_NON_POSITIVE_MAX_ITER = 40,  /* SolverNonPositiveMaxIter   steady_base.rs:21      */
+    VFVM_ERR_SOLVER_NON_POSITIVE_TOLERANCE = 41, /* SolverNonPositiveTolerance steady_base.rs:22      */
+    VFVM_ERR_SOLVER_INVALID_DAMPING = 42,        /* SolverInvalidDamping       steady_base.rs:23      */
+    VFVM_ERR_SOLVER_NON_POSITIVE_TIME_STEP = 43, /* SolverNonPositiveTimeStep  transient_base.rs:21   */
+    VFVM_ERR_SOLVER_NON_POSITIVE_NUM_STEPS = 44, /* SolverNonPositiveNumTimeSteps transient_base.rs:22 */
+    VFVM_ERR_STEADY_DID_NOT_CONVERGE = 45,       /* SteadyDidNotConverge       steady_base.rs:116     */
+    VFVM_ERR_STEADY_NON_FINITE_RESIDUAL = 46,    /* SteadyNonFiniteResidual    steady_base.rs:101     */
+    VFVM_ERR_TRANSIENT_DID_NOT_CONVERGE = 47,    /* TransientDidNotConverge    transient_base.rs:127  */
+    VFVM_ERR_TRANSIENT_NON_FINITE_RESIDUAL = 48  /* TransientNonFiniteResidual transient_base.rs:119  */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* vfvm_last_error(void);
+/* Library version string. */
+const char* vfvm_version(void);
+
+/* =============================== mesh (host) =============================== */
+
+typedef struct vfvm_mesh vfvm_mesh;
+
+typedef struct vfvm_mesh_info {
+    int64_t num_pts;      /* interior points (vid < num_pts)                         */
+    int64_t num_nds;      /* boundary nodes  (vid = num_pts + nid)                   */
+    int64_t num_tets;
+    int64_t num_edges;    /* directed Delaunay edges (CSR length)                    */
+    int64_t num_subfaces; /* boundary sub-faces                                      */
+    int32_t num_bnd;      /* boundary groups                                         */
+    int32_t max_degree;
+} vfvm_mesh_info;
+
+/* Jittered unit-cube lattice of n^3 vertices (h = 1/(n-1)), jitter uniform in
+ * [-jitter*h, +jitter*h] per free axis from a SplitMix64 hash of (seed,i,j,k,axis),
+ * quantised to multiples of 2^-22; exact Delaunay tetrahedralisation; Voronoi dual.
+ * Boundaries (in id order): "x0","x1","y0","y1","z0","z1"; edge and corner nodes
+ * belong to the first face of that order that contains them.
+ * Replaces Variables::new_bounds (src/base/vars.rs:45) + Tetrahedron::update_tet
+ * (src/base/geom_tet.rs:58) for the lattice meshes. */
+int vfvm_mesh_lattice(int32_t n, uint64_t seed, double jitter, vfvm_mesh** out);
+
+/* Mesh from caller-supplied vertices and tetrahedra (any orientation); builds the
+ * dual.  pos: 3*(num_pts+num_nds) AoS.  tets: 4*num_tets vids.  Boundaries:
+ * bnd_ptr[num_bnd+1] offsets into bnd_nid (node ids, each node on one boundary).
+ * Replaces Tetrahedron::update_tet (src/base/geom_tet.rs:58-350). */
+int vfvm_mesh_from_tets(int64_t num_pts, int64_t num_nds, const double* pos,
+                        int64_t num_tets, const int32_t* tets,
+                        int32_t num_bnd, const int64_t* bnd_ptr, const int32_t* bnd_nid,
+                        vfvm_mesh** out);
+
+int vfvm_mesh_info_get(const vfvm_mesh* mesh, vfvm_mesh_info* info);
+
+/* Array ids for vfvm_mesh_get (element type and length in brackets). */
+enum vfvm_mesh_array {
+    VFVM_MESH_POS = 0,       /* f64 [3N] AoS                         */
+    VFVM_MESH_TETS = 1,      /* i32 [4T]                             */
+    VFVM_MESH_BND_PTR = 2,   /* i64 [num_bnd+1]                      */
+    VFVM_MESH_BND_NID = 3,   /* i32 [bnd_ptr[num_bnd]]               */
+    VFVM_MESH_ROW_PTR = 4,   /* i32 [N+1]                            */
+    VFVM_MESH_NGH = 5,       /* i32 [E]                              */
+    VFVM_MESH_DIST = 6,      /* f64 [E]                              */
+    VFVM_MESH_AREA = 7,      /* f64 [E]                              */
+    VFVM_MESH_NORM = 8,      /* f64 [3E] SoA                         */
+    VFVM_MESH_SKEW = 9,      /* f64 [3E] SoA                         */
+    VFVM_MESH_GRAD_WGT = 10, /* f64 [3E] SoA                         */
+    VFVM_MESH_VOL = 11,      /* f64 [N]                              */
+    VFVM_MESH_CSKEW = 12,    /* f64 [3N] SoA                         */
+    VFVM_MESH_BSF_PTR = 13,  /* i32 [N+1]                            */
+    VFVM_MESH_BSF_VJ = 14,   /* i32 [S]                              */
+    VFVM_MESH_BSF_VK = 15,   /* i32 [S]                              */
+    VFVM_MESH_BSF_AREA = 16, /* f64 [S]                              */
+    VFVM_MESH_BSF_NORM = 17, /* f64 [3S] SoA                         */
+    VFVM_MESH_BSF_WGT = 18   /* f64 [3S] SoA                         */
+};
+int vfvm_mesh_get(const vfvm_mesh* mesh, int32_t array_id, void* dst);
+void vfvm_mesh_destroy(vfvm_mesh* mesh);
+
+/* ============================ device context ============================== */
+
+typedef struct vfvm_ctx vfvm_ctx;
+
+int vfvm_create(int32_t device, vfvm_ctx** out);
+void vfvm_destroy(vfvm_ctx* ctx);
+
+/* Uploads the dual geometry of `mesh` to HBM (SoA, CSR edge order) and resets
+ * fields, operators and the pattern.  The mesh may be destroyed afterwards. */
+int vfvm_mesh_upload(vfvm_ctx* ctx, const vfvm_mesh* mesh);
+
+/* ---- fields (ScalarDomain / VectorDomain / ScalarBoundary / ScalarFlow) ---- */
+enum vfvm_field_kind {
+    VFVM_FIELD_CONSTANT = 0, /* SclDomType::Constant  (scl_dom.rs:9)               */
+    VFVM_FIELD_FUNCTION = 1, /* SclDomType::Function  (values evaluated by caller) */
+    VFVM_FIELD_UNKNOWN = 2   /* SclDomType::Unknown   (start assigned by driver)   */
+};
+/* values: N doubles (NULL -> all equal to `cnst`).  Gradients are computed on the
+ * device by the least-squares gather of ScalarDomain::update_grad (scl_dom.rs:146). */
+int vfvm_scldom_add(vfvm_ctx* ctx, int32_t kind, double cnst, const double* values, int32_t* id);
+int vfvm_scldom_set(vfvm_ctx* ctx, int32_t id, const double* values);
+int vfvm_scldom_get(vfvm_ctx* ctx, int32_t id, double* values);
+/* Copies value into value_prev (ScalarDomain::update_prev, scl_dom.rs:141). */
+int vfvm_scldom_update_prev(vfvm_ctx* ctx, int32_t id);
+/* values: 3N doubles SoA (x[N], y[N], z[N]). */
+int vfvm_vecdom_add(vfvm_ctx* ctx, int32_t kind, const double* cnst3, const double* values, int32_t* id);
+int vfvm_vecdom_set(vfvm_ctx* ctx, int32_t id, const double* values);
+/* values: one double per node of boundary bnd_id, in the boundary's node order. */
+int vfvm_sclbnd_add(vfvm_ctx* ctx, int32_t bnd_id, double cnst, const double* values, int32_t* id);
+int vfvm_sclbnd_set(vfvm_ctx* ctx, int32_t id, const double* values);
+/* Flow data (ScalarFlow, scl_flw.rs:3-22) with a prescribed velocity field:
+ * the face mass flux of flow_helper::update_flux (flow_helper.rs:72-179). */
+int vfvm_sclflw_add(vfvm_ctx* ctx, int32_t vel_id, int32_t pres_id, int32_t den_id,
+                    int32_t visc_id, int32_t fce_id, int32_t* id);
+
+/* ---- operators: one descriptor per reference operator struct ---- */
+enum vfvm_op_kind {
+    /* domain operators; geom_id = dom_id (0) */
+    VFVM_OP_SCL_DOM_DIFFUSION = 1,        /* OpSclDomDiffusion{unk,diff,drv}      opscl_dom_diff.rs:7     a = diff, b = drv     */
+    VFVM_OP_SCL_DOM_PRODUCTION = 2,       /* OpSclDomProduction{unk,prd}          opscl_dom_prod.rs        a = prd               */
+    VFVM_OP_SCL_DOM_SOURCE = 3,           /* OpSclDomSource{unk,src}              opscl_dom_src.rs         a = src               */
+    VFVM_OP_SCL_DOM_TIME = 4,             /* OpSclDomTime{unk,wgt}                opscl_dom_time.rs:7      a = wgt               */
+    VFVM_OP_SCL_DOM_ADVECTION_UNITY = 5,  /* OpSclDomAdvectionUnity{unk,diff?,flw} opscl_dom_advunity.rs   a = diff or -1, b = flw */
+    /* boundary operators; geom_id = bnd_id */
+    VFVM_OP_SCL_BND_DIRICHLET = 20,       /* OpSclBndDirichlet{unk,val}           opscl_bnd_dir.rs         a = sclbnd value      */
+    VFVM_OP_SCL_BND_NEUMANN = 21,         /* OpSclBndNeumann{unk,flux}            opscl_bnd_neu.rs         a = sclbnd flux       */
+    VFVM_OP_SCL_BND_DIFFUSION = 22,       /* OpSclBndDiffusion{unk,diff,drv}      opscl_bnd_diff.rs        a = diff, b = drv     */
+    VFVM_OP_SCL_BND_PRODUCTION = 23,      /* OpSclBndProduction{unk,prd}          opscl_bnd_prod.rs        a = prd               */
+    VFVM_OP_SCL_BND_SOURCE = 24,          /* OpSclBndSource{unk,src}              opscl_bnd_src.rs         a = src               */
+    VFVM_OP_SCL_BND_TIME = 25,            /* OpSclBndTime{unk,wgt}                opscl_bnd_time.rs        a = wgt               */
+    VFVM_OP_SCL_BND_ADVECTION_UNITY = 26  /* OpSclBndAdvectionUnity{unk,diff?,vel,flw} opscl_bnd_advunity.rs a = diff or -1, b = flw */
+};
+typedef struct vfvm_op {
+    int32_t kind;     /* vfvm_op_kind                          */
+    int32_t geom_id;  /* dom_id (0) or bnd_id                  */
+    int32_t unk_id;   /* unknown scalar (scldom id)            */
+    int32_t a, b;     /* per-kind field ids, see vfvm_op_kind  */
+} vfvm_op;
+
+/* Fixes the unknown's block start (SteadyBase::initial_start), builds the row-sorted
+ * CSR pattern of the operator list and the edge->slot / diag->slot maps on the host,
+ * uploads them (LinearBase::update_sparse).  The list is applied in array order, as
+ * assemble_matrix applies its operators. */
+int vfvm_pattern_build(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, int64_t* nnz);
+int vfvm_pattern_get(vfvm_ctx* ctx, int32_t* row_ptr, int32_t* col);
+
+/* update_grad + update_flow + assemble_matrix + build_matrix on the device: writes
+ * the CSR values and b in place, in a fixed per-row summation order, no atomics.
+ * dt > 0 enables the apply_time operators (dt <= 0 with a TIME op is an error). */
+int vfvm_assemble(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, double dt);
+int vfvm_matrix_get(vfvm_ctx* ctx, double* vals, double* b);
+
+/* ---- linear solve (LinearGmres) ---- */
+enum vfvm_precision { VFVM_FP64 = 0, VFVM_FP32_REFINED = 1 };
+enum vfvm_precond { VFVM_PRECOND_NONE = 0, VFVM_PRECOND_JACOBI = 1, VFVM_PRECOND_ILU0 = 2 };
+typedef struct vfvm_solve_opts {
+    double tol;          /* relative residual ||b-Ax||/||b|| target (default 1e-6, linear_gmres.rs:21) */
+    int32_t restart;     /* Krylov dimension per cycle (default 50)                    */
+    int32_t max_outer;   /* restart cycles (default 200)                               */
+    int32_t precision;   /* vfvm_precision                                             */
+    int32_t precond;     /* vfvm_precond (default Jacobi, linear_gmres.rs:79)          */
+    double inner_tol;    /* FP32 inner relative tolerance per refinement step (1e-5)   */
+    int32_t max_refine;  /* maximum refinement steps (FP32_REFINED)                    */
+    int32_t reserved;
+} vfvm_solve_opts;
+typedef struct vfvm_solve_stats {
+    int32_t iterations;      /* Arnoldi steps                               */
+    int32_t restarts;        /* restart cycles                              */
+    int32_t refinements;     /* outer FP64 refinement steps                 */
+    int32_t status;          /* vfvm_status of the solve                    */
+    double rel_residual;     /* final ||b-Ax||/||b|| computed in FP64       */
+    double seconds;          /* device wall time of the solve (HIP events) */
+    double setup_seconds;    /* preconditioner set-up part of `seconds`     */
+} vfvm_solve_stats;
+void vfvm_solve_opts_default(vfvm_solve_opts* opts);
+/* Solves the assembled system; x: host array, initial guess in, solution out
+ * (LinearBase::solve_matrix, linear_gmres.rs:76-84). */
+int vfvm_solve(vfvm_ctx* ctx, const vfvm_solve_opts* opts, double* x, vfvm_solve_stats* stats);
+/* Same, on a device-resident x (hipMalloc'd, mat_size doubles) on the context stream. */
+int vfvm_solve_device(vfvm_ctx* ctx, const vfvm_solve_opts* opts, double* x_dev, vfvm_solve_stats* stats);
+/* r = b - A x in FP64 for a host x; returns ||r||_2 / ||b||_2. */
+int vfvm_residual(vfvm_ctx* ctx, const double* x, double* rel_residual);
+
+/* ---- drivers ---- */
+typedef struct vfvm_driver_stats {
+    int32_t picard_iterations;   /* total Picard iterations                   */
+    int32_t linear_iterations;   /* total Arnoldi steps                       */
+    double residual;             /* last scaled residual (scaling.rs:113)     */
+    double time_assembly;        /* "Matrix Assembly" bucket, seconds         */
+    double time_solve;           /* "Matrix Solve" bucket, seconds            */
+} vfvm_driver_stats;
+/* SteadyBase::solve (steady_base.rs:16-135) for the unknown of the built pattern:
+ * Dirichlet initial values, assemble, Picard loop with damping, scaled residual. */
+int vfvm_steady_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* opts,
+                      int32_t max_iter, double tol, double damp, vfvm_driver_stats* stats);
+/* TransientBase::solve (transient_base.rs:16-160): implicit Euler, Picard per step. */
+int vfvm_transient_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* opts,
+                         double dt, int32_t num_ts, int32_t max_iter, double tol, double damp,
+                         vfvm_driver_stats* stats);
+
+/* ---- measurement ---- */
+/* Enables per-launch HIP-event timing of the named kernel class on the context stream. */
+enum vfvm_kernel_class { VFVM_K_SPMV = 0, VFVM_K_ORTHO = 1, VFVM_K_ASSEMBLE = 2, VFVM_K_PRECOND = 3, VFVM_K_COUNT = 4 };
+int vfvm_timing_enable(vfvm_ctx* ctx, int32_t enable);
+/* Returns launches and total milliseconds of one kernel class since the last reset, and
+ * the algorithmic bytes those launches moved (per the byte model of each kernel). */
+int vfvm_timing_get(vfvm_ctx* ctx, int32_t kclass, int64_t* launches, double* total_ms, double* bytes);
+int vfvm_timing_reset(vfvm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VFVM_H */

@@ -1,0 +1,533 @@
+"""CPU oracle for the vfvm3d hot path — TEST INFRASTRUCTURE ONLY.
+
+This module restates the reference's algorithms (msl-multiphysics/vfvchem-vfvm3d,
+pure Rust) in numpy so that the HIP path can be checked against it.  Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg may import it, and only as
+the checker or the timed CPU baseline; the product path never calls into it.
+
+Every function cites the reference file:line it follows.  Floating-point
+expressions keep the reference's operand order (elementwise numpy operations are
+IEEE-rounded one at a time, like the reference's f64 arithmetic), and every
+accumulation that the reference performs sequentially is performed sequentially
+here (np.add.at in the reference's order), so results are reproducible to the bit.
+
+Pinning (how each function is checked independently of itself):
+  dual()            invariants: closed interior cells (sum A n = 0), sum of CV
+                    volumes = mesh volume, LSQ exactness for linear fields,
+                    brute-force single-tetrahedron closed form; Delaunay input
+                    checked against scipy.spatial.Delaunay (Qhull).
+  update_grad()     exact for linear fields (closed form).
+  flux_prescribed() antisymmetry F(v->n) = -F(n->v) and F = rho (u.n) A for a
+                    uniform velocity (closed form).
+  assemble()        row sums, symmetry of diffusion, exactness on linear fields,
+                    1D closed form T = x + q x(1-x)/(2k) of heat_analytic.rs:9,
+                    power-law limits; pattern {v} u ngh(v) / {v}.
+  solve (C, oracle/gmres_ref.c) and scipy SuperLU: residual and cross-check.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+__all__ = [
+    "dual", "update_grad", "update_grad_vec", "flux_prescribed", "assemble",
+    "coo_to_csr", "Op",
+]
+
+
+# --------------------------------------------------------------------------- helpers
+def _dot(a, b):
+    # dot3 of src/base/delaunay.rs:190-192: a.0*b.0 + a.1*b.1 + a.2*b.2 (left to right)
+    return a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2]
+
+
+def _cross(a, b):
+    # cross3 of src/base/delaunay.rs:195-197
+    return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
+                     a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                     a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], axis=-1)
+
+
+def _scale(s, a):
+    return np.asarray(s)[..., None] * a
+
+
+def _len(a):
+    return np.sqrt(_dot(a, a))
+
+
+def _circumcircle(a, b, c):
+    # src/base/delaunay.rs:204-211
+    u, v = b - a, c - a
+    w = _cross(u, v)
+    num = _scale(_dot(u, u), _cross(v, w)) + _scale(_dot(v, v), _cross(w, u))
+    return a + _scale(0.5 / _dot(w, w), num)
+
+
+def _circumsphere(a, b, c, d):
+    # src/base/delaunay.rs:213-218
+    u, v, w = b - a, c - a, d - a
+    num = (_scale(_dot(u, u), _cross(v, w)) + _scale(_dot(v, v), _cross(w, u))) + _scale(_dot(w, w), _cross(u, v))
+    return a + _scale(0.5 / _dot(u, _cross(v, w)), num)
+
+
+# --------------------------------------------------------------------------- dual
+def dual(pos: np.ndarray, tets: np.ndarray, num_pts: int) -> dict:
+    """Voronoi dual of a tetrahedralisation; src/base/geom_tet.rs:58-293.
+
+    pos: (N,3) float64; tets: (T,4) int in the reference orientation
+    (geom_tet.rs:13).  Returns the arrays in CSR layout (neighbours ascending,
+    geom_tet.rs:174-196), SoA vectors shaped (3, E) / (3, N) / (3, S).
+    Raises ValueError('points on boundary') as TetraPointsOnBoundary
+    (geom_tet.rs:119-120) and ValueError('singular gradient') as
+    TetraSingularGradient (geom_tet.rs:284).
+    """
+    pos = np.asarray(pos, np.float64)
+    tets = np.asarray(tets, np.int64)
+    N = pos.shape[0]
+    T = tets.shape[0]
+
+    # step 1: boundary faces (geom_tet.rs:99-120)
+    fkeys = np.stack([np.sort(tets[:, [(k + 1) % 4, (k + 2) % 4, (k + 3) % 4]], axis=1) for k in range(4)], axis=1)
+    flat = fkeys.reshape(-1, 3)
+    _, inv, counts = np.unique(flat, axis=0, return_inverse=True, return_counts=True)
+    once = (counts[inv.ravel()] == 1).reshape(T, 4)
+    surf_order = [[1, 3, 2], [0, 2, 3], [0, 3, 1], [0, 1, 2]]
+    surf = []
+    tt, kk = np.nonzero(once)  # row-major: tet order, then k order (geom_tet.rs:110-116)
+    for t, k in zip(tt, kk):
+        surf.append(tets[t, surf_order[k]])
+    surf = np.array(surf, np.int64).reshape(-1, 3)
+    if np.any(surf < num_pts):
+        raise ValueError("points on boundary")
+
+    # step 2: face pieces (geom_tet.rs:122-171), vectorised over tets, accumulated in order
+    c_t = _circumsphere(pos[tets[:, 0]], pos[tets[:, 1]], pos[tets[:, 2]], pos[tets[:, 3]])
+    pieces_key = []
+    pieces_area = []
+    pieces_sk = []
+    cent_idx = []
+    cent_val = []
+    edges = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    rec = []  # per (edge, side) in loop order: key_min, key_max, area, area*(cent-m), (vid,val) pairs
+    for (li, lj) in edges:
+        other = [l for l in range(4) if l != li and l != lj]
+        vi, vj = tets[:, li], tets[:, lj]
+        vk, vl = tets[:, other[0]], tets[:, other[1]]
+        pi, pj = pos[vi], pos[vj]
+        dist = _len(pj - pi)
+        e = _scale(1.0 / dist, pj - pi)
+        m = _scale(0.5, pi + pj)
+        sides = []
+        for (va, vb) in [(vk, vl), (vl, vk)]:
+            pa, pb = pos[va], pos[vb]
+            c_f = _circumcircle(pi, pj, pa)
+            u = (pa - m) - _scale(_dot(pa - m, e), e)
+            u = _scale(1.0 / _len(u), u)
+            n = _cross(e, u)
+            flip = _dot(pb - m, n) < 0.0
+            n = np.where(flip[:, None], -1.0 * n, n)
+            s1 = _dot(c_f - m, u)
+            s2 = _dot(c_t - c_f, n)
+            area = 0.5 * s1 * s2
+            cent = _scale(1.0 / 3.0, m + (c_f + c_t))
+            skc = _scale(area, cent - m)
+            vol = area * dist / 6.0
+            ci = _scale(vol * 0.75, cent - pi)
+            cj = _scale(vol * 0.75, cent - pj)
+            sides.append((area, skc, ci, cj))
+        rec.append((np.minimum(vi, vj), np.maximum(vi, vj), vi, vj, sides))
+    # sequence index so that the sequential accumulation follows tet, edge, side order
+    K = len(edges) * 2
+    kmin = np.empty((T, K), np.int64)
+    kmax = np.empty((T, K), np.int64)
+    ar = np.empty((T, K))
+    sk = np.empty((T, K, 3))
+    cv = np.empty((T, 2 * K), np.int64)
+    cc = np.empty((T, 2 * K, 3))
+    for q, (a, b, vi, vj, sides) in enumerate(rec):
+        for s, (area, skc, ci, cj) in enumerate(sides):
+            col = 2 * q + s
+            kmin[:, col] = a
+            kmax[:, col] = b
+            ar[:, col] = area
+            sk[:, col] = skc
+            cv[:, 2 * col] = vi
+            cc[:, 2 * col] = ci
+            cv[:, 2 * col + 1] = vj
+            cc[:, 2 * col + 1] = cj
+    key = kmin.ravel() * N + kmax.ravel()
+    ukey, kinv = np.unique(key, return_inverse=True)
+    e_area = np.zeros(ukey.size)
+    e_sk = np.zeros((ukey.size, 3))
+    np.add.at(e_area, kinv, ar.ravel())
+    for d in range(3):
+        col = np.zeros(ukey.size)
+        np.add.at(col, kinv, sk.reshape(-1, 3)[:, d])
+        e_sk[:, d] = col
+    cent_sum = np.zeros((N, 3))
+    for d in range(3):
+        col = np.zeros(N)
+        np.add.at(col, cv.ravel(), cc.reshape(-1, 3)[:, d])
+        cent_sum[:, d] = col
+
+    # step 3: sorted edges -> neighbour lists, face data, CV volume (geom_tet.rs:173-201)
+    ea = ukey // N
+    eb = ukey % N
+    pa, pb = pos[ea], pos[eb]
+    dist = _len(pb - pa)
+    norm = _scale(1.0 / dist, pb - pa)
+    big = np.abs(e_area) > 1e-12 * dist * dist
+    with np.errstate(divide="ignore", invalid="ignore"):
+        skew = np.where(big[:, None], _scale(1.0 / e_area, e_sk), 0.0)
+    # directed edges: rows ascending, neighbours ascending
+    src = np.concatenate([ea, eb])
+    dst = np.concatenate([eb, ea])
+    und = np.concatenate([np.arange(ukey.size), np.arange(ukey.size)])
+    sgn = np.concatenate([np.ones(ukey.size), -np.ones(ukey.size)])
+    order = np.lexsort((dst, src))
+    src, dst, und, sgn = src[order], dst[order], und[order], sgn[order]
+    E = src.size
+    row_ptr = np.zeros(N + 1, np.int64)
+    np.add.at(row_ptr, src + 1, 1)
+    row_ptr = np.cumsum(row_ptr)
+    # CV volume accumulated over the sorted edge list: for each vertex, edges in key order
+    vol = np.zeros(N)
+    contrib = e_area * dist / 6.0
+    seq_v = np.stack([ea, eb], axis=1).ravel()
+    seq_c = np.stack([contrib, contrib], axis=1).ravel()
+    np.add.at(vol, seq_v, seq_c)
+    cskew = _scale(1.0 / vol, cent_sum)
+
+    out = dict(
+        num_pts=int(num_pts), N=N,
+        row_ptr=row_ptr.astype(np.int32), ngh=dst.astype(np.int32),
+        dist=dist[und], area=e_area[und],
+        norm=(norm[und] * sgn[:, None]).T.copy(),
+        skew=skew[und].T.copy(), vol=vol, cskew=cskew.T.copy(),
+    )
+    if np.any(sgn < 0):
+        # scale3(-1.0, norm) is an exact sign flip, as multiplying by -1 here
+        pass
+
+    # step 4: boundary sub-faces (geom_tet.rs:208-259), per surface triangle in order
+    bvid, bvj, bvk, barea, bnorm, bwgt = [], [], [], [], [], []
+    bnd_sum = np.zeros(N)
+    if surf.size:
+        p0, p1, p2 = pos[surf[:, 0]], pos[surf[:, 1]], pos[surf[:, 2]]
+        nrm = _cross(p1 - p0, p2 - p0)
+        nrm = _scale(1.0 / _len(nrm), nrm)
+        c = _circumcircle(p0, p1, p2)
+        sq = [_dot(p2 - p1, p2 - p1), _dot(p0 - p2, p0 - p2), _dot(p1 - p0, p1 - p0)]
+        lam_raw = [sq[0] * (sq[1] + sq[2] - sq[0]), sq[1] * (sq[2] + sq[0] - sq[1]), sq[2] * (sq[0] + sq[1] - sq[2])]
+        lam_sum = lam_raw[0] + lam_raw[1] + lam_raw[2]
+        lam = [lr / lam_sum for lr in lam_raw]
+        per_loc = []
+        for loc in range(3):
+            vi, vj, vk = surf[:, loc], surf[:, (loc + 1) % 3], surf[:, (loc + 2) % 3]
+            pi, pj, pk = pos[vi], pos[vj], pos[vk]
+            area = []
+            for (pa_, pb_) in [(pj, pk), (pk, pj)]:
+                m = _scale(0.5, pi + pa_)
+                e = pa_ - pi
+                ln = _len(e)
+                u = (pb_ - m) - _scale(_dot(pb_ - m, e) / (ln * ln), e)
+                area.append(0.25 * ln * _dot(c - m, _scale(1.0 / _len(u), u)))
+            area_sum = area[0] + area[1]
+            l0, l1, l2 = lam[loc], lam[(loc + 1) % 3], lam[(loc + 2) % 3]
+            cent_j = np.stack([(1.5 + l0) / 3.0, (0.5 + l1) / 3.0, l2 / 3.0], axis=-1)
+            cent_k = np.stack([(1.5 + l0) / 3.0, l1 / 3.0, (0.5 + l2) / 3.0], axis=-1)
+            ok = np.abs(area_sum) > 1e-12 * _len(pj - pi) * _len(pk - pi)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                w = _scale(1.0 / area_sum, _scale(area[0], cent_j) + _scale(area[1], cent_k))
+            w = np.where(ok[:, None], w, np.array([1.0, 0.0, 0.0]))
+            per_loc.append((vi, vj, vk, area_sum, w))
+        # push order: triangle by triangle, loc 0..2 (geom_tet.rs:213-258)
+        nf = surf.shape[0]
+        vi_all = np.stack([p[0] for p in per_loc], axis=1).ravel()
+        vj_all = np.stack([p[1] for p in per_loc], axis=1).ravel()
+        vk_all = np.stack([p[2] for p in per_loc], axis=1).ravel()
+        ar_all = np.stack([p[3] for p in per_loc], axis=1).ravel()
+        w_all = np.stack([p[4] for p in per_loc], axis=1).reshape(-1, 3)
+        n_all = np.repeat(nrm, 3, axis=0)
+        np.add.at(bnd_sum, vi_all, ar_all)
+        o = np.argsort(vi_all, kind="stable")
+        bsf_ptr = np.zeros(N + 1, np.int64)
+        np.add.at(bsf_ptr, vi_all + 1, 1)
+        bsf_ptr = np.cumsum(bsf_ptr)
+        out.update(bsf_ptr=bsf_ptr.astype(np.int32), bsf_vj=vj_all[o].astype(np.int32),
+                   bsf_vk=vk_all[o].astype(np.int32), bsf_area=ar_all[o],
+                   bsf_norm=n_all[o].T.copy(), bsf_wgt=w_all[o].T.copy())
+    else:
+        out.update(bsf_ptr=np.zeros(N + 1, np.int32), bsf_vj=np.zeros(0, np.int32), bsf_vk=np.zeros(0, np.int32),
+                   bsf_area=np.zeros(0), bsf_norm=np.zeros((3, 0)), bsf_wgt=np.zeros((3, 0)))
+    out["bnd_sum"] = bnd_sum
+    out["surf"] = surf
+
+    # step 5: least-squares gradient weights (geom_tet.rs:261-293)
+    rows = np.repeat(np.arange(N), np.diff(row_ptr))
+    d = pos[dst] - pos[rows]
+    omega = 1.0 / (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+    M = np.zeros((N, 3, 3))
+    for r in range(3):
+        for s in range(3):
+            col = np.zeros(N)
+            np.add.at(col, rows, omega * d[:, r] * d[:, s])
+            M[:, r, s] = col
+    m_ = M
+    adj = np.empty((N, 3, 3))
+    adj[:, 0, 0] = m_[:, 1, 1] * m_[:, 2, 2] - m_[:, 1, 2] * m_[:, 2, 1]
+    adj[:, 0, 1] = m_[:, 0, 2] * m_[:, 2, 1] - m_[:, 0, 1] * m_[:, 2, 2]
+    adj[:, 0, 2] = m_[:, 0, 1] * m_[:, 1, 2] - m_[:, 0, 2] * m_[:, 1, 1]
+    adj[:, 1, 0] = m_[:, 1, 2] * m_[:, 2, 0] - m_[:, 1, 0] * m_[:, 2, 2]
+    adj[:, 1, 1] = m_[:, 0, 0] * m_[:, 2, 2] - m_[:, 0, 2] * m_[:, 2, 0]
+    adj[:, 1, 2] = m_[:, 0, 2] * m_[:, 1, 0] - m_[:, 0, 0] * m_[:, 1, 2]
+    adj[:, 2, 0] = m_[:, 1, 0] * m_[:, 2, 1] - m_[:, 1, 1] * m_[:, 2, 0]
+    adj[:, 2, 1] = m_[:, 0, 1] * m_[:, 2, 0] - m_[:, 0, 0] * m_[:, 2, 1]
+    adj[:, 2, 2] = m_[:, 0, 0] * m_[:, 1, 1] - m_[:, 0, 1] * m_[:, 1, 0]
+    det = m_[:, 0, 0] * adj[:, 0, 0] + m_[:, 0, 1] * adj[:, 1, 0] + m_[:, 0, 2] * adj[:, 2, 0]
+    tr = m_[:, 0, 0] + m_[:, 1, 1] + m_[:, 2, 2]
+    if not np.all(det > 1e-12 * (tr * tr * tr)):
+        raise ValueError("singular gradient")
+    om2 = 1.0 / _dot(d, d)
+    gw = np.empty((3, E))
+    for r in range(3):
+        gw[r] = om2 * (adj[rows, r, 0] * d[:, 0] + adj[rows, r, 1] * d[:, 1] + adj[rows, r, 2] * d[:, 2]) / det[rows]
+    out["gw"] = gw
+    out["rows"] = rows.astype(np.int32)
+    return out
+
+
+# --------------------------------------------------------------------------- gradients
+def update_grad(g: dict, value: np.ndarray) -> np.ndarray:
+    """LSQ gradient, ScalarDomain::update_grad (src/base/scl_dom.rs:146-164).
+
+    grad_c = sum_loc w_{c,loc} (u_n - u_c), accumulated in loc order from 0.0.
+    Returns (3, N).
+    """
+    rows, ngh = g["rows"], g["ngh"]
+    delta = value[ngh] - value[rows]
+    out = np.zeros((3, g["N"]))
+    for r in range(3):
+        col = np.zeros(g["N"])
+        np.add.at(col, rows, g["gw"][r] * delta)
+        out[r] = col
+    return out
+
+
+def update_grad_vec(g: dict, value3: np.ndarray) -> np.ndarray:
+    """9-component gradient, VectorDomain::update_grad (src/base/vec_dom.rs:146-178).
+
+    value3: (3, N) SoA.  Returns (9, N) ordered (u_x,u_y,u_z,v_x,...,w_z).
+    """
+    out = np.empty((9, g["N"]))
+    for c in range(3):
+        out[3 * c:3 * c + 3] = update_grad(g, value3[c])
+    return out
+
+
+# --------------------------------------------------------------------------- flow
+def flux_prescribed(g: dict, vel: np.ndarray, vel_grad: np.ndarray, den: np.ndarray, den_grad: np.ndarray) -> np.ndarray:
+    """Face mass flux of flow_helper::update_flux (src/operator/core/flow_helper.rs:118-171)
+    for a prescribed velocity with uniform pressure and zero body force.
+
+    With p constant the pressure-difference term (p_n - p_v)/dn is exactly zero and
+    fce = 0, so vel_rc (flow_helper.rs:167) reduces to the momentum interpolation term
+    mom_e, which is rc * grad(p) with grad(p) = p * sum_m c_m / V = 0 for a closed
+    control volume; it is taken as 0 here (a round-off-level difference from the
+    reference, which evaluates sum_m c_m in floating point).
+    F = den_f * (u_f nx + v_f ny + w_f nz) * A, den_f and u_f skew-corrected
+    arithmetic means (flow_helper.rs:158-161).  Returns (E,).
+    """
+    rows, ngh = g["rows"], g["ngh"]
+    sx, sy, sz = g["skew"]
+    nx, ny, nz = g["norm"]
+    A = g["area"]
+    dv, dn = den[rows], den[ngh]
+    den_f = 0.5 * (dv + dn) + 0.5 * (sx * (den_grad[0][rows] + den_grad[0][ngh]) + sy * (den_grad[1][rows] + den_grad[1][ngh]) + sz * (den_grad[2][rows] + den_grad[2][ngh]))
+
+    def face(c):
+        val = vel[c]
+        gx, gy, gz = vel_grad[3 * c], vel_grad[3 * c + 1], vel_grad[3 * c + 2]
+        return 0.5 * (val[rows] + val[ngh]) + 0.5 * (sx * (gx[rows] + gx[ngh]) + sy * (gy[rows] + gy[ngh]) + sz * (gz[rows] + gz[ngh]))
+
+    u_f, v_f, w_f = face(0), face(1), face(2)
+    vel_rc = 0.0
+    return den_f * (u_f * nx + v_f * ny + w_f * nz + vel_rc) * A
+
+
+# --------------------------------------------------------------------------- assembly
+class Op:
+    """Operator descriptor mirroring the reference operator structs (same kinds and
+    id meanings as include/vfvm.h's vfvm_op)."""
+
+    SCL_DOM_DIFFUSION = 1
+    SCL_DOM_PRODUCTION = 2
+    SCL_DOM_SOURCE = 3
+    SCL_DOM_TIME = 4
+    SCL_DOM_ADVECTION_UNITY = 5
+    SCL_BND_DIRICHLET = 20
+    SCL_BND_NEUMANN = 21
+    SCL_BND_DIFFUSION = 22
+    SCL_BND_PRODUCTION = 23
+    SCL_BND_SOURCE = 24
+    SCL_BND_TIME = 25
+    SCL_BND_ADVECTION_UNITY = 26
+
+    def __init__(self, kind, geom_id, unk_id, a=-1, b=-1):
+        self.kind, self.geom_id, self.unk_id, self.a, self.b = kind, geom_id, unk_id, a, b
+
+
+def _diff_face(G, Gg, v, n, skew):
+    # harmonic mean + skew correction + guard; opscl_dom_diff.rs:78-80
+    Gv, Gn = G[v], G[n]
+    ssum = Gv + Gn
+    with np.errstate(divide="ignore", invalid="ignore"):
+        diff_m = np.where(ssum != 0.0, 2.0 * Gv * Gn / ssum, 0.0)
+    sx, sy, sz = skew
+    diff_f = diff_m + 0.5 * (sx * (Gg[0][v] + Gg[0][n]) + sy * (Gg[1][v] + Gg[1][n]) + sz * (Gg[2][v] + Gg[2][n]))
+    return np.where(diff_f > 0.0, diff_f, diff_m)
+
+
+def _pow5(x):
+    # powi(5) as binary exponentiation: x^4 * x
+    x2 = x * x
+    x4 = x2 * x2
+    return x4 * x
+
+
+def assemble(g: dict, ops, scl: dict, sclbnd: dict, flux: dict, bnd_nid: list, dt: float = 0.0,
+             value_prev: dict | None = None):
+    """Triplets and b of assemble_matrix for the scalar operators, in push order.
+
+    g: dual(); ops: list of Op in the caller's assemble_matrix order; scl: id ->
+    (value[N], grad[3,N]); sclbnd: id -> values per boundary node; flux: flw id ->
+    F[E]; bnd_nid: list of node-id arrays per boundary.  The single unknown's block
+    starts at 0 (SteadyHeat::initial_start, steady_heat.rs:68-78).
+    Returns rows, cols, vals (push order), b.
+    """
+    N, NP = g["N"], g["num_pts"]
+    rp, ngh = g["row_ptr"], g["ngh"]
+    rowsE = g["rows"]
+    R, C, V = [], [], []
+    b = np.zeros(N)
+    pts = np.arange(NP)
+    edge_of_pts = rowsE < NP
+
+    def edges_of(vset_mask_rows):
+        # directed edges whose row is in the vertex set, in (row, loc) order
+        return np.nonzero(vset_mask_rows)[0]
+
+    def push_pairs(e_idx, coeff):
+        # per face: (v, n) +coeff then (v, v) -coeff (opscl_dom_diff.rs:84-85)
+        v = rowsE[e_idx]
+        n = ngh[e_idx]
+        rr = np.stack([v, v], axis=1).ravel()
+        cc = np.stack([n, v], axis=1).ravel()
+        vv = np.stack([coeff, -coeff], axis=1).ravel()
+        R.append(rr); C.append(cc); V.append(vv)
+
+    def b_add(vids, vals):
+        np.add.at(b, vids, vals)
+
+    for op in ops:
+        k = op.kind
+        if k in (Op.SCL_DOM_DIFFUSION, Op.SCL_DOM_ADVECTION_UNITY, Op.SCL_DOM_PRODUCTION, Op.SCL_DOM_SOURCE, Op.SCL_DOM_TIME):
+            vids = pts
+            emask = edge_of_pts
+        else:
+            vids = NP + np.asarray(bnd_nid[op.geom_id], np.int64)
+            m = np.zeros(N, bool)
+            m[vids] = True
+            emask = m[rowsE]
+        if k in (Op.SCL_DOM_DIFFUSION, Op.SCL_BND_DIFFUSION):
+            # opscl_dom_diff.rs:40-91, opscl_bnd_diff.rs:40-96
+            G, Gg = scl[op.a]
+            e = edges_of(emask)
+            df = _diff_face(G, Gg, rowsE[e], ngh[e], g["skew"][:, e])
+            push_pairs(e, df * g["area"][e] / g["dist"][e])
+        elif k in (Op.SCL_DOM_ADVECTION_UNITY, Op.SCL_BND_ADVECTION_UNITY):
+            # opscl_dom_advunity.rs:46-115, opscl_bnd_advunity.rs:52-..
+            e = edges_of(emask)
+            if op.a >= 0:
+                G, Gg = scl[op.a]
+                df = _diff_face(G, Gg, rowsE[e], ngh[e], g["skew"][:, e])
+            else:
+                df = np.zeros(e.size)
+            cond = df * g["area"][e] / g["dist"][e]
+            F = flux[op.b][e]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                pe = np.where(cond != 0.0, F / cond, 0.0)
+            pl = _pow5(np.maximum(1.0 - 0.1 * np.abs(pe), 0.0))
+            coeff = cond * (pl - 1.0) + np.maximum(-F, 0.0)
+            push_pairs(e, coeff)
+        elif k in (Op.SCL_DOM_PRODUCTION, Op.SCL_BND_PRODUCTION):
+            # opscl_dom_prod.rs:39-76
+            P, Pg = scl[op.a]
+            _, Ug = scl[op.unk_id]
+            sx, sy, sz = g["cskew"][:, vids]
+            vol = g["vol"][vids]
+            prd_c = P[vids] + Pg[0][vids] * sx + Pg[1][vids] * sy + Pg[2][vids] * sz
+            unk_skew = Ug[0][vids] * sx + Ug[1][vids] * sy + Ug[2][vids] * sz
+            R.append(vids); C.append(vids); V.append(prd_c * vol)
+            b_add(vids, -P[vids] * unk_skew * vol)
+        elif k in (Op.SCL_DOM_SOURCE, Op.SCL_BND_SOURCE):
+            # opscl_dom_src.rs:37-68
+            S, Sg = scl[op.a]
+            sx, sy, sz = g["cskew"][:, vids]
+            src_c = S[vids] + Sg[0][vids] * sx + Sg[1][vids] * sy + Sg[2][vids] * sz
+            b_add(vids, -src_c * g["vol"][vids])
+        elif k in (Op.SCL_DOM_TIME, Op.SCL_BND_TIME):
+            # opscl_dom_time.rs:35-62 (apply_time)
+            if not dt > 0.0:
+                raise ValueError("OperatorUseApplyTime")
+            W, _ = scl[op.a]
+            prev = value_prev[op.unk_id]
+            vol = g["vol"][vids]
+            R.append(vids); C.append(vids); V.append(-W[vids] * vol / dt)
+            b_add(vids, -W[vids] * prev[vids] * vol / dt)
+        elif k == Op.SCL_BND_DIRICHLET:
+            # opscl_bnd_dir.rs:56-78
+            val = np.asarray(sclbnd[op.a], np.float64)
+            R.append(vids); C.append(vids); V.append(np.ones(vids.size))
+            b_add(vids, val)
+        elif k == Op.SCL_BND_NEUMANN:
+            # opscl_bnd_neu.rs:36-87: per node, per sub-face, b += A (w_v N_v + w_j N_j + w_k N_k)
+            val = np.asarray(sclbnd[op.a], np.float64)
+            idx_of = -np.ones(N, np.int64)
+            idx_of[vids] = np.arange(vids.size)
+            bp = g["bsf_ptr"]
+            for ii, v in enumerate(vids):
+                fv = val[ii]
+                for s in range(bp[v], bp[v + 1]):
+                    j, kk = g["bsf_vj"][s], g["bsf_vk"][s]
+                    fj = val[idx_of[j]] if idx_of[j] >= 0 else fv
+                    fk = val[idx_of[kk]] if idx_of[kk] >= 0 else fv
+                    wv, wj, wk = g["bsf_wgt"][:, s]
+                    b[v] += g["bsf_area"][s] * (wv * fv + wj * fj + wk * fk)
+        else:
+            raise ValueError(f"unsupported operator kind {k}")
+    rows = np.concatenate(R) if R else np.zeros(0, np.int64)
+    cols = np.concatenate(C) if C else np.zeros(0, np.int64)
+    vals = np.concatenate(V) if V else np.zeros(0)
+    return rows.astype(np.int64), cols.astype(np.int64), vals, b
+
+
+def coo_to_csr(rows, cols, vals, n):
+    """Pattern + duplicate summation of SparseColMat::try_new_from_indices /
+    new_from_argsort (linear_gmres.rs:59,71), stored row-major.
+
+    Duplicates are summed sequentially in push order (a stable sort keeps it).
+    Returns row_ptr[n+1], col[nnz], val[nnz].
+    """
+    order = np.lexsort((cols, rows))  # stable: push order among equal (row, col)
+    r, c = rows[order], cols[order]
+    key = r * n + c
+    uk, first = np.unique(key, return_index=True)
+    seg = np.searchsorted(uk, key)
+    out = np.zeros(uk.size)
+    np.add.at(out, seg, vals[order])
+    ur = uk // n
+    uc = uk % n
+    row_ptr = np.zeros(n + 1, np.int64)
+    np.add.at(row_ptr, ur + 1, 1)
+    return np.cumsum(row_ptr), uc.astype(np.int64), out

@@ -1,0 +1,735 @@
+// C ABI: device context, fields, operator pattern, assembly, solve and drivers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+
+using namespace vfvm;
+
+// ------------------------------------------------------------------ errors
+namespace {
+thread_local std::string g_last_error;
+}
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+int clear_error() {
+    g_last_error.clear();
+    return VFVM_OK;
+}
+extern "C" const char* vfvm_last_error(void) { return g_last_error.c_str(); }
+extern "C" const char* vfvm_version(void) { return "vfvm-mi355x 0.1.0 (gfx950)"; }
+
+struct vfvm_ctx {
+    Ctx c;
+};
+
+namespace {
+
+constexpr int BLK = 256;
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
+
+__global__ void k_fill(int64_t n, double v, double* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) out[i] = v;
+}
+// u[NP + nid[k]] = val[k]  (OpSclBndDirichlet::apply_initial, opscl_bnd_dir.rs:34-53)
+__global__ void k_scatter_bnd(int64_t cnt, int64_t NP, const int32_t* __restrict__ nid, const double* __restrict__ val,
+                              double* __restrict__ u) {
+    int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (k < cnt) u[NP + nid[k]] = val[k];
+}
+// x_damp = (1 - d) x_iter + d x_udmp  (steady_base.rs:80)
+__global__ void k_damp(int64_t n, double d, const double* __restrict__ xi, const double* __restrict__ xu, double* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) out[i] = (1.0 - d) * xi[i] + d * xu[i];
+}
+// partials of sum (x_new - x_old)^2, max |x_old|, max |x_new|, non-finite count (scaling.rs:113-135)
+__global__ void k_picard(int64_t n, const double* __restrict__ xo, const double* __restrict__ xn, double* __restrict__ part) {
+    __shared__ double s0[BLK], s1[BLK], s2[BLK], s3[BLK];
+    int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    double d2 = 0.0, mo = 0.0, mn = 0.0, nf = 0.0;
+    if (i < n) {
+        double a = xo[i], b = xn[i];
+        if (!isfinite(b)) nf = 1.0;
+        double d = fabs(b - a);
+        d2 = d * d;
+        mo = fabs(a);
+        mn = fabs(b);
+    }
+    s0[threadIdx.x] = d2; s1[threadIdx.x] = mo; s2[threadIdx.x] = mn; s3[threadIdx.x] = nf;
+    __syncthreads();
+    for (int off = BLK / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            s0[threadIdx.x] += s0[threadIdx.x + off];
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + off]);
+            s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + off]);
+            s3[threadIdx.x] += s3[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[4 * blockIdx.x] = s0[0]; part[4 * blockIdx.x + 1] = s1[0];
+        part[4 * blockIdx.x + 2] = s2[0]; part[4 * blockIdx.x + 3] = s3[0];
+    }
+}
+
+bool is_dom_op(int k) { return k >= 1 && k < 20; }
+bool is_bnd_op(int k) { return k >= 20 && k < 40; }
+
+struct Compiled {
+    int nclass = 0;
+    int32_t unk = -1;
+    std::vector<std::vector<vfvm_op>> per_class;
+    std::vector<uint8_t> has_edges, has_diag;
+    bool has_time = false;
+};
+
+int check_scl(Ctx& c, int32_t id) { return id >= 0 && id < (int32_t)c.scl.size(); }
+
+// validates an operator list and splits it by row class (points, then each boundary),
+// keeping the list order inside each class (the reference's push order)
+int compile_ops(Ctx& c, const vfvm_op* ops, int32_t nops, Compiled& out) {
+    if (!c.has_mesh) return set_error(VFVM_ERR_INVALID_ARGUMENT, "no mesh uploaded");
+    if (nops <= 0 || !ops) return set_error(VFVM_ERR_INVALID_ARGUMENT, "empty operator list");
+    out.nclass = 1 + c.g.num_bnd;
+    if (out.nclass > MAX_CLASSES) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "too many boundaries");
+    out.per_class.assign(out.nclass, {});
+    out.has_edges.assign(out.nclass, 0);
+    out.has_diag.assign(out.nclass, 0);
+    for (int32_t i = 0; i < nops; ++i) {
+        const vfvm_op& op = ops[i];
+        if (!check_scl(c, op.unk_id)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "operator " + std::to_string(i) + ": unknown id out of range");
+        if (c.scl[op.unk_id].kind != VFVM_FIELD_UNKNOWN)
+            return set_error(VFVM_ERR_OPERATOR_NON_UNKNOWN, "operator " + std::to_string(i) + ": row field is not an unknown (OperatorNonUnknownInLinearSystem)");
+        if (out.unk < 0) out.unk = op.unk_id;
+        else if (out.unk != op.unk_id) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "operators on more than one unknown are outside the built path");
+        int cls;
+        if (is_dom_op(op.kind)) {
+            if (op.geom_id != 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "domain operator: only domain 0 exists");
+            cls = 0;
+        } else if (is_bnd_op(op.kind)) {
+            if (op.geom_id < 0 || op.geom_id >= c.g.num_bnd) return set_error(VFVM_ERR_INVALID_ARGUMENT, "boundary operator: boundary id out of range");
+            cls = 1 + op.geom_id;
+        } else {
+            return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "unknown operator kind " + std::to_string(op.kind));
+        }
+        switch (op.kind) {
+            case VFVM_OP_SCL_DOM_DIFFUSION:
+            case VFVM_OP_SCL_BND_DIFFUSION:
+                if (!check_scl(c, op.a)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "diffusion: coefficient id out of range");
+                if (op.b != op.unk_id) {
+                    if (check_scl(c, op.b) && c.scl[op.b].kind != VFVM_FIELD_UNKNOWN)
+                        return set_error(VFVM_ERR_OPERATOR_NON_UNKNOWN, "diffusion: driving field is not an unknown (OperatorNonUnknownInLinearSystem)");
+                    return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "diffusion: cross-diffusion into another unknown is outside the built path");
+                }
+                out.has_edges[cls] = 1;
+                break;
+            case VFVM_OP_SCL_DOM_ADVECTION_UNITY:
+            case VFVM_OP_SCL_BND_ADVECTION_UNITY:
+                if (op.a >= 0 && !check_scl(c, op.a)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "advection: diffusion id out of range");
+                if (op.b < 0 || op.b >= (int32_t)c.flw.size()) return set_error(VFVM_ERR_INVALID_ARGUMENT, "advection: flow id out of range");
+                out.has_edges[cls] = 1;
+                break;
+            case VFVM_OP_SCL_DOM_PRODUCTION:
+            case VFVM_OP_SCL_BND_PRODUCTION:
+            case VFVM_OP_SCL_DOM_TIME:
+            case VFVM_OP_SCL_BND_TIME:
+                if (!check_scl(c, op.a)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "coefficient id out of range");
+                out.has_diag[cls] = 1;
+                if (op.kind == VFVM_OP_SCL_DOM_TIME || op.kind == VFVM_OP_SCL_BND_TIME) out.has_time = true;
+                break;
+            case VFVM_OP_SCL_DOM_SOURCE:
+            case VFVM_OP_SCL_BND_SOURCE:
+                if (!check_scl(c, op.a)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "source id out of range");
+                break;
+            case VFVM_OP_SCL_BND_DIRICHLET:
+            case VFVM_OP_SCL_BND_NEUMANN:
+                if (op.a < 0 || op.a >= (int32_t)c.sbnd.size() || c.sbnd[op.a].bnd != op.geom_id)
+                    return set_error(VFVM_ERR_INVALID_ARGUMENT, "boundary value id out of range or on another boundary");
+                if (op.kind == VFVM_OP_SCL_BND_DIRICHLET) out.has_diag[cls] = 1;
+                break;
+            default:
+                return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "unknown operator kind " + std::to_string(op.kind));
+        }
+        if ((int)out.per_class[cls].size() >= MAX_OPS) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "too many operators on one row class");
+        out.per_class[cls].push_back(op);
+    }
+    return VFVM_OK;
+}
+
+int build_program(Ctx& c, const Compiled& cp, DevProgram& p) {
+    std::memset(&p, 0, sizeof(p));
+    SclDom& u = c.scl[cp.unk];
+    for (int cls = 0; cls < cp.nclass; ++cls) {
+        p.nops[cls] = (int32_t)cp.per_class[cls].size();
+        for (size_t k = 0; k < cp.per_class[cls].size(); ++k) {
+            const vfvm_op& op = cp.per_class[cls][k];
+            DevOp& d = p.ops[cls][k];
+            d.kind = op.kind;
+            d.u_grad = u.grad.p;
+            d.u_prev = u.value_prev.p;
+            switch (op.kind) {
+                case VFVM_OP_SCL_BND_DIRICHLET:
+                case VFVM_OP_SCL_BND_NEUMANN:
+                    d.f_val = c.sbnd[op.a].value.p;
+                    break;
+                case VFVM_OP_SCL_DOM_ADVECTION_UNITY:
+                case VFVM_OP_SCL_BND_ADVECTION_UNITY:
+                    d.has_diff = op.a >= 0;
+                    if (op.a >= 0) { d.f_val = c.scl[op.a].value.p; d.f_grad = c.scl[op.a].grad.p; }
+                    d.flux = c.flw[op.b].flux.p;
+                    break;
+                default:
+                    d.f_val = c.scl[op.a].value.p;
+                    d.f_grad = c.scl[op.a].grad.p;
+            }
+        }
+    }
+    return VFVM_OK;
+}
+
+int sync_err(Ctx& c) {
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    return VFVM_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+extern "C" int vfvm_create(int32_t device, vfvm_ctx** out) {
+    if (!out) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return set_error(VFVM_ERR_HIP, "vfvm_create: no HIP device available");
+    if (device < 0 || device >= count) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_create: device index out of range");
+    VFVM_HIP_TRY(hipSetDevice(device));
+    auto* h = new vfvm_ctx();
+    h->c.device = device;
+    e = hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete h; return set_error(VFVM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    *out = h;
+    return clear_error();
+}
+
+extern "C" void vfvm_destroy(vfvm_ctx* ctx) {
+    if (!ctx) return;
+    Ctx& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (auto e : c.ev_pool) (void)hipEventDestroy(e);
+    if (c.h_pinned) (void)hipHostFree(c.h_pinned);
+    hipStream_t s = c.stream;
+    delete ctx;  // frees device buffers
+    if (s) (void)hipStreamDestroy(s);
+}
+
+#define CTX_GUARD(ctx)                                                           \
+    if (!(ctx)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "context is NULL"); \
+    Ctx& c = (ctx)->c;                                                           \
+    VFVM_HIP_TRY(hipSetDevice(c.device))
+
+extern "C" int vfvm_mesh_upload(vfvm_ctx* ctx, const vfvm_mesh* mesh) {
+    CTX_GUARD(ctx);
+    const Mesh* mp = vfvm_mesh_ptr(mesh);
+    if (!mp || !mp->has_dual) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_upload: mesh without dual");
+    const Mesh& m = *mp;
+    // reset everything that depends on the mesh
+    c.scl.clear(); c.vec.clear(); c.sbnd.clear(); c.flw.clear();
+    c.has_pattern = false; c.has_matrix = false; c.ilu_levels_ready = false;
+    DevGeom& g = c.g;
+    g.N = m.num_tot(); g.NP = m.num_pts; g.E = m.num_edges(); g.S = m.num_subfaces();
+    g.num_bnd = (int32_t)m.bnd_name.size();
+    hipStream_t s = c.stream;
+    VFVM_HIP_TRY(g.row_ptr.upload(m.row_ptr.data(), m.row_ptr.size(), s));
+    VFVM_HIP_TRY(g.ngh.upload(m.ngh.data(), m.ngh.size(), s));
+    std::vector<int32_t> erow(g.E);
+    for (int64_t v = 0; v < g.N; ++v)
+        for (int32_t e = m.row_ptr[v]; e < m.row_ptr[v + 1]; ++e) erow[e] = (int32_t)v;
+    VFVM_HIP_TRY(g.erow.upload(erow.data(), erow.size(), s));
+    VFVM_HIP_TRY(g.dist.upload(m.dist.data(), m.dist.size(), s));
+    VFVM_HIP_TRY(g.area.upload(m.area.data(), m.area.size(), s));
+    VFVM_HIP_TRY(g.norm.upload(m.norm.data(), m.norm.size(), s));
+    VFVM_HIP_TRY(g.skew.upload(m.skew.data(), m.skew.size(), s));
+    VFVM_HIP_TRY(g.gw.upload(m.gw.data(), m.gw.size(), s));
+    VFVM_HIP_TRY(g.vol.upload(m.vol.data(), m.vol.size(), s));
+    VFVM_HIP_TRY(g.cskew.upload(m.cskew.data(), m.cskew.size(), s));
+    VFVM_HIP_TRY(g.bsf_ptr.upload(m.bsf_ptr.data(), m.bsf_ptr.size(), s));
+    VFVM_HIP_TRY(g.bsf_vj.upload(m.bsf_vj.data(), m.bsf_vj.size(), s));
+    VFVM_HIP_TRY(g.bsf_vk.upload(m.bsf_vk.data(), m.bsf_vk.size(), s));
+    VFVM_HIP_TRY(g.bsf_area.upload(m.bsf_area.data(), m.bsf_area.size(), s));
+    VFVM_HIP_TRY(g.bsf_wgt.upload(m.bsf_wgt.data(), m.bsf_wgt.size(), s));
+    std::vector<int32_t> vbnd(g.N, -1), bidx(g.N, -1);
+    for (int32_t b = 0; b < g.num_bnd; ++b)
+        for (int64_t k = m.bnd_ptr[b]; k < m.bnd_ptr[b + 1]; ++k) {
+            int64_t v = m.num_pts + m.bnd_nid[k];
+            vbnd[v] = b;
+            bidx[v] = (int32_t)(k - m.bnd_ptr[b]);
+        }
+    VFVM_HIP_TRY(g.vbnd.upload(vbnd.data(), vbnd.size(), s));
+    VFVM_HIP_TRY(g.bidx.upload(bidx.data(), bidx.size(), s));
+    g.bnd_ptr = m.bnd_ptr;
+    g.bnd_nid = m.bnd_nid;
+    g.h_row_ptr = m.row_ptr;
+    g.h_ngh = m.ngh;
+    if (int st = sync_err(c)) return st;
+    c.has_mesh = true;
+    return clear_error();
+}
+
+// ------------------------------------------------------------------ fields
+extern "C" int vfvm_scldom_add(vfvm_ctx* ctx, int32_t kind, double cnst, const double* values, int32_t* id) {
+    CTX_GUARD(ctx);
+    if (!c.has_mesh || !id || kind < 0 || kind > 2) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_scldom_add: invalid argument");
+    const int64_t N = c.g.N;
+    c.scl.emplace_back();
+    SclDom& f = c.scl.back();
+    f.kind = kind;
+    f.cnst = cnst;
+    VFVM_HIP_TRY(f.value.alloc(N));
+    VFVM_HIP_TRY(f.value_prev.alloc(N));
+    VFVM_HIP_TRY(f.grad.alloc(3 * N));
+    VFVM_HIP_TRY(hipMemsetAsync(f.grad.p, 0, 3 * N * sizeof(double), c.stream));
+    if (values) VFVM_HIP_TRY(hipMemcpyAsync(f.value.p, values, N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    else hipLaunchKernelGGL(k_fill, dim3(grid_for(N)), dim3(BLK), 0, c.stream, N, cnst, f.value.p);
+    VFVM_HIP_TRY(hipMemcpyAsync(f.value_prev.p, f.value.p, N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    if (int st = sync_err(c)) return st;
+    *id = (int32_t)c.scl.size() - 1;
+    return clear_error();
+}
+
+extern "C" int vfvm_scldom_set(vfvm_ctx* ctx, int32_t id, const double* values) {
+    CTX_GUARD(ctx);
+    if (!check_scl(c, id) || !values) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_scldom_set: invalid argument");
+    VFVM_HIP_TRY(hipMemcpyAsync(c.scl[id].value.p, values, c.g.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (int st = sync_err(c)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_scldom_get(vfvm_ctx* ctx, int32_t id, double* values) {
+    CTX_GUARD(ctx);
+    if (!check_scl(c, id) || !values) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_scldom_get: invalid argument");
+    VFVM_HIP_TRY(hipMemcpyAsync(values, c.scl[id].value.p, c.g.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (int st = sync_err(c)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_scldom_update_prev(vfvm_ctx* ctx, int32_t id) {
+    CTX_GUARD(ctx);
+    if (!check_scl(c, id)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_scldom_update_prev: id out of range");
+    VFVM_HIP_TRY(hipMemcpyAsync(c.scl[id].value_prev.p, c.scl[id].value.p, c.g.N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    if (int st = sync_err(c)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_vecdom_add(vfvm_ctx* ctx, int32_t kind, const double* cnst3, const double* values, int32_t* id) {
+    CTX_GUARD(ctx);
+    if (!c.has_mesh || !id || kind < 0 || kind > 1 || (!values && !cnst3))
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_vecdom_add: invalid argument (vector unknowns are outside the built path)");
+    const int64_t N = c.g.N;
+    c.vec.emplace_back();
+    VecDom& f = c.vec.back();
+    f.kind = kind;
+    VFVM_HIP_TRY(f.value.alloc(3 * N));
+    VFVM_HIP_TRY(f.grad.alloc(9 * N));
+    VFVM_HIP_TRY(hipMemsetAsync(f.grad.p, 0, 9 * N * sizeof(double), c.stream));
+    if (values) VFVM_HIP_TRY(hipMemcpyAsync(f.value.p, values, 3 * N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    else
+        for (int k = 0; k < 3; ++k) hipLaunchKernelGGL(k_fill, dim3(grid_for(N)), dim3(BLK), 0, c.stream, N, cnst3[k], f.value.p + k * N);
+    if (int st = sync_err(c)) return st;
+    *id = (int32_t)c.vec.size() - 1;
+    return clear_error();
+}
+
+extern "C" int vfvm_vecdom_set(vfvm_ctx* ctx, int32_t id, const double* values) {
+    CTX_GUARD(ctx);
+    if (id < 0 || id >= (int32_t)c.vec.size() || !values) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_vecdom_set: invalid argument");
+    VFVM_HIP_TRY(hipMemcpyAsync(c.vec[id].value.p, values, 3 * c.g.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (int st = sync_err(c)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_sclbnd_add(vfvm_ctx* ctx, int32_t bnd_id, double cnst, const double* values, int32_t* id) {
+    CTX_GUARD(ctx);
+    if (!c.has_mesh || !id || bnd_id < 0 || bnd_id >= c.g.num_bnd) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_sclbnd_add: invalid argument");
+    const int64_t cnt = c.g.bnd_ptr[bnd_id + 1] - c.g.bnd_ptr[bnd_id];
+    c.sbnd.emplace_back();
+    SclBnd& f = c.sbnd.back();
+    f.bnd = bnd_id;
+    VFVM_HIP_TRY(f.value.alloc(std::max<int64_t>(cnt, 1)));
+    if (values) VFVM_HIP_TRY(hipMemcpyAsync(f.value.p, values, cnt * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    else hipLaunchKernelGGL(k_fill, dim3(grid_for(std::max<int64_t>(cnt, 1))), dim3(BLK), 0, c.stream, std::max<int64_t>(cnt, 1), cnst, f.value.p);
+    if (int st = sync_err(c)) return st;
+    *id = (int32_t)c.sbnd.size() - 1;
+    return clear_error();
+}
+
+extern "C" int vfvm_sclbnd_set(vfvm_ctx* ctx, int32_t id, const double* values) {
+    CTX_GUARD(ctx);
+    if (id < 0 || id >= (int32_t)c.sbnd.size() || !values) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_sclbnd_set: invalid argument");
+    const int32_t b = c.sbnd[id].bnd;
+    const int64_t cnt = c.g.bnd_ptr[b + 1] - c.g.bnd_ptr[b];
+    VFVM_HIP_TRY(hipMemcpyAsync(c.sbnd[id].value.p, values, cnt * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (int st = sync_err(c)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_sclflw_add(vfvm_ctx* ctx, int32_t vel_id, int32_t pres_id, int32_t den_id, int32_t visc_id, int32_t fce_id, int32_t* id) {
+    CTX_GUARD(ctx);
+    if (!id || vel_id < 0 || vel_id >= (int32_t)c.vec.size() || !check_scl(c, pres_id) || !check_scl(c, den_id) ||
+        !check_scl(c, visc_id) || fce_id < 0 || fce_id >= (int32_t)c.vec.size())
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_sclflw_add: field id out of range");
+    if (c.scl[pres_id].kind != VFVM_FIELD_CONSTANT)
+        return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "vfvm_sclflw_add: only a prescribed flow (uniform constant pressure) is on the built path");
+    c.flw.emplace_back();
+    SclFlw& f = c.flw.back();
+    f.vel = vel_id; f.pres = pres_id; f.den = den_id; f.visc = visc_id; f.fce = fce_id;
+    VFVM_HIP_TRY(f.flux.alloc(c.g.E));
+    *id = (int32_t)c.flw.size() - 1;
+    return clear_error();
+}
+
+// ------------------------------------------------------------------ pattern (LinearBase::update_sparse)
+extern "C" int vfvm_pattern_build(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, int64_t* nnz) {
+    CTX_GUARD(ctx);
+    Compiled cp;
+    if (int st = compile_ops(c, ops, nops, cp)) return st;
+    const DevGeom& g = c.g;
+    const int64_t N = g.N;
+    // single unknown: block start 0 (SteadyHeat::initial_start, steady_heat.rs:68-78)
+    c.unk = cp.unk;
+    c.n = N;
+    std::vector<int32_t> vbnd_h(N, -1);
+    for (int32_t b = 0; b < g.num_bnd; ++b)
+        for (int64_t k = g.bnd_ptr[b]; k < g.bnd_ptr[b + 1]; ++k) vbnd_h[g.NP + g.bnd_nid[k]] = b;
+    c.h_mrow.assign(N + 1, 0);
+    c.h_mcol.clear();
+    std::vector<int32_t> eslot(g.E, -1), dslot(N, -1);
+    for (int64_t v = 0; v < N; ++v) {
+        const int cls = vbnd_h[v] < 0 ? 0 : 1 + vbnd_h[v];
+        const int32_t es = g.h_row_ptr[v], ee = g.h_row_ptr[v + 1];
+        if (cp.has_edges[cls]) {
+            // {v} u ngh(v), ascending (opscl_dom_diff.rs:84-85)
+            bool placed = false;
+            for (int32_t e = es; e < ee; ++e) {
+                const int32_t n = g.h_ngh[e];
+                if (!placed && n > v) { dslot[v] = (int32_t)c.h_mcol.size(); c.h_mcol.push_back((int32_t)v); placed = true; }
+                eslot[e] = (int32_t)c.h_mcol.size();
+                c.h_mcol.push_back(n);
+            }
+            if (!placed) { dslot[v] = (int32_t)c.h_mcol.size(); c.h_mcol.push_back((int32_t)v); }
+        } else if (cp.has_diag[cls]) {
+            // {v}: Dirichlet, production- or time-only rows (opscl_bnd_dir.rs:73-74)
+            dslot[v] = (int32_t)c.h_mcol.size();
+            c.h_mcol.push_back((int32_t)v);
+        }
+        if ((int64_t)c.h_mcol.size() > INT32_MAX) return set_error(VFVM_ERR_INVALID_ARGUMENT, "pattern exceeds 32-bit indices");
+        c.h_mrow[v + 1] = (int32_t)c.h_mcol.size();
+    }
+    c.nnz = (int64_t)c.h_mcol.size();
+    hipStream_t s = c.stream;
+    VFVM_HIP_TRY(c.mrow.upload(c.h_mrow.data(), c.h_mrow.size(), s));
+    VFVM_HIP_TRY(c.mcol.upload(c.h_mcol.data(), c.h_mcol.size(), s));
+    VFVM_HIP_TRY(c.eslot.upload(eslot.data(), eslot.size(), s));
+    VFVM_HIP_TRY(c.dslot.upload(dslot.data(), dslot.size(), s));
+    VFVM_HIP_TRY(c.vals.alloc(std::max<int64_t>(c.nnz, 1)));
+    VFVM_HIP_TRY(c.b.alloc(N));
+    DevProgram p;
+    build_program(c, cp, p);
+    VFVM_HIP_TRY(c.prog.upload(&p, 1, s));
+    if (int st = sync_err(c)) return st;
+    c.pattern_ops.assign(ops, ops + nops);
+    c.has_pattern = true;
+    c.has_matrix = false;
+    c.ilu_levels_ready = false;
+    if (nnz) *nnz = c.nnz;
+    return clear_error();
+}
+
+extern "C" int vfvm_pattern_get(vfvm_ctx* ctx, int32_t* row_ptr, int32_t* col) {
+    CTX_GUARD(ctx);
+    if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_pattern_get: no pattern built");
+    if (row_ptr) std::memcpy(row_ptr, c.h_mrow.data(), c.h_mrow.size() * sizeof(int32_t));
+    if (col) std::memcpy(col, c.h_mcol.data(), c.h_mcol.size() * sizeof(int32_t));
+    return clear_error();
+}
+
+// ------------------------------------------------------------------ assembly
+namespace {
+int assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
+    if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "assemble: no pattern built (call vfvm_pattern_build)");
+    Compiled cp;
+    if (int st = compile_ops(c, ops, nops, cp)) return st;
+    if (cp.unk != c.unk) return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: operators act on another unknown than the pattern");
+    {
+        Compiled pp;
+        if (int st = compile_ops(c, c.pattern_ops.data(), (int32_t)c.pattern_ops.size(), pp)) return st;
+        if (pp.has_edges != cp.has_edges || pp.has_diag != cp.has_diag)
+            return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: operators do not fit the built pattern");
+    }
+    if (cp.has_time && !(dt > 0.0)) return set_error(VFVM_ERR_OPERATOR_USE_APPLY_TIME, "time operators need dt > 0 (OperatorUseApplyTime)");
+    // gradients of every non-constant field the operators read (ScalarDomain::update_grad)
+    std::vector<uint8_t> need_s(c.scl.size(), 0), need_v(c.vec.size(), 0), need_f(c.flw.size(), 0);
+    for (int32_t i = 0; i < nops; ++i) {
+        const vfvm_op& op = ops[i];
+        switch (op.kind) {
+            case VFVM_OP_SCL_DOM_PRODUCTION:
+            case VFVM_OP_SCL_BND_PRODUCTION:
+                need_s[op.a] = 1; need_s[op.unk_id] = 1; break;
+            case VFVM_OP_SCL_DOM_ADVECTION_UNITY:
+            case VFVM_OP_SCL_BND_ADVECTION_UNITY:
+                if (op.a >= 0) need_s[op.a] = 1;
+                need_f[op.b] = 1; break;
+            case VFVM_OP_SCL_BND_DIRICHLET:
+            case VFVM_OP_SCL_BND_NEUMANN:
+                break;
+            default:
+                need_s[op.a] = 1;
+        }
+    }
+    for (size_t f = 0; f < c.flw.size(); ++f)
+        if (need_f[f]) { need_s[c.flw[f].den] = 1; need_v[c.flw[f].vel] = 1; }
+    for (size_t i = 0; i < c.scl.size(); ++i)
+        if (need_s[i] && c.scl[i].kind != VFVM_FIELD_CONSTANT)
+            if (int st = launch_grad(c, c.scl[i].value.p, c.scl[i].grad.p)) return st;
+    for (size_t i = 0; i < c.vec.size(); ++i)
+        if (need_v[i] && c.vec[i].kind != VFVM_FIELD_CONSTANT)
+            if (int st = launch_grad_vec(c, c.vec[i].value.p, c.vec[i].grad.p)) return st;
+    for (size_t f = 0; f < c.flw.size(); ++f)
+        if (need_f[f]) if (int st = launch_flux_prescribed(c, c.flw[f])) return st;
+    DevProgram p;
+    build_program(c, cp, p);
+    VFVM_HIP_TRY(hipMemcpyAsync(c.prog.p, &p, sizeof(p), hipMemcpyHostToDevice, c.stream));
+    if (int st = launch_assemble(c, dt)) return st;
+    c.has_matrix = true;
+    return VFVM_OK;
+}
+}  // namespace
+
+extern "C" int vfvm_assemble(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, double dt) {
+    CTX_GUARD(ctx);
+    if (int st = assemble_impl(c, ops, nops, dt)) return st;
+    if (int st = sync_err(c)) return st;
+    if (c.timing) if (int st = ktime_flush(c)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_matrix_get(vfvm_ctx* ctx, double* vals, double* b) {
+    CTX_GUARD(ctx);
+    if (!c.has_matrix) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_matrix_get: nothing assembled");
+    if (vals) VFVM_HIP_TRY(hipMemcpyAsync(vals, c.vals.p, c.nnz * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (b) VFVM_HIP_TRY(hipMemcpyAsync(b, c.b.p, c.n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (int st = sync_err(c)) return st;
+    return clear_error();
+}
+
+// ------------------------------------------------------------------ solve
+extern "C" void vfvm_solve_opts_default(vfvm_solve_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->tol = 1e-6;
+    o->restart = 50;
+    o->max_outer = 200;
+    o->precision = VFVM_FP64;
+    o->precond = VFVM_PRECOND_JACOBI;
+    o->inner_tol = 1e-5;
+    o->max_refine = 0;
+}
+
+extern "C" int vfvm_solve_device(vfvm_ctx* ctx, const vfvm_solve_opts* opts, double* x_dev, vfvm_solve_stats* stats) {
+    CTX_GUARD(ctx);
+    if (!opts || !x_dev) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_solve_device: NULL argument");
+    if (!c.has_matrix) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_solve_device: nothing assembled");
+    vfvm_solve_stats st;
+    int r = gmres_solve(c, *opts, x_dev, st);
+    if (stats) *stats = st;
+    if (r != VFVM_OK) return r;
+    return clear_error();
+}
+
+extern "C" int vfvm_solve(vfvm_ctx* ctx, const vfvm_solve_opts* opts, double* x, vfvm_solve_stats* stats) {
+    CTX_GUARD(ctx);
+    if (!opts || !x) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_solve: NULL argument");
+    if (!c.has_matrix) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_solve: nothing assembled");
+    VFVM_HIP_TRY(c.x64.alloc(c.n));
+    VFVM_HIP_TRY(hipMemcpyAsync(c.x64.p, x, c.n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    vfvm_solve_stats st;
+    int r = gmres_solve(c, *opts, c.x64.p, st);
+    if (stats) *stats = st;
+    VFVM_HIP_TRY(hipMemcpyAsync(x, c.x64.p, c.n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (int s2 = sync_err(c)) return s2;
+    if (r != VFVM_OK) return r;
+    return clear_error();
+}
+
+extern "C" int vfvm_residual(vfvm_ctx* ctx, const double* x, double* rel_residual) {
+    CTX_GUARD(ctx);
+    if (!x || !rel_residual) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_residual: NULL argument");
+    if (!c.has_matrix) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_residual: nothing assembled");
+    VFVM_HIP_TRY(c.x64.alloc(c.n));
+    VFVM_HIP_TRY(hipMemcpyAsync(c.x64.p, x, c.n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    double bn = 0.0;
+    if (int st = residual_norm(c, c.x64.p, rel_residual, &bn)) return st;
+    return clear_error();
+}
+
+// ------------------------------------------------------------------ drivers
+namespace {
+
+int initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
+    // OpSclBndDirichlet::apply_initial for every Dirichlet operator (steady_base.rs:53-54)
+    for (int32_t i = 0; i < nops; ++i) {
+        if (ops[i].kind != VFVM_OP_SCL_BND_DIRICHLET) continue;
+        const int32_t b = ops[i].geom_id;
+        const int64_t off = c.g.bnd_ptr[b], cnt = c.g.bnd_ptr[b + 1] - off;
+        if (cnt == 0) continue;
+        DevBuf<int32_t> nid;
+        VFVM_HIP_TRY(nid.upload(c.g.bnd_nid.data() + off, cnt, c.stream));
+        hipLaunchKernelGGL(k_scatter_bnd, dim3(grid_for(cnt)), dim3(BLK), 0, c.stream, cnt, c.g.NP, nid.p, c.sbnd[ops[i].a].value.p, c.scl[ops[i].unk_id].value.p);
+        VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    }
+    return VFVM_OK;
+}
+
+// scaled Picard residual for one scalar field (scaling.rs:88-135)
+int picard_residual(Ctx& c, const double* xo, const double* xn, double* res) {
+    const int64_t n = c.n;
+    const unsigned nb = grid_for(n);
+    DevBuf<double> part;
+    VFVM_HIP_TRY(part.alloc(4 * (size_t)nb));
+    hipLaunchKernelGGL(k_picard, dim3(nb), dim3(BLK), 0, c.stream, n, xo, xn, part.p);
+    std::vector<double> h(4 * (size_t)nb);
+    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    double sum = 0.0, mo = 0.0, mn = 0.0, nf = 0.0;
+    for (unsigned k = 0; k < nb; ++k) { sum += h[4 * k]; mo = std::max(mo, h[4 * k + 1]); mn = std::max(mn, h[4 * k + 2]); nf += h[4 * k + 3]; }
+    if (nf > 0.0) { *res = NAN; return VFVM_OK; }
+    const double rms = std::sqrt(sum / (double)std::max<int64_t>(n, 1));
+    const double scale = std::max(std::max(mo, mn), 1e-30);
+    *res = rms / scale;
+    return VFVM_OK;
+}
+
+using clk = std::chrono::steady_clock;
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* opts, double dt, int32_t max_iter,
+                double tol, double damp, vfvm_driver_stats& ds, int ts, bool transient) {
+    const int64_t n = c.n;
+    SclDom& u = c.scl[c.unk];
+    DevBuf<double> xi, xu;
+    VFVM_HIP_TRY(xi.alloc(n));
+    VFVM_HIP_TRY(xu.alloc(n));
+    auto t0 = clk::now();
+    // x_iter = current unknown values (_unkn_to_solu, steady_base.rs:137)
+    VFVM_HIP_TRY(hipMemcpyAsync(xi.p, u.value.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    if (int st = assemble_impl(c, ops, nops, dt)) return st;
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    auto t1 = clk::now();
+    ds.time_assembly += secs(t0, t1);
+    for (int32_t iter = 0; iter < max_iter; ++iter) {
+        auto ti0 = clk::now();
+        VFVM_HIP_TRY(hipMemcpyAsync(xu.p, xi.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        vfvm_solve_stats st;
+        if (int r = gmres_solve(c, *opts, xu.p, st)) return r;
+        ds.linear_iterations += st.iterations;
+        auto ti1 = clk::now();
+        ds.time_solve += secs(ti0, ti1);
+        // damping into the unknown; re-assemble at the new iterate
+        hipLaunchKernelGGL(k_damp, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, damp, xi.p, xu.p, u.value.p);
+        if (int st2 = assemble_impl(c, ops, nops, dt)) return st2;
+        VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+        ds.time_assembly += secs(ti1, clk::now());
+        double res = 0.0;
+        if (int st3 = picard_residual(c, xi.p, u.value.p, &res)) return st3;
+        ds.residual = res;
+        ds.picard_iterations += 1;
+        if (!std::isfinite(res))
+            return set_error(transient ? VFVM_ERR_TRANSIENT_NON_FINITE_RESIDUAL : VFVM_ERR_STEADY_NON_FINITE_RESIDUAL,
+                             "non-finite Picard residual at iteration " + std::to_string(iter) + (transient ? " of step " + std::to_string(ts) : ""));
+        VFVM_HIP_TRY(hipMemcpyAsync(xi.p, u.value.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        if (res < tol) return VFVM_OK;
+    }
+    char buf[160];
+    snprintf(buf, sizeof(buf), "Picard iteration did not converge in %d iterations (residual %.3e)", max_iter, ds.residual);
+    return set_error(transient ? VFVM_ERR_TRANSIENT_DID_NOT_CONVERGE : VFVM_ERR_STEADY_DID_NOT_CONVERGE, buf);
+}
+
+}  // namespace
+
+extern "C" int vfvm_steady_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* opts,
+                                 int32_t max_iter, double tol, double damp, vfvm_driver_stats* stats) {
+    CTX_GUARD(ctx);
+    // steady_base.rs:21-23
+    if (max_iter <= 0) return set_error(VFVM_ERR_SOLVER_NON_POSITIVE_MAX_ITER, "max_iter must be positive");
+    if (std::isnan(tol) || tol <= 0.0) return set_error(VFVM_ERR_SOLVER_NON_POSITIVE_TOLERANCE, "tol must be positive");
+    if (std::isnan(damp) || damp <= 0.0 || damp > 1.0) return set_error(VFVM_ERR_SOLVER_INVALID_DAMPING, "damp must be in (0, 1]");
+    if (!opts) return set_error(VFVM_ERR_INVALID_ARGUMENT, "opts is NULL");
+    vfvm_driver_stats ds;
+    std::memset(&ds, 0, sizeof(ds));
+    if (int st = vfvm_pattern_build(ctx, ops, nops, nullptr)) return st;
+    if (int st = initial_dirichlet(c, ops, nops)) return st;
+    int r = picard_loop(c, ops, nops, opts, 0.0, max_iter, tol, damp, ds, 0, false);
+    if (stats) *stats = ds;
+    if (r != VFVM_OK) return r;
+    return clear_error();
+}
+
+extern "C" int vfvm_transient_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* opts,
+                                    double dt, int32_t num_ts, int32_t max_iter, double tol, double damp,
+                                    vfvm_driver_stats* stats) {
+    CTX_GUARD(ctx);
+    // transient_base.rs:21-26
+    if (std::isnan(dt) || dt <= 0.0) return set_error(VFVM_ERR_SOLVER_NON_POSITIVE_TIME_STEP, "dt must be positive");
+    if (num_ts <= 0) return set_error(VFVM_ERR_SOLVER_NON_POSITIVE_NUM_STEPS, "num_ts must be positive");
+    if (max_iter <= 0) return set_error(VFVM_ERR_SOLVER_NON_POSITIVE_MAX_ITER, "max_iter must be positive");
+    if (std::isnan(tol) || tol <= 0.0) return set_error(VFVM_ERR_SOLVER_NON_POSITIVE_TOLERANCE, "tol must be positive");
+    if (std::isnan(damp) || damp <= 0.0 || damp > 1.0) return set_error(VFVM_ERR_SOLVER_INVALID_DAMPING, "damp must be in (0, 1]");
+    if (!opts) return set_error(VFVM_ERR_INVALID_ARGUMENT, "opts is NULL");
+    vfvm_driver_stats ds;
+    std::memset(&ds, 0, sizeof(ds));
+    if (int st = vfvm_pattern_build(ctx, ops, nops, nullptr)) return st;
+    if (int st = initial_dirichlet(c, ops, nops)) return st;
+    // the initial state is the previous state of the first step (transient_base.rs:59-62)
+    SclDom& u = c.scl[c.unk];
+    VFVM_HIP_TRY(hipMemcpyAsync(u.value_prev.p, u.value.p, c.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    int r = VFVM_OK;
+    for (int32_t ts = 0; ts < num_ts && r == VFVM_OK; ++ts) {
+        r = picard_loop(c, ops, nops, opts, dt, max_iter, tol, damp, ds, ts, true);
+        if (r == VFVM_OK) VFVM_HIP_TRY(hipMemcpyAsync(u.value_prev.p, u.value.p, c.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    }
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    if (stats) *stats = ds;
+    if (r != VFVM_OK) return r;
+    return clear_error();
+}
+
+// ------------------------------------------------------------------ timing
+extern "C" int vfvm_timing_enable(vfvm_ctx* ctx, int32_t enable) {
+    CTX_GUARD(ctx);
+    c.timing = enable != 0;
+    return clear_error();
+}
+extern "C" int vfvm_timing_get(vfvm_ctx* ctx, int32_t kclass, int64_t* launches, double* total_ms, double* bytes) {
+    CTX_GUARD(ctx);
+    if (kclass < 0 || kclass >= VFVM_K_COUNT) return set_error(VFVM_ERR_INVALID_ARGUMENT, "kernel class out of range");
+    if (int st = ktime_flush(c)) return st;
+    if (launches) *launches = c.kt[kclass].launches;
+    if (total_ms) *total_ms = c.kt[kclass].ms;
+    if (bytes) *bytes = c.kt[kclass].bytes;
+    return clear_error();
+}
+extern "C" int vfvm_timing_reset(vfvm_ctx* ctx) {
+    CTX_GUARD(ctx);
+    if (int st = ktime_flush(c)) return st;
+    for (auto& k : c.kt) k = KTimer();
+    return clear_error();
+}

@@ -1,0 +1,163 @@
+// Device context: geometry, fields, pattern, matrix and solver workspace in HBM.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "mesh.hpp"
+
+namespace vfvm {
+
+#define VFVM_HIP_TRY(expr)                                                                    \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return set_error(VFVM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// Owning device allocation.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; return *this; }
+    ~DevBuf() { reset(); }
+    void reset() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    hipError_t alloc(size_t count) {
+        if (count == n && p) return hipSuccess;
+        reset();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e == hipSuccess) n = count; else p = nullptr;
+        return e;
+    }
+    hipError_t upload(const T* h, size_t count, hipStream_t s) {
+        hipError_t e = alloc(count);
+        if (e != hipSuccess || count == 0) return e;
+        return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+};
+
+// Device copy of the dual geometry (SoA, CSR edge order).
+struct DevGeom {
+    int64_t N = 0, NP = 0, E = 0, S = 0;
+    int32_t num_bnd = 0;
+    DevBuf<int32_t> row_ptr, ngh, erow;       // erow[e] = row of directed edge e
+    DevBuf<double> dist, area, norm, skew, gw; // norm/skew/gw: 3E SoA
+    DevBuf<double> vol, cskew;                 // cskew: 3N SoA
+    DevBuf<int32_t> bsf_ptr, bsf_vj, bsf_vk;
+    DevBuf<double> bsf_area, bsf_wgt;          // bsf_wgt: 3S SoA
+    DevBuf<int32_t> vbnd, bidx;                // vertex -> boundary (-1: point), index in boundary
+    std::vector<int64_t> bnd_ptr;              // host copy
+    std::vector<int32_t> bnd_nid;
+    std::vector<int32_t> h_row_ptr, h_ngh;     // host copies for the pattern build
+};
+
+struct SclDom {
+    int kind = 0;
+    double cnst = 0.0;
+    bool grad_valid = false;
+    DevBuf<double> value, grad, value_prev;  // grad: 3N SoA
+};
+struct VecDom {
+    int kind = 0;
+    bool grad_valid = false;
+    DevBuf<double> value, grad;  // value 3N SoA, grad 9N SoA (u_x,u_y,u_z,v_x,...)
+};
+struct SclBnd {
+    int32_t bnd = 0;
+    DevBuf<double> value;  // per node of the boundary
+};
+struct SclFlw {
+    int32_t vel = -1, pres = -1, den = -1, visc = -1, fce = -1;
+    DevBuf<double> flux;  // E
+};
+
+// One operator as the assembly kernel reads it.
+struct DevOp {
+    int32_t kind;
+    int32_t has_diff;
+    const double* f_val;   // coefficient field value (or sclbnd values)
+    const double* f_grad;  // coefficient field gradient (3N SoA)
+    const double* u_grad;  // unknown gradient (production)
+    const double* u_prev;  // unknown previous value (time)
+    const double* flux;    // face mass flux (advection)
+};
+constexpr int MAX_CLASSES = 16;  // class 0 = points, class 1 + b = boundary b
+constexpr int MAX_OPS = 8;
+struct DevProgram {
+    int32_t nops[MAX_CLASSES];
+    DevOp ops[MAX_CLASSES][MAX_OPS];
+};
+
+struct KTimer {
+    int64_t launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevGeom g;
+    bool has_mesh = false;
+    std::vector<SclDom> scl;
+    std::vector<VecDom> vec;
+    std::vector<SclBnd> sbnd;
+    std::vector<SclFlw> flw;
+
+    // pattern
+    bool has_pattern = false;
+    int32_t unk = -1;
+    int64_t n = 0, nnz = 0;
+    std::vector<int32_t> h_mrow, h_mcol;
+    DevBuf<int32_t> mrow, mcol, eslot, dslot;
+    std::vector<vfvm_op> pattern_ops;
+    DevBuf<DevProgram> prog;
+    DevBuf<double> vals, b;
+    bool has_matrix = false;
+
+    // solver workspace
+    DevBuf<double> V64, w64, z64, x64, r64, dinv64, avals64, part, scal;
+    DevBuf<float> V32, w32, avals32;
+    DevBuf<int32_t> ilu_diag, ilu_lvl_rows, ilu_lvl_ptr;
+    DevBuf<double> ilu64;
+    DevBuf<float> ilu32;
+    std::vector<int32_t> h_lvl_ptr;
+    bool ilu_levels_ready = false;
+    double* h_pinned = nullptr;  // small pinned readback buffer
+
+    // timing
+    bool timing = false;
+    std::array<KTimer, VFVM_K_COUNT> kt;
+    std::vector<hipEvent_t> ev_pool;
+    struct Pending { int kclass; double bytes; hipEvent_t a, b; int step; };
+    int cur_step = -1;
+    std::vector<Pending> pending;
+    size_t ev_next = 0;
+};
+
+// timing helpers (no-ops unless timing is on)
+void ktime_begin(Ctx& c, int kclass, hipEvent_t* a);
+void ktime_end(Ctx& c, int kclass, double bytes, hipEvent_t a);
+int ktime_flush(Ctx& c);
+
+// kernels (kernels_asm.hip)
+int launch_grad(Ctx& c, const double* value, double* grad);
+int launch_grad_vec(Ctx& c, const double* value3, double* grad9);
+int launch_flux_prescribed(Ctx& c, SclFlw& f);
+int launch_assemble(Ctx& c, double dt);
+
+// solver (krylov.hip)
+int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x_dev, vfvm_solve_stats& st);
+int residual_norm(Ctx& c, const double* x_dev, double* rel, double* bnorm);
+
+}  // namespace vfvm

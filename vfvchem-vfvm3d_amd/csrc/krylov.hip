@@ -1,0 +1,792 @@
+// Restarted GMRES on the device: CSR SpMV, Jacobi and ILU(0) right
+// preconditioning, classical Gram-Schmidt with one re-orthogonalisation pass
+// (CGS2), Givens least squares, FP64 and FP32-inner / FP64-refined variants.
+//
+// Replaces LinearGmres::solve_matrix (src/linear/linear_gmres.rs:76-84), which
+// calls faer_gmres::restarted_gmres(A, b, x, restart, max_outer, tol, Jacobi).
+// faer_gmres 0.5.0 is not vendored with the reference; this solver implements
+// the textbook algorithm (Saad, Iterative Methods for Sparse Linear Systems,
+// 2nd ed., Alg. 6.9 GMRES(m) with right preconditioning, section 10.3 ILU(0))
+// and stops on the FP64 relative residual ||b - A x||_2 / ||b||_2 <= tol.
+//
+// Device-resident state: the Krylov basis V (column-major, n x (m+1)), the
+// Hessenberg matrix, Givens rotations and the control word live in HBM; the
+// host only reads the control word every few steps.  Reductions are two-stage
+// and fixed-order, so results are bitwise reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace vfvm {
+
+namespace {
+
+constexpr int BLK = 256;
+constexpr int WAVE = 64;
+constexpr int CHUNK = 8;      // basis vectors per pass of the multi-dot
+constexpr int MAX_RESTART = 256;
+
+// ---------------------------------------------------------------- state layout (doubles)
+struct StateLayout {
+    int m;
+    __host__ __device__ int64_t H() const { return 0; }                           // (m+1) x m column-major
+    __host__ __device__ int64_t cs() const { return (int64_t)(m + 1) * m; }
+    __host__ __device__ int64_t sn() const { return cs() + m; }
+    __host__ __device__ int64_t g() const { return sn() + m; }                     // m+1
+    __host__ __device__ int64_t y() const { return g() + m + 1; }                  // m
+    __host__ __device__ int64_t h1() const { return y() + m; }                     // m+1
+    __host__ __device__ int64_t h2() const { return h1() + m + 1; }                // m+1
+    __host__ __device__ int64_t misc() const { return h2() + m + 1; }              // [0]=beta [1]=inv_h [2]=target [3]=res_est [4]=scale
+    __host__ __device__ int64_t total() const { return misc() + 8; }
+};
+// control word (ints): [0] active, [1] steps done in this cycle, [2] non-finite flag
+constexpr int CTRL_INTS = 4;
+
+__device__ inline double wave_sum(double v) {
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_down(v, off, WAVE);
+    return v;
+}
+
+// fixed-order block sum over BLK threads; result valid in thread 0
+__device__ inline double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < BLK / WAVE; ++k) r += sh[k];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------- SpMV
+// LPR lanes per row; scalar rows carry ~17 entries, so 4 or 8 lanes per row
+template <class T, int LPR>
+__global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                              const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
+                                              const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0] == 0) return;
+    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t row = tid / LPR;
+    const int lane = (int)(tid % LPR);
+    if (row >= n) return;
+    T s = 0;
+    const int32_t e1 = rp[row + 1];
+    for (int32_t k = rp[row] + lane; k < e1; k += LPR) s += val[k] * x[col[k]];
+    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
+    if (lane == 0) y[row] = s;
+}
+
+// r = b - A x (FP64) and per-block partial sums of r^2 and b^2
+template <int LPR>
+__global__ void __launch_bounds__(BLK) k_residual(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                  const double* __restrict__ val, const double* __restrict__ x,
+                                                  const double* __restrict__ b, double* __restrict__ r,
+                                                  double* __restrict__ part, int nblk) {
+    __shared__ double sh[BLK / WAVE];
+    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t row = tid / LPR;
+    const int lane = (int)(tid % LPR);
+    double s = 0.0;
+    if (row < n) {
+        const int32_t e1 = rp[row + 1];
+        for (int32_t k = rp[row] + lane; k < e1; k += LPR) s += val[k] * x[col[k]];
+    }
+    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
+    double rr = 0.0, bb = 0.0;
+    if (row < n && lane == 0) {
+        double bi = b[row];
+        double ri = bi - s;
+        r[row] = ri;
+        rr = ri * ri;
+        bb = bi * bi;
+    }
+    double t1 = block_sum(rr, sh);
+    double t2 = block_sum(bb, sh);
+    if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[nblk + blockIdx.x] = t2; }
+}
+
+// ---------------------------------------------------------------- multi-dot: part[k][blk] = sum_i V[k][i] w[i]
+template <class T>
+__global__ void __launch_bounds__(BLK) k_mdot(int64_t n, int nv, const T* __restrict__ V, int64_t ldv,
+                                              const T* __restrict__ w, double* __restrict__ part, int nblk,
+                                              int64_t rows_per_blk, const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0] == 0) return;
+    __shared__ double sh[CHUNK][BLK / WAVE];
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_blk;
+    const int64_t r1 = min(n, r0 + rows_per_blk);
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    for (int k0 = 0; k0 < nv; k0 += CHUNK) {
+        double acc[CHUNK];
+#pragma unroll
+        for (int q = 0; q < CHUNK; ++q) acc[q] = 0.0;
+        const int kq = min(CHUNK, nv - k0);
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += BLK) {
+            const double wi = (double)w[i];
+#pragma unroll
+            for (int q = 0; q < CHUNK; ++q)
+                if (q < kq) acc[q] += (double)V[(int64_t)(k0 + q) * ldv + i] * wi;
+        }
+#pragma unroll
+        for (int q = 0; q < CHUNK; ++q) {
+            double s = wave_sum(acc[q]);
+            if (lane == 0) sh[q][wid] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < kq) {
+            double s = 0.0;
+            for (int k = 0; k < BLK / WAVE; ++k) s += sh[threadIdx.x][k];
+            part[(int64_t)(k0 + threadIdx.x) * nblk + blockIdx.x] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// out[k] (+)= sum_blk part[k][blk]; one block per k
+__global__ void __launch_bounds__(BLK) k_reduce(int nblk, const double* __restrict__ part, double* __restrict__ out,
+                                                int accumulate, const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0] == 0) return;
+    __shared__ double sh[BLK / WAVE];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += BLK) s += part[(int64_t)k * nblk + i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[k] = accumulate ? out[k] + s : s;
+}
+
+// w -= sum_k V[k] h[k]; optional per-block partial of ||w||^2
+template <class T>
+__global__ void __launch_bounds__(BLK) k_update(int64_t n, int nv, const T* __restrict__ V, int64_t ldv,
+                                                const double* __restrict__ h, T* __restrict__ w,
+                                                double* __restrict__ part_nrm, int64_t rows_per_blk,
+                                                const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0] == 0) return;
+    __shared__ double hs[MAX_RESTART + 1];
+    __shared__ double sh[BLK / WAVE];
+    for (int k = threadIdx.x; k < nv; k += BLK) hs[k] = h[k];
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_blk;
+    const int64_t r1 = min(n, r0 + rows_per_blk);
+    double nrm = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLK) {
+        double s = (double)w[i];
+        for (int k = 0; k < nv; ++k) s -= (double)V[(int64_t)k * ldv + i] * hs[k];
+        T st = (T)s;
+        w[i] = st;
+        nrm += (double)st * (double)st;
+    }
+    if (part_nrm) {
+        double t = block_sum(nrm, sh);
+        if (threadIdx.x == 0) part_nrm[blockIdx.x] = t;
+    }
+}
+
+// dst = src * S[misc + idx]
+template <class TS, class TD>
+__global__ void __launch_bounds__(BLK) k_scale(int64_t n, const TS* __restrict__ src, TD* __restrict__ dst,
+                                               const double* __restrict__ S, int64_t idx, const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0] == 0) return;
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = (TD)((double)src[i] * S[idx]);
+}
+
+// start of a cycle: beta = ||r|| from residual partials; g = [beta, 0, ...]; inv = 1/beta
+__global__ void __launch_bounds__(BLK) k_cycle_start(int nblk, const double* __restrict__ part, double* __restrict__ S,
+                                                     StateLayout L, int* __restrict__ ctrl, double target_rel,
+                                                     int use_given_beta) {
+    __shared__ double sh[BLK / WAVE];
+    double s = 0.0, sb = 0.0;
+    if (!use_given_beta) {
+        for (int i = threadIdx.x; i < nblk; i += BLK) { s += part[i]; sb += part[nblk + i]; }
+    }
+    s = block_sum(s, sh);
+    sb = block_sum(sb, sh);
+    if (threadIdx.x == 0) {
+        double beta = use_given_beta ? S[L.misc() + 0] : sqrt(s);
+        if (!use_given_beta) S[L.misc() + 5] = sqrt(sb);  // ||b||
+        S[L.misc() + 0] = beta;
+        S[L.misc() + 1] = beta > 0.0 ? 1.0 / beta : 0.0;
+        S[L.misc() + 2] = target_rel;
+        S[L.misc() + 3] = beta;
+        S[L.misc() + 4] = 1.0;
+        for (int i = 0; i <= L.m; ++i) S[L.g() + i] = 0.0;
+        S[L.g()] = beta;
+        ctrl[0] = (beta > 0.0 && isfinite(beta)) ? 1 : 0;
+        ctrl[1] = 0;
+        ctrl[2] = isfinite(beta) ? 0 : 1;
+    }
+}
+
+// Arnoldi step j: h = h1 + h2, h_{j+1,j} = ||w||, Givens update, convergence test
+__global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int nblk, const double* __restrict__ part_nrm,
+                                                      double* __restrict__ S, StateLayout L, int* __restrict__ ctrl) {
+    if (ctrl[0] == 0) return;
+    __shared__ double sh[BLK / WAVE];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += BLK) s += part_nrm[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x != 0) return;
+    const int m = L.m;
+    double* H = S + L.H();
+    double* cs = S + L.cs();
+    double* sn = S + L.sn();
+    double* g = S + L.g();
+    const double* h1 = S + L.h1();
+    const double* h2 = S + L.h2();
+    const int64_t cj = (int64_t)j * (m + 1);
+    for (int i = 0; i <= j; ++i) H[cj + i] = h1[i] + h2[i];
+    const double hn = sqrt(s);
+    H[cj + j + 1] = hn;
+    for (int i = 0; i < j; ++i) {
+        double a = H[cj + i], b = H[cj + i + 1];
+        H[cj + i] = cs[i] * a + sn[i] * b;
+        H[cj + i + 1] = -sn[i] * a + cs[i] * b;
+    }
+    double a = H[cj + j], b = H[cj + j + 1];
+    double r = hypot(a, b);
+    double c = r > 0.0 ? a / r : 1.0;
+    double sgn = r > 0.0 ? b / r : 0.0;
+    cs[j] = c;
+    sn[j] = sgn;
+    H[cj + j] = r;
+    H[cj + j + 1] = 0.0;
+    g[j + 1] = -sgn * g[j];
+    g[j] = c * g[j];
+    const double res = fabs(g[j + 1]);
+    S[L.misc() + 3] = res;
+    S[L.misc() + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
+    ctrl[1] = j + 1;
+    const bool finite = isfinite(res) && isfinite(hn) && isfinite(r);
+    if (!finite) ctrl[2] = 1;
+    if (!finite || res <= S[L.misc() + 2] || hn == 0.0 || r == 0.0) ctrl[0] = 0;
+}
+
+// y = H(1:k,1:k) \ g(1:k), k = steps done
+__global__ void k_backsolve(double* __restrict__ S, StateLayout L, const int* __restrict__ ctrl) {
+    if (threadIdx.x != 0) return;
+    const int k = ctrl[1];
+    const int m = L.m;
+    const double* H = S + L.H();
+    const double* g = S + L.g();
+    double* y = S + L.y();
+    for (int i = k - 1; i >= 0; --i) {
+        double s = g[i];
+        for (int l = i + 1; l < k; ++l) s -= H[(int64_t)l * (m + 1) + i] * y[l];
+        double d = H[(int64_t)i * (m + 1) + i];
+        y[i] = d != 0.0 ? s / d : 0.0;
+    }
+}
+
+// z = sum_{l<k} V[l] y[l] (z in FP64, overwritten)
+template <class T>
+__global__ void __launch_bounds__(BLK) k_combine(int64_t n, const T* __restrict__ V, int64_t ldv,
+                                                 const double* __restrict__ S, StateLayout L, const int* __restrict__ ctrl,
+                                                 double* __restrict__ z) {
+    __shared__ double ys[MAX_RESTART];
+    const int k = ctrl[1];
+    for (int l = threadIdx.x; l < k; l += BLK) ys[l] = S[L.y() + l];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int l = 0; l < k; ++l) s += (double)V[(int64_t)l * ldv + i] * ys[l];
+    z[i] = s;
+}
+
+// x += scale * dinv .* z
+__global__ void __launch_bounds__(BLK) k_xupdate(int64_t n, const double* __restrict__ z, const double* __restrict__ dinv,
+                                                 double scale, double* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    x[i] += scale * (dinv ? dinv[i] * z[i] : z[i]);
+}
+
+// Jacobi: dinv = 1 / diag(A) (1 where the diagonal is absent or zero)
+__global__ void __launch_bounds__(BLK) k_dinv(int64_t n, const int32_t* __restrict__ dslot, const double* __restrict__ val,
+                                              double* __restrict__ dinv, int jacobi) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    double d = (jacobi && dslot[i] >= 0) ? val[dslot[i]] : 1.0;
+    dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// right-scaled matrix A D^-1 in the working precision
+template <class T>
+__global__ void __launch_bounds__(BLK) k_scale_cols(int64_t nnz, const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                    const double* __restrict__ dinv, T* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (k >= nnz) return;
+    out[k] = (T)(val[k] * dinv[col[k]]);
+}
+
+// ---------------------------------------------------------------- ILU(0) (Saad Alg. 10.4), level-scheduled
+// factor in place of a copy of the (scaled) matrix values; rows of one level are independent
+template <class T>
+__global__ void __launch_bounds__(BLK) k_ilu0_factor_level(const int32_t* __restrict__ rows, int32_t nrows,
+                                                           const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                           const int32_t* __restrict__ diag, T* __restrict__ lu) {
+    const int32_t t = blockIdx.x * BLK + threadIdx.x;
+    if (t >= nrows) return;
+    const int32_t i = rows[t];
+    const int32_t s = rp[i], e = rp[i + 1], di = diag[i];
+    for (int32_t kk = s; kk < di; ++kk) {
+        const int32_t k = col[kk];
+        const T piv = lu[diag[k]];
+        const T lik = lu[kk] / piv;
+        lu[kk] = lik;
+        // a_ij -= l_ik u_kj for j in row k right of its diagonal, when (i, j) is in the pattern
+        int32_t p = kk + 1;
+        for (int32_t q = diag[k] + 1; q < rp[k + 1]; ++q) {
+            const int32_t j = col[q];
+            while (p < e && col[p] < j) ++p;
+            if (p < e && col[p] == j) lu[p] -= lik * lu[q];
+        }
+    }
+}
+
+// forward (L, unit diagonal) then backward (U) substitution, one level per launch
+template <class T>
+__global__ void __launch_bounds__(BLK) k_ilu0_lower(const int32_t* __restrict__ rows, int32_t nrows,
+                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                    const int32_t* __restrict__ diag, const T* __restrict__ lu,
+                                                    const T* __restrict__ rhs, T* __restrict__ z, const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0] == 0) return;
+    const int32_t t = blockIdx.x * BLK + threadIdx.x;
+    if (t >= nrows) return;
+    const int32_t i = rows[t];
+    T s = rhs[i];
+    for (int32_t k = rp[i]; k < diag[i]; ++k) s -= lu[k] * z[col[k]];
+    z[i] = s;
+}
+template <class T>
+__global__ void __launch_bounds__(BLK) k_ilu0_upper(const int32_t* __restrict__ rows, int32_t nrows,
+                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                    const int32_t* __restrict__ diag, const T* __restrict__ lu,
+                                                    T* __restrict__ z, const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0] == 0) return;
+    const int32_t t = blockIdx.x * BLK + threadIdx.x;
+    if (t >= nrows) return;
+    const int32_t i = rows[t];
+    T s = z[i];
+    for (int32_t k = diag[i] + 1; k < rp[i + 1]; ++k) s -= lu[k] * z[col[k]];
+    z[i] = s / lu[diag[i]];
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
+
+struct HipEvents {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    hipError_t create() {
+        for (auto& x : e) { hipError_t r = hipEventCreate(&x); if (r != hipSuccess) return r; }
+        return hipSuccess;
+    }
+    ~HipEvents() { for (auto& x : e) if (x) (void)hipEventDestroy(x); }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- timing helpers
+void ktime_begin(Ctx& c, int kclass, hipEvent_t* a) {
+    (void)kclass;
+    *a = nullptr;
+    if (!c.timing) return;
+    if (c.ev_next + 2 > c.ev_pool.size()) {
+        for (int k = 0; k < 256; ++k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            c.ev_pool.push_back(e);
+        }
+    }
+    *a = c.ev_pool[c.ev_next++];
+    (void)hipEventRecord(*a, c.stream);
+}
+void ktime_end(Ctx& c, int kclass, double bytes, hipEvent_t a) {
+    if (!c.timing || !a) return;
+    hipEvent_t b = c.ev_pool[c.ev_next++];
+    (void)hipEventRecord(b, c.stream);
+    c.pending.push_back({kclass, bytes, a, b, c.cur_step});
+}
+int ktime_flush(Ctx& c) {
+    if (c.pending.empty()) { c.ev_next = 0; return VFVM_OK; }
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    for (auto& p : c.pending) {
+        float ms = 0.0f;
+        VFVM_HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
+        if (p.bytes >= 0.0) {
+            c.kt[p.kclass].launches += 1;
+            c.kt[p.kclass].ms += ms;
+            c.kt[p.kclass].bytes += p.bytes;
+        }
+    }
+    c.pending.clear();
+    c.ev_next = 0;
+    return VFVM_OK;
+}
+
+// ---------------------------------------------------------------- solver
+namespace {
+
+struct Work {
+    int64_t n, nnz;
+    int m;
+    int nblk;           // blocks of the reductions
+    int64_t rows_per_blk;
+    StateLayout L;
+    int* ctrl;          // device control word
+    int* h_ctrl;        // pinned
+    double* S;          // device state
+};
+
+// LPR choice: average row length
+inline int pick_lpr(int64_t n, int64_t nnz) {
+    double avg = n > 0 ? (double)nnz / (double)n : 1.0;
+    return avg > 24.0 ? 16 : (avg > 10.0 ? 8 : 4);
+}
+
+template <class T>
+int spmv_launch(Ctx& c, int lpr, int64_t n, const T* val, const T* x, T* y, const int* ctrl) {
+    unsigned grid = grid_for(n * lpr);
+    switch (lpr) {
+        case 4: hipLaunchKernelGGL((k_spmv<T, 4>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, val, x, y, ctrl); break;
+        case 8: hipLaunchKernelGGL((k_spmv<T, 8>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, val, x, y, ctrl); break;
+        default: hipLaunchKernelGGL((k_spmv<T, 16>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, val, x, y, ctrl); break;
+    }
+    VFVM_HIP_TRY(hipGetLastError());
+    return VFVM_OK;
+}
+
+int residual_launch(Ctx& c, int lpr, const double* x, double* r, double* part, int nblk_r) {
+    const int64_t n = c.n;
+    unsigned grid = (unsigned)nblk_r;
+    switch (lpr) {
+        case 4: hipLaunchKernelGGL((k_residual<4>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, c.vals.p, x, c.b.p, r, part, nblk_r); break;
+        case 8: hipLaunchKernelGGL((k_residual<8>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, c.vals.p, x, c.b.p, r, part, nblk_r); break;
+        default: hipLaunchKernelGGL((k_residual<16>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, c.vals.p, x, c.b.p, r, part, nblk_r); break;
+    }
+    VFVM_HIP_TRY(hipGetLastError());
+    return VFVM_OK;
+}
+
+// ILU(0) level sets of the lower-triangular dependency graph (row i depends on col < i)
+int ilu_levels(Ctx& c) {
+    if (c.ilu_levels_ready) return VFVM_OK;
+    const int64_t n = c.n;
+    std::vector<int32_t> lvl(n, 0), diag(n, -1);
+    int32_t maxl = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t l = 0;
+        for (int32_t k = c.h_mrow[i]; k < c.h_mrow[i + 1]; ++k) {
+            int32_t j = c.h_mcol[k];
+            if (j < i) l = std::max(l, lvl[j] + 1);
+            if (j == i) diag[i] = k;
+        }
+        if (diag[i] < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "ILU(0): a row has no diagonal entry");
+        lvl[i] = l;
+        maxl = std::max(maxl, l);
+    }
+    // upper sweep runs levels in reverse of the transposed dependency; level of the upper
+    // graph (row i depends on col > i) computed separately
+    std::vector<int32_t> ulvl(n, 0);
+    int32_t maxu = 0;
+    for (int64_t i = n - 1; i >= 0; --i) {
+        int32_t l = 0;
+        for (int32_t k = diag[i] + 1; k < c.h_mrow[i + 1]; ++k) l = std::max(l, ulvl[c.h_mcol[k]] + 1);
+        ulvl[i] = l;
+        maxu = std::max(maxu, l);
+    }
+    // rows grouped: [lower levels 0..maxl][upper levels 0..maxu]
+    std::vector<int32_t> ptr(maxl + 2 + maxu + 1, 0), rows(2 * n);
+    for (int64_t i = 0; i < n; ++i) { ptr[lvl[i] + 1]++; ptr[maxl + 1 + ulvl[i] + 1]++; }
+    for (size_t k = 1; k < ptr.size(); ++k) ptr[k] += ptr[k - 1];
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t i = 0; i < n; ++i) { rows[fill[lvl[i]]++] = (int32_t)i; rows[fill[maxl + 1 + ulvl[i]]++] = (int32_t)i; }
+    c.h_lvl_ptr = ptr;
+    c.h_lvl_ptr.push_back(maxl + 1);  // number of lower levels, appended
+    VFVM_HIP_TRY(c.ilu_lvl_rows.upload(rows.data(), rows.size(), c.stream));
+    VFVM_HIP_TRY(c.ilu_diag.upload(diag.data(), diag.size(), c.stream));
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    c.ilu_levels_ready = true;
+    return VFVM_OK;
+}
+
+template <class T>
+int ilu_factor(Ctx& c, T* lu) {
+    const int nlow = c.h_lvl_ptr.back();
+    for (int l = 0; l < nlow; ++l) {
+        int32_t s = c.h_lvl_ptr[l], e = c.h_lvl_ptr[l + 1];
+        if (e > s) hipLaunchKernelGGL((k_ilu0_factor_level<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream,
+                                      c.ilu_lvl_rows.p + s, e - s, c.mrow.p, c.mcol.p, c.ilu_diag.p, lu);
+    }
+    VFVM_HIP_TRY(hipGetLastError());
+    return VFVM_OK;
+}
+
+template <class T>
+int ilu_apply(Ctx& c, const T* lu, const T* rhs, T* z, const int* ctrl) {
+    const int nlow = c.h_lvl_ptr.back();
+    const int ntot = (int)c.h_lvl_ptr.size() - 2;  // number of level groups
+    hipEvent_t ev = nullptr;
+    ktime_begin(c, VFVM_K_PRECOND, &ev);
+    for (int l = 0; l < nlow; ++l) {
+        int32_t s = c.h_lvl_ptr[l], e = c.h_lvl_ptr[l + 1];
+        hipLaunchKernelGGL((k_ilu0_lower<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, c.ilu_lvl_rows.p + s, e - s,
+                           c.mrow.p, c.mcol.p, c.ilu_diag.p, lu, rhs, z, ctrl);
+    }
+    for (int l = nlow; l < ntot; ++l) {
+        int32_t s = c.h_lvl_ptr[l], e = c.h_lvl_ptr[l + 1];
+        hipLaunchKernelGGL((k_ilu0_upper<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, c.ilu_lvl_rows.p + s, e - s,
+                           c.mrow.p, c.mcol.p, c.ilu_diag.p, lu, z, ctrl);
+    }
+    VFVM_HIP_TRY(hipGetLastError());
+    ktime_end(c, VFVM_K_PRECOND, (double)c.nnz * (sizeof(T) + 4) + 4.0 * (c.n + 1) + 3.0 * sizeof(T) * c.n, ev);
+    return VFVM_OK;
+}
+
+// One restart cycle of GMRES on the operator A M^-1 (M = D for Jacobi through the scaled
+// matrix, M = LU for ILU(0)) starting from V[0] (already normalised).  Leaves ctrl[1] = k
+// steps and y = H^-1 g in the state.
+template <class T>
+int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, const T* aval, const T* lu, int check_every, int* steps_out) {
+    const int64_t n = w.n;
+    const int lpr = pick_lpr(n, w.nnz);
+    int steps = 0;
+    bool synced_last = false;
+    for (int j = 0; j < w.m; ++j) {
+        c.cur_step = j;
+        T* vj = V + (int64_t)j * n;
+        // w = A M^-1 v_j
+        const T* xin = vj;
+        if (lu) {
+            if (int st = ilu_apply<T>(c, lu, vj, zv, w.ctrl)) return st;
+            xin = zv;
+        }
+        hipEvent_t ev = nullptr;
+        ktime_begin(c, VFVM_K_SPMV, &ev);
+        if (int st = spmv_launch<T>(c, lpr, n, aval, xin, wv, w.ctrl)) return st;
+        ktime_end(c, VFVM_K_SPMV, (double)w.nnz * (sizeof(T) + 4) + 4.0 * (n + 1) + 2.0 * sizeof(T) * n, ev);
+        // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2, ||w||
+        const int nv = j + 1;
+        double* part = c.part.p;
+        double* part_nrm = part + (int64_t)(MAX_RESTART + 1) * w.nblk;
+        ktime_begin(c, VFVM_K_ORTHO, &ev);
+        hipLaunchKernelGGL((k_mdot<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, wv, part, w.nblk, w.rows_per_blk, w.ctrl);
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, w.nblk, part, w.S + w.L.h1(), 0, w.ctrl);
+        hipLaunchKernelGGL((k_update<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, w.S + w.L.h1(), wv, (double*)nullptr, w.rows_per_blk, w.ctrl);
+        hipLaunchKernelGGL((k_mdot<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, wv, part, w.nblk, w.rows_per_blk, w.ctrl);
+        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, w.nblk, part, w.S + w.L.h2(), 0, w.ctrl);
+        hipLaunchKernelGGL((k_update<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, w.S + w.L.h2(), wv, part_nrm, w.rows_per_blk, w.ctrl);
+        hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(BLK), 0, c.stream, j, w.nblk, part_nrm, w.S, w.L, w.ctrl);
+        hipLaunchKernelGGL((k_scale<T, T>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, wv, V + (int64_t)(j + 1) * n, w.S, w.L.misc() + 1, w.ctrl);
+        VFVM_HIP_TRY(hipGetLastError());
+        // algorithmic bytes of CGS2 at step j: two multi-dots reading (j+2) vectors, two updates
+        // reading (j+2) and writing one, the scaling reading and writing one
+        ktime_end(c, VFVM_K_ORTHO, (double)sizeof(T) * n * (2.0 * (nv + 1) + 2.0 * (nv + 2) + 2.0), ev);
+        c.cur_step = -1;
+        synced_last = false;
+        if ((j + 1) % check_every == 0 || j + 1 == w.m) {
+            VFVM_HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, CTRL_INTS * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+            VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+            synced_last = true;
+            steps = w.h_ctrl[1];
+            if (c.timing) {
+                for (auto& p : c.pending)
+                    if (p.step >= steps) p.bytes = -1.0;  // launches after convergence did no work
+                if (int st = ktime_flush(c)) return st;
+            }
+            if (w.h_ctrl[0] == 0) break;
+        }
+    }
+    if (!synced_last) {
+        VFVM_HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, CTRL_INTS * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+        steps = w.h_ctrl[1];
+    }
+    hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(WAVE), 0, c.stream, w.S, w.L, w.ctrl);
+    VFVM_HIP_TRY(hipGetLastError());
+    *steps_out = steps;
+    return VFVM_OK;
+}
+
+}  // namespace
+
+int residual_norm(Ctx& c, const double* x_dev, double* rel, double* bnorm) {
+    const int64_t n = c.n;
+    const int lpr = pick_lpr(n, c.nnz);
+    const int nblk_r = (int)grid_for(n * lpr);
+    VFVM_HIP_TRY(c.r64.alloc(n));
+    DevBuf<double> part;
+    VFVM_HIP_TRY(part.alloc(2 * (size_t)nblk_r));
+    if (int st = residual_launch(c, lpr, x_dev, c.r64.p, part.p, nblk_r)) return st;
+    std::vector<double> h(2 * (size_t)nblk_r);
+    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    double rr = 0.0, bb = 0.0;
+    for (int k = 0; k < nblk_r; ++k) { rr += h[k]; bb += h[nblk_r + k]; }
+    *bnorm = std::sqrt(bb);
+    *rel = bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : std::sqrt(rr);
+    return VFVM_OK;
+}
+
+int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& st) {
+    std::memset(&st, 0, sizeof(st));
+    if (!(o.tol > 0.0)) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_TOLERANCE, "GMRES: tolerance must be positive");
+    if (o.restart <= 0 || o.restart > MAX_RESTART) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_RESTART, "GMRES: restart must be in [1, 256]");
+    if (o.max_outer <= 0) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_MAX_OUTER, "GMRES: max_outer must be positive");
+    if (o.precond < VFVM_PRECOND_NONE || o.precond > VFVM_PRECOND_ILU0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown preconditioner");
+    if (o.precision != VFVM_FP64 && o.precision != VFVM_FP32_REFINED) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown precision");
+    const bool mixed = o.precision == VFVM_FP32_REFINED;
+    const int64_t n = c.n;
+    const int m = o.restart;
+    Work w;
+    w.n = n; w.nnz = c.nnz; w.m = m;
+    w.nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 4 * BLK - 1) / (4 * BLK)));
+    w.rows_per_blk = (n + w.nblk - 1) / w.nblk;
+    w.L.m = m;
+    const int lpr = pick_lpr(n, c.nnz);
+    const int nblk_r = (int)grid_for(n * lpr);
+
+    // workspace
+    VFVM_HIP_TRY(c.scal.alloc((size_t)w.L.total() + 64));
+    w.S = c.scal.p;
+    w.ctrl = reinterpret_cast<int*>(c.scal.p + w.L.total());
+    if (!c.h_pinned) VFVM_HIP_TRY(hipHostMalloc((void**)&c.h_pinned, 4096, hipHostMallocDefault));
+    w.h_ctrl = reinterpret_cast<int*>(c.h_pinned);
+    VFVM_HIP_TRY(c.part.alloc((size_t)(MAX_RESTART + 2) * w.nblk + 2 * (size_t)nblk_r + 16));
+    VFVM_HIP_TRY(c.r64.alloc(n));
+    VFVM_HIP_TRY(c.dinv64.alloc(n));
+    VFVM_HIP_TRY(c.w64.alloc(n));
+    VFVM_HIP_TRY(c.z64.alloc(n));
+    const bool ilu = o.precond == VFVM_PRECOND_ILU0;
+    const bool jac = o.precond == VFVM_PRECOND_JACOBI;
+    if (ilu) {
+        if (int s = ilu_levels(c)) return s;
+    }
+    if (!mixed) VFVM_HIP_TRY(c.V64.alloc((size_t)n * (m + 1)));
+    else { VFVM_HIP_TRY(c.V32.alloc((size_t)n * (m + 1))); VFVM_HIP_TRY(c.w32.alloc(2 * (size_t)n)); }
+
+    HipEvents ev;
+    VFVM_HIP_TRY(ev.create());
+    VFVM_HIP_TRY(hipEventRecord(ev.e[0], c.stream));
+
+    // preconditioner set-up (inside the timed solve)
+    hipLaunchKernelGGL(k_dinv, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.dslot.p, c.vals.p, c.dinv64.p, jac ? 1 : 0);
+    const double* dinv_x = jac ? c.dinv64.p : nullptr;
+    const void* aval = nullptr;
+    const void* lu = nullptr;
+    if (!mixed) {
+        if (jac) {
+            VFVM_HIP_TRY(c.avals64.alloc(c.nnz));
+            hipLaunchKernelGGL((k_scale_cols<double>), dim3(grid_for(c.nnz)), dim3(BLK), 0, c.stream, c.nnz, c.mcol.p, c.vals.p, c.dinv64.p, c.avals64.p);
+            aval = c.avals64.p;
+        } else {
+            aval = c.vals.p;
+        }
+        if (ilu) {
+            VFVM_HIP_TRY(c.ilu64.alloc(c.nnz));
+            VFVM_HIP_TRY(hipMemcpyAsync(c.ilu64.p, c.vals.p, c.nnz * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+            if (int s = ilu_factor<double>(c, c.ilu64.p)) return s;
+            lu = c.ilu64.p;
+        }
+    } else {
+        // FP32 copy of A D^-1 (Jacobi) or of A (ILU(0), none); dinv = 1 unless Jacobi
+        VFVM_HIP_TRY(c.avals32.alloc(c.nnz));
+        hipLaunchKernelGGL((k_scale_cols<float>), dim3(grid_for(c.nnz)), dim3(BLK), 0, c.stream, c.nnz, c.mcol.p, c.vals.p, c.dinv64.p, c.avals32.p);
+        aval = c.avals32.p;
+        if (ilu) {
+            VFVM_HIP_TRY(c.ilu32.alloc(c.nnz));
+            VFVM_HIP_TRY(hipMemcpyAsync(c.ilu32.p, c.avals32.p, c.nnz * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+            if (int s = ilu_factor<float>(c, c.ilu32.p)) return s;
+            lu = c.ilu32.p;
+        }
+    }
+    VFVM_HIP_TRY(hipGetLastError());
+    VFVM_HIP_TRY(hipEventRecord(ev.e[1], c.stream));
+
+    const int check_every = 8;
+    double* part_r = c.part.p + (size_t)(MAX_RESTART + 2) * w.nblk;
+    int total_steps = 0, cycles = 0, refinements = 0;
+    double rel = 0.0, bnorm = 0.0;
+    int status = VFVM_ERR_LINEAR_DID_NOT_CONVERGE;
+    const int max_ref = o.max_refine > 0 ? o.max_refine : o.max_outer;
+    for (;;) {
+        // true FP64 residual r = b - A x, beta = ||r||
+        if (int s = residual_launch(c, lpr, x, c.r64.p, part_r, nblk_r)) return s;
+        hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(BLK), 0, c.stream, nblk_r, part_r, w.S, w.L, w.ctrl, 0.0, 0);
+        VFVM_HIP_TRY(hipMemcpyAsync(c.h_pinned + 8, w.S + w.L.misc(), 8 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+        const double beta = c.h_pinned[8 + 0];
+        bnorm = c.h_pinned[8 + 5];
+        rel = bnorm > 0.0 ? beta / bnorm : beta;
+        if (!std::isfinite(beta)) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
+        if (rel <= o.tol || beta == 0.0) { status = VFVM_OK; break; }
+        if (cycles >= o.max_outer || (mixed && refinements >= max_ref)) break;
+        // Arnoldi target: FP64 stops on the estimate of ||b - A x||; the FP32 inner solve
+        // reduces the current residual by inner_tol
+        const double target_abs = mixed ? std::max(o.inner_tol, 1e-7) * beta : 0.999 * o.tol * bnorm;
+        hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(BLK), 0, c.stream, nblk_r, part_r, w.S, w.L, w.ctrl, target_abs, 1);
+        int steps = 0;
+        if (!mixed) {
+            hipLaunchKernelGGL((k_scale<double, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V64.p, w.S, w.L.misc() + 1, w.ctrl);
+            if (int s = arnoldi_cycle<double>(c, w, c.V64.p, c.w64.p, c.z64.p, (const double*)aval, (const double*)lu, check_every, &steps)) return s;
+            if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
+            // x += M^-1 (V y)
+            hipLaunchKernelGGL((k_combine<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V64.p, n, w.S, w.L, w.ctrl, c.w64.p);
+            if (lu) {
+                if (int s = ilu_apply<double>(c, (const double*)lu, c.w64.p, c.z64.p, nullptr)) return s;
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, 1.0, x);
+            } else {
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.w64.p, dinv_x, 1.0, x);
+            }
+        } else {
+            // inner FP32 solve of A M^-1 u = r from u = 0, V0 = r / beta in FP32
+            hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V32.p, w.S, w.L.misc() + 1, w.ctrl);
+            if (int s = arnoldi_cycle<float>(c, w, c.V32.p, c.w32.p, c.w32.p + n, (const float*)aval, (const float*)lu, check_every, &steps)) return s;
+            ++refinements;
+            if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
+            // x += M^-1 (V y), V y accumulated in FP64
+            hipLaunchKernelGGL((k_combine<float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V32.p, n, w.S, w.L, w.ctrl, c.z64.p);
+            if (lu) {
+                float* t32 = c.w32.p;
+                float* z32 = c.w32.p + n;
+                hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, t32, w.S, w.L.misc() + 4, (const int*)nullptr);
+                if (int s = ilu_apply<float>(c, (const float*)lu, t32, z32, nullptr)) return s;
+                hipLaunchKernelGGL((k_scale<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z32, c.z64.p, w.S, w.L.misc() + 4, (const int*)nullptr);
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, 1.0, x);
+            } else {
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, dinv_x, 1.0, x);
+            }
+        }
+        VFVM_HIP_TRY(hipGetLastError());
+        total_steps += steps;
+        ++cycles;
+    }
+    VFVM_HIP_TRY(hipEventRecord(ev.e[2], c.stream));
+    VFVM_HIP_TRY(hipEventSynchronize(ev.e[2]));
+    float ms_all = 0.f, ms_setup = 0.f;
+    VFVM_HIP_TRY(hipEventElapsedTime(&ms_all, ev.e[0], ev.e[2]));
+    VFVM_HIP_TRY(hipEventElapsedTime(&ms_setup, ev.e[0], ev.e[1]));
+    if (c.timing) if (int s = ktime_flush(c)) return s;
+    st.iterations = total_steps;
+    st.restarts = cycles;
+    st.refinements = refinements;
+    st.rel_residual = rel;
+    st.seconds = ms_all * 1e-3;
+    st.setup_seconds = ms_setup * 1e-3;
+    st.status = status;
+    if (status != VFVM_OK) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "GMRES did not converge: relative residual %.3e after %d steps", rel, total_steps);
+        return set_error(status, buf);
+    }
+    return VFVM_OK;
+}
+
+}  // namespace vfvm

@@ -1,0 +1,59 @@
+// Host-side mesh: jittered-lattice vertex set, exact Delaunay tetrahedralisation
+// and the Voronoi dual arrays that the device kernels read.
+//
+// Vertex numbering follows the reference: points (interior vertices) first, then
+// nodes (boundary vertices), vid = pid for points and vid = num_pts + nid for nodes
+// (src/readme.txt:58-59, src/base/geom_tet.rs:11-12).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace vfvm {
+
+struct Mesh {
+    // vertices
+    int64_t num_pts = 0;            // interior points
+    int64_t num_nds = 0;            // boundary nodes
+    std::vector<double> pos;        // 3 * num_tot, AoS (x, y, z)
+    int64_t num_tot() const { return num_pts + num_nds; }
+
+    // boundaries: node ids (nid, local to the node block) per boundary, ascending
+    std::vector<std::string> bnd_name;
+    std::vector<int64_t> bnd_ptr;   // num_bnd + 1
+    std::vector<int32_t> bnd_nid;   // concatenated nids
+    std::vector<int32_t> nd_bnd;    // nid -> boundary id (each node on exactly one boundary)
+
+    // tetrahedra, reference orientation (p3 below the counterclockwise face p0 p1 p2)
+    std::vector<int32_t> tets;      // 4 * num_tets
+
+    // ---- Voronoi dual (filled by build_dual; restates src/base/geom_tet.rs:58-350) ----
+    bool has_dual = false;
+    std::vector<int32_t> row_ptr;   // num_tot + 1; neighbour CSR (directed edges)
+    std::vector<int32_t> ngh;       // E; ascending per row (geom_tet.rs:174-196)
+    std::vector<double> dist;       // E; edge length
+    std::vector<double> area;       // E; signed Voronoi face area
+    std::vector<double> norm;       // 3E SoA: nx[E], ny[E], nz[E]; outward unit normal
+    std::vector<double> skew;       // 3E SoA; face centre minus edge midpoint
+    std::vector<double> gw;         // 3E SoA; least-squares gradient weights
+    std::vector<double> vol;        // N; control-volume volume
+    std::vector<double> cskew;      // 3N SoA; control-volume centroid minus vertex
+    // boundary sub-faces (geom_tet.rs:208-259)
+    std::vector<int32_t> bsf_ptr;   // N + 1
+    std::vector<int32_t> bsf_vj, bsf_vk;  // other two vids of the surface triangle
+    std::vector<double> bsf_area;   // signed sub-face area
+    std::vector<double> bsf_norm;   // 3 * S SoA
+    std::vector<double> bsf_wgt;    // 3 * S SoA (w_v, w_j, w_k)
+    std::vector<double> bnd_sum;    // N; boundary CV area
+
+    int64_t num_edges() const { return (int64_t)ngh.size(); }
+    int64_t num_subfaces() const { return (int64_t)bsf_vj.size(); }
+};
+
+// Status codes shared with the C ABI (mirrors of VFVChemError variants on this path).
+int lattice_mesh(int n, uint64_t seed, double jitter, Mesh& m, std::string& err);
+int delaunay_int(const std::vector<int64_t>& ixyz, const std::vector<int32_t>& order,
+                 std::vector<int32_t>& tets_out, std::string& err);
+int build_dual(Mesh& m, int nthreads, std::string& err);
+
+}  // namespace vfvm

@@ -1,0 +1,143 @@
+"""ctypes binding of the C ABI declared in include/vfvm.h.
+
+The shared library lib/libvfvm.so next to this package is required: there is
+no fallback implementation, and loading fails loudly when it is absent.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libvfvm.so")
+
+# status codes (include/vfvm.h, vfvm_status)
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_HIP = 2
+ERR_TETRA_DEGENERATE_INPUT = 10
+ERR_TETRA_POINTS_ON_BOUNDARY = 11
+ERR_TETRA_SINGULAR_GRADIENT = 12
+ERR_OPERATOR_NON_UNKNOWN = 20
+ERR_OPERATOR_USE_APPLY_TIME = 21
+ERR_OPERATOR_UNSUPPORTED = 22
+ERR_PATTERN_MISMATCH = 23
+ERR_LINEAR_NON_POSITIVE_TOLERANCE = 30
+ERR_LINEAR_NON_POSITIVE_RESTART = 31
+ERR_LINEAR_NON_POSITIVE_MAX_OUTER = 32
+ERR_LINEAR_DID_NOT_CONVERGE = 33
+ERR_LINEAR_NON_FINITE = 34
+ERR_SOLVER_NON_POSITIVE_MAX_ITER = 40
+ERR_SOLVER_NON_POSITIVE_TOLERANCE = 41
+ERR_SOLVER_INVALID_DAMPING = 42
+ERR_SOLVER_NON_POSITIVE_TIME_STEP = 43
+ERR_SOLVER_NON_POSITIVE_NUM_STEPS = 44
+ERR_STEADY_DID_NOT_CONVERGE = 45
+ERR_STEADY_NON_FINITE_RESIDUAL = 46
+ERR_TRANSIENT_DID_NOT_CONVERGE = 47
+ERR_TRANSIENT_NON_FINITE_RESIDUAL = 48
+
+STATUS_NAMES = {v: k for k, v in globals().items() if k.startswith("ERR_") or k == "OK"}
+
+
+class VFVChemError(RuntimeError):
+    """Error raised for a non-zero status; mirrors VFVChemError (src/base/error.rs:5)."""
+
+    def __init__(self, code: int, message: str):
+        self.code = code
+        self.kind = STATUS_NAMES.get(code, str(code))
+        super().__init__(f"{self.kind}: {message}")
+
+
+class MeshInfo(C.Structure):
+    _fields_ = [("num_pts", C.c_int64), ("num_nds", C.c_int64), ("num_tets", C.c_int64),
+                ("num_edges", C.c_int64), ("num_subfaces", C.c_int64), ("num_bnd", C.c_int32),
+                ("max_degree", C.c_int32)]
+
+
+class OpStruct(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("geom_id", C.c_int32), ("unk_id", C.c_int32), ("a", C.c_int32), ("b", C.c_int32)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("restart", C.c_int32), ("max_outer", C.c_int32), ("precision", C.c_int32),
+                ("precond", C.c_int32), ("inner_tol", C.c_double), ("max_refine", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("restarts", C.c_int32), ("refinements", C.c_int32), ("status", C.c_int32),
+                ("rel_residual", C.c_double), ("seconds", C.c_double), ("setup_seconds", C.c_double)]
+
+
+class DriverStats(C.Structure):
+    _fields_ = [("picard_iterations", C.c_int32), ("linear_iterations", C.c_int32), ("residual", C.c_double),
+                ("time_assembly", C.c_double), ("time_solve", C.c_double)]
+
+
+_P = C.c_void_p
+_I = C.c_int32
+_L = C.c_int64
+_D = C.c_double
+_PD = C.POINTER(C.c_double)
+_PI = C.POINTER(C.c_int32)
+_PL = C.POINTER(C.c_int64)
+
+# (name, restype, argtypes) for every entry point of include/vfvm.h
+SIGNATURES = [
+    ("vfvm_last_error", C.c_char_p, []),
+    ("vfvm_version", C.c_char_p, []),
+    ("vfvm_mesh_lattice", _I, [_I, C.c_uint64, _D, C.POINTER(_P)]),
+    ("vfvm_mesh_from_tets", _I, [_L, _L, _PD, _L, _PI, _I, _PL, _PI, C.POINTER(_P)]),
+    ("vfvm_mesh_info_get", _I, [_P, C.POINTER(MeshInfo)]),
+    ("vfvm_mesh_get", _I, [_P, _I, _P]),
+    ("vfvm_mesh_destroy", None, [_P]),
+    ("vfvm_create", _I, [_I, C.POINTER(_P)]),
+    ("vfvm_destroy", None, [_P]),
+    ("vfvm_mesh_upload", _I, [_P, _P]),
+    ("vfvm_scldom_add", _I, [_P, _I, _D, _PD, _PI]),
+    ("vfvm_scldom_set", _I, [_P, _I, _PD]),
+    ("vfvm_scldom_get", _I, [_P, _I, _PD]),
+    ("vfvm_scldom_update_prev", _I, [_P, _I]),
+    ("vfvm_vecdom_add", _I, [_P, _I, _PD, _PD, _PI]),
+    ("vfvm_vecdom_set", _I, [_P, _I, _PD]),
+    ("vfvm_sclbnd_add", _I, [_P, _I, _D, _PD, _PI]),
+    ("vfvm_sclbnd_set", _I, [_P, _I, _PD]),
+    ("vfvm_sclflw_add", _I, [_P, _I, _I, _I, _I, _I, _PI]),
+    ("vfvm_pattern_build", _I, [_P, C.POINTER(OpStruct), _I, _PL]),
+    ("vfvm_pattern_get", _I, [_P, _PI, _PI]),
+    ("vfvm_assemble", _I, [_P, C.POINTER(OpStruct), _I, _D]),
+    ("vfvm_matrix_get", _I, [_P, _PD, _PD]),
+    ("vfvm_solve_opts_default", None, [C.POINTER(SolveOpts)]),
+    ("vfvm_solve", _I, [_P, C.POINTER(SolveOpts), _PD, C.POINTER(SolveStats)]),
+    ("vfvm_solve_device", _I, [_P, C.POINTER(SolveOpts), _P, C.POINTER(SolveStats)]),
+    ("vfvm_residual", _I, [_P, _PD, _PD]),
+    ("vfvm_steady_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _I, _D, _D, C.POINTER(DriverStats)]),
+    ("vfvm_transient_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, _D, _D, C.POINTER(DriverStats)]),
+    ("vfvm_timing_enable", _I, [_P, _I]),
+    ("vfvm_timing_get", _I, [_P, _I, _PL, _PD, _PD]),
+    ("vfvm_timing_reset", _I, [_P]),
+]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"vfvm: native library not built: {LIB_PATH} (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(code: int) -> None:
+    if code != OK:
+        msg = lib.vfvm_last_error()
+        raise VFVChemError(code, msg.decode() if msg else "")
+
+
+def ptr(a, ctype=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,327 @@
+"""Host-side mirror of the reference's user API for the scalar transport path.
+
+Names, argument order and error behaviour follow the reference:
+  Variables            src/base/vars.rs:38-213
+  SteadyHeat           src/steady/core/steady_heat.rs:9-183
+  TransientHeat        src/transient/core/transient_heat.rs
+  SteadyMass           src/steady/core/steady_mass.rs (one component)
+  SteadyMassFlow       src/steady/core/steady_massflow.rs, mass part with a
+                       prescribed velocity (add_flow_prescribed instead of an
+                       unknown velocity)
+  LinearGmres          src/linear/linear_gmres.rs:16-50
+The operator lists these classes build are applied in the order of each
+reference solver's assemble_matrix; the Picard loops run in the library
+(vfvm_steady_solve / vfvm_transient_solve, mirroring steady_base.rs:16-135 and
+transient_base.rs:16-160).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib as L
+from .core import (FIELD_CONSTANT, FIELD_FUNCTION, FIELD_UNKNOWN, FP32_REFINED, FP64, PRECOND_ILU0, PRECOND_JACOBI,
+                   PRECOND_NONE, OP_SCL_BND_ADVECTION_UNITY, OP_SCL_BND_DIFFUSION, OP_SCL_BND_DIRICHLET,
+                   OP_SCL_BND_NEUMANN, OP_SCL_BND_PRODUCTION, OP_SCL_BND_SOURCE, OP_SCL_BND_TIME,
+                   OP_SCL_DOM_ADVECTION_UNITY, OP_SCL_DOM_DIFFUSION, OP_SCL_DOM_PRODUCTION, OP_SCL_DOM_SOURCE,
+                   OP_SCL_DOM_TIME, Context, Mesh)
+from ._lib import VFVChemError
+
+BOUNDARY_NAMES = ("x0", "x1", "y0", "y1", "z0", "z1")
+
+
+class Variables:
+    """Mesh, fields and their device copies (Variables, src/base/vars.rs:7-34)."""
+
+    def __init__(self, mesh: Mesh, device: int = 0):
+        self.mesh = mesh
+        self.ctx = Context(device)
+        self.ctx.upload(mesh)
+        self.pos = mesh.get("pos")
+        self.dom = []
+        self.bnd = []           # mesh boundary index per bnd_id
+        self.scldom = []        # (kind, func, unkn_ids)
+        self.sclbnd = []        # (bnd_id, func, unkn_ids)
+        self.vecdom = []
+        self.sclflw = []
+
+    @staticmethod
+    def new_lattice(n: int, seed: int = 0, jitter: float = 0.3, device: int = 0) -> "Variables":
+        """Jittered unit-cube lattice (replaces Variables::new_bounds, vars.rs:45, for the lattice meshes)."""
+        return Variables(Mesh.lattice(n, seed, jitter), device)
+
+    def add_dom(self, grp3d_id: str = "cube") -> int:
+        if self.dom:
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, "one domain per mesh on the built path")
+        self.dom.append(grp3d_id)
+        return 0
+
+    def add_bnd(self, dom_id: int, grp2d_id: str) -> int:
+        if grp2d_id not in BOUNDARY_NAMES or dom_id != 0:
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, f"BoundaryGroupIDNotFound: {grp2d_id}")
+        self.bnd.append(BOUNDARY_NAMES.index(grp2d_id))
+        return len(self.bnd) - 1
+
+    def _bnd_pos(self, bnd_id):
+        nids = self.mesh.bnd_nodes(self.bnd[bnd_id])
+        return self.pos[self.mesh.num_pts + nids]
+
+    # scalar domain fields (vars.rs:80-102)
+    def add_scldom_cnst(self, dom_id: int, value: float, file_path: str = "") -> int:
+        i = self.ctx.scldom_add(FIELD_CONSTANT, value)
+        self.scldom.append((FIELD_CONSTANT, None, []))
+        return i
+
+    def add_scldom_func(self, dom_id: int, func, unkn_id=(), file_path: str = "") -> int:
+        """func(t, pos[N,3], [values of each unknown]) -> values[N] (vectorised closure)."""
+        vals = np.asarray(func(0.0, self.pos, [self.ctx.scldom_get(u) for u in unkn_id]), np.float64)
+        i = self.ctx.scldom_add(FIELD_FUNCTION, 0.0, vals)
+        self.scldom.append((FIELD_FUNCTION, func, list(unkn_id)))
+        return i
+
+    def add_scldom_unkn(self, dom_id: int, value_init: float, file_path: str = "") -> int:
+        i = self.ctx.scldom_add(FIELD_UNKNOWN, value_init)
+        self.scldom.append((FIELD_UNKNOWN, None, []))
+        return i
+
+    def add_sclbnd_cnst(self, bnd_id: int, value: float, file_path: str = "") -> int:
+        i = self.ctx.sclbnd_add(self.bnd[bnd_id], value)
+        self.sclbnd.append((bnd_id, None, []))
+        return i
+
+    def add_sclbnd_func(self, bnd_id: int, func, unkn_id=(), file_path: str = "") -> int:
+        vals = np.asarray(func(0.0, self._bnd_pos(bnd_id), []), np.float64)
+        i = self.ctx.sclbnd_add(self.bnd[bnd_id], 0.0, vals)
+        self.sclbnd.append((bnd_id, func, list(unkn_id)))
+        return i
+
+    # vector domain fields (vars.rs:145-167); values SoA (3, N)
+    def add_vecdom_cnst(self, dom_id: int, value, file_path: str = "") -> int:
+        i = self.ctx.vecdom_add(FIELD_CONSTANT, cnst3=np.asarray(value, np.float64))
+        self.vecdom.append((FIELD_CONSTANT, None, []))
+        return i
+
+    def add_vecdom_func(self, dom_id: int, func, unkn_id=(), file_path: str = "") -> int:
+        vals = np.asarray(func(0.0, self.pos, []), np.float64).reshape(3, -1)
+        i = self.ctx.vecdom_add(FIELD_FUNCTION, values=vals)
+        self.vecdom.append((FIELD_FUNCTION, func, list(unkn_id)))
+        return i
+
+    def add_sclflw(self, dom_id: int, vel_id: int, pres_id: int, den_id: int, visc_id: int, fce_id: int) -> int:
+        i = self.ctx.sclflw_add(vel_id, pres_id, den_id, visc_id, fce_id)
+        self.sclflw.append((vel_id, pres_id, den_id, visc_id, fce_id))
+        return i
+
+    def update_func(self, t: float) -> None:
+        """_update_func (steady_base.rs:242-261): re-evaluate function fields on the host."""
+        for i, (kind, func, unkn) in enumerate(self.scldom):
+            if kind == FIELD_FUNCTION:
+                self.ctx.scldom_set(i, func(t, self.pos, [self.ctx.scldom_get(u) for u in unkn]))
+        for i, (bnd_id, func, unkn) in enumerate(self.sclbnd):
+            if func is not None:
+                self.ctx.sclbnd_set(i, func(t, self._bnd_pos(bnd_id), []))
+        for i, (kind, func, unkn) in enumerate(self.vecdom):
+            if kind == FIELD_FUNCTION:
+                self.ctx.vecdom_set(i, np.asarray(func(t, self.pos, []), np.float64).reshape(3, -1))
+
+    def value(self, scldom_id: int) -> np.ndarray:
+        return self.ctx.scldom_get(scldom_id)
+
+
+class LinearGmres:
+    """Jacobi-preconditioned restarted GMRES (LinearGmres, linear_gmres.rs:16-50).
+
+    Extra keyword options select the device variants: precision FP64 or
+    FP32_REFINED, precond JACOBI / ILU0 / NONE."""
+
+    def __init__(self, tol=1e-6, restart=50, max_outer=200, precision=FP64, precond=PRECOND_JACOBI, inner_tol=1e-5,
+                 max_refine=0):
+        self.opts = dict(tol=tol, restart=restart, max_outer=max_outer, precision=precision, precond=precond,
+                         inner_tol=inner_tol, max_refine=max_refine)
+
+    @staticmethod
+    def new(num_thread: int) -> "LinearGmres":
+        return LinearGmres.new_with_options(num_thread, 1e-6, 50, 200)
+
+    @staticmethod
+    def new_with_options(num_thread: int, tol: float, restart: int, max_outer: int, **kw) -> "LinearGmres":
+        # input checks of linear_gmres.rs:26-37
+        if num_thread == 0:
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, "LinearGmresNonPositiveThreads")
+        if not tol > 0.0:
+            raise VFVChemError(L.ERR_LINEAR_NON_POSITIVE_TOLERANCE, "LinearGmresNonPositiveTolerance")
+        if restart == 0:
+            raise VFVChemError(L.ERR_LINEAR_NON_POSITIVE_RESTART, "LinearGmresNonPositiveRestart")
+        if max_outer == 0:
+            raise VFVChemError(L.ERR_LINEAR_NON_POSITIVE_MAX_OUTER, "LinearGmresNonPositiveMaxOuter")
+        return LinearGmres(tol=tol, restart=restart, max_outer=max_outer, **kw)
+
+
+def _depends_on_unknowns(vars: Variables) -> bool:
+    return any(kind == FIELD_FUNCTION and unkn for kind, _, unkn in vars.scldom)
+
+
+class SteadyBase:
+    """SteadyBase::solve (steady_base.rs:16-135) over the operator list of assemble_matrix."""
+
+    def operators(self, vars: Variables):
+        raise NotImplementedError
+
+    def solve(self, vars: Variables, lin: LinearGmres, max_iter: int, tol: float, damp: float):
+        if _depends_on_unknowns(vars):
+            raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "function fields of unknowns need the host Picard loop (not on the built path)")
+        vars.update_func(0.0)
+        self.stats = vars.ctx.steady_solve(self.operators(vars), max_iter, tol, damp, **lin.opts)
+        return self.stats
+
+
+class TransientBase:
+    """TransientBase::solve (transient_base.rs:16-160), implicit Euler with a Picard loop per step."""
+
+    def operators(self, vars: Variables):
+        raise NotImplementedError
+
+    def solve(self, vars: Variables, lin: LinearGmres, dt: float, num_ts: int, num_ts_write: int, max_iter: int, tol: float,
+              damp: float):
+        if num_ts_write == 0:
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, "SolverNonPositiveWriteFrequency")
+        if _depends_on_unknowns(vars) or any(k == FIELD_FUNCTION for k, _, _ in vars.scldom):
+            # time-dependent closures: step from the host
+            raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "time-dependent function fields need the host time loop (not on the built path)")
+        self.stats = vars.ctx.transient_solve(self.operators(vars), dt, num_ts, max_iter, tol, damp, **lin.opts)
+        return self.stats
+
+
+class SteadyHeat(SteadyBase):
+    """SteadyHeat (steady_heat.rs:9-183), one domain."""
+
+    def __init__(self):
+        self.itr = []       # (dom, temp, cond, hprd, hsrc)
+        self.temp_bnd = []  # (bnd, temp)
+        self.hflx_bnd = []  # (bnd, hflx)
+
+    @staticmethod
+    def new():
+        return SteadyHeat()
+
+    def add_heat_dom(self, dom_id, temp_id, cond_id, hprd_id, hsrc_id):
+        self.itr.append((dom_id, temp_id, cond_id, hprd_id, hsrc_id))
+
+    def add_temp_bnd(self, bnd_id, temp_id):
+        self.temp_bnd.append((bnd_id, temp_id))
+
+    def add_hflx_bnd(self, bnd_id, hflx_id):
+        self.hflx_bnd.append((bnd_id, hflx_id))
+
+    def operators(self, vars: Variables):
+        # assemble_matrix order, steady_heat.rs:143-171
+        ops = []
+        (dom, T, k, hp, hs) = self.itr[0]
+        ops += [(OP_SCL_DOM_DIFFUSION, 0, T, k, T), (OP_SCL_DOM_PRODUCTION, 0, T, hp, -1), (OP_SCL_DOM_SOURCE, 0, T, hs, -1)]
+        for (b, tv) in self.temp_bnd:
+            ops.append((OP_SCL_BND_DIRICHLET, vars.bnd[b], T, tv, -1))
+        for (b, q) in self.hflx_bnd:
+            mb = vars.bnd[b]
+            ops += [(OP_SCL_BND_DIFFUSION, mb, T, k, T), (OP_SCL_BND_PRODUCTION, mb, T, hp, -1),
+                    (OP_SCL_BND_SOURCE, mb, T, hs, -1), (OP_SCL_BND_NEUMANN, mb, T, q, -1)]
+        return ops
+
+
+class TransientHeat(TransientBase):
+    """TransientHeat (transient_heat.rs), one domain."""
+
+    def __init__(self):
+        self.itr = []
+        self.temp_bnd = []
+        self.hflx_bnd = []
+
+    @staticmethod
+    def new():
+        return TransientHeat()
+
+    def add_heat_dom(self, dom_id, temp_id, vlcp_id, cond_id, hprd_id, hsrc_id):
+        self.itr.append((dom_id, temp_id, vlcp_id, cond_id, hprd_id, hsrc_id))
+
+    def add_temp_bnd(self, bnd_id, temp_id):
+        self.temp_bnd.append((bnd_id, temp_id))
+
+    def add_hflx_bnd(self, bnd_id, hflx_id):
+        self.hflx_bnd.append((bnd_id, hflx_id))
+
+    def operators(self, vars: Variables):
+        # assemble_matrix order, transient_heat.rs:152-171
+        (dom, T, w, k, hp, hs) = self.itr[0]
+        ops = [(OP_SCL_DOM_TIME, 0, T, w, -1), (OP_SCL_DOM_DIFFUSION, 0, T, k, T), (OP_SCL_DOM_PRODUCTION, 0, T, hp, -1),
+               (OP_SCL_DOM_SOURCE, 0, T, hs, -1)]
+        for (b, tv) in self.temp_bnd:
+            ops.append((OP_SCL_BND_DIRICHLET, vars.bnd[b], T, tv, -1))
+        for (b, q) in self.hflx_bnd:
+            mb = vars.bnd[b]
+            ops += [(OP_SCL_BND_TIME, mb, T, w, -1), (OP_SCL_BND_DIFFUSION, mb, T, k, T), (OP_SCL_BND_PRODUCTION, mb, T, hp, -1),
+                    (OP_SCL_BND_SOURCE, mb, T, hs, -1), (OP_SCL_BND_NEUMANN, mb, T, q, -1)]
+        return ops
+
+
+class SteadyMassFlow(SteadyBase):
+    """Mass part of SteadyMassFlow (steady_massflow.rs) for one component on a
+    prescribed flow: diffusion, production, source, power-law advection with the
+    unity weight (OpSclDomAdvectionUnity), Dirichlet, Neumann (mflx) and outflow
+    (mout) boundaries, in the order of steady_massflow.rs:455-545."""
+
+    def __init__(self):
+        self.itr = None
+        self.flw = None
+        self.conc_bnd, self.mflx_bnd, self.mout_bnd = [], [], []
+
+    @staticmethod
+    def new(num_comp: int = 1):
+        if num_comp != 1:
+            raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "one component on the built path")
+        return SteadyMassFlow()
+
+    def add_mass_dom(self, comp_id, dom_id, conc_id, diff_ids: dict, mprd_id, msrc_id):
+        if set(diff_ids) != {0}:
+            raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "cross-diffusion is outside the built path")
+        self.itr = (conc_id, diff_ids[0], mprd_id, msrc_id)
+
+    def add_flow_prescribed(self, dom_id, flw_id):
+        self.flw = flw_id
+
+    def add_conc_bnd(self, comp_id, bnd_id, conc_id):
+        self.conc_bnd.append((bnd_id, conc_id))
+
+    def add_mflx_bnd(self, comp_id, bnd_id, mflx_id):
+        self.mflx_bnd.append((bnd_id, mflx_id))
+
+    def add_mout_bnd(self, comp_id, bnd_id):
+        self.mout_bnd.append(bnd_id)
+
+    def operators(self, vars: Variables):
+        c, D, mp, ms = self.itr
+        f = self.flw
+        ops = [(OP_SCL_DOM_DIFFUSION, 0, c, D, c), (OP_SCL_DOM_PRODUCTION, 0, c, mp, -1), (OP_SCL_DOM_SOURCE, 0, c, ms, -1)]
+        if f is not None:
+            ops.append((OP_SCL_DOM_ADVECTION_UNITY, 0, c, D, f))
+        for (b, cv) in self.conc_bnd:
+            ops.append((OP_SCL_BND_DIRICHLET, vars.bnd[b], c, cv, -1))
+        for (b, q) in self.mflx_bnd:
+            mb = vars.bnd[b]
+            ops += [(OP_SCL_BND_DIFFUSION, mb, c, D, c), (OP_SCL_BND_PRODUCTION, mb, c, mp, -1),
+                    (OP_SCL_BND_SOURCE, mb, c, ms, -1), (OP_SCL_BND_NEUMANN, mb, c, q, -1)]
+        if f is not None:
+            for (b, q) in self.mflx_bnd:
+                ops.append((OP_SCL_BND_ADVECTION_UNITY, vars.bnd[b], c, D, f))
+        for b in self.mout_bnd:
+            mb = vars.bnd[b]
+            ops += [(OP_SCL_BND_DIFFUSION, mb, c, D, c), (OP_SCL_BND_ADVECTION_UNITY, mb, c, D, f),
+                    (OP_SCL_BND_PRODUCTION, mb, c, mp, -1), (OP_SCL_BND_SOURCE, mb, c, ms, -1)]
+        return ops
+
+
+class SteadyMass(SteadyMassFlow):
+    """SteadyMass (steady_mass.rs), one component: SteadyMassFlow without flow."""
+
+    @staticmethod
+    def new(num_comp: int = 1):
+        if num_comp != 1:
+            raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "one component on the built path")
+        return SteadyMass()

@@ -1,0 +1,267 @@
+"""Thin object layer over the C ABI: meshes, device contexts, operators."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib, ptr
+
+# operator kinds (include/vfvm.h, vfvm_op_kind)
+OP_SCL_DOM_DIFFUSION = 1
+OP_SCL_DOM_PRODUCTION = 2
+OP_SCL_DOM_SOURCE = 3
+OP_SCL_DOM_TIME = 4
+OP_SCL_DOM_ADVECTION_UNITY = 5
+OP_SCL_BND_DIRICHLET = 20
+OP_SCL_BND_NEUMANN = 21
+OP_SCL_BND_DIFFUSION = 22
+OP_SCL_BND_PRODUCTION = 23
+OP_SCL_BND_SOURCE = 24
+OP_SCL_BND_TIME = 25
+OP_SCL_BND_ADVECTION_UNITY = 26
+
+FIELD_CONSTANT, FIELD_FUNCTION, FIELD_UNKNOWN = 0, 1, 2
+FP64, FP32_REFINED = 0, 1
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_ILU0 = 0, 1, 2
+K_SPMV, K_ORTHO, K_ASSEMBLE, K_PRECOND = 0, 1, 2, 3
+
+_MESH_ARRAYS = {
+    "pos": (0, np.float64), "tets": (1, np.int32), "bnd_ptr": (2, np.int64), "bnd_nid": (3, np.int32),
+    "row_ptr": (4, np.int32), "ngh": (5, np.int32), "dist": (6, np.float64), "area": (7, np.float64),
+    "norm": (8, np.float64), "skew": (9, np.float64), "gw": (10, np.float64), "vol": (11, np.float64),
+    "cskew": (12, np.float64), "bsf_ptr": (13, np.int32), "bsf_vj": (14, np.int32), "bsf_vk": (15, np.int32),
+    "bsf_area": (16, np.float64), "bsf_norm": (17, np.float64), "bsf_wgt": (18, np.float64),
+}
+
+
+class Mesh:
+    """Host mesh with its Voronoi dual (Tetrahedron, src/base/geom_tet.rs:5-46)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        info = L.MeshInfo()
+        check(lib.vfvm_mesh_info_get(self._h, C.byref(info)))
+        self.num_pts = info.num_pts
+        self.num_nds = info.num_nds
+        self.num_tot = info.num_pts + info.num_nds
+        self.num_tets = info.num_tets
+        self.num_edges = info.num_edges
+        self.num_subfaces = info.num_subfaces
+        self.num_bnd = info.num_bnd
+        self.max_degree = info.max_degree
+        self._cache = {}
+
+    @classmethod
+    def lattice(cls, n: int, seed: int = 0, jitter: float = 0.3) -> "Mesh":
+        h = C.c_void_p()
+        check(lib.vfvm_mesh_lattice(int(n), int(seed), float(jitter), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_tets(cls, pos, tets, num_pts: int, bnd_nids) -> "Mesh":
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+        bptr = np.zeros(len(bnd_nids) + 1, np.int64)
+        for b, ids in enumerate(bnd_nids):
+            bptr[b + 1] = bptr[b] + len(ids)
+        bnid = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in bnd_nids]) if bnd_nids else np.zeros(0, np.int32), np.int32)
+        h = C.c_void_p()
+        check(lib.vfvm_mesh_from_tets(int(num_pts), int(pos.shape[0] - num_pts), ptr(pos), int(tets.shape[0]), ptr(tets, C.c_int32),
+                                      len(bnd_nids), ptr(bptr, C.c_int64), ptr(bnid, C.c_int32), C.byref(h)))
+        return cls(h)
+
+    def _size(self, name):
+        N, E, S, T = self.num_tot, self.num_edges, self.num_subfaces, self.num_tets
+        return {"pos": 3 * N, "tets": 4 * T, "bnd_ptr": self.num_bnd + 1, "bnd_nid": None, "row_ptr": N + 1, "ngh": E,
+                "dist": E, "area": E, "norm": 3 * E, "skew": 3 * E, "gw": 3 * E, "vol": N, "cskew": 3 * N,
+                "bsf_ptr": N + 1, "bsf_vj": S, "bsf_vk": S, "bsf_area": S, "bsf_norm": 3 * S, "bsf_wgt": 3 * S}[name]
+
+    def get(self, name: str) -> np.ndarray:
+        if name in self._cache:
+            return self._cache[name]
+        aid, dt = _MESH_ARRAYS[name]
+        n = self._size(name)
+        if n is None:
+            n = int(self.get("bnd_ptr")[-1])
+        out = np.empty(max(n, 1), dt)
+        check(lib.vfvm_mesh_get(self._h, aid, out.ctypes.data_as(C.c_void_p)))
+        out = out[:n]
+        if name == "pos":
+            out = out.reshape(-1, 3)
+        elif name == "tets":
+            out = out.reshape(-1, 4)
+        elif name in ("norm", "skew", "gw", "bsf_norm", "bsf_wgt", "cskew"):
+            out = out.reshape(3, -1)
+        self._cache[name] = out
+        return out
+
+    def bnd_nodes(self, b: int) -> np.ndarray:
+        p = self.get("bnd_ptr")
+        return self.get("bnd_nid")[p[b]:p[b + 1]]
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib.vfvm_mesh_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def make_ops(ops):
+    arr = (L.OpStruct * len(ops))()
+    for i, o in enumerate(ops):
+        arr[i].kind, arr[i].geom_id, arr[i].unk_id, arr[i].a, arr[i].b = (int(v) for v in o)
+    return arr
+
+
+def solve_opts(tol=1e-6, restart=50, max_outer=200, precision=FP64, precond=PRECOND_JACOBI, inner_tol=1e-5, max_refine=0):
+    o = L.SolveOpts()
+    lib.vfvm_solve_opts_default(C.byref(o))
+    o.tol, o.restart, o.max_outer, o.precision, o.precond, o.inner_tol, o.max_refine = (
+        float(tol), int(restart), int(max_outer), int(precision), int(precond), float(inner_tol), int(max_refine))
+    return o
+
+
+class Context:
+    """One device context (one HIP stream on one GPU)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        check(lib.vfvm_create(int(device), C.byref(self._h)))
+        self.mesh = None
+
+    def close(self):
+        if self._h:
+            lib.vfvm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, mesh: Mesh):
+        check(lib.vfvm_mesh_upload(self._h, mesh._h))
+        self.mesh = mesh
+
+    # fields
+    def scldom_add(self, kind, cnst=0.0, values=None) -> int:
+        i = C.c_int32()
+        v = None if values is None else np.ascontiguousarray(values, np.float64)
+        check(lib.vfvm_scldom_add(self._h, int(kind), float(cnst), None if v is None else ptr(v), C.byref(i)))
+        return i.value
+
+    def scldom_set(self, fid, values):
+        v = np.ascontiguousarray(values, np.float64)
+        check(lib.vfvm_scldom_set(self._h, int(fid), ptr(v)))
+
+    def scldom_get(self, fid) -> np.ndarray:
+        out = np.empty(self.mesh.num_tot)
+        check(lib.vfvm_scldom_get(self._h, int(fid), ptr(out)))
+        return out
+
+    def scldom_update_prev(self, fid):
+        check(lib.vfvm_scldom_update_prev(self._h, int(fid)))
+
+    def vecdom_add(self, kind, cnst3=None, values=None) -> int:
+        i = C.c_int32()
+        c3 = None if cnst3 is None else np.ascontiguousarray(cnst3, np.float64)
+        v = None if values is None else np.ascontiguousarray(values, np.float64)
+        check(lib.vfvm_vecdom_add(self._h, int(kind), None if c3 is None else ptr(c3), None if v is None else ptr(v), C.byref(i)))
+        return i.value
+
+    def vecdom_set(self, fid, values):
+        v = np.ascontiguousarray(values, np.float64)
+        check(lib.vfvm_vecdom_set(self._h, int(fid), ptr(v)))
+
+    def sclbnd_add(self, bnd, cnst=0.0, values=None) -> int:
+        i = C.c_int32()
+        v = None if values is None else np.ascontiguousarray(values, np.float64)
+        check(lib.vfvm_sclbnd_add(self._h, int(bnd), float(cnst), None if v is None else ptr(v), C.byref(i)))
+        return i.value
+
+    def sclbnd_set(self, fid, values):
+        v = np.ascontiguousarray(values, np.float64)
+        check(lib.vfvm_sclbnd_set(self._h, int(fid), ptr(v)))
+
+    def sclflw_add(self, vel, pres, den, visc, fce) -> int:
+        i = C.c_int32()
+        check(lib.vfvm_sclflw_add(self._h, int(vel), int(pres), int(den), int(visc), int(fce), C.byref(i)))
+        return i.value
+
+    # pattern / assembly
+    def pattern_build(self, ops) -> int:
+        arr = make_ops(ops)
+        nnz = C.c_int64()
+        check(lib.vfvm_pattern_build(self._h, arr, len(ops), C.byref(nnz)))
+        self.nnz = nnz.value
+        return nnz.value
+
+    def pattern_get(self):
+        rp = np.empty(self.mesh.num_tot + 1, np.int32)
+        col = np.empty(max(self.nnz, 1), np.int32)
+        check(lib.vfvm_pattern_get(self._h, ptr(rp, C.c_int32), ptr(col, C.c_int32)))
+        return rp, col[:self.nnz]
+
+    def assemble(self, ops, dt=0.0):
+        arr = make_ops(ops)
+        check(lib.vfvm_assemble(self._h, arr, len(ops), float(dt)))
+
+    def matrix_get(self):
+        vals = np.empty(max(self.nnz, 1))
+        b = np.empty(self.mesh.num_tot)
+        check(lib.vfvm_matrix_get(self._h, ptr(vals), ptr(b)))
+        return vals[:self.nnz], b
+
+    def solve(self, x0=None, **kw):
+        x = np.zeros(self.mesh.num_tot) if x0 is None else np.array(x0, np.float64)
+        st = L.SolveStats()
+        o = solve_opts(**kw)
+        code = lib.vfvm_solve(self._h, C.byref(o), ptr(x), C.byref(st))
+        check(code)
+        return x, st
+
+    def solve_device(self, x_dev_ptr: int, **kw):
+        st = L.SolveStats()
+        o = solve_opts(**kw)
+        check(lib.vfvm_solve_device(self._h, C.byref(o), C.c_void_p(x_dev_ptr), C.byref(st)))
+        return st
+
+    def residual(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        r = C.c_double()
+        check(lib.vfvm_residual(self._h, ptr(x), C.byref(r)))
+        return r.value
+
+    def steady_solve(self, ops, max_iter, tol, damp, **kw):
+        arr = make_ops(ops)
+        st = L.DriverStats()
+        o = solve_opts(**kw)
+        check(lib.vfvm_steady_solve(self._h, arr, len(ops), C.byref(o), int(max_iter), float(tol), float(damp), C.byref(st)))
+        return st
+
+    def transient_solve(self, ops, dt, num_ts, max_iter, tol, damp, **kw):
+        arr = make_ops(ops)
+        st = L.DriverStats()
+        o = solve_opts(**kw)
+        check(lib.vfvm_transient_solve(self._h, arr, len(ops), C.byref(o), float(dt), int(num_ts), int(max_iter), float(tol),
+                                       float(damp), C.byref(st)))
+        return st
+
+    # timing
+    def timing_enable(self, on=True):
+        check(lib.vfvm_timing_enable(self._h, 1 if on else 0))
+
+    def timing_reset(self):
+        check(lib.vfvm_timing_reset(self._h))
+
+    def timing_get(self, kclass):
+        n = C.c_int64()
+        ms = C.c_double()
+        by = C.c_double()
+        check(lib.vfvm_timing_get(self._h, int(kclass), C.byref(n), C.byref(ms), C.byref(by)))
+        return n.value, ms.value, by.value
