@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Headline benchmark: solved DOF/s at 1e-10 relative residual, FP32-inner /
+FP64-refined GMRES vs plain FP64 GMRES, on the steady convection-diffusion
+workload (100^3 jittered lattice, 1,000,000 vertices, cellular flow at cell
+Peclet 5, D = 1e-2, c = 1 at x = 0, zero diffusive flux elsewhere).
+
+A step is one linear solve of the assembled system from x0 = 0 to
+||b - A x||_2 / ||b||_2 <= 1e-10 (FP64 residual), preconditioner set-up
+included, with the matrix, b and x resident in HBM.  Mesh, dual and assembly
+are built before the timed region (assembly time is reported beside it).
+
+Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vfvchem-vfvm3d_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cellular_velocity(pos, U0):
+    import numpy as np
+    x, y = pos[:, 0], pos[:, 1]
+    return np.stack([U0 * np.sin(np.pi * x) * np.cos(np.pi * y), -U0 * np.cos(np.pi * x) * np.sin(np.pi * y), 0.0 * x])
+
+
+def build_problem(vfvm, n, D=1e-2):
+    """Fields and operator list of the steady mass transport on a prescribed flow, in
+    the operator order of steady_massflow.rs:455-545 (mass part)."""
+    import numpy as np
+    mesh = vfvm.Mesh.lattice(n, 0, 0.3)
+    ctx = vfvm.Context(0)
+    ctx.upload(mesh)
+    pos = mesh.get("pos")
+    h = 1.0 / (n - 1)
+    U0 = 5.0 * D / h  # cell Peclet number 5
+    c = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0)
+    Dd = ctx.scldom_add(vfvm.FIELD_CONSTANT, D)
+    mp = ctx.scldom_add(vfvm.FIELD_CONSTANT, 0.0)
+    ms = ctx.scldom_add(vfvm.FIELD_CONSTANT, 0.0)
+    den = ctx.scldom_add(vfvm.FIELD_CONSTANT, 1.0)
+    visc = ctx.scldom_add(vfvm.FIELD_CONSTANT, 1.0)
+    pres = ctx.scldom_add(vfvm.FIELD_CONSTANT, 0.0)
+    vel = ctx.vecdom_add(vfvm.FIELD_FUNCTION, values=cellular_velocity(pos, U0))
+    fce = ctx.vecdom_add(vfvm.FIELD_CONSTANT, cnst3=np.zeros(3))
+    f = ctx.sclflw_add(vel, pres, den, visc, fce)
+    cin = ctx.sclbnd_add(0, 1.0)
+    flux_b = [1, 2, 3, 4, 5]
+    q = {b: ctx.sclbnd_add(b, 0.0) for b in flux_b}
+    ops = [(vfvm.OP_SCL_DOM_DIFFUSION, 0, c, Dd, c), (vfvm.OP_SCL_DOM_PRODUCTION, 0, c, mp, -1),
+           (vfvm.OP_SCL_DOM_SOURCE, 0, c, ms, -1), (vfvm.OP_SCL_DOM_ADVECTION_UNITY, 0, c, Dd, f),
+           (vfvm.OP_SCL_BND_DIRICHLET, 0, c, cin, -1)]
+    for b in flux_b:
+        ops += [(vfvm.OP_SCL_BND_DIFFUSION, b, c, Dd, c), (vfvm.OP_SCL_BND_PRODUCTION, b, c, mp, -1),
+                (vfvm.OP_SCL_BND_SOURCE, b, c, ms, -1), (vfvm.OP_SCL_BND_NEUMANN, b, c, q[b], -1)]
+    for b in flux_b:
+        ops.append((vfvm.OP_SCL_BND_ADVECTION_UNITY, b, c, Dd, f))
+    return mesh, ctx, ops
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=100, help="lattice points per axis")
+    ap.add_argument("--precond", default="jacobi", choices=["jacobi", "ilu0", "none"])
+    ap.add_argument("--restart", type=int, default=50)
+    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--cpu-sample-steps", type=int, default=50, help="Arnoldi steps timed on the CPU (0: skip)")
+    ap.add_argument("--variants", default="fp64,mixed")
+    ap.add_argument("--ortho", default="auto", choices=["auto", "cgs", "cgs2"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is a design only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import vfvm
+
+    pc = {"jacobi": vfvm.PRECOND_JACOBI, "ilu0": vfvm.PRECOND_ILU0, "none": vfvm.PRECOND_NONE}[args.precond]
+    t0 = time.time()
+    mesh, ctx, ops = build_problem(vfvm, args.n)
+    t_mesh = time.time() - t0
+    nnz = ctx.pattern_build(ops)
+    ctx.timing_enable(-1)
+    ctx.timing_reset()
+    ta = time.time()
+    ctx.assemble(ops)
+    t_asm_wall = time.time() - ta
+    _, asm_ms, asm_bytes = ctx.timing_get(vfvm.K_ASSEMBLE)
+    N = mesh.num_tot
+    x = torch.zeros(N, dtype=torch.float64, device="cuda")
+    xp = x.data_ptr()
+
+    results = {}
+    variants = [v for v in args.variants.split(",") if v]
+    for var in variants:
+        prec = vfvm.FP64 if var == "fp64" else vfvm.FP32_REFINED
+        kw = dict(tol=args.tol, restart=args.restart, max_outer=2000, precision=prec, precond=pc, inner_tol=args.inner_tol,
+                  ortho={"auto": 0, "cgs": 1, "cgs2": 2}[args.ortho])
+        for _ in range(args.warmup):
+            x.zero_()
+            torch.cuda.synchronize()
+            ctx.solve_device(xp, **kw)
+        ctx.timing_reset()
+        stats = []
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            x.zero_()
+            torch.cuda.synchronize()
+            st = ctx.solve_device(xp, **kw)
+            stats.append((st.iterations, st.refinements, st.rel_residual, st.seconds, st.setup_seconds))
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        kern = {}
+        for name, k in (("spmv", vfvm.K_SPMV), ("mdot", vfvm.K_MDOT), ("update", vfvm.K_UPDATE), ("ilu_sweep", vfvm.K_PRECOND)):
+            nl, ms, by = ctx.timing_get(k)
+            if nl:
+                kern[name] = dict(launches=nl, avg_us=1e3 * ms / nl, total_ms=ms / args.steps, bytes_per_launch=by / nl,
+                                  gbs=by / (ms * 1e-3) / 1e9)
+        err = float((x - 1.0).abs().max().item())
+        results[var] = dict(ms_per_step=1e3 * (t2 - t1) / args.steps, iters=stats[-1][0], refinements=stats[-1][1],
+                            rel_residual=stats[-1][2], device_s=stats[-1][3], setup_s=stats[-1][4], kernels=kern,
+                            max_abs_err_vs_exact=err)
+
+    head = "mixed" if "mixed" in results else variants[0]
+    r = results[head]
+    value = N / (r["ms_per_step"] * 1e-3)
+    # dominant kernel of the headline variant: largest total time
+    dom_name, dom = max(r["kernels"].items(), key=lambda kv: kv[1]["total_ms"])
+    achieved = dom["bytes_per_launch"] / (dom["avg_us"] * 1e-6) / 1e9
+
+    cpu = None
+    if args.cpu_sample_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import vfvm_oracle as O
+        vals, b = ctx.matrix_get()
+        rp, col = ctx.pattern_get()
+        tc = time.perf_counter()
+        _, _, steps_done, _ = O.gmres(rp, col, vals, b, None, args.restart, 1, args.tol, args.cpu_sample_steps)
+        tcpu = time.perf_counter() - tc
+        per_step = tcpu / max(steps_done, 1)
+        need = results.get("fp64", r)["iters"]
+        cpu = {"value": N / (per_step * need), "unit": "DOF/s", "cores": 1, "kind": "port",
+               "sample": f"{steps_done} Arnoldi steps of Jacobi-GMRES({args.restart}) FP64 (oracle/gmres_ref.c, 1 thread) on the "
+                         f"same {N}-DOF system ({tcpu:.1f} s), extrapolated to the {need} steps the FP64 solve needs",
+               "cpu": cpu_model(), "nproc": os.cpu_count()}
+
+    line = {
+        "metric": "solved DOF/s at 1e-10 rel residual (FP32-inner/FP64-refined GMRES; FP64 GMRES beside it)",
+        "value": value,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-refined" if head == "mixed" else "f64",
+        "data": "synthetic (seeded jittered lattice, prescribed cellular velocity)",
+        "config": {"workload": f"steady convection-diffusion, {args.n}^3 jittered lattice ({N} vertices, {N} DOF, {nnz} nnz), "
+                               f"cell Peclet 5, D=1e-2, GMRES({args.restart}) + {args.precond}, tol {args.tol}",
+                   "dof": N, "nnz": nnz, "precond": args.precond, "restart": args.restart},
+        "fp64": results.get("fp64"),
+        "mixed": results.get("mixed"),
+        "mixed_over_fp64": (results["fp64"]["ms_per_step"] / results["mixed"]["ms_per_step"]) if ("fp64" in results and "mixed" in results) else None,
+        "assembly": {"kernel_ms": asm_ms, "wall_ms": 1e3 * t_asm_wall, "bytes": asm_bytes,
+                     "gbs": asm_bytes / (asm_ms * 1e-3) / 1e9 if asm_ms > 0 else None},
+        "mesh_and_dual_s": t_mesh,
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

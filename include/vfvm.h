@@ -200,7 +200,11 @@ int vfvm_matrix_get(vfvm_ctx* ctx, double* vals, double* b);
 
 /* ---- linear solve (LinearGmres) ---- */
 enum vfvm_precision { VFVM_FP64 = 0, VFVM_FP32_REFINED = 1 };
+/* ILU0: ILU(0) of the matrix in multicolour order (greedy colouring of the row graph,
+ * rows sorted by (colour, index)); its triangular sweeps run one launch per colour. */
 enum vfvm_precond { VFVM_PRECOND_NONE = 0, VFVM_PRECOND_JACOBI = 1, VFVM_PRECOND_ILU0 = 2 };
+/* AUTO: CGS2 (one re-orthogonalisation) for FP64 GMRES, CGS for the FP32 inner solve. */
+enum vfvm_ortho { VFVM_ORTHO_AUTO = 0, VFVM_ORTHO_CGS = 1, VFVM_ORTHO_CGS2 = 2 };
 typedef struct vfvm_solve_opts {
     double tol;          /* relative residual ||b-Ax||/||b|| target (default 1e-6, linear_gmres.rs:21) */
     int32_t restart;     /* Krylov dimension per cycle (default 50)                    */
@@ -209,7 +213,7 @@ typedef struct vfvm_solve_opts {
     int32_t precond;     /* vfvm_precond (default Jacobi, linear_gmres.rs:79)          */
     double inner_tol;    /* FP32 inner relative tolerance per refinement step (1e-5)   */
     int32_t max_refine;  /* maximum refinement steps (FP32_REFINED)                    */
-    int32_t reserved;
+    int32_t ortho;       /* vfvm_ortho: Gram-Schmidt variant (0 = auto)                */
 } vfvm_solve_opts;
 typedef struct vfvm_solve_stats {
     int32_t iterations;      /* Arnoldi steps                               */
@@ -247,9 +251,12 @@ int vfvm_transient_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const 
                          vfvm_driver_stats* stats);
 
 /* ---- measurement ---- */
-/* Enables per-launch HIP-event timing of the named kernel class on the context stream. */
-enum vfvm_kernel_class { VFVM_K_SPMV = 0, VFVM_K_ORTHO = 1, VFVM_K_ASSEMBLE = 2, VFVM_K_PRECOND = 3, VFVM_K_COUNT = 4 };
-int vfvm_timing_enable(vfvm_ctx* ctx, int32_t enable);
+/* Per-launch HIP-event timing on the context stream: bit k of `mask` times kernel class k
+ * (-1: all classes, 0: off).  Each timed launch is bracketed by two events. */
+/* SPMV: the Krylov SpMV kernel; MDOT: the multi-dot of the Gram-Schmidt pass; UPDATE: the
+ * w -= V h update; ASSEMBLE: the row-gather assembly; PRECOND: one ILU(0) triangular-sweep launch. */
+enum vfvm_kernel_class { VFVM_K_SPMV = 0, VFVM_K_MDOT = 1, VFVM_K_ASSEMBLE = 2, VFVM_K_PRECOND = 3, VFVM_K_UPDATE = 4, VFVM_K_COUNT = 5 };
+int vfvm_timing_enable(vfvm_ctx* ctx, int32_t mask);
 /* Returns launches and total milliseconds of one kernel class since the last reset, and
  * the algorithmic bytes those launches moved (per the byte model of each kernel). */
 int vfvm_timing_get(vfvm_ctx* ctx, int32_t kclass, int64_t* launches, double* total_ms, double* bytes);

@@ -531,3 +531,37 @@ def coo_to_csr(rows, cols, vals, n):
     row_ptr = np.zeros(n + 1, np.int64)
     np.add.at(row_ptr, ur + 1, 1)
     return np.cumsum(row_ptr), uc.astype(np.int64), out
+
+
+# --------------------------------------------------------------------------- CPU GMRES (C)
+def _liboracle():
+    import ctypes as C
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "liboracle.so")
+    lib = C.CDLL(path)
+    P64 = C.POINTER(C.c_int64)
+    PD = C.POINTER(C.c_double)
+    lib.orc_gmres.restype = C.c_int
+    lib.orc_gmres.argtypes = [C.c_int64, P64, P64, PD, PD, PD, C.c_int, C.c_int, C.c_double, C.c_int64, P64, PD]
+    return lib
+
+
+def gmres(rp, col, val, b, x0=None, restart=50, max_outer=200, tol=1e-6, max_steps=0):
+    """Jacobi-preconditioned restarted GMRES in FP64, single thread
+    (oracle/gmres_ref.c; LinearGmres::solve_matrix, linear_gmres.rs:76-84).
+    Returns (x, converged, steps, rel_residual)."""
+    import ctypes as C
+    lib = _liboracle()
+    n = len(rp) - 1
+    rp = np.ascontiguousarray(rp, np.int64)
+    col = np.ascontiguousarray(col, np.int64)
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    it = C.c_int64()
+    rel = C.c_double()
+    P64 = C.POINTER(C.c_int64)
+    PD = C.POINTER(C.c_double)
+    st = lib.orc_gmres(n, rp.ctypes.data_as(P64), col.ctypes.data_as(P64), val.ctypes.data_as(PD), b.ctypes.data_as(PD),
+                       x.ctypes.data_as(PD), int(restart), int(max_outer), float(tol), int(max_steps), C.byref(it), C.byref(rel))
+    return x, st == 0, it.value, rel.value

@@ -518,7 +518,7 @@ extern "C" int vfvm_assemble(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, do
     CTX_GUARD(ctx);
     if (int st = assemble_impl(c, ops, nops, dt)) return st;
     if (int st = sync_err(c)) return st;
-    if (c.timing) if (int st = ktime_flush(c)) return st;
+    if (int st = ktime_flush(c)) return st;
     return clear_error();
 }
 
@@ -715,7 +715,7 @@ extern "C" int vfvm_transient_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t n
 // ------------------------------------------------------------------ timing
 extern "C" int vfvm_timing_enable(vfvm_ctx* ctx, int32_t enable) {
     CTX_GUARD(ctx);
-    c.timing = enable != 0;
+    c.timing_mask = (uint32_t)enable;
     return clear_error();
 }
 extern "C" int vfvm_timing_get(vfvm_ctx* ctx, int32_t kclass, int64_t* launches, double* total_ms, double* bytes) {

@@ -128,15 +128,16 @@ struct Ctx {
     // solver workspace
     DevBuf<double> V64, w64, z64, x64, r64, dinv64, avals64, part, scal;
     DevBuf<float> V32, w32, avals32;
-    DevBuf<int32_t> ilu_diag, ilu_lvl_rows, ilu_lvl_ptr;
+    // multicolour ILU(0): rows grouped by colour, row entries reordered [L | diag | U] by (colour, index)
+    DevBuf<int32_t> ilu_rows, ilu_src, ilu_col, ilu_diag, ilu_color;
     DevBuf<double> ilu64;
     DevBuf<float> ilu32;
-    std::vector<int32_t> h_lvl_ptr;
+    std::vector<int32_t> h_color_ptr;
     bool ilu_levels_ready = false;
     double* h_pinned = nullptr;  // small pinned readback buffer
 
     // timing
-    bool timing = false;
+    uint32_t timing_mask = 0;  // bit k: time kernel class k
     std::array<KTimer, VFVM_K_COUNT> kt;
     std::vector<hipEvent_t> ev_pool;
     struct Pending { int kclass; double bytes; hipEvent_t a, b; int step; };

@@ -1,22 +1,23 @@
-// Restarted GMRES on the device: CSR SpMV, Jacobi and ILU(0) right
-// preconditioning, classical Gram-Schmidt with one re-orthogonalisation pass
-// (CGS2), Givens least squares, FP64 and FP32-inner / FP64-refined variants.
+// Restarted GMRES on the device: CSR SpMV, Jacobi and multicolour ILU(0) right
+// preconditioning, classical Gram-Schmidt (CGS, or CGS2 with one
+// re-orthogonalisation pass), Givens least squares, FP64 and FP32-inner /
+// FP64-refined variants.
 //
 // Replaces LinearGmres::solve_matrix (src/linear/linear_gmres.rs:76-84), which
 // calls faer_gmres::restarted_gmres(A, b, x, restart, max_outer, tol, Jacobi).
 // faer_gmres 0.5.0 is not vendored with the reference; this solver implements
 // the textbook algorithm (Saad, Iterative Methods for Sparse Linear Systems,
-// 2nd ed., Alg. 6.9 GMRES(m) with right preconditioning, section 10.3 ILU(0))
-// and stops on the FP64 relative residual ||b - A x||_2 / ||b||_2 <= tol.
+// 2nd ed., Alg. 6.10 GMRES(m) with right preconditioning; Alg. 10.4 ILU(0);
+// section 12.4 multicolour orderings) and stops on the FP64 relative residual
+// ||b - A x||_2 / ||b||_2 <= tol, recomputed from x at every restart.
 //
 // Device-resident state: the Krylov basis V (column-major, n x (m+1)), the
 // Hessenberg matrix, Givens rotations and the control word live in HBM; the
-// host only reads the control word every few steps.  Reductions are two-stage
-// and fixed-order, so results are bitwise reproducible run to run.
+// host reads the control word every few steps.  Reductions are two-stage and
+// fixed-order, so results are bitwise reproducible run to run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -29,20 +30,20 @@ namespace {
 
 constexpr int BLK = 256;
 constexpr int WAVE = 64;
-constexpr int CHUNK = 8;      // basis vectors per pass of the multi-dot
+constexpr int CHUNK = 8;  // basis vectors per pass of the multi-dot
 constexpr int MAX_RESTART = 256;
 
 // ---------------------------------------------------------------- state layout (doubles)
 struct StateLayout {
     int m;
-    __host__ __device__ int64_t H() const { return 0; }                           // (m+1) x m column-major
+    __host__ __device__ int64_t H() const { return 0; }                    // (m+1) x m column-major
     __host__ __device__ int64_t cs() const { return (int64_t)(m + 1) * m; }
     __host__ __device__ int64_t sn() const { return cs() + m; }
-    __host__ __device__ int64_t g() const { return sn() + m; }                     // m+1
-    __host__ __device__ int64_t y() const { return g() + m + 1; }                  // m
-    __host__ __device__ int64_t h1() const { return y() + m; }                     // m+1
-    __host__ __device__ int64_t h2() const { return h1() + m + 1; }                // m+1
-    __host__ __device__ int64_t misc() const { return h2() + m + 1; }              // [0]=beta [1]=inv_h [2]=target [3]=res_est [4]=scale
+    __host__ __device__ int64_t g() const { return sn() + m; }              // m+1
+    __host__ __device__ int64_t y() const { return g() + m + 1; }           // m
+    __host__ __device__ int64_t h1() const { return y() + m; }              // m+1
+    __host__ __device__ int64_t h2() const { return h1() + m + 1; }         // m+1
+    __host__ __device__ int64_t misc() const { return h2() + m + 1; }       // [0]=beta [1]=inv_h [2]=target [3]=res_est [4]=1.0 [5]=||b||
     __host__ __device__ int64_t total() const { return misc() + 8; }
 };
 // control word (ints): [0] active, [1] steps done in this cycle, [2] non-finite flag
@@ -67,7 +68,7 @@ __device__ inline double block_sum(double v, double* sh) {
 }
 
 // ---------------------------------------------------------------- SpMV
-// LPR lanes per row; scalar rows carry ~17 entries, so 4 or 8 lanes per row
+// LPR lanes per row; scalar rows carry ~17 entries
 template <class T, int LPR>
 __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                               const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
@@ -128,11 +129,12 @@ __global__ void __launch_bounds__(BLK) k_mdot(int64_t n, int nv, const T* __rest
 #pragma unroll
         for (int q = 0; q < CHUNK; ++q) acc[q] = 0.0;
         const int kq = min(CHUNK, nv - k0);
+        const T* Vk = V + (int64_t)k0 * ldv;
         for (int64_t i = r0 + threadIdx.x; i < r1; i += BLK) {
             const double wi = (double)w[i];
 #pragma unroll
             for (int q = 0; q < CHUNK; ++q)
-                if (q < kq) acc[q] += (double)V[(int64_t)(k0 + q) * ldv + i] * wi;
+                if (q < kq) acc[q] += (double)Vk[(int64_t)q * ldv + i] * wi;
         }
 #pragma unroll
         for (int q = 0; q < CHUNK; ++q) {
@@ -149,16 +151,16 @@ __global__ void __launch_bounds__(BLK) k_mdot(int64_t n, int nv, const T* __rest
     }
 }
 
-// out[k] (+)= sum_blk part[k][blk]; one block per k
+// out[k] = sum_blk part[k][blk]; one block per k
 __global__ void __launch_bounds__(BLK) k_reduce(int nblk, const double* __restrict__ part, double* __restrict__ out,
-                                                int accumulate, const int* __restrict__ ctrl) {
+                                                const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
     __shared__ double sh[BLK / WAVE];
     const int k = blockIdx.x;
     double s = 0.0;
     for (int i = threadIdx.x; i < nblk; i += BLK) s += part[(int64_t)k * nblk + i];
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[k] = accumulate ? out[k] + s : s;
+    if (threadIdx.x == 0) out[k] = s;
 }
 
 // w -= sum_k V[k] h[k]; optional per-block partial of ||w||^2
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__(BLK) k_update(int64_t n, int nv, const T* __re
     }
 }
 
-// dst = src * S[misc + idx]
+// dst = src * S[idx]
 template <class TS, class TD>
 __global__ void __launch_bounds__(BLK) k_scale(int64_t n, const TS* __restrict__ src, TD* __restrict__ dst,
                                                const double* __restrict__ S, int64_t idx, const int* __restrict__ ctrl) {
@@ -198,9 +200,10 @@ __global__ void __launch_bounds__(BLK) k_scale(int64_t n, const TS* __restrict__
     dst[i] = (TD)((double)src[i] * S[idx]);
 }
 
-// start of a cycle: beta = ||r|| from residual partials; g = [beta, 0, ...]; inv = 1/beta
+// start of a cycle: beta = ||r|| from residual partials (or the stored beta);
+// g = [beta, 0, ...]; inv = 1/beta; target of the Arnoldi estimate
 __global__ void __launch_bounds__(BLK) k_cycle_start(int nblk, const double* __restrict__ part, double* __restrict__ S,
-                                                     StateLayout L, int* __restrict__ ctrl, double target_rel,
+                                                     StateLayout L, int* __restrict__ ctrl, double target_abs,
                                                      int use_given_beta) {
     __shared__ double sh[BLK / WAVE];
     double s = 0.0, sb = 0.0;
@@ -214,7 +217,7 @@ __global__ void __launch_bounds__(BLK) k_cycle_start(int nblk, const double* __r
         if (!use_given_beta) S[L.misc() + 5] = sqrt(sb);  // ||b||
         S[L.misc() + 0] = beta;
         S[L.misc() + 1] = beta > 0.0 ? 1.0 / beta : 0.0;
-        S[L.misc() + 2] = target_rel;
+        S[L.misc() + 2] = target_abs;
         S[L.misc() + 3] = beta;
         S[L.misc() + 4] = 1.0;
         for (int i = 0; i <= L.m; ++i) S[L.g() + i] = 0.0;
@@ -225,8 +228,8 @@ __global__ void __launch_bounds__(BLK) k_cycle_start(int nblk, const double* __r
     }
 }
 
-// Arnoldi step j: h = h1 + h2, h_{j+1,j} = ||w||, Givens update, convergence test
-__global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int nblk, const double* __restrict__ part_nrm,
+// Arnoldi step j: h = h1 (+ h2), h_{j+1,j} = ||w||, Givens update, convergence test
+__global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int reorth, int nblk, const double* __restrict__ part_nrm,
                                                       double* __restrict__ S, StateLayout L, int* __restrict__ ctrl) {
     if (ctrl[0] == 0) return;
     __shared__ double sh[BLK / WAVE];
@@ -242,7 +245,7 @@ __global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int nblk, const dou
     const double* h1 = S + L.h1();
     const double* h2 = S + L.h2();
     const int64_t cj = (int64_t)j * (m + 1);
-    for (int i = 0; i <= j; ++i) H[cj + i] = h1[i] + h2[i];
+    for (int i = 0; i <= j; ++i) H[cj + i] = reorth ? h1[i] + h2[i] : h1[i];
     const double hn = sqrt(s);
     H[cj + j + 1] = hn;
     for (int i = 0; i < j; ++i) {
@@ -285,7 +288,7 @@ __global__ void k_backsolve(double* __restrict__ S, StateLayout L, const int* __
     }
 }
 
-// z = sum_{l<k} V[l] y[l] (z in FP64, overwritten)
+// z = sum_{l<k} V[l] y[l] (z in FP64)
 template <class T>
 __global__ void __launch_bounds__(BLK) k_combine(int64_t n, const T* __restrict__ V, int64_t ldv,
                                                  const double* __restrict__ S, StateLayout L, const int* __restrict__ ctrl,
@@ -301,12 +304,12 @@ __global__ void __launch_bounds__(BLK) k_combine(int64_t n, const T* __restrict_
     z[i] = s;
 }
 
-// x += scale * dinv .* z
+// x += dinv .* z (dinv == nullptr: x += z)
 __global__ void __launch_bounds__(BLK) k_xupdate(int64_t n, const double* __restrict__ z, const double* __restrict__ dinv,
-                                                 double scale, double* __restrict__ x) {
+                                                 double* __restrict__ x) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
-    x[i] += scale * (dinv ? dinv[i] * z[i] : z[i]);
+    x[i] += dinv ? dinv[i] * z[i] : z[i];
 }
 
 // Jacobi: dinv = 1 / diag(A) (1 where the diagonal is absent or zero)
@@ -327,57 +330,81 @@ __global__ void __launch_bounds__(BLK) k_scale_cols(int64_t nnz, const int32_t* 
     out[k] = (T)(val[k] * dinv[col[k]]);
 }
 
-// ---------------------------------------------------------------- ILU(0) (Saad Alg. 10.4), level-scheduled
-// factor in place of a copy of the (scaled) matrix values; rows of one level are independent
+// ---------------------------------------------------------------- multicolour ILU(0)
+// Ordering sigma(j) = (colour(j), j); no two rows of one colour share an entry, so all
+// rows of a colour are independent in the factorisation and in both sweeps.  Row
+// entries are stored [L (sigma-ascending) | diag | U (sigma-ascending)].
 template <class T>
-__global__ void __launch_bounds__(BLK) k_ilu0_factor_level(const int32_t* __restrict__ rows, int32_t nrows,
-                                                           const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                           const int32_t* __restrict__ diag, T* __restrict__ lu) {
+__global__ void __launch_bounds__(BLK) k_ilu_gather(int64_t nnz, const int32_t* __restrict__ src, const double* __restrict__ val,
+                                                    T* __restrict__ lu) {
+    const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (k >= nnz) return;
+    lu[k] = (T)val[src[k]];
+}
+
+__device__ inline int64_t sigma_key(const int32_t* __restrict__ color, int64_t n, int32_t j) {
+    return (int64_t)color[j] * n + j;
+}
+
+// IKJ ILU(0) (Saad Alg. 10.4) for the rows of one colour
+template <class T>
+__global__ void __launch_bounds__(BLK) k_ilu_factor(const int32_t* __restrict__ rows, int32_t nrows, int64_t n,
+                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                    const int32_t* __restrict__ diag, const int32_t* __restrict__ color,
+                                                    T* __restrict__ lu) {
     const int32_t t = blockIdx.x * BLK + threadIdx.x;
     if (t >= nrows) return;
     const int32_t i = rows[t];
-    const int32_t s = rp[i], e = rp[i + 1], di = diag[i];
-    for (int32_t kk = s; kk < di; ++kk) {
+    const int32_t e = rp[i + 1], di = diag[i];
+    for (int32_t kk = rp[i]; kk < di; ++kk) {
         const int32_t k = col[kk];
-        const T piv = lu[diag[k]];
-        const T lik = lu[kk] / piv;
+        const T lik = lu[kk] / lu[diag[k]];
         lu[kk] = lik;
-        // a_ij -= l_ik u_kj for j in row k right of its diagonal, when (i, j) is in the pattern
+        // a_ij -= l_ik u_kj for j right of k in sigma order, when (i, j) is in the pattern
         int32_t p = kk + 1;
         for (int32_t q = diag[k] + 1; q < rp[k + 1]; ++q) {
             const int32_t j = col[q];
-            while (p < e && col[p] < j) ++p;
+            const int64_t kj = sigma_key(color, n, j);
+            while (p < e && sigma_key(color, n, col[p]) < kj) ++p;
             if (p < e && col[p] == j) lu[p] -= lik * lu[q];
         }
     }
 }
 
-// forward (L, unit diagonal) then backward (U) substitution, one level per launch
-template <class T>
-__global__ void __launch_bounds__(BLK) k_ilu0_lower(const int32_t* __restrict__ rows, int32_t nrows,
-                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                    const int32_t* __restrict__ diag, const T* __restrict__ lu,
-                                                    const T* __restrict__ rhs, T* __restrict__ z, const int* __restrict__ ctrl) {
+// forward sweep (L has a unit diagonal): z_i = r_i - sum_{L} l_ij z_j
+template <class T, int LPR>
+__global__ void __launch_bounds__(BLK) k_ilu_lower(const int32_t* __restrict__ rows, int32_t nrows,
+                                                   const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                   const int32_t* __restrict__ diag, const T* __restrict__ lu,
+                                                   const T* __restrict__ rhs, T* __restrict__ z, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
-    const int32_t t = blockIdx.x * BLK + threadIdx.x;
+    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t t = tid / LPR;
+    const int lane = (int)(tid % LPR);
     if (t >= nrows) return;
     const int32_t i = rows[t];
-    T s = rhs[i];
-    for (int32_t k = rp[i]; k < diag[i]; ++k) s -= lu[k] * z[col[k]];
-    z[i] = s;
+    T s = 0;
+    for (int32_t k = rp[i] + lane; k < diag[i]; k += LPR) s += lu[k] * z[col[k]];
+    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
+    if (lane == 0) z[i] = rhs[i] - s;
 }
-template <class T>
-__global__ void __launch_bounds__(BLK) k_ilu0_upper(const int32_t* __restrict__ rows, int32_t nrows,
-                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                    const int32_t* __restrict__ diag, const T* __restrict__ lu,
-                                                    T* __restrict__ z, const int* __restrict__ ctrl) {
+// backward sweep: z_i = (z_i - sum_{U} u_ij z_j) / u_ii
+template <class T, int LPR>
+__global__ void __launch_bounds__(BLK) k_ilu_upper(const int32_t* __restrict__ rows, int32_t nrows,
+                                                   const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                   const int32_t* __restrict__ diag, const T* __restrict__ lu,
+                                                   T* __restrict__ z, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
-    const int32_t t = blockIdx.x * BLK + threadIdx.x;
+    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t t = tid / LPR;
+    const int lane = (int)(tid % LPR);
     if (t >= nrows) return;
     const int32_t i = rows[t];
-    T s = z[i];
-    for (int32_t k = diag[i] + 1; k < rp[i + 1]; ++k) s -= lu[k] * z[col[k]];
-    z[i] = s / lu[diag[i]];
+    T s = 0;
+    const int32_t e = rp[i + 1];
+    for (int32_t k = diag[i] + 1 + lane; k < e; k += LPR) s += lu[k] * z[col[k]];
+    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
+    if (lane == 0) z[i] = (z[i] - s) / lu[diag[i]];
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
@@ -385,19 +412,24 @@ inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 struct HipEvents {
     hipEvent_t e[3] = {nullptr, nullptr, nullptr};
     hipError_t create() {
-        for (auto& x : e) { hipError_t r = hipEventCreate(&x); if (r != hipSuccess) return r; }
+        for (auto& x : e) {
+            hipError_t r = hipEventCreate(&x);
+            if (r != hipSuccess) return r;
+        }
         return hipSuccess;
     }
-    ~HipEvents() { for (auto& x : e) if (x) (void)hipEventDestroy(x); }
+    ~HipEvents() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
 };
 
 }  // namespace
 
 // ---------------------------------------------------------------- timing helpers
 void ktime_begin(Ctx& c, int kclass, hipEvent_t* a) {
-    (void)kclass;
     *a = nullptr;
-    if (!c.timing) return;
+    if (!(c.timing_mask & (1u << kclass))) return;
     if (c.ev_next + 2 > c.ev_pool.size()) {
         for (int k = 0; k < 256; ++k) {
             hipEvent_t e;
@@ -409,13 +441,16 @@ void ktime_begin(Ctx& c, int kclass, hipEvent_t* a) {
     (void)hipEventRecord(*a, c.stream);
 }
 void ktime_end(Ctx& c, int kclass, double bytes, hipEvent_t a) {
-    if (!c.timing || !a) return;
+    if (!a) return;
     hipEvent_t b = c.ev_pool[c.ev_next++];
     (void)hipEventRecord(b, c.stream);
     c.pending.push_back({kclass, bytes, a, b, c.cur_step});
 }
 int ktime_flush(Ctx& c) {
-    if (c.pending.empty()) { c.ev_next = 0; return VFVM_OK; }
+    if (c.pending.empty()) {
+        c.ev_next = 0;
+        return VFVM_OK;
+    }
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     for (auto& p : c.pending) {
         float ms = 0.0f;
@@ -437,15 +472,14 @@ namespace {
 struct Work {
     int64_t n, nnz;
     int m;
-    int nblk;           // blocks of the reductions
+    int nblk;  // blocks of the reductions
     int64_t rows_per_blk;
     StateLayout L;
-    int* ctrl;          // device control word
-    int* h_ctrl;        // pinned
-    double* S;          // device state
+    int* ctrl;    // device control word
+    int* h_ctrl;  // pinned
+    double* S;    // device state
 };
 
-// LPR choice: average row length
 inline int pick_lpr(int64_t n, int64_t nnz) {
     double avg = n > 0 ? (double)nnz / (double)n : 1.0;
     return avg > 24.0 ? 16 : (avg > 10.0 ? 8 : 4);
@@ -475,94 +509,135 @@ int residual_launch(Ctx& c, int lpr, const double* x, double* r, double* part, i
     return VFVM_OK;
 }
 
-// ILU(0) level sets of the lower-triangular dependency graph (row i depends on col < i)
-int ilu_levels(Ctx& c) {
+// Multicolour ordering of the matrix pattern and the [L | diag | U] entry order per row.
+int ilu_setup(Ctx& c) {
     if (c.ilu_levels_ready) return VFVM_OK;
     const int64_t n = c.n;
-    std::vector<int32_t> lvl(n, 0), diag(n, -1);
-    int32_t maxl = 0;
+    const std::vector<int32_t>& rp = c.h_mrow;
+    const std::vector<int32_t>& cl = c.h_mcol;
+    // symmetrised adjacency (rows of one colour must not touch each other in A or A^T)
+    std::vector<int32_t> tcnt(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) tcnt[cl[k] + 1]++;
+    for (int64_t i = 0; i < n; ++i) tcnt[i + 1] += tcnt[i];
+    std::vector<int32_t> trows(tcnt[n]);
+    {
+        std::vector<int32_t> fill(tcnt.begin(), tcnt.end() - 1);
+        for (int64_t i = 0; i < n; ++i)
+            for (int32_t k = rp[i]; k < rp[i + 1]; ++k) trows[fill[cl[k]]++] = (int32_t)i;
+    }
+    // greedy colouring in natural order
+    std::vector<int32_t> color(n, -1);
+    std::vector<int64_t> mark(64, -1);
+    int32_t ncol = 0;
     for (int64_t i = 0; i < n; ++i) {
-        int32_t l = 0;
-        for (int32_t k = c.h_mrow[i]; k < c.h_mrow[i + 1]; ++k) {
-            int32_t j = c.h_mcol[k];
-            if (j < i) l = std::max(l, lvl[j] + 1);
-            if (j == i) diag[i] = k;
+        auto take = [&](int32_t j) {
+            if (j != i && color[j] >= 0) {
+                if (color[j] >= (int32_t)mark.size()) mark.resize(color[j] + 1, -1);
+                mark[color[j]] = i;
+            }
+        };
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) take(cl[k]);
+        for (int32_t k = tcnt[i]; k < tcnt[i + 1]; ++k) take(trows[k]);
+        int32_t cc = 0;
+        while (cc < (int32_t)mark.size() && mark[cc] == i) ++cc;
+        color[i] = cc;
+        ncol = std::max(ncol, cc + 1);
+    }
+    // rows sorted by (colour, index)
+    std::vector<int32_t> cptr(ncol + 1, 0), rows(n);
+    for (int64_t i = 0; i < n; ++i) cptr[color[i] + 1]++;
+    for (int32_t q = 0; q < ncol; ++q) cptr[q + 1] += cptr[q];
+    {
+        std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+        for (int64_t i = 0; i < n; ++i) rows[fill[color[i]]++] = (int32_t)i;
+    }
+    // per-row entry order [L | diag | U] by sigma
+    std::vector<int32_t> src(c.nnz), col(c.nnz), diag(n, -1);
+    std::vector<std::pair<int64_t, int32_t>> tmp;
+    for (int64_t i = 0; i < n; ++i) {
+        tmp.clear();
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) tmp.push_back({(int64_t)color[cl[k]] * n + cl[k], k});
+        std::sort(tmp.begin(), tmp.end());
+        const int64_t si = (int64_t)color[i] * n + i;
+        for (size_t q = 0; q < tmp.size(); ++q) {
+            const int32_t k = tmp[q].second;
+            src[rp[i] + q] = k;
+            col[rp[i] + q] = cl[k];
+            if (tmp[q].first == si) diag[i] = rp[i] + (int32_t)q;
         }
         if (diag[i] < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "ILU(0): a row has no diagonal entry");
-        lvl[i] = l;
-        maxl = std::max(maxl, l);
     }
-    // upper sweep runs levels in reverse of the transposed dependency; level of the upper
-    // graph (row i depends on col > i) computed separately
-    std::vector<int32_t> ulvl(n, 0);
-    int32_t maxu = 0;
-    for (int64_t i = n - 1; i >= 0; --i) {
-        int32_t l = 0;
-        for (int32_t k = diag[i] + 1; k < c.h_mrow[i + 1]; ++k) l = std::max(l, ulvl[c.h_mcol[k]] + 1);
-        ulvl[i] = l;
-        maxu = std::max(maxu, l);
-    }
-    // rows grouped: [lower levels 0..maxl][upper levels 0..maxu]
-    std::vector<int32_t> ptr(maxl + 2 + maxu + 1, 0), rows(2 * n);
-    for (int64_t i = 0; i < n; ++i) { ptr[lvl[i] + 1]++; ptr[maxl + 1 + ulvl[i] + 1]++; }
-    for (size_t k = 1; k < ptr.size(); ++k) ptr[k] += ptr[k - 1];
-    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
-    for (int64_t i = 0; i < n; ++i) { rows[fill[lvl[i]]++] = (int32_t)i; rows[fill[maxl + 1 + ulvl[i]]++] = (int32_t)i; }
-    c.h_lvl_ptr = ptr;
-    c.h_lvl_ptr.push_back(maxl + 1);  // number of lower levels, appended
-    VFVM_HIP_TRY(c.ilu_lvl_rows.upload(rows.data(), rows.size(), c.stream));
+    VFVM_HIP_TRY(c.ilu_rows.upload(rows.data(), rows.size(), c.stream));
+    VFVM_HIP_TRY(c.ilu_src.upload(src.data(), src.size(), c.stream));
+    VFVM_HIP_TRY(c.ilu_col.upload(col.data(), col.size(), c.stream));
     VFVM_HIP_TRY(c.ilu_diag.upload(diag.data(), diag.size(), c.stream));
+    VFVM_HIP_TRY(c.ilu_color.upload(color.data(), color.size(), c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    c.h_color_ptr = cptr;
     c.ilu_levels_ready = true;
     return VFVM_OK;
 }
 
+// lu <- ILU(0) of A in multicolour order, values gathered from the CSR values
 template <class T>
-int ilu_factor(Ctx& c, T* lu) {
-    const int nlow = c.h_lvl_ptr.back();
-    for (int l = 0; l < nlow; ++l) {
-        int32_t s = c.h_lvl_ptr[l], e = c.h_lvl_ptr[l + 1];
-        if (e > s) hipLaunchKernelGGL((k_ilu0_factor_level<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream,
-                                      c.ilu_lvl_rows.p + s, e - s, c.mrow.p, c.mcol.p, c.ilu_diag.p, lu);
+int ilu_factor(Ctx& c, T* lu, const double* val) {
+    hipLaunchKernelGGL((k_ilu_gather<T>), dim3(grid_for(c.nnz)), dim3(BLK), 0, c.stream, c.nnz, c.ilu_src.p, val, lu);
+    const int ncol = (int)c.h_color_ptr.size() - 1;
+    for (int q = 0; q < ncol; ++q) {
+        int32_t s = c.h_color_ptr[q], e = c.h_color_ptr[q + 1];
+        if (e > s)
+            hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, c.ilu_rows.p + s, e - s, c.n,
+                               c.mrow.p, c.ilu_col.p, c.ilu_diag.p, c.ilu_color.p, lu);
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
 }
 
+// z = (LU)^-1 rhs: forward sweep over colours ascending, backward sweep descending
 template <class T>
 int ilu_apply(Ctx& c, const T* lu, const T* rhs, T* z, const int* ctrl) {
-    const int nlow = c.h_lvl_ptr.back();
-    const int ntot = (int)c.h_lvl_ptr.size() - 2;  // number of level groups
-    hipEvent_t ev = nullptr;
-    ktime_begin(c, VFVM_K_PRECOND, &ev);
-    for (int l = 0; l < nlow; ++l) {
-        int32_t s = c.h_lvl_ptr[l], e = c.h_lvl_ptr[l + 1];
-        hipLaunchKernelGGL((k_ilu0_lower<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, c.ilu_lvl_rows.p + s, e - s,
-                           c.mrow.p, c.mcol.p, c.ilu_diag.p, lu, rhs, z, ctrl);
+    constexpr int LPR = 4;
+    const int ncol = (int)c.h_color_ptr.size() - 1;
+    // algorithmic bytes of one sweep launch over r rows: about half of their entries
+    // (value + column), rows list, diag, rp (3 ints), rhs/z reads and the z write
+    const double per_entry = 0.5 * (double)c.nnz / (double)c.n * (sizeof(T) + 4.0);
+    for (int q = 0; q < ncol; ++q) {
+        int32_t s = c.h_color_ptr[q], e = c.h_color_ptr[q + 1];
+        if (e == s) continue;
+        hipEvent_t ev = nullptr;
+        ktime_begin(c, VFVM_K_PRECOND, &ev);
+        hipLaunchKernelGGL((k_ilu_lower<T, LPR>), dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK), 0, c.stream,
+                           c.ilu_rows.p + s, e - s, c.mrow.p, c.ilu_col.p, c.ilu_diag.p, lu, rhs, z, ctrl);
+        ktime_end(c, VFVM_K_PRECOND, (double)(e - s) * (per_entry + 16.0 + 2.0 * sizeof(T)), ev);
     }
-    for (int l = nlow; l < ntot; ++l) {
-        int32_t s = c.h_lvl_ptr[l], e = c.h_lvl_ptr[l + 1];
-        hipLaunchKernelGGL((k_ilu0_upper<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, c.ilu_lvl_rows.p + s, e - s,
-                           c.mrow.p, c.mcol.p, c.ilu_diag.p, lu, z, ctrl);
+    for (int q = ncol - 1; q >= 0; --q) {
+        int32_t s = c.h_color_ptr[q], e = c.h_color_ptr[q + 1];
+        if (e == s) continue;
+        hipEvent_t ev = nullptr;
+        ktime_begin(c, VFVM_K_PRECOND, &ev);
+        hipLaunchKernelGGL((k_ilu_upper<T, LPR>), dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK), 0, c.stream,
+                           c.ilu_rows.p + s, e - s, c.mrow.p, c.ilu_col.p, c.ilu_diag.p, lu, z, ctrl);
+        ktime_end(c, VFVM_K_PRECOND, (double)(e - s) * (per_entry + 16.0 + 3.0 * sizeof(T)), ev);
     }
     VFVM_HIP_TRY(hipGetLastError());
-    ktime_end(c, VFVM_K_PRECOND, (double)c.nnz * (sizeof(T) + 4) + 4.0 * (c.n + 1) + 3.0 * sizeof(T) * c.n, ev);
     return VFVM_OK;
 }
 
-// One restart cycle of GMRES on the operator A M^-1 (M = D for Jacobi through the scaled
-// matrix, M = LU for ILU(0)) starting from V[0] (already normalised).  Leaves ctrl[1] = k
-// steps and y = H^-1 g in the state.
+// One restart cycle of GMRES on A M^-1 (M = D through the scaled matrix for Jacobi,
+// M = LU for ILU(0)) from V[0] (normalised).  Leaves ctrl[1] = k steps, y = H^-1 g.
 template <class T>
-int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, const T* aval, const T* lu, int check_every, int* steps_out) {
+int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, const T* aval, const T* lu, bool reorth, int check_every,
+                  int* steps_out) {
     const int64_t n = w.n;
     const int lpr = pick_lpr(n, w.nnz);
     int steps = 0;
     bool synced_last = false;
+    double* part = c.part.p;
+    double* part_nrm = part + (int64_t)(MAX_RESTART + 1) * w.nblk;
     for (int j = 0; j < w.m; ++j) {
         c.cur_step = j;
         T* vj = V + (int64_t)j * n;
-        // w = A M^-1 v_j
         const T* xin = vj;
         if (lu) {
             if (int st = ilu_apply<T>(c, lu, vj, zv, w.ctrl)) return st;
@@ -572,23 +647,24 @@ int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, const T* aval, const T* l
         ktime_begin(c, VFVM_K_SPMV, &ev);
         if (int st = spmv_launch<T>(c, lpr, n, aval, xin, wv, w.ctrl)) return st;
         ktime_end(c, VFVM_K_SPMV, (double)w.nnz * (sizeof(T) + 4) + 4.0 * (n + 1) + 2.0 * sizeof(T) * n, ev);
-        // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2, ||w||
         const int nv = j + 1;
-        double* part = c.part.p;
-        double* part_nrm = part + (int64_t)(MAX_RESTART + 1) * w.nblk;
-        ktime_begin(c, VFVM_K_ORTHO, &ev);
-        hipLaunchKernelGGL((k_mdot<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, wv, part, w.nblk, w.rows_per_blk, w.ctrl);
-        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, w.nblk, part, w.S + w.L.h1(), 0, w.ctrl);
-        hipLaunchKernelGGL((k_update<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, w.S + w.L.h1(), wv, (double*)nullptr, w.rows_per_blk, w.ctrl);
-        hipLaunchKernelGGL((k_mdot<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, wv, part, w.nblk, w.rows_per_blk, w.ctrl);
-        hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, w.nblk, part, w.S + w.L.h2(), 0, w.ctrl);
-        hipLaunchKernelGGL((k_update<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, w.S + w.L.h2(), wv, part_nrm, w.rows_per_blk, w.ctrl);
-        hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(BLK), 0, c.stream, j, w.nblk, part_nrm, w.S, w.L, w.ctrl);
+        const double bytes_dot = (double)sizeof(T) * n * (nv + 1);
+        const double bytes_upd = (double)sizeof(T) * n * (nv + 2);
+        for (int pass = 0; pass < (reorth ? 2 : 1); ++pass) {
+            double* hout = w.S + (pass == 0 ? w.L.h1() : w.L.h2());
+            const bool last = pass == (reorth ? 1 : 0);
+            ktime_begin(c, VFVM_K_MDOT, &ev);
+            hipLaunchKernelGGL((k_mdot<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, wv, part, w.nblk, w.rows_per_blk, w.ctrl);
+            ktime_end(c, VFVM_K_MDOT, bytes_dot, ev);
+            hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, w.nblk, part, hout, w.ctrl);
+            ktime_begin(c, VFVM_K_UPDATE, &ev);
+            hipLaunchKernelGGL((k_update<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, hout, wv,
+                               last ? part_nrm : (double*)nullptr, w.rows_per_blk, w.ctrl);
+            ktime_end(c, VFVM_K_UPDATE, bytes_upd, ev);
+        }
+        hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(BLK), 0, c.stream, j, reorth ? 1 : 0, w.nblk, part_nrm, w.S, w.L, w.ctrl);
         hipLaunchKernelGGL((k_scale<T, T>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, wv, V + (int64_t)(j + 1) * n, w.S, w.L.misc() + 1, w.ctrl);
         VFVM_HIP_TRY(hipGetLastError());
-        // algorithmic bytes of CGS2 at step j: two multi-dots reading (j+2) vectors, two updates
-        // reading (j+2) and writing one, the scaling reading and writing one
-        ktime_end(c, VFVM_K_ORTHO, (double)sizeof(T) * n * (2.0 * (nv + 1) + 2.0 * (nv + 2) + 2.0), ev);
         c.cur_step = -1;
         synced_last = false;
         if ((j + 1) % check_every == 0 || j + 1 == w.m) {
@@ -596,9 +672,9 @@ int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, const T* aval, const T* l
             VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
             synced_last = true;
             steps = w.h_ctrl[1];
-            if (c.timing) {
+            if (!c.pending.empty()) {
                 for (auto& p : c.pending)
-                    if (p.step >= steps) p.bytes = -1.0;  // launches after convergence did no work
+                    if (p.step >= steps) p.bytes = -1.0;  // launches after convergence returned at once
                 if (int st = ktime_flush(c)) return st;
             }
             if (w.h_ctrl[0] == 0) break;
@@ -642,11 +718,15 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     if (o.max_outer <= 0) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_MAX_OUTER, "GMRES: max_outer must be positive");
     if (o.precond < VFVM_PRECOND_NONE || o.precond > VFVM_PRECOND_ILU0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown preconditioner");
     if (o.precision != VFVM_FP64 && o.precision != VFVM_FP32_REFINED) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown precision");
+    if (o.ortho < VFVM_ORTHO_AUTO || o.ortho > VFVM_ORTHO_CGS2) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown orthogonalisation");
     const bool mixed = o.precision == VFVM_FP32_REFINED;
+    const bool reorth = o.ortho == VFVM_ORTHO_CGS2 || (o.ortho == VFVM_ORTHO_AUTO && !mixed);
     const int64_t n = c.n;
     const int m = o.restart;
     Work w;
-    w.n = n; w.nnz = c.nnz; w.m = m;
+    w.n = n;
+    w.nnz = c.nnz;
+    w.m = m;
     w.nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 4 * BLK - 1) / (4 * BLK)));
     w.rows_per_blk = (n + w.nblk - 1) / w.nblk;
     w.L.m = m;
@@ -666,11 +746,13 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     VFVM_HIP_TRY(c.z64.alloc(n));
     const bool ilu = o.precond == VFVM_PRECOND_ILU0;
     const bool jac = o.precond == VFVM_PRECOND_JACOBI;
-    if (ilu) {
-        if (int s = ilu_levels(c)) return s;
-    }
+    if (ilu)
+        if (int s = ilu_setup(c)) return s;
     if (!mixed) VFVM_HIP_TRY(c.V64.alloc((size_t)n * (m + 1)));
-    else { VFVM_HIP_TRY(c.V32.alloc((size_t)n * (m + 1))); VFVM_HIP_TRY(c.w32.alloc(2 * (size_t)n)); }
+    else {
+        VFVM_HIP_TRY(c.V32.alloc((size_t)n * (m + 1)));
+        VFVM_HIP_TRY(c.w32.alloc(2 * (size_t)n));
+    }
 
     HipEvents ev;
     VFVM_HIP_TRY(ev.create());
@@ -691,8 +773,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
         }
         if (ilu) {
             VFVM_HIP_TRY(c.ilu64.alloc(c.nnz));
-            VFVM_HIP_TRY(hipMemcpyAsync(c.ilu64.p, c.vals.p, c.nnz * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-            if (int s = ilu_factor<double>(c, c.ilu64.p)) return s;
+            if (int s = ilu_factor<double>(c, c.ilu64.p, c.vals.p)) return s;
             lu = c.ilu64.p;
         }
     } else {
@@ -702,8 +783,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
         aval = c.avals32.p;
         if (ilu) {
             VFVM_HIP_TRY(c.ilu32.alloc(c.nnz));
-            VFVM_HIP_TRY(hipMemcpyAsync(c.ilu32.p, c.avals32.p, c.nnz * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
-            if (int s = ilu_factor<float>(c, c.ilu32.p)) return s;
+            if (int s = ilu_factor<float>(c, c.ilu32.p, c.vals.p)) return s;
             lu = c.ilu32.p;
         }
     }
@@ -735,20 +815,20 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
         int steps = 0;
         if (!mixed) {
             hipLaunchKernelGGL((k_scale<double, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V64.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<double>(c, w, c.V64.p, c.w64.p, c.z64.p, (const double*)aval, (const double*)lu, check_every, &steps)) return s;
+            if (int s = arnoldi_cycle<double>(c, w, c.V64.p, c.w64.p, c.z64.p, (const double*)aval, (const double*)lu, reorth, check_every, &steps)) return s;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y)
             hipLaunchKernelGGL((k_combine<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V64.p, n, w.S, w.L, w.ctrl, c.w64.p);
             if (lu) {
                 if (int s = ilu_apply<double>(c, (const double*)lu, c.w64.p, c.z64.p, nullptr)) return s;
-                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, 1.0, x);
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
             } else {
-                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.w64.p, dinv_x, 1.0, x);
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.w64.p, dinv_x, x);
             }
         } else {
             // inner FP32 solve of A M^-1 u = r from u = 0, V0 = r / beta in FP32
             hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V32.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<float>(c, w, c.V32.p, c.w32.p, c.w32.p + n, (const float*)aval, (const float*)lu, check_every, &steps)) return s;
+            if (int s = arnoldi_cycle<float>(c, w, c.V32.p, c.w32.p, c.w32.p + n, (const float*)aval, (const float*)lu, reorth, check_every, &steps)) return s;
             ++refinements;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y), V y accumulated in FP64
@@ -759,9 +839,9 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
                 hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, t32, w.S, w.L.misc() + 4, (const int*)nullptr);
                 if (int s = ilu_apply<float>(c, (const float*)lu, t32, z32, nullptr)) return s;
                 hipLaunchKernelGGL((k_scale<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z32, c.z64.p, w.S, w.L.misc() + 4, (const int*)nullptr);
-                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, 1.0, x);
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
             } else {
-                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, dinv_x, 1.0, x);
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, dinv_x, x);
             }
         }
         VFVM_HIP_TRY(hipGetLastError());
@@ -773,7 +853,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     float ms_all = 0.f, ms_setup = 0.f;
     VFVM_HIP_TRY(hipEventElapsedTime(&ms_all, ev.e[0], ev.e[2]));
     VFVM_HIP_TRY(hipEventElapsedTime(&ms_setup, ev.e[0], ev.e[1]));
-    if (c.timing) if (int s = ktime_flush(c)) return s;
+    if (int s = ktime_flush(c)) return s;
     st.iterations = total_steps;
     st.restarts = cycles;
     st.refinements = refinements;

@@ -61,7 +61,7 @@ class OpStruct(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("restart", C.c_int32), ("max_outer", C.c_int32), ("precision", C.c_int32),
-                ("precond", C.c_int32), ("inner_tol", C.c_double), ("max_refine", C.c_int32), ("reserved", C.c_int32)]
+                ("precond", C.c_int32), ("inner_tol", C.c_double), ("max_refine", C.c_int32), ("ortho", C.c_int32)]
 
 
 class SolveStats(C.Structure):

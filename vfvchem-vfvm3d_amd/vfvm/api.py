@@ -170,7 +170,7 @@ class SteadyBase:
         if _depends_on_unknowns(vars):
             raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "function fields of unknowns need the host Picard loop (not on the built path)")
         vars.update_func(0.0)
-        self.stats = vars.ctx.steady_solve(self.operators(vars), max_iter, tol, damp, **lin.opts)
+        self.stats = vars.ctx.steady_solve(self.operators(vars), max_iter, tol, damp, lin=lin.opts)
         return self.stats
 
 
@@ -187,7 +187,7 @@ class TransientBase:
         if _depends_on_unknowns(vars) or any(k == FIELD_FUNCTION for k, _, _ in vars.scldom):
             # time-dependent closures: step from the host
             raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "time-dependent function fields need the host time loop (not on the built path)")
-        self.stats = vars.ctx.transient_solve(self.operators(vars), dt, num_ts, max_iter, tol, damp, **lin.opts)
+        self.stats = vars.ctx.transient_solve(self.operators(vars), dt, num_ts, max_iter, tol, damp, lin=lin.opts)
         return self.stats
 
 

@@ -25,7 +25,8 @@ OP_SCL_BND_ADVECTION_UNITY = 26
 FIELD_CONSTANT, FIELD_FUNCTION, FIELD_UNKNOWN = 0, 1, 2
 FP64, FP32_REFINED = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_ILU0 = 0, 1, 2
-K_SPMV, K_ORTHO, K_ASSEMBLE, K_PRECOND = 0, 1, 2, 3
+K_SPMV, K_MDOT, K_ASSEMBLE, K_PRECOND, K_UPDATE = 0, 1, 2, 3, 4
+ORTHO_AUTO, ORTHO_CGS, ORTHO_CGS2 = 0, 1, 2
 
 _MESH_ARRAYS = {
     "pos": (0, np.float64), "tets": (1, np.int32), "bnd_ptr": (2, np.int64), "bnd_nid": (3, np.int32),
@@ -117,11 +118,12 @@ def make_ops(ops):
     return arr
 
 
-def solve_opts(tol=1e-6, restart=50, max_outer=200, precision=FP64, precond=PRECOND_JACOBI, inner_tol=1e-5, max_refine=0):
+def solve_opts(tol=1e-6, restart=50, max_outer=200, precision=FP64, precond=PRECOND_JACOBI, inner_tol=1e-5, max_refine=0,
+               ortho=0):
     o = L.SolveOpts()
     lib.vfvm_solve_opts_default(C.byref(o))
-    o.tol, o.restart, o.max_outer, o.precision, o.precond, o.inner_tol, o.max_refine = (
-        float(tol), int(restart), int(max_outer), int(precision), int(precond), float(inner_tol), int(max_refine))
+    o.tol, o.restart, o.max_outer, o.precision, o.precond, o.inner_tol, o.max_refine, o.ortho = (
+        float(tol), int(restart), int(max_outer), int(precision), int(precond), float(inner_tol), int(max_refine), int(ortho))
     return o
 
 
@@ -237,24 +239,27 @@ class Context:
         check(lib.vfvm_residual(self._h, ptr(x), C.byref(r)))
         return r.value
 
-    def steady_solve(self, ops, max_iter, tol, damp, **kw):
+    def steady_solve(self, ops, max_iter, tol, damp, lin=None):
+        """SteadyBase::solve; lin: dict of linear-solver options (solve_opts keywords)."""
         arr = make_ops(ops)
         st = L.DriverStats()
-        o = solve_opts(**kw)
+        o = solve_opts(**(lin or {}))
         check(lib.vfvm_steady_solve(self._h, arr, len(ops), C.byref(o), int(max_iter), float(tol), float(damp), C.byref(st)))
         return st
 
-    def transient_solve(self, ops, dt, num_ts, max_iter, tol, damp, **kw):
+    def transient_solve(self, ops, dt, num_ts, max_iter, tol, damp, lin=None):
+        """TransientBase::solve; lin: dict of linear-solver options (solve_opts keywords)."""
         arr = make_ops(ops)
         st = L.DriverStats()
-        o = solve_opts(**kw)
+        o = solve_opts(**(lin or {}))
         check(lib.vfvm_transient_solve(self._h, arr, len(ops), C.byref(o), float(dt), int(num_ts), int(max_iter), float(tol),
                                        float(damp), C.byref(st)))
         return st
 
     # timing
-    def timing_enable(self, on=True):
-        check(lib.vfvm_timing_enable(self._h, 1 if on else 0))
+    def timing_enable(self, mask=-1):
+        """Bit k of mask times kernel class k (K_SPMV, K_MDOT, ...); -1: all, 0: off."""
+        check(lib.vfvm_timing_enable(self._h, int(mask)))
 
     def timing_reset(self):
         check(lib.vfvm_timing_reset(self._h))
