@@ -128,10 +128,11 @@ struct Ctx {
     // solver workspace
     DevBuf<double> V64, w64, z64, x64, r64, dinv64, avals64, part, scal;
     DevBuf<float> V32, w32, avals32;
-    // multicolour ILU(0): rows grouped by colour, row entries reordered [L | diag | U] by (colour, index)
-    DevBuf<int32_t> ilu_rows, ilu_src, ilu_col, ilu_diag, ilu_color;
-    DevBuf<double> ilu64;
-    DevBuf<float> ilu32;
+    // multicolour ILU(0) in colour-major storage: row t of the permuted system is row perm[t]
+    // of A, rows sorted by (colour, index); entries [L | diag | U] with permuted columns
+    DevBuf<int32_t> ilu_perm, ilu_prp, ilu_src, ilu_pcol, ilu_pdiag;
+    DevBuf<double> ilu64, zp64;
+    DevBuf<float> ilu32, zp32;
     std::vector<int32_t> h_color_ptr;
     bool ilu_levels_ready = false;
     double* h_pinned = nullptr;  // small pinned readback buffer

@@ -15,6 +15,7 @@
 // Hessenberg matrix, Givens rotations and the control word live in HBM; the
 // host reads the control word every few steps.  Reductions are two-stage and
 // fixed-order, so results are bitwise reproducible run to run.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -331,9 +332,10 @@ __global__ void __launch_bounds__(BLK) k_scale_cols(int64_t nnz, const int32_t* 
 }
 
 // ---------------------------------------------------------------- multicolour ILU(0)
-// Ordering sigma(j) = (colour(j), j); no two rows of one colour share an entry, so all
-// rows of a colour are independent in the factorisation and in both sweeps.  Row
-// entries are stored [L (sigma-ascending) | diag | U (sigma-ascending)].
+// Rows are ordered by sigma(i) = (colour(i), i) and stored colour-major: permuted row t
+// is row perm[t] of A, its entries are [L | diag | U] with permuted column indices in
+// ascending order.  No two rows of one colour share an entry, so the rows of a colour
+// are independent in the factorisation and in both triangular sweeps.
 template <class T>
 __global__ void __launch_bounds__(BLK) k_ilu_gather(int64_t nnz, const int32_t* __restrict__ src, const double* __restrict__ val,
                                                     T* __restrict__ lu) {
@@ -342,69 +344,66 @@ __global__ void __launch_bounds__(BLK) k_ilu_gather(int64_t nnz, const int32_t* 
     lu[k] = (T)val[src[k]];
 }
 
-__device__ inline int64_t sigma_key(const int32_t* __restrict__ color, int64_t n, int32_t j) {
-    return (int64_t)color[j] * n + j;
-}
-
-// IKJ ILU(0) (Saad Alg. 10.4) for the rows of one colour
+// IKJ ILU(0) (Saad Alg. 10.4) for the permuted rows [t0, t1) of one colour
 template <class T>
-__global__ void __launch_bounds__(BLK) k_ilu_factor(const int32_t* __restrict__ rows, int32_t nrows, int64_t n,
-                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                    const int32_t* __restrict__ diag, const int32_t* __restrict__ color,
+__global__ void __launch_bounds__(BLK) k_ilu_factor(int32_t t0, int32_t t1, const int32_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
                                                     T* __restrict__ lu) {
-    const int32_t t = blockIdx.x * BLK + threadIdx.x;
-    if (t >= nrows) return;
-    const int32_t i = rows[t];
-    const int32_t e = rp[i + 1], di = diag[i];
-    for (int32_t kk = rp[i]; kk < di; ++kk) {
+    const int32_t t = t0 + blockIdx.x * BLK + threadIdx.x;
+    if (t >= t1) return;
+    const int32_t e = rp[t + 1], dt = diag[t];
+    for (int32_t kk = rp[t]; kk < dt; ++kk) {
         const int32_t k = col[kk];
         const T lik = lu[kk] / lu[diag[k]];
         lu[kk] = lik;
-        // a_ij -= l_ik u_kj for j right of k in sigma order, when (i, j) is in the pattern
+        // a_tj -= l_tk u_kj for j > k, when (t, j) is in the pattern
         int32_t p = kk + 1;
         for (int32_t q = diag[k] + 1; q < rp[k + 1]; ++q) {
             const int32_t j = col[q];
-            const int64_t kj = sigma_key(color, n, j);
-            while (p < e && sigma_key(color, n, col[p]) < kj) ++p;
+            while (p < e && col[p] < j) ++p;
             if (p < e && col[p] == j) lu[p] -= lik * lu[q];
         }
     }
 }
 
-// forward sweep (L has a unit diagonal): z_i = r_i - sum_{L} l_ij z_j
+// forward sweep over the rows [t0, t1) of one colour (L has a unit diagonal):
+// zp_t = rhs[perm[t]] - sum_L l_tj zp_j
 template <class T, int LPR>
-__global__ void __launch_bounds__(BLK) k_ilu_lower(const int32_t* __restrict__ rows, int32_t nrows,
+__global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                    const int32_t* __restrict__ diag, const T* __restrict__ lu,
-                                                   const T* __restrict__ rhs, T* __restrict__ z, const int* __restrict__ ctrl) {
+                                                   const T* __restrict__ rhs, T* __restrict__ zp, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
     const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    const int64_t t = tid / LPR;
+    const int32_t t = t0 + (int32_t)(tid / LPR);
     const int lane = (int)(tid % LPR);
-    if (t >= nrows) return;
-    const int32_t i = rows[t];
+    if (t >= t1) return;
     T s = 0;
-    for (int32_t k = rp[i] + lane; k < diag[i]; k += LPR) s += lu[k] * z[col[k]];
+    const int32_t d = diag[t];
+    for (int32_t k = rp[t] + lane; k < d; k += LPR) s += lu[k] * zp[col[k]];
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-    if (lane == 0) z[i] = rhs[i] - s;
+    if (lane == 0) zp[t] = rhs[perm[t]] - s;
 }
-// backward sweep: z_i = (z_i - sum_{U} u_ij z_j) / u_ii
+// backward sweep: zp_t = (zp_t - sum_U u_tj zp_j) / u_tt, z[perm[t]] = zp_t
 template <class T, int LPR>
-__global__ void __launch_bounds__(BLK) k_ilu_upper(const int32_t* __restrict__ rows, int32_t nrows,
+__global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                    const int32_t* __restrict__ diag, const T* __restrict__ lu,
-                                                   T* __restrict__ z, const int* __restrict__ ctrl) {
+                                                   T* __restrict__ zp, T* __restrict__ z, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
     const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    const int64_t t = tid / LPR;
+    const int32_t t = t0 + (int32_t)(tid / LPR);
     const int lane = (int)(tid % LPR);
-    if (t >= nrows) return;
-    const int32_t i = rows[t];
+    if (t >= t1) return;
     T s = 0;
-    const int32_t e = rp[i + 1];
-    for (int32_t k = diag[i] + 1 + lane; k < e; k += LPR) s += lu[k] * z[col[k]];
+    const int32_t d = diag[t], e = rp[t + 1];
+    for (int32_t k = d + 1 + lane; k < e; k += LPR) s += lu[k] * zp[col[k]];
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-    if (lane == 0) z[i] = (z[i] - s) / lu[diag[i]];
+    if (lane == 0) {
+        const T v = (zp[t] - s) / lu[d];
+        zp[t] = v;
+        z[perm[t]] = v;
+    }
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
@@ -469,6 +468,28 @@ int ktime_flush(Ctx& c) {
 // ---------------------------------------------------------------- solver
 namespace {
 
+// Launches `kernel`; when its class is timed, the launch carries start/stop events
+// (hipExtLaunchKernelGGL records them in the dispatch itself, no extra queue commands).
+template <typename K, typename... Args>
+void launch_timed(Ctx& c, int kclass, double bytes, K kernel, dim3 grid, dim3 block, Args... args) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c.timing_mask & (1u << kclass)) {
+        if (c.ev_next + 2 > c.ev_pool.size()) {
+            for (int k = 0; k < 256; ++k) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) break;
+                c.ev_pool.push_back(e);
+            }
+        }
+        if (c.ev_next + 2 <= c.ev_pool.size()) {
+            a = c.ev_pool[c.ev_next++];
+            b = c.ev_pool[c.ev_next++];
+        }
+    }
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, c.stream, a, b, 0, args...);
+    if (a) c.pending.push_back({kclass, bytes, a, b, c.cur_step});
+}
+
 struct Work {
     int64_t n, nnz;
     int m;
@@ -486,12 +507,14 @@ inline int pick_lpr(int64_t n, int64_t nnz) {
 }
 
 template <class T>
-int spmv_launch(Ctx& c, int lpr, int64_t n, const T* val, const T* x, T* y, const int* ctrl) {
+int spmv_launch(Ctx& c, int lpr, int64_t n, const T* val, const T* x, T* y, const int* ctrl, double bytes) {
     unsigned grid = grid_for(n * lpr);
+    const int32_t* rp = c.mrow.p;
+    const int32_t* cl = c.mcol.p;
     switch (lpr) {
-        case 4: hipLaunchKernelGGL((k_spmv<T, 4>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, val, x, y, ctrl); break;
-        case 8: hipLaunchKernelGGL((k_spmv<T, 8>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, val, x, y, ctrl); break;
-        default: hipLaunchKernelGGL((k_spmv<T, 16>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, val, x, y, ctrl); break;
+        case 4: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 4>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, ctrl); break;
+        case 8: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 8>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, ctrl); break;
+        default: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 16>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, ctrl); break;
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
@@ -509,7 +532,7 @@ int residual_launch(Ctx& c, int lpr, const double* x, double* r, double* part, i
     return VFVM_OK;
 }
 
-// Multicolour ordering of the matrix pattern and the [L | diag | U] entry order per row.
+// Multicolour ordering of the matrix pattern and the colour-major [L | diag | U] storage.
 int ilu_setup(Ctx& c) {
     if (c.ilu_levels_ready) return VFVM_OK;
     const int64_t n = c.n;
@@ -544,35 +567,40 @@ int ilu_setup(Ctx& c) {
         color[i] = cc;
         ncol = std::max(ncol, cc + 1);
     }
-    // rows sorted by (colour, index)
-    std::vector<int32_t> cptr(ncol + 1, 0), rows(n);
+    // permutation: rows sorted by (colour, index)
+    std::vector<int32_t> cptr(ncol + 1, 0), perm(n), iperm(n);
     for (int64_t i = 0; i < n; ++i) cptr[color[i] + 1]++;
     for (int32_t q = 0; q < ncol; ++q) cptr[q + 1] += cptr[q];
     {
         std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
-        for (int64_t i = 0; i < n; ++i) rows[fill[color[i]]++] = (int32_t)i;
-    }
-    // per-row entry order [L | diag | U] by sigma
-    std::vector<int32_t> src(c.nnz), col(c.nnz), diag(n, -1);
-    std::vector<std::pair<int64_t, int32_t>> tmp;
-    for (int64_t i = 0; i < n; ++i) {
-        tmp.clear();
-        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) tmp.push_back({(int64_t)color[cl[k]] * n + cl[k], k});
-        std::sort(tmp.begin(), tmp.end());
-        const int64_t si = (int64_t)color[i] * n + i;
-        for (size_t q = 0; q < tmp.size(); ++q) {
-            const int32_t k = tmp[q].second;
-            src[rp[i] + q] = k;
-            col[rp[i] + q] = cl[k];
-            if (tmp[q].first == si) diag[i] = rp[i] + (int32_t)q;
+        for (int64_t i = 0; i < n; ++i) {
+            const int32_t t = fill[color[i]]++;
+            perm[t] = (int32_t)i;
+            iperm[i] = t;
         }
-        if (diag[i] < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "ILU(0): a row has no diagonal entry");
     }
-    VFVM_HIP_TRY(c.ilu_rows.upload(rows.data(), rows.size(), c.stream));
+    // colour-major CSR, columns permuted and ascending
+    std::vector<int32_t> prp(n + 1, 0), src(c.nnz), pcol(c.nnz), pdiag(n, -1);
+    std::vector<std::pair<int32_t, int32_t>> tmp;
+    for (int64_t t = 0; t < n; ++t) {
+        const int32_t i = perm[t];
+        tmp.clear();
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) tmp.push_back({iperm[cl[k]], k});
+        std::sort(tmp.begin(), tmp.end());
+        const int32_t base = prp[t];
+        for (size_t q = 0; q < tmp.size(); ++q) {
+            pcol[base + q] = tmp[q].first;
+            src[base + q] = tmp[q].second;
+            if (tmp[q].first == t) pdiag[t] = base + (int32_t)q;
+        }
+        prp[t + 1] = base + (int32_t)tmp.size();
+        if (pdiag[t] < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "ILU(0): a row has no diagonal entry");
+    }
+    VFVM_HIP_TRY(c.ilu_perm.upload(perm.data(), perm.size(), c.stream));
+    VFVM_HIP_TRY(c.ilu_prp.upload(prp.data(), prp.size(), c.stream));
     VFVM_HIP_TRY(c.ilu_src.upload(src.data(), src.size(), c.stream));
-    VFVM_HIP_TRY(c.ilu_col.upload(col.data(), col.size(), c.stream));
-    VFVM_HIP_TRY(c.ilu_diag.upload(diag.data(), diag.size(), c.stream));
-    VFVM_HIP_TRY(c.ilu_color.upload(color.data(), color.size(), c.stream));
+    VFVM_HIP_TRY(c.ilu_pcol.upload(pcol.data(), pcol.size(), c.stream));
+    VFVM_HIP_TRY(c.ilu_pdiag.upload(pdiag.data(), pdiag.size(), c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     c.h_color_ptr = cptr;
     c.ilu_levels_ready = true;
@@ -587,38 +615,36 @@ int ilu_factor(Ctx& c, T* lu, const double* val) {
     for (int q = 0; q < ncol; ++q) {
         int32_t s = c.h_color_ptr[q], e = c.h_color_ptr[q + 1];
         if (e > s)
-            hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, c.ilu_rows.p + s, e - s, c.n,
-                               c.mrow.p, c.ilu_col.p, c.ilu_diag.p, c.ilu_color.p, lu);
+            hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, s, e, c.ilu_prp.p,
+                               c.ilu_pcol.p, c.ilu_pdiag.p, lu);
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
 }
 
-// z = (LU)^-1 rhs: forward sweep over colours ascending, backward sweep descending
+// z = (LU)^-1 rhs: forward sweep over colours ascending, backward sweep descending, in the
+// permuted work vector zp; rhs is gathered by the forward sweep and z scattered by the
+// backward sweep
 template <class T>
-int ilu_apply(Ctx& c, const T* lu, const T* rhs, T* z, const int* ctrl) {
+int ilu_apply(Ctx& c, const T* lu, const T* rhs, T* z, T* zp, const int* ctrl) {
     constexpr int LPR = 4;
     const int ncol = (int)c.h_color_ptr.size() - 1;
-    // algorithmic bytes of one sweep launch over r rows: about half of their entries
-    // (value + column), rows list, diag, rp (3 ints), rhs/z reads and the z write
-    const double per_entry = 0.5 * (double)c.nnz / (double)c.n * (sizeof(T) + 4.0);
+    // algorithmic bytes of one sweep launch over r rows: the triangle's entries (value +
+    // column, about half of each row), rp/diag/perm (12 B), rhs or z reads and writes
+    const double per_row_entries = 0.5 * (double)c.nnz / (double)c.n * (sizeof(T) + 4.0);
     for (int q = 0; q < ncol; ++q) {
         int32_t s = c.h_color_ptr[q], e = c.h_color_ptr[q + 1];
         if (e == s) continue;
-        hipEvent_t ev = nullptr;
-        ktime_begin(c, VFVM_K_PRECOND, &ev);
-        hipLaunchKernelGGL((k_ilu_lower<T, LPR>), dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK), 0, c.stream,
-                           c.ilu_rows.p + s, e - s, c.mrow.p, c.ilu_col.p, c.ilu_diag.p, lu, rhs, z, ctrl);
-        ktime_end(c, VFVM_K_PRECOND, (double)(e - s) * (per_entry + 16.0 + 2.0 * sizeof(T)), ev);
+        launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 2.0 * sizeof(T)),
+                     k_ilu_lower<T, LPR>, dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK),
+                     s, e, c.ilu_perm.p, c.ilu_prp.p, c.ilu_pcol.p, c.ilu_pdiag.p, lu, rhs, zp, ctrl);
     }
     for (int q = ncol - 1; q >= 0; --q) {
         int32_t s = c.h_color_ptr[q], e = c.h_color_ptr[q + 1];
         if (e == s) continue;
-        hipEvent_t ev = nullptr;
-        ktime_begin(c, VFVM_K_PRECOND, &ev);
-        hipLaunchKernelGGL((k_ilu_upper<T, LPR>), dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK), 0, c.stream,
-                           c.ilu_rows.p + s, e - s, c.mrow.p, c.ilu_col.p, c.ilu_diag.p, lu, z, ctrl);
-        ktime_end(c, VFVM_K_PRECOND, (double)(e - s) * (per_entry + 16.0 + 3.0 * sizeof(T)), ev);
+        launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 3.0 * sizeof(T)),
+                     k_ilu_upper<T, LPR>, dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK),
+                     s, e, c.ilu_perm.p, c.ilu_prp.p, c.ilu_pcol.p, c.ilu_pdiag.p, lu, zp, z, ctrl);
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
@@ -627,7 +653,7 @@ int ilu_apply(Ctx& c, const T* lu, const T* rhs, T* z, const int* ctrl) {
 // One restart cycle of GMRES on A M^-1 (M = D through the scaled matrix for Jacobi,
 // M = LU for ILU(0)) from V[0] (normalised).  Leaves ctrl[1] = k steps, y = H^-1 g.
 template <class T>
-int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, const T* aval, const T* lu, bool reorth, int check_every,
+int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, T* zp, const T* aval, const T* lu, bool reorth, int check_every,
                   int* steps_out) {
     const int64_t n = w.n;
     const int lpr = pick_lpr(n, w.nnz);
@@ -635,32 +661,27 @@ int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, const T* aval, const T* l
     bool synced_last = false;
     double* part = c.part.p;
     double* part_nrm = part + (int64_t)(MAX_RESTART + 1) * w.nblk;
+    const double bytes_spmv = (double)w.nnz * (sizeof(T) + 4) + 4.0 * (n + 1) + 2.0 * sizeof(T) * n;
     for (int j = 0; j < w.m; ++j) {
         c.cur_step = j;
         T* vj = V + (int64_t)j * n;
         const T* xin = vj;
         if (lu) {
-            if (int st = ilu_apply<T>(c, lu, vj, zv, w.ctrl)) return st;
+            if (int st = ilu_apply<T>(c, lu, vj, zv, zp, w.ctrl)) return st;
             xin = zv;
         }
-        hipEvent_t ev = nullptr;
-        ktime_begin(c, VFVM_K_SPMV, &ev);
-        if (int st = spmv_launch<T>(c, lpr, n, aval, xin, wv, w.ctrl)) return st;
-        ktime_end(c, VFVM_K_SPMV, (double)w.nnz * (sizeof(T) + 4) + 4.0 * (n + 1) + 2.0 * sizeof(T) * n, ev);
+        if (int st = spmv_launch<T>(c, lpr, n, aval, xin, wv, w.ctrl, bytes_spmv)) return st;
         const int nv = j + 1;
         const double bytes_dot = (double)sizeof(T) * n * (nv + 1);
         const double bytes_upd = (double)sizeof(T) * n * (nv + 2);
         for (int pass = 0; pass < (reorth ? 2 : 1); ++pass) {
             double* hout = w.S + (pass == 0 ? w.L.h1() : w.L.h2());
             const bool last = pass == (reorth ? 1 : 0);
-            ktime_begin(c, VFVM_K_MDOT, &ev);
-            hipLaunchKernelGGL((k_mdot<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, wv, part, w.nblk, w.rows_per_blk, w.ctrl);
-            ktime_end(c, VFVM_K_MDOT, bytes_dot, ev);
+            launch_timed(c, VFVM_K_MDOT, bytes_dot, k_mdot<T>, dim3(w.nblk), dim3(BLK), n, nv, (const T*)V, n, (const T*)wv,
+                         part, w.nblk, w.rows_per_blk, (const int*)w.ctrl);
             hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, w.nblk, part, hout, w.ctrl);
-            ktime_begin(c, VFVM_K_UPDATE, &ev);
-            hipLaunchKernelGGL((k_update<T>), dim3(w.nblk), dim3(BLK), 0, c.stream, n, nv, V, n, hout, wv,
-                               last ? part_nrm : (double*)nullptr, w.rows_per_blk, w.ctrl);
-            ktime_end(c, VFVM_K_UPDATE, bytes_upd, ev);
+            launch_timed(c, VFVM_K_UPDATE, bytes_upd, k_update<T>, dim3(w.nblk), dim3(BLK), n, nv, (const T*)V, n,
+                         (const double*)hout, wv, last ? part_nrm : (double*)nullptr, w.rows_per_blk, (const int*)w.ctrl);
         }
         hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(BLK), 0, c.stream, j, reorth ? 1 : 0, w.nblk, part_nrm, w.S, w.L, w.ctrl);
         hipLaunchKernelGGL((k_scale<T, T>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, wv, V + (int64_t)(j + 1) * n, w.S, w.L.misc() + 1, w.ctrl);
@@ -746,8 +767,11 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     VFVM_HIP_TRY(c.z64.alloc(n));
     const bool ilu = o.precond == VFVM_PRECOND_ILU0;
     const bool jac = o.precond == VFVM_PRECOND_JACOBI;
-    if (ilu)
+    if (ilu) {
         if (int s = ilu_setup(c)) return s;
+        if (mixed) VFVM_HIP_TRY(c.zp32.alloc(n));
+        else VFVM_HIP_TRY(c.zp64.alloc(n));
+    }
     if (!mixed) VFVM_HIP_TRY(c.V64.alloc((size_t)n * (m + 1)));
     else {
         VFVM_HIP_TRY(c.V32.alloc((size_t)n * (m + 1)));
@@ -815,12 +839,12 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
         int steps = 0;
         if (!mixed) {
             hipLaunchKernelGGL((k_scale<double, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V64.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<double>(c, w, c.V64.p, c.w64.p, c.z64.p, (const double*)aval, (const double*)lu, reorth, check_every, &steps)) return s;
+            if (int s = arnoldi_cycle<double>(c, w, c.V64.p, c.w64.p, c.z64.p, c.zp64.p, (const double*)aval, (const double*)lu, reorth, check_every, &steps)) return s;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y)
             hipLaunchKernelGGL((k_combine<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V64.p, n, w.S, w.L, w.ctrl, c.w64.p);
             if (lu) {
-                if (int s = ilu_apply<double>(c, (const double*)lu, c.w64.p, c.z64.p, nullptr)) return s;
+                if (int s = ilu_apply<double>(c, (const double*)lu, (const double*)c.w64.p, c.z64.p, c.zp64.p, (const int*)nullptr)) return s;
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
             } else {
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.w64.p, dinv_x, x);
@@ -828,7 +852,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
         } else {
             // inner FP32 solve of A M^-1 u = r from u = 0, V0 = r / beta in FP32
             hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V32.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<float>(c, w, c.V32.p, c.w32.p, c.w32.p + n, (const float*)aval, (const float*)lu, reorth, check_every, &steps)) return s;
+            if (int s = arnoldi_cycle<float>(c, w, c.V32.p, c.w32.p, c.w32.p + n, c.zp32.p, (const float*)aval, (const float*)lu, reorth, check_every, &steps)) return s;
             ++refinements;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y), V y accumulated in FP64
@@ -837,7 +861,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
                 float* t32 = c.w32.p;
                 float* z32 = c.w32.p + n;
                 hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, t32, w.S, w.L.misc() + 4, (const int*)nullptr);
-                if (int s = ilu_apply<float>(c, (const float*)lu, t32, z32, nullptr)) return s;
+                if (int s = ilu_apply<float>(c, (const float*)lu, (const float*)t32, z32, c.zp32.p, (const int*)nullptr)) return s;
                 hipLaunchKernelGGL((k_scale<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z32, c.z64.p, w.S, w.L.misc() + 4, (const int*)nullptr);
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
             } else {
