@@ -171,3 +171,69 @@ def test_c2_full_size_uniform_solution(ctx):
         # the cellular flow with one Dirichlet face is poorly conditioned (closed
         # streamlines): the 1e-10 residual bounds the error only to ~1e-4
         assert np.abs(x - 1.0).max() <= 1e-3
+
+
+def test_ilu_single_launch_equals_per_colour():
+    """The cooperative single-launch ILU(0) application and the per-colour launches run the
+    same per-row sums in the same order: bitwise equal solutions."""
+    import os
+    print("[ilu launch modes]", flush=True)
+    p = mass_c2(vfvm.Mesh.lattice(12), c_out=0.0)
+    outs = []
+    for mode in ("0", "1"):
+        os.environ["VFVM_ILU_PER_COLOUR"] = mode
+        try:
+            c = vfvm.Context(0)
+            p.to_device(c)
+            c.pattern_build(p.ops)
+            c.assemble(p.ops)
+            x, st = c.solve(None, tol=1e-12, precond=vfvm.PRECOND_ILU0, precision=vfvm.FP64)
+            outs.append(x)
+            c.close()
+        finally:
+            os.environ.pop("VFVM_ILU_PER_COLOUR", None)
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_reference_api_mirror_steady_heat():
+    """The host mirror of the reference API (Variables / SteadyHeat / LinearGmres) solves
+    config 1 through the C ABI and matches the oracle's LU solution."""
+    print("[api mirror]", flush=True)
+    vars = vfvm.Variables.new_lattice(12)
+    dom = vars.add_dom()
+    b0, b1 = vars.add_bnd(dom, "x0"), vars.add_bnd(dom, "x1")
+    T = vars.add_scldom_unkn(dom, 0.0)
+    k = vars.add_scldom_cnst(dom, 1.0)
+    zero = vars.add_scldom_cnst(dom, 0.0)
+    q = vars.add_scldom_cnst(dom, 1.0)
+    t0, t1 = vars.add_sclbnd_cnst(b0, 0.0), vars.add_sclbnd_cnst(b1, 1.0)
+    phys = vfvm.SteadyHeat.new()
+    phys.add_heat_dom(dom, T, k, zero, q)
+    phys.add_temp_bnd(b0, t0)
+    phys.add_temp_bnd(b1, t1)
+    for name in ("y0", "y1", "z0", "z1"):
+        b = vars.add_bnd(dom, name)
+        phys.add_hflx_bnd(b, vars.add_sclbnd_cnst(b, 0.0))
+    lin = vfvm.LinearGmres.new_with_options(1, 1e-13, 50, 200, precond=vfvm.PRECOND_ILU0)
+    phys.solve(vars, lin, 10, 1e-8, 1.0)
+    Tg = vars.value(T)
+    p = heat_c1(vars.mesh)
+    orp, ocol, oval, ob, _ = p.oracle_system()
+    x_lu = _lu(orp, ocol, oval, ob)
+    assert np.linalg.norm(Tg - x_lu) / np.linalg.norm(x_lu) <= 1e-10
+    # error behaviour of SteadyBase::solve (steady_base.rs:21-23)
+    with pytest.raises(vfvm.VFVChemError) as e:
+        phys.solve(vars, lin, 0, 1e-8, 1.0)
+    assert e.value.code == 40
+    with pytest.raises(vfvm.VFVChemError) as e:
+        phys.solve(vars, lin, 10, 1e-8, 1.5)
+    assert e.value.code == 42
+
+
+def test_linear_did_not_converge_is_reported(ctx):
+    """GMRES failure maps to LinearGmresDidNotConverge (linear_gmres.rs:81)."""
+    p = heat_c1(vfvm.Mesh.lattice(10))
+    _assembled(ctx, p)
+    with pytest.raises(vfvm.VFVChemError) as e:
+        ctx.solve(None, tol=1e-14, restart=2, max_outer=1)
+    assert e.value.code == 33

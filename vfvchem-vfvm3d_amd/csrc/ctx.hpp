@@ -130,7 +130,8 @@ struct Ctx {
     DevBuf<float> V32, w32, avals32;
     // multicolour ILU(0) in colour-major storage: row t of the permuted system is row perm[t]
     // of A, rows sorted by (colour, index); entries [L | diag | U] with permuted columns
-    DevBuf<int32_t> ilu_perm, ilu_prp, ilu_src, ilu_pcol, ilu_pdiag;
+    DevBuf<int32_t> ilu_perm, ilu_prp, ilu_src, ilu_pcol, ilu_pdiag, ilu_cptr;
+    int ilu_coop_blocks = -1;  // blocks of the cooperative sweep launch (0: per-colour launches)
     DevBuf<double> ilu64, zp64;
     DevBuf<float> ilu32, zp32;
     std::vector<int32_t> h_color_ptr;

@@ -16,6 +16,7 @@
 // host reads the control word every few steps.  Reductions are two-stage and
 // fixed-order, so results are bitwise reproducible run to run.
 #include <hip/hip_ext.h>
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,6 +25,8 @@
 #include <vector>
 
 #include "ctx.hpp"
+
+namespace cg = cooperative_groups;
 
 namespace vfvm {
 
@@ -115,40 +118,43 @@ __global__ void __launch_bounds__(BLK) k_residual(int64_t n, const int32_t* __re
     if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[nblk + blockIdx.x] = t2; }
 }
 
-// ---------------------------------------------------------------- multi-dot: part[k][blk] = sum_i V[k][i] w[i]
+// ---------------------------------------------------------------- multi-dot: part[k][wave] = sum_i V[k][i] w[i]
+// 2-D grid: x = row ranges (one per wave), y = groups of CHUNK basis vectors.  Each lane
+// keeps CHUNK dot products in registers and each wave writes one partial per vector:
+// no block barriers.
 template <class T>
 __global__ void __launch_bounds__(BLK) k_mdot(int64_t n, int nv, const T* __restrict__ V, int64_t ldv,
-                                              const T* __restrict__ w, double* __restrict__ part, int nblk,
-                                              int64_t rows_per_blk, const int* __restrict__ ctrl) {
+                                              const T* __restrict__ w, double* __restrict__ part, int nwav,
+                                              int64_t rows_per_wave, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
-    __shared__ double sh[CHUNK][BLK / WAVE];
-    const int64_t r0 = (int64_t)blockIdx.x * rows_per_blk;
-    const int64_t r1 = min(n, r0 + rows_per_blk);
-    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
-    for (int k0 = 0; k0 < nv; k0 += CHUNK) {
-        double acc[CHUNK];
+    const int lane = threadIdx.x % WAVE;
+    const int wv = blockIdx.x * (BLK / WAVE) + threadIdx.x / WAVE;
+    const int k0 = blockIdx.y * CHUNK;
+    const int64_t r0 = (int64_t)wv * rows_per_wave;
+    const int64_t r1 = min(n, r0 + rows_per_wave);
+    const int kq = min(CHUNK, nv - k0);
+    const T* Vk = V + (int64_t)k0 * ldv;
+    double acc[CHUNK];
 #pragma unroll
-        for (int q = 0; q < CHUNK; ++q) acc[q] = 0.0;
-        const int kq = min(CHUNK, nv - k0);
-        const T* Vk = V + (int64_t)k0 * ldv;
-        for (int64_t i = r0 + threadIdx.x; i < r1; i += BLK) {
+    for (int q = 0; q < CHUNK; ++q) acc[q] = 0.0;
+    if (kq == CHUNK) {
+        for (int64_t i = r0 + lane; i < r1; i += WAVE) {
+            const double wi = (double)w[i];
+#pragma unroll
+            for (int q = 0; q < CHUNK; ++q) acc[q] += (double)Vk[(int64_t)q * ldv + i] * wi;
+        }
+    } else {
+        for (int64_t i = r0 + lane; i < r1; i += WAVE) {
             const double wi = (double)w[i];
 #pragma unroll
             for (int q = 0; q < CHUNK; ++q)
                 if (q < kq) acc[q] += (double)Vk[(int64_t)q * ldv + i] * wi;
         }
+    }
 #pragma unroll
-        for (int q = 0; q < CHUNK; ++q) {
-            double s = wave_sum(acc[q]);
-            if (lane == 0) sh[q][wid] = s;
-        }
-        __syncthreads();
-        if (threadIdx.x < kq) {
-            double s = 0.0;
-            for (int k = 0; k < BLK / WAVE; ++k) s += sh[threadIdx.x][k];
-            part[(int64_t)(k0 + threadIdx.x) * nblk + blockIdx.x] = s;
-        }
-        __syncthreads();
+    for (int q = 0; q < CHUNK; ++q) {
+        const double s = wave_sum(acc[q]);
+        if (lane == 0 && q < kq) part[(int64_t)(k0 + q) * nwav + wv] = s;
     }
 }
 
@@ -229,48 +235,54 @@ __global__ void __launch_bounds__(BLK) k_cycle_start(int nblk, const double* __r
     }
 }
 
-// Arnoldi step j: h = h1 (+ h2), h_{j+1,j} = ||w||, Givens update, convergence test
+// Arnoldi step j: h = h1 (+ h2), h_{j+1,j} = ||w||, Givens update, convergence test.
+// The column and the rotations are staged in LDS by all threads; one thread applies the
+// j rotations in order.
 __global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int reorth, int nblk, const double* __restrict__ part_nrm,
                                                       double* __restrict__ S, StateLayout L, int* __restrict__ ctrl) {
     if (ctrl[0] == 0) return;
     __shared__ double sh[BLK / WAVE];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += BLK) s += part_nrm[i];
-    s = block_sum(s, sh);
-    if (threadIdx.x != 0) return;
+    __shared__ double hc[MAX_RESTART + 1], cs_s[MAX_RESTART], sn_s[MAX_RESTART];
     const int m = L.m;
     double* H = S + L.H();
-    double* cs = S + L.cs();
-    double* sn = S + L.sn();
-    double* g = S + L.g();
-    const double* h1 = S + L.h1();
-    const double* h2 = S + L.h2();
     const int64_t cj = (int64_t)j * (m + 1);
-    for (int i = 0; i <= j; ++i) H[cj + i] = reorth ? h1[i] + h2[i] : h1[i];
-    const double hn = sqrt(s);
-    H[cj + j + 1] = hn;
-    for (int i = 0; i < j; ++i) {
-        double a = H[cj + i], b = H[cj + i + 1];
-        H[cj + i] = cs[i] * a + sn[i] * b;
-        H[cj + i + 1] = -sn[i] * a + cs[i] * b;
+    for (int i = threadIdx.x; i <= j; i += BLK) {
+        hc[i] = reorth ? S[L.h1() + i] + S[L.h2() + i] : S[L.h1() + i];
+        if (i < j) { cs_s[i] = S[L.cs() + i]; sn_s[i] = S[L.sn() + i]; }
     }
-    double a = H[cj + j], b = H[cj + j + 1];
-    double r = hypot(a, b);
-    double c = r > 0.0 ? a / r : 1.0;
-    double sgn = r > 0.0 ? b / r : 0.0;
-    cs[j] = c;
-    sn[j] = sgn;
-    H[cj + j] = r;
-    H[cj + j + 1] = 0.0;
-    g[j + 1] = -sgn * g[j];
-    g[j] = c * g[j];
-    const double res = fabs(g[j + 1]);
-    S[L.misc() + 3] = res;
-    S[L.misc() + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
-    ctrl[1] = j + 1;
-    const bool finite = isfinite(res) && isfinite(hn) && isfinite(r);
-    if (!finite) ctrl[2] = 1;
-    if (!finite || res <= S[L.misc() + 2] || hn == 0.0 || r == 0.0) ctrl[0] = 0;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += BLK) s += part_nrm[i];
+    s = block_sum(s, sh);  // contains the barrier that publishes hc/cs_s/sn_s
+    if (threadIdx.x == 0) {
+        const double hn = sqrt(s);
+        hc[j + 1] = hn;
+        for (int i = 0; i < j; ++i) {
+            const double a = hc[i], b = hc[i + 1];
+            hc[i] = cs_s[i] * a + sn_s[i] * b;
+            hc[i + 1] = -sn_s[i] * a + cs_s[i] * b;
+        }
+        const double a = hc[j], b = hc[j + 1];
+        const double r = hypot(a, b);
+        const double c = r > 0.0 ? a / r : 1.0;
+        const double sgn = r > 0.0 ? b / r : 0.0;
+        S[L.cs() + j] = c;
+        S[L.sn() + j] = sgn;
+        hc[j] = r;
+        hc[j + 1] = 0.0;
+        double* g = S + L.g();
+        const double gj = g[j];
+        g[j + 1] = -sgn * gj;
+        g[j] = c * gj;
+        const double res = fabs(sgn * gj);
+        S[L.misc() + 3] = res;
+        S[L.misc() + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
+        ctrl[1] = j + 1;
+        const bool finite = isfinite(res) && isfinite(hn) && isfinite(r);
+        if (!finite) ctrl[2] = 1;
+        if (!finite || res <= S[L.misc() + 2] || hn == 0.0 || r == 0.0) ctrl[0] = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= j + 1; i += BLK) H[cj + i] = hc[i];
 }
 
 // y = H(1:k,1:k) \ g(1:k), k = steps done
@@ -406,6 +418,49 @@ __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const
     }
 }
 
+// Both sweeps in one cooperative launch: the grid walks the colours in order, with a
+// grid-wide barrier between colours (the barrier orders and publishes the zp writes of
+// one colour before the next reads them).  All blocks leave together when ctrl says the
+// cycle has converged, before the first barrier.
+template <class T, int LPR>
+__global__ void __launch_bounds__(BLK) k_ilu_apply_coop(int ncol, const int32_t* __restrict__ cptr,
+                                                        const int32_t* __restrict__ perm, const int32_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
+                                                        const T* __restrict__ lu, const T* __restrict__ rhs,
+                                                        T* __restrict__ zp, T* __restrict__ z, const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0] == 0) return;
+    cg::grid_group grid = cg::this_grid();
+    const int64_t gt = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t rstride = (int64_t)gridDim.x * BLK / LPR;
+    const int lane = (int)(gt % LPR);
+    for (int q = 0; q < ncol; ++q) {
+        const int32_t t1 = cptr[q + 1];
+        for (int64_t t = cptr[q] + gt / LPR; t < t1; t += rstride) {
+            T s = 0;
+            const int32_t d = diag[t];
+            for (int32_t k = rp[t] + lane; k < d; k += LPR) s += lu[k] * zp[col[k]];
+            for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
+            if (lane == 0) zp[t] = rhs[perm[t]] - s;
+        }
+        grid.sync();
+    }
+    for (int q = ncol - 1; q >= 0; --q) {
+        const int32_t t1 = cptr[q + 1];
+        for (int64_t t = cptr[q] + gt / LPR; t < t1; t += rstride) {
+            T s = 0;
+            const int32_t d = diag[t], e = rp[t + 1];
+            for (int32_t k = d + 1 + lane; k < e; k += LPR) s += lu[k] * zp[col[k]];
+            for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
+            if (lane == 0) {
+                const T v = (zp[t] - s) / lu[d];
+                zp[t] = v;
+                z[perm[t]] = v;
+            }
+        }
+        if (q > 0) grid.sync();
+    }
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 
 struct HipEvents {
@@ -493,8 +548,10 @@ void launch_timed(Ctx& c, int kclass, double bytes, K kernel, dim3 grid, dim3 bl
 struct Work {
     int64_t n, nnz;
     int m;
-    int nblk;  // blocks of the reductions
+    int nblk;  // blocks of the update kernel (one norm partial each)
     int64_t rows_per_blk;
+    int nwav;  // waves of the multi-dot (one partial per wave and basis vector)
+    int64_t rows_per_wave;
     StateLayout L;
     int* ctrl;    // device control word
     int* h_ctrl;  // pinned
@@ -530,6 +587,16 @@ int residual_launch(Ctx& c, int lpr, const double* x, double* r, double* part, i
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
+}
+
+// part[k][wave] of the multi-dot over basis vectors [0, nv)
+template <class T>
+void mdot_launch(Ctx& c, Work& w, int nv, const T* V, const T* wv, double* part) {
+    const dim3 grid(w.nwav / (BLK / WAVE), (nv + CHUNK - 1) / CHUNK), block(BLK);
+    // algorithmic bytes: the nv basis vectors and w once
+    const double bytes = (double)sizeof(T) * w.n * (nv + 1);
+    launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T>, grid, block, w.n, nv, V, w.n, wv, part, w.nwav, w.rows_per_wave,
+                 (const int*)w.ctrl);
 }
 
 // Multicolour ordering of the matrix pattern and the colour-major [L | diag | U] storage.
@@ -632,6 +699,42 @@ int ilu_apply(Ctx& c, const T* lu, const T* rhs, T* z, T* zp, const int* ctrl) {
     // algorithmic bytes of one sweep launch over r rows: the triangle's entries (value +
     // column, about half of each row), rp/diag/perm (12 B), rhs or z reads and writes
     const double per_row_entries = 0.5 * (double)c.nnz / (double)c.n * (sizeof(T) + 4.0);
+    if (c.ilu_coop_blocks < 0) {
+        // one cooperative launch per application when the device supports it
+        c.ilu_coop_blocks = 0;
+        int coop = 0, ncu = 0, per_cu = 0;
+        const char* off = getenv("VFVM_ILU_PER_COLOUR");
+        if (!(off && off[0] == '1') &&
+            hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c.device) == hipSuccess && coop &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ilu_apply_coop<T, LPR>, BLK, 0) == hipSuccess &&
+            per_cu > 0)
+            c.ilu_coop_blocks = std::min(per_cu, 4) * ncu;
+        (void)hipGetLastError();
+    }
+    if (c.ilu_coop_blocks > 0) {
+        if (c.ilu_cptr.n != c.h_color_ptr.size())
+            VFVM_HIP_TRY(c.ilu_cptr.upload(c.h_color_ptr.data(), c.h_color_ptr.size(), c.stream));
+        const int32_t* cptr = c.ilu_cptr.p;
+        const int32_t* perm = c.ilu_perm.p;
+        const int32_t* rp = c.ilu_prp.p;
+        const int32_t* cl = c.ilu_pcol.p;
+        const int32_t* dg = c.ilu_pdiag.p;
+        int nc = ncol;
+        void* args[] = {&nc, &cptr, &perm, &rp, &cl, &dg, (void*)&lu, (void*)&rhs, &zp, &z, (void*)&ctrl};
+        hipEvent_t ev = nullptr;
+        ktime_begin(c, VFVM_K_PRECOND, &ev);
+        hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_ilu_apply_coop<T, LPR>),
+                                                  dim3(c.ilu_coop_blocks), dim3(BLK), args, 0, c.stream);
+        if (e == hipSuccess) {
+            ktime_end(c, VFVM_K_PRECOND, (double)c.n * (2.0 * per_row_entries + 24.0 + 5.0 * sizeof(T)), ev);
+            return VFVM_OK;
+        }
+        // not launchable cooperatively (e.g. under a tool): fall back to one launch per colour
+        (void)hipGetLastError();
+        c.ilu_coop_blocks = 0;
+        if (ev) c.ev_next--;
+    }
     for (int q = 0; q < ncol; ++q) {
         int32_t s = c.h_color_ptr[q], e = c.h_color_ptr[q + 1];
         if (e == s) continue;
@@ -660,7 +763,7 @@ int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, T* zp, const T* aval, con
     int steps = 0;
     bool synced_last = false;
     double* part = c.part.p;
-    double* part_nrm = part + (int64_t)(MAX_RESTART + 1) * w.nblk;
+    double* part_nrm = part + (int64_t)(MAX_RESTART + 1) * w.nwav;
     const double bytes_spmv = (double)w.nnz * (sizeof(T) + 4) + 4.0 * (n + 1) + 2.0 * sizeof(T) * n;
     for (int j = 0; j < w.m; ++j) {
         c.cur_step = j;
@@ -672,14 +775,12 @@ int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, T* zp, const T* aval, con
         }
         if (int st = spmv_launch<T>(c, lpr, n, aval, xin, wv, w.ctrl, bytes_spmv)) return st;
         const int nv = j + 1;
-        const double bytes_dot = (double)sizeof(T) * n * (nv + 1);
         const double bytes_upd = (double)sizeof(T) * n * (nv + 2);
         for (int pass = 0; pass < (reorth ? 2 : 1); ++pass) {
             double* hout = w.S + (pass == 0 ? w.L.h1() : w.L.h2());
             const bool last = pass == (reorth ? 1 : 0);
-            launch_timed(c, VFVM_K_MDOT, bytes_dot, k_mdot<T>, dim3(w.nblk), dim3(BLK), n, nv, (const T*)V, n, (const T*)wv,
-                         part, w.nblk, w.rows_per_blk, (const int*)w.ctrl);
-            hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, w.nblk, part, hout, w.ctrl);
+            mdot_launch<T>(c, w, nv, V, wv, part);
+            hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, w.nwav, part, hout, w.ctrl);
             launch_timed(c, VFVM_K_UPDATE, bytes_upd, k_update<T>, dim3(w.nblk), dim3(BLK), n, nv, (const T*)V, n,
                          (const double*)hout, wv, last ? part_nrm : (double*)nullptr, w.rows_per_blk, (const int*)w.ctrl);
         }
@@ -750,6 +851,9 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     w.m = m;
     w.nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 4 * BLK - 1) / (4 * BLK)));
     w.rows_per_blk = (n + w.nblk - 1) / w.nblk;
+    w.nwav = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + WAVE * 8 - 1) / (WAVE * 8)));
+    w.nwav = (w.nwav + BLK / WAVE - 1) / (BLK / WAVE) * (BLK / WAVE);
+    w.rows_per_wave = (n + w.nwav - 1) / w.nwav;
     w.L.m = m;
     const int lpr = pick_lpr(n, c.nnz);
     const int nblk_r = (int)grid_for(n * lpr);
@@ -760,7 +864,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     w.ctrl = reinterpret_cast<int*>(c.scal.p + w.L.total());
     if (!c.h_pinned) VFVM_HIP_TRY(hipHostMalloc((void**)&c.h_pinned, 4096, hipHostMallocDefault));
     w.h_ctrl = reinterpret_cast<int*>(c.h_pinned);
-    VFVM_HIP_TRY(c.part.alloc((size_t)(MAX_RESTART + 2) * w.nblk + 2 * (size_t)nblk_r + 16));
+    VFVM_HIP_TRY(c.part.alloc((size_t)(MAX_RESTART + 1) * w.nwav + w.nblk + 2 * (size_t)nblk_r + 16));
     VFVM_HIP_TRY(c.r64.alloc(n));
     VFVM_HIP_TRY(c.dinv64.alloc(n));
     VFVM_HIP_TRY(c.w64.alloc(n));
@@ -815,7 +919,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     VFVM_HIP_TRY(hipEventRecord(ev.e[1], c.stream));
 
     const int check_every = 8;
-    double* part_r = c.part.p + (size_t)(MAX_RESTART + 2) * w.nblk;
+    double* part_r = c.part.p + (size_t)(MAX_RESTART + 1) * w.nwav + w.nblk;
     int total_steps = 0, cycles = 0, refinements = 0;
     double rel = 0.0, bnorm = 0.0;
     int status = VFVM_ERR_LINEAR_DID_NOT_CONVERGE;
