@@ -181,7 +181,7 @@ def test_ilu_single_launch_equals_per_colour():
     p = mass_c2(vfvm.Mesh.lattice(12), c_out=0.0)
     outs = []
     for mode in ("0", "1"):
-        os.environ["VFVM_ILU_PER_COLOUR"] = mode
+        os.environ["VFVM_ILU_COOP"] = mode
         try:
             c = vfvm.Context(0)
             p.to_device(c)
@@ -191,7 +191,7 @@ def test_ilu_single_launch_equals_per_colour():
             outs.append(x)
             c.close()
         finally:
-            os.environ.pop("VFVM_ILU_PER_COLOUR", None)
+            os.environ.pop("VFVM_ILU_COOP", None)
     assert np.array_equal(outs[0], outs[1])
 
 
