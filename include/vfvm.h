@@ -250,6 +250,59 @@ int vfvm_transient_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const 
                          double dt, int32_t num_ts, int32_t max_iter, double tol, double damp,
                          vfvm_driver_stats* stats);
 
+/* ---- discrete adjoint (no reference counterpart: the reference has no adjoint) ----
+ * Forward map of a linear problem (no operator coefficient depends on the unknown):
+ *   steady    (num_ts = 0):  A T = f
+ *   transient (num_ts > 0):  A T^n = B T^(n-1) + f,  n = 1..num_ts  (implicit Euler,
+ *                            TransientBase::solve, transient_base.rs:16-160, one linear
+ *                            solve per step, initial guess T^(n-1))
+ * with A, f from the operator list (as vfvm_assemble) and B = diag(-w V / dt) from its TIME
+ * operators.  For an objective J whose derivative `seed` = dJ/dT^num_ts is given, the
+ * backward pass solves A^T l^N = seed, A^T l^n = B^T l^(n+1) (transposed GMRES, initial
+ * guess l^(n+1)), replaying each checkpoint segment forward to recover T^n, and returns
+ *   dJ/dp = sum_n  l^n . (df/dp + dB/dp T^(n-1) - dA/dp T^n)
+ * for the requested parameters, accumulated per directed edge and per vertex on the device
+ * with no atomics.  Supported operators: DIFFUSION, ADVECTION_UNITY, SOURCE, TIME (dom and
+ * bnd), DIRICHLET, NEUMANN; PRODUCTION (whose right-hand side reads the unknown's gradient)
+ * and coefficients equal to the unknown are rejected with VFVM_ERR_OPERATOR_UNSUPPORTED. */
+typedef struct vfvm_adjoint_stats {
+    int32_t steps;                 /* time steps of the forward map (0: steady)            */
+    int32_t checkpoints;           /* states kept by the forward pass                      */
+    int32_t forward_iterations;    /* Arnoldi steps of the forward pass                    */
+    int32_t recompute_iterations;  /* Arnoldi steps of the segment replays                 */
+    int32_t adjoint_iterations;    /* Arnoldi steps of the transposed solves               */
+    int32_t recomputed_steps;      /* forward steps replayed by the backward pass          */
+    int32_t replay_mismatches;     /* entries of replayed states that differ from the
+                                      stored checkpoint they must reproduce (expected 0:
+                                      the kernels are deterministic)                      */
+    int32_t reserved;
+    double forward_seconds;        /* time-step loop of vfvm_adjoint_forward (wall)        */
+    double backward_seconds;       /* vfvm_adjoint_backward after its set-up (wall)        */
+    double recompute_seconds;      /* replay part of backward_seconds                      */
+    double adjoint_solve_seconds;  /* transposed-solve part of backward_seconds            */
+    double setup_seconds;          /* host pattern/colouring set-up of both passes         */
+} vfvm_adjoint_stats;
+/* Runs the forward map from the unknown's current values (Dirichlet values applied first,
+ * as vfvm_transient_solve), keeps T^0, T^K, T^2K, ... (K = checkpoint_every >= 1) and the
+ * final state, and leaves T^num_ts in the unknown. */
+int vfvm_adjoint_forward(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* opts,
+                         double dt, int32_t num_ts, int32_t checkpoint_every, vfvm_adjoint_stats* stats);
+enum vfvm_wrt_kind {
+    VFVM_WRT_SCLDOM = 0,   /* per-vertex values of a scalar field (N entries): diffusion
+                              coefficient, source or time weight, every use summed        */
+    VFVM_WRT_SCLBND = 1,   /* per-node Dirichlet values of a boundary field (boundary size) */
+    VFVM_WRT_INITIAL = 2   /* initial state T^0 as given before the Dirichlet values (N)    */
+};
+typedef struct vfvm_adjoint_wrt {
+    int32_t kind;  /* vfvm_wrt_kind                                  */
+    int32_t id;    /* field id (ignored for VFVM_WRT_INITIAL)        */
+    double* grad;  /* output, host memory                            */
+} vfvm_adjoint_wrt;
+/* Backward pass of the last vfvm_adjoint_forward.  seed: host array of N entries.  opts:
+ * the transposed solves' options.  The unknown holds T^num_ts again on return. */
+int vfvm_adjoint_backward(vfvm_ctx* ctx, const vfvm_solve_opts* opts, const double* seed, int32_t nwrt,
+                          const vfvm_adjoint_wrt* wrt, vfvm_adjoint_stats* stats);
+
 /* ---- measurement ---- */
 /* Per-launch HIP-event timing on the context stream: bit k of `mask` times kernel class k
  * (-1: all classes, 0: off).  Each timed launch is bracketed by two events. */

@@ -565,3 +565,181 @@ def gmres(rp, col, val, b, x0=None, restart=50, max_outer=200, tol=1e-6, max_ste
     st = lib.orc_gmres(n, rp.ctypes.data_as(P64), col.ctypes.data_as(P64), val.ctypes.data_as(PD), b.ctypes.data_as(PD),
                        x.ctypes.data_as(PD), int(restart), int(max_outer), float(tol), int(max_steps), C.byref(it), C.byref(rel))
     return x, st == 0, it.value, rel.value
+
+
+# --------------------------------------------------------------------------- discrete adjoint
+# The reference has no adjoint; these functions restate the textbook discrete adjoint of
+# the implicit-Euler map built from assemble() (one linear solve per step, operator
+# coefficients independent of the unknown).  They are pinned by central finite differences
+# of linear_map() itself (tests/test_cpu_adjoint_oracle.py), not by a second copy of the
+# formulas.  Sparse LU (scipy SuperLU) solves both the forward and the transposed systems.
+
+def _op_rows(g, op, bnd_nid):
+    """Vertices of the operator's row class: points for domain operators, the boundary's
+    nodes for boundary operators (operator_base.rs, xid = start + vid)."""
+    if op.kind < 20:
+        return np.arange(g["num_pts"])
+    return g["num_pts"] + np.asarray(bnd_nid[op.geom_id], np.int64)
+
+
+def _csr_system(g, ops, scl, sclbnd, flux, bnd_nid, unk, prev, dt):
+    import scipy.sparse as sp
+    N = g["N"]
+    r, c, v, b = assemble(g, ops, scl, sclbnd, flux, bnd_nid, dt=dt, value_prev={unk: prev})
+    rp, col, val = coo_to_csr(r, c, v, N)
+    return sp.csr_matrix((val, col, rp), shape=(N, N)), b
+
+
+def apply_initial_dirichlet(g, ops, sclbnd, bnd_nid, T0):
+    """OpSclBndDirichlet::apply_initial (opscl_bnd_dir.rs:34-53) on a copy of T0."""
+    T = np.array(T0, np.float64)
+    for op in ops:
+        if op.kind == Op.SCL_BND_DIRICHLET:
+            T[g["num_pts"] + np.asarray(bnd_nid[op.geom_id], np.int64)] = sclbnd[op.a]
+    return T
+
+
+def linear_map(g, ops, scl, sclbnd, flux, bnd_nid, unk, T0, dt=0.0, num_ts=0):
+    """States [T^0, ..., T^num_ts] of the linear map (steady: [T^0, T]) solved by sparse LU;
+    T^0 is T0 with the Dirichlet values applied (transient_base.rs:16-160, one solve per step)."""
+    from scipy.sparse.linalg import spsolve
+    T = apply_initial_dirichlet(g, ops, sclbnd, bnd_nid, T0)
+    states = [T]
+    for _ in range(max(num_ts, 1)):
+        A, b = _csr_system(g, ops, scl, sclbnd, flux, bnd_nid, unk, T, dt if num_ts > 0 else 0.0)
+        T = spsolve(A.tocsc(), b)
+        states.append(T)
+    return states
+
+
+def time_diag(g, ops, scl, bnd_nid, dt):
+    """B = d b / d T^(n-1): diag of -w V / dt over the TIME operators (opscl_dom_time.rs:46-58)."""
+    bd = np.zeros(g["N"])
+    for op in ops:
+        if op.kind in (Op.SCL_DOM_TIME, Op.SCL_BND_TIME):
+            rows = _op_rows(g, op, bnd_nid)
+            W = scl[op.a][0]
+            np.add.at(bd, rows, -W[rows] * g["vol"][rows] / dt)
+    return bd
+
+
+def adjoint_accumulate(g, ops, scl, sclbnd, flux, bnd_nid, unk, states, dt, num_ts, seed):
+    """Transposed solves A^T l^N = seed, A^T l^n = B l^(n+1) and the accumulators
+    a_e = -sum_n l_v (T_n - T_v) (e = (v, n)), ls = sum_n l, tv = sum_n l (T^n - T^(n-1)).
+    Returns dict(lams={n: l^n}, aE, ls, tv, bd)."""
+    from scipy.sparse.linalg import splu
+    A, _ = _csr_system(g, ops, scl, sclbnd, flux, bnd_nid, unk, states[-1], dt if num_ts > 0 else 0.0)
+    lu = splu(A.T.tocsc())
+    bd = time_diag(g, ops, scl, bnd_nid, dt) if num_ts > 0 else np.zeros(g["N"])
+    rows, ngh = g["rows"].astype(np.int64), g["ngh"].astype(np.int64)
+    aE = np.zeros(rows.size)
+    ls = np.zeros(g["N"])
+    tv = np.zeros(g["N"])
+    last = len(states) - 1
+    lams = {}
+    lam = lu.solve(np.asarray(seed, np.float64))
+    for k in range(last, 0, -1):
+        if k != last:
+            lam = lu.solve(bd * lams[k + 1])
+        lams[k] = lam
+        Tn = states[k]
+        aE -= lam[rows] * (Tn[ngh] - Tn[rows])
+        ls += lam
+        if num_ts > 0:
+            tv += lam * (Tn - states[k - 1])
+    return dict(lams=lams, aE=aE, ls=ls, tv=tv, bd=bd)
+
+
+def grad_matrices(g):
+    """update_grad (scl_dom.rs:146-164) as sparse matrices: (grad G)_c = W_c G."""
+    import scipy.sparse as sp
+    N = g["N"]
+    rows, ngh = g["rows"].astype(np.int64), g["ngh"].astype(np.int64)
+    out = []
+    for c in range(3):
+        w = g["gw"][c]
+        W = sp.csr_matrix((w, (rows, ngh)), shape=(N, N))
+        dsum = np.zeros(N)
+        np.add.at(dsum, rows, w)
+        out.append((W - sp.diags(dsum)).tocsr())
+    return out
+
+
+def adjoint_field_gradient(g, ops, scl, flux, bnd_nid, acc, dt, fid, constant):
+    """dJ/dG for the per-vertex values of scalar field fid, summed over its uses as a
+    diffusion coefficient (DIFFUSION, ADVECTION_UNITY), a SOURCE and a TIME weight.
+    Scatter form: every directed edge adds its partials to both of its vertices."""
+    N = g["N"]
+    rows, ngh = g["rows"].astype(np.int64), g["ngh"].astype(np.int64)
+    G, Gg = scl[fid]
+    grad = np.zeros(N)
+    adj_gG = np.zeros((3, N))
+    ad_all = g["area"] / g["dist"]
+    # d(matrix entry)/d(face value), summed over the operators of each edge's row
+    dface = np.zeros(rows.size)
+    for op in ops:
+        if op.a != fid:
+            continue
+        if op.kind in (Op.SCL_DOM_DIFFUSION, Op.SCL_BND_DIFFUSION, Op.SCL_DOM_ADVECTION_UNITY, Op.SCL_BND_ADVECTION_UNITY):
+            m = np.zeros(N, bool)
+            m[_op_rows(g, op, bnd_nid)] = True
+            e = np.nonzero(m[rows])[0]
+            if op.kind in (Op.SCL_DOM_DIFFUSION, Op.SCL_BND_DIFFUSION):
+                dface[e] += ad_all[e]
+            else:
+                df = _diff_face(G, Gg, rows[e], ngh[e], g["skew"][:, e])
+                cond = df * ad_all[e]
+                F = flux[op.b][e]
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    pe = np.where(cond != 0.0, F / cond, 0.0)
+                x = np.maximum(1.0 - 0.1 * np.abs(pe), 0.0)
+                # d/dcond [cond (x^5 - 1)] with d|Pe|/dcond = -|Pe|/cond
+                dface[e] += ad_all[e] * (x ** 5 - 1.0 + 0.5 * x ** 4 * np.abs(pe))
+        elif op.kind in (Op.SCL_DOM_SOURCE, Op.SCL_BND_SOURCE):
+            r = _op_rows(g, op, bnd_nid)
+            w = -acc["ls"][r] * g["vol"][r]
+            np.add.at(grad, r, w)
+            np.add.at(adj_gG.T, r, (w * g["cskew"][:, r]).T)
+        elif op.kind in (Op.SCL_DOM_TIME, Op.SCL_BND_TIME):
+            r = _op_rows(g, op, bnd_nid)
+            np.add.at(grad, r, acc["tv"][r] * g["vol"][r] / dt)
+    s = acc["aE"] * dface  # adjoint of each edge's face value
+    Gv, Gn = G[rows], G[ngh]
+    ssum = Gv + Gn
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dm_dv = np.where(ssum != 0.0, 2.0 * Gn * Gn / (ssum * ssum), 0.0)
+        dm_dn = np.where(ssum != 0.0, 2.0 * Gv * Gv / (ssum * ssum), 0.0)
+        diff_m = np.where(ssum != 0.0, 2.0 * Gv * Gn / ssum, 0.0)
+    sk = g["skew"]
+    diff_s = diff_m + 0.5 * (sk[0] * (Gg[0][rows] + Gg[0][ngh]) + sk[1] * (Gg[1][rows] + Gg[1][ngh]) + sk[2] * (Gg[2][rows] + Gg[2][ngh]))
+    chi = (diff_s > 0.0).astype(np.float64)
+    np.add.at(grad, rows, s * dm_dv)
+    np.add.at(grad, ngh, s * dm_dn)
+    for c in range(3):
+        np.add.at(adj_gG[c], rows, 0.5 * s * chi * sk[c])
+        np.add.at(adj_gG[c], ngh, 0.5 * s * chi * sk[c])
+    if not constant:
+        for c, W in enumerate(grad_matrices(g)):
+            grad += W.T @ adj_gG[c]
+    return grad
+
+
+def adjoint_dirichlet_gradient(g, ops, bnd_nid, acc, sid):
+    """dJ/d(values of boundary field sid) through its Dirichlet rows (b_v = value)."""
+    out = None
+    for op in ops:
+        if op.kind == Op.SCL_BND_DIRICHLET and op.a == sid:
+            r = g["num_pts"] + np.asarray(bnd_nid[op.geom_id], np.int64)
+            out = acc["ls"][r] if out is None else out + acc["ls"][r]
+    return out
+
+
+def adjoint_initial_gradient(g, ops, bnd_nid, acc, num_ts):
+    """dJ/dT^0 = B l^1, zero where the Dirichlet values overwrite T^0."""
+    if num_ts == 0:
+        return np.zeros(g["N"])
+    out = acc["bd"] * acc["lams"][1]
+    for op in ops:
+        if op.kind == Op.SCL_BND_DIRICHLET:
+            out[g["num_pts"] + np.asarray(bnd_nid[op.geom_id], np.int64)] = 0.0
+    return out

@@ -2,6 +2,9 @@ import os
 import sys
 
 import pytest
+# torch's ROCm runtime must be loaded before the library initialises its own HIP runtime
+# (tests that hand torch tensors to the library need both)
+import torch  # noqa: F401
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (os.path.join(ROOT, "vfvchem-vfvm3d_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):

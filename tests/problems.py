@@ -59,10 +59,9 @@ class Problem:
         return ctx
 
     # ---- oracle
-    def oracle_system(self, g=None, value_prev=None, dt=0.0, unknown_value=None):
-        """Assembled CSR (row_ptr, col, val) and b of the oracle, and the dual used."""
-        if g is None:
-            g = O.dual(self.pos, self.mesh.get("tets"), self.mesh.num_pts)
+    def oracle_inputs(self, g, unknown_value=None):
+        """Fields (value, gradient), boundary values, face fluxes, boundary node lists and
+        operators in the oracle's form."""
         scl = {}
         for i, (kind, v) in enumerate(self.scl):
             if unknown_value is not None and kind == vfvm.FIELD_UNKNOWN:
@@ -77,6 +76,13 @@ class Problem:
         bnd_nid = [self.mesh.bnd_nodes(b) for b in range(6)]
         sb = {i: v for i, (b, v) in enumerate(self.sbnd)}
         ops = [O.Op(*o) for o in self.ops]
+        return scl, sb, flux, bnd_nid, ops
+
+    def oracle_system(self, g=None, value_prev=None, dt=0.0, unknown_value=None):
+        """Assembled CSR (row_ptr, col, val) and b of the oracle, and the dual used."""
+        if g is None:
+            g = O.dual(self.pos, self.mesh.get("tets"), self.mesh.num_pts)
+        scl, sb, flux, bnd_nid, ops = self.oracle_inputs(g, unknown_value)
         r, c, v, b = O.assemble(g, ops, scl, sb, flux, bnd_nid, dt=dt, value_prev=value_prev)
         rp, col, val = O.coo_to_csr(r, c, v, self.N)
         return rp, col, val, b, g
@@ -110,10 +116,10 @@ def mass_c2(mesh, D=1e-2, U0=None, c_out=None, prod=0.0):
     n = round(mesh.num_tot ** (1 / 3))
     h = 1.0 / (n - 1)
     if U0 is None:
-        U0 = 5.0 * D / h
+        U0 = 5.0 * (D if np.isscalar(D) else float(np.mean(D))) / h
     p = Problem(mesh)
     c = p.scldom(vfvm.FIELD_UNKNOWN, 0.0)
-    Dd = p.scldom(vfvm.FIELD_CONSTANT, D)
+    Dd = p.scldom(vfvm.FIELD_CONSTANT if np.isscalar(D) else vfvm.FIELD_FUNCTION, D)
     mp = p.scldom(vfvm.FIELD_CONSTANT, prod)
     ms = p.scldom(vfvm.FIELD_CONSTANT, 0.0)
     den = p.scldom(vfvm.FIELD_CONSTANT, 1.0)
@@ -139,5 +145,36 @@ def mass_c2(mesh, D=1e-2, U0=None, c_out=None, prod=0.0):
                   (vfvm.OP_SCL_BND_SOURCE, b, c, ms, -1), (vfvm.OP_SCL_BND_NEUMANN, b, c, q[b], -1)]
     for b in flux_bnds:
         p.ops.append((vfvm.OP_SCL_BND_ADVECTION_UNITY, b, c, Dd, f))
-    p.unknown = c
+    p.unknown, p.d_id, p.cin_id = c, Dd, cin
     return p
+
+
+def heat_c3(mesh, k, w=1.0, q=0.0, t_in=1.0):
+    """Transient heat (TransientHeat operator order, transient_heat.rs:152-171): per-vertex
+    conductivity k, weight rho c_p = w, source q, T = t_in at x=0, zero heat flux on the
+    other five faces, T0 = 0.  k, q: scalars (constant fields) or per-vertex arrays."""
+    p = Problem(mesh)
+    T = p.scldom(vfvm.FIELD_UNKNOWN, 0.0)
+    ww = p.scldom(vfvm.FIELD_CONSTANT if np.isscalar(w) else vfvm.FIELD_FUNCTION, w)
+    kk = p.scldom(vfvm.FIELD_CONSTANT if np.isscalar(k) else vfvm.FIELD_FUNCTION, k)
+    hp = p.scldom(vfvm.FIELD_CONSTANT, 0.0)
+    hs = p.scldom(vfvm.FIELD_CONSTANT if np.isscalar(q) else vfvm.FIELD_FUNCTION, q)
+    tin = p.sclbnd(X0, t_in)
+    fl = {b: p.sclbnd(b, 0.0) for b in (X1, Y0, Y1, Z0, Z1)}
+    p.ops = [(vfvm.OP_SCL_DOM_TIME, 0, T, ww, -1), (vfvm.OP_SCL_DOM_DIFFUSION, 0, T, kk, T),
+             (vfvm.OP_SCL_DOM_PRODUCTION, 0, T, hp, -1), (vfvm.OP_SCL_DOM_SOURCE, 0, T, hs, -1),
+             (vfvm.OP_SCL_BND_DIRICHLET, X0, T, tin, -1)]
+    for b in (X1, Y0, Y1, Z0, Z1):
+        p.ops += [(vfvm.OP_SCL_BND_TIME, b, T, ww, -1), (vfvm.OP_SCL_BND_DIFFUSION, b, T, kk, T),
+                  (vfvm.OP_SCL_BND_PRODUCTION, b, T, hp, -1), (vfvm.OP_SCL_BND_SOURCE, b, T, hs, -1),
+                  (vfvm.OP_SCL_BND_NEUMANN, b, T, fl[b], -1)]
+    p.unknown, p.k_id, p.w_id, p.q_id, p.tin_id = T, kk, ww, hs, tin
+    return p
+
+
+def face_mean_weights(mesh, b=X1):
+    """dJ/dT of J = mean of T over the nodes of boundary b."""
+    w = np.zeros(mesh.num_tot)
+    nodes = mesh.num_pts + mesh.bnd_nodes(b).astype(np.int64)
+    w[nodes] = 1.0 / nodes.size
+    return w

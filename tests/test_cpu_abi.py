@@ -28,10 +28,30 @@ def test_library_exports_every_declared_symbol():
     assert {n for n, _, _ in L.SIGNATURES} == set(names)
 
 
-def test_struct_layouts_match_header():
-    assert C.sizeof(L.SolveOpts) == 40
-    assert C.sizeof(L.OpStruct) == 20
-    assert C.sizeof(L.MeshInfo) == 48
+_STRUCTS = [("vfvm_mesh_info", L.MeshInfo), ("vfvm_op", L.OpStruct), ("vfvm_solve_opts", L.SolveOpts),
+            ("vfvm_solve_stats", L.SolveStats), ("vfvm_driver_stats", L.DriverStats),
+            ("vfvm_adjoint_stats", L.AdjointStats), ("vfvm_adjoint_wrt", L.AdjointWrt)]
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Sizes and field offsets of every ctypes structure against the C compiler's layout
+    of include/vfvm.h (gcc on a generated probe)."""
+    import subprocess
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "vfvm.h"', "int main(void) {"]
+    for cname, cls in _STRUCTS:
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines())
+    for cname, cls in _STRUCTS:
+        assert int(got[cname]) == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(cls, fname).offset, (cname, fname)
 
 
 def test_version_and_error_string():

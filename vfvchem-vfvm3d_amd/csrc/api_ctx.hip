@@ -27,10 +27,6 @@ int clear_error() {
 extern "C" const char* vfvm_last_error(void) { return g_last_error.c_str(); }
 extern "C" const char* vfvm_version(void) { return "vfvm-mi355x 0.1.0 (gfx950)"; }
 
-struct vfvm_ctx {
-    Ctx c;
-};
-
 namespace {
 
 constexpr int BLK = 256;
@@ -196,12 +192,12 @@ int build_program(Ctx& c, const Compiled& cp, DevProgram& p) {
     return VFVM_OK;
 }
 
-int sync_err(Ctx& c) {
+}  // namespace
+
+int vfvm::sync_err(Ctx& c) {
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     return VFVM_OK;
 }
-
-}  // namespace
 
 // ------------------------------------------------------------------ context
 extern "C" int vfvm_create(int32_t device, vfvm_ctx** out) {
@@ -232,11 +228,6 @@ extern "C" void vfvm_destroy(vfvm_ctx* ctx) {
     if (s) (void)hipStreamDestroy(s);
 }
 
-#define CTX_GUARD(ctx)                                                           \
-    if (!(ctx)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "context is NULL"); \
-    Ctx& c = (ctx)->c;                                                           \
-    VFVM_HIP_TRY(hipSetDevice(c.device))
-
 extern "C" int vfvm_mesh_upload(vfvm_ctx* ctx, const vfvm_mesh* mesh) {
     CTX_GUARD(ctx);
     const Mesh* mp = vfvm_mesh_ptr(mesh);
@@ -244,8 +235,10 @@ extern "C" int vfvm_mesh_upload(vfvm_ctx* ctx, const vfvm_mesh* mesh) {
     const Mesh& m = *mp;
     // reset everything that depends on the mesh
     c.scl.clear(); c.vec.clear(); c.sbnd.clear(); c.flw.clear();
-    c.has_pattern = false; c.has_matrix = false; c.ilu_levels_ready = false;
+    c.has_pattern = false; c.has_matrix = false; c.ilu.ready = false;
+    c.has_transpose = false; c.tilu.ready = false; c.tape.valid = false;
     DevGeom& g = c.g;
+    g.rev.reset();
     g.N = m.num_tot(); g.NP = m.num_pts; g.E = m.num_edges(); g.S = m.num_subfaces();
     g.num_bnd = (int32_t)m.bnd_name.size();
     hipStream_t s = c.stream;
@@ -449,7 +442,10 @@ extern "C" int vfvm_pattern_build(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nop
     c.pattern_ops.assign(ops, ops + nops);
     c.has_pattern = true;
     c.has_matrix = false;
-    c.ilu_levels_ready = false;
+    c.ilu.ready = false;
+    c.has_transpose = false;
+    c.tilu.ready = false;
+    c.tape.valid = false;
     if (nnz) *nnz = c.nnz;
     return clear_error();
 }
@@ -463,8 +459,7 @@ extern "C" int vfvm_pattern_get(vfvm_ctx* ctx, int32_t* row_ptr, int32_t* col) {
 }
 
 // ------------------------------------------------------------------ assembly
-namespace {
-int assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
+int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
     if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "assemble: no pattern built (call vfvm_pattern_build)");
     Compiled cp;
     if (int st = compile_ops(c, ops, nops, cp)) return st;
@@ -512,7 +507,6 @@ int assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
     c.has_matrix = true;
     return VFVM_OK;
 }
-}  // namespace
 
 extern "C" int vfvm_assemble(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, double dt) {
     CTX_GUARD(ctx);
@@ -549,7 +543,7 @@ extern "C" int vfvm_solve_device(vfvm_ctx* ctx, const vfvm_solve_opts* opts, dou
     if (!opts || !x_dev) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_solve_device: NULL argument");
     if (!c.has_matrix) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_solve_device: nothing assembled");
     vfvm_solve_stats st;
-    int r = gmres_solve(c, *opts, x_dev, st);
+    int r = gmres_solve(c, primal_view(c), *opts, x_dev, st);
     if (stats) *stats = st;
     if (r != VFVM_OK) return r;
     return clear_error();
@@ -562,7 +556,7 @@ extern "C" int vfvm_solve(vfvm_ctx* ctx, const vfvm_solve_opts* opts, double* x,
     VFVM_HIP_TRY(c.x64.alloc(c.n));
     VFVM_HIP_TRY(hipMemcpyAsync(c.x64.p, x, c.n * sizeof(double), hipMemcpyHostToDevice, c.stream));
     vfvm_solve_stats st;
-    int r = gmres_solve(c, *opts, c.x64.p, st);
+    int r = gmres_solve(c, primal_view(c), *opts, c.x64.p, st);
     if (stats) *stats = st;
     VFVM_HIP_TRY(hipMemcpyAsync(x, c.x64.p, c.n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     if (int s2 = sync_err(c)) return s2;
@@ -577,14 +571,12 @@ extern "C" int vfvm_residual(vfvm_ctx* ctx, const double* x, double* rel_residua
     VFVM_HIP_TRY(c.x64.alloc(c.n));
     VFVM_HIP_TRY(hipMemcpyAsync(c.x64.p, x, c.n * sizeof(double), hipMemcpyHostToDevice, c.stream));
     double bn = 0.0;
-    if (int st = residual_norm(c, c.x64.p, rel_residual, &bn)) return st;
+    if (int st = residual_norm(c, primal_view(c), c.x64.p, rel_residual, &bn)) return st;
     return clear_error();
 }
 
 // ------------------------------------------------------------------ drivers
-namespace {
-
-int initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
+int vfvm::initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
     // OpSclBndDirichlet::apply_initial for every Dirichlet operator (steady_base.rs:53-54)
     for (int32_t i = 0; i < nops; ++i) {
         if (ops[i].kind != VFVM_OP_SCL_BND_DIRICHLET) continue;
@@ -598,6 +590,8 @@ int initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
     }
     return VFVM_OK;
 }
+
+namespace {
 
 // scaled Picard residual for one scalar field (scaling.rs:88-135)
 int picard_residual(Ctx& c, const double* xo, const double* xn, double* res) {
@@ -639,7 +633,7 @@ int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts*
         auto ti0 = clk::now();
         VFVM_HIP_TRY(hipMemcpyAsync(xu.p, xi.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
         vfvm_solve_stats st;
-        if (int r = gmres_solve(c, *opts, xu.p, st)) return r;
+        if (int r = gmres_solve(c, primal_view(c), *opts, xu.p, st)) return r;
         ds.linear_iterations += st.iterations;
         auto ti1 = clk::now();
         ds.time_solve += secs(ti0, ti1);

@@ -56,6 +56,7 @@ struct DevGeom {
     DevBuf<int32_t> bsf_ptr, bsf_vj, bsf_vk;
     DevBuf<double> bsf_area, bsf_wgt;          // bsf_wgt: 3S SoA
     DevBuf<int32_t> vbnd, bidx;                // vertex -> boundary (-1: point), index in boundary
+    DevBuf<int32_t> rev;                       // rev[e] = directed edge (n, v) of e = (v, n); built on demand
     std::vector<int64_t> bnd_ptr;              // host copy
     std::vector<int32_t> bnd_nid;
     std::vector<int32_t> h_row_ptr, h_ngh;     // host copies for the pattern build
@@ -98,6 +99,45 @@ struct DevProgram {
     DevOp ops[MAX_CLASSES][MAX_OPS];
 };
 
+// Multicolour ILU(0) ordering of one pattern, in colour-major storage: row t of the
+// permuted system is row perm[t] of the matrix, rows sorted by (colour, index); entries
+// [L | diag | U] with permuted columns; src[k] = CSR slot of permuted entry k.
+struct IluData {
+    DevBuf<int32_t> perm, prp, src, pcol, pdiag, cptr;
+    std::vector<int32_t> h_color_ptr;
+    int coop_blocks = -1;  // blocks of the cooperative sweep launch (0: per-colour launches)
+    bool ready = false;
+};
+
+// One linear system as the solver reads it (non-owning).
+struct SysView {
+    int64_t n = 0, nnz = 0;
+    const int32_t* row = nullptr;
+    const int32_t* col = nullptr;
+    const int32_t* dslot = nullptr;
+    const double* vals = nullptr;
+    const double* b = nullptr;
+    const std::vector<int32_t>* h_row = nullptr;
+    const std::vector<int32_t>* h_col = nullptr;
+    IluData* ilu = nullptr;
+};
+
+// Checkpoint tape of the adjoint forward pass: the operator list, step data and the
+// states T^0, T^K, T^2K, ... (K = checkpoint interval) kept in HBM.
+struct AdjTape {
+    bool valid = false;
+    std::vector<vfvm_op> ops;
+    double dt = 0.0;
+    int32_t num_ts = 0;  // 0: steady
+    int32_t every = 0;
+    vfvm_solve_opts opts{};
+    DevBuf<double> ckpt;   // (num_ts / every + 1) states of n entries
+    DevBuf<double> seg;    // (every + 1) states T^first .. T^(first + count) of one segment
+    int32_t seg_first = -1, seg_count = 0;
+    DevBuf<double> final_state;
+    vfvm_adjoint_stats stats{};
+};
+
 struct KTimer {
     int64_t launches = 0;
     double ms = 0.0;
@@ -125,18 +165,24 @@ struct Ctx {
     DevBuf<double> vals, b;
     bool has_matrix = false;
 
-    // solver workspace
+    IluData ilu;  // ILU(0) ordering of the pattern above
+
+    // transposed system A^T (adjoint solves): CSR of the columns of A, tsrc[k] = slot of
+    // entry k in vals; values gathered on demand
+    bool has_transpose = false;
+    std::vector<int32_t> h_trow, h_tcol;
+    DevBuf<int32_t> trow, tcol, tsrc, tdslot;
+    DevBuf<double> tvals, tb;
+    IluData tilu;
+
+    // solver workspace (shared by both systems)
     DevBuf<double> V64, w64, z64, x64, r64, dinv64, avals64, part, scal;
     DevBuf<float> V32, w32, avals32;
-    // multicolour ILU(0) in colour-major storage: row t of the permuted system is row perm[t]
-    // of A, rows sorted by (colour, index); entries [L | diag | U] with permuted columns
-    DevBuf<int32_t> ilu_perm, ilu_prp, ilu_src, ilu_pcol, ilu_pdiag, ilu_cptr;
-    int ilu_coop_blocks = -1;  // blocks of the cooperative sweep launch (0: per-colour launches)
     DevBuf<double> ilu64, zp64;
     DevBuf<float> ilu32, zp32;
-    std::vector<int32_t> h_color_ptr;
-    bool ilu_levels_ready = false;
     double* h_pinned = nullptr;  // small pinned readback buffer
+
+    AdjTape tape;
 
     // timing
     uint32_t timing_mask = 0;  // bit k: time kernel class k
@@ -160,7 +206,25 @@ int launch_flux_prescribed(Ctx& c, SclFlw& f);
 int launch_assemble(Ctx& c, double dt);
 
 // solver (krylov.hip)
-int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x_dev, vfvm_solve_stats& st);
-int residual_norm(Ctx& c, const double* x_dev, double* rel, double* bnorm);
+SysView primal_view(Ctx& c);
+SysView transpose_view(Ctx& c);
+int gmres_solve(Ctx& c, const SysView& s, const vfvm_solve_opts& o, double* x_dev, vfvm_solve_stats& st);
+// host-side preconditioner ordering of a pattern (ILU(0) colouring), done once per pattern
+int solver_setup(Ctx& c, const SysView& s, const vfvm_solve_opts& o);
+int residual_norm(Ctx& c, const SysView& s, const double* x_dev, double* rel, double* bnorm);
+
+// drivers and adjoint set-up shared across translation units (api_ctx.hip)
+int assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt);
+int initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops);
+int sync_err(Ctx& c);
 
 }  // namespace vfvm
+
+struct vfvm_ctx {
+    vfvm::Ctx c;
+};
+
+#define CTX_GUARD(ctx)                                                           \
+    if (!(ctx)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "context is NULL"); \
+    vfvm::Ctx& c = (ctx)->c;                                                     \
+    VFVM_HIP_TRY(hipSetDevice(c.device))

@@ -564,10 +564,10 @@ inline int pick_lpr(int64_t n, int64_t nnz) {
 }
 
 template <class T>
-int spmv_launch(Ctx& c, int lpr, int64_t n, const T* val, const T* x, T* y, const int* ctrl, double bytes) {
+int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, const T* x, T* y, const int* ctrl, double bytes) {
     unsigned grid = grid_for(n * lpr);
-    const int32_t* rp = c.mrow.p;
-    const int32_t* cl = c.mcol.p;
+    const int32_t* rp = sy.row;
+    const int32_t* cl = sy.col;
     switch (lpr) {
         case 4: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 4>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, ctrl); break;
         case 8: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 8>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, ctrl); break;
@@ -577,13 +577,13 @@ int spmv_launch(Ctx& c, int lpr, int64_t n, const T* val, const T* x, T* y, cons
     return VFVM_OK;
 }
 
-int residual_launch(Ctx& c, int lpr, const double* x, double* r, double* part, int nblk_r) {
-    const int64_t n = c.n;
+int residual_launch(Ctx& c, const SysView& sy, int lpr, const double* x, double* r, double* part, int nblk_r) {
+    const int64_t n = sy.n;
     unsigned grid = (unsigned)nblk_r;
     switch (lpr) {
-        case 4: hipLaunchKernelGGL((k_residual<4>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, c.vals.p, x, c.b.p, r, part, nblk_r); break;
-        case 8: hipLaunchKernelGGL((k_residual<8>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, c.vals.p, x, c.b.p, r, part, nblk_r); break;
-        default: hipLaunchKernelGGL((k_residual<16>), dim3(grid), dim3(BLK), 0, c.stream, n, c.mrow.p, c.mcol.p, c.vals.p, x, c.b.p, r, part, nblk_r); break;
+        case 4: hipLaunchKernelGGL((k_residual<4>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
+        case 8: hipLaunchKernelGGL((k_residual<8>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
+        default: hipLaunchKernelGGL((k_residual<16>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
@@ -600,11 +600,12 @@ void mdot_launch(Ctx& c, Work& w, int nv, const T* V, const T* wv, double* part)
 }
 
 // Multicolour ordering of the matrix pattern and the colour-major [L | diag | U] storage.
-int ilu_setup(Ctx& c) {
-    if (c.ilu_levels_ready) return VFVM_OK;
-    const int64_t n = c.n;
-    const std::vector<int32_t>& rp = c.h_mrow;
-    const std::vector<int32_t>& cl = c.h_mcol;
+int ilu_setup(Ctx& c, const SysView& sy) {
+    IluData& I = *sy.ilu;
+    if (I.ready) return VFVM_OK;
+    const int64_t n = sy.n;
+    const std::vector<int32_t>& rp = *sy.h_row;
+    const std::vector<int32_t>& cl = *sy.h_col;
     // symmetrised adjacency (rows of one colour must not touch each other in A or A^T)
     std::vector<int32_t> tcnt(n + 1, 0);
     for (int64_t i = 0; i < n; ++i)
@@ -647,7 +648,7 @@ int ilu_setup(Ctx& c) {
         }
     }
     // colour-major CSR, columns permuted and ascending
-    std::vector<int32_t> prp(n + 1, 0), src(c.nnz), pcol(c.nnz), pdiag(n, -1);
+    std::vector<int32_t> prp(n + 1, 0), src(sy.nnz), pcol(sy.nnz), pdiag(n, -1);
     std::vector<std::pair<int32_t, int32_t>> tmp;
     for (int64_t t = 0; t < n; ++t) {
         const int32_t i = perm[t];
@@ -663,28 +664,29 @@ int ilu_setup(Ctx& c) {
         prp[t + 1] = base + (int32_t)tmp.size();
         if (pdiag[t] < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "ILU(0): a row has no diagonal entry");
     }
-    VFVM_HIP_TRY(c.ilu_perm.upload(perm.data(), perm.size(), c.stream));
-    VFVM_HIP_TRY(c.ilu_prp.upload(prp.data(), prp.size(), c.stream));
-    VFVM_HIP_TRY(c.ilu_src.upload(src.data(), src.size(), c.stream));
-    VFVM_HIP_TRY(c.ilu_pcol.upload(pcol.data(), pcol.size(), c.stream));
-    VFVM_HIP_TRY(c.ilu_pdiag.upload(pdiag.data(), pdiag.size(), c.stream));
-    VFVM_HIP_TRY(c.ilu_cptr.upload(cptr.data(), cptr.size(), c.stream));
+    VFVM_HIP_TRY(I.perm.upload(perm.data(), perm.size(), c.stream));
+    VFVM_HIP_TRY(I.prp.upload(prp.data(), prp.size(), c.stream));
+    VFVM_HIP_TRY(I.src.upload(src.data(), src.size(), c.stream));
+    VFVM_HIP_TRY(I.pcol.upload(pcol.data(), pcol.size(), c.stream));
+    VFVM_HIP_TRY(I.pdiag.upload(pdiag.data(), pdiag.size(), c.stream));
+    VFVM_HIP_TRY(I.cptr.upload(cptr.data(), cptr.size(), c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
-    c.h_color_ptr = cptr;
-    c.ilu_levels_ready = true;
+    I.h_color_ptr = cptr;
+    I.ready = true;
     return VFVM_OK;
 }
 
 // lu <- ILU(0) of A in multicolour order, values gathered from the CSR values
 template <class T>
-int ilu_factor(Ctx& c, T* lu, const double* val) {
-    hipLaunchKernelGGL((k_ilu_gather<T>), dim3(grid_for(c.nnz)), dim3(BLK), 0, c.stream, c.nnz, c.ilu_src.p, val, lu);
-    const int ncol = (int)c.h_color_ptr.size() - 1;
+int ilu_factor(Ctx& c, const SysView& sy, T* lu, const double* val) {
+    const IluData& I = *sy.ilu;
+    hipLaunchKernelGGL((k_ilu_gather<T>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, I.src.p, val, lu);
+    const int ncol = (int)I.h_color_ptr.size() - 1;
     for (int q = 0; q < ncol; ++q) {
-        int32_t s = c.h_color_ptr[q], e = c.h_color_ptr[q + 1];
+        int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e > s)
-            hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, s, e, c.ilu_prp.p,
-                               c.ilu_pcol.p, c.ilu_pdiag.p, lu);
+            hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, s, e, I.prp.p,
+                               I.pcol.p, I.pdiag.p, lu);
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
@@ -694,18 +696,19 @@ int ilu_factor(Ctx& c, T* lu, const double* val) {
 // permuted work vector zp; rhs is gathered by the forward sweep and z scattered by the
 // backward sweep
 template <class T>
-int ilu_apply(Ctx& c, const T* lu, const T* rhs, T* z, T* zp, const int* ctrl) {
+int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, T* z, T* zp, const int* ctrl) {
     constexpr int LPR = 4;
-    const int ncol = (int)c.h_color_ptr.size() - 1;
+    IluData& I = *sy.ilu;
+    const int ncol = (int)I.h_color_ptr.size() - 1;
     // algorithmic bytes of one sweep launch over r rows: the triangle's entries (value +
     // column, about half of each row), rp/diag/perm (12 B), rhs or z reads and writes
-    const double per_row_entries = 0.5 * (double)c.nnz / (double)c.n * (sizeof(T) + 4.0);
-    if (c.ilu_coop_blocks < 0) {
+    const double per_row_entries = 0.5 * (double)sy.nnz / (double)sy.n * (sizeof(T) + 4.0);
+    if (I.coop_blocks < 0) {
         // The single cooperative launch is opt-in (VFVM_ILU_COOP=1): measured on MI355X at
         // 1M rows and 11 colours it takes 2.5 ms per application against 0.17 ms for the
         // 22 per-colour launches, the grid-wide barrier being far costlier than a kernel
         // boundary.
-        c.ilu_coop_blocks = 0;
+        I.coop_blocks = 0;
         int coop = 0, ncu = 0, per_cu = 0;
         const char* on = getenv("VFVM_ILU_COOP");
         if (on && on[0] == '1' &&
@@ -713,43 +716,43 @@ int ilu_apply(Ctx& c, const T* lu, const T* rhs, T* z, T* zp, const int* ctrl) {
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess &&
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ilu_apply_coop<T, LPR>, BLK, 0) == hipSuccess &&
             per_cu > 0)
-            c.ilu_coop_blocks = std::min(per_cu, 4) * ncu;
+            I.coop_blocks = std::min(per_cu, 4) * ncu;
         (void)hipGetLastError();
     }
-    if (c.ilu_coop_blocks > 0) {
-        const int32_t* cptr = c.ilu_cptr.p;
-        const int32_t* perm = c.ilu_perm.p;
-        const int32_t* rp = c.ilu_prp.p;
-        const int32_t* cl = c.ilu_pcol.p;
-        const int32_t* dg = c.ilu_pdiag.p;
+    if (I.coop_blocks > 0) {
+        const int32_t* cptr = I.cptr.p;
+        const int32_t* perm = I.perm.p;
+        const int32_t* rp = I.prp.p;
+        const int32_t* cl = I.pcol.p;
+        const int32_t* dg = I.pdiag.p;
         int nc = ncol;
         void* args[] = {&nc, &cptr, &perm, &rp, &cl, &dg, (void*)&lu, (void*)&rhs, &zp, &z, (void*)&ctrl};
         hipEvent_t ev = nullptr;
         ktime_begin(c, VFVM_K_PRECOND, &ev);
         hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_ilu_apply_coop<T, LPR>),
-                                                  dim3(c.ilu_coop_blocks), dim3(BLK), args, 0, c.stream);
+                                                  dim3(I.coop_blocks), dim3(BLK), args, 0, c.stream);
         if (e == hipSuccess) {
-            ktime_end(c, VFVM_K_PRECOND, (double)c.n * (2.0 * per_row_entries + 24.0 + 5.0 * sizeof(T)), ev);
+            ktime_end(c, VFVM_K_PRECOND, (double)sy.n * (2.0 * per_row_entries + 24.0 + 5.0 * sizeof(T)), ev);
             return VFVM_OK;
         }
         // not launchable cooperatively (e.g. under a tool): fall back to one launch per colour
         (void)hipGetLastError();
-        c.ilu_coop_blocks = 0;
+        I.coop_blocks = 0;
         if (ev) c.ev_next--;
     }
     for (int q = 0; q < ncol; ++q) {
-        int32_t s = c.h_color_ptr[q], e = c.h_color_ptr[q + 1];
+        int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
         launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 2.0 * sizeof(T)),
                      k_ilu_lower<T, LPR>, dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK),
-                     s, e, c.ilu_perm.p, c.ilu_prp.p, c.ilu_pcol.p, c.ilu_pdiag.p, lu, rhs, zp, ctrl);
+                     s, e, I.perm.p, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, zp, ctrl);
     }
     for (int q = ncol - 1; q >= 0; --q) {
-        int32_t s = c.h_color_ptr[q], e = c.h_color_ptr[q + 1];
+        int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
         launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 3.0 * sizeof(T)),
                      k_ilu_upper<T, LPR>, dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK),
-                     s, e, c.ilu_perm.p, c.ilu_prp.p, c.ilu_pcol.p, c.ilu_pdiag.p, lu, zp, z, ctrl);
+                     s, e, I.perm.p, I.prp.p, I.pcol.p, I.pdiag.p, lu, zp, z, ctrl);
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
@@ -758,7 +761,7 @@ int ilu_apply(Ctx& c, const T* lu, const T* rhs, T* z, T* zp, const int* ctrl) {
 // One restart cycle of GMRES on A M^-1 (M = D through the scaled matrix for Jacobi,
 // M = LU for ILU(0)) from V[0] (normalised).  Leaves ctrl[1] = k steps, y = H^-1 g.
 template <class T>
-int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, T* zp, const T* aval, const T* lu, bool reorth, int check_every,
+int arnoldi_cycle(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, T* zp, const T* aval, const T* lu, bool reorth, int check_every,
                   int* steps_out) {
     const int64_t n = w.n;
     const int lpr = pick_lpr(n, w.nnz);
@@ -772,10 +775,10 @@ int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, T* zp, const T* aval, con
         T* vj = V + (int64_t)j * n;
         const T* xin = vj;
         if (lu) {
-            if (int st = ilu_apply<T>(c, lu, vj, zv, zp, w.ctrl)) return st;
+            if (int st = ilu_apply<T>(c, sy, lu, vj, zv, zp, w.ctrl)) return st;
             xin = zv;
         }
-        if (int st = spmv_launch<T>(c, lpr, n, aval, xin, wv, w.ctrl, bytes_spmv)) return st;
+        if (int st = spmv_launch<T>(c, sy, lpr, n, aval, xin, wv, w.ctrl, bytes_spmv)) return st;
         const int nv = j + 1;
         const double bytes_upd = (double)sizeof(T) * n * (nv + 2);
         for (int pass = 0; pass < (reorth ? 2 : 1); ++pass) {
@@ -817,14 +820,39 @@ int arnoldi_cycle(Ctx& c, Work& w, T* V, T* wv, T* zv, T* zp, const T* aval, con
 
 }  // namespace
 
-int residual_norm(Ctx& c, const double* x_dev, double* rel, double* bnorm) {
-    const int64_t n = c.n;
-    const int lpr = pick_lpr(n, c.nnz);
+SysView primal_view(Ctx& c) {
+    SysView s;
+    s.n = c.n; s.nnz = c.nnz;
+    s.row = c.mrow.p; s.col = c.mcol.p; s.dslot = c.dslot.p;
+    s.vals = c.vals.p; s.b = c.b.p;
+    s.h_row = &c.h_mrow; s.h_col = &c.h_mcol;
+    s.ilu = &c.ilu;
+    return s;
+}
+
+SysView transpose_view(Ctx& c) {
+    SysView s;
+    s.n = c.n; s.nnz = c.nnz;
+    s.row = c.trow.p; s.col = c.tcol.p; s.dslot = c.tdslot.p;
+    s.vals = c.tvals.p; s.b = c.tb.p;
+    s.h_row = &c.h_trow; s.h_col = &c.h_tcol;
+    s.ilu = &c.tilu;
+    return s;
+}
+
+int solver_setup(Ctx& c, const SysView& sy, const vfvm_solve_opts& o) {
+    if (o.precond == VFVM_PRECOND_ILU0) return ilu_setup(c, sy);
+    return VFVM_OK;
+}
+
+int residual_norm(Ctx& c, const SysView& sy, const double* x_dev, double* rel, double* bnorm) {
+    const int64_t n = sy.n;
+    const int lpr = pick_lpr(n, sy.nnz);
     const int nblk_r = (int)grid_for(n * lpr);
     VFVM_HIP_TRY(c.r64.alloc(n));
     DevBuf<double> part;
     VFVM_HIP_TRY(part.alloc(2 * (size_t)nblk_r));
-    if (int st = residual_launch(c, lpr, x_dev, c.r64.p, part.p, nblk_r)) return st;
+    if (int st = residual_launch(c, sy, lpr, x_dev, c.r64.p, part.p, nblk_r)) return st;
     std::vector<double> h(2 * (size_t)nblk_r);
     VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
@@ -835,7 +863,7 @@ int residual_norm(Ctx& c, const double* x_dev, double* rel, double* bnorm) {
     return VFVM_OK;
 }
 
-int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& st) {
+int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& st) {
     std::memset(&st, 0, sizeof(st));
     if (!(o.tol > 0.0)) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_TOLERANCE, "GMRES: tolerance must be positive");
     if (o.restart <= 0 || o.restart > MAX_RESTART) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_RESTART, "GMRES: restart must be in [1, 256]");
@@ -845,11 +873,11 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     if (o.ortho < VFVM_ORTHO_AUTO || o.ortho > VFVM_ORTHO_CGS2) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown orthogonalisation");
     const bool mixed = o.precision == VFVM_FP32_REFINED;
     const bool reorth = o.ortho == VFVM_ORTHO_CGS2 || (o.ortho == VFVM_ORTHO_AUTO && !mixed);
-    const int64_t n = c.n;
+    const int64_t n = sy.n;
     const int m = o.restart;
     Work w;
     w.n = n;
-    w.nnz = c.nnz;
+    w.nnz = sy.nnz;
     w.m = m;
     w.nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 4 * BLK - 1) / (4 * BLK)));
     w.rows_per_blk = (n + w.nblk - 1) / w.nblk;
@@ -857,7 +885,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     w.nwav = (w.nwav + BLK / WAVE - 1) / (BLK / WAVE) * (BLK / WAVE);
     w.rows_per_wave = (n + w.nwav - 1) / w.nwav;
     w.L.m = m;
-    const int lpr = pick_lpr(n, c.nnz);
+    const int lpr = pick_lpr(n, sy.nnz);
     const int nblk_r = (int)grid_for(n * lpr);
 
     // workspace
@@ -874,7 +902,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     const bool ilu = o.precond == VFVM_PRECOND_ILU0;
     const bool jac = o.precond == VFVM_PRECOND_JACOBI;
     if (ilu) {
-        if (int s = ilu_setup(c)) return s;
+        if (int s = ilu_setup(c, sy)) return s;
         if (mixed) VFVM_HIP_TRY(c.zp32.alloc(n));
         else VFVM_HIP_TRY(c.zp64.alloc(n));
     }
@@ -889,31 +917,31 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     VFVM_HIP_TRY(hipEventRecord(ev.e[0], c.stream));
 
     // preconditioner set-up (inside the timed solve)
-    hipLaunchKernelGGL(k_dinv, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.dslot.p, c.vals.p, c.dinv64.p, jac ? 1 : 0);
+    hipLaunchKernelGGL(k_dinv, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.dslot, sy.vals, c.dinv64.p, jac ? 1 : 0);
     const double* dinv_x = jac ? c.dinv64.p : nullptr;
     const void* aval = nullptr;
     const void* lu = nullptr;
     if (!mixed) {
         if (jac) {
-            VFVM_HIP_TRY(c.avals64.alloc(c.nnz));
-            hipLaunchKernelGGL((k_scale_cols<double>), dim3(grid_for(c.nnz)), dim3(BLK), 0, c.stream, c.nnz, c.mcol.p, c.vals.p, c.dinv64.p, c.avals64.p);
+            VFVM_HIP_TRY(c.avals64.alloc(sy.nnz));
+            hipLaunchKernelGGL((k_scale_cols<double>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, sy.col, sy.vals, c.dinv64.p, c.avals64.p);
             aval = c.avals64.p;
         } else {
-            aval = c.vals.p;
+            aval = sy.vals;
         }
         if (ilu) {
-            VFVM_HIP_TRY(c.ilu64.alloc(c.nnz));
-            if (int s = ilu_factor<double>(c, c.ilu64.p, c.vals.p)) return s;
+            VFVM_HIP_TRY(c.ilu64.alloc(sy.nnz));
+            if (int s = ilu_factor<double>(c, sy, c.ilu64.p, sy.vals)) return s;
             lu = c.ilu64.p;
         }
     } else {
         // FP32 copy of A D^-1 (Jacobi) or of A (ILU(0), none); dinv = 1 unless Jacobi
-        VFVM_HIP_TRY(c.avals32.alloc(c.nnz));
-        hipLaunchKernelGGL((k_scale_cols<float>), dim3(grid_for(c.nnz)), dim3(BLK), 0, c.stream, c.nnz, c.mcol.p, c.vals.p, c.dinv64.p, c.avals32.p);
+        VFVM_HIP_TRY(c.avals32.alloc(sy.nnz));
+        hipLaunchKernelGGL((k_scale_cols<float>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, sy.col, sy.vals, c.dinv64.p, c.avals32.p);
         aval = c.avals32.p;
         if (ilu) {
-            VFVM_HIP_TRY(c.ilu32.alloc(c.nnz));
-            if (int s = ilu_factor<float>(c, c.ilu32.p, c.vals.p)) return s;
+            VFVM_HIP_TRY(c.ilu32.alloc(sy.nnz));
+            if (int s = ilu_factor<float>(c, sy, c.ilu32.p, sy.vals)) return s;
             lu = c.ilu32.p;
         }
     }
@@ -928,7 +956,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
     const int max_ref = o.max_refine > 0 ? o.max_refine : o.max_outer;
     for (;;) {
         // true FP64 residual r = b - A x, beta = ||r||
-        if (int s = residual_launch(c, lpr, x, c.r64.p, part_r, nblk_r)) return s;
+        if (int s = residual_launch(c, sy, lpr, x, c.r64.p, part_r, nblk_r)) return s;
         hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(BLK), 0, c.stream, nblk_r, part_r, w.S, w.L, w.ctrl, 0.0, 0);
         VFVM_HIP_TRY(hipMemcpyAsync(c.h_pinned + 8, w.S + w.L.misc(), 8 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
@@ -945,12 +973,12 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
         int steps = 0;
         if (!mixed) {
             hipLaunchKernelGGL((k_scale<double, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V64.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<double>(c, w, c.V64.p, c.w64.p, c.z64.p, c.zp64.p, (const double*)aval, (const double*)lu, reorth, check_every, &steps)) return s;
+            if (int s = arnoldi_cycle<double>(c, sy, w, c.V64.p, c.w64.p, c.z64.p, c.zp64.p, (const double*)aval, (const double*)lu, reorth, check_every, &steps)) return s;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y)
             hipLaunchKernelGGL((k_combine<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V64.p, n, w.S, w.L, w.ctrl, c.w64.p);
             if (lu) {
-                if (int s = ilu_apply<double>(c, (const double*)lu, (const double*)c.w64.p, c.z64.p, c.zp64.p, (const int*)nullptr)) return s;
+                if (int s = ilu_apply<double>(c, sy, (const double*)lu, (const double*)c.w64.p, c.z64.p, c.zp64.p, (const int*)nullptr)) return s;
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
             } else {
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.w64.p, dinv_x, x);
@@ -958,7 +986,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
         } else {
             // inner FP32 solve of A M^-1 u = r from u = 0, V0 = r / beta in FP32
             hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V32.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<float>(c, w, c.V32.p, c.w32.p, c.w32.p + n, c.zp32.p, (const float*)aval, (const float*)lu, reorth, check_every, &steps)) return s;
+            if (int s = arnoldi_cycle<float>(c, sy, w, c.V32.p, c.w32.p, c.w32.p + n, c.zp32.p, (const float*)aval, (const float*)lu, reorth, check_every, &steps)) return s;
             ++refinements;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y), V y accumulated in FP64
@@ -967,7 +995,7 @@ int gmres_solve(Ctx& c, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& s
                 float* t32 = c.w32.p;
                 float* z32 = c.w32.p + n;
                 hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, t32, w.S, w.L.misc() + 4, (const int*)nullptr);
-                if (int s = ilu_apply<float>(c, (const float*)lu, (const float*)t32, z32, c.zp32.p, (const int*)nullptr)) return s;
+                if (int s = ilu_apply<float>(c, sy, (const float*)lu, (const float*)t32, z32, c.zp32.p, (const int*)nullptr)) return s;
                 hipLaunchKernelGGL((k_scale<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z32, c.z64.p, w.S, w.L.misc() + 4, (const int*)nullptr);
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
             } else {

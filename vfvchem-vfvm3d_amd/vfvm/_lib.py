@@ -74,6 +74,18 @@ class DriverStats(C.Structure):
                 ("time_assembly", C.c_double), ("time_solve", C.c_double)]
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("checkpoints", C.c_int32), ("forward_iterations", C.c_int32),
+                ("recompute_iterations", C.c_int32), ("adjoint_iterations", C.c_int32), ("recomputed_steps", C.c_int32),
+                ("replay_mismatches", C.c_int32), ("reserved", C.c_int32),
+                ("forward_seconds", C.c_double), ("backward_seconds", C.c_double), ("recompute_seconds", C.c_double),
+                ("adjoint_solve_seconds", C.c_double), ("setup_seconds", C.c_double)]
+
+
+class AdjointWrt(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("id", C.c_int32), ("grad", C.POINTER(C.c_double))]
+
+
 _P = C.c_void_p
 _I = C.c_int32
 _L = C.c_int64
@@ -113,6 +125,8 @@ SIGNATURES = [
     ("vfvm_residual", _I, [_P, _PD, _PD]),
     ("vfvm_steady_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _I, _D, _D, C.POINTER(DriverStats)]),
     ("vfvm_transient_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, _D, _D, C.POINTER(DriverStats)]),
+    ("vfvm_adjoint_forward", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, C.POINTER(AdjointStats)]),
+    ("vfvm_adjoint_backward", _I, [_P, C.POINTER(SolveOpts), _PD, _I, C.POINTER(AdjointWrt), C.POINTER(AdjointStats)]),
     ("vfvm_timing_enable", _I, [_P, _I]),
     ("vfvm_timing_get", _I, [_P, _I, _PL, _PD, _PD]),
     ("vfvm_timing_reset", _I, [_P]),

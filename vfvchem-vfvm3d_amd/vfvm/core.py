@@ -27,6 +27,7 @@ FP64, FP32_REFINED = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_ILU0 = 0, 1, 2
 K_SPMV, K_MDOT, K_ASSEMBLE, K_PRECOND, K_UPDATE = 0, 1, 2, 3, 4
 ORTHO_AUTO, ORTHO_CGS, ORTHO_CGS2 = 0, 1, 2
+WRT_SCLDOM, WRT_SCLBND, WRT_INITIAL = 0, 1, 2
 
 _MESH_ARRAYS = {
     "pos": (0, np.float64), "tets": (1, np.int32), "bnd_ptr": (2, np.int64), "bnd_nid": (3, np.int32),
@@ -134,6 +135,7 @@ class Context:
         self._h = C.c_void_p()
         check(lib.vfvm_create(int(device), C.byref(self._h)))
         self.mesh = None
+        self._sclbnd_size = []
 
     def close(self):
         if self._h:
@@ -149,6 +151,7 @@ class Context:
     def upload(self, mesh: Mesh):
         check(lib.vfvm_mesh_upload(self._h, mesh._h))
         self.mesh = mesh
+        self._sclbnd_size = []
 
     # fields
     def scldom_add(self, kind, cnst=0.0, values=None) -> int:
@@ -184,6 +187,7 @@ class Context:
         i = C.c_int32()
         v = None if values is None else np.ascontiguousarray(values, np.float64)
         check(lib.vfvm_sclbnd_add(self._h, int(bnd), float(cnst), None if v is None else ptr(v), C.byref(i)))
+        self._sclbnd_size.append(len(self.mesh.bnd_nodes(bnd)))
         return i.value
 
     def sclbnd_set(self, fid, values):
@@ -255,6 +259,36 @@ class Context:
         check(lib.vfvm_transient_solve(self._h, arr, len(ops), C.byref(o), float(dt), int(num_ts), int(max_iter), float(tol),
                                        float(damp), C.byref(st)))
         return st
+
+    # discrete adjoint
+    def adjoint_forward(self, ops, dt=0.0, num_ts=0, checkpoint_every=10, lin=None):
+        """Forward pass of the linear steady (num_ts=0) or implicit-Euler map, keeping
+        every checkpoint_every-th state on the device; leaves T^num_ts in the unknown."""
+        arr = make_ops(ops)
+        st = L.AdjointStats()
+        o = solve_opts(**(lin or {}))
+        check(lib.vfvm_adjoint_forward(self._h, arr, len(ops), C.byref(o), float(dt), int(num_ts), int(checkpoint_every),
+                                       C.byref(st)))
+        return st
+
+    def adjoint_backward(self, seed, wrt, lin=None):
+        """Backward pass for dJ/dT^num_ts = seed.  wrt: list of (kind, id) with kind one
+        of WRT_SCLDOM, WRT_SCLBND, WRT_INITIAL.  Returns (list of gradients, stats)."""
+        n = self.mesh.num_tot
+        seed = np.ascontiguousarray(seed, np.float64)
+        if seed.shape != (n,):
+            raise ValueError(f"seed must have {n} entries")
+        outs = []
+        arr = (L.AdjointWrt * max(len(wrt), 1))()
+        for i, (kind, fid) in enumerate(wrt):
+            size = self._sclbnd_size[fid] if kind == WRT_SCLBND else n
+            g = np.zeros(size)
+            outs.append(g)
+            arr[i].kind, arr[i].id, arr[i].grad = int(kind), int(fid), ptr(g)
+        st = L.AdjointStats()
+        o = solve_opts(**(lin or {}))
+        check(lib.vfvm_adjoint_backward(self._h, C.byref(o), ptr(seed), len(wrt), arr, C.byref(st)))
+        return outs, st
 
     # timing
     def timing_enable(self, mask=-1):
