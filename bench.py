@@ -9,6 +9,13 @@ A step is one linear solve of the assembled system from x0 = 0 to
 included, with the matrix, b and x resident in HBM.  Mesh, dual and assembly
 are built before the timed region (assembly time is reported beside it).
 
+Beside it, the adjoint line of the transient heat workload (126^3 jittered lattice,
+2,000,376 vertices, lognormal conductivity from a seeded Gaussian random field with
+correlation length 0.1, implicit Euler dt = 1e-3 for 200 steps, J = mean T on the x = 1
+face): one forward pass with a checkpoint every 10 steps and one backward pass (segment
+replays + transposed GMRES + assembly VJP) for dJ/dk at every vertex, FP32-inner /
+FP64-refined solves, reported as the backward/forward time ratio.
+
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
@@ -65,6 +72,66 @@ def build_problem(vfvm, n, D=1e-2):
     return mesh, ctx, ops
 
 
+def build_heat(vfvm, n, seed=0):
+    """Transient heat on the jittered lattice (TransientHeat operator order,
+    transient_heat.rs:152-171): rho c_p = 1, lognormal k, no source, T = 1 at x = 0, zero
+    heat flux on the other faces, T0 = 0."""
+    from vfvm.fields import lognormal_conductivity
+    mesh = vfvm.Mesh.lattice(n, seed, 0.3)
+    k = lognormal_conductivity(mesh.get("pos"), ell=0.1, modes=1024, seed=seed)
+    ctx = vfvm.Context(0)
+    ctx.upload(mesh)
+    T = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0)
+    w = ctx.scldom_add(vfvm.FIELD_CONSTANT, 1.0)
+    kk = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, k)
+    hp = ctx.scldom_add(vfvm.FIELD_CONSTANT, 0.0)
+    hs = ctx.scldom_add(vfvm.FIELD_CONSTANT, 0.0)
+    tin = ctx.sclbnd_add(0, 1.0)
+    ops = [(vfvm.OP_SCL_DOM_TIME, 0, T, w, -1), (vfvm.OP_SCL_DOM_DIFFUSION, 0, T, kk, T),
+           (vfvm.OP_SCL_DOM_PRODUCTION, 0, T, hp, -1), (vfvm.OP_SCL_DOM_SOURCE, 0, T, hs, -1),
+           (vfvm.OP_SCL_BND_DIRICHLET, 0, T, tin, -1)]
+    for b in (1, 2, 3, 4, 5):
+        q = ctx.sclbnd_add(b, 0.0)
+        ops += [(vfvm.OP_SCL_BND_TIME, b, T, w, -1), (vfvm.OP_SCL_BND_DIFFUSION, b, T, kk, T),
+                (vfvm.OP_SCL_BND_PRODUCTION, b, T, hp, -1), (vfvm.OP_SCL_BND_SOURCE, b, T, hs, -1),
+                (vfvm.OP_SCL_BND_NEUMANN, b, T, q, -1)]
+    return mesh, ctx, ops, T, kk
+
+
+def run_adjoint(vfvm, args):
+    """Forward + backward of the transient heat workload; the adjoint/forward time ratio."""
+    import numpy as np
+    t0 = time.time()
+    mesh, ctx, ops, T, kk = build_heat(vfvm, args.adjoint_n)
+    t_setup = time.time() - t0
+    N = mesh.num_tot
+    lin = dict(tol=args.tol, restart=args.restart, max_outer=2000, precision=vfvm.FP32_REFINED,
+               precond=vfvm.PRECOND_ILU0, inner_tol=args.inner_tol, ortho=1)
+    p = mesh.get("bnd_ptr")
+    x1 = mesh.num_pts + mesh.get("bnd_nid")[p[1]:p[2]].astype(np.int64)
+    seed = np.zeros(N)
+    seed[x1] = 1.0 / x1.size
+    fst = ctx.adjoint_forward(ops, dt=1e-3, num_ts=args.adjoint_steps, checkpoint_every=args.adjoint_every, lin=lin)
+    J = float(seed @ ctx.scldom_get(T))
+    (gk,), st = ctx.adjoint_backward(seed, [(vfvm.WRT_SCLDOM, kk)], lin=lin)
+    ctx.close()
+    return {
+        "workload": f"transient heat, {args.adjoint_n}^3 jittered lattice ({N} vertices), lognormal k (RFF, l=0.1, "
+                    f"1024 modes, seed 0), implicit Euler dt=1e-3 x {args.adjoint_steps} steps, J = mean T on x=1, "
+                    f"dJ/dk per vertex, checkpoint every {args.adjoint_every}, GMRES({args.restart}) + ilu0 "
+                    f"FP32-inner/FP64-refined, tol {args.tol}",
+        "forward_s": fst.forward_seconds, "backward_s": st.backward_seconds,
+        "adjoint_over_forward": st.backward_seconds / fst.forward_seconds,
+        "recompute_s": st.recompute_seconds, "transposed_solve_s": st.adjoint_solve_seconds,
+        "setup_s": st.setup_seconds, "forward_iters": fst.forward_iterations,
+        "recompute_iters": st.recompute_iterations, "adjoint_iters": st.adjoint_iterations,
+        "recomputed_steps": st.recomputed_steps, "checkpoints": st.checkpoints,
+        "replay_mismatches": st.replay_mismatches, "J": J, "grad_abs_max": float(np.abs(gk).max()),
+        "grad_sum": float(gk.sum()), "mesh_and_field_s": t_setup,
+        "forward_dof_steps_per_s": N * args.adjoint_steps / fst.forward_seconds,
+    }
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -87,6 +154,10 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--cpu-sample-steps", type=int, default=50, help="Arnoldi steps timed on the CPU (0: skip)")
+    ap.add_argument("--adjoint-n", type=int, default=126, help="lattice points per axis of the adjoint workload")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=10)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--variants", default="fp64,mixed")
     ap.add_argument("--ortho", default="cgs", choices=["auto", "cgs", "cgs2"])
     args = ap.parse_args()
@@ -183,8 +254,12 @@ def main():
                          f"same {N}-DOF system ({tcpu:.1f} s), extrapolated to the {need} steps the FP64 solve needs",
                "cpu": cpu_model(), "nproc": os.cpu_count()}
 
+    ctx.close()
+    adjoint = None if args.no_adjoint else run_adjoint(vfvm, args)
+
     line = {
-        "metric": "solved DOF/s at 1e-10 rel residual (FP32-inner/FP64-refined GMRES; FP64 GMRES beside it)",
+        "metric": "solved DOF/s at 1e-10 rel residual (FP32-inner/FP64-refined GMRES; FP64 GMRES beside it); "
+                  "adjoint/forward time ratio in 'adjoint'",
         "value": value,
         "unit": "DOF/s",
         "n_gpus": 1,
@@ -208,6 +283,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(line))
 

@@ -232,6 +232,13 @@ int vfvm_solve(vfvm_ctx* ctx, const vfvm_solve_opts* opts, double* x, vfvm_solve
 int vfvm_solve_device(vfvm_ctx* ctx, const vfvm_solve_opts* opts, double* x_dev, vfvm_solve_stats* stats);
 /* r = b - A x in FP64 for a host x; returns ||r||_2 / ||b||_2. */
 int vfvm_residual(vfvm_ctx* ctx, const double* x, double* rel_residual);
+/* z = M^-1 r for the assembled matrix with the preconditioner of opts (precond, and
+ * precision: FP32_REFINED factors and applies ILU(0) in FP32), host arrays.  No reference
+ * counterpart (the reference's Jacobi lives inside faer_gmres); exposed for parity tests. */
+int vfvm_precond_apply(vfvm_ctx* ctx, const vfvm_solve_opts* opts, const double* r, double* z);
+/* The ILU(0) row order of the built pattern: perm[t] = row of A at position t (rows sorted
+ * by (colour, index)), color_ptr[ncolors + 1] = first position of each colour. */
+int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, int32_t* ncolors);
 
 /* ---- drivers ---- */
 typedef struct vfvm_driver_stats {

@@ -743,3 +743,29 @@ def adjoint_initial_gradient(g, ops, bnd_nid, acc, num_ts):
         if op.kind == Op.SCL_BND_DIRICHLET:
             out[g["num_pts"] + np.asarray(bnd_nid[op.geom_id], np.int64)] = 0.0
     return out
+
+
+# --------------------------------------------------------------------------- ILU(0)
+def ilu0(A, perm):
+    """ILU(0) of P A P^T (row t of P A P^T = row perm[t] of A) by the IKJ variant
+    (Saad, Iterative Methods for Sparse Linear Systems, Alg. 10.4): returns (L, U) with a
+    unit lower L and LU = P A P^T on the pattern of P A P^T.  Dense row updates; small n."""
+    import scipy.sparse as sp
+    n = A.shape[0]
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), perm)), shape=(n, n))
+    Ap = (P @ A @ P.T).tocsr()
+    Ap.sort_indices()
+    W = Ap.toarray()
+    S = Ap.copy()
+    S.data[:] = 1.0
+    pat = S.toarray() != 0.0  # the stored pattern, explicit zeros included
+    for i in range(Ap.shape[0]):
+        for k in Ap.indices[Ap.indptr[i]:Ap.indptr[i + 1]]:
+            if k >= i:
+                break
+            W[i, k] /= W[k, k]
+            upd = pat[i, k + 1:]
+            W[i, k + 1:][upd] -= W[i, k] * W[k, k + 1:][upd]
+    L = sp.csr_matrix(np.tril(W, -1) * pat + np.eye(n))
+    U = sp.csr_matrix(np.triu(W) * pat)
+    return L, U

@@ -173,26 +173,41 @@ def test_c2_full_size_uniform_solution(ctx):
         assert np.abs(x - 1.0).max() <= 1e-3
 
 
-def test_ilu_single_launch_equals_per_colour():
-    """The cooperative single-launch ILU(0) application and the per-colour launches run the
-    same per-row sums in the same order: bitwise equal solutions."""
-    import os
-    print("[ilu launch modes]", flush=True)
-    p = mass_c2(vfvm.Mesh.lattice(12), c_out=0.0)
-    outs = []
-    for mode in ("0", "1"):
-        os.environ["VFVM_ILU_COOP"] = mode
-        try:
-            c = vfvm.Context(0)
-            p.to_device(c)
-            c.pattern_build(p.ops)
-            c.assemble(p.ops)
-            x, st = c.solve(None, tol=1e-12, precond=vfvm.PRECOND_ILU0, precision=vfvm.FP64)
-            outs.append(x)
-            c.close()
-        finally:
-            os.environ.pop("VFVM_ILU_COOP", None)
-    assert np.array_equal(outs[0], outs[1])
+@pytest.mark.parametrize("prob", ["heat", "mass"])
+def test_ilu0_application_vs_oracle(ctx, prob):
+    """M^-1 r of the device ILU(0) against the oracle's ILU(0) (Saad Alg. 10.4, IKJ) of the
+    same matrix in the device's multicolour row order; the oracle's factors are pinned by
+    (LU)_ij = A_ij on the pattern."""
+    print(f"[ilu0 {prob}]", flush=True)
+    mesh = vfvm.Mesh.lattice(8)
+    p = heat_c1(mesh) if prob == "heat" else mass_c2(mesh, c_out=0.0, prod=0.3)
+    rp, col, vals, b = _assembled(ctx, p)
+    perm, cptr = ctx.ilu_order()
+    n = len(rp) - 1
+    assert np.array_equal(np.sort(perm), np.arange(n)) and cptr[0] == 0 and cptr[-1] == n
+    A = sp.csr_matrix((vals, col, rp), shape=(n, n))
+    # no two rows of one colour are coupled (in A or A^T)
+    colour = np.empty(n, np.int64)
+    for q in range(len(cptr) - 1):
+        colour[perm[cptr[q]:cptr[q + 1]]] = q
+    coo = A.tocoo()
+    off = coo.row != coo.col
+    assert np.all(colour[coo.row[off]] != colour[coo.col[off]])
+    L, U = O.ilu0(A, perm)
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), perm)), shape=(n, n))
+    Ap = (P @ A @ P.T).tocsr()
+    LU = (L @ U).tocsr()
+    mask = Ap.copy()
+    mask.data[:] = 1.0
+    assert np.abs((LU.multiply(mask) - Ap).data).max() <= 1e-12 * np.abs(Ap.data).max()
+    r = np.random.default_rng(4).normal(size=n)
+    z_ref = P.T @ spla.spsolve_triangular(U, spla.spsolve_triangular(L, P @ r, lower=True, unit_diagonal=True), lower=False)
+    z = ctx.precond_apply(r, precond=vfvm.PRECOND_ILU0, precision=vfvm.FP64)
+    assert np.abs(z - z_ref).max() <= 1e-12 * np.abs(z_ref).max()
+    z32 = ctx.precond_apply(r, precond=vfvm.PRECOND_ILU0, precision=vfvm.FP32_REFINED)
+    assert np.abs(z32 - z_ref).max() <= 1e-4 * np.abs(z_ref).max()
+    zj = ctx.precond_apply(r, precond=vfvm.PRECOND_JACOBI)
+    assert np.allclose(zj, r / A.diagonal(), rtol=1e-15, atol=0)
 
 
 def test_reference_api_mirror_steady_heat():

@@ -303,6 +303,7 @@ extern "C" int vfvm_scldom_set(vfvm_ctx* ctx, int32_t id, const double* values) 
     CTX_GUARD(ctx);
     if (!check_scl(c, id) || !values) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_scldom_set: invalid argument");
     VFVM_HIP_TRY(hipMemcpyAsync(c.scl[id].value.p, values, c.g.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    c.scl[id].grad_valid = false;
     if (int st = sync_err(c)) return st;
     return clear_error();
 }
@@ -346,6 +347,7 @@ extern "C" int vfvm_vecdom_set(vfvm_ctx* ctx, int32_t id, const double* values) 
     CTX_GUARD(ctx);
     if (id < 0 || id >= (int32_t)c.vec.size() || !values) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_vecdom_set: invalid argument");
     VFVM_HIP_TRY(hipMemcpyAsync(c.vec[id].value.p, values, 3 * c.g.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    c.vec[id].grad_valid = false;
     if (int st = sync_err(c)) return st;
     return clear_error();
 }
@@ -492,12 +494,20 @@ int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
     }
     for (size_t f = 0; f < c.flw.size(); ++f)
         if (need_f[f]) { need_s[c.flw[f].den] = 1; need_v[c.flw[f].vel] = 1; }
-    for (size_t i = 0; i < c.scl.size(); ++i)
-        if (need_s[i] && c.scl[i].kind != VFVM_FIELD_CONSTANT)
-            if (int st = launch_grad(c, c.scl[i].value.p, c.scl[i].grad.p)) return st;
-    for (size_t i = 0; i < c.vec.size(); ++i)
-        if (need_v[i] && c.vec[i].kind != VFVM_FIELD_CONSTANT)
-            if (int st = launch_grad_vec(c, c.vec[i].value.p, c.vec[i].grad.p)) return st;
+    // a function field's gradient is kept until its values are set again; an unknown's
+    // values change with every solve
+    for (size_t i = 0; i < c.scl.size(); ++i) {
+        SclDom& f = c.scl[i];
+        if (!need_s[i] || f.kind == VFVM_FIELD_CONSTANT || (f.kind == VFVM_FIELD_FUNCTION && f.grad_valid)) continue;
+        if (int st = launch_grad(c, f.value.p, f.grad.p)) return st;
+        f.grad_valid = f.kind == VFVM_FIELD_FUNCTION;
+    }
+    for (size_t i = 0; i < c.vec.size(); ++i) {
+        VecDom& f = c.vec[i];
+        if (!need_v[i] || f.kind == VFVM_FIELD_CONSTANT || (f.kind == VFVM_FIELD_FUNCTION && f.grad_valid)) continue;
+        if (int st = launch_grad_vec(c, f.value.p, f.grad.p)) return st;
+        f.grad_valid = f.kind == VFVM_FIELD_FUNCTION;
+    }
     for (size_t f = 0; f < c.flw.size(); ++f)
         if (need_f[f]) if (int st = launch_flux_prescribed(c, c.flw[f])) return st;
     DevProgram p;
@@ -572,6 +582,21 @@ extern "C" int vfvm_residual(vfvm_ctx* ctx, const double* x, double* rel_residua
     VFVM_HIP_TRY(hipMemcpyAsync(c.x64.p, x, c.n * sizeof(double), hipMemcpyHostToDevice, c.stream));
     double bn = 0.0;
     if (int st = residual_norm(c, primal_view(c), c.x64.p, rel_residual, &bn)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_precond_apply(vfvm_ctx* ctx, const vfvm_solve_opts* opts, const double* r, double* z) {
+    CTX_GUARD(ctx);
+    if (!opts || !r || !z) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_precond_apply: NULL argument");
+    if (!c.has_matrix) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_precond_apply: nothing assembled");
+    if (int st = precond_apply(c, *opts, r, z)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, int32_t* ncolors) {
+    CTX_GUARD(ctx);
+    if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_ilu_order: no pattern built");
+    if (int st = ilu_order(c, perm, color_ptr, ncolors)) return st;
     return clear_error();
 }
 

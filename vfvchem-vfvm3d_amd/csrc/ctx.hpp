@@ -103,9 +103,8 @@ struct DevProgram {
 // permuted system is row perm[t] of the matrix, rows sorted by (colour, index); entries
 // [L | diag | U] with permuted columns; src[k] = CSR slot of permuted entry k.
 struct IluData {
-    DevBuf<int32_t> perm, prp, src, pcol, pdiag, cptr;
+    DevBuf<int32_t> perm, prp, src, pcol, pdiag;
     std::vector<int32_t> h_color_ptr;
-    int coop_blocks = -1;  // blocks of the cooperative sweep launch (0: per-colour launches)
     bool ready = false;
 };
 
@@ -178,7 +177,7 @@ struct Ctx {
     // solver workspace (shared by both systems)
     DevBuf<double> V64, w64, z64, x64, r64, dinv64, avals64, part, scal;
     DevBuf<float> V32, w32, avals32;
-    DevBuf<double> ilu64, zp64;
+    DevBuf<double> ilu64, zp64;  // factor values, permuted work vector
     DevBuf<float> ilu32, zp32;
     double* h_pinned = nullptr;  // small pinned readback buffer
 
@@ -211,6 +210,9 @@ SysView transpose_view(Ctx& c);
 int gmres_solve(Ctx& c, const SysView& s, const vfvm_solve_opts& o, double* x_dev, vfvm_solve_stats& st);
 // host-side preconditioner ordering of a pattern (ILU(0) colouring), done once per pattern
 int solver_setup(Ctx& c, const SysView& s, const vfvm_solve_opts& o);
+// z = M^-1 r for the assembled matrix (host arrays); the ILU(0) row order
+int precond_apply(Ctx& c, const vfvm_solve_opts& o, const double* r_host, double* z_host);
+int ilu_order(Ctx& c, int32_t* perm, int32_t* color_ptr, int32_t* ncolors);
 int residual_norm(Ctx& c, const SysView& s, const double* x_dev, double* rel, double* bnorm);
 
 // drivers and adjoint set-up shared across translation units (api_ctx.hip)

@@ -18,23 +18,51 @@ constexpr int BLK = 256;
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 
-// ScalarDomain::update_grad (src/base/scl_dom.rs:146-164)
+// ScalarDomain::update_grad (src/base/scl_dom.rs:146-164): grad u_c = sum_loc w_loc (u_n - u_c)
+// accumulated in loc order.  LPR lanes per vertex load the edge weights of a row
+// contiguously; the sum walks the lanes' terms with shuffles in loc order (bit-identical to
+// a sequential walk).  Rows of degree > 2 LPR take the sequential walk in their first lane.
+template <int LPR>
 __global__ void __launch_bounds__(BLK) k_grad(int64_t N, int64_t E, const int32_t* __restrict__ rp,
                                               const int32_t* __restrict__ ngh, const double* __restrict__ gw,
                                               const double* __restrict__ val, double* __restrict__ grad) {
-    int64_t v = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t v = tid / LPR;
+    const int lane = (int)(tid % LPR);
     if (v >= N) return;
     const double uc = val[v];
+    const int32_t es = rp[v], ee = rp[v + 1], deg = ee - es;
     double gx = 0.0, gy = 0.0, gz = 0.0;
-    for (int32_t e = rp[v]; e < rp[v + 1]; ++e) {
-        double d = val[ngh[e]] - uc;
-        gx += gw[e] * d;
-        gy += gw[E + e] * d;
-        gz += gw[2 * E + e] * d;
+    if (deg <= 2 * LPR) {
+        double t[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        for (int h = 0; h < 2; ++h) {
+            const int32_t e = es + h * LPR + lane;
+            if (e < ee) {
+                const double d = val[ngh[e]] - uc;
+                t[h][0] = gw[e] * d;
+                t[h][1] = gw[E + e] * d;
+                t[h][2] = gw[2 * E + e] * d;
+            }
+        }
+        for (int k = 0; k < deg; ++k) {
+            const int h = k / LPR, src = k % LPR;
+            gx += __shfl(h ? t[1][0] : t[0][0], src, LPR);
+            gy += __shfl(h ? t[1][1] : t[0][1], src, LPR);
+            gz += __shfl(h ? t[1][2] : t[0][2], src, LPR);
+        }
+    } else if (lane == 0) {
+        for (int32_t e = es; e < ee; ++e) {
+            const double d = val[ngh[e]] - uc;
+            gx += gw[e] * d;
+            gy += gw[E + e] * d;
+            gz += gw[2 * E + e] * d;
+        }
     }
-    grad[v] = gx;
-    grad[N + v] = gy;
-    grad[2 * N + v] = gz;
+    if (lane == 0) {
+        grad[v] = gx;
+        grad[N + v] = gy;
+        grad[2 * N + v] = gz;
+    }
 }
 
 // VectorDomain::update_grad (src/base/vec_dom.rs:146-178); value 3N SoA -> grad 9N SoA
@@ -119,96 +147,145 @@ __device__ inline double face_diff(const double* __restrict__ G, const double* _
     return diff_f > 0.0 ? diff_f : diff_m;
 }
 
+// Matrix coefficient of directed edge e = (v, n) for an edge operator.
+__device__ inline double edge_coeff(const DevOp& op, const AsmArgs& a, int64_t v, int64_t e) {
+    const int64_t N = a.N, E = a.E;
+    const int64_t n = a.ngh[e];
+    if (op.kind == VFVM_OP_SCL_DOM_DIFFUSION || op.kind == VFVM_OP_SCL_BND_DIFFUSION) {
+        // opscl_dom_diff.rs:67-86, opscl_bnd_diff.rs:72-91
+        double df = face_diff(op.f_val, op.f_grad, N, v, n, a.skew[e], a.skew[E + e], a.skew[2 * E + e]);
+        return df * a.area[e] / a.dist[e];
+    }
+    // opscl_dom_advunity.rs:70-112: power law A = max(0, 1 - 0.1|Pe|)^5
+    double df = op.has_diff ? face_diff(op.f_val, op.f_grad, N, v, n, a.skew[e], a.skew[E + e], a.skew[2 * E + e]) : 0.0;
+    double cond = df * a.area[e] / a.dist[e];
+    double flux = op.flux[e];
+    double pe = cond != 0.0 ? flux / cond : 0.0;
+    double base = fmax(1.0 - 0.1 * fabs(pe), 0.0);
+    double b2 = base * base;
+    double pl = b2 * b2 * base;
+    return cond * (pl - 1.0) + fmax(-flux, 0.0);
+}
+
+__device__ inline bool is_edge_op(int k) {
+    return k == VFVM_OP_SCL_DOM_DIFFUSION || k == VFVM_OP_SCL_BND_DIFFUSION || k == VFVM_OP_SCL_DOM_ADVECTION_UNITY ||
+           k == VFVM_OP_SCL_BND_ADVECTION_UNITY;
+}
+
+// Diagonal and right-hand-side contributions of a per-vertex operator.
+__device__ inline void vertex_op(const DevOp& op, const AsmArgs& a, int64_t v, int32_t vb, double& dacc, double& bacc) {
+    const int64_t N = a.N, S = a.S;
+    switch (op.kind) {
+        case VFVM_OP_SCL_DOM_PRODUCTION:
+        case VFVM_OP_SCL_BND_PRODUCTION: {
+            // opscl_dom_prod.rs:56-72
+            const double sx = a.cskew[v], sy = a.cskew[N + v], sz = a.cskew[2 * N + v];
+            const double vol = a.vol[v];
+            const double pv = op.f_val[v];
+            const double prd_c = pv + op.f_grad[v] * sx + op.f_grad[N + v] * sy + op.f_grad[2 * N + v] * sz;
+            const double unk_skew = op.u_grad[v] * sx + op.u_grad[N + v] * sy + op.u_grad[2 * N + v] * sz;
+            dacc += prd_c * vol;
+            bacc += -pv * unk_skew * vol;
+        } break;
+        case VFVM_OP_SCL_DOM_SOURCE:
+        case VFVM_OP_SCL_BND_SOURCE: {
+            // opscl_dom_src.rs:50-64
+            const double sx = a.cskew[v], sy = a.cskew[N + v], sz = a.cskew[2 * N + v];
+            const double src_c = op.f_val[v] + op.f_grad[v] * sx + op.f_grad[N + v] * sy + op.f_grad[2 * N + v] * sz;
+            bacc += -src_c * a.vol[v];
+        } break;
+        case VFVM_OP_SCL_DOM_TIME:
+        case VFVM_OP_SCL_BND_TIME: {
+            // opscl_dom_time.rs:46-58
+            const double vol = a.vol[v];
+            const double w = op.f_val[v];
+            dacc += -w * vol / a.dt;
+            bacc += -w * op.u_prev[v] * vol / a.dt;
+        } break;
+        case VFVM_OP_SCL_BND_DIRICHLET: {
+            // opscl_bnd_dir.rs:65-75
+            dacc += 1.0;
+            bacc += op.f_val[a.bidx[v]];
+        } break;
+        case VFVM_OP_SCL_BND_NEUMANN: {
+            // opscl_bnd_neu.rs:54-83
+            const double fv = op.f_val[a.bidx[v]];
+            for (int32_t s = a.bsf_ptr[v]; s < a.bsf_ptr[v + 1]; ++s) {
+                const int32_t j = a.bsf_vj[s], k = a.bsf_vk[s];
+                const double fj = a.vbnd[j] == vb ? op.f_val[a.bidx[j]] : fv;
+                const double fk = a.vbnd[k] == vb ? op.f_val[a.bidx[k]] : fv;
+                bacc += a.bsf_area[s] * (a.bsf_wgt[s] * fv + a.bsf_wgt[S + s] * fj + a.bsf_wgt[2 * S + s] * fk);
+            }
+        } break;
+        default:
+            break;
+    }
+}
+
+// One row per LPR lanes: lane l handles the directed edges es + l, es + l + LPR (rows of
+// degree <= 2 LPR), so the per-edge geometry and coefficient loads of a wave are
+// contiguous.  Each matrix entry is summed in operator order in its lane's registers
+// (starting from 0, as the reference's duplicate summation does); the diagonal is summed
+// in the reference's push order (operator, then neighbour) by walking the lanes' edge
+// coefficients with shuffles, so the result is bit-identical to a sequential row walk.
+// Rows of higher degree take the sequential walk in their first lane.
+template <int LPR>
 __global__ void __launch_bounds__(BLK) k_assemble(AsmArgs a) {
-    int64_t v = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (v >= a.N) return;
-    const int64_t N = a.N, E = a.E, S = a.S;
+    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t v = tid / LPR;
+    const int lane = (int)(tid % LPR);
+    if (v >= a.N) return;  // whole row groups leave together (BLK is a multiple of LPR)
     const int32_t vb = a.vbnd[v];
     const int cls = vb < 0 ? 0 : 1 + vb;
     const int nops = a.prog->nops[cls];
-    const int32_t ms = a.mrow[v], me = a.mrow[v + 1];
-    for (int32_t k = ms; k < me; ++k) a.vals[k] = 0.0;
     const int32_t es = a.rp[v], ee = a.rp[v + 1];
+    const int32_t deg = ee - es;
     double dacc = 0.0, bacc = 0.0;
-    for (int o = 0; o < nops; ++o) {
-        const DevOp op = a.prog->ops[cls][o];
-        switch (op.kind) {
-            case VFVM_OP_SCL_DOM_DIFFUSION:
-            case VFVM_OP_SCL_BND_DIFFUSION: {
-                // opscl_dom_diff.rs:67-86, opscl_bnd_diff.rs:72-91
+    if (deg <= 2 * LPR) {
+        const int32_t e0 = es + lane, e1 = es + LPR + lane;
+        const bool h0 = e0 < ee, h1 = e1 < ee;
+        double v0 = 0.0, v1 = 0.0;
+        bool any_edge = false;
+        for (int o = 0; o < nops; ++o) {
+            const DevOp op = a.prog->ops[cls][o];
+            if (is_edge_op(op.kind)) {
+                any_edge = true;
+                const double c0 = h0 ? edge_coeff(op, a, v, e0) : 0.0;
+                const double c1 = h1 ? edge_coeff(op, a, v, e1) : 0.0;
+                v0 += c0;
+                v1 += c1;
+                const int n0 = deg < LPR ? deg : LPR;
+                for (int k = 0; k < n0; ++k) dacc += -__shfl(c0, k, LPR);
+                for (int k = 0; k < deg - LPR; ++k) dacc += -__shfl(c1, k, LPR);
+            } else if (lane == 0) {
+                vertex_op(op, a, v, vb, dacc, bacc);
+            }
+        }
+        if (any_edge) {
+            if (h0) a.vals[a.eslot[e0]] = v0;
+            if (h1) a.vals[a.eslot[e1]] = v1;
+        }
+    } else if (lane == 0) {
+        const int32_t ms = a.mrow[v], me = a.mrow[v + 1];
+        for (int32_t k = ms; k < me; ++k) a.vals[k] = 0.0;
+        for (int o = 0; o < nops; ++o) {
+            const DevOp op = a.prog->ops[cls][o];
+            if (is_edge_op(op.kind)) {
                 for (int32_t e = es; e < ee; ++e) {
-                    const int64_t n = a.ngh[e];
-                    double df = face_diff(op.f_val, op.f_grad, N, v, n, a.skew[e], a.skew[E + e], a.skew[2 * E + e]);
-                    double coeff = df * a.area[e] / a.dist[e];
+                    const double coeff = edge_coeff(op, a, v, e);
                     a.vals[a.eslot[e]] += coeff;
                     dacc += -coeff;
                 }
-            } break;
-            case VFVM_OP_SCL_DOM_ADVECTION_UNITY:
-            case VFVM_OP_SCL_BND_ADVECTION_UNITY: {
-                // opscl_dom_advunity.rs:70-112: power law A = max(0, 1 - 0.1|Pe|)^5
-                for (int32_t e = es; e < ee; ++e) {
-                    const int64_t n = a.ngh[e];
-                    double df = op.has_diff ? face_diff(op.f_val, op.f_grad, N, v, n, a.skew[e], a.skew[E + e], a.skew[2 * E + e]) : 0.0;
-                    double cond = df * a.area[e] / a.dist[e];
-                    double flux = op.flux[e];
-                    double pe = cond != 0.0 ? flux / cond : 0.0;
-                    double base = fmax(1.0 - 0.1 * fabs(pe), 0.0);
-                    double b2 = base * base;
-                    double pl = b2 * b2 * base;
-                    double coeff = cond * (pl - 1.0) + fmax(-flux, 0.0);
-                    a.vals[a.eslot[e]] += coeff;
-                    dacc += -coeff;
-                }
-            } break;
-            case VFVM_OP_SCL_DOM_PRODUCTION:
-            case VFVM_OP_SCL_BND_PRODUCTION: {
-                // opscl_dom_prod.rs:56-72
-                const double sx = a.cskew[v], sy = a.cskew[N + v], sz = a.cskew[2 * N + v];
-                const double vol = a.vol[v];
-                const double pv = op.f_val[v];
-                const double prd_c = pv + op.f_grad[v] * sx + op.f_grad[N + v] * sy + op.f_grad[2 * N + v] * sz;
-                const double unk_skew = op.u_grad[v] * sx + op.u_grad[N + v] * sy + op.u_grad[2 * N + v] * sz;
-                dacc += prd_c * vol;
-                bacc += -pv * unk_skew * vol;
-            } break;
-            case VFVM_OP_SCL_DOM_SOURCE:
-            case VFVM_OP_SCL_BND_SOURCE: {
-                // opscl_dom_src.rs:50-64
-                const double sx = a.cskew[v], sy = a.cskew[N + v], sz = a.cskew[2 * N + v];
-                const double src_c = op.f_val[v] + op.f_grad[v] * sx + op.f_grad[N + v] * sy + op.f_grad[2 * N + v] * sz;
-                bacc += -src_c * a.vol[v];
-            } break;
-            case VFVM_OP_SCL_DOM_TIME:
-            case VFVM_OP_SCL_BND_TIME: {
-                // opscl_dom_time.rs:46-58
-                const double vol = a.vol[v];
-                const double w = op.f_val[v];
-                dacc += -w * vol / a.dt;
-                bacc += -w * op.u_prev[v] * vol / a.dt;
-            } break;
-            case VFVM_OP_SCL_BND_DIRICHLET: {
-                // opscl_bnd_dir.rs:65-75
-                dacc += 1.0;
-                bacc += op.f_val[a.bidx[v]];
-            } break;
-            case VFVM_OP_SCL_BND_NEUMANN: {
-                // opscl_bnd_neu.rs:54-83
-                const double fv = op.f_val[a.bidx[v]];
-                for (int32_t s = a.bsf_ptr[v]; s < a.bsf_ptr[v + 1]; ++s) {
-                    const int32_t j = a.bsf_vj[s], k = a.bsf_vk[s];
-                    const double fj = a.vbnd[j] == vb ? op.f_val[a.bidx[j]] : fv;
-                    const double fk = a.vbnd[k] == vb ? op.f_val[a.bidx[k]] : fv;
-                    bacc += a.bsf_area[s] * (a.bsf_wgt[s] * fv + a.bsf_wgt[S + s] * fj + a.bsf_wgt[2 * S + s] * fk);
-                }
-            } break;
-            default:
-                break;
+            } else {
+                vertex_op(op, a, v, vb, dacc, bacc);
+            }
         }
     }
-    const int32_t ds = a.dslot[v];
-    if (ds >= 0) a.vals[ds] = dacc;
-    a.b[v] = bacc;
+    if (lane == 0) {
+        const int32_t ds = a.dslot[v];
+        if (ds >= 0) a.vals[ds] = dacc;
+        a.b[v] = bacc;
+    }
 }
 
 }  // namespace
@@ -216,7 +293,8 @@ __global__ void __launch_bounds__(BLK) k_assemble(AsmArgs a) {
 int launch_grad(Ctx& c, const double* value, double* grad) {
     DevGeom& g = c.g;
     if (g.N == 0) return VFVM_OK;
-    hipLaunchKernelGGL(k_grad, dim3(grid_for(g.N)), dim3(BLK), 0, c.stream, g.N, g.E, g.row_ptr.p, g.ngh.p, g.gw.p, value, grad);
+    constexpr int LPR = 16;
+    hipLaunchKernelGGL((k_grad<LPR>), dim3(grid_for(g.N * LPR)), dim3(BLK), 0, c.stream, g.N, g.E, g.row_ptr.p, g.ngh.p, g.gw.p, value, grad);
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
 }
@@ -258,7 +336,8 @@ int launch_assemble(Ctx& c, double dt) {
     double bytes = (double)g.E * (8.0 + 40.0 + 8.0 + 32.0) + (double)g.N * 64.0;
     hipEvent_t ev = nullptr;
     ktime_begin(c, VFVM_K_ASSEMBLE, &ev);
-    hipLaunchKernelGGL(k_assemble, dim3(grid_for(g.N)), dim3(BLK), 0, c.stream, a);
+    constexpr int LPR = 16;
+    hipLaunchKernelGGL((k_assemble<LPR>), dim3(grid_for(g.N * LPR)), dim3(BLK), 0, c.stream, a);
     VFVM_HIP_TRY(hipGetLastError());
     ktime_end(c, VFVM_K_ASSEMBLE, bytes, ev);
     return VFVM_OK;

@@ -16,7 +16,6 @@
 // host reads the control word every few steps.  Reductions are two-stage and
 // fixed-order, so results are bitwise reproducible run to run.
 #include <hip/hip_ext.h>
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,7 +25,6 @@
 
 #include "ctx.hpp"
 
-namespace cg = cooperative_groups;
 
 namespace vfvm {
 
@@ -378,13 +376,16 @@ __global__ void __launch_bounds__(BLK) k_ilu_factor(int32_t t0, int32_t t1, cons
     }
 }
 
-// forward sweep over the rows [t0, t1) of one colour (L has a unit diagonal):
-// zp_t = rhs[perm[t]] - sum_L l_tj zp_j
-template <class T, int LPR>
+// Forward sweep over the rows [t0, t1) of one colour (L has a unit diagonal):
+//   zp_t = rhs[perm[t]] - sum_L l_tj zp_j
+// The last colour has no U entries: with FINAL its backward sweep is folded in
+// (zp_t = z[perm[t]] = zp_t / u_tt).
+template <class T, int LPR, bool FINAL>
 __global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                    const int32_t* __restrict__ diag, const T* __restrict__ lu,
-                                                   const T* __restrict__ rhs, T* __restrict__ zp, const int* __restrict__ ctrl) {
+                                                   const T* __restrict__ rhs, T* __restrict__ zp, T* __restrict__ z,
+                                                   const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
     const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
     const int32_t t = t0 + (int32_t)(tid / LPR);
@@ -394,9 +395,20 @@ __global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const
     const int32_t d = diag[t];
     for (int32_t k = rp[t] + lane; k < d; k += LPR) s += lu[k] * zp[col[k]];
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-    if (lane == 0) zp[t] = rhs[perm[t]] - s;
+    if (lane == 0) {
+        const int32_t r = perm[t];
+        const T v = rhs[r] - s;
+        if (FINAL) {
+            const T zv = v / lu[d];
+            zp[t] = zv;
+            z[r] = zv;
+        } else {
+            zp[t] = v;
+        }
+    }
 }
-// backward sweep: zp_t = (zp_t - sum_U u_tj zp_j) / u_tt, z[perm[t]] = zp_t
+// backward sweep over one colour below the last: zp_t = (zp_t - sum_U u_tj zp_j) / u_tt,
+// z[perm[t]] = zp_t
 template <class T, int LPR>
 __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
@@ -412,53 +424,24 @@ __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const
     for (int32_t k = d + 1 + lane; k < e; k += LPR) s += lu[k] * zp[col[k]];
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
     if (lane == 0) {
+        const int32_t r = perm[t];
         const T v = (zp[t] - s) / lu[d];
         zp[t] = v;
-        z[perm[t]] = v;
+        z[r] = v;
     }
 }
 
-// Both sweeps in one cooperative launch: the grid walks the colours in order, with a
-// grid-wide barrier between colours (the barrier orders and publishes the zp writes of
-// one colour before the next reads them).  All blocks leave together when ctrl says the
-// cycle has converged, before the first barrier.
-template <class T, int LPR>
-__global__ void __launch_bounds__(BLK) k_ilu_apply_coop(int ncol, const int32_t* __restrict__ cptr,
-                                                        const int32_t* __restrict__ perm, const int32_t* __restrict__ rp,
-                                                        const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
-                                                        const T* __restrict__ lu, const T* __restrict__ rhs,
-                                                        T* __restrict__ zp, T* __restrict__ z, const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[0] == 0) return;
-    cg::grid_group grid = cg::this_grid();
-    const int64_t gt = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    const int64_t rstride = (int64_t)gridDim.x * BLK / LPR;
-    const int lane = (int)(gt % LPR);
-    for (int q = 0; q < ncol; ++q) {
-        const int32_t t1 = cptr[q + 1];
-        for (int64_t t = cptr[q] + gt / LPR; t < t1; t += rstride) {
-            T s = 0;
-            const int32_t d = diag[t];
-            for (int32_t k = rp[t] + lane; k < d; k += LPR) s += lu[k] * zp[col[k]];
-            for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-            if (lane == 0) zp[t] = rhs[perm[t]] - s;
-        }
-        grid.sync();
-    }
-    for (int q = ncol - 1; q >= 0; --q) {
-        const int32_t t1 = cptr[q + 1];
-        for (int64_t t = cptr[q] + gt / LPR; t < t1; t += rstride) {
-            T s = 0;
-            const int32_t d = diag[t], e = rp[t + 1];
-            for (int32_t k = d + 1 + lane; k < e; k += LPR) s += lu[k] * zp[col[k]];
-            for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-            if (lane == 0) {
-                const T v = (zp[t] - s) / lu[d];
-                zp[t] = v;
-                z[perm[t]] = v;
-            }
-        }
-        if (q > 0) grid.sync();
-    }
+template <class TS, class TD>
+__global__ void __launch_bounds__(BLK) k_convert(int64_t n, const TS* __restrict__ x, TD* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) y[i] = (TD)x[i];
+}
+
+// z = dinv * r
+__global__ void __launch_bounds__(BLK) k_xupdate_set(int64_t n, const double* __restrict__ r, const double* __restrict__ dinv,
+                                                     double* __restrict__ z) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) z[i] = r[i] * dinv[i];
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
@@ -617,21 +600,67 @@ int ilu_setup(Ctx& c, const SysView& sy) {
         for (int64_t i = 0; i < n; ++i)
             for (int32_t k = rp[i]; k < rp[i + 1]; ++k) trows[fill[cl[k]]++] = (int32_t)i;
     }
-    // greedy colouring in natural order
+    // deduplicated symmetrised adjacency without the diagonal
+    std::vector<int32_t> aptr(n + 1, 0), adj;
+    adj.reserve((size_t)rp[n] * 2);
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t a = rp[i], ae = rp[i + 1], b = tcnt[i], be = tcnt[i + 1];
+        int32_t last = -1;
+        while (a < ae || b < be) {
+            int32_t j;
+            if (b >= be || (a < ae && cl[a] <= trows[b])) j = cl[a++];
+            else j = trows[b++];
+            if (j != (int32_t)i && j != last) { adj.push_back(j); last = j; }
+        }
+        aptr[i + 1] = (int32_t)adj.size();
+    }
+    // colouring order: smallest-last (Matula-Beck degeneracy order, reversed), which needs
+    // fewer colours than the natural order on Delaunay graphs; VFVM_ILU_ORDER=natural keeps
+    // the natural order
+    std::vector<int32_t> order(n);
+    const char* ord_env = getenv("VFVM_ILU_ORDER");
+    if (ord_env && std::string(ord_env) == "natural") {
+        for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+    } else {
+        std::vector<int32_t> deg(n);
+        int32_t maxd = 0;
+        for (int64_t i = 0; i < n; ++i) { deg[i] = aptr[i + 1] - aptr[i]; maxd = std::max(maxd, deg[i]); }
+        std::vector<std::vector<int32_t>> bucket(maxd + 1);
+        for (int64_t i = n - 1; i >= 0; --i) bucket[deg[i]].push_back((int32_t)i);
+        std::vector<uint8_t> gone(n, 0);
+        int32_t cur = 0;
+        for (int64_t cnt = n - 1; cnt >= 0; --cnt) {
+            int32_t v = -1;
+            for (;;) {
+                while (bucket[cur].empty()) ++cur;
+                v = bucket[cur].back();
+                bucket[cur].pop_back();
+                if (!gone[v] && deg[v] == cur) break;  // skip stale entries
+            }
+            gone[v] = 1;
+            order[cnt] = v;
+            for (int32_t k = aptr[v]; k < aptr[v + 1]; ++k) {
+                const int32_t u = adj[k];
+                if (!gone[u]) bucket[--deg[u]].push_back(u);
+            }
+            cur = std::max(0, cur - 1);
+        }
+    }
+    // greedy colouring in that order
     std::vector<int32_t> color(n, -1);
     std::vector<int64_t> mark(64, -1);
     int32_t ncol = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        auto take = [&](int32_t j) {
-            if (j != i && color[j] >= 0) {
-                if (color[j] >= (int32_t)mark.size()) mark.resize(color[j] + 1, -1);
-                mark[color[j]] = i;
+    for (int64_t oi = 0; oi < n; ++oi) {
+        const int32_t i = order[oi];
+        for (int32_t k = aptr[i]; k < aptr[i + 1]; ++k) {
+            const int32_t cj = color[adj[k]];
+            if (cj >= 0) {
+                if (cj >= (int32_t)mark.size()) mark.resize(cj + 1, -1);
+                mark[cj] = oi;
             }
-        };
-        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) take(cl[k]);
-        for (int32_t k = tcnt[i]; k < tcnt[i + 1]; ++k) take(trows[k]);
+        }
         int32_t cc = 0;
-        while (cc < (int32_t)mark.size() && mark[cc] == i) ++cc;
+        while (cc < (int32_t)mark.size() && mark[cc] == oi) ++cc;
         color[i] = cc;
         ncol = std::max(ncol, cc + 1);
     }
@@ -669,7 +698,6 @@ int ilu_setup(Ctx& c, const SysView& sy) {
     VFVM_HIP_TRY(I.src.upload(src.data(), src.size(), c.stream));
     VFVM_HIP_TRY(I.pcol.upload(pcol.data(), pcol.size(), c.stream));
     VFVM_HIP_TRY(I.pdiag.upload(pdiag.data(), pdiag.size(), c.stream));
-    VFVM_HIP_TRY(I.cptr.upload(cptr.data(), cptr.size(), c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     I.h_color_ptr = cptr;
     I.ready = true;
@@ -692,62 +720,29 @@ int ilu_factor(Ctx& c, const SysView& sy, T* lu, const double* val) {
     return VFVM_OK;
 }
 
-// z = (LU)^-1 rhs: forward sweep over colours ascending, backward sweep descending, in the
-// permuted work vector zp; rhs is gathered by the forward sweep and z scattered by the
-// backward sweep
+// z = (LU)^-1 rhs: forward sweep over the colours ascending (the last one also finishing
+// its backward sweep), backward sweep over the other colours descending, in the permuted
+// work vector zp: 2 colours - 1 launches.
 template <class T>
 int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, T* z, T* zp, const int* ctrl) {
     constexpr int LPR = 4;
-    IluData& I = *sy.ilu;
+    const IluData& I = *sy.ilu;
     const int ncol = (int)I.h_color_ptr.size() - 1;
     // algorithmic bytes of one sweep launch over r rows: the triangle's entries (value +
     // column, about half of each row), rp/diag/perm (12 B), rhs or z reads and writes
     const double per_row_entries = 0.5 * (double)sy.nnz / (double)sy.n * (sizeof(T) + 4.0);
-    if (I.coop_blocks < 0) {
-        // The single cooperative launch is opt-in (VFVM_ILU_COOP=1): measured on MI355X at
-        // 1M rows and 11 colours it takes 2.5 ms per application against 0.17 ms for the
-        // 22 per-colour launches, the grid-wide barrier being far costlier than a kernel
-        // boundary.
-        I.coop_blocks = 0;
-        int coop = 0, ncu = 0, per_cu = 0;
-        const char* on = getenv("VFVM_ILU_COOP");
-        if (on && on[0] == '1' &&
-            hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c.device) == hipSuccess && coop &&
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ilu_apply_coop<T, LPR>, BLK, 0) == hipSuccess &&
-            per_cu > 0)
-            I.coop_blocks = std::min(per_cu, 4) * ncu;
-        (void)hipGetLastError();
-    }
-    if (I.coop_blocks > 0) {
-        const int32_t* cptr = I.cptr.p;
-        const int32_t* perm = I.perm.p;
-        const int32_t* rp = I.prp.p;
-        const int32_t* cl = I.pcol.p;
-        const int32_t* dg = I.pdiag.p;
-        int nc = ncol;
-        void* args[] = {&nc, &cptr, &perm, &rp, &cl, &dg, (void*)&lu, (void*)&rhs, &zp, &z, (void*)&ctrl};
-        hipEvent_t ev = nullptr;
-        ktime_begin(c, VFVM_K_PRECOND, &ev);
-        hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_ilu_apply_coop<T, LPR>),
-                                                  dim3(I.coop_blocks), dim3(BLK), args, 0, c.stream);
-        if (e == hipSuccess) {
-            ktime_end(c, VFVM_K_PRECOND, (double)sy.n * (2.0 * per_row_entries + 24.0 + 5.0 * sizeof(T)), ev);
-            return VFVM_OK;
-        }
-        // not launchable cooperatively (e.g. under a tool): fall back to one launch per colour
-        (void)hipGetLastError();
-        I.coop_blocks = 0;
-        if (ev) c.ev_next--;
-    }
     for (int q = 0; q < ncol; ++q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
-        launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 2.0 * sizeof(T)),
-                     k_ilu_lower<T, LPR>, dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK),
-                     s, e, I.perm.p, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, zp, ctrl);
+        const dim3 grid(grid_for((int64_t)(e - s) * LPR));
+        if (q == ncol - 1)
+            launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 3.0 * sizeof(T)),
+                         k_ilu_lower<T, LPR, true>, grid, dim3(BLK), s, e, I.perm.p, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, zp, z, ctrl);
+        else
+            launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 2.0 * sizeof(T)),
+                         k_ilu_lower<T, LPR, false>, grid, dim3(BLK), s, e, I.perm.p, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, zp, z, ctrl);
     }
-    for (int q = ncol - 1; q >= 0; --q) {
+    for (int q = ncol - 2; q >= 0; --q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
         launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 3.0 * sizeof(T)),
@@ -842,6 +837,57 @@ SysView transpose_view(Ctx& c) {
 
 int solver_setup(Ctx& c, const SysView& sy, const vfvm_solve_opts& o) {
     if (o.precond == VFVM_PRECOND_ILU0) return ilu_setup(c, sy);
+    return VFVM_OK;
+}
+
+int precond_apply(Ctx& c, const vfvm_solve_opts& o, const double* r_host, double* z_host) {
+    const SysView sy = primal_view(c);
+    const int64_t n = sy.n;
+    if (o.precond < VFVM_PRECOND_NONE || o.precond > VFVM_PRECOND_ILU0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "unknown preconditioner");
+    if (o.precision != VFVM_FP64 && o.precision != VFVM_FP32_REFINED) return set_error(VFVM_ERR_INVALID_ARGUMENT, "unknown precision");
+    DevBuf<double> r, z;
+    VFVM_HIP_TRY(r.alloc(n));
+    VFVM_HIP_TRY(z.alloc(n));
+    VFVM_HIP_TRY(hipMemcpyAsync(r.p, r_host, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (o.precond == VFVM_PRECOND_ILU0) {
+        if (int s = ilu_setup(c, sy)) return s;
+        if (o.precision == VFVM_FP64) {
+            VFVM_HIP_TRY(c.ilu64.alloc(sy.nnz));
+            if (int s = ilu_factor<double>(c, sy, c.ilu64.p, sy.vals)) return s;
+            VFVM_HIP_TRY(c.zp64.alloc(n));
+            if (int s = ilu_apply<double>(c, sy, c.ilu64.p, r.p, z.p, c.zp64.p, nullptr)) return s;
+        } else {
+            DevBuf<float> r32, z32;
+            VFVM_HIP_TRY(r32.alloc(n));
+            VFVM_HIP_TRY(z32.alloc(n));
+            VFVM_HIP_TRY(c.ilu32.alloc(sy.nnz));
+            if (int s = ilu_factor<float>(c, sy, c.ilu32.p, sy.vals)) return s;
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, r.p, r32.p);
+            VFVM_HIP_TRY(c.zp32.alloc(n));
+            if (int s = ilu_apply<float>(c, sy, c.ilu32.p, r32.p, z32.p, c.zp32.p, nullptr)) return s;
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z32.p, z.p);
+            VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+        }
+    } else {
+        VFVM_HIP_TRY(c.dinv64.alloc(n));
+        hipLaunchKernelGGL(k_dinv, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.dslot, sy.vals, c.dinv64.p,
+                           o.precond == VFVM_PRECOND_JACOBI ? 1 : 0);
+        hipLaunchKernelGGL(k_xupdate_set, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, r.p, c.dinv64.p, z.p);
+    }
+    VFVM_HIP_TRY(hipGetLastError());
+    VFVM_HIP_TRY(hipMemcpyAsync(z_host, z.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    return VFVM_OK;
+}
+
+int ilu_order(Ctx& c, int32_t* perm, int32_t* color_ptr, int32_t* ncolors) {
+    const SysView sy = primal_view(c);
+    if (int s = ilu_setup(c, sy)) return s;
+    const IluData& I = *sy.ilu;
+    const int32_t nc = (int32_t)I.h_color_ptr.size() - 1;
+    if (ncolors) *ncolors = nc;
+    if (color_ptr) std::memcpy(color_ptr, I.h_color_ptr.data(), I.h_color_ptr.size() * sizeof(int32_t));
+    if (perm) VFVM_HIP_TRY(hipMemcpy(perm, I.perm.p, sy.n * sizeof(int32_t), hipMemcpyDeviceToHost));
     return VFVM_OK;
 }
 

@@ -123,6 +123,8 @@ SIGNATURES = [
     ("vfvm_solve", _I, [_P, C.POINTER(SolveOpts), _PD, C.POINTER(SolveStats)]),
     ("vfvm_solve_device", _I, [_P, C.POINTER(SolveOpts), _P, C.POINTER(SolveStats)]),
     ("vfvm_residual", _I, [_P, _PD, _PD]),
+    ("vfvm_precond_apply", _I, [_P, C.POINTER(SolveOpts), _PD, _PD]),
+    ("vfvm_ilu_order", _I, [_P, _PI, _PI, _PI]),
     ("vfvm_steady_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _I, _D, _D, C.POINTER(DriverStats)]),
     ("vfvm_transient_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, _D, _D, C.POINTER(DriverStats)]),
     ("vfvm_adjoint_forward", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, C.POINTER(AdjointStats)]),

@@ -243,6 +243,23 @@ class Context:
         check(lib.vfvm_residual(self._h, ptr(x), C.byref(r)))
         return r.value
 
+    def precond_apply(self, r, **kw):
+        """z = M^-1 r with the preconditioner of the solve options kw."""
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty_like(r)
+        o = solve_opts(**kw)
+        check(lib.vfvm_precond_apply(self._h, C.byref(o), ptr(r), ptr(z)))
+        return z
+
+    def ilu_order(self):
+        """(perm, color_ptr): the ILU(0) row order of the built pattern."""
+        n = self.mesh.num_tot
+        perm = np.empty(n, np.int32)
+        cptr = np.empty(n + 1, np.int32)
+        nc = C.c_int32()
+        check(lib.vfvm_ilu_order(self._h, ptr(perm, C.c_int32), ptr(cptr, C.c_int32), C.byref(nc)))
+        return perm, cptr[:nc.value + 1].copy()
+
     def steady_solve(self, ops, max_iter, tol, damp, lin=None):
         """SteadyBase::solve; lin: dict of linear-solver options (solve_opts keywords)."""
         arr = make_ops(ops)
