@@ -210,6 +210,27 @@ def test_ilu0_application_vs_oracle(ctx, prob):
     assert np.allclose(zj, r / A.diagonal(), rtol=1e-15, atol=0)
 
 
+@pytest.mark.parametrize("prec,pc", [(vfvm.FP64, vfvm.PRECOND_ILU0), (vfvm.FP32_REFINED, vfvm.PRECOND_ILU0),
+                                     (vfvm.FP64, vfvm.PRECOND_JACOBI)])
+def test_cycle_graphs_match_eager_launches(ctx, prec, pc):
+    """Solves replayed from the captured cycle graphs (kernel timing off) equal, bit for bit,
+    the same solves launched kernel by kernel (timing on), over repeated replays with eager
+    work in between."""
+    print(f"[cycle graphs prec={prec} pc={pc}]", flush=True)
+    p = mass_c2(vfvm.Mesh.lattice(12), c_out=0.0, prod=0.3)
+    _assembled(ctx, p)
+    kw = dict(tol=1e-11, restart=30, max_outer=400, precond=pc, precision=prec)
+    xs = []
+    for mode in (0, 0, -1, 0, -1, 0):
+        ctx.timing_enable(mode)
+        x, st = ctx.solve(None, **kw)
+        assert st.rel_residual <= 1e-11
+        xs.append(x)
+    ctx.timing_enable(0)
+    for x in xs[1:]:
+        assert np.array_equal(x, xs[0])
+
+
 def test_reference_api_mirror_steady_heat():
     """The host mirror of the reference API (Variables / SteadyHeat / LinearGmres) solves
     config 1 through the C ABI and matches the oracle's LU solution."""

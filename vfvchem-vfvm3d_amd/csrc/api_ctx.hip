@@ -210,6 +210,8 @@ extern "C" int vfvm_create(int32_t device, vfvm_ctx** out) {
     VFVM_HIP_TRY(hipSetDevice(device));
     auto* h = new vfvm_ctx();
     h->c.device = device;
+    const char* gr = getenv("VFVM_GRAPHS");
+    h->c.graphs_enabled = !(gr && gr[0] == '0');
     e = hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete h; return set_error(VFVM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
     *out = h;
@@ -222,6 +224,8 @@ extern "C" void vfvm_destroy(vfvm_ctx* ctx) {
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     for (auto e : c.ev_pool) (void)hipEventDestroy(e);
+    for (auto& g : c.graphs)
+        for (auto e : g.chunks) (void)hipGraphExecDestroy(e);
     if (c.h_pinned) (void)hipHostFree(c.h_pinned);
     hipStream_t s = c.stream;
     delete ctx;  // frees device buffers

@@ -106,6 +106,7 @@ struct IluData {
     DevBuf<int32_t> perm, prp, src, pcol, pdiag;
     std::vector<int32_t> h_color_ptr;
     bool ready = false;
+    uint64_t generation = 0;  // bumped by every set-up (graphs bake the layout in)
 };
 
 // One linear system as the solver reads it (non-owning).
@@ -135,6 +136,13 @@ struct AdjTape {
     int32_t seg_first = -1, seg_count = 0;
     DevBuf<double> final_state;
     vfvm_adjoint_stats stats{};
+};
+
+// Captured launches of one restart cycle (one executable graph per chunk of steps).
+struct CycleGraphs {
+    std::array<uint64_t, 20> key{};
+    std::vector<hipGraphExec_t> chunks;
+    uint64_t last_use = 0;
 };
 
 struct KTimer {
@@ -177,11 +185,16 @@ struct Ctx {
     // solver workspace (shared by both systems)
     DevBuf<double> V64, w64, z64, x64, r64, dinv64, avals64, part, scal;
     DevBuf<float> V32, w32, avals32;
-    DevBuf<double> ilu64, zp64;  // factor values, permuted work vector
-    DevBuf<float> ilu32, zp32;
+    DevBuf<double> ilu64, ap64, bp64, xp64;  // ILU factor; A, b, x in the colour-major order
+    DevBuf<float> ilu32;
     double* h_pinned = nullptr;  // small pinned readback buffer
 
     AdjTape tape;
+
+    // cycle graphs (krylov.hip)
+    bool graphs_enabled = true;
+    uint64_t graph_clock = 0;
+    std::vector<CycleGraphs> graphs;
 
     // timing
     uint32_t timing_mask = 0;  // bit k: time kernel class k

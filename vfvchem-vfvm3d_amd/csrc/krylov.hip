@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -46,8 +47,12 @@ struct StateLayout {
     __host__ __device__ int64_t h1() const { return y() + m; }              // m+1
     __host__ __device__ int64_t h2() const { return h1() + m + 1; }         // m+1
     __host__ __device__ int64_t misc() const { return h2() + m + 1; }       // [0]=beta [1]=inv_h [2]=target [3]=res_est [4]=1.0 [5]=||b||
-    __host__ __device__ int64_t total() const { return misc() + 8; }
+    __host__ __device__ int64_t sig() const { return misc() + 8; }          // m+1: basis scale factors
+    __host__ __device__ int64_t total() const { return sig() + m + 1; }
 };
+// The basis is kept unnormalised: v_k = sig_k V[k], sig_0 = 1 (V[0] = r / beta is
+// normalised explicitly) and sig_(j+1) = 1 / h_(j+1,j); every consumer folds sig into its
+// coefficients, which saves a scaling pass per Arnoldi step.
 // control word (ints): [0] active, [1] steps done in this cycle, [2] non-finite flag
 constexpr int CTRL_INTS = 4;
 
@@ -74,7 +79,7 @@ __device__ inline double block_sum(double v, double* sh) {
 template <class T, int LPR>
 __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                               const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
-                                              const int* __restrict__ ctrl) {
+                                              const double* __restrict__ scale, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
     const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
     const int64_t row = tid / LPR;
@@ -84,7 +89,7 @@ __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restri
     const int32_t e1 = rp[row + 1];
     for (int32_t k = rp[row] + lane; k < e1; k += LPR) s += val[k] * x[col[k]];
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-    if (lane == 0) y[row] = s;
+    if (lane == 0) y[row] = scale ? (T)(s * (T)scale[0]) : s;
 }
 
 // r = b - A x (FP64) and per-block partial sums of r^2 and b^2
@@ -116,77 +121,175 @@ __global__ void __launch_bounds__(BLK) k_residual(int64_t n, const int32_t* __re
     if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[nblk + blockIdx.x] = t2; }
 }
 
-// ---------------------------------------------------------------- multi-dot: part[k][wave] = sum_i V[k][i] w[i]
-// 2-D grid: x = row ranges (one per wave), y = groups of CHUNK basis vectors.  Each lane
-// keeps CHUNK dot products in registers and each wave writes one partial per vector:
-// no block barriers.
+// 16-byte vector of T (4 floats or 2 doubles) for the basis-vector streams
+template <class T> struct Vec16;
+template <> struct Vec16<float> { typedef float4 type; static constexpr int N = 4; };
+template <> struct Vec16<double> { typedef double2 type; static constexpr int N = 2; };
+template <class T> __device__ inline T vget(const typename Vec16<T>::type& v, int e);
+template <> __device__ inline float vget<float>(const float4& v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
+template <> __device__ inline double vget<double>(const double2& v, int e) { return e == 0 ? v.x : v.y; }
+
+// ---------------------------------------------------------------- multi-dot: part[k][blk] = V[k] . w over a block's rows
+// 2-D grid: x = row ranges (BLK / WAVE waves per block, one range per wave, a multiple of
+// the vector width), y = groups of CHUNK basis vectors.  Each lane streams 16-byte vectors
+// of w and of CHUNK basis vectors (ldv a multiple of the vector width) and keeps CHUNK dot
+// products in registers; the block's waves are summed in LDS into one partial per vector.
 template <class T>
 __global__ void __launch_bounds__(BLK) k_mdot(int64_t n, int nv, const T* __restrict__ V, int64_t ldv,
-                                              const T* __restrict__ w, double* __restrict__ part, int nwav,
-                                              int64_t rows_per_wave, const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[0] == 0) return;
-    const int lane = threadIdx.x % WAVE;
-    const int wv = blockIdx.x * (BLK / WAVE) + threadIdx.x / WAVE;
+                                              const T* __restrict__ w, double* __restrict__ part, int64_t rows_per_wave,
+                                              const int* __restrict__ ctrl) {
+    if (ctrl[0] == 0) return;
+    typedef typename Vec16<T>::type VT;
+    constexpr int VW = Vec16<T>::N;
+    __shared__ double wsum[BLK / WAVE][CHUNK];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    const int nbx = gridDim.x;
+    const int wv = blockIdx.x * (BLK / WAVE) + wid;
     const int k0 = blockIdx.y * CHUNK;
     const int64_t r0 = (int64_t)wv * rows_per_wave;
     const int64_t r1 = min(n, r0 + rows_per_wave);
+    const int64_t rv = r1 > r0 ? r0 + (r1 - r0) / VW * VW : r0;  // end of the vector part
     const int kq = min(CHUNK, nv - k0);
     const T* Vk = V + (int64_t)k0 * ldv;
     double acc[CHUNK];
 #pragma unroll
     for (int q = 0; q < CHUNK; ++q) acc[q] = 0.0;
-    if (kq == CHUNK) {
-        for (int64_t i = r0 + lane; i < r1; i += WAVE) {
-            const double wi = (double)w[i];
+    for (int64_t i = r0 + (int64_t)lane * VW; i < rv; i += (int64_t)WAVE * VW) {
+        const VT w4 = *reinterpret_cast<const VT*>(w + i);
 #pragma unroll
-            for (int q = 0; q < CHUNK; ++q) acc[q] += (double)Vk[(int64_t)q * ldv + i] * wi;
-        }
-    } else {
-        for (int64_t i = r0 + lane; i < r1; i += WAVE) {
-            const double wi = (double)w[i];
+        for (int q = 0; q < CHUNK; ++q) {
+            if (q < kq) {
+                const VT v4 = *reinterpret_cast<const VT*>(Vk + (int64_t)q * ldv + i);
 #pragma unroll
-            for (int q = 0; q < CHUNK; ++q)
-                if (q < kq) acc[q] += (double)Vk[(int64_t)q * ldv + i] * wi;
+                for (int e = 0; e < VW; ++e) acc[q] += (double)vget<T>(v4, e) * (double)vget<T>(w4, e);
+            }
         }
+    }
+    for (int64_t i = rv + lane; i < r1; i += WAVE) {
+        const double wi = (double)w[i];
+#pragma unroll
+        for (int q = 0; q < CHUNK; ++q)
+            if (q < kq) acc[q] += (double)Vk[(int64_t)q * ldv + i] * wi;
     }
 #pragma unroll
     for (int q = 0; q < CHUNK; ++q) {
-        const double s = wave_sum(acc[q]);
-        if (lane == 0 && q < kq) part[(int64_t)(k0 + q) * nwav + wv] = s;
+        const double v = wave_sum(acc[q]);
+        if (lane == 0) wsum[wid][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kq) {
+        double v = 0.0;
+        for (int k = 0; k < BLK / WAVE; ++k) v += wsum[k][threadIdx.x];
+        part[(int64_t)(k0 + threadIdx.x) * nbx + blockIdx.x] = v;
     }
 }
 
-// out[k] = sum_blk part[k][blk]; one block per k
-__global__ void __launch_bounds__(BLK) k_reduce(int nblk, const double* __restrict__ part, double* __restrict__ out,
-                                                const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[0] == 0) return;
+// out[k] = sig_k sum_blk part[k][blk]; one block per k
+__global__ void __launch_bounds__(BLK) k_reduce(int nblk, const double* __restrict__ part, const double* __restrict__ sig,
+                                                double* __restrict__ out, const int* __restrict__ ctrl) {
+    if (ctrl[0] == 0) return;
     __shared__ double sh[BLK / WAVE];
     const int k = blockIdx.x;
     double s = 0.0;
     for (int i = threadIdx.x; i < nblk; i += BLK) s += part[(int64_t)k * nblk + i];
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[k] = s;
+    if (threadIdx.x == 0) out[k] = s * sig[k];
 }
 
-// w -= sum_k V[k] h[k]; optional per-block partial of ||w||^2
+// Arnoldi step j (one block): h = h1 (+ h2), h_(j+1,j) = ||w|| from the update's partials,
+// Givens update of the column and of g, convergence test, sig_(j+1) = 1 / h_(j+1,j).  The
+// column and the rotations are staged in LDS by all threads; one thread applies the j
+// rotations in order.
+__global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int reorth, int nblk, const double* __restrict__ part_nrm,
+                                                      double* __restrict__ S, StateLayout L, int* __restrict__ ctrl) {
+    if (ctrl[0] == 0) return;
+    __shared__ double sh[BLK / WAVE];
+    __shared__ double hc[MAX_RESTART + 1], cs_s[MAX_RESTART], sn_s[MAX_RESTART];
+    const int m = L.m;
+    double* H = S + L.H();
+    const int64_t cj = (int64_t)j * (m + 1);
+    for (int i = threadIdx.x; i <= j; i += BLK) {
+        hc[i] = reorth ? S[L.h1() + i] + S[L.h2() + i] : S[L.h1() + i];
+        if (i < j) { cs_s[i] = S[L.cs() + i]; sn_s[i] = S[L.sn() + i]; }
+    }
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += BLK) s += part_nrm[i];
+    s = block_sum(s, sh);  // contains the barrier that publishes hc/cs_s/sn_s
+    if (threadIdx.x == 0) {
+        const double hn = sqrt(s);
+        hc[j + 1] = hn;
+        for (int i = 0; i < j; ++i) {
+            const double a = hc[i], b = hc[i + 1];
+            hc[i] = cs_s[i] * a + sn_s[i] * b;
+            hc[i + 1] = -sn_s[i] * a + cs_s[i] * b;
+        }
+        const double a = hc[j], b = hc[j + 1];
+        const double r = hypot(a, b);
+        const double c = r > 0.0 ? a / r : 1.0;
+        const double sgn = r > 0.0 ? b / r : 0.0;
+        S[L.cs() + j] = c;
+        S[L.sn() + j] = sgn;
+        hc[j] = r;
+        hc[j + 1] = 0.0;
+        double* g = S + L.g();
+        const double gj = g[j];
+        g[j + 1] = -sgn * gj;
+        g[j] = c * gj;
+        const double res = fabs(sgn * gj);
+        S[L.misc() + 3] = res;
+        S[L.misc() + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
+        S[L.sig() + j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
+        ctrl[1] = j + 1;
+        const bool finite = isfinite(res) && isfinite(hn) && isfinite(r);
+        if (!finite) ctrl[2] = 1;
+        if (!finite || res <= S[L.misc() + 2] || hn == 0.0 || r == 0.0) ctrl[0] = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= j + 1; i += BLK) H[cj + i] = hc[i];
+}
+
+// wout = win - sum_k V[k] (sig_k h[k]) (wout may alias win), in 16-byte vectors over the
+// block's rows (a multiple of the vector width); optional per-block partial of ||wout||^2
 template <class T>
 __global__ void __launch_bounds__(BLK) k_update(int64_t n, int nv, const T* __restrict__ V, int64_t ldv,
-                                                const double* __restrict__ h, T* __restrict__ w,
-                                                double* __restrict__ part_nrm, int64_t rows_per_blk,
+                                                const double* __restrict__ h, const double* __restrict__ sig, const T* win,
+                                                T* wout, double* __restrict__ part_nrm, int64_t rows_per_blk,
                                                 const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[0] == 0) return;
+    if (ctrl[0] == 0) return;
+    typedef typename Vec16<T>::type VT;
+    constexpr int VW = Vec16<T>::N;
     __shared__ double hs[MAX_RESTART + 1];
     __shared__ double sh[BLK / WAVE];
-    for (int k = threadIdx.x; k < nv; k += BLK) hs[k] = h[k];
+    for (int k = threadIdx.x; k < nv; k += BLK) hs[k] = h[k] * sig[k];
     __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_blk;
     const int64_t r1 = min(n, r0 + rows_per_blk);
+    const int64_t rv = r1 > r0 ? r0 + (r1 - r0) / VW * VW : r0;
     double nrm = 0.0;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += BLK) {
-        double s = (double)w[i];
-        for (int k = 0; k < nv; ++k) s -= (double)V[(int64_t)k * ldv + i] * hs[k];
-        T st = (T)s;
-        w[i] = st;
+    for (int64_t i = r0 + (int64_t)threadIdx.x * VW; i < rv; i += (int64_t)BLK * VW) {
+        const VT w4 = *reinterpret_cast<const VT*>(win + i);
+        double sv[VW];
+#pragma unroll
+        for (int e = 0; e < VW; ++e) sv[e] = (double)vget<T>(w4, e);
+        for (int k = 0; k < nv; ++k) {
+            const VT v4 = *reinterpret_cast<const VT*>(V + (int64_t)k * ldv + i);
+            const double hk = hs[k];
+#pragma unroll
+            for (int e = 0; e < VW; ++e) sv[e] -= (double)vget<T>(v4, e) * hk;
+        }
+        T out[VW];
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            out[e] = (T)sv[e];
+            nrm += (double)out[e] * (double)out[e];
+        }
+        *reinterpret_cast<VT*>(wout + i) = *reinterpret_cast<const VT*>(out);
+    }
+    for (int64_t i = rv + threadIdx.x; i < r1; i += BLK) {
+        double sv = (double)win[i];
+        for (int k = 0; k < nv; ++k) sv -= (double)V[(int64_t)k * ldv + i] * hs[k];
+        const T st = (T)sv;
+        wout[i] = st;
         nrm += (double)st * (double)st;
     }
     if (part_nrm) {
@@ -227,60 +330,11 @@ __global__ void __launch_bounds__(BLK) k_cycle_start(int nblk, const double* __r
         S[L.misc() + 4] = 1.0;
         for (int i = 0; i <= L.m; ++i) S[L.g() + i] = 0.0;
         S[L.g()] = beta;
+        S[L.sig()] = 1.0;
         ctrl[0] = (beta > 0.0 && isfinite(beta)) ? 1 : 0;
         ctrl[1] = 0;
         ctrl[2] = isfinite(beta) ? 0 : 1;
     }
-}
-
-// Arnoldi step j: h = h1 (+ h2), h_{j+1,j} = ||w||, Givens update, convergence test.
-// The column and the rotations are staged in LDS by all threads; one thread applies the
-// j rotations in order.
-__global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int reorth, int nblk, const double* __restrict__ part_nrm,
-                                                      double* __restrict__ S, StateLayout L, int* __restrict__ ctrl) {
-    if (ctrl[0] == 0) return;
-    __shared__ double sh[BLK / WAVE];
-    __shared__ double hc[MAX_RESTART + 1], cs_s[MAX_RESTART], sn_s[MAX_RESTART];
-    const int m = L.m;
-    double* H = S + L.H();
-    const int64_t cj = (int64_t)j * (m + 1);
-    for (int i = threadIdx.x; i <= j; i += BLK) {
-        hc[i] = reorth ? S[L.h1() + i] + S[L.h2() + i] : S[L.h1() + i];
-        if (i < j) { cs_s[i] = S[L.cs() + i]; sn_s[i] = S[L.sn() + i]; }
-    }
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += BLK) s += part_nrm[i];
-    s = block_sum(s, sh);  // contains the barrier that publishes hc/cs_s/sn_s
-    if (threadIdx.x == 0) {
-        const double hn = sqrt(s);
-        hc[j + 1] = hn;
-        for (int i = 0; i < j; ++i) {
-            const double a = hc[i], b = hc[i + 1];
-            hc[i] = cs_s[i] * a + sn_s[i] * b;
-            hc[i + 1] = -sn_s[i] * a + cs_s[i] * b;
-        }
-        const double a = hc[j], b = hc[j + 1];
-        const double r = hypot(a, b);
-        const double c = r > 0.0 ? a / r : 1.0;
-        const double sgn = r > 0.0 ? b / r : 0.0;
-        S[L.cs() + j] = c;
-        S[L.sn() + j] = sgn;
-        hc[j] = r;
-        hc[j + 1] = 0.0;
-        double* g = S + L.g();
-        const double gj = g[j];
-        g[j + 1] = -sgn * gj;
-        g[j] = c * gj;
-        const double res = fabs(sgn * gj);
-        S[L.misc() + 3] = res;
-        S[L.misc() + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
-        ctrl[1] = j + 1;
-        const bool finite = isfinite(res) && isfinite(hn) && isfinite(r);
-        if (!finite) ctrl[2] = 1;
-        if (!finite || res <= S[L.misc() + 2] || hn == 0.0 || r == 0.0) ctrl[0] = 0;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i <= j + 1; i += BLK) H[cj + i] = hc[i];
 }
 
 // y = H(1:k,1:k) \ g(1:k), k = steps done
@@ -299,14 +353,14 @@ __global__ void k_backsolve(double* __restrict__ S, StateLayout L, const int* __
     }
 }
 
-// z = sum_{l<k} V[l] y[l] (z in FP64)
+// z = sum_{l<k} v_l y[l] = sum_l V[l] (sig_l y[l]) (z in FP64)
 template <class T>
 __global__ void __launch_bounds__(BLK) k_combine(int64_t n, const T* __restrict__ V, int64_t ldv,
                                                  const double* __restrict__ S, StateLayout L, const int* __restrict__ ctrl,
                                                  double* __restrict__ z) {
     __shared__ double ys[MAX_RESTART];
     const int k = ctrl[1];
-    for (int l = threadIdx.x; l < k; l += BLK) ys[l] = S[L.y() + l];
+    for (int l = threadIdx.x; l < k; l += BLK) ys[l] = S[L.y() + l] * S[L.sig() + l];
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
@@ -376,59 +430,57 @@ __global__ void __launch_bounds__(BLK) k_ilu_factor(int32_t t0, int32_t t1, cons
     }
 }
 
-// Forward sweep over the rows [t0, t1) of one colour (L has a unit diagonal):
-//   zp_t = rhs[perm[t]] - sum_L l_tj zp_j
-// The last colour has no U entries: with FINAL its backward sweep is folded in
-// (zp_t = z[perm[t]] = zp_t / u_tt).
+// Forward sweep over the rows [t0, t1) of one colour of the colour-major system
+// (L has a unit diagonal), in place in z:  z_t = c rhs_t - sum_L l_tj z_j  (c = *rscale or 1)
+// The last colour has no U entries: with FINAL its backward sweep is folded in.
 template <class T, int LPR, bool FINAL>
-__global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const int32_t* __restrict__ perm,
-                                                   const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                   const int32_t* __restrict__ diag, const T* __restrict__ lu,
-                                                   const T* __restrict__ rhs, T* __restrict__ zp, T* __restrict__ z,
+__global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
+                                                   const T* __restrict__ lu, const T* __restrict__ rhs,
+                                                   const double* __restrict__ rscale, T* __restrict__ z,
                                                    const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
     const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
     const int32_t t = t0 + (int32_t)(tid / LPR);
     const int lane = (int)(tid % LPR);
     if (t >= t1) return;
+    const int32_t k0 = rp[t], d = diag[t];
+    const T r = rscale ? (T)(rhs[t] * (T)rscale[0]) : rhs[t];
     T s = 0;
-    const int32_t d = diag[t];
-    for (int32_t k = rp[t] + lane; k < d; k += LPR) s += lu[k] * zp[col[k]];
+    for (int32_t k = k0 + lane; k < d; k += LPR) s += lu[k] * z[col[k]];
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-    if (lane == 0) {
-        const int32_t r = perm[t];
-        const T v = rhs[r] - s;
-        if (FINAL) {
-            const T zv = v / lu[d];
-            zp[t] = zv;
-            z[r] = zv;
-        } else {
-            zp[t] = v;
-        }
-    }
+    if (lane == 0) z[t] = FINAL ? (r - s) / lu[d] : r - s;
 }
-// backward sweep over one colour below the last: zp_t = (zp_t - sum_U u_tj zp_j) / u_tt,
-// z[perm[t]] = zp_t
+// backward sweep over one colour below the last, in place: z_t = (z_t - sum_U u_tj z_j) / u_tt
 template <class T, int LPR>
-__global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const int32_t* __restrict__ perm,
-                                                   const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                   const int32_t* __restrict__ diag, const T* __restrict__ lu,
-                                                   T* __restrict__ zp, T* __restrict__ z, const int* __restrict__ ctrl) {
+__global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
+                                                   const T* __restrict__ lu, T* __restrict__ z, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
     const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
     const int32_t t = t0 + (int32_t)(tid / LPR);
     const int lane = (int)(tid % LPR);
     if (t >= t1) return;
-    T s = 0;
     const int32_t d = diag[t], e = rp[t + 1];
-    for (int32_t k = d + 1 + lane; k < e; k += LPR) s += lu[k] * zp[col[k]];
+    T s = 0;
+    for (int32_t k = d + 1 + lane; k < e; k += LPR) s += lu[k] * z[col[k]];
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-    if (lane == 0) {
-        const int32_t r = perm[t];
-        const T v = (zp[t] - s) / lu[d];
-        zp[t] = v;
-        z[r] = v;
-    }
+    if (lane == 0) z[t] = (z[t] - s) / lu[d];
+}
+
+// out[t] = in[perm[t]]
+template <class T>
+__global__ void __launch_bounds__(BLK) k_gather_perm(int64_t n, const int32_t* __restrict__ perm, const T* __restrict__ in,
+                                                     T* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (t < n) out[t] = in[perm[t]];
+}
+// out[perm[t]] = in[t]
+template <class T>
+__global__ void __launch_bounds__(BLK) k_scatter_perm(int64_t n, const int32_t* __restrict__ perm, const T* __restrict__ in,
+                                                      T* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (t < n) out[perm[t]] = in[t];
 }
 
 template <class TS, class TD>
@@ -524,12 +576,17 @@ void launch_timed(Ctx& c, int kclass, double bytes, K kernel, dim3 grid, dim3 bl
             b = c.ev_pool[c.ev_next++];
         }
     }
-    hipExtLaunchKernelGGL(kernel, grid, block, 0, c.stream, a, b, 0, args...);
-    if (a) c.pending.push_back({kclass, bytes, a, b, c.cur_step});
+    if (a) {
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, c.stream, a, b, 0, args...);
+        c.pending.push_back({kclass, bytes, a, b, c.cur_step});
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, 0, c.stream, args...);
+    }
 }
 
 struct Work {
     int64_t n, nnz;
+    int64_t ldv;  // basis stride: n rounded up to 64 entries (16-byte aligned columns)
     int m;
     int nblk;  // blocks of the update kernel (one norm partial each)
     int64_t rows_per_blk;
@@ -547,14 +604,15 @@ inline int pick_lpr(int64_t n, int64_t nnz) {
 }
 
 template <class T>
-int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, const T* x, T* y, const int* ctrl, double bytes) {
+int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, const T* x, T* y, const double* scale,
+                const int* ctrl, double bytes) {
     unsigned grid = grid_for(n * lpr);
     const int32_t* rp = sy.row;
     const int32_t* cl = sy.col;
     switch (lpr) {
-        case 4: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 4>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, ctrl); break;
-        case 8: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 8>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, ctrl); break;
-        default: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 16>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, ctrl); break;
+        case 4: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 4>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
+        case 8: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 8>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
+        default: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 16>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
@@ -572,14 +630,18 @@ int residual_launch(Ctx& c, const SysView& sy, int lpr, const double* x, double*
     return VFVM_OK;
 }
 
-// part[k][wave] of the multi-dot over basis vectors [0, nv)
+// hout[k] = sig_k V[k] . wv over the basis vectors [0, nv): per-block partials, then one
+// reduction block per vector
 template <class T>
-void mdot_launch(Ctx& c, Work& w, int nv, const T* V, const T* wv, double* part) {
-    const dim3 grid(w.nwav / (BLK / WAVE), (nv + CHUNK - 1) / CHUNK), block(BLK);
+void mdot_launch(Ctx& c, Work& w, int nv, const T* V, const T* wv, double* part, double* hout) {
+    const int nbx = w.nwav / (BLK / WAVE);
+    const dim3 grid(nbx, (nv + CHUNK - 1) / CHUNK), block(BLK);
     // algorithmic bytes: the nv basis vectors and w once
     const double bytes = (double)sizeof(T) * w.n * (nv + 1);
-    launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T>, grid, block, w.n, nv, V, w.n, wv, part, w.nwav, w.rows_per_wave,
+    launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
                  (const int*)w.ctrl);
+    hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, nbx, (const double*)part, (const double*)(w.S + w.L.sig()),
+                       hout, (const int*)w.ctrl);
 }
 
 // Multicolour ordering of the matrix pattern and the colour-major [L | diag | U] storage.
@@ -698,6 +760,7 @@ int ilu_setup(Ctx& c, const SysView& sy) {
     VFVM_HIP_TRY(I.src.upload(src.data(), src.size(), c.stream));
     VFVM_HIP_TRY(I.pcol.upload(pcol.data(), pcol.size(), c.stream));
     VFVM_HIP_TRY(I.pdiag.upload(pdiag.data(), pdiag.size(), c.stream));
+    ++I.generation;
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     I.h_color_ptr = cptr;
     I.ready = true;
@@ -720,92 +783,162 @@ int ilu_factor(Ctx& c, const SysView& sy, T* lu, const double* val) {
     return VFVM_OK;
 }
 
-// z = (LU)^-1 rhs: forward sweep over the colours ascending (the last one also finishing
-// its backward sweep), backward sweep over the other colours descending, in the permuted
-// work vector zp: 2 colours - 1 launches.
+// z = (LU)^-1 rhs in the colour-major space: forward sweep over the colours ascending (the
+// last one also finishing its backward sweep), backward sweep over the other colours
+// descending, in place in z (z must not alias rhs): 2 colours - 1 launches.
 template <class T>
-int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, T* z, T* zp, const int* ctrl) {
+int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double* rscale, T* z, const int* ctrl) {
     constexpr int LPR = 4;
     const IluData& I = *sy.ilu;
     const int ncol = (int)I.h_color_ptr.size() - 1;
     // algorithmic bytes of one sweep launch over r rows: the triangle's entries (value +
-    // column, about half of each row), rp/diag/perm (12 B), rhs or z reads and writes
+    // column, about half of each row), rp/diag (8 B), the row's rhs/z reads and writes
     const double per_row_entries = 0.5 * (double)sy.nnz / (double)sy.n * (sizeof(T) + 4.0);
     for (int q = 0; q < ncol; ++q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
         const dim3 grid(grid_for((int64_t)(e - s) * LPR));
+        const double bytes = (double)(e - s) * (per_row_entries + 8.0 + 2.0 * sizeof(T));
         if (q == ncol - 1)
-            launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 3.0 * sizeof(T)),
-                         k_ilu_lower<T, LPR, true>, grid, dim3(BLK), s, e, I.perm.p, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, zp, z, ctrl);
+            launch_timed(c, VFVM_K_PRECOND, bytes, k_ilu_lower<T, LPR, true>, grid, dim3(BLK), s, e, I.prp.p, I.pcol.p,
+                         I.pdiag.p, lu, rhs, rscale, z, ctrl);
         else
-            launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 2.0 * sizeof(T)),
-                         k_ilu_lower<T, LPR, false>, grid, dim3(BLK), s, e, I.perm.p, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, zp, z, ctrl);
+            launch_timed(c, VFVM_K_PRECOND, bytes, k_ilu_lower<T, LPR, false>, grid, dim3(BLK), s, e, I.prp.p, I.pcol.p,
+                         I.pdiag.p, lu, rhs, rscale, z, ctrl);
     }
     for (int q = ncol - 2; q >= 0; --q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
-        launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 12.0 + 3.0 * sizeof(T)),
+        launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 8.0 + 2.0 * sizeof(T)),
                      k_ilu_upper<T, LPR>, dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK),
-                     s, e, I.perm.p, I.prp.p, I.pcol.p, I.pdiag.p, lu, zp, z, ctrl);
+                     s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, z, ctrl);
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
 }
 
-// One restart cycle of GMRES on A M^-1 (M = D through the scaled matrix for Jacobi,
-// M = LU for ILU(0)) from V[0] (normalised).  Leaves ctrl[1] = k steps, y = H^-1 g.
+// The launches of Arnoldi step j of the cycle on A M^-1 (M = D through the scaled matrix for
+// Jacobi, M = LU for ILU(0)).
 template <class T>
-int arnoldi_cycle(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, T* zp, const T* aval, const T* lu, bool reorth, int check_every,
-                  int* steps_out) {
+int issue_step(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* aval, const T* lu, bool reorth, int j) {
     const int64_t n = w.n;
     const int lpr = pick_lpr(n, w.nnz);
-    int steps = 0;
-    bool synced_last = false;
     double* part = c.part.p;
     double* part_nrm = part + (int64_t)(MAX_RESTART + 1) * w.nwav;
     const double bytes_spmv = (double)w.nnz * (sizeof(T) + 4) + 4.0 * (n + 1) + 2.0 * sizeof(T) * n;
-    for (int j = 0; j < w.m; ++j) {
-        c.cur_step = j;
-        T* vj = V + (int64_t)j * n;
-        const T* xin = vj;
-        if (lu) {
-            if (int st = ilu_apply<T>(c, sy, lu, vj, zv, zp, w.ctrl)) return st;
-            xin = zv;
-        }
-        if (int st = spmv_launch<T>(c, sy, lpr, n, aval, xin, wv, w.ctrl, bytes_spmv)) return st;
-        const int nv = j + 1;
-        const double bytes_upd = (double)sizeof(T) * n * (nv + 2);
-        for (int pass = 0; pass < (reorth ? 2 : 1); ++pass) {
-            double* hout = w.S + (pass == 0 ? w.L.h1() : w.L.h2());
-            const bool last = pass == (reorth ? 1 : 0);
-            mdot_launch<T>(c, w, nv, V, wv, part);
-            hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, w.nwav, part, hout, w.ctrl);
-            launch_timed(c, VFVM_K_UPDATE, bytes_upd, k_update<T>, dim3(w.nblk), dim3(BLK), n, nv, (const T*)V, n,
-                         (const double*)hout, wv, last ? part_nrm : (double*)nullptr, w.rows_per_blk, (const int*)w.ctrl);
-        }
-        hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(BLK), 0, c.stream, j, reorth ? 1 : 0, w.nblk, part_nrm, w.S, w.L, w.ctrl);
-        hipLaunchKernelGGL((k_scale<T, T>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, wv, V + (int64_t)(j + 1) * n, w.S, w.L.misc() + 1, w.ctrl);
-        VFVM_HIP_TRY(hipGetLastError());
-        c.cur_step = -1;
-        synced_last = false;
-        if ((j + 1) % check_every == 0 || j + 1 == w.m) {
-            VFVM_HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, CTRL_INTS * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-            VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
-            synced_last = true;
-            steps = w.h_ctrl[1];
-            if (!c.pending.empty()) {
-                for (auto& p : c.pending)
-                    if (p.step >= steps) p.bytes = -1.0;  // launches after convergence returned at once
-                if (int st = ktime_flush(c)) return st;
-            }
-            if (w.h_ctrl[0] == 0) break;
-        }
+    // v_j = sig_j V[j]: the scale enters the preconditioner's right-hand side, or the SpMV
+    // output when there is no ILU
+    T* vj = V + (int64_t)j * w.ldv;
+    const double* sigj = w.S + w.L.sig() + j;
+    const T* xin = vj;
+    const double* yscale = sigj;
+    if (lu) {
+        if (int st = ilu_apply<T>(c, sy, lu, vj, sigj, zv, w.ctrl)) return st;
+        xin = zv;
+        yscale = nullptr;
     }
-    if (!synced_last) {
+    if (int st = spmv_launch<T>(c, sy, lpr, n, aval, xin, wv, yscale, w.ctrl, bytes_spmv)) return st;
+    // V[j+1] = w - sum_k v_k h_k (CGS; CGS2 repeats the pass on V[j+1]), then the Arnoldi step
+    const int nv = j + 1;
+    T* vnext = V + (int64_t)(j + 1) * w.ldv;
+    const double bytes_upd = (double)sizeof(T) * n * (nv + 2);
+    for (int pass = 0; pass < (reorth ? 2 : 1); ++pass) {
+        double* hout = w.S + (pass == 0 ? w.L.h1() : w.L.h2());
+        const bool last = pass == (reorth ? 1 : 0);
+        const T* src = pass == 0 ? (const T*)wv : (const T*)vnext;
+        mdot_launch<T>(c, w, nv, V, src, part, hout);
+        launch_timed(c, VFVM_K_UPDATE, bytes_upd, k_update<T>, dim3(w.nblk), dim3(BLK), n, nv, (const T*)V, w.ldv,
+                     (const double*)hout, (const double*)(w.S + w.L.sig()), src, vnext,
+                     last ? part_nrm : (double*)nullptr, w.rows_per_blk, (const int*)w.ctrl);
+    }
+    hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(BLK), 0, c.stream, j, reorth ? 1 : 0, w.nblk, (const double*)part_nrm,
+                       w.S, w.L, w.ctrl);
+    VFVM_HIP_TRY(hipGetLastError());
+    return VFVM_OK;
+}
+
+// Cycle graphs: the launches of steps [CHECK * q, CHECK * q + CHECK) of a cycle captured as
+// one HIP graph each, keyed by every pointer and size baked into them; replayed for every
+// cycle of every solve with the same key.  Launches after convergence return at once (the
+// kernels test the control word).
+constexpr int CHECK = 8;
+
+template <class T>
+CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* aval, const T* lu, bool reorth,
+                          int* status) {
+    const IluData* I = lu ? sy.ilu : nullptr;
+    std::array<uint64_t, 20> key = {(uint64_t)(uintptr_t)V, (uint64_t)(uintptr_t)wv, (uint64_t)(uintptr_t)zv,
+                                    (uint64_t)(uintptr_t)aval, (uint64_t)(uintptr_t)lu, (uint64_t)(uintptr_t)w.S,
+                                    (uint64_t)(uintptr_t)w.ctrl, (uint64_t)(uintptr_t)c.part.p,
+                                    (uint64_t)(uintptr_t)sy.row, (uint64_t)(uintptr_t)sy.col, (uint64_t)w.n,
+                                    (uint64_t)w.nnz, (uint64_t)w.m, (uint64_t)reorth, (uint64_t)sizeof(T),
+                                    (uint64_t)(uintptr_t)I, I ? I->generation : 0, (uint64_t)w.nblk, (uint64_t)w.nwav, (uint64_t)w.ldv};
+    for (auto& g : c.graphs)
+        if (g.key == key) { g.last_use = ++c.graph_clock; return &g; }
+    if (c.graphs.size() >= 4) {
+        auto old = std::min_element(c.graphs.begin(), c.graphs.end(),
+                                    [](const CycleGraphs& a, const CycleGraphs& b) { return a.last_use < b.last_use; });
+        for (auto e : old->chunks) (void)hipGraphExecDestroy(e);
+        c.graphs.erase(old);
+    }
+    CycleGraphs g;
+    g.key = key;
+    g.last_use = ++c.graph_clock;
+    for (int j0 = 0; j0 < w.m; j0 += CHECK) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) { *status = set_error(VFVM_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e)); break; }
+        int st = VFVM_OK;
+        for (int j = j0; j < std::min(w.m, j0 + CHECK) && st == VFVM_OK; ++j)
+            st = issue_step<T>(c, sy, w, V, wv, zv, aval, lu, reorth, j);
+        e = hipStreamEndCapture(c.stream, &graph);
+        if (st != VFVM_OK) { *status = st; if (graph) (void)hipGraphDestroy(graph); break; }
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) { *status = set_error(VFVM_ERR_HIP, std::string("cycle graph: ") + hipGetErrorString(e)); break; }
+        g.chunks.push_back(exec);
+    }
+    if ((int)g.chunks.size() * CHECK < w.m) {
+        for (auto e : g.chunks) (void)hipGraphExecDestroy(e);
+        return nullptr;
+    }
+    c.graphs.push_back(std::move(g));
+    return &c.graphs.back();
+}
+
+// One restart cycle from V[0] (normalised): leaves ctrl[1] = k steps, y = H^-1 g.  Runs the
+// cycle graphs, or the launches one by one when kernel timing is on (the per-launch events
+// need the eager path) or graphs are off (VFVM_GRAPHS=0).
+template <class T>
+int arnoldi_cycle(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* aval, const T* lu, bool reorth,
+                  int* steps_out) {
+    int steps = 0;
+    CycleGraphs* cg = nullptr;
+    if (c.timing_mask == 0 && c.graphs_enabled) {
+        int status = VFVM_OK;
+        cg = cycle_graphs<T>(c, sy, w, V, wv, zv, aval, lu, reorth, &status);
+        if (status != VFVM_OK) return status;
+    }
+    for (int j0 = 0; j0 < w.m; j0 += CHECK) {
+        if (cg) {
+            VFVM_HIP_TRY(hipGraphLaunch(cg->chunks[j0 / CHECK], c.stream));
+        } else {
+            for (int j = j0; j < std::min(w.m, j0 + CHECK); ++j) {
+                c.cur_step = j;
+                if (int st = issue_step<T>(c, sy, w, V, wv, zv, aval, lu, reorth, j)) return st;
+            }
+            c.cur_step = -1;
+        }
         VFVM_HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, CTRL_INTS * sizeof(int), hipMemcpyDeviceToHost, c.stream));
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         steps = w.h_ctrl[1];
+        if (!c.pending.empty()) {
+            for (auto& p : c.pending)
+                if (p.step >= steps) p.bytes = -1.0;  // launches after convergence returned at once
+            if (int st = ktime_flush(c)) return st;
+        }
+        if (w.h_ctrl[0] == 0) break;
     }
     hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(WAVE), 0, c.stream, w.S, w.L, w.ctrl);
     VFVM_HIP_TRY(hipGetLastError());
@@ -854,18 +987,25 @@ int precond_apply(Ctx& c, const vfvm_solve_opts& o, const double* r_host, double
         if (o.precision == VFVM_FP64) {
             VFVM_HIP_TRY(c.ilu64.alloc(sy.nnz));
             if (int s = ilu_factor<double>(c, sy, c.ilu64.p, sy.vals)) return s;
-            VFVM_HIP_TRY(c.zp64.alloc(n));
-            if (int s = ilu_apply<double>(c, sy, c.ilu64.p, r.p, z.p, c.zp64.p, nullptr)) return s;
+            DevBuf<double> rp_, zp_;
+            VFVM_HIP_TRY(rp_.alloc(n));
+            VFVM_HIP_TRY(zp_.alloc(n));
+            hipLaunchKernelGGL((k_gather_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.ilu->perm.p, (const double*)r.p, rp_.p);
+            if (int s = ilu_apply<double>(c, sy, c.ilu64.p, rp_.p, nullptr, zp_.p, nullptr)) return s;
+            hipLaunchKernelGGL((k_scatter_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.ilu->perm.p, (const double*)zp_.p, z.p);
         } else {
             DevBuf<float> r32, z32;
             VFVM_HIP_TRY(r32.alloc(n));
             VFVM_HIP_TRY(z32.alloc(n));
             VFVM_HIP_TRY(c.ilu32.alloc(sy.nnz));
             if (int s = ilu_factor<float>(c, sy, c.ilu32.p, sy.vals)) return s;
-            hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, r.p, r32.p);
-            VFVM_HIP_TRY(c.zp32.alloc(n));
-            if (int s = ilu_apply<float>(c, sy, c.ilu32.p, r32.p, z32.p, c.zp32.p, nullptr)) return s;
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z32.p, z.p);
+            DevBuf<double> zp_;
+            VFVM_HIP_TRY(zp_.alloc(n));
+            hipLaunchKernelGGL((k_gather_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.ilu->perm.p, (const double*)r.p, zp_.p);
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, (const double*)zp_.p, r32.p);
+            if (int s = ilu_apply<float>(c, sy, c.ilu32.p, r32.p, nullptr, z32.p, nullptr)) return s;
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, (const float*)z32.p, zp_.p);
+            hipLaunchKernelGGL((k_scatter_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.ilu->perm.p, (const double*)zp_.p, z.p);
             VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         }
     } else {
@@ -909,7 +1049,7 @@ int residual_norm(Ctx& c, const SysView& sy, const double* x_dev, double* rel, d
     return VFVM_OK;
 }
 
-int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, vfvm_solve_stats& st) {
+int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, vfvm_solve_stats& st) {
     std::memset(&st, 0, sizeof(st));
     if (!(o.tol > 0.0)) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_TOLERANCE, "GMRES: tolerance must be positive");
     if (o.restart <= 0 || o.restart > MAX_RESTART) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_RESTART, "GMRES: restart must be in [1, 256]");
@@ -919,6 +1059,26 @@ int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, 
     if (o.ortho < VFVM_ORTHO_AUTO || o.ortho > VFVM_ORTHO_CGS2) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown orthogonalisation");
     const bool mixed = o.precision == VFVM_FP32_REFINED;
     const bool reorth = o.ortho == VFVM_ORTHO_CGS2 || (o.ortho == VFVM_ORTHO_AUTO && !mixed);
+    const bool ilu = o.precond == VFVM_PRECOND_ILU0;
+    const bool jac = o.precond == VFVM_PRECOND_JACOBI;
+    // With ILU(0) the whole solve runs in the colour-major space of the factor: A' = P A P^T
+    // shares the factor's index arrays, b' = P b, x' = P x, so the triangular sweeps read and
+    // write their rows in place, with no indirection.
+    SysView sy = sy0;
+    double* x = x0;
+    if (ilu) {
+        if (int s = ilu_setup(c, sy0)) return s;
+        const IluData& I = *sy0.ilu;
+        VFVM_HIP_TRY(c.ap64.alloc(sy0.nnz));
+        VFVM_HIP_TRY(c.bp64.alloc(sy0.n));
+        VFVM_HIP_TRY(c.xp64.alloc(sy0.n));
+        sy.row = I.prp.p;
+        sy.col = I.pcol.p;
+        sy.dslot = I.pdiag.p;
+        sy.vals = c.ap64.p;
+        sy.b = c.bp64.p;
+        x = c.xp64.p;
+    }
     const int64_t n = sy.n;
     const int m = o.restart;
     Work w;
@@ -926,10 +1086,11 @@ int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, 
     w.nnz = sy.nnz;
     w.m = m;
     w.nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 4 * BLK - 1) / (4 * BLK)));
-    w.rows_per_blk = (n + w.nblk - 1) / w.nblk;
+    w.rows_per_blk = ((n + w.nblk - 1) / w.nblk + 3) / 4 * 4;
     w.nwav = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + WAVE * 8 - 1) / (WAVE * 8)));
     w.nwav = (w.nwav + BLK / WAVE - 1) / (BLK / WAVE) * (BLK / WAVE);
-    w.rows_per_wave = (n + w.nwav - 1) / w.nwav;
+    w.rows_per_wave = ((n + w.nwav - 1) / w.nwav + 3) / 4 * 4;
+    w.ldv = (n + 63) / 64 * 64;
     w.L.m = m;
     const int lpr = pick_lpr(n, sy.nnz);
     const int nblk_r = (int)grid_for(n * lpr);
@@ -945,16 +1106,9 @@ int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, 
     VFVM_HIP_TRY(c.dinv64.alloc(n));
     VFVM_HIP_TRY(c.w64.alloc(n));
     VFVM_HIP_TRY(c.z64.alloc(n));
-    const bool ilu = o.precond == VFVM_PRECOND_ILU0;
-    const bool jac = o.precond == VFVM_PRECOND_JACOBI;
-    if (ilu) {
-        if (int s = ilu_setup(c, sy)) return s;
-        if (mixed) VFVM_HIP_TRY(c.zp32.alloc(n));
-        else VFVM_HIP_TRY(c.zp64.alloc(n));
-    }
-    if (!mixed) VFVM_HIP_TRY(c.V64.alloc((size_t)n * (m + 1)));
+    if (!mixed) VFVM_HIP_TRY(c.V64.alloc((size_t)w.ldv * (m + 1)));
     else {
-        VFVM_HIP_TRY(c.V32.alloc((size_t)n * (m + 1)));
+        VFVM_HIP_TRY(c.V32.alloc((size_t)w.ldv * (m + 1)));
         VFVM_HIP_TRY(c.w32.alloc(2 * (size_t)n));
     }
 
@@ -962,7 +1116,13 @@ int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, 
     VFVM_HIP_TRY(ev.create());
     VFVM_HIP_TRY(hipEventRecord(ev.e[0], c.stream));
 
-    // preconditioner set-up (inside the timed solve)
+    // permuted system and preconditioner set-up (inside the timed solve)
+    if (ilu) {
+        const IluData& I = *sy0.ilu;
+        hipLaunchKernelGGL((k_ilu_gather<double>), dim3(grid_for(sy0.nnz)), dim3(BLK), 0, c.stream, sy0.nnz, I.src.p, sy0.vals, c.ap64.p);
+        hipLaunchKernelGGL((k_gather_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, I.perm.p, sy0.b, c.bp64.p);
+        hipLaunchKernelGGL((k_gather_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, I.perm.p, (const double*)x0, c.xp64.p);
+    }
     hipLaunchKernelGGL(k_dinv, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.dslot, sy.vals, c.dinv64.p, jac ? 1 : 0);
     const double* dinv_x = jac ? c.dinv64.p : nullptr;
     const void* aval = nullptr;
@@ -977,7 +1137,7 @@ int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, 
         }
         if (ilu) {
             VFVM_HIP_TRY(c.ilu64.alloc(sy.nnz));
-            if (int s = ilu_factor<double>(c, sy, c.ilu64.p, sy.vals)) return s;
+            if (int s = ilu_factor<double>(c, sy0, c.ilu64.p, sy0.vals)) return s;
             lu = c.ilu64.p;
         }
     } else {
@@ -987,14 +1147,13 @@ int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, 
         aval = c.avals32.p;
         if (ilu) {
             VFVM_HIP_TRY(c.ilu32.alloc(sy.nnz));
-            if (int s = ilu_factor<float>(c, sy, c.ilu32.p, sy.vals)) return s;
+            if (int s = ilu_factor<float>(c, sy0, c.ilu32.p, sy0.vals)) return s;
             lu = c.ilu32.p;
         }
     }
     VFVM_HIP_TRY(hipGetLastError());
     VFVM_HIP_TRY(hipEventRecord(ev.e[1], c.stream));
 
-    const int check_every = 8;
     double* part_r = c.part.p + (size_t)(MAX_RESTART + 1) * w.nwav + w.nblk;
     int total_steps = 0, cycles = 0, refinements = 0;
     double rel = 0.0, bnorm = 0.0;
@@ -1019,12 +1178,12 @@ int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, 
         int steps = 0;
         if (!mixed) {
             hipLaunchKernelGGL((k_scale<double, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V64.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<double>(c, sy, w, c.V64.p, c.w64.p, c.z64.p, c.zp64.p, (const double*)aval, (const double*)lu, reorth, check_every, &steps)) return s;
+            if (int s = arnoldi_cycle<double>(c, sy, w, c.V64.p, c.w64.p, c.z64.p, (const double*)aval, (const double*)lu, reorth, &steps)) return s;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y)
-            hipLaunchKernelGGL((k_combine<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V64.p, n, w.S, w.L, w.ctrl, c.w64.p);
+            hipLaunchKernelGGL((k_combine<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V64.p, w.ldv, w.S, w.L, w.ctrl, c.w64.p);
             if (lu) {
-                if (int s = ilu_apply<double>(c, sy, (const double*)lu, (const double*)c.w64.p, c.z64.p, c.zp64.p, (const int*)nullptr)) return s;
+                if (int s = ilu_apply<double>(c, sy, (const double*)lu, (const double*)c.w64.p, nullptr, c.z64.p, (const int*)nullptr)) return s;
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
             } else {
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.w64.p, dinv_x, x);
@@ -1032,16 +1191,16 @@ int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, 
         } else {
             // inner FP32 solve of A M^-1 u = r from u = 0, V0 = r / beta in FP32
             hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V32.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<float>(c, sy, w, c.V32.p, c.w32.p, c.w32.p + n, c.zp32.p, (const float*)aval, (const float*)lu, reorth, check_every, &steps)) return s;
+            if (int s = arnoldi_cycle<float>(c, sy, w, c.V32.p, c.w32.p, c.w32.p + n, (const float*)aval, (const float*)lu, reorth, &steps)) return s;
             ++refinements;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y), V y accumulated in FP64
-            hipLaunchKernelGGL((k_combine<float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V32.p, n, w.S, w.L, w.ctrl, c.z64.p);
+            hipLaunchKernelGGL((k_combine<float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V32.p, w.ldv, w.S, w.L, w.ctrl, c.z64.p);
             if (lu) {
                 float* t32 = c.w32.p;
                 float* z32 = c.w32.p + n;
                 hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, t32, w.S, w.L.misc() + 4, (const int*)nullptr);
-                if (int s = ilu_apply<float>(c, sy, (const float*)lu, (const float*)t32, z32, c.zp32.p, (const int*)nullptr)) return s;
+                if (int s = ilu_apply<float>(c, sy, (const float*)lu, (const float*)t32, nullptr, z32, (const int*)nullptr)) return s;
                 hipLaunchKernelGGL((k_scale<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z32, c.z64.p, w.S, w.L.misc() + 4, (const int*)nullptr);
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
             } else {
@@ -1052,6 +1211,7 @@ int gmres_solve(Ctx& c, const SysView& sy, const vfvm_solve_opts& o, double* x, 
         total_steps += steps;
         ++cycles;
     }
+    if (ilu) hipLaunchKernelGGL((k_scatter_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy0.ilu->perm.p, (const double*)x, x0);
     VFVM_HIP_TRY(hipEventRecord(ev.e[2], c.stream));
     VFVM_HIP_TRY(hipEventSynchronize(ev.e[2]));
     float ms_all = 0.f, ms_setup = 0.f;
