@@ -158,6 +158,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="first copy MB megabytes device-to-device 5 times (a plain copy kernel of known size, "
+                         "used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--variants", default="fp64,mixed")
     ap.add_argument("--ortho", default="cgs", choices=["auto", "cgs", "cgs2"])
     args = ap.parse_args()
@@ -170,6 +173,13 @@ def main():
     import torch
     import vfvm
 
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     pc = {"jacobi": vfvm.PRECOND_JACOBI, "ilu0": vfvm.PRECOND_ILU0, "none": vfvm.PRECOND_NONE}[args.precond]
     t0 = time.time()
     mesh, ctx, ops = build_problem(vfvm, args.n)
