@@ -132,6 +132,20 @@ def run_adjoint(vfvm, args):
     }
 
 
+def measured_traffic(kernel, variant):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "ilu_sweep_traffic_convdiff_100cubed.md")
+    if variant != "mixed" or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -291,7 +305,8 @@ def main():
                      "gbs": asm_bytes / (asm_ms * 1e-3) / 1e9 if asm_ms > 0 else None},
         "mesh_and_dual_s": t_mesh,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_name, head),
+                     "algorithmic_bytes_per_launch": dom["bytes_per_launch"]},
         "cpu_baseline": cpu,
         "adjoint": adjoint,
     }

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ctx.hpp"
+#include "xcd.hpp"
 
 namespace vfvm {
 
@@ -26,7 +27,7 @@ template <int LPR>
 __global__ void __launch_bounds__(BLK) k_grad(int64_t N, int64_t E, const int32_t* __restrict__ rp,
                                               const int32_t* __restrict__ ngh, const double* __restrict__ gw,
                                               const double* __restrict__ val, double* __restrict__ grad) {
-    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
     const int64_t v = tid / LPR;
     const int lane = (int)(tid % LPR);
     if (v >= N) return;
@@ -69,7 +70,7 @@ __global__ void __launch_bounds__(BLK) k_grad(int64_t N, int64_t E, const int32_
 __global__ void __launch_bounds__(BLK) k_grad_vec(int64_t N, int64_t E, const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ ngh, const double* __restrict__ gw,
                                                   const double* __restrict__ val, double* __restrict__ grad) {
-    int64_t v = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    int64_t v = (int64_t)xcd_block() * BLK + threadIdx.x;
     if (v >= N) return;
     double uc[3] = {val[v], val[N + v], val[2 * N + v]};
     double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -96,7 +97,7 @@ __global__ void __launch_bounds__(BLK) k_flux_prescribed(int64_t N, int64_t E, c
                                                          const double* __restrict__ den, const double* __restrict__ dgrad,
                                                          const double* __restrict__ vel, const double* __restrict__ vgrad,
                                                          double* __restrict__ flux) {
-    int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    int64_t e = (int64_t)xcd_block() * BLK + threadIdx.x;
     if (e >= E) return;
     const int64_t v = erow[e], n = ngh[e];
     const double sx = skew[e], sy = skew[E + e], sz = skew[2 * E + e];
@@ -231,7 +232,7 @@ __device__ inline void vertex_op(const DevOp& op, const AsmArgs& a, int64_t v, i
 // Rows of higher degree take the sequential walk in their first lane.
 template <int LPR>
 __global__ void __launch_bounds__(BLK) k_assemble(AsmArgs a) {
-    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
     const int64_t v = tid / LPR;
     const int lane = (int)(tid % LPR);
     if (v >= a.N) return;  // whole row groups leave together (BLK is a multiple of LPR)

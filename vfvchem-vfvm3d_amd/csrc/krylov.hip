@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "xcd.hpp"
 
 
 namespace vfvm {
@@ -81,7 +82,7 @@ __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restri
                                               const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
                                               const double* __restrict__ scale, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
-    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
     const int64_t row = tid / LPR;
     const int lane = (int)(tid % LPR);
     if (row >= n) return;
@@ -92,29 +93,28 @@ __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restri
     if (lane == 0) y[row] = scale ? (T)(s * (T)scale[0]) : s;
 }
 
-// r = b - A x (FP64) and per-block partial sums of r^2 and b^2
+// r = b - A x (FP64) and per-block partial sums of r^2 and b^2 (grid-stride over rows)
 template <int LPR>
 __global__ void __launch_bounds__(BLK) k_residual(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                   const double* __restrict__ val, const double* __restrict__ x,
                                                   const double* __restrict__ b, double* __restrict__ r,
                                                   double* __restrict__ part, int nblk) {
     __shared__ double sh[BLK / WAVE];
-    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    const int64_t row = tid / LPR;
-    const int lane = (int)(tid % LPR);
-    double s = 0.0;
-    if (row < n) {
+    const int lane = (int)(threadIdx.x % LPR);
+    const int64_t rows_per_pass = (int64_t)gridDim.x * (BLK / LPR);
+    double rr = 0.0, bb = 0.0;
+    for (int64_t row = ((int64_t)xcd_block() * BLK + threadIdx.x) / LPR; row < n; row += rows_per_pass) {
+        double s = 0.0;
         const int32_t e1 = rp[row + 1];
         for (int32_t k = rp[row] + lane; k < e1; k += LPR) s += val[k] * x[col[k]];
-    }
-    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-    double rr = 0.0, bb = 0.0;
-    if (row < n && lane == 0) {
-        double bi = b[row];
-        double ri = bi - s;
-        r[row] = ri;
-        rr = ri * ri;
-        bb = bi * bi;
+        for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
+        if (lane == 0) {
+            const double bi = b[row];
+            const double ri = bi - s;
+            r[row] = ri;
+            rr += ri * ri;
+            bb += bi * bi;
+        }
     }
     double t1 = block_sum(rr, sh);
     double t2 = block_sum(bb, sh);
@@ -337,20 +337,36 @@ __global__ void __launch_bounds__(BLK) k_cycle_start(int nblk, const double* __r
     }
 }
 
-// y = H(1:k,1:k) \ g(1:k), k = steps done
-__global__ void k_backsolve(double* __restrict__ S, StateLayout L, const int* __restrict__ ctrl) {
-    if (threadIdx.x != 0) return;
+// y = H(1:k,1:k) \ g(1:k), k = steps done: one wave, H staged in LDS, each row's inner
+// product summed across the lanes in a fixed order
+__global__ void __launch_bounds__(WAVE) k_backsolve(double* __restrict__ S, StateLayout L, const int* __restrict__ ctrl) {
+    __shared__ double Hs[64 * 64];
+    __shared__ double ys[MAX_RESTART];
     const int k = ctrl[1];
     const int m = L.m;
+    const int lane = threadIdx.x;
     const double* H = S + L.H();
     const double* g = S + L.g();
-    double* y = S + L.y();
+    // columns 0..k-1, rows 0..k-1 (k <= 64 staged; longer cycles read H directly)
+    const bool staged = k <= 64;
+    if (staged)
+        for (int idx = lane; idx < k * k; idx += WAVE) {
+            const int l = idx / k, i = idx % k;
+            Hs[l * 64 + i] = H[(int64_t)l * (m + 1) + i];
+        }
+    __syncthreads();
     for (int i = k - 1; i >= 0; --i) {
-        double s = g[i];
-        for (int l = i + 1; l < k; ++l) s -= H[(int64_t)l * (m + 1) + i] * y[l];
-        double d = H[(int64_t)i * (m + 1) + i];
-        y[i] = d != 0.0 ? s / d : 0.0;
+        double p = 0.0;
+        for (int l = i + 1 + lane; l < k; l += WAVE)
+            p += (staged ? Hs[l * 64 + i] : H[(int64_t)l * (m + 1) + i]) * ys[l];
+        p = wave_sum(p);
+        if (lane == 0) {
+            const double d = staged ? Hs[i * 64 + i] : H[(int64_t)i * (m + 1) + i];
+            ys[i] = d != 0.0 ? (g[i] - p) / d : 0.0;
+        }
+        __syncthreads();
     }
+    for (int i = lane; i < k; i += WAVE) S[L.y() + i] = ys[i];
 }
 
 // z = sum_{l<k} v_l y[l] = sum_l V[l] (sig_l y[l]) (z in FP64)
@@ -440,7 +456,7 @@ __global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const
                                                    const double* __restrict__ rscale, T* __restrict__ z,
                                                    const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
-    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
     const int32_t t = t0 + (int32_t)(tid / LPR);
     const int lane = (int)(tid % LPR);
     if (t >= t1) return;
@@ -457,7 +473,7 @@ __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const
                                                    const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
                                                    const T* __restrict__ lu, T* __restrict__ z, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
-    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
     const int32_t t = t0 + (int32_t)(tid / LPR);
     const int lane = (int)(tid % LPR);
     if (t >= t1) return;
@@ -1034,7 +1050,7 @@ int ilu_order(Ctx& c, int32_t* perm, int32_t* color_ptr, int32_t* ncolors) {
 int residual_norm(Ctx& c, const SysView& sy, const double* x_dev, double* rel, double* bnorm) {
     const int64_t n = sy.n;
     const int lpr = pick_lpr(n, sy.nnz);
-    const int nblk_r = (int)grid_for(n * lpr);
+    const int nblk_r = (int)std::min<int64_t>(1024, grid_for(n * lpr));
     VFVM_HIP_TRY(c.r64.alloc(n));
     DevBuf<double> part;
     VFVM_HIP_TRY(part.alloc(2 * (size_t)nblk_r));
@@ -1093,7 +1109,7 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     w.ldv = (n + 63) / 64 * 64;
     w.L.m = m;
     const int lpr = pick_lpr(n, sy.nnz);
-    const int nblk_r = (int)grid_for(n * lpr);
+    const int nblk_r = (int)std::min<int64_t>(1024, grid_for(n * lpr));
 
     // workspace
     VFVM_HIP_TRY(c.scal.alloc((size_t)w.L.total() + 64));
