@@ -23,6 +23,10 @@
  *   LinearBase::solve_matrix           src/linear/linear_gmres.rs:76-84 -> vfvm_solve
  *   SteadyBase::solve                  src/steady/steady_base.rs:16-135 -> vfvm_steady_solve
  *   TransientBase::solve               src/transient/transient_base.rs:16-160 -> vfvm_transient_solve
+ *   SteadyFlow (flow operator list)    src/steady/core/steady_flow.rs:133-303 -> vfvm_op kinds 40-62 with
+ *                                      vfvm_pattern_build / vfvm_assemble / vfvm_steady_solve
+ *   flow_helper::update_flow           src/operator/core/flow_helper.rs:3-179 -> vfvm_update_flow
+ *   Variables::add_vecbnd_{cnst,func}  src/base/vars.rs:169-184           -> vfvm_vecbnd_add
  */
 #ifndef VFVM_H
 #define VFVM_H
@@ -79,6 +83,7 @@ typedef struct vfvm_mesh_info {
     int64_t num_subfaces; /* boundary sub-faces                                      */
     int32_t num_bnd;      /* boundary groups                                         */
     int32_t max_degree;
+    int64_t num_grad_coeff; /* pressure-gradient coefficients (0 until built)         */
 } vfvm_mesh_info;
 
 /* Jittered unit-cube lattice of n^3 vertices (h = 1/(n-1)), jitter uniform in
@@ -101,6 +106,12 @@ int vfvm_mesh_from_tets(int64_t num_pts, int64_t num_nds, const double* pos,
 
 int vfvm_mesh_info_get(const vfvm_mesh* mesh, vfvm_mesh_info* info);
 
+/* Builds the pressure-gradient coefficients of the flow operators: for every vertex v,
+ * int_V grad(p) dV = sum_m (c_x, c_y, c_z)_m p_m over the 2-ring m of v, with the skew
+ * correction of the face values fully implicit (Tetrahedron::grad_coeff,
+ * src/base/geom_tet.rs:295-347).  Needed before uploading a mesh for the flow operators. */
+int vfvm_mesh_build_grad_coeff(vfvm_mesh* mesh);
+
 /* Array ids for vfvm_mesh_get (element type and length in brackets). */
 enum vfvm_mesh_array {
     VFVM_MESH_POS = 0,       /* f64 [3N] AoS                         */
@@ -121,7 +132,10 @@ enum vfvm_mesh_array {
     VFVM_MESH_BSF_VK = 15,   /* i32 [S]                              */
     VFVM_MESH_BSF_AREA = 16, /* f64 [S]                              */
     VFVM_MESH_BSF_NORM = 17, /* f64 [3S] SoA                         */
-    VFVM_MESH_BSF_WGT = 18   /* f64 [3S] SoA                         */
+    VFVM_MESH_BSF_WGT = 18,  /* f64 [3S] SoA                         */
+    VFVM_MESH_GC_PTR = 19,   /* i32 [N+1] pressure-gradient coefficients, CSR over vertices */
+    VFVM_MESH_GC_VID = 20,   /* i32 [G] vertex m (the 2-ring of v, ascending)            */
+    VFVM_MESH_GC_C = 21      /* f64 [3G] SoA (c_x, c_y, c_z)                              */
 };
 int vfvm_mesh_get(const vfvm_mesh* mesh, int32_t array_id, void* dst);
 void vfvm_mesh_destroy(vfvm_mesh* mesh);
@@ -153,6 +167,7 @@ int vfvm_scldom_update_prev(vfvm_ctx* ctx, int32_t id);
 /* values: 3N doubles SoA (x[N], y[N], z[N]). */
 int vfvm_vecdom_add(vfvm_ctx* ctx, int32_t kind, const double* cnst3, const double* values, int32_t* id);
 int vfvm_vecdom_set(vfvm_ctx* ctx, int32_t id, const double* values);
+int vfvm_vecdom_get(vfvm_ctx* ctx, int32_t id, double* values);
 /* values: one double per node of boundary bnd_id, in the boundary's node order. */
 int vfvm_sclbnd_add(vfvm_ctx* ctx, int32_t bnd_id, double cnst, const double* values, int32_t* id);
 int vfvm_sclbnd_set(vfvm_ctx* ctx, int32_t id, const double* values);
@@ -160,6 +175,17 @@ int vfvm_sclbnd_set(vfvm_ctx* ctx, int32_t id, const double* values);
  * the face mass flux of flow_helper::update_flux (flow_helper.rs:72-179). */
 int vfvm_sclflw_add(vfvm_ctx* ctx, int32_t vel_id, int32_t pres_id, int32_t den_id,
                     int32_t visc_id, int32_t fce_id, int32_t* id);
+/* Boundary velocity (VectorBoundary, src/base/vec_bnd.rs) for OpVecBndDirichlet: values
+ * are 3 x count doubles SoA (x[count], y[count], z[count]) in the boundary's node order
+ * (NULL -> every node takes cnst3[0..2]). */
+int vfvm_vecbnd_add(vfvm_ctx* ctx, int32_t bnd_id, const double* cnst3, const double* values, int32_t* id);
+/* flow_helper::update_flow (flow_helper.rs:3-179) for a flow with an unknown pressure: the
+ * gradients of its fields, rc = V / a_P, the momentum term rc (grad p - fce_c) and the
+ * Rhie-Chow face mass flux, from the current field values. */
+int vfvm_update_flow(vfvm_ctx* ctx, int32_t flw_id);
+/* Reads a flow's data: what = 0 face mass flux (E, CSR edge order), 1 rc (N),
+ * 2 momentum term (3N SoA). */
+int vfvm_sclflw_get(vfvm_ctx* ctx, int32_t flw_id, int32_t what, double* out);
 
 /* ---- operators: one descriptor per reference operator struct ---- */
 enum vfvm_op_kind {
@@ -176,14 +202,41 @@ enum vfvm_op_kind {
     VFVM_OP_SCL_BND_PRODUCTION = 23,      /* OpSclBndProduction{unk,prd}          opscl_bnd_prod.rs        a = prd               */
     VFVM_OP_SCL_BND_SOURCE = 24,          /* OpSclBndSource{unk,src}              opscl_bnd_src.rs         a = src               */
     VFVM_OP_SCL_BND_TIME = 25,            /* OpSclBndTime{unk,wgt}                opscl_bnd_time.rs        a = wgt               */
-    VFVM_OP_SCL_BND_ADVECTION_UNITY = 26  /* OpSclBndAdvectionUnity{unk,diff?,vel,flw} opscl_bnd_advunity.rs a = diff or -1, b = flw */
+    VFVM_OP_SCL_BND_ADVECTION_UNITY = 26, /* OpSclBndAdvectionUnity{unk,diff?,vel,flw} opscl_bnd_advunity.rs a = diff or -1, b = flw */
+    /* flow operators (SteadyFlow, steady_flow.rs:133-240): unknown velocity vecdom `vel` and
+     * pressure scldom `pres` of one flow (sclflw `flw`, whose den/visc/fce ids the operators
+     * must name); domain operators geom_id = dom_id (0) */
+    VFVM_OP_VEC_DOM_ADVECTION = 40,       /* OpVecDomAdvection{vel,den,visc,flw}  opvec_dom_adv.rs:47     unk = vel, a = den, b = visc, c = flw */
+    VFVM_OP_VEC_DOM_GRADIENT = 41,        /* OpVecDomGradient{vel,pres}           opvec_dom_grad.rs:39    unk = vel, a = pres   */
+    VFVM_OP_VEC_DOM_DIFFUSION = 42,       /* OpVecDomDiffusion{vel,visc}          opvec_dom_diff.rs:39    unk = vel, a = visc   */
+    VFVM_OP_VEC_DOM_TRANSPOSE = 43,       /* OpVecDomTranspose{vel,visc}          opvec_dom_dtrn.rs:39    unk = vel, a = visc   */
+    VFVM_OP_VEC_DOM_DILATATORY = 44,      /* OpVecDomDilatatory{vel,visc}         opvec_dom_ddil.rs:41    unk = vel, a = visc   */
+    VFVM_OP_VEC_DOM_SOURCE = 45,          /* OpVecDomSource{vel,fce}              opvec_dom_src.rs:37     unk = vel, a = fce    */
+    VFVM_OP_SCL_DOM_DIVERGENCE = 46,      /* OpSclDomDivergence{vel,pres,den}     opscl_dom_div.rs:68     unk = pres, a = vel, b = den */
+    VFVM_OP_SCL_DOM_PRESSURE_STABILIZER = 47, /* OpSclDomPressureStabilizer{pres,den,fce,flw} opscl_dom_pstab.rs:52 unk = pres, a = den, b = fce, c = flw */
+    VFVM_OP_SCL_DOM_PRESSURE_REFERENCE = 48,  /* OpSclDomPressureReference{pres,nid,pres_ref} opscl_dom_pref.rs:48 unk = pres, a = nid, value = pres_ref */
+    /* flow boundary operators; geom_id = bnd_id */
+    VFVM_OP_VEC_BND_DIRICHLET = 60,       /* OpVecBndDirichlet{vel,val}           opvec_bnd_dir.rs:56     unk = vel, a = vecbnd value */
+    VFVM_OP_SCL_BND_DIVERGENCE = 61,      /* OpSclBndDivergence{vel,pres,den}     opscl_bnd_div.rs:73     unk = pres, a = vel, b = den */
+    VFVM_OP_SCL_BND_PRESSURE_STABILIZER = 62 /* OpSclBndPressureStabilizer{pres,den,fce,flw} opscl_bnd_pstab.rs:49 unk = pres, a = den, b = fce, c = flw */
 };
 typedef struct vfvm_op {
-    int32_t kind;     /* vfvm_op_kind                          */
-    int32_t geom_id;  /* dom_id (0) or bnd_id                  */
-    int32_t unk_id;   /* unknown scalar (scldom id)            */
-    int32_t a, b;     /* per-kind field ids, see vfvm_op_kind  */
+    int32_t kind;     /* vfvm_op_kind                                         */
+    int32_t geom_id;  /* dom_id (0) or bnd_id                                 */
+    int32_t unk_id;   /* unknown scalar (scldom id) or velocity (vecdom id)   */
+    int32_t a, b, c;  /* per-kind field ids, see vfvm_op_kind (-1: unused)    */
+    double value;     /* per-kind constant (reference pressure), else 0      */
 } vfvm_op;
+/* An operator list that holds flow operators describes the coupled velocity-pressure system
+ * of SteadyFlow: unknown vector [u (N), v (N), w (N), p (N)] (steady_flow.rs:94-104), the
+ * eight domain operators once each in the order of assemble_matrix (steady_flow.rs:244-254),
+ * for every boundary the triple {VEC_BND_DIRICHLET, SCL_BND_DIVERGENCE,
+ * SCL_BND_PRESSURE_STABILIZER} (steady_flow.rs:256-260), and at most one
+ * SCL_DOM_PRESSURE_REFERENCE (applied last, steady_flow.rs:297-300).  The mesh needs its
+ * pressure-gradient coefficients (vfvm_mesh_build_grad_coeff).  The velocity rows of the
+ * points hold [c N + ({v} u ngh(v)) | 3N + grad_coeff(v)], the pressure rows
+ * [c N + grad_coeff(v), c = 0..2 | 3N + 3-ring(v)], all ascending; every entry is the sum of
+ * its pushes in the reference's push order. */
 
 /* Fixes the unknown's block start (SteadyBase::initial_start), builds the row-sorted
  * CSR pattern of the operator list and the edge->slot / diag->slot maps on the host,
@@ -202,7 +255,10 @@ int vfvm_matrix_get(vfvm_ctx* ctx, double* vals, double* b);
 enum vfvm_precision { VFVM_FP64 = 0, VFVM_FP32_REFINED = 1 };
 /* ILU0: ILU(0) of the matrix in multicolour order (greedy colouring of the row graph,
  * rows sorted by (colour, index)); its triangular sweeps run one launch per colour. */
-enum vfvm_precond { VFVM_PRECOND_NONE = 0, VFVM_PRECOND_JACOBI = 1, VFVM_PRECOND_ILU0 = 2 };
+/* BJACOBI: block Jacobi over the unknowns of one vertex (b = mat_size / N unknowns, entries
+ * v, N + v, ..., (b-1) N + v; b <= 6): the b x b diagonal blocks inverted at set-up (Gauss-Jordan
+ * with partial pivoting; a singular block is replaced by the identity). */
+enum vfvm_precond { VFVM_PRECOND_NONE = 0, VFVM_PRECOND_JACOBI = 1, VFVM_PRECOND_ILU0 = 2, VFVM_PRECOND_BJACOBI = 3 };
 /* AUTO: CGS2 (one re-orthogonalisation) for FP64 GMRES, CGS for the FP32 inner solve. */
 enum vfvm_ortho { VFVM_ORTHO_AUTO = 0, VFVM_ORTHO_CGS = 1, VFVM_ORTHO_CGS2 = 2 };
 typedef struct vfvm_solve_opts {

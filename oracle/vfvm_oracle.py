@@ -769,3 +769,411 @@ def ilu0(A, perm):
     L = sp.csr_matrix(np.tril(W, -1) * pat + np.eye(n))
     U = sp.csr_matrix(np.triu(W) * pat)
     return L, U
+
+
+# --------------------------------------------------------------------------- flow (SteadyFlow)
+# Pinning of this section (tests/test_cpu_flow_oracle.py), independent of the restatement:
+#   grad_coeff()     sum_m c_m p_m = V grad(p) for linear p at every vertex (divergence
+#                    theorem, closed form); C++ builder bitwise equal.
+#   update_flow()    flux antisymmetry F(v->n) = -F(n->v); hydrostatic balance (linear p,
+#                    body force = grad p) gives F = rho_f (u_f . n) A exactly
+#                    (opscl_dom_pstab.rs:31); rc > 0 and rc = V / a_P with a_P >= sum cond.
+#   assemble_flow()  momentum rows: velocity row sums 0, gradient block on linear p = -V grad p;
+#                    continuity rows: divergence block on linear u = -rho V div(u) (points and
+#                    nodes), pressure-stabiliser block annihilates linear p; transpose and
+#                    dilatatory b vanish for linear u with constant viscosity; pattern
+#                    2-ring / 3-ring; the reference-pressure row is the identity.
+
+def grad_coeff(g: dict):
+    """Pressure-gradient coefficients, Tetrahedron step 6 (src/base/geom_tet.rs:295-347).
+
+    For vertex v the divergence-theorem gradient with skew-corrected face values,
+    pushed per face (arithmetic mean, then the (p_m - p_s) skew terms of s = v, n) and per
+    boundary sub-face (linear interpolation), stably sorted by vertex and merged by
+    sequential summation.  Returns gc_ptr[N+1], gc_vid[G], gc_c (3, G).
+    """
+    N = g["N"]
+    rp, ngh = g["row_ptr"], g["ngh"]
+    area, norm, skew, gw = g["area"], g["norm"], g["skew"], g["gw"]
+    bp = g["bsf_ptr"]
+    ptr = [0]
+    vid_out, c_out = [], []
+    for v in range(N):
+        t = []
+        for e in range(rp[v], rp[v + 1]):
+            n = int(ngh[e])
+            A = float(area[e])
+            nx, ny, nz = float(norm[0, e]), float(norm[1, e]), float(norm[2, e])
+            sx, sy, sz = float(skew[0, e]), float(skew[1, e]), float(skew[2, e])
+            t.append((v, 0.5 * A * nx, 0.5 * A * ny, 0.5 * A * nz))
+            t.append((n, 0.5 * A * nx, 0.5 * A * ny, 0.5 * A * nz))
+            for s in (v, n):
+                for q in range(rp[s], rp[s + 1]):
+                    wx, wy, wz = float(gw[0, q]), float(gw[1, q]), float(gw[2, q])
+                    ws = 0.5 * (wx * sx + wy * sy + wz * sz) * A
+                    t.append((int(ngh[q]), ws * nx, ws * ny, ws * nz))
+                    t.append((s, -ws * nx, -ws * ny, -ws * nz))
+        for s in range(bp[v], bp[v + 1]):
+            A = float(g["bsf_area"][s])
+            nx, ny, nz = (float(x) for x in g["bsf_norm"][:, s])
+            wv, wj, wk = (float(x) for x in g["bsf_wgt"][:, s])
+            t.append((v, wv * A * nx, wv * A * ny, wv * A * nz))
+            t.append((int(g["bsf_vj"][s]), wj * A * nx, wj * A * ny, wj * A * nz))
+            t.append((int(g["bsf_vk"][s]), wk * A * nx, wk * A * ny, wk * A * nz))
+        t.sort(key=lambda x: x[0])  # stable, as sort_by_key
+        merged = []
+        for m, cx, cy, cz in t:
+            if merged and merged[-1][0] == m:
+                pm = merged[-1]
+                merged[-1] = (m, pm[1] + cx, pm[2] + cy, pm[3] + cz)
+            else:
+                merged.append((m, cx, cy, cz))
+        for m, cx, cy, cz in merged:
+            vid_out.append(m)
+            c_out.append((cx, cy, cz))
+        ptr.append(len(vid_out))
+    return (np.asarray(ptr, np.int64), np.asarray(vid_out, np.int64),
+            np.asarray(c_out, np.float64).reshape(-1, 3).T.copy())
+
+
+def _face_mean(f, fg, v, n, sx, sy, sz):
+    # skew-corrected arithmetic mean (opscl_dom_div.rs:111-112, flow_helper.rs:47)
+    m = 0.5 * (f[v] + f[n])
+    return m + 0.5 * (sx * (fg[0][v] + fg[0][n]) + sy * (fg[1][v] + fg[1][n]) + sz * (fg[2][v] + fg[2][n]))
+
+
+def _visc_face(f, fg, v, n, sx, sy, sz):
+    # harmonic mean + skew correction + guard (opvec_dom_adv.rs:93-95, flow_helper.rs:48-50)
+    fv, fn = f[v], f[n]
+    s = fv + fn
+    with np.errstate(divide="ignore", invalid="ignore"):
+        m = np.where(s != 0.0, 2.0 * fv * fn / s, 0.0)
+    ff = m + 0.5 * (sx * (fg[0][v] + fg[0][n]) + sy * (fg[1][v] + fg[1][n]) + sz * (fg[2][v] + fg[2][n]))
+    return np.where(ff > 0.0, ff, m)
+
+
+def _fce_centroid(g, fce, fceg, vids):
+    # body force at the control-volume centroid (opvec_dom_src.rs:58-61)
+    cx, cy, cz = g["cskew"][:, vids]
+    return [fce[c][vids] + fceg[3 * c][vids] * cx + fceg[3 * c + 1][vids] * cy + fceg[3 * c + 2][vids] * cz for c in range(3)]
+
+
+def update_flow(g: dict, gc, vel, velg, pres, den, deng, visc, viscg, fce, fceg):
+    """flow_helper::update_flow + update_flux (src/operator/core/flow_helper.rs:3-179).
+
+    rc_v = V_v / a_P, a_P = sum_faces cond A(|Pe|) + max(F_c, 0) accumulated in face order;
+    mom_v = rc_v (grad p_v - fce_c,v) with grad p_v = sum_gc c p / V (sequential); face
+    mass flux F = rho_f (u_f . n + vel_rc) A with vel_rc the Rhie-Chow correction.
+    vel (3,N), velg (9,N), den/visc (N), grads (3,N), fce (3,N), fceg (9,N).
+    Returns rc (N), mom (3,N), flux (E).
+    """
+    N = g["N"]
+    rows, ngh = g["rows"].astype(np.int64), g["ngh"].astype(np.int64)
+    sx, sy, sz = g["skew"]
+    nx, ny, nz = g["norm"]
+    A, d = g["area"], g["dist"]
+    den_f = 0.5 * (den[rows] + den[ngh]) + 0.5 * (sx * (deng[0][rows] + deng[0][ngh]) + sy * (deng[1][rows] + deng[1][ngh]) + sz * (deng[2][rows] + deng[2][ngh]))
+    visc_f = _visc_face(visc, viscg, rows, ngh, sx, sy, sz)
+    cond = visc_f * A / d
+    Fc = den_f * 0.5 * ((vel[0][rows] + vel[0][ngh]) * nx + (vel[1][rows] + vel[1][ngh]) * ny + (vel[2][rows] + vel[2][ngh]) * nz) * A
+    with np.errstate(divide="ignore", invalid="ignore"):
+        pe = np.where(cond != 0.0, Fc / cond, 0.0)
+    term = cond * _pow5(np.maximum(1.0 - 0.1 * np.abs(pe), 0.0)) + np.maximum(Fc, 0.0)
+    coeff_p = np.zeros(N)
+    np.add.at(coeff_p, rows, term)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rc = np.where(coeff_p > 0.0, g["vol"] / coeff_p, 0.0)
+    gc_ptr, gc_vid, gc_c = gc
+    grow = np.repeat(np.arange(N), np.diff(gc_ptr))
+    vol = g["vol"]
+    pg = []
+    for c in range(3):
+        acc = np.zeros(N)
+        np.add.at(acc, grow, gc_c[c] * pres[gc_vid] / vol[grow])
+        pg.append(acc)
+    allv = np.arange(N)
+    fcc = _fce_centroid(g, fce, fceg, allv)
+    mom = np.stack([rc * (pg[c] - fcc[c]) for c in range(3)])
+
+    def uface(c):
+        gx, gy, gz = velg[3 * c], velg[3 * c + 1], velg[3 * c + 2]
+        return 0.5 * (vel[c][rows] + vel[c][ngh]) + 0.5 * (sx * (gx[rows] + gx[ngh]) + sy * (gy[rows] + gy[ngh]) + sz * (gz[rows] + gz[ngh]))
+
+    u_f, v_f, w_f = uface(0), uface(1), uface(2)
+    fce_e = 0.5 * (nx * (fce[0][rows] + fce[0][ngh]) + ny * (fce[1][rows] + fce[1][ngh]) + nz * (fce[2][rows] + fce[2][ngh]))
+    mom_e = 0.5 * (nx * (mom[0][rows] + mom[0][ngh]) + ny * (mom[1][rows] + mom[1][ngh]) + nz * (mom[2][rows] + mom[2][ngh]))
+    vel_rc = -0.5 * (rc[rows] + rc[ngh]) * ((pres[ngh] - pres[rows]) / d - fce_e) + mom_e
+    flux = den_f * (u_f * nx + v_f * ny + w_f * nz + vel_rc) * A
+    return rc, mom, flux
+
+
+def assemble_flow(g: dict, gc, fields: dict, bnd_vel: list, bnd_nid: list, pref=None):
+    """Triplets and b of SteadyFlow::assemble_matrix (src/steady/core/steady_flow.rs:244-303)
+    in push order, unknowns [u, v, w, p] (steady_flow.rs:94-104).
+
+    fields: vel (3,N), velg (9,N), den, deng, visc, viscg, fce (3,N), fceg (9,N), rc, flux
+    (from update_flow).  bnd_vel: list of (bnd_id, values (3, count)) in add_vel_bnd order,
+    each boundary getting OpVecBndDirichlet, OpSclBndDivergence, OpSclBndPressureStabilizer.
+    pref: (nid, p_ref) or None (OpSclDomPressureReference, applied last).
+    Operators: opvec_dom_adv.rs:47-116, opvec_dom_grad.rs:39-70, opvec_dom_diff.rs:39-94,
+    opvec_dom_dtrn.rs:39-113, opvec_dom_ddil.rs:41-112, opvec_dom_src.rs:37-73,
+    opscl_dom_div.rs:68-153, opscl_dom_pstab.rs:52-151, opvec_bnd_dir.rs:56-80,
+    opscl_bnd_div.rs:73-187, opscl_bnd_pstab.rs:49-153, opscl_dom_pref.rs:48-63.
+    Returns rows, cols, vals (push order), b (4N).
+    """
+    N, NP = g["N"], g["num_pts"]
+    rp, ngh = g["row_ptr"], g["ngh"]
+    rowsE = g["rows"].astype(np.int64)
+    area, dist, norm, skew, gw = g["area"], g["dist"], g["norm"], g["skew"], g["gw"]
+    vel, velg = fields["vel"], fields["velg"]
+    den, deng = fields["den"], fields["deng"]
+    visc, viscg = fields["visc"], fields["viscg"]
+    fce, fceg = fields["fce"], fields["fceg"]
+    rc, flux = fields["rc"], fields["flux"]
+    gc_ptr, gc_vid, gc_c = gc
+    b = np.zeros(4 * N)
+    R, C, V = [], [], []
+    P = 3 * N  # pressure block start
+    pts = np.arange(NP)
+    e_pts = np.nonzero(rowsE < NP)[0]
+    ev, en = rowsE[e_pts], ngh[e_pts].astype(np.int64)
+    sx, sy, sz = skew[:, e_pts]
+    nx, ny, nz = norm[:, e_pts]
+    A, d = area[e_pts], dist[e_pts]
+    visc_f = _visc_face(visc, viscg, ev, en, sx, sy, sz)
+
+    def push_vec_pairs(coeff):
+        # per face, per component: (c v, c n) +coeff, (c v, c v) -coeff
+        rr = np.stack([c * N + ev for c in range(3) for _ in range(2)], axis=1).ravel()
+        cc = np.stack([x for c in range(3) for x in (c * N + en, c * N + ev)], axis=1).ravel()
+        vv = np.stack([x for c in range(3) for x in (coeff, -coeff)], axis=1).ravel()
+        R.append(rr); C.append(cc); V.append(vv)
+
+    # OpVecDomAdvection
+    cond = visc_f * A / d
+    F = flux[e_pts]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        pe = np.where(cond != 0.0, F / cond, 0.0)
+    push_vec_pairs(cond * (_pow5(np.maximum(1.0 - 0.1 * np.abs(pe), 0.0)) - 1.0) + np.maximum(-F, 0.0))
+    # OpVecDomGradient
+    grow = np.repeat(np.arange(N), np.diff(gc_ptr))
+    gsel = np.nonzero(grow < NP)[0]
+    gv, gm = grow[gsel], gc_vid[gsel]
+    R.append(np.stack([c * N + gv for c in range(3)], axis=1).ravel())
+    C.append(np.stack([P + gm] * 3, axis=1).ravel())
+    V.append(np.stack([-gc_c[c][gsel] for c in range(3)], axis=1).ravel())
+    # OpVecDomDiffusion
+    push_vec_pairs(visc_f * A / d)
+    # OpVecDomTranspose (b): per vertex sums in face order
+    fg = [[0.5 * (velg[3 * j + c][ev] + velg[3 * j + c][en]) for c in range(3)] for j in range(3)]  # fg[j][c] = d u_j / d x_c
+    for c in range(3):
+        t = visc_f * (fg[0][c] * nx + fg[1][c] * ny + fg[2][c] * nz) * A
+        s = np.zeros(N)
+        np.add.at(s, ev, t)
+        b[c * N + pts] += -s[pts]
+    # OpVecDomDilatatory (b)
+    divv = velg[0] + velg[4] + velg[8]
+    lam_f = (-2.0 / 3.0) * visc_f
+    div_f = 0.5 * (divv[ev] + divv[en])
+    co = lam_f * div_f * A
+    for c, n_c in enumerate((nx, ny, nz)):
+        s = np.zeros(N)
+        np.add.at(s, ev, co * n_c)
+        b[c * N + pts] += -s[pts]
+    # OpVecDomSource (b)
+    fcc = _fce_centroid(g, fce, fceg, pts)
+    for c in range(3):
+        b[c * N + pts] += -fcc[c] * g["vol"][pts]
+
+    def continuity(v, R_, C_, V_, bsub):
+        """Divergence + pressure-stabiliser pushes of vertex v's row (pure Python, push
+        order).  bsub: also push the boundary sub-face terms of the divergence operator."""
+        row = P + v
+        den_v = den[v]
+        rc_v, vol_v = rc[v], g["vol"][v]
+        out_r, out_c, out_v = [], [], []
+        pst_r, pst_c, pst_v = [], [], []
+        bacc = []
+        fcv = [float(x[0]) for x in _fce_centroid(g, fce, fceg, np.array([v]))]
+        for e in range(rp[v], rp[v + 1]):
+            n = int(ngh[e])
+            face, dd = float(area[e]), float(dist[e])
+            enx, eny, enz = (float(x) for x in norm[:, e])
+            esx, esy, esz = (float(x) for x in skew[:, e])
+            den_f = float(_face_mean(den, deng, v, n, esx, esy, esz))
+            coeff = 0.5 * den_f * face
+            for c, n_c in enumerate((enx, eny, enz)):
+                out_r += [row, row]; out_c += [c * N + n, c * N + v]; out_v += [-coeff * n_c, -coeff * n_c]
+            for q in range(rp[v], rp[v + 1]):
+                m = int(ngh[q])
+                ws = float(gw[0, q]) * esx + float(gw[1, q]) * esy + float(gw[2, q]) * esz
+                for c, n_c in enumerate((enx, eny, enz)):
+                    val = -coeff * ws * n_c
+                    out_r += [row, row]; out_c += [c * N + m, c * N + v]; out_v += [val, -val]
+            for q in range(rp[n], rp[n + 1]):
+                m = int(ngh[q])
+                ws = float(gw[0, q]) * esx + float(gw[1, q]) * esy + float(gw[2, q]) * esz
+                for c, n_c in enumerate((enx, eny, enz)):
+                    val = -coeff * ws * n_c
+                    out_r += [row, row]; out_c += [c * N + m, c * N + n]; out_v += [val, -val]
+        if bsub:
+            for s in range(g["bsf_ptr"][v], g["bsf_ptr"][v + 1]):
+                a_ = float(g["bsf_area"][s])
+                snx, sny, snz = (float(x) for x in g["bsf_norm"][:, s])
+                wv, wj, wk = (float(x) for x in g["bsf_wgt"][:, s])
+                j, k = int(g["bsf_vj"][s]), int(g["bsf_vk"][s])
+                for m, cf in ((v, wv * den_v * a_), (j, wj * den[j] * a_), (k, wk * den[k] * a_)):
+                    for c, n_c in enumerate((snx, sny, snz)):
+                        out_r.append(row); out_c.append(c * N + m); out_v.append(-cf * n_c)
+        for e in range(rp[v], rp[v + 1]):
+            n = int(ngh[e])
+            face, dd = float(area[e]), float(dist[e])
+            enx, eny, enz = (float(x) for x in norm[:, e])
+            esx, esy, esz = (float(x) for x in skew[:, e])
+            den_f = float(_face_mean(den, deng, v, n, esx, esy, esz))
+            rc_n, vol_n = rc[n], g["vol"][n]
+            coeff = den_f * 0.5 * (rc_v + rc_n) * face
+            pst_r += [row, row]; pst_c += [P + n, P + v]; pst_v += [coeff / dd, -coeff / dd]
+            coeff_v = den_f * 0.5 * rc_v / vol_v * face
+            for q in range(gc_ptr[v], gc_ptr[v + 1]):
+                pst_r.append(row); pst_c.append(P + int(gc_vid[q]))
+                pst_v.append(-coeff_v * (gc_c[0, q] * enx + gc_c[1, q] * eny + gc_c[2, q] * enz))
+            coeff_n = den_f * 0.5 * rc_n / vol_n * face
+            for q in range(gc_ptr[n], gc_ptr[n + 1]):
+                pst_r.append(row); pst_c.append(P + int(gc_vid[q]))
+                pst_v.append(-coeff_n * (gc_c[0, q] * enx + gc_c[1, q] * eny + gc_c[2, q] * enz))
+            fcn = [float(x[0]) for x in _fce_centroid(g, fce, fceg, np.array([n]))]
+            frc = [0.5 * (rc_v + rc_n) * 0.5 * (fce[c][v] + fce[c][n]) - 0.5 * (rc_v * fcv[c] + rc_n * fcn[c]) for c in range(3)]
+            bacc.append(den_f * (frc[0] * enx + frc[1] * eny + frc[2] * enz) * face)
+        return (out_r, out_c, out_v), (pst_r, pst_c, pst_v), bacc
+
+    # OpSclDomDivergence, then OpSclDomPressureStabilizer, over the points
+    div_t, pst_t, bq = [], [], []
+    for v in range(NP):
+        dt_, pt_, ba = continuity(v, R, C, V, False)
+        div_t.append(dt_); pst_t.append(pt_); bq.append((v, ba))
+    for r_, c_, v_ in div_t:
+        R.append(np.asarray(r_, np.int64)); C.append(np.asarray(c_, np.int64)); V.append(np.asarray(v_, np.float64))
+    for (r_, c_, v_), (v, ba) in zip(pst_t, bq):
+        R.append(np.asarray(r_, np.int64)); C.append(np.asarray(c_, np.int64)); V.append(np.asarray(v_, np.float64))
+        for t in ba:
+            b[P + v] += t
+    # velocity boundaries: OpVecBndDirichlet, OpSclBndDivergence, OpSclBndPressureStabilizer
+    for bid, val in bnd_vel:
+        nodes = NP + np.asarray(bnd_nid[bid], np.int64)
+        val = np.asarray(val, np.float64).reshape(3, -1)
+        for i, v in enumerate(nodes):
+            for c in range(3):
+                R.append(np.array([c * N + v])); C.append(np.array([c * N + v])); V.append(np.array([1.0]))
+            for c in range(3):
+                b[c * N + v] += val[c, i]
+        div_t, pst_t, bq = [], [], []
+        for v in nodes:
+            dt_, pt_, ba = continuity(int(v), R, C, V, True)
+            div_t.append(dt_); pst_t.append(pt_); bq.append((int(v), ba))
+        for r_, c_, v_ in div_t:
+            R.append(np.asarray(r_, np.int64)); C.append(np.asarray(c_, np.int64)); V.append(np.asarray(v_, np.float64))
+        for (r_, c_, v_), (v, ba) in zip(pst_t, bq):
+            R.append(np.asarray(r_, np.int64)); C.append(np.asarray(c_, np.int64)); V.append(np.asarray(v_, np.float64))
+            for t in ba:
+                b[P + v] += t
+    rows = np.concatenate(R)
+    cols = np.concatenate(C)
+    vals = np.concatenate(V)
+    if pref is not None:
+        xid = P + NP + int(pref[0])
+        keep = rows != xid
+        rows, cols, vals = np.append(rows[keep], xid), np.append(cols[keep], xid), np.append(vals[keep], 1.0)
+        b[xid] = float(pref[1])
+    return rows, cols, vals, b
+
+
+def flow_fields(g, gc, vel, pres, den, visc, fce, vel_const=False, den_const=True, visc_const=True, fce_const=True):
+    """Gradients (update_grad of every field, zero for constants) and the flow data of
+    update_flow, as assemble_flow reads them."""
+    N = g["N"]
+    velg = np.zeros((9, N)) if vel_const else update_grad_vec(g, vel)
+    deng = np.zeros((3, N)) if den_const else update_grad(g, den)
+    viscg = np.zeros((3, N)) if visc_const else update_grad(g, visc)
+    fceg = np.zeros((9, N)) if fce_const else update_grad_vec(g, fce)
+    rc, mom, flux = update_flow(g, gc, vel, velg, pres, den, deng, visc, viscg, fce, fceg)
+    return dict(vel=vel, velg=velg, pres=pres, den=den, deng=deng, visc=visc, viscg=viscg, fce=fce, fceg=fceg, rc=rc,
+                mom=mom, flux=flux)
+
+
+def scaled_residual_flow(N, x_old, x_new, den):
+    """Picard residual of the flow unknowns [vel (vector field), pres (pressure field)]
+    (src/base/scaling.rs:88-135): max over the two fields of rms(x_new - x_old) / scale, the
+    rms over N entries (per component for the vector), pressure changes below the round-off
+    dead band 1e-11 max|p| not counted, scales max|vel| and max(range p, rho_max U_max^2)."""
+    if not np.all(np.isfinite(x_new)):
+        return float("nan")
+    vo, vn = x_old[:3 * N].reshape(3, N), x_new[:3 * N].reshape(3, N)
+    po, pn = x_old[3 * N:], x_new[3 * N:]
+    vmax = max(np.sqrt((vo * vo).sum(axis=0)).max(), np.sqrt((vn * vn).sum(axis=0)).max())
+    floor = 1e-30
+    r_v = np.sqrt(((np.abs(vn - vo)) ** 2).sum() / max(N, 1)) / max(vmax, floor)
+    dead = 1e-11 * max(np.abs(po).max(), np.abs(pn).max())
+    r_p = np.sqrt((np.maximum(np.abs(pn - po) - dead, 0.0) ** 2).sum() / max(N, 1))
+    rng = max(po.max(), pn.max()) - min(po.min(), pn.min())
+    rho = np.abs(den).max()
+    r_p /= max(rng, rho * vmax * vmax, floor)
+    return float(max(r_v, r_p))
+
+
+def picard_flow(g, gc, vel0, pres0, den, visc, fce, bnd_vel, bnd_nid, pref, max_iter, tol, damp):
+    """SteadyBase::solve (src/steady/steady_base.rs:16-135) for SteadyFlow with constant
+    density, viscosity and body force: Dirichlet velocities into the initial state
+    (opvec_bnd_dir.rs:34-53), then Picard iterations solving each linear system with
+    SuperLU, damping, re-assembly and the scaled residual.  Returns vel, pres, iterations,
+    residuals."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    N, NP = g["N"], g["num_pts"]
+    vel = np.array(vel0, np.float64).reshape(3, N).copy()
+    pres = np.array(pres0, np.float64).copy()
+    for bid, val in bnd_vel:
+        nodes = NP + np.asarray(bnd_nid[bid], np.int64)
+        vel[:, nodes] = np.asarray(val).reshape(3, -1)
+
+    def system(vel, pres):
+        f = flow_fields(g, gc, vel, pres, den, visc, fce)
+        r, c, v, b = assemble_flow(g, gc, f, bnd_vel, bnd_nid, pref)
+        rp, col, val = coo_to_csr(r, c, v, 4 * N)
+        return sp.csr_matrix((val, col, rp), shape=(4 * N, 4 * N)).tocsc(), b
+
+    x_iter = np.concatenate([vel.ravel(), pres])
+    A, b = system(vel, pres)
+    res_hist = []
+    for it in range(max_iter):
+        x_u = spla.spsolve(A, b)
+        x_d = (1.0 - damp) * x_iter + damp * x_u
+        vel, pres = x_d[:3 * N].reshape(3, N).copy(), x_d[3 * N:].copy()
+        A, b = system(vel, pres)
+        res = scaled_residual_flow(N, x_iter, x_d, den)
+        res_hist.append(res)
+        if not np.isfinite(res):
+            raise FloatingPointError("SteadyNonFiniteResidual")
+        x_iter = x_d
+        if res < tol:
+            return vel, pres, it + 1, res_hist
+    raise RuntimeError("SteadyDidNotConverge")
+
+
+def bjacobi_apply(rp, col, val, r, N):
+    """z = B^-1 r for the block diagonal B of the unknowns of each vertex (v, N+v, ...)."""
+    n = rp.size - 1
+    nb = n // N
+    z = np.empty(n)
+    for v in range(N):
+        idx = np.arange(nb) * N + v
+        B = np.zeros((nb, nb))
+        for i, row in enumerate(idx):
+            cs = col[rp[row]:rp[row + 1]]
+            vs = val[rp[row]:rp[row + 1]]
+            for j, cc in enumerate(idx):
+                k = np.nonzero(cs == cc)[0]
+                if k.size:
+                    B[i, j] = vs[k[0]]
+        z[idx] = np.linalg.solve(B, r[idx])
+    return z

@@ -178,3 +178,97 @@ def face_mean_weights(mesh, b=X1):
     nodes = mesh.num_pts + mesh.bnd_nodes(b).astype(np.int64)
     w[nodes] = 1.0 / nodes.size
     return w
+
+
+class FlowProblem:
+    """SteadyFlow composition (steady_flow.rs:133-240) of one domain with a velocity condition
+    on every boundary and a reference pressure, for the device context and the oracle.
+    Field arrays are host copies: vel (3,N), pres, den, visc (N), fce (3,N); `const` marks the
+    fields uploaded as constants (zero gradient)."""
+
+    def __init__(self, mesh: vfvm.Mesh, vel, pres, den, visc, fce, bnd_vel, pref=(0, 0.0), const=("den", "visc", "fce")):
+        self.mesh = mesh
+        self.N = mesh.num_tot
+        self.vel = np.asarray(vel, np.float64).reshape(3, self.N).copy()
+        self.pres = np.broadcast_to(np.asarray(pres, np.float64), (self.N,)).copy()
+        self.den = np.broadcast_to(np.asarray(den, np.float64), (self.N,)).copy()
+        self.visc = np.broadcast_to(np.asarray(visc, np.float64), (self.N,)).copy()
+        self.fce = np.asarray(fce, np.float64).reshape(3, self.N).copy()
+        self.bnd_vel = [(b, np.asarray(v, np.float64).reshape(3, -1)) for b, v in bnd_vel]
+        self.pref = pref
+        self.const = set(const)
+
+    def ops(self, ids):
+        vel, pres, den, visc, fce, flw, vb = ids
+        o = [(vfvm.OP_VEC_DOM_ADVECTION, 0, vel, den, visc, flw), (vfvm.OP_VEC_DOM_GRADIENT, 0, vel, pres, -1),
+             (vfvm.OP_VEC_DOM_DIFFUSION, 0, vel, visc, -1), (vfvm.OP_VEC_DOM_TRANSPOSE, 0, vel, visc, -1),
+             (vfvm.OP_VEC_DOM_DILATATORY, 0, vel, visc, -1), (vfvm.OP_VEC_DOM_SOURCE, 0, vel, fce, -1),
+             (vfvm.OP_SCL_DOM_DIVERGENCE, 0, pres, vel, den), (vfvm.OP_SCL_DOM_PRESSURE_STABILIZER, 0, pres, den, fce, flw)]
+        for (b, _), vid in zip(self.bnd_vel, vb):
+            o += [(vfvm.OP_VEC_BND_DIRICHLET, b, vel, vid, -1), (vfvm.OP_SCL_BND_DIVERGENCE, b, pres, vel, den),
+                  (vfvm.OP_SCL_BND_PRESSURE_STABILIZER, b, pres, den, fce, flw)]
+        if self.pref is not None:
+            o.append((vfvm.OP_SCL_DOM_PRESSURE_REFERENCE, 0, pres, int(self.pref[0]), -1, -1, float(self.pref[1])))
+        return o
+
+    def to_device(self, ctx: vfvm.Context):
+        if self.mesh.num_grad_coeff == 0:
+            self.mesh.build_grad_coeff()
+        ctx.upload(self.mesh)
+        kind = lambda name: vfvm.FIELD_CONSTANT if name in self.const else vfvm.FIELD_FUNCTION
+        vel = ctx.vecdom_add(vfvm.FIELD_UNKNOWN, values=self.vel)
+        pres = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0, self.pres)
+        den = ctx.scldom_add(kind("den"), 0.0, self.den)
+        visc = ctx.scldom_add(kind("visc"), 0.0, self.visc)
+        fce = ctx.vecdom_add(kind("fce"), values=self.fce)
+        flw = ctx.sclflw_add(vel, pres, den, visc, fce)
+        vb = [ctx.vecbnd_add(b, values=v) for b, v in self.bnd_vel]
+        self.ids = (vel, pres, den, visc, fce, flw, vb)
+        return self.ops(self.ids)
+
+    def oracle(self, g=None, gc=None):
+        """Oracle dual, grad_coeff, fields and the assembled CSR system at the current values."""
+        if g is None:
+            g = O.dual(self.mesh.get("pos"), self.mesh.get("tets"), self.mesh.num_pts)
+        if gc is None:
+            gc = O.grad_coeff(g)
+        f = O.flow_fields(g, gc, self.vel, self.pres, self.den, self.visc, self.fce, vel_const=False,
+                          den_const="den" in self.const, visc_const="visc" in self.const, fce_const="fce" in self.const)
+        bnd_nid = [self.mesh.bnd_nodes(b) for b in range(self.mesh.num_bnd)]
+        r, c, v, b = O.assemble_flow(g, gc, f, self.bnd_vel, bnd_nid, self.pref)
+        rp, col, val = O.coo_to_csr(r, c, v, 4 * self.N)
+        return g, gc, f, (rp, col, val, b)
+
+
+def lid_cavity(mesh, rho=1.0, mu=0.01, lid=1.0, pref=(0, 0.0)):
+    """Lid-driven cavity (flow_lid.rs): no-slip walls, lid velocity (lid, 0, 0) on z = 1,
+    zero body force, reference pressure at node pref[0]; zero initial fields."""
+    N = mesh.num_tot
+    bnd = []
+    for b in range(6):
+        cnt = len(mesh.bnd_nodes(b))
+        v = np.zeros((3, cnt))
+        if b == Z1:
+            v[0] = lid
+        bnd.append((b, v))
+    return FlowProblem(mesh, np.zeros((3, N)), 0.0, rho, mu, np.zeros((3, N)), bnd, pref)
+
+
+def flow_general(mesh, seed=0):
+    """Flow data with every term active: smooth non-zero velocity and pressure, variable
+    density, viscosity and body force (function fields), non-zero wall velocities."""
+    rng = np.random.default_rng(seed)
+    pos = mesh.get("pos")
+    x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]
+    N = mesh.num_tot
+    vel = np.stack([np.sin(np.pi * x) * np.cos(np.pi * y) + 0.3 * z, -np.cos(np.pi * x) * np.sin(np.pi * y) + 0.2 * x * z,
+                    0.5 * x * y - 0.1 * z * z])
+    pres = np.cos(2.0 * x) + y * z - 0.5 * x * x
+    den = 1.0 + 0.3 * x + 0.2 * y * z
+    visc = 0.02 * (1.0 + 0.5 * np.sin(3.0 * y) + 0.2 * x)
+    fce = np.stack([0.1 + 0.2 * y, -0.3 * x * z, -1.0 + 0.4 * x * y])
+    bnd = []
+    for b in range(6):
+        nodes = mesh.num_pts + mesh.bnd_nodes(b).astype(np.int64)
+        bnd.append((b, vel[:, nodes] + 0.05 * rng.standard_normal((3, nodes.size))))
+    return FlowProblem(mesh, vel, pres, den, visc, fce, bnd, pref=(3, 0.25), const=())

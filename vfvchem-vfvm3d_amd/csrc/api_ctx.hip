@@ -238,8 +238,8 @@ extern "C" int vfvm_mesh_upload(vfvm_ctx* ctx, const vfvm_mesh* mesh) {
     if (!mp || !mp->has_dual) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_upload: mesh without dual");
     const Mesh& m = *mp;
     // reset everything that depends on the mesh
-    c.scl.clear(); c.vec.clear(); c.sbnd.clear(); c.flw.clear();
-    c.has_pattern = false; c.has_matrix = false; c.ilu.ready = false;
+    c.scl.clear(); c.vec.clear(); c.sbnd.clear(); c.flw.clear(); c.vbndv.clear();
+    c.has_pattern = false; c.has_matrix = false; c.ilu.ready = false; c.flow = false;
     c.has_transpose = false; c.tilu.ready = false; c.tape.valid = false;
     DevGeom& g = c.g;
     g.rev.reset();
@@ -264,6 +264,18 @@ extern "C" int vfvm_mesh_upload(vfvm_ctx* ctx, const vfvm_mesh* mesh) {
     VFVM_HIP_TRY(g.bsf_vk.upload(m.bsf_vk.data(), m.bsf_vk.size(), s));
     VFVM_HIP_TRY(g.bsf_area.upload(m.bsf_area.data(), m.bsf_area.size(), s));
     VFVM_HIP_TRY(g.bsf_wgt.upload(m.bsf_wgt.data(), m.bsf_wgt.size(), s));
+    VFVM_HIP_TRY(g.bsf_norm.upload(m.bsf_norm.data(), m.bsf_norm.size(), s));
+    g.G = m.has_grad_coeff ? (int64_t)m.gc_vid.size() : 0;
+    if (m.has_grad_coeff) {
+        VFVM_HIP_TRY(g.gc_ptr.upload(m.gc_ptr.data(), m.gc_ptr.size(), s));
+        VFVM_HIP_TRY(g.gc_vid.upload(m.gc_vid.data(), m.gc_vid.size(), s));
+        VFVM_HIP_TRY(g.gc_c.upload(m.gc_c.data(), m.gc_c.size(), s));
+        g.h_gc_ptr = m.gc_ptr;
+        g.h_gc_vid = m.gc_vid;
+    } else {
+        g.gc_ptr.reset(); g.gc_vid.reset(); g.gc_c.reset();
+        g.h_gc_ptr.clear(); g.h_gc_vid.clear();
+    }
     std::vector<int32_t> vbnd(g.N, -1), bidx(g.N, -1);
     for (int32_t b = 0; b < g.num_bnd; ++b)
         for (int64_t k = m.bnd_ptr[b]; k < m.bnd_ptr[b + 1]; ++k) {
@@ -330,8 +342,8 @@ extern "C" int vfvm_scldom_update_prev(vfvm_ctx* ctx, int32_t id) {
 
 extern "C" int vfvm_vecdom_add(vfvm_ctx* ctx, int32_t kind, const double* cnst3, const double* values, int32_t* id) {
     CTX_GUARD(ctx);
-    if (!c.has_mesh || !id || kind < 0 || kind > 1 || (!values && !cnst3))
-        return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_vecdom_add: invalid argument (vector unknowns are outside the built path)");
+    if (!c.has_mesh || !id || kind < 0 || kind > 2 || (!values && !cnst3))
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_vecdom_add: invalid argument");
     const int64_t N = c.g.N;
     c.vec.emplace_back();
     VecDom& f = c.vec.back();
@@ -386,8 +398,12 @@ extern "C" int vfvm_sclflw_add(vfvm_ctx* ctx, int32_t vel_id, int32_t pres_id, i
     if (!id || vel_id < 0 || vel_id >= (int32_t)c.vec.size() || !check_scl(c, pres_id) || !check_scl(c, den_id) ||
         !check_scl(c, visc_id) || fce_id < 0 || fce_id >= (int32_t)c.vec.size())
         return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_sclflw_add: field id out of range");
-    if (c.scl[pres_id].kind != VFVM_FIELD_CONSTANT)
-        return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "vfvm_sclflw_add: only a prescribed flow (uniform constant pressure) is on the built path");
+    // a constant pressure: prescribed flow (face flux from the velocity alone); an unknown
+    // pressure: a flow solved by the flow operators (Rhie-Chow face flux)
+    if (c.scl[pres_id].kind == VFVM_FIELD_FUNCTION)
+        return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "vfvm_sclflw_add: the pressure must be a constant (prescribed flow) or an unknown (solved flow)");
+    if (c.scl[pres_id].kind == VFVM_FIELD_UNKNOWN && c.vec[vel_id].kind != VFVM_FIELD_UNKNOWN)
+        return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "vfvm_sclflw_add: a solved flow needs an unknown velocity");
     c.flw.emplace_back();
     SclFlw& f = c.flw.back();
     f.vel = vel_id; f.pres = pres_id; f.den = den_id; f.visc = visc_id; f.fce = fce_id;
@@ -396,9 +412,68 @@ extern "C" int vfvm_sclflw_add(vfvm_ctx* ctx, int32_t vel_id, int32_t pres_id, i
     return clear_error();
 }
 
+extern "C" int vfvm_vecdom_get(vfvm_ctx* ctx, int32_t id, double* values) {
+    CTX_GUARD(ctx);
+    if (id < 0 || id >= (int32_t)c.vec.size() || !values) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_vecdom_get: invalid argument");
+    VFVM_HIP_TRY(hipMemcpyAsync(values, c.vec[id].value.p, 3 * c.g.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (int st = sync_err(c)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_update_flow(vfvm_ctx* ctx, int32_t flw_id) {
+    CTX_GUARD(ctx);
+    if (int st = flow_update(c, flw_id)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_sclflw_get(vfvm_ctx* ctx, int32_t flw_id, int32_t what, double* out) {
+    CTX_GUARD(ctx);
+    if (flw_id < 0 || flw_id >= (int32_t)c.flw.size() || !out || what < 0 || what > 2)
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_sclflw_get: invalid argument");
+    const SclFlw& f = c.flw[flw_id];
+    const DevBuf<double>& src = what == 0 ? f.flux : (what == 1 ? f.rc : f.mom);
+    const size_t cnt = what == 0 ? (size_t)c.g.E : (what == 1 ? (size_t)c.g.N : 3 * (size_t)c.g.N);
+    if (src.n < cnt || !src.p) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_sclflw_get: the flow data has not been computed");
+    VFVM_HIP_TRY(hipMemcpyAsync(out, src.p, cnt * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (int st = sync_err(c)) return st;
+    return clear_error();
+}
+
+extern "C" int vfvm_vecbnd_add(vfvm_ctx* ctx, int32_t bnd_id, const double* cnst3, const double* values, int32_t* id) {
+    CTX_GUARD(ctx);
+    if (!c.has_mesh || !id || bnd_id < 0 || bnd_id >= c.g.num_bnd || (!values && !cnst3))
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_vecbnd_add: invalid argument");
+    const int64_t cnt = c.g.bnd_ptr[bnd_id + 1] - c.g.bnd_ptr[bnd_id];
+    c.vbndv.emplace_back();
+    VecBnd& f = c.vbndv.back();
+    f.bnd = bnd_id;
+    const int64_t m = std::max<int64_t>(cnt, 1);
+    VFVM_HIP_TRY(f.value.alloc(3 * m));
+    if (values) VFVM_HIP_TRY(hipMemcpyAsync(f.value.p, values, 3 * cnt * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    else
+        for (int k = 0; k < 3; ++k) hipLaunchKernelGGL(k_fill, dim3(grid_for(m)), dim3(BLK), 0, c.stream, m, cnst3[k], f.value.p + k * m);
+    if (int st = sync_err(c)) return st;
+    *id = (int32_t)c.vbndv.size() - 1;
+    return clear_error();
+}
+
 // ------------------------------------------------------------------ pattern (LinearBase::update_sparse)
 extern "C" int vfvm_pattern_build(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, int64_t* nnz) {
     CTX_GUARD(ctx);
+    c.has_pattern = false;
+    c.has_matrix = false;
+    c.ilu.ready = false;
+    c.has_transpose = false;
+    c.tilu.ready = false;
+    c.tape.valid = false;
+    c.flow = false;
+    if (is_flow_list(ops, nops)) {
+        // coupled velocity-pressure system (SteadyFlow)
+        if (int st = flow_pattern_build(c, ops, nops)) return st;
+        c.has_pattern = true;
+        if (nnz) *nnz = c.nnz;
+        return clear_error();
+    }
     Compiled cp;
     if (int st = compile_ops(c, ops, nops, cp)) return st;
     const DevGeom& g = c.g;
@@ -467,6 +542,7 @@ extern "C" int vfvm_pattern_get(vfvm_ctx* ctx, int32_t* row_ptr, int32_t* col) {
 // ------------------------------------------------------------------ assembly
 int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
     if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "assemble: no pattern built (call vfvm_pattern_build)");
+    if (c.flow || is_flow_list(ops, nops)) return flow_assemble(c, ops, nops);
     Compiled cp;
     if (int st = compile_ops(c, ops, nops, cp)) return st;
     if (cp.unk != c.unk) return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: operators act on another unknown than the pattern");
@@ -606,6 +682,7 @@ extern "C" int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, 
 
 // ------------------------------------------------------------------ drivers
 int vfvm::initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
+    if (is_flow_list(ops, nops)) return flow_initial_dirichlet(c, ops, nops);
     // OpSclBndDirichlet::apply_initial for every Dirichlet operator (steady_base.rs:53-54)
     for (int32_t i = 0; i < nops; ++i) {
         if (ops[i].kind != VFVM_OP_SCL_BND_DIRICHLET) continue;
@@ -700,7 +777,8 @@ extern "C" int vfvm_steady_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops
     std::memset(&ds, 0, sizeof(ds));
     if (int st = vfvm_pattern_build(ctx, ops, nops, nullptr)) return st;
     if (int st = initial_dirichlet(c, ops, nops)) return st;
-    int r = picard_loop(c, ops, nops, opts, 0.0, max_iter, tol, damp, ds, 0, false);
+    int r = c.flow ? flow_picard(c, ops, nops, *opts, max_iter, tol, damp, ds)
+                   : picard_loop(c, ops, nops, opts, 0.0, max_iter, tol, damp, ds, 0, false);
     if (stats) *stats = ds;
     if (r != VFVM_OK) return r;
     return clear_error();
@@ -717,6 +795,7 @@ extern "C" int vfvm_transient_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t n
     if (std::isnan(tol) || tol <= 0.0) return set_error(VFVM_ERR_SOLVER_NON_POSITIVE_TOLERANCE, "tol must be positive");
     if (std::isnan(damp) || damp <= 0.0 || damp > 1.0) return set_error(VFVM_ERR_SOLVER_INVALID_DAMPING, "damp must be in (0, 1]");
     if (!opts) return set_error(VFVM_ERR_INVALID_ARGUMENT, "opts is NULL");
+    if (is_flow_list(ops, nops)) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "transient flow (TransientFlow) is outside the built path");
     vfvm_driver_stats ds;
     std::memset(&ds, 0, sizeof(ds));
     if (int st = vfvm_pattern_build(ctx, ops, nops, nullptr)) return st;

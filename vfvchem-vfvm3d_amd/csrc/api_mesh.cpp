@@ -81,6 +81,15 @@ extern "C" int vfvm_mesh_info_get(const vfvm_mesh* mesh, vfvm_mesh_info* info) {
     int32_t md = 0;
     for (int64_t v = 0; v + 1 < (int64_t)m.row_ptr.size(); ++v) md = std::max(md, m.row_ptr[v + 1] - m.row_ptr[v]);
     info->max_degree = md;
+    info->num_grad_coeff = m.has_grad_coeff ? (int64_t)m.gc_vid.size() : 0;
+    return clear_error();
+}
+
+extern "C" int vfvm_mesh_build_grad_coeff(vfvm_mesh* mesh) {
+    if (!mesh) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_build_grad_coeff: NULL mesh");
+    std::string err;
+    int st = build_grad_coeff(mesh->m, err);
+    if (st != VFVM_OK) return set_error(st, err);
     return clear_error();
 }
 
@@ -113,6 +122,9 @@ extern "C" int vfvm_mesh_get(const vfvm_mesh* mesh, int32_t id, void* dst) {
         case VFVM_MESH_BSF_AREA: return copy_out(m.bsf_area, dst);
         case VFVM_MESH_BSF_NORM: return copy_out(m.bsf_norm, dst);
         case VFVM_MESH_BSF_WGT: return copy_out(m.bsf_wgt, dst);
+        case VFVM_MESH_GC_PTR: return copy_out(m.gc_ptr, dst);
+        case VFVM_MESH_GC_VID: return copy_out(m.gc_vid, dst);
+        case VFVM_MESH_GC_C: return copy_out(m.gc_c, dst);
         default: return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_get: unknown array id");
     }
 }

@@ -54,12 +54,17 @@ struct DevGeom {
     DevBuf<double> dist, area, norm, skew, gw; // norm/skew/gw: 3E SoA
     DevBuf<double> vol, cskew;                 // cskew: 3N SoA
     DevBuf<int32_t> bsf_ptr, bsf_vj, bsf_vk;
-    DevBuf<double> bsf_area, bsf_wgt;          // bsf_wgt: 3S SoA
+    DevBuf<double> bsf_area, bsf_wgt, bsf_norm; // bsf_wgt, bsf_norm: 3S SoA
     DevBuf<int32_t> vbnd, bidx;                // vertex -> boundary (-1: point), index in boundary
     DevBuf<int32_t> rev;                       // rev[e] = directed edge (n, v) of e = (v, n); built on demand
     std::vector<int64_t> bnd_ptr;              // host copy
     std::vector<int32_t> bnd_nid;
     std::vector<int32_t> h_row_ptr, h_ngh;     // host copies for the pattern build
+    // pressure-gradient coefficients (flow operators), when the mesh has them
+    int64_t G = 0;
+    DevBuf<int32_t> gc_ptr, gc_vid;
+    DevBuf<double> gc_c;                       // 3G SoA
+    std::vector<int32_t> h_gc_ptr, h_gc_vid;
 };
 
 struct SclDom {
@@ -80,6 +85,12 @@ struct SclBnd {
 struct SclFlw {
     int32_t vel = -1, pres = -1, den = -1, visc = -1, fce = -1;
     DevBuf<double> flux;  // E
+    DevBuf<double> rc;    // N: V / a_P (Rhie-Chow coefficient)
+    DevBuf<double> mom;   // 3N SoA: rc (grad p - fce_c)
+};
+struct VecBnd {
+    int32_t bnd = 0;
+    DevBuf<double> value;  // 3 x (nodes of the boundary) SoA
 };
 
 // One operator as the assembly kernel reads it.
@@ -140,9 +151,26 @@ struct AdjTape {
 
 // Captured launches of one restart cycle (one executable graph per chunk of steps).
 struct CycleGraphs {
-    std::array<uint64_t, 20> key{};
+    std::array<uint64_t, 22> key{};
     std::vector<hipGraphExec_t> chunks;
     uint64_t last_use = 0;
+};
+
+// Coupled velocity-pressure system of a flow operator list (flow.hip).
+struct FlowSys {
+    int32_t vel = -1, pres = -1, den = -1, visc = -1, fce = -1, flw = -1;
+    std::vector<int32_t> vbnd_val;  // per boundary: vecbnd id of its velocity condition
+    int64_t pref_row = -1;          // row of the reference pressure (-1: none)
+    double pref_val = 0.0;
+    int32_t gmax = 0, rmax = 0;     // largest grad_coeff list and 3-ring of a continuity row
+    int32_t lpr = 16;               // lanes per momentum row
+};
+
+// Block-Jacobi inverse blocks (b x b per vertex, entry (i, j) of vertex v at (i b + j) N + v).
+struct BlockJacobi {
+    int32_t nb = 0;
+    DevBuf<double> inv64;
+    DevBuf<float> inv32;
 };
 
 struct KTimer {
@@ -160,6 +188,7 @@ struct Ctx {
     std::vector<VecDom> vec;
     std::vector<SclBnd> sbnd;
     std::vector<SclFlw> flw;
+    std::vector<VecBnd> vbndv;
 
     // pattern
     bool has_pattern = false;
@@ -173,6 +202,11 @@ struct Ctx {
     bool has_matrix = false;
 
     IluData ilu;  // ILU(0) ordering of the pattern above
+    BlockJacobi bjac;
+
+    // coupled flow system (when the pattern was built from flow operators)
+    bool flow = false;
+    FlowSys fs;
 
     // transposed system A^T (adjoint solves): CSR of the columns of A, tsrc[k] = slot of
     // entry k in vals; values gathered on demand
@@ -227,6 +261,15 @@ int solver_setup(Ctx& c, const SysView& s, const vfvm_solve_opts& o);
 int precond_apply(Ctx& c, const vfvm_solve_opts& o, const double* r_host, double* z_host);
 int ilu_order(Ctx& c, int32_t* perm, int32_t* color_ptr, int32_t* ncolors);
 int residual_norm(Ctx& c, const SysView& s, const double* x_dev, double* rel, double* bnorm);
+
+// flow operators (flow.hip)
+bool is_flow_list(const vfvm_op* ops, int32_t nops);
+int flow_pattern_build(Ctx& c, const vfvm_op* ops, int32_t nops);
+int flow_assemble(Ctx& c, const vfvm_op* ops, int32_t nops);
+int flow_update(Ctx& c, int32_t flw);
+int flow_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops);
+int flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts& o, int32_t max_iter, double tol,
+                double damp, vfvm_driver_stats& ds);
 
 // drivers and adjoint set-up shared across translation units (api_ctx.hip)
 int assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt);

@@ -512,6 +512,93 @@ __global__ void __launch_bounds__(BLK) k_xupdate_set(int64_t n, const double* __
     if (i < n) z[i] = r[i] * dinv[i];
 }
 
+// ---------------------------------------------------------------- block Jacobi
+// Inverse of the NB x NB block of vertex v (unknowns v, N + v, ..., (NB-1) N + v): the block is
+// gathered from the CSR rows by binary search (columns ascending), inverted by Gauss-Jordan
+// elimination with partial pivoting in registers (compile-time indices only), and stored as
+// inv[(i NB + j) N + v]; a singular or non-finite block becomes the identity.
+template <int NB>
+__global__ void __launch_bounds__(BLK) k_bjac_setup(int64_t N, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                    const double* __restrict__ val, double* __restrict__ inv,
+                                                    float* __restrict__ inv32) {
+    const int64_t v = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (v >= N) return;
+    double a[NB][2 * NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int64_t row = (int64_t)i * N + v;
+        const int32_t s = rp[row], e = rp[row + 1];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int32_t target = (int32_t)((int64_t)j * N + v);
+            int32_t lo = s, hi = e;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (col[mid] < target) lo = mid + 1;
+                else hi = mid;
+            }
+            a[i][j] = (lo < e && col[lo] == target) ? val[lo] : 0.0;
+            a[i][NB + j] = i == j ? 1.0 : 0.0;
+        }
+    }
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        // pivot row p >= k of largest magnitude, swapped into row k
+        int p = k;
+        double best = fabs(a[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < NB; ++i)
+            if (fabs(a[i][k]) > best) { best = fabs(a[i][k]); p = i; }
+#pragma unroll
+        for (int i = k + 1; i < NB; ++i) {
+            if (i == p) {
+#pragma unroll
+                for (int j = 0; j < 2 * NB; ++j) { const double t = a[k][j]; a[k][j] = a[i][j]; a[i][j] = t; }
+            }
+        }
+        if (!(best > 0.0) || !isfinite(best)) ok = false;
+        const double piv = 1.0 / a[k][k];
+#pragma unroll
+        for (int j = 0; j < 2 * NB; ++j) a[k][j] *= piv;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            if (i == k) continue;
+            const double f = a[i][k];
+#pragma unroll
+            for (int j = 0; j < 2 * NB; ++j) a[i][j] -= f * a[k][j];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            double x = ok ? a[i][NB + j] : (i == j ? 1.0 : 0.0);
+            if (!isfinite(x)) x = i == j ? 1.0 : 0.0;
+            inv[((int64_t)i * NB + j) * N + v] = x;
+            if (inv32) inv32[((int64_t)i * NB + j) * N + v] = (float)x;
+        }
+}
+// z = B^-1 (c r) per vertex block (c = *rscale or 1); z must not alias r
+template <class T, int NB>
+__global__ void __launch_bounds__(BLK) k_bjac_apply(int64_t N, const T* __restrict__ inv, const T* __restrict__ r,
+                                                    const double* __restrict__ rscale, T* __restrict__ z,
+                                                    const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0] == 0) return;
+    const int64_t v = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (v >= N) return;
+    T rr[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) rr[j] = rscale ? (T)(r[(int64_t)j * N + v] * (T)rscale[0]) : r[(int64_t)j * N + v];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        T acc = 0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc += inv[((int64_t)i * NB + j) * N + v] * rr[j];
+        z[(int64_t)i * N + v] = acc;
+    }
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 
 struct HipEvents {
@@ -833,10 +920,64 @@ int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double
     return VFVM_OK;
 }
 
-// The launches of Arnoldi step j of the cycle on A M^-1 (M = D through the scaled matrix for
-// Jacobi, M = LU for ILU(0)).
+// Explicitly applied preconditioner of a cycle: the ILU(0) factor or the block-Jacobi
+// inverse blocks (Jacobi enters through the column-scaled matrix instead).
 template <class T>
-int issue_step(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* aval, const T* lu, bool reorth, int j) {
+struct Pc {
+    const T* lu = nullptr;
+    const T* bj = nullptr;
+    int nb = 0;
+    int64_t nv = 0;  // vertices (block-Jacobi blocks)
+};
+
+template <class T>
+int bjac_apply(Ctx& c, const Pc<T>& pc, const T* r, const double* rscale, T* z, const int* ctrl) {
+    const double bytes = (double)sizeof(T) * pc.nv * (pc.nb * pc.nb + 2 * pc.nb);
+    const dim3 grid(grid_for(pc.nv));
+    switch (pc.nb) {
+        case 1: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 1>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
+        case 2: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 2>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
+        case 3: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 3>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
+        case 4: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 4>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
+        case 5: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 5>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
+        case 6: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 6>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
+        default: return set_error(VFVM_ERR_INVALID_ARGUMENT, "block Jacobi: block size must be 1..6");
+    }
+    VFVM_HIP_TRY(hipGetLastError());
+    return VFVM_OK;
+}
+
+// block-Jacobi set-up: blocks of nb = n / N unknowns per vertex
+int bjac_setup(Ctx& c, const SysView& sy, bool with32) {
+    const int64_t N = c.g.N;
+    if (N <= 0 || sy.n % N != 0 || sy.n / N < 1 || sy.n / N > 6)
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "block Jacobi: the system size must be b x (vertices) with b in 1..6");
+    const int nb = (int)(sy.n / N);
+    c.bjac.nb = nb;
+    VFVM_HIP_TRY(c.bjac.inv64.alloc((size_t)nb * nb * N));
+    float* i32 = nullptr;
+    if (with32) {
+        VFVM_HIP_TRY(c.bjac.inv32.alloc((size_t)nb * nb * N));
+        i32 = c.bjac.inv32.p;
+    }
+    const dim3 grid(grid_for(N));
+    switch (nb) {
+        case 1: hipLaunchKernelGGL(k_bjac_setup<1>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
+        case 2: hipLaunchKernelGGL(k_bjac_setup<2>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
+        case 3: hipLaunchKernelGGL(k_bjac_setup<3>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
+        case 4: hipLaunchKernelGGL(k_bjac_setup<4>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
+        case 5: hipLaunchKernelGGL(k_bjac_setup<5>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
+        default: hipLaunchKernelGGL(k_bjac_setup<6>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
+    }
+    VFVM_HIP_TRY(hipGetLastError());
+    return VFVM_OK;
+}
+
+// The launches of Arnoldi step j of the cycle on A M^-1 (M = D through the scaled matrix for
+// Jacobi, M = LU for ILU(0), M = block diagonal for block Jacobi).
+template <class T>
+int issue_step(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* aval, const Pc<T>& pc, bool reorth, int j) {
+    const T* lu = pc.lu;
     const int64_t n = w.n;
     const int lpr = pick_lpr(n, w.nnz);
     double* part = c.part.p;
@@ -850,6 +991,10 @@ int issue_step(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* 
     const double* yscale = sigj;
     if (lu) {
         if (int st = ilu_apply<T>(c, sy, lu, vj, sigj, zv, w.ctrl)) return st;
+        xin = zv;
+        yscale = nullptr;
+    } else if (pc.bj) {
+        if (int st = bjac_apply<T>(c, pc, vj, sigj, zv, w.ctrl)) return st;
         xin = zv;
         yscale = nullptr;
     }
@@ -880,10 +1025,12 @@ int issue_step(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* 
 constexpr int CHECK = 8;
 
 template <class T>
-CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* aval, const T* lu, bool reorth,
+CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* aval, const Pc<T>& pc, bool reorth,
                           int* status) {
+    const T* lu = pc.lu;
     const IluData* I = lu ? sy.ilu : nullptr;
-    std::array<uint64_t, 20> key = {(uint64_t)(uintptr_t)V, (uint64_t)(uintptr_t)wv, (uint64_t)(uintptr_t)zv,
+    std::array<uint64_t, 22> key = {(uint64_t)(uintptr_t)pc.bj, (uint64_t)pc.nb, (uint64_t)(uintptr_t)V,
+                                    (uint64_t)(uintptr_t)wv, (uint64_t)(uintptr_t)zv,
                                     (uint64_t)(uintptr_t)aval, (uint64_t)(uintptr_t)lu, (uint64_t)(uintptr_t)w.S,
                                     (uint64_t)(uintptr_t)w.ctrl, (uint64_t)(uintptr_t)c.part.p,
                                     (uint64_t)(uintptr_t)sy.row, (uint64_t)(uintptr_t)sy.col, (uint64_t)w.n,
@@ -907,7 +1054,7 @@ CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv
         if (e != hipSuccess) { *status = set_error(VFVM_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e)); break; }
         int st = VFVM_OK;
         for (int j = j0; j < std::min(w.m, j0 + CHECK) && st == VFVM_OK; ++j)
-            st = issue_step<T>(c, sy, w, V, wv, zv, aval, lu, reorth, j);
+            st = issue_step<T>(c, sy, w, V, wv, zv, aval, pc, reorth, j);
         e = hipStreamEndCapture(c.stream, &graph);
         if (st != VFVM_OK) { *status = st; if (graph) (void)hipGraphDestroy(graph); break; }
         if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -927,13 +1074,13 @@ CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv
 // cycle graphs, or the launches one by one when kernel timing is on (the per-launch events
 // need the eager path) or graphs are off (VFVM_GRAPHS=0).
 template <class T>
-int arnoldi_cycle(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* aval, const T* lu, bool reorth,
+int arnoldi_cycle(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* aval, const Pc<T>& pc, bool reorth,
                   int* steps_out) {
     int steps = 0;
     CycleGraphs* cg = nullptr;
     if (c.timing_mask == 0 && c.graphs_enabled) {
         int status = VFVM_OK;
-        cg = cycle_graphs<T>(c, sy, w, V, wv, zv, aval, lu, reorth, &status);
+        cg = cycle_graphs<T>(c, sy, w, V, wv, zv, aval, pc, reorth, &status);
         if (status != VFVM_OK) return status;
     }
     for (int j0 = 0; j0 < w.m; j0 += CHECK) {
@@ -942,7 +1089,7 @@ int arnoldi_cycle(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const 
         } else {
             for (int j = j0; j < std::min(w.m, j0 + CHECK); ++j) {
                 c.cur_step = j;
-                if (int st = issue_step<T>(c, sy, w, V, wv, zv, aval, lu, reorth, j)) return st;
+                if (int st = issue_step<T>(c, sy, w, V, wv, zv, aval, pc, reorth, j)) return st;
             }
             c.cur_step = -1;
         }
@@ -992,7 +1139,7 @@ int solver_setup(Ctx& c, const SysView& sy, const vfvm_solve_opts& o) {
 int precond_apply(Ctx& c, const vfvm_solve_opts& o, const double* r_host, double* z_host) {
     const SysView sy = primal_view(c);
     const int64_t n = sy.n;
-    if (o.precond < VFVM_PRECOND_NONE || o.precond > VFVM_PRECOND_ILU0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "unknown preconditioner");
+    if (o.precond < VFVM_PRECOND_NONE || o.precond > VFVM_PRECOND_BJACOBI) return set_error(VFVM_ERR_INVALID_ARGUMENT, "unknown preconditioner");
     if (o.precision != VFVM_FP64 && o.precision != VFVM_FP32_REFINED) return set_error(VFVM_ERR_INVALID_ARGUMENT, "unknown precision");
     DevBuf<double> r, z;
     VFVM_HIP_TRY(r.alloc(n));
@@ -1022,6 +1169,24 @@ int precond_apply(Ctx& c, const vfvm_solve_opts& o, const double* r_host, double
             if (int s = ilu_apply<float>(c, sy, c.ilu32.p, r32.p, nullptr, z32.p, nullptr)) return s;
             hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, (const float*)z32.p, zp_.p);
             hipLaunchKernelGGL((k_scatter_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.ilu->perm.p, (const double*)zp_.p, z.p);
+            VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+        }
+    } else if (o.precond == VFVM_PRECOND_BJACOBI) {
+        const bool fp32 = o.precision != VFVM_FP64;
+        if (int s = bjac_setup(c, sy, fp32)) return s;
+        if (!fp32) {
+            Pc<double> pc;
+            pc.bj = c.bjac.inv64.p; pc.nb = c.bjac.nb; pc.nv = c.g.N;
+            if (int s = bjac_apply<double>(c, pc, (const double*)r.p, nullptr, z.p, nullptr)) return s;
+        } else {
+            DevBuf<float> r32, z32;
+            VFVM_HIP_TRY(r32.alloc(n));
+            VFVM_HIP_TRY(z32.alloc(n));
+            Pc<float> pc;
+            pc.bj = c.bjac.inv32.p; pc.nb = c.bjac.nb; pc.nv = c.g.N;
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, (const double*)r.p, r32.p);
+            if (int s = bjac_apply<float>(c, pc, (const float*)r32.p, nullptr, z32.p, nullptr)) return s;
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, (const float*)z32.p, z.p);
             VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         }
     } else {
@@ -1070,13 +1235,14 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     if (!(o.tol > 0.0)) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_TOLERANCE, "GMRES: tolerance must be positive");
     if (o.restart <= 0 || o.restart > MAX_RESTART) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_RESTART, "GMRES: restart must be in [1, 256]");
     if (o.max_outer <= 0) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_MAX_OUTER, "GMRES: max_outer must be positive");
-    if (o.precond < VFVM_PRECOND_NONE || o.precond > VFVM_PRECOND_ILU0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown preconditioner");
+    if (o.precond < VFVM_PRECOND_NONE || o.precond > VFVM_PRECOND_BJACOBI) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown preconditioner");
     if (o.precision != VFVM_FP64 && o.precision != VFVM_FP32_REFINED) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown precision");
     if (o.ortho < VFVM_ORTHO_AUTO || o.ortho > VFVM_ORTHO_CGS2) return set_error(VFVM_ERR_INVALID_ARGUMENT, "GMRES: unknown orthogonalisation");
     const bool mixed = o.precision == VFVM_FP32_REFINED;
     const bool reorth = o.ortho == VFVM_ORTHO_CGS2 || (o.ortho == VFVM_ORTHO_AUTO && !mixed);
     const bool ilu = o.precond == VFVM_PRECOND_ILU0;
     const bool jac = o.precond == VFVM_PRECOND_JACOBI;
+    const bool bjac = o.precond == VFVM_PRECOND_BJACOBI;
     // With ILU(0) the whole solve runs in the colour-major space of the factor: A' = P A P^T
     // shares the factor's index arrays, b' = P b, x' = P x, so the triangular sweeps read and
     // write their rows in place, with no indirection.
@@ -1167,6 +1333,16 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
             lu = c.ilu32.p;
         }
     }
+    if (bjac)
+        if (int s = bjac_setup(c, sy, mixed)) return s;
+    Pc<double> pc64;
+    Pc<float> pc32;
+    pc64.lu = (const double*)(mixed ? nullptr : lu);
+    pc32.lu = (const float*)(mixed ? lu : nullptr);
+    if (bjac) {
+        pc64.bj = c.bjac.inv64.p; pc64.nb = c.bjac.nb; pc64.nv = c.g.N;
+        pc32.bj = c.bjac.inv32.p; pc32.nb = c.bjac.nb; pc32.nv = c.g.N;
+    }
     VFVM_HIP_TRY(hipGetLastError());
     VFVM_HIP_TRY(hipEventRecord(ev.e[1], c.stream));
 
@@ -1194,12 +1370,15 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
         int steps = 0;
         if (!mixed) {
             hipLaunchKernelGGL((k_scale<double, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V64.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<double>(c, sy, w, c.V64.p, c.w64.p, c.z64.p, (const double*)aval, (const double*)lu, reorth, &steps)) return s;
+            if (int s = arnoldi_cycle<double>(c, sy, w, c.V64.p, c.w64.p, c.z64.p, (const double*)aval, pc64, reorth, &steps)) return s;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y)
             hipLaunchKernelGGL((k_combine<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V64.p, w.ldv, w.S, w.L, w.ctrl, c.w64.p);
             if (lu) {
                 if (int s = ilu_apply<double>(c, sy, (const double*)lu, (const double*)c.w64.p, nullptr, c.z64.p, (const int*)nullptr)) return s;
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
+            } else if (bjac) {
+                if (int s = bjac_apply<double>(c, pc64, (const double*)c.w64.p, nullptr, c.z64.p, (const int*)nullptr)) return s;
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
             } else {
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.w64.p, dinv_x, x);
@@ -1207,7 +1386,7 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
         } else {
             // inner FP32 solve of A M^-1 u = r from u = 0, V0 = r / beta in FP32
             hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V32.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<float>(c, sy, w, c.V32.p, c.w32.p, c.w32.p + n, (const float*)aval, (const float*)lu, reorth, &steps)) return s;
+            if (int s = arnoldi_cycle<float>(c, sy, w, c.V32.p, c.w32.p, c.w32.p + n, (const float*)aval, pc32, reorth, &steps)) return s;
             ++refinements;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y), V y accumulated in FP64
@@ -1219,6 +1398,10 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
                 if (int s = ilu_apply<float>(c, sy, (const float*)lu, (const float*)t32, nullptr, z32, (const int*)nullptr)) return s;
                 hipLaunchKernelGGL((k_scale<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z32, c.z64.p, w.S, w.L.misc() + 4, (const int*)nullptr);
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
+            } else if (bjac) {
+                // the FP64 inverse blocks on the FP64 combination
+                if (int s = bjac_apply<double>(c, pc64, (const double*)c.z64.p, nullptr, c.w64.p, (const int*)nullptr)) return s;
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.w64.p, (const double*)nullptr, x);
             } else {
                 hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, dinv_x, x);
             }

@@ -45,6 +45,12 @@ struct Mesh {
     std::vector<double> bsf_norm;   // 3 * S SoA
     std::vector<double> bsf_wgt;    // 3 * S SoA (w_v, w_j, w_k)
     std::vector<double> bnd_sum;    // N; boundary CV area
+    // pressure-gradient coefficients (geom_tet.rs:295-347), built on demand by build_grad_coeff:
+    // CSR over vertices of (vid_m, c_x, c_y, c_z), vid_m ascending (the 2-ring of v)
+    bool has_grad_coeff = false;
+    std::vector<int32_t> gc_ptr;    // N + 1
+    std::vector<int32_t> gc_vid;    // G
+    std::vector<double> gc_c;       // 3G SoA
 
     int64_t num_edges() const { return (int64_t)ngh.size(); }
     int64_t num_subfaces() const { return (int64_t)bsf_vj.size(); }
@@ -55,5 +61,6 @@ int lattice_mesh(int n, uint64_t seed, double jitter, Mesh& m, std::string& err)
 int delaunay_int(const std::vector<int64_t>& ixyz, const std::vector<int32_t>& order,
                  std::vector<int32_t>& tets_out, std::string& err);
 int build_dual(Mesh& m, int nthreads, std::string& err);
+int build_grad_coeff(Mesh& m, std::string& err);
 
 }  // namespace vfvm

@@ -52,11 +52,12 @@ class VFVChemError(RuntimeError):
 class MeshInfo(C.Structure):
     _fields_ = [("num_pts", C.c_int64), ("num_nds", C.c_int64), ("num_tets", C.c_int64),
                 ("num_edges", C.c_int64), ("num_subfaces", C.c_int64), ("num_bnd", C.c_int32),
-                ("max_degree", C.c_int32)]
+                ("max_degree", C.c_int32), ("num_grad_coeff", C.c_int64)]
 
 
 class OpStruct(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("geom_id", C.c_int32), ("unk_id", C.c_int32), ("a", C.c_int32), ("b", C.c_int32)]
+    _fields_ = [("kind", C.c_int32), ("geom_id", C.c_int32), ("unk_id", C.c_int32), ("a", C.c_int32), ("b", C.c_int32),
+                ("c", C.c_int32), ("value", C.c_double)]
 
 
 class SolveOpts(C.Structure):
@@ -101,6 +102,7 @@ SIGNATURES = [
     ("vfvm_mesh_lattice", _I, [_I, C.c_uint64, _D, C.POINTER(_P)]),
     ("vfvm_mesh_from_tets", _I, [_L, _L, _PD, _L, _PI, _I, _PL, _PI, C.POINTER(_P)]),
     ("vfvm_mesh_info_get", _I, [_P, C.POINTER(MeshInfo)]),
+    ("vfvm_mesh_build_grad_coeff", _I, [_P]),
     ("vfvm_mesh_get", _I, [_P, _I, _P]),
     ("vfvm_mesh_destroy", None, [_P]),
     ("vfvm_create", _I, [_I, C.POINTER(_P)]),
@@ -112,9 +114,13 @@ SIGNATURES = [
     ("vfvm_scldom_update_prev", _I, [_P, _I]),
     ("vfvm_vecdom_add", _I, [_P, _I, _PD, _PD, _PI]),
     ("vfvm_vecdom_set", _I, [_P, _I, _PD]),
+    ("vfvm_vecdom_get", _I, [_P, _I, _PD]),
     ("vfvm_sclbnd_add", _I, [_P, _I, _D, _PD, _PI]),
     ("vfvm_sclbnd_set", _I, [_P, _I, _PD]),
     ("vfvm_sclflw_add", _I, [_P, _I, _I, _I, _I, _I, _PI]),
+    ("vfvm_vecbnd_add", _I, [_P, _I, _PD, _PD, _PI]),
+    ("vfvm_update_flow", _I, [_P, _I]),
+    ("vfvm_sclflw_get", _I, [_P, _I, _I, _PD]),
     ("vfvm_pattern_build", _I, [_P, C.POINTER(OpStruct), _I, _PL]),
     ("vfvm_pattern_get", _I, [_P, _PI, _PI]),
     ("vfvm_assemble", _I, [_P, C.POINTER(OpStruct), _I, _D]),

@@ -21,10 +21,23 @@ OP_SCL_BND_PRODUCTION = 23
 OP_SCL_BND_SOURCE = 24
 OP_SCL_BND_TIME = 25
 OP_SCL_BND_ADVECTION_UNITY = 26
+OP_VEC_DOM_ADVECTION = 40
+OP_VEC_DOM_GRADIENT = 41
+OP_VEC_DOM_DIFFUSION = 42
+OP_VEC_DOM_TRANSPOSE = 43
+OP_VEC_DOM_DILATATORY = 44
+OP_VEC_DOM_SOURCE = 45
+OP_SCL_DOM_DIVERGENCE = 46
+OP_SCL_DOM_PRESSURE_STABILIZER = 47
+OP_SCL_DOM_PRESSURE_REFERENCE = 48
+OP_VEC_BND_DIRICHLET = 60
+OP_SCL_BND_DIVERGENCE = 61
+OP_SCL_BND_PRESSURE_STABILIZER = 62
+FLW_FLUX, FLW_RC, FLW_MOM = 0, 1, 2
 
 FIELD_CONSTANT, FIELD_FUNCTION, FIELD_UNKNOWN = 0, 1, 2
 FP64, FP32_REFINED = 0, 1
-PRECOND_NONE, PRECOND_JACOBI, PRECOND_ILU0 = 0, 1, 2
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_ILU0, PRECOND_BJACOBI = 0, 1, 2, 3
 K_SPMV, K_MDOT, K_ASSEMBLE, K_PRECOND, K_UPDATE = 0, 1, 2, 3, 4
 ORTHO_AUTO, ORTHO_CGS, ORTHO_CGS2 = 0, 1, 2
 WRT_SCLDOM, WRT_SCLBND, WRT_INITIAL = 0, 1, 2
@@ -35,6 +48,7 @@ _MESH_ARRAYS = {
     "norm": (8, np.float64), "skew": (9, np.float64), "gw": (10, np.float64), "vol": (11, np.float64),
     "cskew": (12, np.float64), "bsf_ptr": (13, np.int32), "bsf_vj": (14, np.int32), "bsf_vk": (15, np.int32),
     "bsf_area": (16, np.float64), "bsf_norm": (17, np.float64), "bsf_wgt": (18, np.float64),
+    "gc_ptr": (19, np.int32), "gc_vid": (20, np.int32), "gc_c": (21, np.float64),
 }
 
 
@@ -53,7 +67,16 @@ class Mesh:
         self.num_subfaces = info.num_subfaces
         self.num_bnd = info.num_bnd
         self.max_degree = info.max_degree
+        self.num_grad_coeff = info.num_grad_coeff
         self._cache = {}
+
+    def build_grad_coeff(self):
+        """Pressure-gradient coefficients of the flow operators (geom_tet.rs:295-347)."""
+        check(lib.vfvm_mesh_build_grad_coeff(self._h))
+        info = L.MeshInfo()
+        check(lib.vfvm_mesh_info_get(self._h, C.byref(info)))
+        self.num_grad_coeff = info.num_grad_coeff
+        return self
 
     @classmethod
     def lattice(cls, n: int, seed: int = 0, jitter: float = 0.3) -> "Mesh":
@@ -78,7 +101,8 @@ class Mesh:
         N, E, S, T = self.num_tot, self.num_edges, self.num_subfaces, self.num_tets
         return {"pos": 3 * N, "tets": 4 * T, "bnd_ptr": self.num_bnd + 1, "bnd_nid": None, "row_ptr": N + 1, "ngh": E,
                 "dist": E, "area": E, "norm": 3 * E, "skew": 3 * E, "gw": 3 * E, "vol": N, "cskew": 3 * N,
-                "bsf_ptr": N + 1, "bsf_vj": S, "bsf_vk": S, "bsf_area": S, "bsf_norm": 3 * S, "bsf_wgt": 3 * S}[name]
+                "bsf_ptr": N + 1, "bsf_vj": S, "bsf_vk": S, "bsf_area": S, "bsf_norm": 3 * S, "bsf_wgt": 3 * S,
+                "gc_ptr": N + 1, "gc_vid": self.num_grad_coeff, "gc_c": 3 * self.num_grad_coeff}[name]
 
     def get(self, name: str) -> np.ndarray:
         if name in self._cache:
@@ -94,7 +118,7 @@ class Mesh:
             out = out.reshape(-1, 3)
         elif name == "tets":
             out = out.reshape(-1, 4)
-        elif name in ("norm", "skew", "gw", "bsf_norm", "bsf_wgt", "cskew"):
+        elif name in ("norm", "skew", "gw", "bsf_norm", "bsf_wgt", "cskew", "gc_c"):
             out = out.reshape(3, -1)
         self._cache[name] = out
         return out
@@ -113,9 +137,15 @@ class Mesh:
 
 
 def make_ops(ops):
+    """Operator descriptors from tuples (kind, geom_id, unk_id, a, b[, c[, value]])."""
     arr = (L.OpStruct * len(ops))()
     for i, o in enumerate(ops):
-        arr[i].kind, arr[i].geom_id, arr[i].unk_id, arr[i].a, arr[i].b = (int(v) for v in o)
+        o = tuple(o)
+        if len(o) < 5 or len(o) > 7:
+            raise ValueError("operator tuple must be (kind, geom_id, unk_id, a, b[, c[, value]])")
+        arr[i].kind, arr[i].geom_id, arr[i].unk_id, arr[i].a, arr[i].b = (int(v) for v in o[:5])
+        arr[i].c = int(o[5]) if len(o) > 5 else -1
+        arr[i].value = float(o[6]) if len(o) > 6 else 0.0
     return arr
 
 
@@ -199,16 +229,43 @@ class Context:
         check(lib.vfvm_sclflw_add(self._h, int(vel), int(pres), int(den), int(visc), int(fce), C.byref(i)))
         return i.value
 
+    def vecdom_get(self, fid) -> np.ndarray:
+        out = np.empty(3 * self.mesh.num_tot)
+        check(lib.vfvm_vecdom_get(self._h, int(fid), ptr(out)))
+        return out.reshape(3, -1)
+
+    def vecbnd_add(self, bnd, cnst3=None, values=None) -> int:
+        """Boundary velocity (3 x count SoA, or one constant vector)."""
+        i = C.c_int32()
+        c3 = None if cnst3 is None else np.ascontiguousarray(cnst3, np.float64)
+        v = None if values is None else np.ascontiguousarray(values, np.float64)
+        check(lib.vfvm_vecbnd_add(self._h, int(bnd), None if c3 is None else ptr(c3), None if v is None else ptr(v), C.byref(i)))
+        return i.value
+
+    def update_flow(self, flw):
+        check(lib.vfvm_update_flow(self._h, int(flw)))
+
+    def sclflw_get(self, flw, what) -> np.ndarray:
+        size = {FLW_FLUX: self.mesh.num_edges, FLW_RC: self.mesh.num_tot, FLW_MOM: 3 * self.mesh.num_tot}[what]
+        out = np.empty(size)
+        check(lib.vfvm_sclflw_get(self._h, int(flw), int(what), ptr(out)))
+        return out
+
     # pattern / assembly
     def pattern_build(self, ops) -> int:
         arr = make_ops(ops)
         nnz = C.c_int64()
         check(lib.vfvm_pattern_build(self._h, arr, len(ops), C.byref(nnz)))
         self.nnz = nnz.value
+        # unknowns: one scalar, or the flow system [u, v, w, p] (steady_flow.rs:94-104)
+        self.n = (4 if any(int(o[0]) >= OP_VEC_DOM_ADVECTION for o in ops) else 1) * self.mesh.num_tot
         return nnz.value
 
+    def _size(self):
+        return getattr(self, "n", self.mesh.num_tot)
+
     def pattern_get(self):
-        rp = np.empty(self.mesh.num_tot + 1, np.int32)
+        rp = np.empty(self._size() + 1, np.int32)
         col = np.empty(max(self.nnz, 1), np.int32)
         check(lib.vfvm_pattern_get(self._h, ptr(rp, C.c_int32), ptr(col, C.c_int32)))
         return rp, col[:self.nnz]
@@ -219,12 +276,12 @@ class Context:
 
     def matrix_get(self):
         vals = np.empty(max(self.nnz, 1))
-        b = np.empty(self.mesh.num_tot)
+        b = np.empty(self._size())
         check(lib.vfvm_matrix_get(self._h, ptr(vals), ptr(b)))
         return vals[:self.nnz], b
 
     def solve(self, x0=None, **kw):
-        x = np.zeros(self.mesh.num_tot) if x0 is None else np.array(x0, np.float64)
+        x = np.zeros(self._size()) if x0 is None else np.array(x0, np.float64)
         st = L.SolveStats()
         o = solve_opts(**kw)
         code = lib.vfvm_solve(self._h, C.byref(o), ptr(x), C.byref(st))
@@ -253,7 +310,7 @@ class Context:
 
     def ilu_order(self):
         """(perm, color_ptr): the ILU(0) row order of the built pattern."""
-        n = self.mesh.num_tot
+        n = self._size()
         perm = np.empty(n, np.int32)
         cptr = np.empty(n + 1, np.int32)
         nc = C.c_int32()
@@ -265,6 +322,7 @@ class Context:
         arr = make_ops(ops)
         st = L.DriverStats()
         o = solve_opts(**(lin or {}))
+        self.n = (4 if any(int(op[0]) >= OP_VEC_DOM_ADVECTION for op in ops) else 1) * self.mesh.num_tot
         check(lib.vfvm_steady_solve(self._h, arr, len(ops), C.byref(o), int(max_iter), float(tol), float(damp), C.byref(st)))
         return st
 
