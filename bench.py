@@ -16,6 +16,10 @@ face): one forward pass with a checkpoint every 10 steps and one backward pass (
 replays + transposed GMRES + assembly VJP) for dJ/dk at every vertex, FP32-inner /
 FP64-refined solves, reported as the backward/forward time ratio.
 
+And the flow line (lid-driven cavity at Re = 100 on an 80^3 jittered lattice, 2,048,000
+DOF): the coupled velocity-pressure system after a few Picard iterations, solved FP64 and
+FP32-inner / FP64-refined with GMRES(100) + 4x4 block Jacobi.
+
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
@@ -132,6 +136,106 @@ def run_adjoint(vfvm, args):
     }
 
 
+def build_lid_cavity(vfvm, n, rho=1.0, mu=0.01, lid=1.0):
+    """Steady lid-driven cavity (SteadyFlow composition of flow_lid.rs): no-slip walls, lid
+    (lid, 0, 0) on z = 1, zero body force, reference pressure 0 at node 0."""
+    import numpy as np
+    mesh = vfvm.Mesh.lattice(n, 0, 0.3)
+    mesh.build_grad_coeff()
+    ctx = vfvm.Context(0)
+    ctx.upload(mesh)
+    vel = ctx.vecdom_add(vfvm.FIELD_UNKNOWN, cnst3=np.zeros(3))
+    pres = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0)
+    den = ctx.scldom_add(vfvm.FIELD_CONSTANT, rho)
+    visc = ctx.scldom_add(vfvm.FIELD_CONSTANT, mu)
+    fce = ctx.vecdom_add(vfvm.FIELD_CONSTANT, cnst3=np.zeros(3))
+    flw = ctx.sclflw_add(vel, pres, den, visc, fce)
+    ops = [(vfvm.OP_VEC_DOM_ADVECTION, 0, vel, den, visc, flw), (vfvm.OP_VEC_DOM_GRADIENT, 0, vel, pres, -1),
+           (vfvm.OP_VEC_DOM_DIFFUSION, 0, vel, visc, -1), (vfvm.OP_VEC_DOM_TRANSPOSE, 0, vel, visc, -1),
+           (vfvm.OP_VEC_DOM_DILATATORY, 0, vel, visc, -1), (vfvm.OP_VEC_DOM_SOURCE, 0, vel, fce, -1),
+           (vfvm.OP_SCL_DOM_DIVERGENCE, 0, pres, vel, den), (vfvm.OP_SCL_DOM_PRESSURE_STABILIZER, 0, pres, den, fce, flw)]
+    for b in range(6):
+        vb = ctx.vecbnd_add(b, cnst3=np.array([lid, 0.0, 0.0]) if b == 5 else np.zeros(3))
+        ops += [(vfvm.OP_VEC_BND_DIRICHLET, b, vel, vb, -1), (vfvm.OP_SCL_BND_DIVERGENCE, b, pres, vel, den),
+                (vfvm.OP_SCL_BND_PRESSURE_STABILIZER, b, pres, den, fce, flw)]
+    ops.append((vfvm.OP_SCL_DOM_PRESSURE_REFERENCE, 0, pres, 0, -1, -1, 0.0))
+    return mesh, ctx, ops, vel, pres
+
+
+def run_flow(vfvm, args):
+    """Lid-driven cavity at Re = 100: Picard iterations of the coupled velocity-pressure system
+    with GMRES(100) + 4x4 block Jacobi; the linear solve of the system after a few Picard
+    iterations timed FP64 vs FP32-inner/FP64-refined (solved DOF/s at 1e-10)."""
+    import torch
+    t0 = time.time()
+    mesh, ctx, ops, vel, pres = build_lid_cavity(vfvm, args.flow_n)
+    t_setup = time.time() - t0
+    N = mesh.num_tot
+    ndof = 4 * N
+    lin = dict(tol=1e-8, restart=args.flow_restart, max_outer=400, precision=vfvm.FP32_REFINED,
+               precond=vfvm.PRECOND_BJACOBI, inner_tol=1e-4, ortho=1)
+    # a few damped Picard iterations (tol unreachable: the driver stops at max_iter) so the
+    # timed system carries the advection of a developed flow
+    tp = time.time()
+    picard = None
+    try:
+        ctx.steady_solve(ops, args.flow_picard, 1e-300, 0.8, lin=lin)
+    except vfvm.VFVChemError as e:
+        picard = str(e)
+    t_picard = time.time() - tp
+    _, nnz = ctx.system_info()
+    ctx.timing_enable(-1)
+    ctx.timing_reset()
+    ta = time.time()
+    ctx.assemble(ops)
+    torch.cuda.synchronize()
+    t_asm = time.time() - ta
+    x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
+    xp = x.data_ptr()
+    out = {}
+    for var in ("fp64", "mixed"):
+        prec = vfvm.FP64 if var == "fp64" else vfvm.FP32_REFINED
+        kw = dict(tol=args.tol, restart=args.flow_restart, max_outer=2000, precision=prec, precond=vfvm.PRECOND_BJACOBI,
+                  inner_tol=args.inner_tol, ortho=1)
+        ctx.timing_enable(0)
+        x.zero_()
+        torch.cuda.synchronize()
+        ctx.solve_device(xp, **kw)
+        ts = []
+        for _ in range(args.flow_steps):
+            x.zero_()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            st = ctx.solve_device(xp, **kw)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t1)
+        ctx.timing_enable(-1)
+        ctx.timing_reset()
+        x.zero_()
+        torch.cuda.synchronize()
+        ctx.solve_device(xp, **kw)
+        torch.cuda.synchronize()
+        ctx.timing_enable(0)
+        kern = {}
+        for name, k in (("spmv", vfvm.K_SPMV), ("mdot", vfvm.K_MDOT), ("update", vfvm.K_UPDATE), ("bjacobi", vfvm.K_PRECOND)):
+            nl, ms, by = ctx.timing_get(k)
+            if nl:
+                kern[name] = dict(launches=nl, avg_us=1e3 * ms / nl, total_ms=ms, bytes_per_launch=by / nl,
+                                  gbs=by / (ms * 1e-3) / 1e9)
+        ms = 1e3 * sum(ts) / len(ts)
+        out[var] = dict(ms_per_step=ms, dof_per_s=ndof / (ms * 1e-3), iters=st.iterations, refinements=st.refinements,
+                        rel_residual=st.rel_residual, setup_s=st.setup_seconds, kernels=kern)
+    ctx.close()
+    return {
+        "workload": f"steady lid-driven cavity Re=100 (rho=1, mu=0.01, lid (1,0,0)), {args.flow_n}^3 jittered lattice "
+                    f"({N} vertices, {ndof} DOF, {nnz} nnz), system after {args.flow_picard} damped Picard iterations, "
+                    f"GMRES({args.flow_restart}) + 4x4 block Jacobi, tol {args.tol}",
+        "dof": ndof, "nnz": nnz, "fp64": out["fp64"], "mixed": out["mixed"],
+        "mixed_over_fp64": out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"],
+        "assembly_wall_ms": 1e3 * t_asm, "picard_s": t_picard, "picard_note": picard, "mesh_dual_gc_s": t_setup,
+    }
+
+
 def measured_traffic(kernel, variant):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary of this workload (scaled by a copy kernel of known size), or None."""
@@ -172,6 +276,11 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--flow-n", type=int, default=80, help="lattice points per axis of the lid-driven cavity")
+    ap.add_argument("--flow-picard", type=int, default=3, help="Picard iterations before the timed flow solves")
+    ap.add_argument("--flow-restart", type=int, default=100)
+    ap.add_argument("--flow-steps", type=int, default=2)
+    ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first copy MB megabytes device-to-device 5 times (a plain copy kernel of known size, "
                          "used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -280,6 +389,7 @@ def main():
 
     ctx.close()
     adjoint = None if args.no_adjoint else run_adjoint(vfvm, args)
+    flow = None if args.no_flow else run_flow(vfvm, args)
 
     line = {
         "metric": "solved DOF/s at 1e-10 rel residual (FP32-inner/FP64-refined GMRES; FP64 GMRES beside it); "
@@ -309,6 +419,7 @@ def main():
                      "algorithmic_bytes_per_launch": dom["bytes_per_launch"]},
         "cpu_baseline": cpu,
         "adjoint": adjoint,
+        "flow": flow,
     }
     print(json.dumps(line))
 

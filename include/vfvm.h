@@ -244,6 +244,8 @@ typedef struct vfvm_op {
  * assemble_matrix applies its operators. */
 int vfvm_pattern_build(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, int64_t* nnz);
 int vfvm_pattern_get(vfvm_ctx* ctx, int32_t* row_ptr, int32_t* col);
+/* Size of the built system: n = mat_size (rows), nnz (LinearBase::update_sparse's pattern). */
+int vfvm_system_info(vfvm_ctx* ctx, int64_t* n, int64_t* nnz);
 
 /* update_grad + update_flow + assemble_matrix + build_matrix on the device: writes
  * the CSR values and b in place, in a fixed per-row summation order, no atomics.

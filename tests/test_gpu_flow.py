@@ -109,3 +109,29 @@ def test_lid_cavity_picard_vs_oracle(ctx, lid_oracle, precision, bar):
     # the lid drives a recirculation: u < 0 somewhere in the lower half
     pos = mesh.get("pos")
     assert np.min(vel[0][:mesh.num_pts][pos[:mesh.num_pts, 2] < 0.5]) < 0.0
+
+
+def test_reference_api_mirror_steady_flow(ctx, lid_oracle):
+    """flow_lid.rs through the mirrored API (Variables / SteadyFlow / LinearGmres) gives the
+    same solution as the operator list of the oracle's lid cavity."""
+    print("[flow api mirror]", flush=True)
+    mesh, (ovel, opres, its, hist) = lid_oracle
+    mesh.build_grad_coeff()
+    vars = vfvm.Variables(mesh)
+    dom = vars.add_dom("cube")
+    bnds = [vars.add_bnd(dom, name) for name in ("x0", "x1", "y0", "y1", "z0", "z1")]
+    vel = vars.add_vecdom_unkn(dom, (0.0, 0.0, 0.0))
+    pres = vars.add_scldom_unkn(dom, 0.0)
+    den = vars.add_scldom_cnst(dom, 1.0)
+    visc = vars.add_scldom_cnst(dom, 0.05)
+    fce = vars.add_vecdom_cnst(dom, (0.0, 0.0, 0.0))
+    phys = vfvm.SteadyFlow.new()
+    phys.add_flow_dom(dom, vel, pres, den, visc, fce)
+    for k, b in enumerate(bnds):
+        phys.add_vel_bnd(b, vars.add_vecbnd_cnst(b, (1.0, 0.0, 0.0) if k == 5 else (0.0, 0.0, 0.0)))
+    phys.set_pres_ref(dom, 0, 0.0)
+    lin = vfvm.LinearGmres.new_with_options(1, 1e-13, 100, 400, precond=vfvm.PRECOND_BJACOBI)
+    st = phys.solve(vars, lin, 50, 1e-6, 0.8)
+    assert st.picard_iterations == its
+    v = vars.value_vec(vel)
+    assert np.max(np.abs(v - ovel)) <= 1e-8 * np.max(np.abs(ovel))

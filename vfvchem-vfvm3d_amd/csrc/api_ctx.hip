@@ -531,6 +531,14 @@ extern "C" int vfvm_pattern_build(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nop
     return clear_error();
 }
 
+extern "C" int vfvm_system_info(vfvm_ctx* ctx, int64_t* n, int64_t* nnz) {
+    CTX_GUARD(ctx);
+    if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_system_info: no pattern built");
+    if (n) *n = c.n;
+    if (nnz) *nnz = c.nnz;
+    return clear_error();
+}
+
 extern "C" int vfvm_pattern_get(vfvm_ctx* ctx, int32_t* row_ptr, int32_t* col) {
     CTX_GUARD(ctx);
     if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_pattern_get: no pattern built");

@@ -123,6 +123,7 @@ SIGNATURES = [
     ("vfvm_sclflw_get", _I, [_P, _I, _I, _PD]),
     ("vfvm_pattern_build", _I, [_P, C.POINTER(OpStruct), _I, _PL]),
     ("vfvm_pattern_get", _I, [_P, _PI, _PI]),
+    ("vfvm_system_info", _I, [_P, _PL, _PL]),
     ("vfvm_assemble", _I, [_P, C.POINTER(OpStruct), _I, _D]),
     ("vfvm_matrix_get", _I, [_P, _PD, _PD]),
     ("vfvm_solve_opts_default", None, [C.POINTER(SolveOpts)]),

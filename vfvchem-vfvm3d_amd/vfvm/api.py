@@ -8,6 +8,9 @@ Names, argument order and error behaviour follow the reference:
   SteadyMassFlow       src/steady/core/steady_massflow.rs, mass part with a
                        prescribed velocity (add_flow_prescribed instead of an
                        unknown velocity)
+  SteadyFlow           src/steady/core/steady_flow.rs:9-321 (velocity boundaries and
+                       a reference pressure; pressure/traction boundaries and
+                       interfaces are outside the built path)
   LinearGmres          src/linear/linear_gmres.rs:16-50
 The operator lists these classes build are applied in the order of each
 reference solver's assemble_matrix; the Picard loops run in the library
@@ -19,6 +22,10 @@ from __future__ import annotations
 import numpy as np
 
 from . import _lib as L
+from .core import (OP_SCL_BND_DIVERGENCE, OP_SCL_BND_PRESSURE_STABILIZER, OP_SCL_DOM_DIVERGENCE,
+                   OP_SCL_DOM_PRESSURE_REFERENCE, OP_SCL_DOM_PRESSURE_STABILIZER, OP_VEC_BND_DIRICHLET,
+                   OP_VEC_DOM_ADVECTION, OP_VEC_DOM_DIFFUSION, OP_VEC_DOM_DILATATORY, OP_VEC_DOM_GRADIENT,
+                   OP_VEC_DOM_SOURCE, OP_VEC_DOM_TRANSPOSE, PRECOND_BJACOBI)
 from .core import (FIELD_CONSTANT, FIELD_FUNCTION, FIELD_UNKNOWN, FP32_REFINED, FP64, PRECOND_ILU0, PRECOND_JACOBI,
                    PRECOND_NONE, OP_SCL_BND_ADVECTION_UNITY, OP_SCL_BND_DIFFUSION, OP_SCL_BND_DIRICHLET,
                    OP_SCL_BND_NEUMANN, OP_SCL_BND_PRODUCTION, OP_SCL_BND_SOURCE, OP_SCL_BND_TIME,
@@ -42,6 +49,7 @@ class Variables:
         self.scldom = []        # (kind, func, unkn_ids)
         self.sclbnd = []        # (bnd_id, func, unkn_ids)
         self.vecdom = []
+        self.vecbnd = []
         self.sclflw = []
 
     @staticmethod
@@ -106,6 +114,23 @@ class Variables:
         self.vecdom.append((FIELD_FUNCTION, func, list(unkn_id)))
         return i
 
+    def add_vecdom_unkn(self, dom_id: int, value_init, file_path: str = "") -> int:
+        i = self.ctx.vecdom_add(FIELD_UNKNOWN, cnst3=np.asarray(value_init, np.float64))
+        self.vecdom.append((FIELD_UNKNOWN, None, []))
+        return i
+
+    # vector boundary fields (vars.rs:169-184); values SoA (3, count)
+    def add_vecbnd_cnst(self, bnd_id: int, value, file_path: str = "") -> int:
+        i = self.ctx.vecbnd_add(self.bnd[bnd_id], cnst3=np.asarray(value, np.float64))
+        self.vecbnd.append((bnd_id, None, []))
+        return i
+
+    def add_vecbnd_func(self, bnd_id: int, func, unkn_id=(), file_path: str = "") -> int:
+        vals = np.asarray(func(0.0, self._bnd_pos(bnd_id), []), np.float64).reshape(3, -1)
+        i = self.ctx.vecbnd_add(self.bnd[bnd_id], values=vals)
+        self.vecbnd.append((bnd_id, func, list(unkn_id)))
+        return i
+
     def add_sclflw(self, dom_id: int, vel_id: int, pres_id: int, den_id: int, visc_id: int, fce_id: int) -> int:
         i = self.ctx.sclflw_add(vel_id, pres_id, den_id, visc_id, fce_id)
         self.sclflw.append((vel_id, pres_id, den_id, visc_id, fce_id))
@@ -125,6 +150,10 @@ class Variables:
 
     def value(self, scldom_id: int) -> np.ndarray:
         return self.ctx.scldom_get(scldom_id)
+
+    def value_vec(self, vecdom_id: int) -> np.ndarray:
+        """Vector field values, SoA (3, N)."""
+        return self.ctx.vecdom_get(vecdom_id)
 
 
 class LinearGmres:
@@ -325,3 +354,54 @@ class SteadyMass(SteadyMassFlow):
         if num_comp != 1:
             raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "one component on the built path")
         return SteadyMass()
+
+
+class SteadyFlow(SteadyBase):
+    """SteadyFlow (steady_flow.rs:9-321): steady incompressible Navier-Stokes with the
+    Rhie-Chow pressure stabiliser; one domain, velocity boundaries, reference pressure.
+    The unknown vector is [u, v, w, p] (initial_start, steady_flow.rs:94-104)."""
+
+    def __init__(self):
+        self.itr = []        # (dom, vel, pres, den, visc, fce)
+        self.vel_bnd = []    # (bnd, vel)
+        self.pref = []       # (dom, nid, pres)
+        self.flw = None
+
+    @staticmethod
+    def new():
+        return SteadyFlow()
+
+    def add_flow_dom(self, dom_id, vel_id, pres_id, den_id, visc_id, fce_id):
+        self.itr.append((dom_id, vel_id, pres_id, den_id, visc_id, fce_id))
+
+    def add_vel_bnd(self, bnd_id, vel_id):
+        self.vel_bnd.append((bnd_id, vel_id))
+
+    def add_pres_bnd(self, bnd_id, pres_id):
+        raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "pressure boundaries are outside the built path")
+
+    def add_trac_bnd(self, bnd_id, pres_id):
+        raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "traction boundaries are outside the built path")
+
+    def set_pres_ref(self, dom_id, nid, pres):
+        self.pref.append((dom_id, int(nid), float(pres)))
+
+    def operators(self, vars: Variables):
+        if len(self.itr) != 1:
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, "one flow domain on the built path")
+        dom, vel, pres, den, visc, fce = self.itr[0]
+        if self.flw is None:
+            # initial_start creates the flow data (steady_flow.rs:106-118)
+            self.flw = vars.add_sclflw(dom, vel, pres, den, visc, fce)
+        f = self.flw
+        ops = [(OP_VEC_DOM_ADVECTION, 0, vel, den, visc, f), (OP_VEC_DOM_GRADIENT, 0, vel, pres, -1),
+               (OP_VEC_DOM_DIFFUSION, 0, vel, visc, -1), (OP_VEC_DOM_TRANSPOSE, 0, vel, visc, -1),
+               (OP_VEC_DOM_DILATATORY, 0, vel, visc, -1), (OP_VEC_DOM_SOURCE, 0, vel, fce, -1),
+               (OP_SCL_DOM_DIVERGENCE, 0, pres, vel, den), (OP_SCL_DOM_PRESSURE_STABILIZER, 0, pres, den, fce, f)]
+        for bnd_id, vb in self.vel_bnd:
+            b = vars.bnd[bnd_id]
+            ops += [(OP_VEC_BND_DIRICHLET, b, vel, vb, -1), (OP_SCL_BND_DIVERGENCE, b, pres, vel, den),
+                    (OP_SCL_BND_PRESSURE_STABILIZER, b, pres, den, fce, f)]
+        for _, nid, p in self.pref:
+            ops.append((OP_SCL_DOM_PRESSURE_REFERENCE, 0, pres, nid, -1, -1, p))
+        return ops

@@ -261,6 +261,13 @@ class Context:
         self.n = (4 if any(int(o[0]) >= OP_VEC_DOM_ADVECTION for o in ops) else 1) * self.mesh.num_tot
         return nnz.value
 
+    def system_info(self):
+        """(n, nnz) of the built system."""
+        n, nnz = C.c_int64(), C.c_int64()
+        check(lib.vfvm_system_info(self._h, C.byref(n), C.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+        return n.value, nnz.value
+
     def _size(self):
         return getattr(self, "n", self.mesh.num_tot)
 
