@@ -164,6 +164,7 @@ struct FlowSys {
     double pref_val = 0.0;
     int32_t gmax = 0, rmax = 0;     // largest grad_coeff list and 3-ring of a continuity row
     int32_t lpr = 16;               // lanes per momentum row
+    int64_t spmv_idx = 0;           // column indices the structured SpMV reads per product
 };
 
 // Block-Jacobi inverse blocks (b x b per vertex, entry (i, j) of vertex v at (i b + j) N + v).

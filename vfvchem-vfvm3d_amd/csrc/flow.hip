@@ -830,6 +830,12 @@ int vfvm::flow_pattern_build(Ctx& c, const vfvm_op* ops, int32_t nops) {
         const int64_t row = 3 * N + v;
         c.h_mrow[row + 1] = row == fs.pref_row ? 1 : 3 * (gp[v + 1] - gp[v]) + r3len[v];
     }
+    fs.spmv_idx = 0;
+    for (int64_t v = 0; v < N; ++v) {
+        const int64_t G = gp[v + 1] - gp[v];
+        const bool pref = 3 * N + v == fs.pref_row;
+        fs.spmv_idx += (v < NP ? (rp[v + 1] - rp[v]) + 1 + G : 3) + (pref ? 1 : G + r3len[v]);
+    }
     int64_t tot = 0;
     for (int64_t i = 0; i < n; ++i) {
         tot += c.h_mrow[i + 1];

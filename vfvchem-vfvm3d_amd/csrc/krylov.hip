@@ -76,7 +76,15 @@ __device__ inline double block_sum(double v, double* sh) {
 }
 
 // ---------------------------------------------------------------- SpMV
-// LPR lanes per row; scalar rows carry ~17 entries
+// LPR lanes per row (scalar rows carry ~17 entries, coupled flow rows ~100-480)
+template <class T>
+__device__ inline T row_dot(const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ x, int32_t k,
+                            int32_t e1, int step) {
+    T s = 0;
+    for (; k < e1; k += step) s += val[k] * x[col[k]];
+    return s;
+}
+
 template <class T, int LPR>
 __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                               const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
@@ -86,11 +94,103 @@ __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restri
     const int64_t row = tid / LPR;
     const int lane = (int)(tid % LPR);
     if (row >= n) return;
-    T s = 0;
-    const int32_t e1 = rp[row + 1];
-    for (int32_t k = rp[row] + lane; k < e1; k += LPR) s += val[k] * x[col[k]];
+    T s = row_dot<T>(col, val, x, rp[row] + lane, rp[row + 1], LPR);
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
     if (lane == 0) y[row] = scale ? (T)(s * (T)scale[0]) : s;
+}
+
+// SpMV of the coupled flow system [u, v, w, p] (rows and columns comp-major), one wave per
+// vertex v computing its four rows together.  The three velocity rows of a point share their
+// column sets ({v} u ngh(v) in their own component, grad_coeff(v) in p) and the continuity
+// row reads the velocity components at grad_coeff(v) and p on the 3-ring, so one pass over
+// grad_coeff(v) gathers p[m] for the three velocity rows and u[m], v[m], w[m] for the
+// continuity row: the column indices are read once per vertex instead of once per entry
+// (the u row's own block, its p block and the continuity row's 3-ring), and every gathered
+// x entry feeds up to three products.  Rows of length 1 (velocity Dirichlet rows, the
+// reference-pressure row) are diagonal.
+struct FlowSpmvGeom {
+    int64_t N, NP;
+    const int32_t* rp;      // vertex adjacency (degree of v)
+    const int32_t* gc_ptr;  // grad_coeff list lengths
+};
+template <class T>
+__device__ inline T wave_sum(T s) {
+    for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
+    return s;
+}
+template <class T>
+__global__ void __launch_bounds__(BLK) k_spmv_flow(FlowSpmvGeom fg, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                   const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
+                                                   const double* __restrict__ scale, const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0] == 0) return;
+    const int64_t N = fg.N;
+    const int64_t v = ((int64_t)xcd_block() * BLK + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (v >= N) return;
+    const T sc = scale ? (T)scale[0] : (T)1;
+    const T* x0 = x;
+    const T* x1 = x + N;
+    const T* x2 = x + 2 * N;
+    const T* xp = x + 3 * N;
+    const int32_t pN3 = (int32_t)(3 * N);
+    const int32_t ru = rp[v], rv = rp[N + v], rw = rp[2 * N + v], rq = rp[3 * N + v];
+    const int32_t lu = rp[v + 1] - ru, lq = rp[3 * N + v + 1] - rq;
+    T su = 0, sv = 0, sw = 0, sq = 0;
+    const int32_t G = fg.gc_ptr[v + 1] - fg.gc_ptr[v];
+    if (lu > 1) {
+        // own-component part of the velocity rows: {v} u ngh(v)
+        const int32_t d1 = fg.rp[v + 1] - fg.rp[v] + 1;
+        for (int32_t k = lane; k < d1; k += WAVE) {
+            const int32_t m = col[ru + k];
+            su += val[ru + k] * x0[m];
+            sv += val[rv + k] * x1[m];
+            sw += val[rw + k] * x2[m];
+        }
+        // p part of the velocity rows and velocity part of the continuity row: grad_coeff(v)
+        if (lq > 1) {
+            for (int32_t q = lane; q < G; q += WAVE) {
+                const int32_t m = col[ru + d1 + q] - pN3;
+                const T pm = xp[m];
+                su += val[ru + d1 + q] * pm;
+                sv += val[rv + d1 + q] * pm;
+                sw += val[rw + d1 + q] * pm;
+                sq += val[rq + q] * x0[m] + val[rq + G + q] * x1[m] + val[rq + 2 * G + q] * x2[m];
+            }
+        } else {
+            for (int32_t q = lane; q < G; q += WAVE) {
+                const T pm = xp[col[ru + d1 + q] - pN3];
+                su += val[ru + d1 + q] * pm;
+                sv += val[rv + d1 + q] * pm;
+                sw += val[rw + d1 + q] * pm;
+            }
+        }
+    } else {
+        if (lane == 0) {
+            su = val[ru] * x[col[ru]];
+            sv = val[rv] * x[col[rv]];
+            sw = val[rw] * x[col[rw]];
+        }
+        if (lq > 1)
+            for (int32_t q = lane; q < G; q += WAVE) {
+                const int32_t m = col[rq + q];
+                sq += val[rq + q] * x0[m] + val[rq + G + q] * x1[m] + val[rq + 2 * G + q] * x2[m];
+            }
+    }
+    if (lq > 1) {
+        for (int32_t i = 3 * G + lane; i < lq; i += WAVE) sq += val[rq + i] * xp[col[rq + i] - pN3];
+    } else if (lane == 0) {
+        sq = val[rq] * x[col[rq]];
+    }
+    su = wave_sum(su);
+    sv = wave_sum(sv);
+    sw = wave_sum(sw);
+    sq = wave_sum(sq);
+    if (lane == 0) {
+        y[v] = su * sc;
+        y[N + v] = sv * sc;
+        y[2 * N + v] = sw * sc;
+        y[3 * N + v] = sq * sc;
+    }
 }
 
 // r = b - A x (FP64) and per-block partial sums of r^2 and b^2 (grid-stride over rows)
@@ -104,9 +204,7 @@ __global__ void __launch_bounds__(BLK) k_residual(int64_t n, const int32_t* __re
     const int64_t rows_per_pass = (int64_t)gridDim.x * (BLK / LPR);
     double rr = 0.0, bb = 0.0;
     for (int64_t row = ((int64_t)xcd_block() * BLK + threadIdx.x) / LPR; row < n; row += rows_per_pass) {
-        double s = 0.0;
-        const int32_t e1 = rp[row + 1];
-        for (int32_t k = rp[row] + lane; k < e1; k += LPR) s += val[k] * x[col[k]];
+        double s = row_dot<double>(col, val, x, rp[row] + lane, rp[row + 1], LPR);
         for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
         if (lane == 0) {
             const double bi = b[row];
@@ -702,8 +800,14 @@ struct Work {
 };
 
 inline int pick_lpr(int64_t n, int64_t nnz) {
+    static const int forced = [] {
+        const char* e = getenv("VFVM_SPMV_LPR");
+        const int v = e ? atoi(e) : 0;
+        return (v == 4 || v == 8 || v == 16 || v == 32 || v == 64) ? v : 0;
+    }();
+    if (forced) return forced;
     double avg = n > 0 ? (double)nnz / (double)n : 1.0;
-    return avg > 24.0 ? 16 : (avg > 10.0 ? 8 : 4);
+    return avg > 96.0 ? 32 : (avg > 24.0 ? 16 : (avg > 10.0 ? 8 : 4));
 }
 
 template <class T>
@@ -712,9 +816,19 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
     unsigned grid = grid_for(n * lpr);
     const int32_t* rp = sy.row;
     const int32_t* cl = sy.col;
+    if (c.flow && rp == c.mrow.p && n == 4 * c.g.N) {
+        // the coupled flow system in its own (unpermuted) order
+        FlowSpmvGeom fg{c.g.N, c.g.NP, c.g.row_ptr.p, c.g.gc_ptr.p};
+        launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T>, dim3(grid_for(c.g.N * WAVE)), dim3(BLK), fg, rp, cl, val, x, y,
+                     scale, ctrl);
+        VFVM_HIP_TRY(hipGetLastError());
+        return VFVM_OK;
+    }
     switch (lpr) {
         case 4: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 4>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
         case 8: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 8>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
+        case 32: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 32>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
+        case 64: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 64>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
         default: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 16>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
     }
     VFVM_HIP_TRY(hipGetLastError());
@@ -727,6 +841,8 @@ int residual_launch(Ctx& c, const SysView& sy, int lpr, const double* x, double*
     switch (lpr) {
         case 4: hipLaunchKernelGGL((k_residual<4>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
         case 8: hipLaunchKernelGGL((k_residual<8>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
+        case 32: hipLaunchKernelGGL((k_residual<32>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
+        case 64: hipLaunchKernelGGL((k_residual<64>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
         default: hipLaunchKernelGGL((k_residual<16>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
     }
     VFVM_HIP_TRY(hipGetLastError());
@@ -982,7 +1098,11 @@ int issue_step(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* 
     const int lpr = pick_lpr(n, w.nnz);
     double* part = c.part.p;
     double* part_nrm = part + (int64_t)(MAX_RESTART + 1) * w.nwav;
-    const double bytes_spmv = (double)w.nnz * (sizeof(T) + 4) + 4.0 * (n + 1) + 2.0 * sizeof(T) * n;
+    // algorithmic bytes: values, the column indices the kernel must read, row pointers, x, y
+    const bool flow_spmv = c.flow && sy.row == c.mrow.p && n == 4 * c.g.N;
+    const double bytes_spmv = flow_spmv ? (double)w.nnz * sizeof(T) + 4.0 * c.fs.spmv_idx + 4.0 * (n + 1) + 8.0 * c.g.N +
+                                              2.0 * sizeof(T) * n
+                                        : (double)w.nnz * (sizeof(T) + 4) + 4.0 * (n + 1) + 2.0 * sizeof(T) * n;
     // v_j = sig_j V[j]: the scale enters the preconditioner's right-hand side, or the SpMV
     // output when there is no ILU
     T* vj = V + (int64_t)j * w.ldv;
