@@ -36,6 +36,11 @@ sys.path.insert(0, os.path.join(ROOT, "vfvchem-vfvm3d_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    """Stage line on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def cellular_velocity(pos, U0):
     import numpy as np
     x, y = pos[:, 0], pos[:, 1]
@@ -183,6 +188,7 @@ def run_flow(vfvm, args):
     except vfvm.VFVChemError as e:
         picard = str(e)
     t_picard = time.time() - tp
+    progress(f"lid cavity: {args.flow_picard} Picard iterations in {t_picard:.1f} s")
     _, nnz = ctx.system_info()
     ctx.timing_enable(-1)
     ctx.timing_reset()
@@ -223,6 +229,7 @@ def run_flow(vfvm, args):
                 kern[name] = dict(launches=nl, avg_us=1e3 * ms / nl, total_ms=ms, bytes_per_launch=by / nl,
                                   gbs=by / (ms * 1e-3) / 1e9)
         ms = 1e3 * sum(ts) / len(ts)
+        progress(f"lid cavity {var}: {ms:.1f} ms per solve")
         out[var] = dict(ms_per_step=ms, dof_per_s=ndof / (ms * 1e-3), iters=st.iterations, refinements=st.refinements,
                         rel_residual=st.rel_residual, setup_s=st.setup_seconds, kernels=kern)
     ctx.close()
@@ -307,6 +314,7 @@ def main():
     t0 = time.time()
     mesh, ctx, ops = build_problem(vfvm, args.n)
     t_mesh = time.time() - t0
+    progress(f"convection-diffusion {args.n}^3: mesh and fields in {t_mesh:.1f} s")
     nnz = ctx.pattern_build(ops)
     ctx.timing_enable(-1)
     ctx.timing_reset()
@@ -355,6 +363,7 @@ def main():
             if nl:
                 kern[name] = dict(launches=nl, avg_us=1e3 * ms / nl, total_ms=ms, bytes_per_launch=by / nl,
                                   gbs=by / (ms * 1e-3) / 1e9)
+        progress(f"convection-diffusion {var}: {1e3 * (t2 - t1) / args.steps:.1f} ms per solve")
         results[var] = dict(ms_per_step=1e3 * (t2 - t1) / args.steps, iters=stats[-1][0], refinements=stats[-1][1],
                             rel_residual=stats[-1][2], device_s=stats[-1][3], setup_s=stats[-1][4], kernels=kern,
                             max_abs_err_vs_exact=err)
@@ -388,7 +397,9 @@ def main():
                "cpu": cpu_model(), "nproc": os.cpu_count()}
 
     ctx.close()
+    progress("adjoint workload")
     adjoint = None if args.no_adjoint else run_adjoint(vfvm, args)
+    progress("lid-driven cavity workload")
     flow = None if args.no_flow else run_flow(vfvm, args)
 
     line = {

@@ -195,6 +195,9 @@ enum vfvm_op_kind {
     VFVM_OP_SCL_DOM_SOURCE = 3,           /* OpSclDomSource{unk,src}              opscl_dom_src.rs         a = src               */
     VFVM_OP_SCL_DOM_TIME = 4,             /* OpSclDomTime{unk,wgt}                opscl_dom_time.rs:7      a = wgt               */
     VFVM_OP_SCL_DOM_ADVECTION_UNITY = 5,  /* OpSclDomAdvectionUnity{unk,diff?,flw} opscl_dom_advunity.rs   a = diff or -1, b = flw */
+    VFVM_OP_SCL_DOM_ADVECTION = 6,        /* OpSclDomAdvection{unk,wgt,diff,den,flw} opscl_dom_adv.rs:49  a = wgt, b = diff, c = flw (den = the flow's) */
+    VFVM_OP_SCL_DOM_TIME_PREVIOUS = 7,    /* OpSclDomTimePrevious{unk,wgt}        opscl_dom_timeprev.rs    a = wgt (its previous values) */
+    VFVM_OP_SCL_DOM_TIME_UNITY = 8,       /* OpSclDomTimeUnity{unk}               opscl_dom_timeunity.rs                         */
     /* boundary operators; geom_id = bnd_id */
     VFVM_OP_SCL_BND_DIRICHLET = 20,       /* OpSclBndDirichlet{unk,val}           opscl_bnd_dir.rs         a = sclbnd value      */
     VFVM_OP_SCL_BND_NEUMANN = 21,         /* OpSclBndNeumann{unk,flux}            opscl_bnd_neu.rs         a = sclbnd flux       */
@@ -203,6 +206,9 @@ enum vfvm_op_kind {
     VFVM_OP_SCL_BND_SOURCE = 24,          /* OpSclBndSource{unk,src}              opscl_bnd_src.rs         a = src               */
     VFVM_OP_SCL_BND_TIME = 25,            /* OpSclBndTime{unk,wgt}                opscl_bnd_time.rs        a = wgt               */
     VFVM_OP_SCL_BND_ADVECTION_UNITY = 26, /* OpSclBndAdvectionUnity{unk,diff?,vel,flw} opscl_bnd_advunity.rs a = diff or -1, b = flw */
+    VFVM_OP_SCL_BND_ADVECTION = 27,       /* OpSclBndAdvection{unk,wgt,diff,vel,den,flw} opscl_bnd_adv.rs a = wgt, b = diff, c = flw */
+    VFVM_OP_SCL_BND_TIME_PREVIOUS = 28,   /* OpSclBndTimePrevious{unk,wgt}        opscl_bnd_timeprev.rs    a = wgt                */
+    VFVM_OP_SCL_BND_TIME_UNITY = 29,      /* OpSclBndTimeUnity{unk}               opscl_bnd_timeunity.rs                         */
     /* flow operators (SteadyFlow, steady_flow.rs:133-240): unknown velocity vecdom `vel` and
      * pressure scldom `pres` of one flow (sclflw `flw`, whose den/visc/fce ids the operators
      * must name); domain operators geom_id = dom_id (0) */
@@ -215,6 +221,9 @@ enum vfvm_op_kind {
     VFVM_OP_SCL_DOM_DIVERGENCE = 46,      /* OpSclDomDivergence{vel,pres,den}     opscl_dom_div.rs:68     unk = pres, a = vel, b = den */
     VFVM_OP_SCL_DOM_PRESSURE_STABILIZER = 47, /* OpSclDomPressureStabilizer{pres,den,fce,flw} opscl_dom_pstab.rs:52 unk = pres, a = den, b = fce, c = flw */
     VFVM_OP_SCL_DOM_PRESSURE_REFERENCE = 48,  /* OpSclDomPressureReference{pres,nid,pres_ref} opscl_dom_pref.rs:48 unk = pres, a = nid, value = pres_ref */
+    VFVM_OP_VEC_DOM_TIME = 49,            /* OpVecDomTime{vel,den}                opvec_dom_time.rs:37    unk = vel, a = den; first
+                                             in the list: a TransientFlow system, whose divergence operators also
+                                             apply their time part (opscl_dom_div.rs:40-66, opscl_bnd_div.rs:40-70) */
     /* flow boundary operators; geom_id = bnd_id */
     VFVM_OP_VEC_BND_DIRICHLET = 60,       /* OpVecBndDirichlet{vel,val}           opvec_bnd_dir.rs:56     unk = vel, a = vecbnd value */
     VFVM_OP_SCL_BND_DIVERGENCE = 61,      /* OpSclBndDivergence{vel,pres,den}     opscl_bnd_div.rs:73     unk = pres, a = vel, b = den */
@@ -229,7 +238,9 @@ typedef struct vfvm_op {
 } vfvm_op;
 /* An operator list that holds flow operators describes the coupled velocity-pressure system
  * of SteadyFlow: unknown vector [u (N), v (N), w (N), p (N)] (steady_flow.rs:94-104), the
- * eight domain operators once each in the order of assemble_matrix (steady_flow.rs:244-254),
+ * eight domain operators once each in the order of assemble_matrix (steady_flow.rs:244-254)
+ * (TransientFlow: VEC_DOM_TIME before them, transient_flow.rs:250-260, solved by
+ * vfvm_transient_solve),
  * for every boundary the triple {VEC_BND_DIRICHLET, SCL_BND_DIVERGENCE,
  * SCL_BND_PRESSURE_STABILIZER} (steady_flow.rs:256-260), and at most one
  * SCL_DOM_PRESSURE_REFERENCE (applied last, steady_flow.rs:297-300).  The mesh needs its

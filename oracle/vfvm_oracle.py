@@ -372,9 +372,16 @@ class Op:
     SCL_BND_SOURCE = 24
     SCL_BND_TIME = 25
     SCL_BND_ADVECTION_UNITY = 26
+    SCL_DOM_ADVECTION = 6
+    SCL_DOM_TIME_PREVIOUS = 7
+    SCL_DOM_TIME_UNITY = 8
+    SCL_BND_ADVECTION = 27
+    SCL_BND_TIME_PREVIOUS = 28
+    SCL_BND_TIME_UNITY = 29
 
-    def __init__(self, kind, geom_id, unk_id, a=-1, b=-1):
+    def __init__(self, kind, geom_id, unk_id, a=-1, b=-1, c=-1, value=0.0):
         self.kind, self.geom_id, self.unk_id, self.a, self.b = kind, geom_id, unk_id, a, b
+        self.c, self.value = c, value
 
 
 def _diff_face(G, Gg, v, n, skew):
@@ -396,13 +403,14 @@ def _pow5(x):
 
 
 def assemble(g: dict, ops, scl: dict, sclbnd: dict, flux: dict, bnd_nid: list, dt: float = 0.0,
-             value_prev: dict | None = None):
+             value_prev: dict | None = None, flow_den: dict | None = None):
     """Triplets and b of assemble_matrix for the scalar operators, in push order.
 
     g: dual(); ops: list of Op in the caller's assemble_matrix order; scl: id ->
     (value[N], grad[3,N]); sclbnd: id -> values per boundary node; flux: flw id ->
-    F[E]; bnd_nid: list of node-id arrays per boundary.  The single unknown's block
-    starts at 0 (SteadyHeat::initial_start, steady_heat.rs:68-78).
+    F[E]; bnd_nid: list of node-id arrays per boundary; value_prev: field id -> previous
+    values; flow_den: flw id -> (rho[N], grad rho[3,N]) of the flow (OpSclDomAdvection).
+    The single unknown's block starts at 0 (SteadyHeat::initial_start, steady_heat.rs:68-78).
     Returns rows, cols, vals (push order), b.
     """
     N, NP = g["N"], g["num_pts"]
@@ -431,7 +439,7 @@ def assemble(g: dict, ops, scl: dict, sclbnd: dict, flux: dict, bnd_nid: list, d
 
     for op in ops:
         k = op.kind
-        if k in (Op.SCL_DOM_DIFFUSION, Op.SCL_DOM_ADVECTION_UNITY, Op.SCL_DOM_PRODUCTION, Op.SCL_DOM_SOURCE, Op.SCL_DOM_TIME):
+        if k < 20:
             vids = pts
             emask = edge_of_pts
         else:
@@ -460,6 +468,36 @@ def assemble(g: dict, ops, scl: dict, sclbnd: dict, flux: dict, bnd_nid: list, d
             pl = _pow5(np.maximum(1.0 - 0.1 * np.abs(pe), 0.0))
             coeff = cond * (pl - 1.0) + np.maximum(-F, 0.0)
             push_pairs(e, coeff)
+        elif k in (Op.SCL_DOM_ADVECTION, Op.SCL_BND_ADVECTION):
+            # opscl_dom_adv.rs:49-125, opscl_bnd_adv.rs: flux = m_f / rho_f * F
+            e = edges_of(emask)
+            v_, n_ = rowsE[e], ngh[e]
+            sx, sy, sz = g["skew"][:, e]
+            G, Gg = scl[op.b]
+            df = _diff_face(G, Gg, v_, n_, g["skew"][:, e])
+            W, Wg = scl[op.a]
+            Rr, Rg = flow_den[op.c]
+            wgt_f = 0.5 * (W[v_] + W[n_]) + 0.5 * (sx * (Wg[0][v_] + Wg[0][n_]) + sy * (Wg[1][v_] + Wg[1][n_]) + sz * (Wg[2][v_] + Wg[2][n_]))
+            den_f = 0.5 * (Rr[v_] + Rr[n_]) + 0.5 * (sx * (Rg[0][v_] + Rg[0][n_]) + sy * (Rg[1][v_] + Rg[1][n_]) + sz * (Rg[2][v_] + Rg[2][n_]))
+            cond = df * g["area"][e] / g["dist"][e]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                F = np.where(den_f != 0.0, wgt_f / den_f * flux[op.c][e], 0.0)
+                pe = np.where(cond != 0.0, F / cond, 0.0)
+            pl = _pow5(np.maximum(1.0 - 0.1 * np.abs(pe), 0.0))
+            push_pairs(e, cond * (pl - 1.0) + np.maximum(-F, 0.0))
+        elif k in (Op.SCL_DOM_TIME_PREVIOUS, Op.SCL_BND_TIME_PREVIOUS, Op.SCL_DOM_TIME_UNITY, Op.SCL_BND_TIME_UNITY):
+            # opscl_dom_timeprev.rs:36-52, opscl_dom_timeunity.rs:35-50 (apply_time)
+            if not dt > 0.0:
+                raise ValueError("OperatorUseApplyTime")
+            vol = g["vol"][vids]
+            prev = value_prev[op.unk_id]
+            if k in (Op.SCL_DOM_TIME_PREVIOUS, Op.SCL_BND_TIME_PREVIOUS):
+                Wp = value_prev[op.a]
+                R.append(vids); C.append(vids); V.append(-Wp[vids] * vol / dt)
+                b_add(vids, -Wp[vids] * prev[vids] * vol / dt)
+            else:
+                R.append(vids); C.append(vids); V.append(-vol / dt)
+                b_add(vids, -prev[vids] * vol / dt)
         elif k in (Op.SCL_DOM_PRODUCTION, Op.SCL_BND_PRODUCTION):
             # opscl_dom_prod.rs:39-76
             P, Pg = scl[op.a]
@@ -907,7 +945,7 @@ def update_flow(g: dict, gc, vel, velg, pres, den, deng, visc, viscg, fce, fceg)
     return rc, mom, flux
 
 
-def assemble_flow(g: dict, gc, fields: dict, bnd_vel: list, bnd_nid: list, pref=None):
+def assemble_flow(g: dict, gc, fields: dict, bnd_vel: list, bnd_nid: list, pref=None, dt=0.0, vel_prev=None, den_prev=None):
     """Triplets and b of SteadyFlow::assemble_matrix (src/steady/core/steady_flow.rs:244-303)
     in push order, unknowns [u, v, w, p] (steady_flow.rs:94-104).
 
@@ -915,6 +953,8 @@ def assemble_flow(g: dict, gc, fields: dict, bnd_vel: list, bnd_nid: list, pref=
     (from update_flow).  bnd_vel: list of (bnd_id, values (3, count)) in add_vel_bnd order,
     each boundary getting OpVecBndDirichlet, OpSclBndDivergence, OpSclBndPressureStabilizer.
     pref: (nid, p_ref) or None (OpSclDomPressureReference, applied last).
+    dt > 0: TransientFlow (transient_flow.rs:248-303): OpVecDomTime first (opvec_dom_time.rs:37-68,
+    previous velocity vel_prev) and the divergence operators' apply_time (den_prev).
     Operators: opvec_dom_adv.rs:47-116, opvec_dom_grad.rs:39-70, opvec_dom_diff.rs:39-94,
     opvec_dom_dtrn.rs:39-113, opvec_dom_ddil.rs:41-112, opvec_dom_src.rs:37-73,
     opscl_dom_div.rs:68-153, opscl_dom_pstab.rs:52-151, opvec_bnd_dir.rs:56-80,
@@ -941,6 +981,14 @@ def assemble_flow(g: dict, gc, fields: dict, bnd_vel: list, bnd_nid: list, pref=
     nx, ny, nz = norm[:, e_pts]
     A, d = area[e_pts], dist[e_pts]
     visc_f = _visc_face(visc, viscg, ev, en, sx, sy, sz)
+    transient = dt > 0.0
+    if transient:
+        # OpVecDomTime::apply_time: diag -rho V / dt, b -rho u_prev V / dt per component
+        vol_p = g["vol"][pts]
+        for c in range(3):
+            R.append(c * N + pts); C.append(c * N + pts); V.append(-den[pts] * vol_p / dt)
+        for c in range(3):
+            b[c * N + pts] += -den[pts] * vel_prev[c][pts] * vol_p / dt
 
     def push_vec_pairs(coeff):
         # per face, per component: (c v, c n) +coeff, (c v, c v) -coeff
@@ -1054,6 +1102,9 @@ def assemble_flow(g: dict, gc, fields: dict, bnd_vel: list, bnd_nid: list, pref=
         div_t.append(dt_); pst_t.append(pt_); bq.append((v, ba))
     for r_, c_, v_ in div_t:
         R.append(np.asarray(r_, np.int64)); C.append(np.asarray(c_, np.int64)); V.append(np.asarray(v_, np.float64))
+    if transient:
+        # OpSclDomDivergence::apply_time (opscl_dom_div.rs:40-66)
+        b[P + pts] += (den[pts] - den_prev[pts]) * g["vol"][pts] / dt
     for (r_, c_, v_), (v, ba) in zip(pst_t, bq):
         R.append(np.asarray(r_, np.int64)); C.append(np.asarray(c_, np.int64)); V.append(np.asarray(v_, np.float64))
         for t in ba:
@@ -1073,6 +1124,9 @@ def assemble_flow(g: dict, gc, fields: dict, bnd_vel: list, bnd_nid: list, pref=
             div_t.append(dt_); pst_t.append(pt_); bq.append((int(v), ba))
         for r_, c_, v_ in div_t:
             R.append(np.asarray(r_, np.int64)); C.append(np.asarray(c_, np.int64)); V.append(np.asarray(v_, np.float64))
+        if transient:
+            # OpSclBndDivergence::apply_time (opscl_bnd_div.rs:40-70)
+            b[P + nodes] += (den[nodes] - den_prev[nodes]) * g["vol"][nodes] / dt
         for (r_, c_, v_), (v, ba) in zip(pst_t, bq):
             R.append(np.asarray(r_, np.int64)); C.append(np.asarray(c_, np.int64)); V.append(np.asarray(v_, np.float64))
             for t in ba:
@@ -1119,6 +1173,47 @@ def scaled_residual_flow(N, x_old, x_new, den):
     rho = np.abs(den).max()
     r_p /= max(rng, rho * vmax * vmax, floor)
     return float(max(r_v, r_p))
+
+
+def transient_flow(g, gc, vel0, pres0, den, visc, fce, bnd_vel, bnd_nid, pref, dt, num_ts, max_iter, tol, damp):
+    """TransientBase::solve (src/transient/transient_base.rs:16-160) for TransientFlow with
+    constant density, viscosity and body force: implicit Euler, damped Picard iterations per
+    step with SuperLU solves, previous state updated after each step.  Returns vel, pres and
+    the Picard iterations of every step."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    N, NP = g["N"], g["num_pts"]
+    vel = np.array(vel0, np.float64).reshape(3, N).copy()
+    pres = np.array(pres0, np.float64).copy()
+    for bid, val in bnd_vel:
+        nodes = NP + np.asarray(bnd_nid[bid], np.int64)
+        vel[:, nodes] = np.asarray(val).reshape(3, -1)
+    vel_prev = vel.copy()
+
+    def system(vel, pres):
+        f = flow_fields(g, gc, vel, pres, den, visc, fce)
+        r, c, v, b = assemble_flow(g, gc, f, bnd_vel, bnd_nid, pref, dt, vel_prev, den)
+        rp, col, val = coo_to_csr(r, c, v, 4 * N)
+        return sp.csr_matrix((val, col, rp), shape=(4 * N, 4 * N)).tocsc(), b
+
+    iters = []
+    x_iter = np.concatenate([vel.ravel(), pres])
+    for ts in range(num_ts):
+        A, b = system(vel, pres)
+        for it in range(max_iter):
+            x_u = spla.spsolve(A, b)
+            x_d = (1.0 - damp) * x_iter + damp * x_u
+            vel, pres = x_d[:3 * N].reshape(3, N).copy(), x_d[3 * N:].copy()
+            A, b = system(vel, pres)
+            res = scaled_residual_flow(N, x_iter, x_d, den)
+            x_iter = x_d
+            if res < tol:
+                iters.append(it + 1)
+                break
+        else:
+            raise RuntimeError("TransientDidNotConverge")
+        vel_prev = vel.copy()
+    return vel, pres, iters
 
 
 def picard_flow(g, gc, vel0, pres0, den, visc, fce, bnd_vel, bnd_nid, pref, max_iter, tol, damp):

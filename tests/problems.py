@@ -76,6 +76,7 @@ class Problem:
         bnd_nid = [self.mesh.bnd_nodes(b) for b in range(6)]
         sb = {i: v for i, (b, v) in enumerate(self.sbnd)}
         ops = [O.Op(*o) for o in self.ops]
+        self.flow_den = {i: scl[den] for i, (vel, pres, den, visc, fce) in enumerate(self.flw)}
         return scl, sb, flux, bnd_nid, ops
 
     def oracle_system(self, g=None, value_prev=None, dt=0.0, unknown_value=None):
@@ -83,7 +84,7 @@ class Problem:
         if g is None:
             g = O.dual(self.pos, self.mesh.get("tets"), self.mesh.num_pts)
         scl, sb, flux, bnd_nid, ops = self.oracle_inputs(g, unknown_value)
-        r, c, v, b = O.assemble(g, ops, scl, sb, flux, bnd_nid, dt=dt, value_prev=value_prev)
+        r, c, v, b = O.assemble(g, ops, scl, sb, flux, bnd_nid, dt=dt, value_prev=value_prev, flow_den=self.flow_den)
         rp, col, val = O.coo_to_csr(r, c, v, self.N)
         return rp, col, val, b, g
 
@@ -200,7 +201,8 @@ class FlowProblem:
 
     def ops(self, ids):
         vel, pres, den, visc, fce, flw, vb = ids
-        o = [(vfvm.OP_VEC_DOM_ADVECTION, 0, vel, den, visc, flw), (vfvm.OP_VEC_DOM_GRADIENT, 0, vel, pres, -1),
+        o = [(vfvm.OP_VEC_DOM_TIME, 0, vel, den, -1)] if getattr(self, "transient", False) else []
+        o += [(vfvm.OP_VEC_DOM_ADVECTION, 0, vel, den, visc, flw), (vfvm.OP_VEC_DOM_GRADIENT, 0, vel, pres, -1),
              (vfvm.OP_VEC_DOM_DIFFUSION, 0, vel, visc, -1), (vfvm.OP_VEC_DOM_TRANSPOSE, 0, vel, visc, -1),
              (vfvm.OP_VEC_DOM_DILATATORY, 0, vel, visc, -1), (vfvm.OP_VEC_DOM_SOURCE, 0, vel, fce, -1),
              (vfvm.OP_SCL_DOM_DIVERGENCE, 0, pres, vel, den), (vfvm.OP_SCL_DOM_PRESSURE_STABILIZER, 0, pres, den, fce, flw)]
@@ -226,7 +228,7 @@ class FlowProblem:
         self.ids = (vel, pres, den, visc, fce, flw, vb)
         return self.ops(self.ids)
 
-    def oracle(self, g=None, gc=None):
+    def oracle(self, g=None, gc=None, dt=0.0, vel_prev=None, den_prev=None):
         """Oracle dual, grad_coeff, fields and the assembled CSR system at the current values."""
         if g is None:
             g = O.dual(self.mesh.get("pos"), self.mesh.get("tets"), self.mesh.num_pts)
@@ -235,7 +237,7 @@ class FlowProblem:
         f = O.flow_fields(g, gc, self.vel, self.pres, self.den, self.visc, self.fce, vel_const=False,
                           den_const="den" in self.const, visc_const="visc" in self.const, fce_const="fce" in self.const)
         bnd_nid = [self.mesh.bnd_nodes(b) for b in range(self.mesh.num_bnd)]
-        r, c, v, b = O.assemble_flow(g, gc, f, self.bnd_vel, bnd_nid, self.pref)
+        r, c, v, b = O.assemble_flow(g, gc, f, self.bnd_vel, bnd_nid, self.pref, dt, vel_prev, den_prev)
         rp, col, val = O.coo_to_csr(r, c, v, 4 * self.N)
         return g, gc, f, (rp, col, val, b)
 
@@ -272,3 +274,41 @@ def flow_general(mesh, seed=0):
         nodes = mesh.num_pts + mesh.bnd_nodes(b).astype(np.int64)
         bnd.append((b, vel[:, nodes] + 0.05 * rng.standard_normal((3, nodes.size))))
     return FlowProblem(mesh, vel, pres, den, visc, fce, bnd, pref=(3, 0.25), const=())
+
+
+def heat_flow_c5like(mesh, transient=False):
+    """Temperature carried by a prescribed flow (the heat part of SteadyHeatFlow /
+    TransientHeatFlow): OpSclDomAdvection with weight rho c_p, conductivity k and the flow's
+    density (all function fields), source, T = 1 at x = 0 and T = 0 at x = 1, zero flux on
+    the other faces; transient: OpSclDomTimePrevious (weight rho c_p) on the points and
+    OpSclBndTimeUnity on the flux boundaries."""
+    pos = mesh.get("pos")
+    x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]
+    p = Problem(mesh)
+    T = p.scldom(vfvm.FIELD_UNKNOWN, 0.0)
+    rcp = p.scldom(vfvm.FIELD_FUNCTION, 2.0 + 0.5 * np.sin(2.0 * x) * (1.0 + y))
+    k = p.scldom(vfvm.FIELD_FUNCTION, 0.05 * (1.0 + 0.3 * z + 0.2 * x * y))
+    q = p.scldom(vfvm.FIELD_FUNCTION, 0.3 * np.cos(3.0 * y))
+    den = p.scldom(vfvm.FIELD_FUNCTION, 1.0 + 0.4 * x * z)
+    visc = p.scldom(vfvm.FIELD_CONSTANT, 1.0)
+    pres = p.scldom(vfvm.FIELD_CONSTANT, 0.0)
+    vel = p.vecdom(vfvm.FIELD_FUNCTION, velocity_cellular(pos, 3.0))
+    fce = p.vecdom(vfvm.FIELD_CONSTANT, np.zeros((3, p.N)))
+    f = p.sclflw(vel, pres, den, visc, fce)
+    t0 = p.sclbnd(X0, 1.0)
+    t1 = p.sclbnd(X1, 0.0)
+    fl = {b: p.sclbnd(b, 0.0) for b in (Y0, Y1, Z0, Z1)}
+    ops = []
+    if transient:
+        ops.append((vfvm.OP_SCL_DOM_TIME_PREVIOUS, 0, T, rcp, -1))
+    ops += [(vfvm.OP_SCL_DOM_DIFFUSION, 0, T, k, T), (vfvm.OP_SCL_DOM_SOURCE, 0, T, q, -1),
+            (vfvm.OP_SCL_DOM_ADVECTION, 0, T, rcp, k, f),
+            (vfvm.OP_SCL_BND_DIRICHLET, X0, T, t0, -1), (vfvm.OP_SCL_BND_DIRICHLET, X1, T, t1, -1)]
+    for b in (Y0, Y1, Z0, Z1):
+        if transient:
+            ops.append((vfvm.OP_SCL_BND_TIME_UNITY, b, T, -1, -1))
+        ops += [(vfvm.OP_SCL_BND_DIFFUSION, b, T, k, T), (vfvm.OP_SCL_BND_SOURCE, b, T, q, -1),
+                (vfvm.OP_SCL_BND_NEUMANN, b, T, fl[b], -1), (vfvm.OP_SCL_BND_ADVECTION, b, T, rcp, k, f)]
+    p.ops = ops
+    p.unknown, p.rcp_id = T, rcp
+    return p

@@ -201,3 +201,35 @@ def test_scaled_residual_flow_closed_form():
     r2 = O.scaled_residual_flow(2, xo, xn2, np.ones(2))
     assert abs(r2 - max(np.sqrt(0.125) / 0.5, np.sqrt(((10.0 - 1e-10) ** 2 * 2) / 2) / 20.0)) < 1e-14
     assert np.isnan(O.scaled_residual_flow(2, xo, np.where(np.arange(8) == 5, np.inf, xn), np.ones(2)))
+
+
+def test_transient_flow_terms(mesh6, g6, gc6):
+    """TransientFlow adds -rho V / dt to the velocity diagonal of the points, -rho u_prev V / dt to
+    their b, and (rho - rho_prev) V / dt to every continuity row's b."""
+    import scipy.sparse as sp
+    p = flow_general(mesh6)
+    N, NP = mesh6.num_tot, mesh6.num_pts
+    f = O.flow_fields(g6, gc6, p.vel, p.pres, p.den, p.visc, p.fce, den_const=False, visc_const=False, fce_const=False)
+    bn = [mesh6.bnd_nodes(k) for k in range(6)]
+    dt = 0.05
+    vprev = 0.5 * p.vel + 0.1
+    dprev = p.den * 0.9
+    r0, c0, v0, b0 = O.assemble_flow(g6, gc6, f, p.bnd_vel, bn, None)
+    r1, c1, v1, b1 = O.assemble_flow(g6, gc6, f, p.bnd_vel, bn, None, dt, vprev, dprev)
+    A0 = sp.csr_matrix(sp.coo_matrix((v0, (r0, c0)), shape=(4 * N, 4 * N)))
+    A1 = sp.csr_matrix(sp.coo_matrix((v1, (r1, c1)), shape=(4 * N, 4 * N)))
+    D = (A1 - A0).tocoo()
+    vol = g6["vol"]
+    pts = np.arange(NP)
+    expect = np.zeros(4 * N)
+    for c in range(3):
+        expect[c * N + pts] = -p.den[pts] * vol[pts] / dt
+    dense_diag = np.zeros(4 * N)
+    np.add.at(dense_diag, D.row[D.row == D.col], D.data[D.row == D.col])
+    assert np.max(np.abs(D.data[D.row != D.col])) <= 1e-12 if np.any(D.row != D.col) else True
+    assert np.max(np.abs(dense_diag - expect)) <= 1e-12 * np.max(np.abs(expect))
+    db = b1 - b0
+    for c in range(3):
+        assert np.max(np.abs(db[c * N + pts] + p.den[pts] * vprev[c][pts] * vol[pts] / dt)) <= 1e-12
+        assert np.all(db[c * N + NP:(c + 1) * N] == 0.0)
+    assert np.max(np.abs(db[3 * N:] - (p.den - dprev) * vol / dt)) <= 1e-12

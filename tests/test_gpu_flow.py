@@ -135,3 +135,48 @@ def test_reference_api_mirror_steady_flow(ctx, lid_oracle):
     assert st.picard_iterations == its
     v = vars.value_vec(vel)
     assert np.max(np.abs(v - ovel)) <= 1e-8 * np.max(np.abs(ovel))
+
+
+def test_transient_flow_assembly_vs_oracle(ctx):
+    """TransientFlow system (velocity time operator, divergence time part) bitwise equal to the
+    oracle, with previous velocity and density different from the current ones."""
+    print("[transient flow assemble]", flush=True)
+    mesh = vfvm.Mesh.lattice(6, 4, 0.3)
+    p = flow_general(mesh)
+    p.transient = True
+    vprev, dprev = p.vel.copy(), p.den.copy()
+    ops = p.to_device(ctx)
+    vel_id, pres_id, den_id = p.ids[0], p.ids[1], p.ids[2]
+    p.vel = 0.7 * vprev + 0.05
+    p.den = 1.1 * dprev
+    ctx.vecdom_set(vel_id, p.vel)
+    ctx.scldom_set(den_id, p.den)
+    ctx.pattern_build(ops)
+    dt = 0.05
+    ctx.assemble(ops, dt)
+    rp, col = ctx.pattern_get()
+    vals, b = ctx.matrix_get()
+    _, _, _, (orp, ocol, oval, ob) = p.oracle(dt=dt, vel_prev=vprev, den_prev=dprev)
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    assert np.array_equal(vals, oval), _mismatch(vals, oval)
+    assert np.array_equal(b, ob), _mismatch(b, ob)
+
+
+def test_transient_lid_cavity_vs_oracle(ctx):
+    print("[transient lid cavity]", flush=True)
+    mesh = vfvm.Mesh.lattice(5, 0, 0.3)
+    p = lid_cavity(mesh, rho=1.0, mu=0.05, lid=1.0)
+    p.transient = True
+    ops = p.to_device(ctx)
+    dt, steps = 0.05, 3
+    st = ctx.transient_solve(ops, dt, steps, 30, 1e-7, 1.0, lin=dict(tol=1e-13, restart=100, max_outer=400,
+                                                                         precision=vfvm.FP64, precond=vfvm.PRECOND_BJACOBI))
+    vel = ctx.vecdom_get(p.ids[0])
+    pres = ctx.scldom_get(p.ids[1])
+    g = O.dual(mesh.get("pos"), mesh.get("tets"), mesh.num_pts)
+    gc = O.grad_coeff(g)
+    ovel, opres, iters = O.transient_flow(g, gc, p.vel, p.pres, p.den, p.visc, p.fce, p.bnd_vel,
+                                          [mesh.bnd_nodes(k) for k in range(6)], p.pref, dt, steps, 30, 1e-7, 1.0)
+    assert st.picard_iterations == sum(iters), (st.picard_iterations, iters)
+    assert np.max(np.abs(vel - ovel)) <= 1e-8 * np.max(np.abs(ovel))
+    assert np.max(np.abs(pres - opres)) <= 1e-8 * np.max(np.abs(opres))

@@ -247,6 +247,10 @@ int adjoint_check_ops(Ctx& c, const vfvm_op* ops, int32_t nops) {
     for (int32_t i = 0; i < nops; ++i) {
         const vfvm_op& op = ops[i];
         const std::string where = "adjoint: operator " + std::to_string(i) + ": ";
+        if (is_kind(op.kind, VFVM_OP_SCL_DOM_ADVECTION, VFVM_OP_SCL_BND_ADVECTION) ||
+            is_kind(op.kind, VFVM_OP_SCL_DOM_TIME_PREVIOUS, VFVM_OP_SCL_BND_TIME_PREVIOUS) ||
+            is_kind(op.kind, VFVM_OP_SCL_DOM_TIME_UNITY, VFVM_OP_SCL_BND_TIME_UNITY))
+            return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, where + "no adjoint (VJP) kernels for this operator kind");
         if (is_kind(op.kind, VFVM_OP_SCL_DOM_PRODUCTION, VFVM_OP_SCL_BND_PRODUCTION)) {
             // b += -P (grad T . cskew) V reads the unknown's gradient: only an identically zero
             // production (as the heat solvers install by default) keeps the map linear

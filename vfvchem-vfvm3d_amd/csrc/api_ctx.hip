@@ -134,13 +134,26 @@ int compile_ops(Ctx& c, const vfvm_op* ops, int32_t nops, Compiled& out) {
                 if (op.b < 0 || op.b >= (int32_t)c.flw.size()) return set_error(VFVM_ERR_INVALID_ARGUMENT, "advection: flow id out of range");
                 out.has_edges[cls] = 1;
                 break;
+            case VFVM_OP_SCL_DOM_ADVECTION:
+            case VFVM_OP_SCL_BND_ADVECTION:
+                if (!check_scl(c, op.a) || !check_scl(c, op.b)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "advection: weight or diffusion id out of range");
+                if (op.c < 0 || op.c >= (int32_t)c.flw.size()) return set_error(VFVM_ERR_INVALID_ARGUMENT, "advection: flow id out of range");
+                out.has_edges[cls] = 1;
+                break;
             case VFVM_OP_SCL_DOM_PRODUCTION:
             case VFVM_OP_SCL_BND_PRODUCTION:
             case VFVM_OP_SCL_DOM_TIME:
             case VFVM_OP_SCL_BND_TIME:
+            case VFVM_OP_SCL_DOM_TIME_PREVIOUS:
+            case VFVM_OP_SCL_BND_TIME_PREVIOUS:
                 if (!check_scl(c, op.a)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "coefficient id out of range");
                 out.has_diag[cls] = 1;
-                if (op.kind == VFVM_OP_SCL_DOM_TIME || op.kind == VFVM_OP_SCL_BND_TIME) out.has_time = true;
+                if (op.kind != VFVM_OP_SCL_DOM_PRODUCTION && op.kind != VFVM_OP_SCL_BND_PRODUCTION) out.has_time = true;
+                break;
+            case VFVM_OP_SCL_DOM_TIME_UNITY:
+            case VFVM_OP_SCL_BND_TIME_UNITY:
+                out.has_diag[cls] = 1;
+                out.has_time = true;
                 break;
             case VFVM_OP_SCL_DOM_SOURCE:
             case VFVM_OP_SCL_BND_SOURCE:
@@ -182,6 +195,22 @@ int build_program(Ctx& c, const Compiled& cp, DevProgram& p) {
                     d.has_diff = op.a >= 0;
                     if (op.a >= 0) { d.f_val = c.scl[op.a].value.p; d.f_grad = c.scl[op.a].grad.p; }
                     d.flux = c.flw[op.b].flux.p;
+                    break;
+                case VFVM_OP_SCL_DOM_ADVECTION:
+                case VFVM_OP_SCL_BND_ADVECTION: {
+                    const SclFlw& fl = c.flw[op.c];
+                    d.has_diff = 1;
+                    d.f_val = c.scl[op.b].value.p; d.f_grad = c.scl[op.b].grad.p;
+                    d.w_val = c.scl[op.a].value.p; d.w_grad = c.scl[op.a].grad.p;
+                    d.r_val = c.scl[fl.den].value.p; d.r_grad = c.scl[fl.den].grad.p;
+                    d.flux = fl.flux.p;
+                } break;
+                case VFVM_OP_SCL_DOM_TIME_PREVIOUS:
+                case VFVM_OP_SCL_BND_TIME_PREVIOUS:
+                    d.f_val = c.scl[op.a].value_prev.p;
+                    break;
+                case VFVM_OP_SCL_DOM_TIME_UNITY:
+                case VFVM_OP_SCL_BND_TIME_UNITY:
                     break;
                 default:
                     d.f_val = c.scl[op.a].value.p;
@@ -354,6 +383,8 @@ extern "C" int vfvm_vecdom_add(vfvm_ctx* ctx, int32_t kind, const double* cnst3,
     if (values) VFVM_HIP_TRY(hipMemcpyAsync(f.value.p, values, 3 * N * sizeof(double), hipMemcpyHostToDevice, c.stream));
     else
         for (int k = 0; k < 3; ++k) hipLaunchKernelGGL(k_fill, dim3(grid_for(N)), dim3(BLK), 0, c.stream, N, cnst3[k], f.value.p + k * N);
+    VFVM_HIP_TRY(f.value_prev.alloc(3 * N));
+    VFVM_HIP_TRY(hipMemcpyAsync(f.value_prev.p, f.value.p, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     if (int st = sync_err(c)) return st;
     *id = (int32_t)c.vec.size() - 1;
     return clear_error();
@@ -550,7 +581,7 @@ extern "C" int vfvm_pattern_get(vfvm_ctx* ctx, int32_t* row_ptr, int32_t* col) {
 // ------------------------------------------------------------------ assembly
 int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
     if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "assemble: no pattern built (call vfvm_pattern_build)");
-    if (c.flow || is_flow_list(ops, nops)) return flow_assemble(c, ops, nops);
+    if (c.flow || is_flow_list(ops, nops)) return flow_assemble_dt(c, ops, nops, dt);
     Compiled cp;
     if (int st = compile_ops(c, ops, nops, cp)) return st;
     if (cp.unk != c.unk) return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: operators act on another unknown than the pattern");
@@ -573,8 +604,15 @@ int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
             case VFVM_OP_SCL_BND_ADVECTION_UNITY:
                 if (op.a >= 0) need_s[op.a] = 1;
                 need_f[op.b] = 1; break;
+            case VFVM_OP_SCL_DOM_ADVECTION:
+            case VFVM_OP_SCL_BND_ADVECTION:
+                need_s[op.a] = 1; need_s[op.b] = 1; need_f[op.c] = 1; break;
             case VFVM_OP_SCL_BND_DIRICHLET:
             case VFVM_OP_SCL_BND_NEUMANN:
+            case VFVM_OP_SCL_DOM_TIME_PREVIOUS:
+            case VFVM_OP_SCL_BND_TIME_PREVIOUS:
+            case VFVM_OP_SCL_DOM_TIME_UNITY:
+            case VFVM_OP_SCL_BND_TIME_UNITY:
                 break;
             default:
                 need_s[op.a] = 1;
@@ -596,8 +634,16 @@ int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
         if (int st = launch_grad_vec(c, f.value.p, f.grad.p)) return st;
         f.grad_valid = f.kind == VFVM_FIELD_FUNCTION;
     }
-    for (size_t f = 0; f < c.flw.size(); ++f)
-        if (need_f[f]) if (int st = launch_flux_prescribed(c, c.flw[f])) return st;
+    // face mass flux: a prescribed flow's from its velocity, a solved flow's (unknown
+    // pressure) from the Rhie-Chow interpolation of its current fields
+    for (size_t f = 0; f < c.flw.size(); ++f) {
+        if (!need_f[f]) continue;
+        if (c.scl[c.flw[f].pres].kind == VFVM_FIELD_UNKNOWN) {
+            if (int st = flow_update(c, (int32_t)f)) return st;
+        } else if (int st = launch_flux_prescribed(c, c.flw[f])) {
+            return st;
+        }
+    }
     DevProgram p;
     build_program(c, cp, p);
     VFVM_HIP_TRY(hipMemcpyAsync(c.prog.p, &p, sizeof(p), hipMemcpyHostToDevice, c.stream));
@@ -803,11 +849,24 @@ extern "C" int vfvm_transient_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t n
     if (std::isnan(tol) || tol <= 0.0) return set_error(VFVM_ERR_SOLVER_NON_POSITIVE_TOLERANCE, "tol must be positive");
     if (std::isnan(damp) || damp <= 0.0 || damp > 1.0) return set_error(VFVM_ERR_SOLVER_INVALID_DAMPING, "damp must be in (0, 1]");
     if (!opts) return set_error(VFVM_ERR_INVALID_ARGUMENT, "opts is NULL");
-    if (is_flow_list(ops, nops)) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "transient flow (TransientFlow) is outside the built path");
     vfvm_driver_stats ds;
     std::memset(&ds, 0, sizeof(ds));
     if (int st = vfvm_pattern_build(ctx, ops, nops, nullptr)) return st;
     if (int st = initial_dirichlet(c, ops, nops)) return st;
+    if (c.flow) {
+        // TransientFlow (transient_flow.rs:248-303 with transient_base.rs:16-160): the initial
+        // state is the previous state of the first step; Picard per step, then _update_prev
+        if (!c.fs.transient) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "transient flow needs the velocity time operator (VEC_DOM_TIME) first");
+        if (int st = flow_update_prev(c)) return st;
+        int r = VFVM_OK;
+        for (int32_t ts = 0; ts < num_ts && r == VFVM_OK; ++ts) {
+            r = flow_picard(c, ops, nops, *opts, max_iter, tol, damp, ds, dt, ts);
+            if (r == VFVM_OK) r = flow_update_prev(c);
+        }
+        if (stats) *stats = ds;
+        if (r != VFVM_OK) return r;
+        return clear_error();
+    }
     // the initial state is the previous state of the first step (transient_base.rs:59-62)
     SclDom& u = c.scl[c.unk];
     VFVM_HIP_TRY(hipMemcpyAsync(u.value_prev.p, u.value.p, c.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));

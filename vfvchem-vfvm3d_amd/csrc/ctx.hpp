@@ -77,6 +77,7 @@ struct VecDom {
     int kind = 0;
     bool grad_valid = false;
     DevBuf<double> value, grad;  // value 3N SoA, grad 9N SoA (u_x,u_y,u_z,v_x,...)
+    DevBuf<double> value_prev;   // 3N: values of the previous time step
 };
 struct SclBnd {
     int32_t bnd = 0;
@@ -102,6 +103,10 @@ struct DevOp {
     const double* u_grad;  // unknown gradient (production)
     const double* u_prev;  // unknown previous value (time)
     const double* flux;    // face mass flux (advection)
+    const double* w_val;   // weight field value and gradient (advection: m)
+    const double* w_grad;
+    const double* r_val;   // density value and gradient (advection: rho)
+    const double* r_grad;
 };
 constexpr int MAX_CLASSES = 16;  // class 0 = points, class 1 + b = boundary b
 constexpr int MAX_OPS = 8;
@@ -165,6 +170,8 @@ struct FlowSys {
     int32_t gmax = 0, rmax = 0;     // largest grad_coeff list and 3-ring of a continuity row
     int32_t lpr = 16;               // lanes per momentum row
     int64_t spmv_idx = 0;           // column indices the structured SpMV reads per product
+    bool transient = false;         // TransientFlow list (velocity time operator first)
+    double dt = 0.0;
 };
 
 // Block-Jacobi inverse blocks (b x b per vertex, entry (i, j) of vertex v at (i b + j) N + v).
@@ -270,7 +277,9 @@ int flow_assemble(Ctx& c, const vfvm_op* ops, int32_t nops);
 int flow_update(Ctx& c, int32_t flw);
 int flow_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops);
 int flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts& o, int32_t max_iter, double tol,
-                double damp, vfvm_driver_stats& ds);
+                double damp, vfvm_driver_stats& ds, double dt = 0.0, int ts = 0);
+int flow_assemble_dt(Ctx& c, const vfvm_op* ops, int32_t nops, double dt);
+int flow_update_prev(Ctx& c);
 
 // drivers and adjoint set-up shared across translation units (api_ctx.hip)
 int assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt);

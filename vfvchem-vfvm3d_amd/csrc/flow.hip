@@ -73,6 +73,8 @@ struct FlowFields {
     double* rc;           // N
     double* mom;          // 3N
     double* flux;         // E
+    const double* vel_prev;  // 3N (transient)
+    const double* den_prev;  // N (transient)
 };
 
 struct FlowAsm {
@@ -87,6 +89,8 @@ struct FlowAsm {
     int64_t pref_row;  // row of the reference pressure (-1: none)
     double pref_val;
     int32_t gm, rm;  // LDS capacities of a continuity row
+    int32_t transient;
+    double dt;
 };
 
 // skew-corrected arithmetic mean of a scalar at the face of edge (v, n)
@@ -259,9 +263,11 @@ __global__ void __launch_bounds__(BLK) k_flow_momentum_rows(FlowAsm a) {
         sd[h][1] = coeff * ny;
         sd[h][2] = coeff * nz;
     }
-    // sequential sums in face order: the diagonal (advection pushes, then diffusion pushes)
-    // and the per-vertex sums of the transpose and dilatatory operators
+    // sequential sums in face order: the diagonal (the time operator's push first in a
+    // transient list, then advection pushes, then diffusion pushes) and the per-vertex sums of
+    // the transpose and dilatatory operators
     double dacc = 0.0;
+    if (a.transient) dacc += -f.den[v] * g.vol[v] / a.dt;  // opvec_dom_time.rs:51-56
     for (int k = 0; k < deg; ++k) {
         const double t = __shfl(k < LPR ? ca[0] : ca[1], k % LPR, LPR);
         dacc += -t;
@@ -300,6 +306,7 @@ __global__ void __launch_bounds__(BLK) k_flow_momentum_rows(FlowAsm a) {
         const double vol = g.vol[v];
         for (int c = 0; c < 3; ++c) {
             double bb = 0.0;
+            if (a.transient) bb += -f.den[v] * f.vel_prev[c * N + v] * vol / a.dt;
             bb += -sumt[c];
             bb += -sumd[c];
             bb += -fc[c] * vol;
@@ -383,6 +390,8 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
     double fcv[3];
     fce_centroid(g, f, v, fcv);
     double bacc = 0.0;
+    // divergence apply_time, before the stabiliser's pushes (opscl_dom_div.rs:40-66)
+    if (a.transient) bacc += (den_v - f.den_prev[v]) * vol_v / a.dt;
     for (int32_t e = es; e < ee; ++e) {
         const int64_t n = g.ngh[e];
         const double face = g.area[e], dist = g.dist[e];
@@ -614,7 +623,13 @@ int flow_compile(Ctx& c, const vfvm_op* ops, int32_t nops, FlowSys& fs) {
             bnd[op.geom_id].push_back(op);
         }
     }
-    if (dom.size() != 8) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "flow: the eight domain operators of SteadyFlow are needed, once each");
+    fs.transient = !dom.empty() && dom[0].kind == VFVM_OP_VEC_DOM_TIME;
+    vfvm_op time_op{};
+    if (fs.transient) {
+        time_op = dom[0];
+        dom.erase(dom.begin());
+    }
+    if (dom.size() != 8) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "flow: the eight domain operators of SteadyFlow are needed, once each (TransientFlow: the velocity time operator first)");
     for (int k = 0; k < 8; ++k)
         if (dom[k].kind != dom_order[k])
             return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "flow: domain operators must come in SteadyFlow::assemble_matrix order (advection, gradient, diffusion, transpose, dilatatory, source, divergence, pressure stabilizer)");
@@ -636,6 +651,8 @@ int flow_compile(Ctx& c, const vfvm_op* ops, int32_t nops, FlowSys& fs) {
     if (int st = same(dom[5].a == fs.fce)) return st;
     if (int st = same(dom[6].unk_id == fs.pres && dom[6].a == fs.vel && dom[6].b == fs.den)) return st;
     if (int st = same(dom[7].unk_id == fs.pres && dom[7].a == fs.den && dom[7].b == fs.fce && dom[7].c == fs.flw)) return st;
+    if (fs.transient)
+        if (int st = same(time_op.unk_id == fs.vel && time_op.a == fs.den)) return st;
     fs.vbnd_val.assign(nb, -1);
     for (int b = 0; b < nb; ++b) {
         const bool has_nodes = g.bnd_ptr[b + 1] > g.bnd_ptr[b];
@@ -677,7 +694,8 @@ FlowAsm make_args(Ctx& c) {
     SclFlw& fl = c.flw[fs.flw];
     a.f = FlowFields{c.vec[fs.vel].value.p, c.vec[fs.vel].grad.p, c.scl[fs.pres].value.p, c.scl[fs.den].value.p,
                      c.scl[fs.den].grad.p, c.scl[fs.visc].value.p, c.scl[fs.visc].grad.p, c.vec[fs.fce].value.p,
-                     c.vec[fs.fce].grad.p, fl.rc.p, fl.mom.p, fl.flux.p};
+                     c.vec[fs.fce].grad.p, fl.rc.p, fl.mom.p, fl.flux.p, c.vec[fs.vel].value_prev.p,
+                     c.scl[fs.den].value_prev.p};
     a.mrow = c.mrow.p;
     a.mcol = c.mcol.p;
     a.vals = c.vals.p;
@@ -691,6 +709,8 @@ FlowAsm make_args(Ctx& c) {
     a.pref_val = fs.pref_val;
     a.gm = fs.gmax;
     a.rm = fs.rmax;
+    a.transient = fs.transient ? 1 : 0;
+    a.dt = fs.dt;
     return a;
 }
 
@@ -922,14 +942,19 @@ int vfvm::flow_update(Ctx& c, int32_t flw) {
     return sync_err(c);
 }
 
-int vfvm::flow_assemble(Ctx& c, const vfvm_op* ops, int32_t nops) {
+int vfvm::flow_assemble(Ctx& c, const vfvm_op* ops, int32_t nops) { return flow_assemble_dt(c, ops, nops, 0.0); }
+
+int vfvm::flow_assemble_dt(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
     if (!c.has_pattern || !c.flow) return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: the built pattern is not a flow pattern");
     FlowSys fs;
     if (int st = flow_compile(c, ops, nops, fs)) return st;
     if (fs.vel != c.fs.vel || fs.pres != c.fs.pres || fs.flw != c.fs.flw || fs.pref_row != c.fs.pref_row)
         return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: the flow operators do not fit the built pattern");
+    if (fs.transient && !(dt > 0.0)) return set_error(VFVM_ERR_OPERATOR_USE_APPLY_TIME, "the velocity time operator needs dt > 0 (OperatorUseApplyTime)");
     c.fs.vbnd_val = fs.vbnd_val;
     c.fs.pref_val = fs.pref_val;
+    c.fs.transient = fs.transient;
+    c.fs.dt = fs.transient ? dt : 0.0;
     if (int st = flow_assemble_impl(c)) return st;
     c.has_matrix = true;
     return VFVM_OK;
@@ -957,7 +982,8 @@ int vfvm::flow_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
 // damping; after each solve the Rhie-Chow flow data and the matrix are rebuilt at the damped
 // iterate, and the scaled residual (max over the velocity and pressure fields) is tested.
 int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts& o, int32_t max_iter, double tol,
-                      double damp, vfvm_driver_stats& ds) {
+                      double damp, vfvm_driver_stats& ds, double dt, int ts) {
+    const bool transient = dt > 0.0;
     const int64_t N = c.g.N, n = 4 * N;
     double* vel = c.vec[c.fs.vel].value.p;
     double* pres = c.scl[c.fs.pres].value.p;
@@ -970,7 +996,7 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
     std::vector<double> h(7 * (size_t)nb);
     auto t0 = clk::now();
     hipLaunchKernelGGL(k_flow_pack, dim3(grid_for(n)), dim3(BLK), 0, c.stream, N, (const double*)vel, (const double*)pres, xi.p);
-    if (int st = flow_assemble(c, ops, nops)) return st;
+    if (int st = flow_assemble_dt(c, ops, nops, dt)) return st;
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     ds.time_assembly += secs(t0, clk::now());
     const double* den = c.scl[c.fs.den].value.p;
@@ -984,7 +1010,7 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
         ds.time_solve += secs(ti0, ti1);
         hipLaunchKernelGGL(k_flow_damp, dim3(grid_for(n)), dim3(BLK), 0, c.stream, N, damp, (const double*)xi.p,
                            (const double*)xu.p, xd.p, vel, pres);
-        if (int r = flow_assemble(c, ops, nops)) return r;
+        if (int r = flow_assemble_dt(c, ops, nops, dt)) return r;
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         ds.time_assembly += secs(ti1, clk::now());
         // scaled residual (scaling.rs:113-135)
@@ -1022,11 +1048,22 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
         ds.residual = res;
         ds.picard_iterations += 1;
         if (!std::isfinite(res))
-            return set_error(VFVM_ERR_STEADY_NON_FINITE_RESIDUAL, "non-finite Picard residual at iteration " + std::to_string(iter));
+            return set_error(transient ? VFVM_ERR_TRANSIENT_NON_FINITE_RESIDUAL : VFVM_ERR_STEADY_NON_FINITE_RESIDUAL,
+                             "non-finite Picard residual at iteration " + std::to_string(iter) + (transient ? " of step " + std::to_string(ts) : ""));
         std::swap(xi, xd);
         if (res < tol) return VFVM_OK;
     }
     char buf[160];
     snprintf(buf, sizeof(buf), "Picard iteration did not converge in %d iterations (residual %.3e)", max_iter, ds.residual);
-    return set_error(VFVM_ERR_STEADY_DID_NOT_CONVERGE, buf);
+    return set_error(transient ? VFVM_ERR_TRANSIENT_DID_NOT_CONVERGE : VFVM_ERR_STEADY_DID_NOT_CONVERGE, buf);
+}
+
+// _update_prev (transient_base.rs:63, :110): every field's previous values := current values
+int vfvm::flow_update_prev(Ctx& c) {
+    const int64_t N = c.g.N;
+    for (auto& f : c.scl)
+        VFVM_HIP_TRY(hipMemcpyAsync(f.value_prev.p, f.value.p, N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    for (auto& f : c.vec)
+        VFVM_HIP_TRY(hipMemcpyAsync(f.value_prev.p, f.value.p, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    return sync_err(c);
 }

@@ -157,10 +157,21 @@ __device__ inline double edge_coeff(const DevOp& op, const AsmArgs& a, int64_t v
         double df = face_diff(op.f_val, op.f_grad, N, v, n, a.skew[e], a.skew[E + e], a.skew[2 * E + e]);
         return df * a.area[e] / a.dist[e];
     }
-    // opscl_dom_advunity.rs:70-112: power law A = max(0, 1 - 0.1|Pe|)^5
-    double df = op.has_diff ? face_diff(op.f_val, op.f_grad, N, v, n, a.skew[e], a.skew[E + e], a.skew[2 * E + e]) : 0.0;
+    // opscl_dom_advunity.rs:70-112: power law A = max(0, 1 - 0.1|Pe|)^5; opscl_dom_adv.rs:86-118
+    // converts the mass flux with the skew-corrected face means of m and rho
+    const double sx = a.skew[e], sy = a.skew[E + e], sz = a.skew[2 * E + e];
+    double df = op.has_diff ? face_diff(op.f_val, op.f_grad, N, v, n, sx, sy, sz) : 0.0;
     double cond = df * a.area[e] / a.dist[e];
     double flux = op.flux[e];
+    if (op.kind == VFVM_OP_SCL_DOM_ADVECTION || op.kind == VFVM_OP_SCL_BND_ADVECTION) {
+        const double* W = op.w_val;
+        const double* Wg = op.w_grad;
+        const double* R = op.r_val;
+        const double* Rg = op.r_grad;
+        const double wgt_f = 0.5 * (W[v] + W[n]) + 0.5 * (sx * (Wg[v] + Wg[n]) + sy * (Wg[N + v] + Wg[N + n]) + sz * (Wg[2 * N + v] + Wg[2 * N + n]));
+        const double den_f = 0.5 * (R[v] + R[n]) + 0.5 * (sx * (Rg[v] + Rg[n]) + sy * (Rg[N + v] + Rg[N + n]) + sz * (Rg[2 * N + v] + Rg[2 * N + n]));
+        flux = den_f != 0.0 ? wgt_f / den_f * flux : 0.0;
+    }
     double pe = cond != 0.0 ? flux / cond : 0.0;
     double base = fmax(1.0 - 0.1 * fabs(pe), 0.0);
     double b2 = base * base;
@@ -170,7 +181,7 @@ __device__ inline double edge_coeff(const DevOp& op, const AsmArgs& a, int64_t v
 
 __device__ inline bool is_edge_op(int k) {
     return k == VFVM_OP_SCL_DOM_DIFFUSION || k == VFVM_OP_SCL_BND_DIFFUSION || k == VFVM_OP_SCL_DOM_ADVECTION_UNITY ||
-           k == VFVM_OP_SCL_BND_ADVECTION_UNITY;
+           k == VFVM_OP_SCL_BND_ADVECTION_UNITY || k == VFVM_OP_SCL_DOM_ADVECTION || k == VFVM_OP_SCL_BND_ADVECTION;
 }
 
 // Diagonal and right-hand-side contributions of a per-vertex operator.
@@ -202,6 +213,21 @@ __device__ inline void vertex_op(const DevOp& op, const AsmArgs& a, int64_t v, i
             const double w = op.f_val[v];
             dacc += -w * vol / a.dt;
             bacc += -w * op.u_prev[v] * vol / a.dt;
+        } break;
+        case VFVM_OP_SCL_DOM_TIME_PREVIOUS:
+        case VFVM_OP_SCL_BND_TIME_PREVIOUS: {
+            // opscl_dom_timeprev.rs:36-52: the weight's previous values
+            const double vol = a.vol[v];
+            const double w = op.f_val[v];
+            dacc += -w * vol / a.dt;
+            bacc += -w * op.u_prev[v] * vol / a.dt;
+        } break;
+        case VFVM_OP_SCL_DOM_TIME_UNITY:
+        case VFVM_OP_SCL_BND_TIME_UNITY: {
+            // opscl_dom_timeunity.rs:35-50
+            const double vol = a.vol[v];
+            dacc += -vol / a.dt;
+            bacc += -op.u_prev[v] * vol / a.dt;
         } break;
         case VFVM_OP_SCL_BND_DIRICHLET: {
             // opscl_bnd_dir.rs:65-75

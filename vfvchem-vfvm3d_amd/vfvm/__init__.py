@@ -10,6 +10,8 @@ from .core import (FIELD_CONSTANT, FIELD_FUNCTION, FIELD_UNKNOWN, FP32_REFINED, 
 from .core import (OP_SCL_BND_ADVECTION_UNITY, OP_SCL_BND_DIFFUSION, OP_SCL_BND_DIRICHLET, OP_SCL_BND_NEUMANN,  # noqa: F401
                    OP_SCL_BND_PRODUCTION, OP_SCL_BND_SOURCE, OP_SCL_BND_TIME, OP_SCL_DOM_ADVECTION_UNITY,
                    OP_SCL_DOM_DIFFUSION, OP_SCL_DOM_PRODUCTION, OP_SCL_DOM_SOURCE, OP_SCL_DOM_TIME)
+from .core import (OP_SCL_BND_ADVECTION, OP_SCL_BND_TIME_PREVIOUS, OP_SCL_BND_TIME_UNITY, OP_SCL_DOM_ADVECTION,  # noqa: F401
+                   OP_SCL_DOM_TIME_PREVIOUS, OP_SCL_DOM_TIME_UNITY, OP_VEC_DOM_TIME)
 from .core import (FLW_FLUX, FLW_MOM, FLW_RC, OP_SCL_BND_DIVERGENCE, OP_SCL_BND_PRESSURE_STABILIZER,  # noqa: F401
                    OP_SCL_DOM_DIVERGENCE, OP_SCL_DOM_PRESSURE_REFERENCE, OP_SCL_DOM_PRESSURE_STABILIZER, OP_VEC_BND_DIRICHLET,
                    OP_VEC_DOM_ADVECTION, OP_VEC_DOM_DIFFUSION, OP_VEC_DOM_DILATATORY, OP_VEC_DOM_GRADIENT, OP_VEC_DOM_SOURCE,

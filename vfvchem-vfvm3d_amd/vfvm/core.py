@@ -21,6 +21,12 @@ OP_SCL_BND_PRODUCTION = 23
 OP_SCL_BND_SOURCE = 24
 OP_SCL_BND_TIME = 25
 OP_SCL_BND_ADVECTION_UNITY = 26
+OP_SCL_DOM_ADVECTION = 6
+OP_SCL_DOM_TIME_PREVIOUS = 7
+OP_SCL_DOM_TIME_UNITY = 8
+OP_SCL_BND_ADVECTION = 27
+OP_SCL_BND_TIME_PREVIOUS = 28
+OP_SCL_BND_TIME_UNITY = 29
 OP_VEC_DOM_ADVECTION = 40
 OP_VEC_DOM_GRADIENT = 41
 OP_VEC_DOM_DIFFUSION = 42
@@ -30,6 +36,7 @@ OP_VEC_DOM_SOURCE = 45
 OP_SCL_DOM_DIVERGENCE = 46
 OP_SCL_DOM_PRESSURE_STABILIZER = 47
 OP_SCL_DOM_PRESSURE_REFERENCE = 48
+OP_VEC_DOM_TIME = 49
 OP_VEC_BND_DIRICHLET = 60
 OP_SCL_BND_DIVERGENCE = 61
 OP_SCL_BND_PRESSURE_STABILIZER = 62
@@ -338,6 +345,7 @@ class Context:
         arr = make_ops(ops)
         st = L.DriverStats()
         o = solve_opts(**(lin or {}))
+        self.n = (4 if any(int(op[0]) >= OP_VEC_DOM_ADVECTION for op in ops) else 1) * self.mesh.num_tot
         check(lib.vfvm_transient_solve(self._h, arr, len(ops), C.byref(o), float(dt), int(num_ts), int(max_iter), float(tol),
                                        float(damp), C.byref(st)))
         return st
