@@ -288,6 +288,8 @@ def main():
     ap.add_argument("--flow-restart", type=int, default=100)
     ap.add_argument("--flow-steps", type=int, default=2)
     ap.add_argument("--no-flow", action="store_true")
+    ap.add_argument("--flow-only", action="store_true",
+                    help="run only the lid-driven cavity line and print it (profiling aid; not the headline line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first copy MB megabytes device-to-device 5 times (a plain copy kernel of known size, "
                          "used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -303,6 +305,9 @@ def main():
     import torch
     import vfvm
 
+    if args.flow_only:
+        print(json.dumps({"flow": run_flow(vfvm, args)}))
+        return
     if args.copy_calibration > 0:
         src = torch.ones(args.copy_calibration * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
