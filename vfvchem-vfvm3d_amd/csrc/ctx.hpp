@@ -136,6 +136,9 @@ struct SysView {
     const std::vector<int32_t>* h_row = nullptr;
     const std::vector<int32_t>* h_col = nullptr;
     IluData* ilu = nullptr;
+    // coupled flow system: 0 generic CSR, 1 structured rows in the comp-major order,
+    // 2 structured rows with the vectors vertex-interleaved (entry (c, v) at 4 v + c)
+    int flow_layout = 0;
 };
 
 // Checkpoint tape of the adjoint forward pass: the operator list, step data and the

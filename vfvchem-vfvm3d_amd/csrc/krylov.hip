@@ -118,7 +118,72 @@ __device__ inline T wave_sum(T s) {
     for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
     return s;
 }
-template <class T>
+// position of unknown (component c, vertex m): comp-major c N + m, or vertex-interleaved 4 m + c
+template <bool IL>
+__device__ inline int64_t fidx(int64_t N, int c, int64_t m) { return IL ? 4 * m + c : c * N + m; }
+
+// the four row sums of vertex v (lane partials, not yet reduced)
+template <class T, bool IL>
+__device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                 const T* __restrict__ val, const T* __restrict__ x, int64_t v, int lane, T& su, T& sv,
+                                 T& sw, T& sq) {
+    const int64_t N = fg.N;
+    const int32_t pN3 = (int32_t)(3 * N);
+    const int32_t ru = rp[v], rv = rp[N + v], rw = rp[2 * N + v], rq = rp[3 * N + v];
+    const int32_t lu = rp[v + 1] - ru, lq = rp[3 * N + v + 1] - rq;
+    su = 0; sv = 0; sw = 0; sq = 0;
+    const int32_t G = fg.gc_ptr[v + 1] - fg.gc_ptr[v];
+    if (lu > 1) {
+        // own-component part of the velocity rows: {v} u ngh(v)
+        const int32_t d1 = fg.rp[v + 1] - fg.rp[v] + 1;
+        for (int32_t k = lane; k < d1; k += WAVE) {
+            const int32_t m = col[ru + k];
+            su += val[ru + k] * x[fidx<IL>(N, 0, m)];
+            sv += val[rv + k] * x[fidx<IL>(N, 1, m)];
+            sw += val[rw + k] * x[fidx<IL>(N, 2, m)];
+        }
+        // p part of the velocity rows and velocity part of the continuity row: grad_coeff(v)
+        for (int32_t q = lane; q < G; q += WAVE) {
+            const int32_t m = col[ru + d1 + q] - pN3;
+            const T pm = x[fidx<IL>(N, 3, m)];
+            su += val[ru + d1 + q] * pm;
+            sv += val[rv + d1 + q] * pm;
+            sw += val[rw + d1 + q] * pm;
+            if (lq > 1)
+                sq += val[rq + q] * x[fidx<IL>(N, 0, m)] + val[rq + G + q] * x[fidx<IL>(N, 1, m)] +
+                      val[rq + 2 * G + q] * x[fidx<IL>(N, 2, m)];
+        }
+    } else {
+        if (lane == 0) {
+            su = val[ru] * x[fidx<IL>(N, 0, v)];
+            sv = val[rv] * x[fidx<IL>(N, 1, v)];
+            sw = val[rw] * x[fidx<IL>(N, 2, v)];
+        }
+        if (lq > 1)
+            for (int32_t q = lane; q < G; q += WAVE) {
+                const int32_t m = col[rq + q];
+                sq += val[rq + q] * x[fidx<IL>(N, 0, m)] + val[rq + G + q] * x[fidx<IL>(N, 1, m)] +
+                      val[rq + 2 * G + q] * x[fidx<IL>(N, 2, m)];
+            }
+    }
+    if (lq > 1) {
+        for (int32_t i = 3 * G + lane; i < lq; i += WAVE) sq += val[rq + i] * x[fidx<IL>(N, 3, col[rq + i] - pN3)];
+    } else if (lane == 0) {
+        sq = val[rq] * x[fidx<IL>(N, 3, v)];
+    }
+}
+
+// SpMV of the coupled flow system [u, v, w, p], one wave per vertex v computing its four rows
+// together.  The three velocity rows of a point share their column sets ({v} u ngh(v) in
+// their own component, grad_coeff(v) in p) and the continuity row reads the velocity
+// components at grad_coeff(v) and p on the 3-ring, so one pass over grad_coeff(v) gathers
+// p[m] for the three velocity rows and u[m], v[m], w[m] for the continuity row: the column
+// indices are read once per vertex instead of once per entry (the u row's own block, its p
+// block and the continuity row's 3-ring), and every gathered x entry feeds up to three
+// products.  With IL the vectors are vertex-interleaved, so the four unknowns of a gathered
+// vertex share one 32-byte (FP64) segment.  Rows of length 1 (velocity Dirichlet rows, the
+// reference-pressure row) are diagonal.
+template <class T, bool IL>
 __global__ void __launch_bounds__(BLK) k_spmv_flow(FlowSpmvGeom fg, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                    const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
                                                    const double* __restrict__ scale, const int* __restrict__ ctrl) {
@@ -127,70 +192,62 @@ __global__ void __launch_bounds__(BLK) k_spmv_flow(FlowSpmvGeom fg, const int32_
     const int64_t v = ((int64_t)xcd_block() * BLK + threadIdx.x) / WAVE;
     const int lane = threadIdx.x % WAVE;
     if (v >= N) return;
-    const T sc = scale ? (T)scale[0] : (T)1;
-    const T* x0 = x;
-    const T* x1 = x + N;
-    const T* x2 = x + 2 * N;
-    const T* xp = x + 3 * N;
-    const int32_t pN3 = (int32_t)(3 * N);
-    const int32_t ru = rp[v], rv = rp[N + v], rw = rp[2 * N + v], rq = rp[3 * N + v];
-    const int32_t lu = rp[v + 1] - ru, lq = rp[3 * N + v + 1] - rq;
-    T su = 0, sv = 0, sw = 0, sq = 0;
-    const int32_t G = fg.gc_ptr[v + 1] - fg.gc_ptr[v];
-    if (lu > 1) {
-        // own-component part of the velocity rows: {v} u ngh(v)
-        const int32_t d1 = fg.rp[v + 1] - fg.rp[v] + 1;
-        for (int32_t k = lane; k < d1; k += WAVE) {
-            const int32_t m = col[ru + k];
-            su += val[ru + k] * x0[m];
-            sv += val[rv + k] * x1[m];
-            sw += val[rw + k] * x2[m];
-        }
-        // p part of the velocity rows and velocity part of the continuity row: grad_coeff(v)
-        if (lq > 1) {
-            for (int32_t q = lane; q < G; q += WAVE) {
-                const int32_t m = col[ru + d1 + q] - pN3;
-                const T pm = xp[m];
-                su += val[ru + d1 + q] * pm;
-                sv += val[rv + d1 + q] * pm;
-                sw += val[rw + d1 + q] * pm;
-                sq += val[rq + q] * x0[m] + val[rq + G + q] * x1[m] + val[rq + 2 * G + q] * x2[m];
-            }
-        } else {
-            for (int32_t q = lane; q < G; q += WAVE) {
-                const T pm = xp[col[ru + d1 + q] - pN3];
-                su += val[ru + d1 + q] * pm;
-                sv += val[rv + d1 + q] * pm;
-                sw += val[rw + d1 + q] * pm;
-            }
-        }
-    } else {
-        if (lane == 0) {
-            su = val[ru] * x[col[ru]];
-            sv = val[rv] * x[col[rv]];
-            sw = val[rw] * x[col[rw]];
-        }
-        if (lq > 1)
-            for (int32_t q = lane; q < G; q += WAVE) {
-                const int32_t m = col[rq + q];
-                sq += val[rq + q] * x0[m] + val[rq + G + q] * x1[m] + val[rq + 2 * G + q] * x2[m];
-            }
-    }
-    if (lq > 1) {
-        for (int32_t i = 3 * G + lane; i < lq; i += WAVE) sq += val[rq + i] * xp[col[rq + i] - pN3];
-    } else if (lane == 0) {
-        sq = val[rq] * x[col[rq]];
-    }
+    T su, sv, sw, sq;
+    flow_rows<T, IL>(fg, rp, col, val, x, v, lane, su, sv, sw, sq);
     su = wave_sum(su);
     sv = wave_sum(sv);
     sw = wave_sum(sw);
     sq = wave_sum(sq);
     if (lane == 0) {
-        y[v] = su * sc;
-        y[N + v] = sv * sc;
-        y[2 * N + v] = sw * sc;
-        y[3 * N + v] = sq * sc;
+        const T sc = scale ? (T)scale[0] : (T)1;
+        y[fidx<IL>(N, 0, v)] = su * sc;
+        y[fidx<IL>(N, 1, v)] = sv * sc;
+        y[fidx<IL>(N, 2, v)] = sw * sc;
+        y[fidx<IL>(N, 3, v)] = sq * sc;
     }
+}
+
+// r = b - A x (FP64) of the flow system, waves striding over the vertices, and per-block
+// partial sums of r^2 and b^2
+template <bool IL>
+__global__ void __launch_bounds__(BLK) k_residual_flow(FlowSpmvGeom fg, const int32_t* __restrict__ rp,
+                                                       const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                       const double* __restrict__ x, const double* __restrict__ b,
+                                                       double* __restrict__ r, double* __restrict__ part, int nblk) {
+    __shared__ double sh[BLK / WAVE];
+    const int64_t N = fg.N;
+    const int lane = threadIdx.x % WAVE;
+    const int64_t waves = (int64_t)gridDim.x * (BLK / WAVE);
+    double rr = 0.0, bb = 0.0;
+    for (int64_t v = ((int64_t)xcd_block() * BLK + threadIdx.x) / WAVE; v < N; v += waves) {
+        double s[4];
+        flow_rows<double, IL>(fg, rp, col, val, x, v, lane, s[0], s[1], s[2], s[3]);
+        for (int c = 0; c < 4; ++c) s[c] = wave_sum(s[c]);
+        if (lane == 0)
+            for (int c = 0; c < 4; ++c) {
+                const int64_t i = fidx<IL>(N, c, v);
+                const double bi = b[i];
+                const double ri = bi - s[c];
+                r[i] = ri;
+                rr += ri * ri;
+                bb += bi * bi;
+            }
+    }
+    double t1 = block_sum(rr, sh);
+    double t2 = block_sum(bb, sh);
+    if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[nblk + blockIdx.x] = t2; }
+}
+
+// comp-major <-> vertex-interleaved copies of an n = 4N vector
+template <class T>
+__global__ void __launch_bounds__(BLK) k_to_interleaved(int64_t N, const T* __restrict__ in, T* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < 4 * N) out[4 * (i % N) + i / N] = in[i];
+}
+template <class T>
+__global__ void __launch_bounds__(BLK) k_from_interleaved(int64_t N, const T* __restrict__ in, T* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < 4 * N) out[i] = in[4 * (i % N) + i / N];
 }
 
 // r = b - A x (FP64) and per-block partial sums of r^2 and b^2 (grid-stride over rows)
@@ -678,7 +735,8 @@ __global__ void __launch_bounds__(BLK) k_bjac_setup(int64_t N, const int32_t* __
         }
 }
 // z = B^-1 (c r) per vertex block (c = *rscale or 1); z must not alias r
-template <class T, int NB>
+// (IL: the vectors hold the NB unknowns of a vertex contiguously, entry (j, v) at NB v + j)
+template <class T, int NB, bool IL>
 __global__ void __launch_bounds__(BLK) k_bjac_apply(int64_t N, const T* __restrict__ inv, const T* __restrict__ r,
                                                     const double* __restrict__ rscale, T* __restrict__ z,
                                                     const int* __restrict__ ctrl) {
@@ -687,13 +745,16 @@ __global__ void __launch_bounds__(BLK) k_bjac_apply(int64_t N, const T* __restri
     if (v >= N) return;
     T rr[NB];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) rr[j] = rscale ? (T)(r[(int64_t)j * N + v] * (T)rscale[0]) : r[(int64_t)j * N + v];
+    for (int j = 0; j < NB; ++j) {
+        const int64_t k = IL ? NB * v + j : (int64_t)j * N + v;
+        rr[j] = rscale ? (T)(r[k] * (T)rscale[0]) : r[k];
+    }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         T acc = 0;
 #pragma unroll
         for (int j = 0; j < NB; ++j) acc += inv[((int64_t)i * NB + j) * N + v] * rr[j];
-        z[(int64_t)i * N + v] = acc;
+        z[IL ? NB * v + i : (int64_t)i * N + v] = acc;
     }
 }
 
@@ -816,11 +877,15 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
     unsigned grid = grid_for(n * lpr);
     const int32_t* rp = sy.row;
     const int32_t* cl = sy.col;
-    if (c.flow && rp == c.mrow.p && n == 4 * c.g.N) {
-        // the coupled flow system in its own (unpermuted) order
+    if (sy.flow_layout) {
+        // the coupled flow system, comp-major or vertex-interleaved vectors
         FlowSpmvGeom fg{c.g.N, c.g.NP, c.g.row_ptr.p, c.g.gc_ptr.p};
-        launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T>, dim3(grid_for(c.g.N * WAVE)), dim3(BLK), fg, rp, cl, val, x, y,
-                     scale, ctrl);
+        if (sy.flow_layout == 2)
+            launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true>, dim3(grid_for(c.g.N * WAVE)), dim3(BLK), fg, rp, cl, val,
+                         x, y, scale, ctrl);
+        else
+            launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, false>, dim3(grid_for(c.g.N * WAVE)), dim3(BLK), fg, rp, cl, val,
+                         x, y, scale, ctrl);
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }
@@ -838,6 +903,15 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
 int residual_launch(Ctx& c, const SysView& sy, int lpr, const double* x, double* r, double* part, int nblk_r) {
     const int64_t n = sy.n;
     unsigned grid = (unsigned)nblk_r;
+    if (sy.flow_layout) {
+        FlowSpmvGeom fg{c.g.N, c.g.NP, c.g.row_ptr.p, c.g.gc_ptr.p};
+        if (sy.flow_layout == 2)
+            hipLaunchKernelGGL((k_residual_flow<true>), dim3(grid), dim3(BLK), 0, c.stream, fg, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r);
+        else
+            hipLaunchKernelGGL((k_residual_flow<false>), dim3(grid), dim3(BLK), 0, c.stream, fg, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r);
+        VFVM_HIP_TRY(hipGetLastError());
+        return VFVM_OK;
+    }
     switch (lpr) {
         case 4: hipLaunchKernelGGL((k_residual<4>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
         case 8: hipLaunchKernelGGL((k_residual<8>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
@@ -1043,22 +1117,24 @@ struct Pc {
     const T* lu = nullptr;
     const T* bj = nullptr;
     int nb = 0;
-    int64_t nv = 0;  // vertices (block-Jacobi blocks)
+    int64_t nv = 0;   // vertices (block-Jacobi blocks)
+    bool il = false;  // vectors vertex-interleaved
 };
 
 template <class T>
 int bjac_apply(Ctx& c, const Pc<T>& pc, const T* r, const double* rscale, T* z, const int* ctrl) {
     const double bytes = (double)sizeof(T) * pc.nv * (pc.nb * pc.nb + 2 * pc.nb);
     const dim3 grid(grid_for(pc.nv));
+#define VFVM_BJ(NB)                                                                                                         \
+    case NB:                                                                                                                \
+        if (pc.il) launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, NB, true>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); \
+        else launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, NB, false>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); \
+        break;
     switch (pc.nb) {
-        case 1: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 1>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
-        case 2: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 2>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
-        case 3: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 3>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
-        case 4: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 4>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
-        case 5: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 5>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
-        case 6: launch_timed(c, VFVM_K_PRECOND, bytes, k_bjac_apply<T, 6>, grid, dim3(BLK), pc.nv, pc.bj, r, rscale, z, ctrl); break;
+        VFVM_BJ(1) VFVM_BJ(2) VFVM_BJ(3) VFVM_BJ(4) VFVM_BJ(5) VFVM_BJ(6)
         default: return set_error(VFVM_ERR_INVALID_ARGUMENT, "block Jacobi: block size must be 1..6");
     }
+#undef VFVM_BJ
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
 }
@@ -1099,7 +1175,7 @@ int issue_step(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* 
     double* part = c.part.p;
     double* part_nrm = part + (int64_t)(MAX_RESTART + 1) * w.nwav;
     // algorithmic bytes: values, the column indices the kernel must read, row pointers, x, y
-    const bool flow_spmv = c.flow && sy.row == c.mrow.p && n == 4 * c.g.N;
+    const bool flow_spmv = sy.flow_layout != 0;
     const double bytes_spmv = flow_spmv ? (double)w.nnz * sizeof(T) + 4.0 * c.fs.spmv_idx + 4.0 * (n + 1) + 8.0 * c.g.N +
                                               2.0 * sizeof(T) * n
                                         : (double)w.nnz * (sizeof(T) + 4) + 4.0 * (n + 1) + 2.0 * sizeof(T) * n;
@@ -1149,7 +1225,8 @@ CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv
                           int* status) {
     const T* lu = pc.lu;
     const IluData* I = lu ? sy.ilu : nullptr;
-    std::array<uint64_t, 22> key = {(uint64_t)(uintptr_t)pc.bj, (uint64_t)pc.nb, (uint64_t)(uintptr_t)V,
+    std::array<uint64_t, 22> key = {(uint64_t)(uintptr_t)pc.bj, (uint64_t)pc.nb + (pc.il ? 256u : 0u) + 512u * (uint64_t)sy.flow_layout,
+                                    (uint64_t)(uintptr_t)V,
                                     (uint64_t)(uintptr_t)wv, (uint64_t)(uintptr_t)zv,
                                     (uint64_t)(uintptr_t)aval, (uint64_t)(uintptr_t)lu, (uint64_t)(uintptr_t)w.S,
                                     (uint64_t)(uintptr_t)w.ctrl, (uint64_t)(uintptr_t)c.part.p,
@@ -1238,6 +1315,7 @@ SysView primal_view(Ctx& c) {
     s.vals = c.vals.p; s.b = c.b.p;
     s.h_row = &c.h_mrow; s.h_col = &c.h_mcol;
     s.ilu = &c.ilu;
+    s.flow_layout = c.flow ? 1 : 0;
     return s;
 }
 
@@ -1368,6 +1446,18 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     // write their rows in place, with no indirection.
     SysView sy = sy0;
     double* x = x0;
+    // the coupled flow system with block Jacobi (or none) runs with vertex-interleaved
+    // vectors: P A P^T with the four unknowns of a vertex adjacent
+    const bool il = sy0.flow_layout != 0 && (bjac || o.precond == VFVM_PRECOND_NONE);
+    if (il) {
+        VFVM_HIP_TRY(c.bp64.alloc(sy0.n));
+        VFVM_HIP_TRY(c.xp64.alloc(sy0.n));
+        sy.flow_layout = 2;
+        sy.b = c.bp64.p;
+        x = c.xp64.p;
+    } else if (sy0.flow_layout && ilu) {
+        sy.flow_layout = 0;
+    }
     if (ilu) {
         if (int s = ilu_setup(c, sy0)) return s;
         const IluData& I = *sy0.ilu;
@@ -1419,6 +1509,10 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     VFVM_HIP_TRY(hipEventRecord(ev.e[0], c.stream));
 
     // permuted system and preconditioner set-up (inside the timed solve)
+    if (il) {
+        hipLaunchKernelGGL((k_to_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, sy0.b, c.bp64.p);
+        hipLaunchKernelGGL((k_to_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, (const double*)x0, c.xp64.p);
+    }
     if (ilu) {
         const IluData& I = *sy0.ilu;
         hipLaunchKernelGGL((k_ilu_gather<double>), dim3(grid_for(sy0.nnz)), dim3(BLK), 0, c.stream, sy0.nnz, I.src.p, sy0.vals, c.ap64.p);
@@ -1460,8 +1554,8 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     pc64.lu = (const double*)(mixed ? nullptr : lu);
     pc32.lu = (const float*)(mixed ? lu : nullptr);
     if (bjac) {
-        pc64.bj = c.bjac.inv64.p; pc64.nb = c.bjac.nb; pc64.nv = c.g.N;
-        pc32.bj = c.bjac.inv32.p; pc32.nb = c.bjac.nb; pc32.nv = c.g.N;
+        pc64.bj = c.bjac.inv64.p; pc64.nb = c.bjac.nb; pc64.nv = c.g.N; pc64.il = il;
+        pc32.bj = c.bjac.inv32.p; pc32.nb = c.bjac.nb; pc32.nv = c.g.N; pc32.il = il;
     }
     VFVM_HIP_TRY(hipGetLastError());
     VFVM_HIP_TRY(hipEventRecord(ev.e[1], c.stream));
@@ -1531,6 +1625,7 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
         ++cycles;
     }
     if (ilu) hipLaunchKernelGGL((k_scatter_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy0.ilu->perm.p, (const double*)x, x0);
+    if (il) hipLaunchKernelGGL((k_from_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, (const double*)x, x0);
     VFVM_HIP_TRY(hipEventRecord(ev.e[2], c.stream));
     VFVM_HIP_TRY(hipEventSynchronize(ev.e[2]));
     float ms_all = 0.f, ms_setup = 0.f;
