@@ -113,17 +113,17 @@ struct FlowSpmvGeom {
     const int32_t* rp;      // vertex adjacency (degree of v)
     const int32_t* gc_ptr;  // grad_coeff list lengths
 };
-template <class T>
+template <class T, int L = WAVE>
 __device__ inline T wave_sum(T s) {
-    for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
+    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
     return s;
 }
 // position of unknown (component c, vertex m): comp-major c N + m, or vertex-interleaved 4 m + c
 template <bool IL>
 __device__ inline int64_t fidx(int64_t N, int c, int64_t m) { return IL ? 4 * m + c : c * N + m; }
 
-// the four row sums of vertex v (lane partials, not yet reduced)
-template <class T, bool IL>
+// the four row sums of vertex v (partials of its L lanes, not yet reduced)
+template <class T, bool IL, int L = WAVE>
 __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                  const T* __restrict__ val, const T* __restrict__ x, int64_t v, int lane, T& su, T& sv,
                                  T& sw, T& sq) {
@@ -136,14 +136,14 @@ __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restri
     if (lu > 1) {
         // own-component part of the velocity rows: {v} u ngh(v)
         const int32_t d1 = fg.rp[v + 1] - fg.rp[v] + 1;
-        for (int32_t k = lane; k < d1; k += WAVE) {
+        for (int32_t k = lane; k < d1; k += L) {
             const int32_t m = col[ru + k];
             su += val[ru + k] * x[fidx<IL>(N, 0, m)];
             sv += val[rv + k] * x[fidx<IL>(N, 1, m)];
             sw += val[rw + k] * x[fidx<IL>(N, 2, m)];
         }
         // p part of the velocity rows and velocity part of the continuity row: grad_coeff(v)
-        for (int32_t q = lane; q < G; q += WAVE) {
+        for (int32_t q = lane; q < G; q += L) {
             const int32_t m = col[ru + d1 + q] - pN3;
             const T pm = x[fidx<IL>(N, 3, m)];
             su += val[ru + d1 + q] * pm;
@@ -160,14 +160,14 @@ __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restri
             sw = val[rw] * x[fidx<IL>(N, 2, v)];
         }
         if (lq > 1)
-            for (int32_t q = lane; q < G; q += WAVE) {
+            for (int32_t q = lane; q < G; q += L) {
                 const int32_t m = col[rq + q];
                 sq += val[rq + q] * x[fidx<IL>(N, 0, m)] + val[rq + G + q] * x[fidx<IL>(N, 1, m)] +
                       val[rq + 2 * G + q] * x[fidx<IL>(N, 2, m)];
             }
     }
     if (lq > 1) {
-        for (int32_t i = 3 * G + lane; i < lq; i += WAVE) sq += val[rq + i] * x[fidx<IL>(N, 3, col[rq + i] - pN3)];
+        for (int32_t i = 3 * G + lane; i < lq; i += L) sq += val[rq + i] * x[fidx<IL>(N, 3, col[rq + i] - pN3)];
     } else if (lane == 0) {
         sq = val[rq] * x[fidx<IL>(N, 3, v)];
     }
@@ -183,21 +183,21 @@ __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restri
 // products.  With IL the vectors are vertex-interleaved, so the four unknowns of a gathered
 // vertex share one 32-byte (FP64) segment.  Rows of length 1 (velocity Dirichlet rows, the
 // reference-pressure row) are diagonal.
-template <class T, bool IL>
+template <class T, bool IL, int L>
 __global__ void __launch_bounds__(BLK) k_spmv_flow(FlowSpmvGeom fg, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                    const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
                                                    const double* __restrict__ scale, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0] == 0) return;
     const int64_t N = fg.N;
-    const int64_t v = ((int64_t)xcd_block() * BLK + threadIdx.x) / WAVE;
-    const int lane = threadIdx.x % WAVE;
+    const int64_t v = ((int64_t)xcd_block() * BLK + threadIdx.x) / L;
+    const int lane = threadIdx.x % L;
     if (v >= N) return;
     T su, sv, sw, sq;
-    flow_rows<T, IL>(fg, rp, col, val, x, v, lane, su, sv, sw, sq);
-    su = wave_sum(su);
-    sv = wave_sum(sv);
-    sw = wave_sum(sw);
-    sq = wave_sum(sq);
+    flow_rows<T, IL, L>(fg, rp, col, val, x, v, lane, su, sv, sw, sq);
+    su = wave_sum<T, L>(su);
+    sv = wave_sum<T, L>(sv);
+    sw = wave_sum<T, L>(sw);
+    sq = wave_sum<T, L>(sq);
     if (lane == 0) {
         const T sc = scale ? (T)scale[0] : (T)1;
         y[fidx<IL>(N, 0, v)] = su * sc;
@@ -880,12 +880,22 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
     if (sy.flow_layout) {
         // the coupled flow system, comp-major or vertex-interleaved vectors
         FlowSpmvGeom fg{c.g.N, c.g.NP, c.g.row_ptr.p, c.g.gc_ptr.p};
-        if (sy.flow_layout == 2)
-            launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true>, dim3(grid_for(c.g.N * WAVE)), dim3(BLK), fg, rp, cl, val,
-                         x, y, scale, ctrl);
-        else
-            launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, false>, dim3(grid_for(c.g.N * WAVE)), dim3(BLK), fg, rp, cl, val,
-                         x, y, scale, ctrl);
+        // lanes per vertex (VFVM_FLOW_LPV=64 selects one vertex per wave)
+        static const int lpv = [] {
+            const char* e = getenv("VFVM_FLOW_LPV");
+            return (e && atoi(e) == 64) ? 64 : 32;
+        }();
+        if (sy.flow_layout == 2) {
+            if (lpv == 32)
+                launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 32>, dim3(grid_for(c.g.N * 32)), dim3(BLK), fg, rp, cl,
+                             val, x, y, scale, ctrl);
+            else
+                launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 64>, dim3(grid_for(c.g.N * 64)), dim3(BLK), fg, rp, cl,
+                             val, x, y, scale, ctrl);
+        } else {
+            launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, false, 64>, dim3(grid_for(c.g.N * WAVE)), dim3(BLK), fg, rp, cl,
+                         val, x, y, scale, ctrl);
+        }
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }
