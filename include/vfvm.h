@@ -326,6 +326,44 @@ int vfvm_transient_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const 
                          double dt, int32_t num_ts, int32_t max_iter, double tol, double damp,
                          vfvm_driver_stats* stats);
 
+/* ---- adaptive time stepping (TransientAdaptive, src/transient/transient_adaptive.rs) ----
+ * Backward Euler with the operator list's time operators, the local truncation error from
+ * the linear-extrapolation predictor through the last two accepted states
+ * (LTE ~ dt / (2 dt + dt_prev) |y - y_p|; constant predictor and factor 1/2 on the first
+ * step), error norm max over the unknown fields of max_i |LTE_i| / (atol + rtol s_f) with the
+ * field scales of scaling.rs, PI controller fac = safety err^-0.35 err_prev^0.2 in
+ * [fac_min, fac_max], rejection on err > 1 (fac = max(fac_min, safety err^-1/2)) or on
+ * Picard failure (dt fac_fail), steps shortened to land on the output times.  Works for a
+ * scalar transient list and for a TransientFlow list. */
+typedef struct vfvm_adaptive_opts {
+    double t_end, dt_init, dt_min, dt_max;  /* AdaptiveOptions (transient_adaptive.rs:42-60) */
+    double rtol, atol;
+    double safety, fac_min, fac_max, fac_fail;
+    int32_t max_reject;      /* consecutive rejections before giving up                 */
+    int32_t max_iter;        /* Picard                                                  */
+    double tol, damp;        /* Picard                                                  */
+    int32_t fixed;           /* 1: dt = dt_init throughout (output times still hit)     */
+    int32_t num_out;         /* output times in out_times (-1: 11 evenly spaced to t_end) */
+    const double* out_times;
+    int32_t num_exclude;     /* scldom ids left out of the error norm (e.g. the pressure) */
+    int32_t reserved;
+    const int32_t* scl_exclude;
+} vfvm_adaptive_opts;
+typedef struct vfvm_adaptive_stats {
+    int32_t accepted, rejected_error, rejected_picard, picard_iterations;
+    int32_t linear_iterations, outputs;
+    double t_final, time_assembly, time_solve;
+} vfvm_adaptive_stats;
+/* Called after each accepted step with the state at t (value_prev still holds t - dt). */
+typedef void (*vfvm_step_hook)(void* user, double t, double dt);
+/* AdaptiveOptions::new (transient_adaptive.rs:62-73). */
+void vfvm_adaptive_opts_default(double dt_init, double t_end, vfvm_adaptive_opts* opts);
+/* TransientAdaptive::solve_adaptive_with (transient_adaptive.rs:81-350).  steps_out (may be
+ * NULL): (t, dt, err) of the first max_steps accepted steps.  lin: the linear solves. */
+int vfvm_transient_adaptive(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* lin,
+                            const vfvm_adaptive_opts* opts, vfvm_step_hook hook, void* user,
+                            double* steps_out, int32_t max_steps, vfvm_adaptive_stats* stats);
+
 /* ---- discrete adjoint (no reference counterpart: the reference has no adjoint) ----
  * Forward map of a linear problem (no operator coefficient depends on the unknown):
  *   steady    (num_ts = 0):  A T = f

@@ -1272,3 +1272,116 @@ def bjacobi_apply(rp, col, val, r, N):
                     B[i, j] = vs[k[0]]
         z[idx] = np.linalg.solve(B, r[idx])
     return z
+
+
+def transient_adaptive(system, x0, t_end, dt_init, dt_min=None, dt_max=None, rtol=1e-3, atol=1e-12, safety=0.9,
+                       fac_min=0.2, fac_max=2.0, fac_fail=0.25, max_reject=30, out_times=None, max_iter=30, tol=1e-6,
+                       damp=1.0, fixed=False, hook=None):
+    """TransientAdaptive::solve_adaptive_with (src/transient/transient_adaptive.rs:81-350) for one
+    scalar unknown.  system(x, x_prev, dt) -> (A, b) assembles at state x with previous state
+    x_prev; x0 already carries the Dirichlet values.  Scalar field norm (scaling.rs:88-135):
+    Picard residual rms(dx) / max|x|, error norm max |LTE| / (atol + rtol max|x|).
+    Returns (x, stats dict, accepted steps [(t, dt, err)])."""
+    from scipy.sparse.linalg import spsolve
+    dt_min = 1e-12 * t_end if dt_min is None else dt_min
+    dt_max = t_end if dt_max is None else dt_max
+    t_eps = 1e-12 * t_end
+    outs = [t_end * i / 10.0 for i in range(11)] if out_times is None else list(out_times)
+    outs = sorted(t for t in outs if t > t_eps and t <= t_end)
+    ded = []
+    for t in outs:
+        if not ded or abs(t - ded[-1]) > t_eps:
+            ded.append(t)
+    outs = ded
+    if outs and outs[-1] >= t_end - t_eps:
+        outs[-1] = t_end
+    else:
+        outs.append(t_end)
+
+    def resid(xo, xn):
+        if not np.all(np.isfinite(xn)):
+            return float("nan")
+        rms = np.sqrt(((np.abs(xn - xo)) ** 2).sum() / max(xn.size, 1))
+        return rms / max(np.abs(xo).max(), np.abs(xn).max(), 1e-30)
+
+    st = dict(accepted=0, rejected_error=0, rejected_picard=0, picard=0, outputs=0)
+    steps = []
+    x_n = np.array(x0, np.float64)
+    x_nm1, dt_prev = None, None
+    t, out_idx = 0.0, 0
+    dt = min(max(dt_init, dt_min), dt_max)
+    err_prev, n_reject, no_grow = 1.0, 0, False
+    while t < t_end - t_eps:
+        t_out = outs[out_idx]
+        dt_step, hits = dt, False
+        if t + dt_step >= t_out - t_eps:
+            dt_step, hits = t_out - t, True
+        elif t + 1.5 * dt_step > t_out:
+            dt_step = 0.5 * (t_out - t)
+        t_new = t_out if hits else t + dt_step
+        x_iter = x_n.copy()
+        A, b = system(x_iter, x_n, dt_step)
+        conv, res = False, float("nan")
+        for _ in range(max_iter):
+            x_u = spsolve(A.tocsc(), b)
+            x_d = (1.0 - damp) * x_iter + damp * x_u
+            A, b = system(x_d, x_n, dt_step)
+            res = resid(x_iter, x_d)
+            st["picard"] += 1
+            x_iter = x_d
+            if not np.isfinite(res):
+                break
+            if res < tol:
+                conv = True
+                break
+        if not conv:
+            st["rejected_picard"] += 1
+            n_reject += 1
+            if fixed or n_reject > max_reject or dt_step <= dt_min * (1.0 + 1e-12):
+                raise RuntimeError("TransientDidNotConverge")
+            dt = max(dt_step * fac_fail, dt_min)
+            no_grow = True
+            continue
+        err = 0.0
+        if not fixed:
+            if x_nm1 is not None:
+                yp = x_n + dt_step / dt_prev * (x_n - x_nm1)
+                fac = dt_step / (2.0 * dt_step + dt_prev)
+            else:
+                yp, fac = x_n, 0.5
+            emax = (fac * np.abs(x_iter - yp)).max()
+            scale = max(np.abs(x_n).max(), np.abs(x_iter).max(), 1e-30)
+            err = emax / (atol + rtol * scale)
+        if err > 1.0 and dt_step > dt_min * (1.0 + 1e-12):
+            st["rejected_error"] += 1
+            n_reject += 1
+            if n_reject > max_reject:
+                raise RuntimeError("TransientDidNotConverge")
+            dt = max(dt_step * max(safety * err ** -0.5, fac_min), dt_min)
+            no_grow = True
+            continue
+        if hook is not None:
+            hook(t_new, dt_step, x_iter)
+        st["accepted"] += 1
+        steps.append((t_new, dt_step, err))
+        x_nm1, dt_prev = x_n, dt_step
+        x_n = x_iter
+        t = t_new
+        n_reject = 0
+        if hits:
+            st["outputs"] += 1
+            out_idx += 1
+            if out_idx >= len(outs):
+                break
+        if fixed:
+            dt = dt_init
+            continue
+        e = max(err, 1e-10)
+        fac = safety * e ** -0.35 * max(err_prev, 1e-10) ** 0.2
+        fac = min(max(fac, fac_min), fac_max)
+        if no_grow:
+            fac, no_grow = min(fac, 1.0), False
+        base = max(dt, dt_step) if hits else dt_step
+        dt = min(max(base * fac, dt_min), dt_max)
+        err_prev = e
+    return x_n, st, steps

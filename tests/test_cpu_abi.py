@@ -30,7 +30,8 @@ def test_library_exports_every_declared_symbol():
 
 _STRUCTS = [("vfvm_mesh_info", L.MeshInfo), ("vfvm_op", L.OpStruct), ("vfvm_solve_opts", L.SolveOpts),
             ("vfvm_solve_stats", L.SolveStats), ("vfvm_driver_stats", L.DriverStats),
-            ("vfvm_adjoint_stats", L.AdjointStats), ("vfvm_adjoint_wrt", L.AdjointWrt)]
+            ("vfvm_adjoint_stats", L.AdjointStats), ("vfvm_adjoint_wrt", L.AdjointWrt),
+            ("vfvm_adaptive_opts", L.AdaptiveOpts), ("vfvm_adaptive_stats", L.AdaptiveStats)]
 
 
 def test_struct_layouts_match_header(tmp_path):

@@ -819,6 +819,10 @@ int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts*
 
 }  // namespace
 
+int vfvm::scalar_picard_residual(Ctx& c, const double* xo, const double* xn, double* res) {
+    return picard_residual(c, xo, xn, res);
+}
+
 extern "C" int vfvm_steady_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* opts,
                                  int32_t max_iter, double tol, double damp, vfvm_driver_stats* stats) {
     CTX_GUARD(ctx);

@@ -283,6 +283,16 @@ int flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts&
                 double damp, vfvm_driver_stats& ds, double dt = 0.0, int ts = 0);
 int flow_assemble_dt(Ctx& c, const vfvm_op* ops, int32_t nops, double dt);
 int flow_update_prev(Ctx& c);
+// x = [vel | pres] and back (the unknowns of the flow system)
+int flow_pack(Ctx& c, double* x);
+int flow_unpack(Ctx& c, const double* x);
+// scaled Picard residual of the flow fields between two states (scaling.rs:113-135), and
+// the field statistics of two states: [0] sum |du|^2, [1] max |vel|, [2] max |p|, [3] min p,
+// [4] max p, [5] non-finite entries of xn, [6] max |rho|
+int flow_residual(Ctx& c, const double* xo, const double* xn, double* res);
+int flow_stats(Ctx& c, const double* xo, const double* xn, double out[7]);
+// scaled Picard residual of the scalar unknown (api_ctx.hip)
+int scalar_picard_residual(Ctx& c, const double* xo, const double* xn, double* res);
 
 // drivers and adjoint set-up shared across translation units (api_ctx.hip)
 int assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt);

@@ -987,19 +987,15 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
     const int64_t N = c.g.N, n = 4 * N;
     double* vel = c.vec[c.fs.vel].value.p;
     double* pres = c.scl[c.fs.pres].value.p;
-    DevBuf<double> xi, xu, xd, part;
+    DevBuf<double> xi, xu, xd;
     VFVM_HIP_TRY(xi.alloc(n));
     VFVM_HIP_TRY(xu.alloc(n));
     VFVM_HIP_TRY(xd.alloc(n));
-    const unsigned nb = grid_for(N);
-    VFVM_HIP_TRY(part.alloc(7 * (size_t)nb));
-    std::vector<double> h(7 * (size_t)nb);
     auto t0 = clk::now();
     hipLaunchKernelGGL(k_flow_pack, dim3(grid_for(n)), dim3(BLK), 0, c.stream, N, (const double*)vel, (const double*)pres, xi.p);
     if (int st = flow_assemble_dt(c, ops, nops, dt)) return st;
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     ds.time_assembly += secs(t0, clk::now());
-    const double* den = c.scl[c.fs.den].value.p;
     for (int32_t iter = 0; iter < max_iter; ++iter) {
         auto ti0 = clk::now();
         VFVM_HIP_TRY(hipMemcpyAsync(xu.p, xi.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
@@ -1013,38 +1009,8 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
         if (int r = flow_assemble_dt(c, ops, nops, dt)) return r;
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         ds.time_assembly += secs(ti1, clk::now());
-        // scaled residual (scaling.rs:113-135)
-        hipLaunchKernelGGL(k_flow_stats, dim3(nb), dim3(BLK), 0, c.stream, N, (const double*)xi.p, (const double*)xd.p, den, part.p);
-        VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-        VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
-        double sv = 0.0, vmax = 0.0, pabs = 0.0, pmin = INFINITY, pmax = -INFINITY, nf = 0.0, rho = 0.0;
-        for (unsigned k = 0; k < nb; ++k) {
-            const double* p = h.data() + 7 * (size_t)k;
-            sv += p[0];
-            vmax = std::max(vmax, p[1]);
-            pabs = std::max(pabs, p[2]);
-            pmin = std::min(pmin, p[3]);
-            pmax = std::max(pmax, p[4]);
-            nf += p[5];
-            rho = std::max(rho, p[6]);
-        }
-        double res;
-        if (nf > 0.0) {
-            res = NAN;
-        } else {
-            const double dead = 1e-11 * pabs;  // PRES_ROUNDOFF (scaling.rs:28)
-            hipLaunchKernelGGL(k_flow_pres_sq, dim3(nb), dim3(BLK), 0, c.stream, N, (const double*)(xi.p + 3 * N),
-                               (const double*)(xd.p + 3 * N), dead, part.p);
-            VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-            VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
-            double sp = 0.0;
-            for (unsigned k = 0; k < nb; ++k) sp += h[k];
-            const double floor = 1e-30;  // SCALE_FLOOR (scaling.rs:27)
-            const double nn = (double)std::max<int64_t>(N, 1);
-            const double scale_p = std::max(std::max(pmax - pmin, rho * vmax * vmax), floor);
-            const double scale_v = std::max(vmax, floor);
-            res = std::max(std::sqrt(sp / nn) / scale_p, std::sqrt(sv / nn) / scale_v);
-        }
+        double res = 0.0;
+        if (int r = flow_residual(c, xi.p, xd.p, &res)) return r;
         ds.residual = res;
         ds.picard_iterations += 1;
         if (!std::isfinite(res))
@@ -1066,4 +1032,67 @@ int vfvm::flow_update_prev(Ctx& c) {
     for (auto& f : c.vec)
         VFVM_HIP_TRY(hipMemcpyAsync(f.value_prev.p, f.value.p, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     return sync_err(c);
+}
+
+int vfvm::flow_pack(Ctx& c, double* x) {
+    const int64_t N = c.g.N;
+    hipLaunchKernelGGL(k_flow_pack, dim3(grid_for(4 * N)), dim3(BLK), 0, c.stream, N, (const double*)c.vec[c.fs.vel].value.p,
+                       (const double*)c.scl[c.fs.pres].value.p, x);
+    VFVM_HIP_TRY(hipGetLastError());
+    return VFVM_OK;
+}
+
+int vfvm::flow_unpack(Ctx& c, const double* x) {
+    const int64_t N = c.g.N;
+    VFVM_HIP_TRY(hipMemcpyAsync(c.vec[c.fs.vel].value.p, x, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    VFVM_HIP_TRY(hipMemcpyAsync(c.scl[c.fs.pres].value.p, x + 3 * N, N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    return VFVM_OK;
+}
+
+int vfvm::flow_stats(Ctx& c, const double* xo, const double* xn, double out[7]) {
+    const int64_t N = c.g.N;
+    const unsigned nb = grid_for(N);
+    DevBuf<double> part;
+    VFVM_HIP_TRY(part.alloc(7 * (size_t)nb));
+    hipLaunchKernelGGL(k_flow_stats, dim3(nb), dim3(BLK), 0, c.stream, N, xo, xn, (const double*)c.scl[c.fs.den].value.p, part.p);
+    std::vector<double> h(7 * (size_t)nb);
+    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    out[0] = 0.0; out[1] = 0.0; out[2] = 0.0; out[3] = INFINITY; out[4] = -INFINITY; out[5] = 0.0; out[6] = 0.0;
+    for (unsigned k = 0; k < nb; ++k) {
+        const double* p = h.data() + 7 * (size_t)k;
+        out[0] += p[0];
+        out[1] = std::max(out[1], p[1]);
+        out[2] = std::max(out[2], p[2]);
+        out[3] = std::min(out[3], p[3]);
+        out[4] = std::max(out[4], p[4]);
+        out[5] += p[5];
+        out[6] = std::max(out[6], p[6]);
+    }
+    return VFVM_OK;
+}
+
+// max over the velocity and pressure fields of rms(x_new - x_old) / scale (scaling.rs:113-135)
+int vfvm::flow_residual(Ctx& c, const double* xo, const double* xn, double* res) {
+    const int64_t N = c.g.N;
+    double st[7];
+    if (int r = flow_stats(c, xo, xn, st)) return r;
+    if (st[5] > 0.0) { *res = NAN; return VFVM_OK; }
+    const double dead = 1e-11 * st[2];  // PRES_ROUNDOFF (scaling.rs:28)
+    const unsigned nb = grid_for(N);
+    DevBuf<double> part;
+    VFVM_HIP_TRY(part.alloc(nb));
+    hipLaunchKernelGGL(k_flow_pres_sq, dim3(nb), dim3(BLK), 0, c.stream, N, xo + 3 * N, xn + 3 * N, dead, part.p);
+    std::vector<double> h(nb);
+    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    double sp = 0.0;
+    for (unsigned k = 0; k < nb; ++k) sp += h[k];
+    const double floor = 1e-30;  // SCALE_FLOOR (scaling.rs:27)
+    const double nn = (double)std::max<int64_t>(N, 1);
+    const double vmax = st[1];
+    const double scale_p = std::max(std::max(st[4] - st[3], st[6] * vmax * vmax), floor);
+    const double scale_v = std::max(vmax, floor);
+    *res = std::max(std::sqrt(sp / nn) / scale_p, std::sqrt(st[0] / nn) / scale_v);
+    return VFVM_OK;
 }

@@ -70,6 +70,24 @@ class SolveStats(C.Structure):
                 ("rel_residual", C.c_double), ("seconds", C.c_double), ("setup_seconds", C.c_double)]
 
 
+class AdaptiveOpts(C.Structure):
+    _fields_ = [("t_end", C.c_double), ("dt_init", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+                ("rtol", C.c_double), ("atol", C.c_double), ("safety", C.c_double), ("fac_min", C.c_double),
+                ("fac_max", C.c_double), ("fac_fail", C.c_double), ("max_reject", C.c_int32), ("max_iter", C.c_int32),
+                ("tol", C.c_double), ("damp", C.c_double), ("fixed", C.c_int32), ("num_out", C.c_int32),
+                ("out_times", C.POINTER(C.c_double)), ("num_exclude", C.c_int32), ("reserved", C.c_int32),
+                ("scl_exclude", C.POINTER(C.c_int32))]
+
+
+class AdaptiveStats(C.Structure):
+    _fields_ = [("accepted", C.c_int32), ("rejected_error", C.c_int32), ("rejected_picard", C.c_int32),
+                ("picard_iterations", C.c_int32), ("linear_iterations", C.c_int32), ("outputs", C.c_int32),
+                ("t_final", C.c_double), ("time_assembly", C.c_double), ("time_solve", C.c_double)]
+
+
+STEP_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_double)
+
+
 class DriverStats(C.Structure):
     _fields_ = [("picard_iterations", C.c_int32), ("linear_iterations", C.c_int32), ("residual", C.c_double),
                 ("time_assembly", C.c_double), ("time_solve", C.c_double)]
@@ -133,6 +151,9 @@ SIGNATURES = [
     ("vfvm_precond_apply", _I, [_P, C.POINTER(SolveOpts), _PD, _PD]),
     ("vfvm_ilu_order", _I, [_P, _PI, _PI, _PI]),
     ("vfvm_steady_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _I, _D, _D, C.POINTER(DriverStats)]),
+    ("vfvm_adaptive_opts_default", None, [_D, _D, C.POINTER(AdaptiveOpts)]),
+    ("vfvm_transient_adaptive", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), C.POINTER(AdaptiveOpts), STEP_HOOK,
+                                     _P, _PD, _I, C.POINTER(AdaptiveStats)]),
     ("vfvm_transient_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, _D, _D, C.POINTER(DriverStats)]),
     ("vfvm_adjoint_forward", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, C.POINTER(AdjointStats)]),
     ("vfvm_adjoint_backward", _I, [_P, C.POINTER(SolveOpts), _PD, _I, C.POINTER(AdjointWrt), C.POINTER(AdjointStats)]),

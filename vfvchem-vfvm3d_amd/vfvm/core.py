@@ -350,6 +350,35 @@ class Context:
                                        float(damp), C.byref(st)))
         return st
 
+    def transient_adaptive(self, ops, dt_init, t_end, lin=None, hook=None, max_steps=100000, **opts):
+        """TransientAdaptive::solve_adaptive_with (transient_adaptive.rs:81-350).  opts: fields
+        of vfvm_adaptive_opts (out_times as a sequence, scl_exclude as a sequence of ids).
+        hook(t, dt) is called after each accepted step.  Returns (stats, steps (k, 3) array of
+        (t, dt, err))."""
+        arr = make_ops(ops)
+        o = L.AdaptiveOpts()
+        lib.vfvm_adaptive_opts_default(float(dt_init), float(t_end), C.byref(o))
+        keep = []
+        for k, v in opts.items():
+            if k == "out_times":
+                a = np.ascontiguousarray(v, np.float64)
+                keep.append(a)
+                o.out_times, o.num_out = ptr(a), a.size
+            elif k == "scl_exclude":
+                a = np.ascontiguousarray(v, np.int32)
+                keep.append(a)
+                o.scl_exclude, o.num_exclude = ptr(a, C.c_int32), a.size
+            else:
+                setattr(o, k, v)
+        lo = solve_opts(**(lin or {}))
+        st = L.AdaptiveStats()
+        steps = np.zeros((max_steps, 3))
+        cb = L.STEP_HOOK((lambda user, t, dt: hook(t, dt)) if hook else (lambda user, t, dt: None))
+        self.n = (4 if any(int(op[0]) >= OP_VEC_DOM_ADVECTION for op in ops) else 1) * self.mesh.num_tot
+        check(lib.vfvm_transient_adaptive(self._h, arr, len(ops), C.byref(lo), C.byref(o), cb, None, ptr(steps), int(max_steps),
+                                          C.byref(st)))
+        return st, steps[:min(st.accepted, max_steps)].copy()
+
     # discrete adjoint
     def adjoint_forward(self, ops, dt=0.0, num_ts=0, checkpoint_every=10, lin=None):
         """Forward pass of the linear steady (num_ts=0) or implicit-Euler map, keeping
