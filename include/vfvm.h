@@ -27,6 +27,7 @@
  *                                      vfvm_pattern_build / vfvm_assemble / vfvm_steady_solve
  *   flow_helper::update_flow           src/operator/core/flow_helper.rs:3-179 -> vfvm_update_flow
  *   Variables::add_vecbnd_{cnst,func}  src/base/vars.rs:169-184           -> vfvm_vecbnd_add
+ *   TransientAdaptive::solve_adaptive_with src/transient/transient_adaptive.rs:81-350 -> vfvm_transient_adaptive
  */
 #ifndef VFVM_H
 #define VFVM_H
