@@ -248,7 +248,10 @@ typedef struct vfvm_op {
  * pressure-gradient coefficients (vfvm_mesh_build_grad_coeff).  The velocity rows of the
  * points hold [c N + ({v} u ngh(v)) | 3N + grad_coeff(v)], the pressure rows
  * [c N + grad_coeff(v), c = 0..2 | 3N + 3-ring(v)], all ascending; every entry is the sum of
- * its pushes in the reference's push order. */
+ * its pushes in the reference's push order.  Scalar operators in the same list (the heat or
+ * species part of SteadyHeatFlow / TransientHeatMassFlow, steady_heatflow.rs:468-505) act
+ * on further scalar unknowns, each a block of N rows after the pressure, in order of first
+ * appearance: x = [u, v, w, p, s_1, s_2, ...] (at most two, block Jacobi's b <= 6). */
 
 /* Fixes the unknown's block start (SteadyBase::initial_start), builds the row-sorted
  * CSR pattern of the operator list and the edge->slot / diag->slot maps on the host,
@@ -311,6 +314,13 @@ int vfvm_precond_apply(vfvm_ctx* ctx, const vfvm_solve_opts* opts, const double*
 int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, int32_t* ncolors);
 
 /* ---- drivers ---- */
+/* Host closure of function fields that depend on unknowns or time (Variables::_update_func,
+ * steady_base.rs:242-261): called with the time t before every assembly of the drivers
+ * (vfvm_steady_solve, vfvm_transient_solve, vfvm_transient_adaptive), after the unknowns
+ * hold the current iterate; it may read fields (vfvm_scldom_get ...) and set function fields
+ * (vfvm_scldom_set, vfvm_vecdom_set).  NULL removes it. */
+typedef void (*vfvm_update_hook)(void* user, double t);
+int vfvm_set_update_hook(vfvm_ctx* ctx, vfvm_update_hook hook, void* user);
 typedef struct vfvm_driver_stats {
     int32_t picard_iterations;   /* total Picard iterations                   */
     int32_t linear_iterations;   /* total Arnoldi steps                       */

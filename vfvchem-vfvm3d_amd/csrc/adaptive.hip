@@ -233,6 +233,7 @@ extern "C" int vfvm_transient_adaptive(vfvm_ctx* ctx, const vfvm_op* ops, int32_
         // Picard iterations at the end of the step from the last accepted state
         auto ta = clk::now();
         VFVM_HIP_TRY(hipMemcpyAsync(xi.p, xn.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        if (int st = call_update_hook(c, t_new)) return st;
         if (int st = sy.assemble(dt_step)) return st;
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         as.time_assembly += secs(ta, clk::now());
@@ -256,6 +257,7 @@ extern "C" int vfvm_transient_adaptive(vfvm_ctx* ctx, const vfvm_op* ops, int32_
             hipLaunchKernelGGL(k_damp_to, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, opt->damp, (const double*)xi.p,
                                (const double*)xu.p, xd.p);
             if (int st = sy.unpack(xd.p)) return st;
+            if (int st = call_update_hook(c, t_new)) return st;
             if (int st = sy.assemble(dt_step)) return st;
             VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
             as.time_assembly += secs(ti1, clk::now());

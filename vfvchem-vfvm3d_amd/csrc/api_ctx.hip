@@ -579,19 +579,9 @@ extern "C" int vfvm_pattern_get(vfvm_ctx* ctx, int32_t* row_ptr, int32_t* col) {
 }
 
 // ------------------------------------------------------------------ assembly
-int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
-    if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "assemble: no pattern built (call vfvm_pattern_build)");
-    if (c.flow || is_flow_list(ops, nops)) return flow_assemble_dt(c, ops, nops, dt);
-    Compiled cp;
-    if (int st = compile_ops(c, ops, nops, cp)) return st;
-    if (cp.unk != c.unk) return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: operators act on another unknown than the pattern");
-    {
-        Compiled pp;
-        if (int st = compile_ops(c, c.pattern_ops.data(), (int32_t)c.pattern_ops.size(), pp)) return st;
-        if (pp.has_edges != cp.has_edges || pp.has_diag != cp.has_diag)
-            return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: operators do not fit the built pattern");
-    }
-    if (cp.has_time && !(dt > 0.0)) return set_error(VFVM_ERR_OPERATOR_USE_APPLY_TIME, "time operators need dt > 0 (OperatorUseApplyTime)");
+// Gradients of every non-constant field a scalar operator list reads (ScalarDomain::update_grad)
+// and the face mass flux of the flows it uses; skip_flw: a flow whose data is already current.
+int vfvm::prepare_scalar_fields(Ctx& c, const vfvm_op* ops, int32_t nops, int32_t skip_flw) {
     // gradients of every non-constant field the operators read (ScalarDomain::update_grad)
     std::vector<uint8_t> need_s(c.scl.size(), 0), need_v(c.vec.size(), 0), need_f(c.flw.size(), 0);
     for (int32_t i = 0; i < nops; ++i) {
@@ -637,19 +627,61 @@ int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
     // face mass flux: a prescribed flow's from its velocity, a solved flow's (unknown
     // pressure) from the Rhie-Chow interpolation of its current fields
     for (size_t f = 0; f < c.flw.size(); ++f) {
-        if (!need_f[f]) continue;
+        if (!need_f[f] || (int32_t)f == skip_flw) continue;
         if (c.scl[c.flw[f].pres].kind == VFVM_FIELD_UNKNOWN) {
             if (int st = flow_update(c, (int32_t)f)) return st;
         } else if (int st = launch_flux_prescribed(c, c.flw[f])) {
             return st;
         }
     }
+    return VFVM_OK;
+}
+
+int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
+    if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "assemble: no pattern built (call vfvm_pattern_build)");
+    if (c.flow || is_flow_list(ops, nops)) return flow_assemble_dt(c, ops, nops, dt);
+    Compiled cp;
+    if (int st = compile_ops(c, ops, nops, cp)) return st;
+    if (cp.unk != c.unk) return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: operators act on another unknown than the pattern");
+    {
+        Compiled pp;
+        if (int st = compile_ops(c, c.pattern_ops.data(), (int32_t)c.pattern_ops.size(), pp)) return st;
+        if (pp.has_edges != cp.has_edges || pp.has_diag != cp.has_diag)
+            return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: operators do not fit the built pattern");
+    }
+    if (cp.has_time && !(dt > 0.0)) return set_error(VFVM_ERR_OPERATOR_USE_APPLY_TIME, "time operators need dt > 0 (OperatorUseApplyTime)");
+    if (int st = prepare_scalar_fields(c, ops, nops, -1)) return st;
     DevProgram p;
     build_program(c, cp, p);
     VFVM_HIP_TRY(hipMemcpyAsync(c.prog.p, &p, sizeof(p), hipMemcpyHostToDevice, c.stream));
     if (int st = launch_assemble(c, dt)) return st;
     c.has_matrix = true;
     return VFVM_OK;
+}
+
+int vfvm::scalar_classes(Ctx& c, const vfvm_op* ops, int32_t nops, int32_t* unk, std::vector<uint8_t>& has_edges,
+                         std::vector<uint8_t>& has_diag, bool* has_time) {
+    Compiled cp;
+    if (int st = compile_ops(c, ops, nops, cp)) return st;
+    *unk = cp.unk;
+    has_edges = cp.has_edges;
+    has_diag = cp.has_diag;
+    *has_time = cp.has_time;
+    return VFVM_OK;
+}
+
+// One scalar unknown's rows inside a coupled system: the scalar assembly kernel writes the
+// block's values through its own edge/diagonal slot maps and its b segment.
+int vfvm::scalar_block_assemble(Ctx& c, ScalarBlock& blk, int64_t row_off, double dt, int32_t skip_flw) {
+    Compiled cp;
+    if (int st = compile_ops(c, blk.ops.data(), (int32_t)blk.ops.size(), cp)) return st;
+    if (cp.has_time && !(dt > 0.0)) return set_error(VFVM_ERR_OPERATOR_USE_APPLY_TIME, "time operators need dt > 0 (OperatorUseApplyTime)");
+    if (int st = prepare_scalar_fields(c, blk.ops.data(), (int32_t)blk.ops.size(), skip_flw)) return st;
+    DevProgram p;
+    build_program(c, cp, p);
+    VFVM_HIP_TRY(blk.prog.alloc(1));
+    VFVM_HIP_TRY(hipMemcpyAsync(blk.prog.p, &p, sizeof(p), hipMemcpyHostToDevice, c.stream));
+    return launch_assemble_block(c, dt, c.mrow.p + row_off, blk.eslot.p, blk.dslot.p, blk.prog.p, c.b.p + row_off);
 }
 
 extern "C" int vfvm_assemble(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, double dt) {
@@ -737,6 +769,10 @@ extern "C" int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, 
 // ------------------------------------------------------------------ drivers
 int vfvm::initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
     if (is_flow_list(ops, nops)) return flow_initial_dirichlet(c, ops, nops);
+    return scalar_initial_dirichlet(c, ops, nops);
+}
+
+int vfvm::scalar_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
     // OpSclBndDirichlet::apply_initial for every Dirichlet operator (steady_base.rs:53-54)
     for (int32_t i = 0; i < nops; ++i) {
         if (ops[i].kind != VFVM_OP_SCL_BND_DIRICHLET) continue;
@@ -777,6 +813,7 @@ double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration
 
 int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* opts, double dt, int32_t max_iter,
                 double tol, double damp, vfvm_driver_stats& ds, int ts, bool transient) {
+    const double t_now = transient ? dt * (double)(ts + 1) : 0.0;  // end of the step (transient_base.rs:71)
     const int64_t n = c.n;
     SclDom& u = c.scl[c.unk];
     DevBuf<double> xi, xu;
@@ -785,6 +822,7 @@ int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts*
     auto t0 = clk::now();
     // x_iter = current unknown values (_unkn_to_solu, steady_base.rs:137)
     VFVM_HIP_TRY(hipMemcpyAsync(xi.p, u.value.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    if (int st = call_update_hook(c, t_now)) return st;
     if (int st = assemble_impl(c, ops, nops, dt)) return st;
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     auto t1 = clk::now();
@@ -799,6 +837,7 @@ int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts*
         ds.time_solve += secs(ti0, ti1);
         // damping into the unknown; re-assemble at the new iterate
         hipLaunchKernelGGL(k_damp, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, damp, xi.p, xu.p, u.value.p);
+        if (int st1 = call_update_hook(c, t_now)) return st1;
         if (int st2 = assemble_impl(c, ops, nops, dt)) return st2;
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         ds.time_assembly += secs(ti1, clk::now());
@@ -821,6 +860,20 @@ int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts*
 
 int vfvm::scalar_picard_residual(Ctx& c, const double* xo, const double* xn, double* res) {
     return picard_residual(c, xo, xn, res);
+}
+
+int vfvm::call_update_hook(Ctx& c, double t) {
+    if (!c.update_hook) return VFVM_OK;
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    c.update_hook(c.update_user, t);
+    return VFVM_OK;
+}
+
+extern "C" int vfvm_set_update_hook(vfvm_ctx* ctx, vfvm_update_hook hook, void* user) {
+    CTX_GUARD(ctx);
+    c.update_hook = hook;
+    c.update_user = user;
+    return clear_error();
 }
 
 extern "C" int vfvm_steady_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* opts,

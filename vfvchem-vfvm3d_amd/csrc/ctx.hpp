@@ -175,6 +175,7 @@ struct FlowSys {
     int64_t spmv_idx = 0;           // column indices the structured SpMV reads per product
     bool transient = false;         // TransientFlow list (velocity time operator first)
     double dt = 0.0;
+    std::vector<int32_t> blk_unk;   // scalar unknowns coupled in (rows (4 + k) N + v)
 };
 
 // Block-Jacobi inverse blocks (b x b per vertex, entry (i, j) of vertex v at (i b + j) N + v).
@@ -182,6 +183,15 @@ struct BlockJacobi {
     int32_t nb = 0;
     DevBuf<double> inv64;
     DevBuf<float> inv32;
+};
+
+// A scalar unknown's rows inside a coupled flow system (heat, species): its operators in list
+// order, its slot maps into the coupled CSR, its device operator program.
+struct ScalarBlock {
+    int32_t unk = -1;
+    std::vector<vfvm_op> ops;
+    DevBuf<int32_t> eslot, dslot;
+    DevBuf<DevProgram> prog;
 };
 
 struct KTimer {
@@ -218,6 +228,12 @@ struct Ctx {
     // coupled flow system (when the pattern was built from flow operators)
     bool flow = false;
     FlowSys fs;
+    std::vector<ScalarBlock> sblk;
+
+    // host closure of function fields that depend on unknowns (Variables::_update_func),
+    // called before every assembly of the drivers
+    vfvm_update_hook update_hook = nullptr;
+    void* update_user = nullptr;
 
     // transposed system A^T (adjoint solves): CSR of the columns of A, tsrc[k] = slot of
     // entry k in vals; values gathered on demand
@@ -261,6 +277,8 @@ int launch_grad(Ctx& c, const double* value, double* grad);
 int launch_grad_vec(Ctx& c, const double* value3, double* grad9);
 int launch_flux_prescribed(Ctx& c, SclFlw& f);
 int launch_assemble(Ctx& c, double dt);
+int launch_assemble_block(Ctx& c, double dt, const int32_t* mrow, const int32_t* eslot, const int32_t* dslot,
+                          const DevProgram* prog, double* b);
 
 // solver (krylov.hip)
 SysView primal_view(Ctx& c);
@@ -293,6 +311,14 @@ int flow_residual(Ctx& c, const double* xo, const double* xn, double* res);
 int flow_stats(Ctx& c, const double* xo, const double* xn, double out[7]);
 // scaled Picard residual of the scalar unknown (api_ctx.hip)
 int scalar_picard_residual(Ctx& c, const double* xo, const double* xn, double* res);
+// scalar operator lists inside a coupled system (api_ctx.hip)
+int prepare_scalar_fields(Ctx& c, const vfvm_op* ops, int32_t nops, int32_t skip_flw);
+int scalar_classes(Ctx& c, const vfvm_op* ops, int32_t nops, int32_t* unk, std::vector<uint8_t>& has_edges,
+                   std::vector<uint8_t>& has_diag, bool* has_time);
+int scalar_block_assemble(Ctx& c, ScalarBlock& blk, int64_t row_off, double dt, int32_t skip_flw);
+int scalar_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops);
+// calls the host update hook (if any) with time t
+int call_update_hook(Ctx& c, double t);
 
 // drivers and adjoint set-up shared across translation units (api_ctx.hip)
 int assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt);

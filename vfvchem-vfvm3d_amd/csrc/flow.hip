@@ -992,7 +992,9 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
     VFVM_HIP_TRY(xu.alloc(n));
     VFVM_HIP_TRY(xd.alloc(n));
     auto t0 = clk::now();
+    const double t_now = transient ? dt * (double)(ts + 1) : 0.0;  // end of the step (transient_base.rs:71)
     hipLaunchKernelGGL(k_flow_pack, dim3(grid_for(n)), dim3(BLK), 0, c.stream, N, (const double*)vel, (const double*)pres, xi.p);
+    if (int st = call_update_hook(c, t_now)) return st;
     if (int st = flow_assemble_dt(c, ops, nops, dt)) return st;
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     ds.time_assembly += secs(t0, clk::now());
@@ -1006,6 +1008,7 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
         ds.time_solve += secs(ti0, ti1);
         hipLaunchKernelGGL(k_flow_damp, dim3(grid_for(n)), dim3(BLK), 0, c.stream, N, damp, (const double*)xi.p,
                            (const double*)xu.p, xd.p, vel, pres);
+        if (int r = call_update_hook(c, t_now)) return r;
         if (int r = flow_assemble_dt(c, ops, nops, dt)) return r;
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         ds.time_assembly += secs(ti1, clk::now());

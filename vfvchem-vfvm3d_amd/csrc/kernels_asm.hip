@@ -346,6 +346,11 @@ int launch_flux_prescribed(Ctx& c, SclFlw& f) {
 }
 
 int launch_assemble(Ctx& c, double dt) {
+    return launch_assemble_block(c, dt, c.mrow.p, c.eslot.p, c.dslot.p, c.prog.p, c.b.p);
+}
+
+int launch_assemble_block(Ctx& c, double dt, const int32_t* mrow, const int32_t* eslot, const int32_t* dslot,
+                          const DevProgram* prog, double* b) {
     DevGeom& g = c.g;
     AsmArgs a;
     a.N = g.N; a.E = g.E; a.S = g.S;
@@ -353,10 +358,10 @@ int launch_assemble(Ctx& c, double dt) {
     a.vol = g.vol.p; a.cskew = g.cskew.p;
     a.bsf_ptr = g.bsf_ptr.p; a.bsf_vj = g.bsf_vj.p; a.bsf_vk = g.bsf_vk.p; a.bsf_area = g.bsf_area.p; a.bsf_wgt = g.bsf_wgt.p;
     a.vbnd = g.vbnd.p; a.bidx = g.bidx.p;
-    a.mrow = c.mrow.p; a.eslot = c.eslot.p; a.dslot = c.dslot.p;
-    a.prog = c.prog.p;
+    a.mrow = mrow; a.eslot = eslot; a.dslot = dslot;
+    a.prog = prog;
     a.dt = dt;
-    a.vals = c.vals.p; a.b = c.b.p;
+    a.vals = c.vals.p; a.b = b;
     // algorithmic bytes: per directed edge ngh, eslot (4+4), area, dist, skew (5 x 8), one value
     // write (8) and the neighbour's coefficient value + gradient (32); per vertex CSR offsets,
     // classes, geometry, b (~64)

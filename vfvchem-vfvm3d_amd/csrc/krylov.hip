@@ -1089,9 +1089,8 @@ int ilu_factor(Ctx& c, const SysView& sy, T* lu, const double* val) {
 // z = (LU)^-1 rhs in the colour-major space: forward sweep over the colours ascending (the
 // last one also finishing its backward sweep), backward sweep over the other colours
 // descending, in place in z (z must not alias rhs): 2 colours - 1 launches.
-template <class T>
-int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double* rscale, T* z, const int* ctrl) {
-    constexpr int LPR = 4;
+template <class T, int LPR>
+int ilu_apply_lpr(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double* rscale, T* z, const int* ctrl) {
     const IluData& I = *sy.ilu;
     const int ncol = (int)I.h_color_ptr.size() - 1;
     // algorithmic bytes of one sweep launch over r rows: the triangle's entries (value +
@@ -1118,6 +1117,26 @@ int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
+}
+
+// lanes per row of the sweeps (VFVM_ILU_LPR = 1, 2, 4 or 8; default 4)
+inline int ilu_lpr() {
+    static const int v = [] {
+        const char* e = getenv("VFVM_ILU_LPR");
+        const int k = e ? atoi(e) : 4;
+        return (k == 1 || k == 2 || k == 4 || k == 8) ? k : 4;
+    }();
+    return v;
+}
+
+template <class T>
+int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double* rscale, T* z, const int* ctrl) {
+    switch (ilu_lpr()) {
+        case 1: return ilu_apply_lpr<T, 1>(c, sy, lu, rhs, rscale, z, ctrl);
+        case 2: return ilu_apply_lpr<T, 2>(c, sy, lu, rhs, rscale, z, ctrl);
+        case 8: return ilu_apply_lpr<T, 8>(c, sy, lu, rhs, rscale, z, ctrl);
+        default: return ilu_apply_lpr<T, 4>(c, sy, lu, rhs, rscale, z, ctrl);
+    }
 }
 
 // Explicitly applied preconditioner of a cycle: the ILU(0) factor or the block-Jacobi

@@ -86,6 +86,7 @@ class AdaptiveStats(C.Structure):
 
 
 STEP_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_double)
+UPDATE_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
 
 
 class DriverStats(C.Structure):
@@ -142,6 +143,7 @@ SIGNATURES = [
     ("vfvm_pattern_build", _I, [_P, C.POINTER(OpStruct), _I, _PL]),
     ("vfvm_pattern_get", _I, [_P, _PI, _PI]),
     ("vfvm_system_info", _I, [_P, _PL, _PL]),
+    ("vfvm_set_update_hook", _I, [_P, UPDATE_HOOK, _P]),
     ("vfvm_assemble", _I, [_P, C.POINTER(OpStruct), _I, _D]),
     ("vfvm_matrix_get", _I, [_P, _PD, _PD]),
     ("vfvm_solve_opts_default", None, [C.POINTER(SolveOpts)]),

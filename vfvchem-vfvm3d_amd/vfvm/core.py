@@ -263,9 +263,9 @@ class Context:
         arr = make_ops(ops)
         nnz = C.c_int64()
         check(lib.vfvm_pattern_build(self._h, arr, len(ops), C.byref(nnz)))
-        self.nnz = nnz.value
-        # unknowns: one scalar, or the flow system [u, v, w, p] (steady_flow.rs:94-104)
-        self.n = (4 if any(int(o[0]) >= OP_VEC_DOM_ADVECTION for o in ops) else 1) * self.mesh.num_tot
+        # unknowns: one scalar, or the flow system [u, v, w, p, coupled scalars...]
+        # (steady_flow.rs:94-104, steady_heatflow.rs:468-505)
+        self.system_info()
         return nnz.value
 
     def system_info(self):
@@ -336,8 +336,8 @@ class Context:
         arr = make_ops(ops)
         st = L.DriverStats()
         o = solve_opts(**(lin or {}))
-        self.n = (4 if any(int(op[0]) >= OP_VEC_DOM_ADVECTION for op in ops) else 1) * self.mesh.num_tot
         check(lib.vfvm_steady_solve(self._h, arr, len(ops), C.byref(o), int(max_iter), float(tol), float(damp), C.byref(st)))
+        self.system_info()
         return st
 
     def transient_solve(self, ops, dt, num_ts, max_iter, tol, damp, lin=None):
@@ -345,9 +345,9 @@ class Context:
         arr = make_ops(ops)
         st = L.DriverStats()
         o = solve_opts(**(lin or {}))
-        self.n = (4 if any(int(op[0]) >= OP_VEC_DOM_ADVECTION for op in ops) else 1) * self.mesh.num_tot
         check(lib.vfvm_transient_solve(self._h, arr, len(ops), C.byref(o), float(dt), int(num_ts), int(max_iter), float(tol),
                                        float(damp), C.byref(st)))
+        self.system_info()
         return st
 
     def transient_adaptive(self, ops, dt_init, t_end, lin=None, hook=None, max_steps=100000, **opts):
@@ -374,10 +374,21 @@ class Context:
         st = L.AdaptiveStats()
         steps = np.zeros((max_steps, 3))
         cb = L.STEP_HOOK((lambda user, t, dt: hook(t, dt)) if hook else (lambda user, t, dt: None))
-        self.n = (4 if any(int(op[0]) >= OP_VEC_DOM_ADVECTION for op in ops) else 1) * self.mesh.num_tot
         check(lib.vfvm_transient_adaptive(self._h, arr, len(ops), C.byref(lo), C.byref(o), cb, None, ptr(steps), int(max_steps),
                                           C.byref(st)))
+        self.system_info()
         return st, steps[:min(st.accepted, max_steps)].copy()
+
+    def set_update_hook(self, hook):
+        """Variables::_update_func (steady_base.rs:242-261): hook(t) runs on the host before
+        every assembly of the drivers, with the unknowns holding the current iterate; it may
+        read fields and set function fields.  None removes it."""
+        if hook is None:
+            self._update_cb = None
+            check(lib.vfvm_set_update_hook(self._h, C.cast(None, L.UPDATE_HOOK), None))
+            return
+        self._update_cb = L.UPDATE_HOOK(lambda user, t: hook(t))
+        check(lib.vfvm_set_update_hook(self._h, self._update_cb, None))
 
     # discrete adjoint
     def adjoint_forward(self, ops, dt=0.0, num_ts=0, checkpoint_every=10, lin=None):
