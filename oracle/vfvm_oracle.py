@@ -1255,6 +1255,85 @@ def picard_flow(g, gc, vel0, pres0, den, visc, fce, bnd_vel, bnd_nid, pref, max_
     raise RuntimeError("SteadyDidNotConverge")
 
 
+def scaled_residual_scalar(x_old, x_new):
+    """Picard residual of one scalar field (src/base/scaling.rs:88-135): rms(x_new - x_old)
+    over its entries divided by max |x| over both states (floor 1e-30)."""
+    if not np.all(np.isfinite(x_new)):
+        return float("nan")
+    n = max(x_new.size, 1)
+    rms = np.sqrt(((np.abs(x_new - x_old)) ** 2).sum() / n)
+    return float(rms / max(np.abs(x_old).max(), np.abs(x_new).max(), 1e-30))
+
+
+def heatflow_solve(g, gc, vel0, pres0, T0, den, visc, fce_of, bnd_vel, bnd_nid, pref, heat_rows, t_dirichlet,
+                   max_iter, tol, damp, dt=0.0, num_ts=0):
+    """SteadyBase::solve / TransientBase::solve (src/steady/steady_base.rs:16-135,
+    src/transient/transient_base.rs:16-160) for SteadyHeatFlow / TransientHeatFlow
+    (src/steady/core/steady_heatflow.rs:154-536) with constant density and viscosity and a
+    body force that depends on the temperature through a host closure (Boussinesq; the
+    reference evaluates such function fields in _update_func before every assembly).
+
+    The solution vector is [vel (3N), pres (N), T (N)] (initial_start, :154-186).  Before
+    every assembly: fce = fce_of(T), gradients, update_flow (Rhie-Chow flux), then the flow
+    rows (assemble_flow) and the temperature rows heat_rows(T, T_prev, f) -> (rows, cols,
+    vals, b) with the flow data f of the same iterate (its flux and density), shifted by 4N.
+    t_dirichlet(T) returns T with the Dirichlet values loaded (apply_initial).  SuperLU
+    solves, damping, the scaled residual as the max over the three fields.  dt = 0: steady;
+    else num_ts implicit-Euler steps with the previous state updated after each step.
+    Returns vel, pres, T and the Picard iterations (total)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    N, NP = g["N"], g["num_pts"]
+    vel = np.array(vel0, np.float64).reshape(3, N).copy()
+    pres = np.array(pres0, np.float64).copy()
+    for bid, val in bnd_vel:
+        nodes = NP + np.asarray(bnd_nid[bid], np.int64)
+        vel[:, nodes] = np.asarray(val).reshape(3, -1)
+    T = t_dirichlet(np.array(T0, np.float64).copy())
+    vel_prev, T_prev = vel.copy(), T.copy()
+    transient = dt > 0.0
+
+    def system(vel, pres, T):
+        fce = fce_of(T)
+        f = flow_fields(g, gc, vel, pres, den, visc, fce, fce_const=False)
+        r, c, v, b = assemble_flow(g, gc, f, bnd_vel, bnd_nid, pref, dt, vel_prev if transient else None,
+                                   den if transient else None)
+        rh, ch, vh, bh = heat_rows(T, T_prev, f)
+        rows = np.concatenate([np.asarray(r, np.int64), 4 * N + np.asarray(rh, np.int64)])
+        cols = np.concatenate([np.asarray(c, np.int64), 4 * N + np.asarray(ch, np.int64)])
+        vals = np.concatenate([np.asarray(v, np.float64), np.asarray(vh, np.float64)])
+        rp, col, val = coo_to_csr(rows, cols, vals, 5 * N)
+        return sp.csr_matrix((val, col, rp), shape=(5 * N, 5 * N)).tocsc(), np.concatenate([b, bh])
+
+    def residual(xo, xn):
+        rf = scaled_residual_flow(N, xo[:4 * N], xn[:4 * N], den)
+        rt = scaled_residual_scalar(xo[4 * N:], xn[4 * N:])
+        if np.isnan(rf) or np.isnan(rt):
+            return float("nan")
+        return max(rf, rt)
+
+    total = 0
+    x_iter = np.concatenate([vel.ravel(), pres, T])
+    for ts in range(max(num_ts, 1) if transient else 1):
+        A, b = system(vel, pres, T)
+        for it in range(max_iter):
+            x_u = spla.spsolve(A, b)
+            x_d = (1.0 - damp) * x_iter + damp * x_u
+            vel, pres, T = x_d[:3 * N].reshape(3, N).copy(), x_d[3 * N:4 * N].copy(), x_d[4 * N:].copy()
+            A, b = system(vel, pres, T)
+            res = residual(x_iter, x_d)
+            if not np.isfinite(res):
+                raise FloatingPointError("NonFiniteResidual")
+            x_iter = x_d
+            total += 1
+            if res < tol:
+                break
+        else:
+            raise RuntimeError("DidNotConverge")
+        vel_prev, T_prev = vel.copy(), T.copy()
+    return vel, pres, T, total
+
+
 def bjacobi_apply(rp, col, val, r, N):
     """z = B^-1 r for the block diagonal B of the unknowns of each vertex (v, N+v, ...)."""
     n = rp.size - 1

@@ -312,3 +312,97 @@ def heat_flow_c5like(mesh, transient=False):
     p.ops = ops
     p.unknown, p.rcp_id = T, rcp
     return p
+
+
+class HeatFlowProblem:
+    """SteadyHeatFlow / TransientHeatFlow composition (steady_heatflow.rs:154-536) of one domain:
+    the flow of a FlowProblem and a temperature unknown carried by the flow's own Rhie-Chow
+    flux (OpSclDomAdvection with weight rho c_p, conductivity k and the flow's density),
+    with a heat source q, T = t_hot on x = 0 and t_cold on x = 1, zero heat flux on the other
+    faces (diffusion, source, Neumann and advection operators there), and a body force
+    fce = (0, 0, gamma (T - t_ref)) evaluated on the host before every assembly (Boussinesq).
+    transient: the velocity time operator first and OpSclDomTime / OpSclBndTime (weight
+    rho c_p) ahead of the temperature operators of each row class."""
+
+    def __init__(self, fp: FlowProblem, rcp, k, q, gamma=0.0, t_ref=0.5, t_hot=1.0, t_cold=0.0, T0=0.0, transient=False):
+        self.fp = fp
+        self.mesh = fp.mesh
+        N = self.N = fp.N
+        b = lambda v: np.broadcast_to(np.asarray(v, np.float64), (N,)).copy()
+        self.rcp, self.k, self.q, self.T0 = b(rcp), b(k), b(q), b(T0)
+        self.gamma, self.t_ref, self.t_hot, self.t_cold = float(gamma), float(t_ref), float(t_hot), float(t_cold)
+        self.transient = transient
+        fp.transient = transient
+        fp.const = fp.const - {"fce"}  # the body force is a function field
+
+    def fce_of(self, T):
+        return np.stack([np.zeros(self.N), np.zeros(self.N), self.gamma * (T - self.t_ref)])
+
+    def heat_ops(self, ids):
+        T, rcp, k, q, flw, t0, t1, fl = ids
+        o = [(vfvm.OP_SCL_DOM_TIME, 0, T, rcp, -1)] if self.transient else []
+        o += [(vfvm.OP_SCL_DOM_DIFFUSION, 0, T, k, T), (vfvm.OP_SCL_DOM_SOURCE, 0, T, q, -1),
+              (vfvm.OP_SCL_DOM_ADVECTION, 0, T, rcp, k, flw),
+              (vfvm.OP_SCL_BND_DIRICHLET, X0, T, t0, -1), (vfvm.OP_SCL_BND_DIRICHLET, X1, T, t1, -1)]
+        for bb in (Y0, Y1, Z0, Z1):
+            if self.transient:
+                o.append((vfvm.OP_SCL_BND_TIME, bb, T, rcp, -1))
+            o += [(vfvm.OP_SCL_BND_DIFFUSION, bb, T, k, T), (vfvm.OP_SCL_BND_SOURCE, bb, T, q, -1),
+                  (vfvm.OP_SCL_BND_NEUMANN, bb, T, fl[bb], -1), (vfvm.OP_SCL_BND_ADVECTION, bb, T, rcp, k, flw)]
+        return o
+
+    def to_device(self, ctx: vfvm.Context):
+        """Uploads every field; returns the coupled operator list in the reference's
+        assemble_matrix order (flow, heat, then the reference pressure)."""
+        fops = self.fp.to_device(ctx)
+        self.fp.fce = self.fce_of(self.T0)
+        ctx.vecdom_set(self.fp.ids[4], self.fp.fce)
+        T = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0, self.T0)
+        rcp = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, self.rcp)
+        k = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, self.k)
+        q = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, self.q)
+        cnt = lambda bb: len(self.mesh.bnd_nodes(bb))
+        t0 = ctx.sclbnd_add(X0, values=np.full(cnt(X0), self.t_hot))
+        t1 = ctx.sclbnd_add(X1, values=np.full(cnt(X1), self.t_cold))
+        fl = {bb: ctx.sclbnd_add(bb, values=np.zeros(cnt(bb))) for bb in (Y0, Y1, Z0, Z1)}
+        self.hids = (T, rcp, k, q, self.fp.ids[5], t0, t1, fl)
+        hops = self.heat_ops(self.hids)
+        pref = [o for o in fops if o[0] == vfvm.OP_SCL_DOM_PRESSURE_REFERENCE]
+        rest = [o for o in fops if o[0] != vfvm.OP_SCL_DOM_PRESSURE_REFERENCE]
+        return rest + hops + pref
+
+    def hook(self, ctx):
+        """The host closure: fce := fce_of(T) from the device's current temperature."""
+        fce_id, T_id = self.fp.ids[4], self.hids[0]
+        return lambda t: ctx.vecdom_set(fce_id, self.fce_of(ctx.scldom_get(T_id)))
+
+    # ---- oracle
+    def oracle_heat_rows(self, g, dt=0.0):
+        """heat_rows(T, T_prev, f) for O.heat_flow_solve / the assembled temperature rows."""
+        T_id, rcp_id, k_id, q_id, flw, t0, t1, fl = self.hids
+        bnd_nid = [self.mesh.bnd_nodes(bb) for bb in range(self.mesh.num_bnd)]
+        fixed = {rcp_id: (self.rcp, O.update_grad(g, self.rcp)), k_id: (self.k, O.update_grad(g, self.k)),
+                 q_id: (self.q, O.update_grad(g, self.q))}
+        sb = {t0: np.full(len(bnd_nid[X0]), self.t_hot), t1: np.full(len(bnd_nid[X1]), self.t_cold)}
+        sb.update({fl[bb]: np.zeros(len(bnd_nid[bb])) for bb in (Y0, Y1, Z0, Z1)})
+        ops = [O.Op(*o) for o in self.heat_ops(self.hids)]
+
+        def rows(T, T_prev, f):
+            scl = dict(fixed)
+            scl[T_id] = (T, O.update_grad(g, T))
+            prev = {T_id: T_prev, rcp_id: self.rcp}
+            return O.assemble(g, ops, scl, sb, {flw: f["flux"]}, bnd_nid, dt=dt, value_prev=prev,
+                              flow_den={flw: (f["den"], f["deng"])})
+        return rows
+
+    def oracle_t_dirichlet(self):
+        NP = self.mesh.num_pts
+        n0 = NP + self.mesh.bnd_nodes(X0).astype(np.int64)
+        n1 = NP + self.mesh.bnd_nodes(X1).astype(np.int64)
+
+        def load(T):
+            T = T.copy()
+            T[n0] = self.t_hot
+            T[n1] = self.t_cold
+            return T
+        return load

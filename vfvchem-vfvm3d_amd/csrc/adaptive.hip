@@ -188,6 +188,7 @@ extern "C" int vfvm_transient_adaptive(vfvm_ctx* ctx, const vfvm_op* ops, int32_
     if (int st = vfvm_pattern_build(ctx, ops, nops, nullptr)) return st;
     if (c.flow && !c.fs.transient)
         return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "adaptive: a flow list needs the velocity time operator (VEC_DOM_TIME) first");
+    if (int st = call_update_hook(c, 0.0)) return st;
     if (int st = initial_dirichlet(c, ops, nops)) return st;
     System sy{c, ops, nops, c.n, c.flow};
     std::vector<ErrField> fields;
@@ -199,6 +200,8 @@ extern "C" int vfvm_transient_adaptive(vfvm_ctx* ctx, const vfvm_op* ops, int32_
     if (c.flow) {
         fields.push_back({0, c.g.N, 1});
         if (!excluded(c.fs.pres)) fields.push_back({3 * c.g.N, c.g.N, 2});
+        for (size_t k = 0; k < c.sblk.size(); ++k)
+            if (!excluded(c.sblk[k].unk)) fields.push_back({(int64_t)(4 + k) * c.g.N, c.g.N, 0});
     } else if (!excluded(c.unk)) {
         fields.push_back({0, c.g.N, 0});
     }

@@ -675,6 +675,8 @@ int vfvm::scalar_classes(Ctx& c, const vfvm_op* ops, int32_t nops, int32_t* unk,
 int vfvm::scalar_block_assemble(Ctx& c, ScalarBlock& blk, int64_t row_off, double dt, int32_t skip_flw) {
     Compiled cp;
     if (int st = compile_ops(c, blk.ops.data(), (int32_t)blk.ops.size(), cp)) return st;
+    if (cp.unk != blk.unk || cp.has_edges != blk.has_edges || cp.has_diag != blk.has_diag)
+        return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: the scalar operators of unknown " + std::to_string(blk.unk) + " do not fit the built pattern");
     if (cp.has_time && !(dt > 0.0)) return set_error(VFVM_ERR_OPERATOR_USE_APPLY_TIME, "time operators need dt > 0 (OperatorUseApplyTime)");
     if (int st = prepare_scalar_fields(c, blk.ops.data(), (int32_t)blk.ops.size(), skip_flw)) return st;
     DevProgram p;
@@ -789,9 +791,8 @@ int vfvm::scalar_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
 
 namespace {
 
-// scaled Picard residual for one scalar field (scaling.rs:88-135)
-int picard_residual(Ctx& c, const double* xo, const double* xn, double* res) {
-    const int64_t n = c.n;
+// scaled Picard residual for one scalar field of n entries (scaling.rs:88-135)
+int picard_residual(Ctx& c, const double* xo, const double* xn, double* res, int64_t n) {
     const unsigned nb = grid_for(n);
     DevBuf<double> part;
     VFVM_HIP_TRY(part.alloc(4 * (size_t)nb));
@@ -842,7 +843,7 @@ int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts*
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         ds.time_assembly += secs(ti1, clk::now());
         double res = 0.0;
-        if (int st3 = picard_residual(c, xi.p, u.value.p, &res)) return st3;
+        if (int st3 = picard_residual(c, xi.p, u.value.p, &res, n)) return st3;
         ds.residual = res;
         ds.picard_iterations += 1;
         if (!std::isfinite(res))
@@ -859,7 +860,11 @@ int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts*
 }  // namespace
 
 int vfvm::scalar_picard_residual(Ctx& c, const double* xo, const double* xn, double* res) {
-    return picard_residual(c, xo, xn, res);
+    return picard_residual(c, xo, xn, res, c.n);
+}
+
+int vfvm::scalar_field_residual(Ctx& c, const double* xo, const double* xn, int64_t n, double* res) {
+    return picard_residual(c, xo, xn, res, n);
 }
 
 int vfvm::call_update_hook(Ctx& c, double t) {
@@ -887,6 +892,7 @@ extern "C" int vfvm_steady_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops
     vfvm_driver_stats ds;
     std::memset(&ds, 0, sizeof(ds));
     if (int st = vfvm_pattern_build(ctx, ops, nops, nullptr)) return st;
+    if (int st = call_update_hook(c, 0.0)) return st;  // _update_func before initial_dirichlet (steady_base.rs:50-54)
     if (int st = initial_dirichlet(c, ops, nops)) return st;
     int r = c.flow ? flow_picard(c, ops, nops, *opts, max_iter, tol, damp, ds)
                    : picard_loop(c, ops, nops, opts, 0.0, max_iter, tol, damp, ds, 0, false);
@@ -909,6 +915,7 @@ extern "C" int vfvm_transient_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t n
     vfvm_driver_stats ds;
     std::memset(&ds, 0, sizeof(ds));
     if (int st = vfvm_pattern_build(ctx, ops, nops, nullptr)) return st;
+    if (int st = call_update_hook(c, 0.0)) return st;  // _update_func before initial_dirichlet (steady_base.rs:50-54)
     if (int st = initial_dirichlet(c, ops, nops)) return st;
     if (c.flow) {
         // TransientFlow (transient_flow.rs:248-303 with transient_base.rs:16-160): the initial

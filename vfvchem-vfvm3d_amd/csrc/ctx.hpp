@@ -137,8 +137,9 @@ struct SysView {
     const std::vector<int32_t>* h_col = nullptr;
     IluData* ilu = nullptr;
     // coupled flow system: 0 generic CSR, 1 structured rows in the comp-major order,
-    // 2 structured rows with the vectors vertex-interleaved (entry (c, v) at 4 v + c)
+    // 2 structured rows with the vectors vertex-interleaved (entry (c, v) at (4 + flow_ns) v + c)
     int flow_layout = 0;
+    int flow_ns = 0;  // scalar unknowns coupled to the flow (rows (4 + k) N + v), at most 2
 };
 
 // Checkpoint tape of the adjoint forward pass: the operator list, step data and the
@@ -176,6 +177,7 @@ struct FlowSys {
     bool transient = false;         // TransientFlow list (velocity time operator first)
     double dt = 0.0;
     std::vector<int32_t> blk_unk;   // scalar unknowns coupled in (rows (4 + k) N + v)
+    std::vector<std::vector<vfvm_op>> blk_ops;  // their operators, in list order
 };
 
 // Block-Jacobi inverse blocks (b x b per vertex, entry (i, j) of vertex v at (i b + j) N + v).
@@ -190,6 +192,7 @@ struct BlockJacobi {
 struct ScalarBlock {
     int32_t unk = -1;
     std::vector<vfvm_op> ops;
+    std::vector<uint8_t> has_edges, has_diag;  // per vertex class, as the pattern was built
     DevBuf<int32_t> eslot, dslot;
     DevBuf<DevProgram> prog;
 };
@@ -311,6 +314,7 @@ int flow_residual(Ctx& c, const double* xo, const double* xn, double* res);
 int flow_stats(Ctx& c, const double* xo, const double* xn, double out[7]);
 // scaled Picard residual of the scalar unknown (api_ctx.hip)
 int scalar_picard_residual(Ctx& c, const double* xo, const double* xn, double* res);
+int scalar_field_residual(Ctx& c, const double* xo, const double* xn, int64_t n, double* res);
 // scalar operator lists inside a coupled system (api_ctx.hip)
 int prepare_scalar_fields(Ctx& c, const vfvm_op* ops, int32_t nops, int32_t skip_flw);
 int scalar_classes(Ctx& c, const vfvm_op* ops, int32_t nops, int32_t* unk, std::vector<uint8_t>& has_edges,

@@ -525,6 +525,17 @@ __global__ void __launch_bounds__(BLK) k_flow_damp(int64_t N, double d, const do
     else pres[i - 3 * N] = x;
 }
 
+// x_damp of a coupled scalar unknown, written to its values and to xd
+__global__ void __launch_bounds__(BLK) k_blk_damp(int64_t N, double d, const double* __restrict__ xi,
+                                                  const double* __restrict__ xu, double* __restrict__ xd,
+                                                  double* __restrict__ val) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= N) return;
+    const double x = (1.0 - d) * xi[i] + d * xu[i];
+    xd[i] = x;
+    val[i] = x;
+}
+
 // Partials of the scaled Picard residual (scaling.rs:88-135) over vertex blocks:
 // [0] sum |du|^2 over the velocity components, [1] max |vel| over both states,
 // [2] max |p| over both states, [3] min p, [4] max p, [5] non-finite entries of x_new,
@@ -593,7 +604,8 @@ __global__ void __launch_bounds__(BLK) k_flow_pres_sq(int64_t N, const double* _
 bool is_flow_kind(int k) { return k >= 40 && k < 70; }
 
 // Validates a flow operator list: the SteadyFlow composition (steady_flow.rs:133-240) with
-// one set of field ids.
+// one set of field ids, plus the scalar operators of coupled unknowns (heat, species:
+// steady_heatflow.rs:188-330), grouped by unknown in order of first appearance.
 int flow_compile(Ctx& c, const vfvm_op* ops, int32_t nops, FlowSys& fs) {
     if (!c.has_mesh) return set_error(VFVM_ERR_INVALID_ARGUMENT, "no mesh uploaded");
     if (!ops || nops <= 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "empty operator list");
@@ -609,11 +621,26 @@ int flow_compile(Ctx& c, const vfvm_op* ops, int32_t nops, FlowSys& fs) {
     std::vector<std::vector<vfvm_op>> bnd(nb);
     int npref = 0;
     vfvm_op pref{};
+    fs.blk_unk.clear();
+    fs.blk_ops.clear();
     for (int32_t i = 0; i < nops; ++i) {
         const vfvm_op& op = ops[i];
-        if (!is_flow_kind(op.kind))
-            return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "operator " + std::to_string(i) + ": scalar operators cannot join a flow operator list on the built path");
-        if (npref) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "the pressure reference must be the last operator (it replaces its row)");
+        if (!is_flow_kind(op.kind)) {
+            // a coupled scalar unknown: its own rows after the pressure, its operators checked
+            // when the pattern is built
+            if (op.unk_id < 0 || op.unk_id >= (int32_t)c.scl.size() || c.scl[op.unk_id].kind != VFVM_FIELD_UNKNOWN)
+                return set_error(VFVM_ERR_OPERATOR_NON_UNKNOWN, "operator " + std::to_string(i) + ": a scalar operator in a flow list must act on an unknown scalar field (OperatorNonUnknownInLinearSystem)");
+            size_t k = 0;
+            while (k < fs.blk_unk.size() && fs.blk_unk[k] != op.unk_id) ++k;
+            if (k == fs.blk_unk.size()) {
+                if (k == 2) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "flow: at most two scalar unknowns can be coupled (six unknowns per vertex)");
+                fs.blk_unk.push_back(op.unk_id);
+                fs.blk_ops.emplace_back();
+            }
+            fs.blk_ops[k].push_back(op);
+            continue;
+        }
+        if (npref) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "the pressure reference must come after every flow operator (it replaces its row)");
         if (op.kind < 60) {
             if (op.geom_id != 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "domain operator: only domain 0 exists");
             if (op.kind == VFVM_OP_SCL_DOM_PRESSURE_REFERENCE) { pref = op; ++npref; }
@@ -643,6 +670,8 @@ int flow_compile(Ctx& c, const vfvm_op* ops, int32_t nops, FlowSys& fs) {
     fs.pres = fl.pres; fs.den = fl.den; fs.visc = fl.visc; fs.fce = fl.fce;
     if (fl.vel != fs.vel) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "flow: the operators' velocity is not the flow's velocity");
     if (c.scl[fs.pres].kind != VFVM_FIELD_UNKNOWN) return set_error(VFVM_ERR_OPERATOR_NON_UNKNOWN, "flow: the pressure is not an unknown (OperatorNonUnknownInLinearSystem)");
+    for (int32_t u : fs.blk_unk)
+        if (u == fs.pres) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "flow: scalar operators cannot act on the flow's pressure");
     auto same = [&](bool ok) { return ok ? VFVM_OK : set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "flow: every operator must name the flow's velocity, pressure, density, viscosity and body force"); };
     for (int k = 0; k < 6; ++k) if (int st = same(dom[k].unk_id == fs.vel)) return st;
     if (int st = same(adv.a == fs.den && adv.b == fs.visc)) return st;
@@ -907,8 +936,49 @@ int vfvm::flow_pattern_build(Ctx& c, const vfvm_op* ops, int32_t nops) {
             });
         for (auto& x : th) x.join();
     }
+    // coupled scalar unknowns: block k holds rows (4 + k) N + v with the scalar pattern of its
+    // operators ({v} u ngh(v) ascending, or {v}) shifted into its own column block
+    const int nblk = (int)fs.blk_unk.size();
+    std::vector<std::vector<int32_t>> b_eslot(nblk), b_dslot(nblk);
+    std::vector<std::vector<uint8_t>> b_edges(nblk), b_diag(nblk);
+    if (nblk > 0) {
+        std::vector<int32_t> vbnd_h(N, -1);
+        for (int32_t b = 0; b < g.num_bnd; ++b)
+            for (int64_t k = g.bnd_ptr[b]; k < g.bnd_ptr[b + 1]; ++k) vbnd_h[NP + g.bnd_nid[k]] = b;
+        c.h_mrow.resize(n + (int64_t)nblk * N + 1);
+        for (int k = 0; k < nblk; ++k) {
+            int32_t unk = -1;
+            bool has_time = false;
+            if (int st = scalar_classes(c, fs.blk_ops[k].data(), (int32_t)fs.blk_ops[k].size(), &unk, b_edges[k], b_diag[k], &has_time)) return st;
+            b_eslot[k].assign(g.E, -1);
+            b_dslot[k].assign(N, -1);
+            const int32_t off = (int32_t)((4 + k) * N);
+            for (int64_t v = 0; v < N; ++v) {
+                const int cls = vbnd_h[v] < 0 ? 0 : 1 + vbnd_h[v];
+                if (b_edges[k][cls]) {
+                    bool placed = false;
+                    for (int32_t e = rp[v]; e < rp[v + 1]; ++e) {
+                        if (!placed && ng[e] > v) { b_dslot[k][v] = (int32_t)c.h_mcol.size(); c.h_mcol.push_back(off + (int32_t)v); placed = true; }
+                        b_eslot[k][e] = (int32_t)c.h_mcol.size();
+                        c.h_mcol.push_back(off + ng[e]);
+                    }
+                    if (!placed) { b_dslot[k][v] = (int32_t)c.h_mcol.size(); c.h_mcol.push_back(off + (int32_t)v); }
+                } else if (b_diag[k][cls]) {
+                    b_dslot[k][v] = (int32_t)c.h_mcol.size();
+                    c.h_mcol.push_back(off + (int32_t)v);
+                }
+                if ((int64_t)c.h_mcol.size() > INT32_MAX) return set_error(VFVM_ERR_INVALID_ARGUMENT, "pattern exceeds 32-bit indices");
+                const int64_t row = (4 + k) * N + v;
+                c.h_mrow[row + 1] = (int32_t)c.h_mcol.size();
+                fs.spmv_idx += c.h_mrow[row + 1] - c.h_mrow[row];
+            }
+            dslot.insert(dslot.end(), b_dslot[k].begin(), b_dslot[k].end());
+        }
+        tot = (int64_t)c.h_mcol.size();
+    }
+    const int64_t ntot = n + (int64_t)nblk * N;
     c.nnz = tot;
-    c.n = n;
+    c.n = ntot;
     c.unk = -1;
     hipStream_t s = c.stream;
     VFVM_HIP_TRY(c.mrow.upload(c.h_mrow.data(), c.h_mrow.size(), s));
@@ -916,7 +986,18 @@ int vfvm::flow_pattern_build(Ctx& c, const vfvm_op* ops, int32_t nops) {
     VFVM_HIP_TRY(c.dslot.upload(dslot.data(), dslot.size(), s));
     c.eslot.reset();
     VFVM_HIP_TRY(c.vals.alloc(std::max<int64_t>(c.nnz, 1)));
-    VFVM_HIP_TRY(c.b.alloc(n));
+    VFVM_HIP_TRY(c.b.alloc(ntot));
+    c.sblk.clear();
+    c.sblk.resize(nblk);
+    for (int k = 0; k < nblk; ++k) {
+        ScalarBlock& blk = c.sblk[k];
+        blk.unk = fs.blk_unk[k];
+        blk.ops = fs.blk_ops[k];
+        blk.has_edges = b_edges[k];
+        blk.has_diag = b_diag[k];
+        VFVM_HIP_TRY(blk.eslot.upload(b_eslot[k].data(), b_eslot[k].size(), s));
+        VFVM_HIP_TRY(blk.dslot.upload(b_dslot[k].data(), b_dslot[k].size(), s));
+    }
     SclFlw& fl = c.flw[fs.flw];
     VFVM_HIP_TRY(fl.rc.alloc(N));
     VFVM_HIP_TRY(fl.mom.alloc(3 * N));
@@ -948,7 +1029,7 @@ int vfvm::flow_assemble_dt(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) 
     if (!c.has_pattern || !c.flow) return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: the built pattern is not a flow pattern");
     FlowSys fs;
     if (int st = flow_compile(c, ops, nops, fs)) return st;
-    if (fs.vel != c.fs.vel || fs.pres != c.fs.pres || fs.flw != c.fs.flw || fs.pref_row != c.fs.pref_row)
+    if (fs.vel != c.fs.vel || fs.pres != c.fs.pres || fs.flw != c.fs.flw || fs.pref_row != c.fs.pref_row || fs.blk_unk != c.fs.blk_unk)
         return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: the flow operators do not fit the built pattern");
     if (fs.transient && !(dt > 0.0)) return set_error(VFVM_ERR_OPERATOR_USE_APPLY_TIME, "the velocity time operator needs dt > 0 (OperatorUseApplyTime)");
     c.fs.vbnd_val = fs.vbnd_val;
@@ -956,6 +1037,11 @@ int vfvm::flow_assemble_dt(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) 
     c.fs.transient = fs.transient;
     c.fs.dt = fs.transient ? dt : 0.0;
     if (int st = flow_assemble_impl(c)) return st;
+    // coupled scalar rows, with the face mass flux just computed (steady_heatflow.rs:466-512)
+    for (size_t k = 0; k < c.sblk.size(); ++k) {
+        c.sblk[k].ops = fs.blk_ops[k];
+        if (int st = scalar_block_assemble(c, c.sblk[k], (int64_t)(4 + k) * c.g.N, dt, c.fs.flw)) return st;
+    }
     c.has_matrix = true;
     return VFVM_OK;
 }
@@ -975,6 +1061,9 @@ int vfvm::flow_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
     if (nn > 0)
         hipLaunchKernelGGL(k_flow_dirichlet_initial, dim3(grid_for(nn)), dim3(BLK), 0, c.stream, a, c.vec[fs.vel].value.p);
     VFVM_HIP_TRY(hipGetLastError());
+    // OpSclBndDirichlet::apply_initial of the coupled scalar unknowns (steady_heatflow.rs:517-528)
+    for (const auto& bo : fs.blk_ops)
+        if (int st = scalar_initial_dirichlet(c, bo.data(), (int32_t)bo.size())) return st;
     return sync_err(c);
 }
 
@@ -984,7 +1073,7 @@ int vfvm::flow_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
 int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts& o, int32_t max_iter, double tol,
                       double damp, vfvm_driver_stats& ds, double dt, int ts) {
     const bool transient = dt > 0.0;
-    const int64_t N = c.g.N, n = 4 * N;
+    const int64_t N = c.g.N, n = c.n;
     double* vel = c.vec[c.fs.vel].value.p;
     double* pres = c.scl[c.fs.pres].value.p;
     DevBuf<double> xi, xu, xd;
@@ -993,7 +1082,7 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
     VFVM_HIP_TRY(xd.alloc(n));
     auto t0 = clk::now();
     const double t_now = transient ? dt * (double)(ts + 1) : 0.0;  // end of the step (transient_base.rs:71)
-    hipLaunchKernelGGL(k_flow_pack, dim3(grid_for(n)), dim3(BLK), 0, c.stream, N, (const double*)vel, (const double*)pres, xi.p);
+    if (int st = flow_pack(c, xi.p)) return st;
     if (int st = call_update_hook(c, t_now)) return st;
     if (int st = flow_assemble_dt(c, ops, nops, dt)) return st;
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
@@ -1006,8 +1095,13 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
         ds.linear_iterations += st.iterations;
         auto ti1 = clk::now();
         ds.time_solve += secs(ti0, ti1);
-        hipLaunchKernelGGL(k_flow_damp, dim3(grid_for(n)), dim3(BLK), 0, c.stream, N, damp, (const double*)xi.p,
+        hipLaunchKernelGGL(k_flow_damp, dim3(grid_for(4 * N)), dim3(BLK), 0, c.stream, N, damp, (const double*)xi.p,
                            (const double*)xu.p, xd.p, vel, pres);
+        for (size_t k = 0; k < c.sblk.size(); ++k) {
+            const int64_t off = (int64_t)(4 + k) * N;
+            hipLaunchKernelGGL(k_blk_damp, dim3(grid_for(N)), dim3(BLK), 0, c.stream, N, damp, (const double*)xi.p + off,
+                               (const double*)xu.p + off, xd.p + off, c.scl[c.sblk[k].unk].value.p);
+        }
         if (int r = call_update_hook(c, t_now)) return r;
         if (int r = flow_assemble_dt(c, ops, nops, dt)) return r;
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
@@ -1037,11 +1131,14 @@ int vfvm::flow_update_prev(Ctx& c) {
     return sync_err(c);
 }
 
+// unknowns -> solution vector [u, v, w, p, s_1, ...] (SteadyBase::_unkn_to_solu, steady_base.rs:137-175)
 int vfvm::flow_pack(Ctx& c, double* x) {
     const int64_t N = c.g.N;
     hipLaunchKernelGGL(k_flow_pack, dim3(grid_for(4 * N)), dim3(BLK), 0, c.stream, N, (const double*)c.vec[c.fs.vel].value.p,
                        (const double*)c.scl[c.fs.pres].value.p, x);
     VFVM_HIP_TRY(hipGetLastError());
+    for (size_t k = 0; k < c.sblk.size(); ++k)
+        VFVM_HIP_TRY(hipMemcpyAsync(x + (4 + k) * N, c.scl[c.sblk[k].unk].value.p, N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     return VFVM_OK;
 }
 
@@ -1049,6 +1146,8 @@ int vfvm::flow_unpack(Ctx& c, const double* x) {
     const int64_t N = c.g.N;
     VFVM_HIP_TRY(hipMemcpyAsync(c.vec[c.fs.vel].value.p, x, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     VFVM_HIP_TRY(hipMemcpyAsync(c.scl[c.fs.pres].value.p, x + 3 * N, N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    for (size_t k = 0; k < c.sblk.size(); ++k)
+        VFVM_HIP_TRY(hipMemcpyAsync(c.scl[c.sblk[k].unk].value.p, x + (4 + k) * N, N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     return VFVM_OK;
 }
 
@@ -1075,7 +1174,8 @@ int vfvm::flow_stats(Ctx& c, const double* xo, const double* xn, double out[7]) 
     return VFVM_OK;
 }
 
-// max over the velocity and pressure fields of rms(x_new - x_old) / scale (scaling.rs:113-135)
+// max over the velocity, pressure and coupled scalar fields of rms(x_new - x_old) / scale
+// (scaling.rs:113-135)
 int vfvm::flow_residual(Ctx& c, const double* xo, const double* xn, double* res) {
     const int64_t N = c.g.N;
     double st[7];
@@ -1096,6 +1196,13 @@ int vfvm::flow_residual(Ctx& c, const double* xo, const double* xn, double* res)
     const double vmax = st[1];
     const double scale_p = std::max(std::max(st[4] - st[3], st[6] * vmax * vmax), floor);
     const double scale_v = std::max(vmax, floor);
-    *res = std::max(std::sqrt(sp / nn) / scale_p, std::sqrt(st[0] / nn) / scale_v);
+    double r = std::max(std::sqrt(sp / nn) / scale_p, std::sqrt(st[0] / nn) / scale_v);
+    for (size_t k = 0; k < c.sblk.size(); ++k) {
+        double rs = 0.0;
+        if (int e = scalar_field_residual(c, xo + (4 + k) * N, xn + (4 + k) * N, N, &rs)) return e;
+        if (std::isnan(rs)) { *res = NAN; return VFVM_OK; }
+        r = std::max(r, rs);
+    }
+    *res = r;
     return VFVM_OK;
 }

@@ -118,15 +118,19 @@ __device__ inline T wave_sum(T s) {
     for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
     return s;
 }
-// position of unknown (component c, vertex m): comp-major c N + m, or vertex-interleaved 4 m + c
-template <bool IL>
-__device__ inline int64_t fidx(int64_t N, int c, int64_t m) { return IL ? 4 * m + c : c * N + m; }
+// position of unknown (component c, vertex m) in a system of S unknowns per vertex:
+// comp-major c N + m, or vertex-interleaved S m + c
+template <bool IL, int S>
+__device__ inline int64_t fidx(int64_t N, int c, int64_t m) { return IL ? (int64_t)S * m + c : c * N + m; }
 
-// the four row sums of vertex v (partials of its L lanes, not yet reduced)
-template <class T, bool IL, int L = WAVE>
+// the row sums of vertex v (partials of its L lanes, not yet reduced): the four flow rows, and
+// the rows of the NS coupled scalar unknowns (rows (4 + k) N + v, columns in their own block:
+// {v} u ngh(v), or {v} alone)
+template <class T, bool IL, int L = WAVE, int NS = 0>
 __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                  const T* __restrict__ val, const T* __restrict__ x, int64_t v, int lane, T& su, T& sv,
-                                 T& sw, T& sq) {
+                                 T& sw, T& sq, T* st) {
+    constexpr int S = 4 + NS;
     const int64_t N = fg.N;
     const int32_t pN3 = (int32_t)(3 * N);
     const int32_t ru = rp[v], rv = rp[N + v], rw = rp[2 * N + v], rq = rp[3 * N + v];
@@ -138,38 +142,47 @@ __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restri
         const int32_t d1 = fg.rp[v + 1] - fg.rp[v] + 1;
         for (int32_t k = lane; k < d1; k += L) {
             const int32_t m = col[ru + k];
-            su += val[ru + k] * x[fidx<IL>(N, 0, m)];
-            sv += val[rv + k] * x[fidx<IL>(N, 1, m)];
-            sw += val[rw + k] * x[fidx<IL>(N, 2, m)];
+            su += val[ru + k] * x[fidx<IL, S>(N, 0, m)];
+            sv += val[rv + k] * x[fidx<IL, S>(N, 1, m)];
+            sw += val[rw + k] * x[fidx<IL, S>(N, 2, m)];
         }
         // p part of the velocity rows and velocity part of the continuity row: grad_coeff(v)
         for (int32_t q = lane; q < G; q += L) {
             const int32_t m = col[ru + d1 + q] - pN3;
-            const T pm = x[fidx<IL>(N, 3, m)];
+            const T pm = x[fidx<IL, S>(N, 3, m)];
             su += val[ru + d1 + q] * pm;
             sv += val[rv + d1 + q] * pm;
             sw += val[rw + d1 + q] * pm;
             if (lq > 1)
-                sq += val[rq + q] * x[fidx<IL>(N, 0, m)] + val[rq + G + q] * x[fidx<IL>(N, 1, m)] +
-                      val[rq + 2 * G + q] * x[fidx<IL>(N, 2, m)];
+                sq += val[rq + q] * x[fidx<IL, S>(N, 0, m)] + val[rq + G + q] * x[fidx<IL, S>(N, 1, m)] +
+                      val[rq + 2 * G + q] * x[fidx<IL, S>(N, 2, m)];
         }
     } else {
         if (lane == 0) {
-            su = val[ru] * x[fidx<IL>(N, 0, v)];
-            sv = val[rv] * x[fidx<IL>(N, 1, v)];
-            sw = val[rw] * x[fidx<IL>(N, 2, v)];
+            su = val[ru] * x[fidx<IL, S>(N, 0, v)];
+            sv = val[rv] * x[fidx<IL, S>(N, 1, v)];
+            sw = val[rw] * x[fidx<IL, S>(N, 2, v)];
         }
         if (lq > 1)
             for (int32_t q = lane; q < G; q += L) {
                 const int32_t m = col[rq + q];
-                sq += val[rq + q] * x[fidx<IL>(N, 0, m)] + val[rq + G + q] * x[fidx<IL>(N, 1, m)] +
-                      val[rq + 2 * G + q] * x[fidx<IL>(N, 2, m)];
+                sq += val[rq + q] * x[fidx<IL, S>(N, 0, m)] + val[rq + G + q] * x[fidx<IL, S>(N, 1, m)] +
+                      val[rq + 2 * G + q] * x[fidx<IL, S>(N, 2, m)];
             }
     }
     if (lq > 1) {
-        for (int32_t i = 3 * G + lane; i < lq; i += L) sq += val[rq + i] * x[fidx<IL>(N, 3, col[rq + i] - pN3)];
+        for (int32_t i = 3 * G + lane; i < lq; i += L) sq += val[rq + i] * x[fidx<IL, S>(N, 3, col[rq + i] - pN3)];
     } else if (lane == 0) {
-        sq = val[rq] * x[fidx<IL>(N, 3, v)];
+        sq = val[rq] * x[fidx<IL, S>(N, 3, v)];
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int64_t row = (int64_t)(4 + k) * N + v;
+        const int32_t rs = rp[row], ls = rp[row + 1] - rs;
+        const int32_t off = (int32_t)((4 + k) * N);
+        T s = 0;
+        for (int32_t i = lane; i < ls; i += L) s += val[rs + i] * x[fidx<IL, S>(N, 4 + k, col[rs + i] - off)];
+        st[k] = s;
     }
 }
 
@@ -183,49 +196,57 @@ __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restri
 // products.  With IL the vectors are vertex-interleaved, so the four unknowns of a gathered
 // vertex share one 32-byte (FP64) segment.  Rows of length 1 (velocity Dirichlet rows, the
 // reference-pressure row) are diagonal.
-template <class T, bool IL, int L>
+template <class T, bool IL, int L, int NS>
 __global__ void __launch_bounds__(BLK) k_spmv_flow(FlowSpmvGeom fg, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                    const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
                                                    const double* __restrict__ scale, const int* __restrict__ ctrl) {
+    constexpr int S = 4 + NS;
     if (ctrl && ctrl[0] == 0) return;
     const int64_t N = fg.N;
     const int64_t v = ((int64_t)xcd_block() * BLK + threadIdx.x) / L;
     const int lane = threadIdx.x % L;
     if (v >= N) return;
-    T su, sv, sw, sq;
-    flow_rows<T, IL, L>(fg, rp, col, val, x, v, lane, su, sv, sw, sq);
+    T su, sv, sw, sq, st[NS + 1];
+    flow_rows<T, IL, L, NS>(fg, rp, col, val, x, v, lane, su, sv, sw, sq, st);
     su = wave_sum<T, L>(su);
     sv = wave_sum<T, L>(sv);
     sw = wave_sum<T, L>(sw);
     sq = wave_sum<T, L>(sq);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) st[k] = wave_sum<T, L>(st[k]);
     if (lane == 0) {
         const T sc = scale ? (T)scale[0] : (T)1;
-        y[fidx<IL>(N, 0, v)] = su * sc;
-        y[fidx<IL>(N, 1, v)] = sv * sc;
-        y[fidx<IL>(N, 2, v)] = sw * sc;
-        y[fidx<IL>(N, 3, v)] = sq * sc;
+        y[fidx<IL, S>(N, 0, v)] = su * sc;
+        y[fidx<IL, S>(N, 1, v)] = sv * sc;
+        y[fidx<IL, S>(N, 2, v)] = sw * sc;
+        y[fidx<IL, S>(N, 3, v)] = sq * sc;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) y[fidx<IL, S>(N, 4 + k, v)] = st[k] * sc;
     }
 }
 
 // r = b - A x (FP64) of the flow system, waves striding over the vertices, and per-block
 // partial sums of r^2 and b^2
-template <bool IL>
+template <bool IL, int NS>
 __global__ void __launch_bounds__(BLK) k_residual_flow(FlowSpmvGeom fg, const int32_t* __restrict__ rp,
                                                        const int32_t* __restrict__ col, const double* __restrict__ val,
                                                        const double* __restrict__ x, const double* __restrict__ b,
                                                        double* __restrict__ r, double* __restrict__ part, int nblk) {
+    constexpr int S = 4 + NS;
     __shared__ double sh[BLK / WAVE];
     const int64_t N = fg.N;
     const int lane = threadIdx.x % WAVE;
     const int64_t waves = (int64_t)gridDim.x * (BLK / WAVE);
     double rr = 0.0, bb = 0.0;
     for (int64_t v = ((int64_t)xcd_block() * BLK + threadIdx.x) / WAVE; v < N; v += waves) {
-        double s[4];
-        flow_rows<double, IL>(fg, rp, col, val, x, v, lane, s[0], s[1], s[2], s[3]);
-        for (int c = 0; c < 4; ++c) s[c] = wave_sum(s[c]);
+        double s[S];
+        flow_rows<double, IL, WAVE, NS>(fg, rp, col, val, x, v, lane, s[0], s[1], s[2], s[3], s + 4);
+#pragma unroll
+        for (int c = 0; c < S; ++c) s[c] = wave_sum(s[c]);
         if (lane == 0)
-            for (int c = 0; c < 4; ++c) {
-                const int64_t i = fidx<IL>(N, c, v);
+#pragma unroll
+            for (int c = 0; c < S; ++c) {
+                const int64_t i = fidx<IL, S>(N, c, v);
                 const double bi = b[i];
                 const double ri = bi - s[c];
                 r[i] = ri;
@@ -240,14 +261,14 @@ __global__ void __launch_bounds__(BLK) k_residual_flow(FlowSpmvGeom fg, const in
 
 // comp-major <-> vertex-interleaved copies of an n = 4N vector
 template <class T>
-__global__ void __launch_bounds__(BLK) k_to_interleaved(int64_t N, const T* __restrict__ in, T* __restrict__ out) {
+__global__ void __launch_bounds__(BLK) k_to_interleaved(int64_t N, int S, const T* __restrict__ in, T* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i < 4 * N) out[4 * (i % N) + i / N] = in[i];
+    if (i < S * N) out[S * (i % N) + i / N] = in[i];
 }
 template <class T>
-__global__ void __launch_bounds__(BLK) k_from_interleaved(int64_t N, const T* __restrict__ in, T* __restrict__ out) {
+__global__ void __launch_bounds__(BLK) k_from_interleaved(int64_t N, int S, const T* __restrict__ in, T* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i < 4 * N) out[i] = in[4 * (i % N) + i / N];
+    if (i < S * N) out[i] = in[S * (i % N) + i / N];
 }
 
 // r = b - A x (FP64) and per-block partial sums of r^2 and b^2 (grid-stride over rows)
@@ -885,17 +906,24 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
             const char* e = getenv("VFVM_FLOW_LPV");
             return (e && atoi(e) == 64) ? 64 : 32;
         }();
-        if (sy.flow_layout == 2) {
-            if (lpv == 32)
-                launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 32>, dim3(grid_for(c.g.N * 32)), dim3(BLK), fg, rp, cl,
-                             val, x, y, scale, ctrl);
-            else
-                launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 64>, dim3(grid_for(c.g.N * 64)), dim3(BLK), fg, rp, cl,
-                             val, x, y, scale, ctrl);
-        } else {
-            launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, false, 64>, dim3(grid_for(c.g.N * WAVE)), dim3(BLK), fg, rp, cl,
-                         val, x, y, scale, ctrl);
+#define VFVM_FLOW_SPMV(NS)                                                                                                   \
+    if (sy.flow_layout == 2) {                                                                                               \
+        if (lpv == 32)                                                                                                       \
+            launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 32, NS>, dim3(grid_for(c.g.N * 32)), dim3(BLK), fg, rp, cl, \
+                         val, x, y, scale, ctrl);                                                                            \
+        else                                                                                                                 \
+            launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 64, NS>, dim3(grid_for(c.g.N * 64)), dim3(BLK), fg, rp, cl, \
+                         val, x, y, scale, ctrl);                                                                            \
+    } else {                                                                                                                 \
+        launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, false, 64, NS>, dim3(grid_for(c.g.N * WAVE)), dim3(BLK), fg, rp,  \
+                     cl, val, x, y, scale, ctrl);                                                                            \
+    }
+        switch (sy.flow_ns) {
+            case 0: VFVM_FLOW_SPMV(0) break;
+            case 1: VFVM_FLOW_SPMV(1) break;
+            default: VFVM_FLOW_SPMV(2) break;
         }
+#undef VFVM_FLOW_SPMV
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }
@@ -915,10 +943,19 @@ int residual_launch(Ctx& c, const SysView& sy, int lpr, const double* x, double*
     unsigned grid = (unsigned)nblk_r;
     if (sy.flow_layout) {
         FlowSpmvGeom fg{c.g.N, c.g.NP, c.g.row_ptr.p, c.g.gc_ptr.p};
-        if (sy.flow_layout == 2)
-            hipLaunchKernelGGL((k_residual_flow<true>), dim3(grid), dim3(BLK), 0, c.stream, fg, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r);
-        else
-            hipLaunchKernelGGL((k_residual_flow<false>), dim3(grid), dim3(BLK), 0, c.stream, fg, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r);
+#define VFVM_FLOW_RES(NS)                                                                                                      \
+    if (sy.flow_layout == 2)                                                                                                   \
+        hipLaunchKernelGGL((k_residual_flow<true, NS>), dim3(grid), dim3(BLK), 0, c.stream, fg, sy.row, sy.col, sy.vals, x, sy.b, r, \
+                           part, nblk_r);                                                                                      \
+    else                                                                                                                       \
+        hipLaunchKernelGGL((k_residual_flow<false, NS>), dim3(grid), dim3(BLK), 0, c.stream, fg, sy.row, sy.col, sy.vals, x, sy.b,   \
+                           r, part, nblk_r);
+        switch (sy.flow_ns) {
+            case 0: VFVM_FLOW_RES(0) break;
+            case 1: VFVM_FLOW_RES(1) break;
+            default: VFVM_FLOW_RES(2) break;
+        }
+#undef VFVM_FLOW_RES
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }
@@ -1254,7 +1291,7 @@ CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv
                           int* status) {
     const T* lu = pc.lu;
     const IluData* I = lu ? sy.ilu : nullptr;
-    std::array<uint64_t, 22> key = {(uint64_t)(uintptr_t)pc.bj, (uint64_t)pc.nb + (pc.il ? 256u : 0u) + 512u * (uint64_t)sy.flow_layout,
+    std::array<uint64_t, 22> key = {(uint64_t)(uintptr_t)pc.bj, (uint64_t)pc.nb + (pc.il ? 256u : 0u) + 512u * (uint64_t)(sy.flow_layout + 4 * sy.flow_ns),
                                     (uint64_t)(uintptr_t)V,
                                     (uint64_t)(uintptr_t)wv, (uint64_t)(uintptr_t)zv,
                                     (uint64_t)(uintptr_t)aval, (uint64_t)(uintptr_t)lu, (uint64_t)(uintptr_t)w.S,
@@ -1345,6 +1382,7 @@ SysView primal_view(Ctx& c) {
     s.h_row = &c.h_mrow; s.h_col = &c.h_mcol;
     s.ilu = &c.ilu;
     s.flow_layout = c.flow ? 1 : 0;
+    s.flow_ns = c.flow ? (int)c.fs.blk_unk.size() : 0;
     return s;
 }
 
@@ -1539,8 +1577,9 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
 
     // permuted system and preconditioner set-up (inside the timed solve)
     if (il) {
-        hipLaunchKernelGGL((k_to_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, sy0.b, c.bp64.p);
-        hipLaunchKernelGGL((k_to_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, (const double*)x0, c.xp64.p);
+        const int S = 4 + sy0.flow_ns;
+        hipLaunchKernelGGL((k_to_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, S, sy0.b, c.bp64.p);
+        hipLaunchKernelGGL((k_to_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, S, (const double*)x0, c.xp64.p);
     }
     if (ilu) {
         const IluData& I = *sy0.ilu;
@@ -1654,7 +1693,8 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
         ++cycles;
     }
     if (ilu) hipLaunchKernelGGL((k_scatter_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy0.ilu->perm.p, (const double*)x, x0);
-    if (il) hipLaunchKernelGGL((k_from_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, (const double*)x, x0);
+    if (il)
+        hipLaunchKernelGGL((k_from_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, 4 + sy0.flow_ns, (const double*)x, x0);
     VFVM_HIP_TRY(hipEventRecord(ev.e[2], c.stream));
     VFVM_HIP_TRY(hipEventSynchronize(ev.e[2]));
     float ms_all = 0.f, ms_setup = 0.f;
