@@ -1268,19 +1268,22 @@ def scaled_residual_scalar(x_old, x_new):
 def heatflow_solve(g, gc, vel0, pres0, T0, den, visc, fce_of, bnd_vel, bnd_nid, pref, heat_rows, t_dirichlet,
                    max_iter, tol, damp, dt=0.0, num_ts=0):
     """SteadyBase::solve / TransientBase::solve (src/steady/steady_base.rs:16-135,
-    src/transient/transient_base.rs:16-160) for SteadyHeatFlow / TransientHeatFlow
-    (src/steady/core/steady_heatflow.rs:154-536) with constant density and viscosity and a
-    body force that depends on the temperature through a host closure (Boussinesq; the
-    reference evaluates such function fields in _update_func before every assembly).
+    src/transient/transient_base.rs:16-160) for SteadyHeatFlow / TransientHeatFlow /
+    TransientHeatMassFlow (src/steady/core/steady_heatflow.rs:154-536,
+    src/transient/core/transient_heatmassflow.rs:248-600) with constant density and
+    viscosity and a body force that depends on the scalar unknowns through a host closure
+    (Boussinesq; the reference evaluates such function fields in _update_func before every
+    assembly).
 
-    The solution vector is [vel (3N), pres (N), T (N)] (initial_start, :154-186).  Before
-    every assembly: fce = fce_of(T), gradients, update_flow (Rhie-Chow flux), then the flow
-    rows (assemble_flow) and the temperature rows heat_rows(T, T_prev, f) -> (rows, cols,
-    vals, b) with the flow data f of the same iterate (its flux and density), shifted by 4N.
-    t_dirichlet(T) returns T with the Dirichlet values loaded (apply_initial).  SuperLU
-    solves, damping, the scaled residual as the max over the three fields.  dt = 0: steady;
-    else num_ts implicit-Euler steps with the previous state updated after each step.
-    Returns vel, pres, T and the Picard iterations (total)."""
+    The solution vector is [vel (3N), pres (N), S] with S the K scalar unknowns (temperature,
+    then concentrations; K = len(T0) / N) (initial_start, steady_heatflow.rs:154-186).
+    Before every assembly: fce = fce_of(S), gradients, update_flow (Rhie-Chow flux), then the
+    flow rows (assemble_flow) and the scalar rows heat_rows(S, S_prev, f) -> (rows, cols,
+    vals, b) over K N rows with the flow data f of the same iterate (its flux and density),
+    shifted by 4N.  t_dirichlet(S) returns S with the Dirichlet values loaded
+    (apply_initial).  SuperLU solves, damping, the scaled residual as the max over the
+    fields.  dt = 0: steady; else num_ts implicit-Euler steps with the previous state
+    updated after each step.  Returns vel, pres, S and the Picard iterations (total)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
     N, NP = g["N"], g["num_pts"]
@@ -1290,6 +1293,7 @@ def heatflow_solve(g, gc, vel0, pres0, T0, den, visc, fce_of, bnd_vel, bnd_nid, 
         nodes = NP + np.asarray(bnd_nid[bid], np.int64)
         vel[:, nodes] = np.asarray(val).reshape(3, -1)
     T = t_dirichlet(np.array(T0, np.float64).copy())
+    K = T.size // N
     vel_prev, T_prev = vel.copy(), T.copy()
     transient = dt > 0.0
 
@@ -1302,15 +1306,16 @@ def heatflow_solve(g, gc, vel0, pres0, T0, den, visc, fce_of, bnd_vel, bnd_nid, 
         rows = np.concatenate([np.asarray(r, np.int64), 4 * N + np.asarray(rh, np.int64)])
         cols = np.concatenate([np.asarray(c, np.int64), 4 * N + np.asarray(ch, np.int64)])
         vals = np.concatenate([np.asarray(v, np.float64), np.asarray(vh, np.float64)])
-        rp, col, val = coo_to_csr(rows, cols, vals, 5 * N)
-        return sp.csr_matrix((val, col, rp), shape=(5 * N, 5 * N)).tocsc(), np.concatenate([b, bh])
+        n = (4 + K) * N
+        rp, col, val = coo_to_csr(rows, cols, vals, n)
+        return sp.csr_matrix((val, col, rp), shape=(n, n)).tocsc(), np.concatenate([b, bh])
 
     def residual(xo, xn):
-        rf = scaled_residual_flow(N, xo[:4 * N], xn[:4 * N], den)
-        rt = scaled_residual_scalar(xo[4 * N:], xn[4 * N:])
-        if np.isnan(rf) or np.isnan(rt):
+        r = [scaled_residual_flow(N, xo[:4 * N], xn[:4 * N], den)]
+        r += [scaled_residual_scalar(xo[(4 + k) * N:(5 + k) * N], xn[(4 + k) * N:(5 + k) * N]) for k in range(K)]
+        if any(np.isnan(x) for x in r):
             return float("nan")
-        return max(rf, rt)
+        return max(r)
 
     total = 0
     x_iter = np.concatenate([vel.ravel(), pres, T])

@@ -315,94 +315,161 @@ def heat_flow_c5like(mesh, transient=False):
 
 
 class HeatFlowProblem:
-    """SteadyHeatFlow / TransientHeatFlow composition (steady_heatflow.rs:154-536) of one domain:
-    the flow of a FlowProblem and a temperature unknown carried by the flow's own Rhie-Chow
-    flux (OpSclDomAdvection with weight rho c_p, conductivity k and the flow's density),
-    with a heat source q, T = t_hot on x = 0 and t_cold on x = 1, zero heat flux on the other
-    faces (diffusion, source, Neumann and advection operators there), and a body force
-    fce = (0, 0, gamma (T - t_ref)) evaluated on the host before every assembly (Boussinesq).
-    transient: the velocity time operator first and OpSclDomTime / OpSclBndTime (weight
-    rho c_p) ahead of the temperature operators of each row class."""
+    """SteadyHeatFlow / TransientHeatFlow / TransientHeatMassFlow composition
+    (steady_heatflow.rs:154-536, transient_heatmassflow.rs:248-600) of one domain: the flow of
+    a FlowProblem and a temperature unknown carried by the flow's own Rhie-Chow flux
+    (OpSclDomAdvection with weight rho c_p, conductivity k and the flow's density), with a
+    heat source q, T = t_hot on x = 0 and t_cold on x = 1, zero heat flux on the other faces
+    (diffusion, source, Neumann and advection operators there), and a body force
+    fce = (0, 0, gamma (T - t_ref)) evaluated on the host before every assembly
+    (Boussinesq).  species: a concentration unknown C after T (diffusivity D, the flow's flux
+    through OpSclDomAdvectionUnity, C = 1 on x = 0 and 0 on x = 1, zero flux elsewhere).
+    transient: the velocity time operator first, OpSclDomTimePrevious / OpSclBndTimePrevious
+    (weight rho c_p for T, the density for C) ahead of the scalar operators of each row class.
+    Ids live in the dict `hid` once the fields are uploaded (or assigned for the oracle)."""
 
-    def __init__(self, fp: FlowProblem, rcp, k, q, gamma=0.0, t_ref=0.5, t_hot=1.0, t_cold=0.0, T0=0.0, transient=False):
+    def __init__(self, fp: FlowProblem, rcp, k, q, gamma=0.0, t_ref=0.5, t_hot=1.0, t_cold=0.0, T0=0.0, transient=False,
+                 species=False, D=0.1, C0=0.0):
         self.fp = fp
         self.mesh = fp.mesh
         N = self.N = fp.N
         b = lambda v: np.broadcast_to(np.asarray(v, np.float64), (N,)).copy()
         self.rcp, self.k, self.q, self.T0 = b(rcp), b(k), b(q), b(T0)
+        self.D, self.C0 = b(D), b(C0)
         self.gamma, self.t_ref, self.t_hot, self.t_cold = float(gamma), float(t_ref), float(t_hot), float(t_cold)
         self.transient = transient
+        self.species = species
+        self.nscl = 2 if species else 1
         fp.transient = transient
         fp.const = fp.const - {"fce"}  # the body force is a function field
 
-    def fce_of(self, T):
+    def fce_of(self, S):
+        """Body force from the scalar unknowns S (T first)."""
+        T = S[:self.N]
         return np.stack([np.zeros(self.N), np.zeros(self.N), self.gamma * (T - self.t_ref)])
 
-    def heat_ops(self, ids):
-        T, rcp, k, q, flw, t0, t1, fl = ids
-        o = [(vfvm.OP_SCL_DOM_TIME, 0, T, rcp, -1)] if self.transient else []
+    def host_ids(self):
+        """Ids as to_device assigns them after FlowProblem.to_device's fields (oracle-only use)."""
+        self.hid = dict(T=2, rcp=3, k=4, q=5, flw=0, den=1, t0=0, t1=1, fl={Y0: 2, Y1: 3, Z0: 4, Z1: 5}, C=6, D=7, c0=6,
+                        c1=7, cfl={Y0: 8, Y1: 9, Z0: 10, Z1: 11})
+        return self.hid
+
+    def heat_ops(self, h):
+        T, rcp, k, q, flw = h["T"], h["rcp"], h["k"], h["q"], h["flw"]
+        o = [(vfvm.OP_SCL_DOM_TIME_PREVIOUS, 0, T, rcp, -1)] if self.transient else []
         o += [(vfvm.OP_SCL_DOM_DIFFUSION, 0, T, k, T), (vfvm.OP_SCL_DOM_SOURCE, 0, T, q, -1),
               (vfvm.OP_SCL_DOM_ADVECTION, 0, T, rcp, k, flw),
-              (vfvm.OP_SCL_BND_DIRICHLET, X0, T, t0, -1), (vfvm.OP_SCL_BND_DIRICHLET, X1, T, t1, -1)]
+              (vfvm.OP_SCL_BND_DIRICHLET, X0, T, h["t0"], -1), (vfvm.OP_SCL_BND_DIRICHLET, X1, T, h["t1"], -1)]
         for bb in (Y0, Y1, Z0, Z1):
             if self.transient:
-                o.append((vfvm.OP_SCL_BND_TIME, bb, T, rcp, -1))
+                o.append((vfvm.OP_SCL_BND_TIME_PREVIOUS, bb, T, rcp, -1))
             o += [(vfvm.OP_SCL_BND_DIFFUSION, bb, T, k, T), (vfvm.OP_SCL_BND_SOURCE, bb, T, q, -1),
-                  (vfvm.OP_SCL_BND_NEUMANN, bb, T, fl[bb], -1), (vfvm.OP_SCL_BND_ADVECTION, bb, T, rcp, k, flw)]
+                  (vfvm.OP_SCL_BND_NEUMANN, bb, T, h["fl"][bb], -1), (vfvm.OP_SCL_BND_ADVECTION, bb, T, rcp, k, flw)]
+        return o
+
+    def species_ops(self, h):
+        C, D, flw, den = h["C"], h["D"], h["flw"], h["den"]
+        o = [(vfvm.OP_SCL_DOM_TIME_PREVIOUS, 0, C, den, -1)] if self.transient else []
+        o += [(vfvm.OP_SCL_DOM_DIFFUSION, 0, C, D, C), (vfvm.OP_SCL_DOM_ADVECTION_UNITY, 0, C, D, flw),
+              (vfvm.OP_SCL_BND_DIRICHLET, X0, C, h["c0"], -1), (vfvm.OP_SCL_BND_DIRICHLET, X1, C, h["c1"], -1)]
+        for bb in (Y0, Y1, Z0, Z1):
+            if self.transient:
+                o.append((vfvm.OP_SCL_BND_TIME_PREVIOUS, bb, C, den, -1))
+            o += [(vfvm.OP_SCL_BND_DIFFUSION, bb, C, D, C), (vfvm.OP_SCL_BND_NEUMANN, bb, C, h["cfl"][bb], -1),
+                  (vfvm.OP_SCL_BND_ADVECTION_UNITY, bb, C, D, flw)]
         return o
 
     def to_device(self, ctx: vfvm.Context):
         """Uploads every field; returns the coupled operator list in the reference's
-        assemble_matrix order (flow, heat, then the reference pressure)."""
+        assemble_matrix order (flow, heat, species, then the reference pressure)."""
         fops = self.fp.to_device(ctx)
         self.fp.fce = self.fce_of(self.T0)
         ctx.vecdom_set(self.fp.ids[4], self.fp.fce)
-        T = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0, self.T0)
-        rcp = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, self.rcp)
-        k = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, self.k)
-        q = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, self.q)
+        h = dict(flw=self.fp.ids[5], den=self.fp.ids[2])
+        h["T"] = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0, self.T0)
+        h["rcp"] = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, self.rcp)
+        h["k"] = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, self.k)
+        h["q"] = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, self.q)
         cnt = lambda bb: len(self.mesh.bnd_nodes(bb))
-        t0 = ctx.sclbnd_add(X0, values=np.full(cnt(X0), self.t_hot))
-        t1 = ctx.sclbnd_add(X1, values=np.full(cnt(X1), self.t_cold))
-        fl = {bb: ctx.sclbnd_add(bb, values=np.zeros(cnt(bb))) for bb in (Y0, Y1, Z0, Z1)}
-        self.hids = (T, rcp, k, q, self.fp.ids[5], t0, t1, fl)
-        hops = self.heat_ops(self.hids)
+        h["t0"] = ctx.sclbnd_add(X0, values=np.full(cnt(X0), self.t_hot))
+        h["t1"] = ctx.sclbnd_add(X1, values=np.full(cnt(X1), self.t_cold))
+        h["fl"] = {bb: ctx.sclbnd_add(bb, values=np.zeros(cnt(bb))) for bb in (Y0, Y1, Z0, Z1)}
+        sops = []
+        if self.species:
+            h["C"] = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0, self.C0)
+            h["D"] = ctx.scldom_add(vfvm.FIELD_FUNCTION, 0.0, self.D)
+            h["c0"] = ctx.sclbnd_add(X0, values=np.ones(cnt(X0)))
+            h["c1"] = ctx.sclbnd_add(X1, values=np.zeros(cnt(X1)))
+            h["cfl"] = {bb: ctx.sclbnd_add(bb, values=np.zeros(cnt(bb))) for bb in (Y0, Y1, Z0, Z1)}
+            sops = self.species_ops(h)
+        self.hid = h
         pref = [o for o in fops if o[0] == vfvm.OP_SCL_DOM_PRESSURE_REFERENCE]
         rest = [o for o in fops if o[0] != vfvm.OP_SCL_DOM_PRESSURE_REFERENCE]
-        return rest + hops + pref
+        return rest + self.heat_ops(h) + sops + pref
 
     def hook(self, ctx):
         """The host closure: fce := fce_of(T) from the device's current temperature."""
-        fce_id, T_id = self.fp.ids[4], self.hids[0]
+        fce_id, T_id = self.fp.ids[4], self.hid["T"]
         return lambda t: ctx.vecdom_set(fce_id, self.fce_of(ctx.scldom_get(T_id)))
 
-    # ---- oracle
-    def oracle_heat_rows(self, g, dt=0.0):
-        """heat_rows(T, T_prev, f) for O.heat_flow_solve / the assembled temperature rows."""
-        T_id, rcp_id, k_id, q_id, flw, t0, t1, fl = self.hids
-        bnd_nid = [self.mesh.bnd_nodes(bb) for bb in range(self.mesh.num_bnd)]
-        fixed = {rcp_id: (self.rcp, O.update_grad(g, self.rcp)), k_id: (self.k, O.update_grad(g, self.k)),
-                 q_id: (self.q, O.update_grad(g, self.q))}
-        sb = {t0: np.full(len(bnd_nid[X0]), self.t_hot), t1: np.full(len(bnd_nid[X1]), self.t_cold)}
-        sb.update({fl[bb]: np.zeros(len(bnd_nid[bb])) for bb in (Y0, Y1, Z0, Z1)})
-        ops = [O.Op(*o) for o in self.heat_ops(self.hids)]
+    def scalar_values(self, ctx):
+        """The scalar unknowns' current values stacked [T, C]."""
+        S = [ctx.scldom_get(self.hid["T"])]
+        if self.species:
+            S.append(ctx.scldom_get(self.hid["C"]))
+        return np.concatenate(S)
 
-        def rows(T, T_prev, f):
+    # ---- oracle
+    def S0(self):
+        return np.concatenate([self.T0, self.C0]) if self.species else self.T0.copy()
+
+    def oracle_heat_rows(self, g, dt=0.0):
+        """rows(S, S_prev, f) -> (rows, cols, vals, b) of the scalar unknowns' rows (T block, then
+        C block at offset N) for O.heatflow_solve and the coupled assembly."""
+        h = self.hid
+        N = self.N
+        bnd_nid = [self.mesh.bnd_nodes(bb) for bb in range(self.mesh.num_bnd)]
+        gr = lambda v: (v, O.update_grad(g, v))
+        fixed = {h["rcp"]: gr(self.rcp), h["k"]: gr(self.k), h["q"]: gr(self.q)}
+        sb = {h["t0"]: np.full(len(bnd_nid[X0]), self.t_hot), h["t1"]: np.full(len(bnd_nid[X1]), self.t_cold)}
+        sb.update({h["fl"][bb]: np.zeros(len(bnd_nid[bb])) for bb in (Y0, Y1, Z0, Z1)})
+        hops = [O.Op(*o) for o in self.heat_ops(h)]
+        if self.species:
+            fixed[h["D"]] = gr(self.D)
+            sb.update({h["c0"]: np.ones(len(bnd_nid[X0])), h["c1"]: np.zeros(len(bnd_nid[X1]))})
+            sb.update({h["cfl"][bb]: np.zeros(len(bnd_nid[bb])) for bb in (Y0, Y1, Z0, Z1)})
+            cops = [O.Op(*o) for o in self.species_ops(h)]
+
+        def rows(S, S_prev, f):
+            flw = h["flw"]
             scl = dict(fixed)
-            scl[T_id] = (T, O.update_grad(g, T))
-            prev = {T_id: T_prev, rcp_id: self.rcp}
-            return O.assemble(g, ops, scl, sb, {flw: f["flux"]}, bnd_nid, dt=dt, value_prev=prev,
-                              flow_den={flw: (f["den"], f["deng"])})
+            scl[h["T"]] = gr(S[:N])
+            scl[h["den"]] = (f["den"], f["deng"])
+            prev = {h["T"]: S_prev[:N], h["rcp"]: self.rcp, h["den"]: f["den"]}
+            r, c, v, b = O.assemble(g, hops, scl, sb, {flw: f["flux"]}, bnd_nid, dt=dt, value_prev=prev,
+                                    flow_den={flw: (f["den"], f["deng"])})
+            if not self.species:
+                return r, c, v, b
+            scl[h["C"]] = gr(S[N:])
+            prev[h["C"]] = S_prev[N:]
+            r2, c2, v2, b2 = O.assemble(g, cops, scl, sb, {flw: f["flux"]}, bnd_nid, dt=dt, value_prev=prev,
+                                        flow_den={flw: (f["den"], f["deng"])})
+            return (np.concatenate([np.asarray(r), N + np.asarray(r2)]), np.concatenate([np.asarray(c), N + np.asarray(c2)]),
+                    np.concatenate([np.asarray(v), np.asarray(v2)]), np.concatenate([b, b2]))
         return rows
 
     def oracle_t_dirichlet(self):
-        NP = self.mesh.num_pts
+        NP, N = self.mesh.num_pts, self.N
         n0 = NP + self.mesh.bnd_nodes(X0).astype(np.int64)
         n1 = NP + self.mesh.bnd_nodes(X1).astype(np.int64)
 
-        def load(T):
-            T = T.copy()
-            T[n0] = self.t_hot
-            T[n1] = self.t_cold
-            return T
+        def load(S):
+            S = S.copy()
+            S[n0] = self.t_hot
+            S[n1] = self.t_cold
+            if self.species:
+                S[N + n0] = 1.0
+                S[N + n1] = 0.0
+            return S
         return load
+

@@ -5,7 +5,8 @@ forms and to the decoupled solvers.
 Pins (each fails for a dropped coupling, a wrong shift of the temperature block, the flux of
 another iterate or a wrong buoyancy sign):
   conduction:   no flow and no buoyancy give T = t_hot + (t_cold - t_hot) x exactly (the
-                diffusion operator is exact for linear fields), velocity zero.
+                diffusion operator is exact for linear fields), velocity zero; likewise the
+                concentration of a second scalar block (C = 1 - x).
   decoupled:    gamma = 0 with a moving lid: the velocity and pressure equal picard_flow's,
                 and T equals a direct solve of the scalar system with the converged flow's
                 flux (the temperature rows read the flux of the same iterate).
@@ -20,9 +21,7 @@ import scipy.sparse.linalg as spla
 
 import vfvm
 import vfvm_oracle as O
-from problems import X0, X1, Y0, Y1, Z0, Z1, HeatFlowProblem, lid_cavity
-
-HOST_IDS = (0, 1, 2, 3, 0, 0, 1, {Y0: 2, Y1: 3, Z0: 4, Z1: 5})
+from problems import HeatFlowProblem, lid_cavity
 
 
 @pytest.fixture(scope="module")
@@ -41,10 +40,10 @@ def gc5(g5):
 
 
 def _solve(hp, g, gc, max_iter=60, tol=1e-9, damp=1.0):
-    hp.hids = HOST_IDS
+    hp.host_ids()
     fp = hp.fp
     bnd_nid = [hp.mesh.bnd_nodes(b) for b in range(6)]
-    return O.heatflow_solve(g, gc, fp.vel, fp.pres, hp.T0, fp.den, fp.visc, hp.fce_of, fp.bnd_vel, bnd_nid, fp.pref,
+    return O.heatflow_solve(g, gc, fp.vel, fp.pres, hp.S0(), fp.den, fp.visc, hp.fce_of, fp.bnd_vel, bnd_nid, fp.pref,
                             hp.oracle_heat_rows(g), hp.oracle_t_dirichlet(), max_iter, tol, damp)
 
 
@@ -55,13 +54,18 @@ def test_scaled_residual_scalar_closed_form():
     assert np.isnan(O.scaled_residual_scalar(xo, np.array([np.nan, 0.0])))
 
 
-def test_conduction_limit_linear_profile(mesh5, g5, gc5):
+@pytest.mark.parametrize("species", [False, True])
+def test_conduction_limit_linear_profile(mesh5, g5, gc5, species):
     fp = lid_cavity(mesh5, rho=1.0, mu=0.1, lid=0.0)
-    hp = HeatFlowProblem(fp, rcp=1.0, k=0.2, q=0.0, gamma=0.0)
-    vel, pres, T, its = _solve(hp, g5, gc5)
+    hp = HeatFlowProblem(fp, rcp=1.0, k=0.2, q=0.0, gamma=0.0, species=species, D=0.3)
+    vel, pres, S, its = _solve(hp, g5, gc5)
     x = mesh5.get("pos")[:, 0]
+    N = mesh5.num_tot
+    assert S.size == (2 if species else 1) * N
     assert np.max(np.abs(vel)) <= 1e-12
-    assert np.max(np.abs(T - (1.0 - x))) <= 1e-11
+    assert np.max(np.abs(S[:N] - (1.0 - x))) <= 1e-11
+    if species:
+        assert np.max(np.abs(S[N:] - (1.0 - x))) <= 1e-11
 
 
 def test_decoupled_matches_flow_and_scalar_solves(mesh5, g5, gc5):
@@ -86,6 +90,7 @@ def test_buoyancy_drives_convection(mesh5, g5, gc5):
     hp = HeatFlowProblem(fp, rcp=1.0, k=0.1, q=0.0, gamma=20.0)
     vel, pres, T, its = _solve(hp, g5, gc5, tol=1e-9, damp=0.8)
     pos = mesh5.get("pos")
+    assert T.size == mesh5.num_tot
     NP = mesh5.num_pts
     x = pos[:NP, 0]
     w = vel[2][:NP]
