@@ -167,6 +167,161 @@ def build_lid_cavity(vfvm, n, rho=1.0, mu=0.01, lid=1.0):
     return mesh, ctx, ops, vel, pres
 
 
+def timed_block_solves(vfvm, ctx, ndof, tol, restart, inner_tol, steps, label, max_outer=2000, events_first=False):
+    """The assembled coupled system solved from x0 = 0 to tol with GMRES(restart) + block
+    Jacobi, FP64 and FP32-inner/FP64-refined: one untimed solve, `steps` timed solves, then
+    one solve with per-kernel HIP events (per-class averages).  events_first: the solve with
+    per-kernel events is the untimed first solve (one solve fewer per variant)."""
+    import torch
+    x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
+    xp = x.data_ptr()
+    out = {}
+
+    def kernel_pass(kw):
+        ctx.timing_enable(-1)
+        ctx.timing_reset()
+        x.zero_()
+        torch.cuda.synchronize()
+        ctx.solve_device(xp, **kw)
+        torch.cuda.synchronize()
+        ctx.timing_enable(0)
+        kern = {}
+        for name, k in (("spmv", vfvm.K_SPMV), ("mdot", vfvm.K_MDOT), ("update", vfvm.K_UPDATE), ("bjacobi", vfvm.K_PRECOND)):
+            nl, ms, by = ctx.timing_get(k)
+            if nl:
+                kern[name] = dict(launches=nl, avg_us=1e3 * ms / nl, total_ms=ms, bytes_per_launch=by / nl,
+                                  gbs=by / (ms * 1e-3) / 1e9)
+        return kern
+
+    for var in ("fp64", "mixed"):
+        prec = vfvm.FP64 if var == "fp64" else vfvm.FP32_REFINED
+        kw = dict(tol=tol, restart=restart, max_outer=max_outer, precision=prec, precond=vfvm.PRECOND_BJACOBI,
+                  inner_tol=inner_tol, ortho=1)
+        if events_first:
+            kern = kernel_pass(kw)
+        else:
+            ctx.timing_enable(0)
+            x.zero_()
+            torch.cuda.synchronize()
+            ctx.solve_device(xp, **kw)
+        ts = []
+        for _ in range(steps):
+            x.zero_()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            st = ctx.solve_device(xp, **kw)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t1)
+        if not events_first:
+            kern = kernel_pass(kw)
+        ms = 1e3 * sum(ts) / len(ts)
+        progress(f"{label} {var}: {ms:.1f} ms per solve ({st.iterations} steps)")
+        out[var] = dict(ms_per_step=ms, dof_per_s=ndof / (ms * 1e-3), iters=st.iterations, refinements=st.refinements,
+                        rel_residual=st.rel_residual, setup_s=st.setup_seconds, kernels=kern)
+    return out
+
+
+def build_heat_mass_flow(vfvm, n, Ra, Pr, Sc):
+    """Natural convection in the unit cube (TransientHeatMassFlow composition,
+    transient_heatmassflow.rs:248-600) in free-fall units (velocity sqrt(g beta dT L), time
+    L / that velocity): rho = 1, mu = sqrt(Pr / Ra), rho c_p = 1, k = 1 / sqrt(Ra Pr), species
+    diffusivity mu / Sc, body force (0, 0, T - 1/2) set from T before every assembly
+    (Boussinesq), T = C = 1 on x = 0 and 0 on x = 1, zero heat / species flux and no-slip on
+    every wall, reference pressure 0 at node 0; fluid at rest, T = C = 0."""
+    import numpy as np
+    mesh = vfvm.Mesh.lattice(n, 0, 0.3)
+    mesh.build_grad_coeff()
+    ctx = vfvm.Context(0)
+    ctx.upload(mesh)
+    N = mesh.num_tot
+    vel = ctx.vecdom_add(vfvm.FIELD_UNKNOWN, cnst3=np.zeros(3))
+    pres = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0)
+    den = ctx.scldom_add(vfvm.FIELD_CONSTANT, 1.0)
+    mu = (Pr / Ra) ** 0.5
+    visc = ctx.scldom_add(vfvm.FIELD_CONSTANT, mu)
+    fce = ctx.vecdom_add(vfvm.FIELD_FUNCTION, cnst3=np.zeros(3))
+    flw = ctx.sclflw_add(vel, pres, den, visc, fce)
+    T = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0)
+    rcp = ctx.scldom_add(vfvm.FIELD_CONSTANT, 1.0)
+    k = ctx.scldom_add(vfvm.FIELD_CONSTANT, (Ra * Pr) ** -0.5)
+    q = ctx.scldom_add(vfvm.FIELD_CONSTANT, 0.0)
+    C = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0)
+    D = ctx.scldom_add(vfvm.FIELD_CONSTANT, mu / Sc)
+    ops = [(vfvm.OP_VEC_DOM_TIME, 0, vel, den, -1),
+           (vfvm.OP_VEC_DOM_ADVECTION, 0, vel, den, visc, flw), (vfvm.OP_VEC_DOM_GRADIENT, 0, vel, pres, -1),
+           (vfvm.OP_VEC_DOM_DIFFUSION, 0, vel, visc, -1), (vfvm.OP_VEC_DOM_TRANSPOSE, 0, vel, visc, -1),
+           (vfvm.OP_VEC_DOM_DILATATORY, 0, vel, visc, -1), (vfvm.OP_VEC_DOM_SOURCE, 0, vel, fce, -1),
+           (vfvm.OP_SCL_DOM_DIVERGENCE, 0, pres, vel, den), (vfvm.OP_SCL_DOM_PRESSURE_STABILIZER, 0, pres, den, fce, flw)]
+    for b in range(6):
+        vb = ctx.vecbnd_add(b, cnst3=np.zeros(3))
+        ops += [(vfvm.OP_VEC_BND_DIRICHLET, b, vel, vb, -1), (vfvm.OP_SCL_BND_DIVERGENCE, b, pres, vel, den),
+                (vfvm.OP_SCL_BND_PRESSURE_STABILIZER, b, pres, den, fce, flw)]
+    # heat, then species (the reference's unknown order [vel, pres, temp, conc])
+    for u, w, dif, src, adv in ((T, rcp, k, q, vfvm.OP_SCL_DOM_ADVECTION), (C, den, D, -1, vfvm.OP_SCL_DOM_ADVECTION_UNITY)):
+        ops += [(vfvm.OP_SCL_DOM_TIME_PREVIOUS, 0, u, w, -1), (vfvm.OP_SCL_DOM_DIFFUSION, 0, u, dif, u)]
+        if src >= 0:
+            ops.append((vfvm.OP_SCL_DOM_SOURCE, 0, u, src, -1))
+        ops.append((adv, 0, u, w, dif, flw) if u == T else (adv, 0, u, dif, flw))
+        ops += [(vfvm.OP_SCL_BND_DIRICHLET, 0, u, ctx.sclbnd_add(0, 1.0), -1),
+                (vfvm.OP_SCL_BND_DIRICHLET, 1, u, ctx.sclbnd_add(1, 0.0), -1)]
+        for b in range(2, 6):
+            ops += [(vfvm.OP_SCL_BND_TIME_PREVIOUS, b, u, w, -1), (vfvm.OP_SCL_BND_DIFFUSION, b, u, dif, u),
+                    (vfvm.OP_SCL_BND_NEUMANN, b, u, ctx.sclbnd_add(b, 0.0), -1)]
+            ops.append((vfvm.OP_SCL_BND_ADVECTION, b, u, w, dif, flw) if u == T else
+                       (vfvm.OP_SCL_BND_ADVECTION_UNITY, b, u, dif, flw))
+    ops.append((vfvm.OP_SCL_DOM_PRESSURE_REFERENCE, 0, pres, 0, -1, -1, 0.0))
+    zero = np.zeros(N)
+
+    def hook(t):
+        ctx.vecdom_set(fce, np.stack([zero, zero, ctx.scldom_get(T) - 0.5]))
+    return mesh, ctx, ops, hook
+
+
+def run_heat_mass_flow(vfvm, args):
+    """Config-5 system (natural convection with a passive species, six unknowns per vertex):
+    implicit-Euler steps from rest with Picard per step, then the linear solve of the
+    transient system at the last state timed FP64 vs FP32-inner/FP64-refined with GMRES(100)
+    + 6x6 block Jacobi (solved DOF/s at 1e-10)."""
+    import torch
+    t0 = time.time()
+    mesh, ctx, ops, hook = build_heat_mass_flow(vfvm, args.hmf_n, 1e4, 0.71, 1.0)
+    ctx.set_update_hook(hook)
+    t_setup = time.time() - t0
+    N = mesh.num_tot
+    ndof = 6 * N
+    dt = 1e-2
+    lin = dict(tol=1e-8, restart=args.hmf_restart, max_outer=100, precision=vfvm.FP32_REFINED,
+               precond=vfvm.PRECOND_BJACOBI, inner_tol=1e-4, ortho=1)
+    tp = time.time()
+    note = None
+    st = None
+    try:
+        st = ctx.transient_solve(ops, dt, args.hmf_ts, 8, 1e-6, 1.0, lin=lin)
+    except vfvm.VFVChemError as e:
+        note = str(e)
+    t_steps = time.time() - tp
+    progress(f"heat-mass-flow {args.hmf_n}^3: {args.hmf_ts} time steps in {t_steps:.1f} s")
+    _, nnz = ctx.system_info()
+    ta = time.time()
+    ctx.assemble(ops, dt)
+    torch.cuda.synchronize()
+    t_asm = time.time() - ta
+    out = timed_block_solves(vfvm, ctx, ndof, args.tol, args.hmf_restart, args.hmf_inner_tol, args.hmf_steps,
+                             "heat-mass-flow", max_outer=100, events_first=True)
+    ctx.set_update_hook(None)
+    ctx.close()
+    return {
+        "workload": f"natural convection Ra=1e4, Pr=0.71, Sc=1 (free-fall units) with a passive species (u, v, w, p, T, C), {args.hmf_n}^3 "
+                    f"jittered lattice ({N} vertices, {ndof} DOF, {nnz} nnz), implicit Euler dt=1e-2, transient system "
+                    f"after {args.hmf_ts} time steps from rest, GMRES({args.hmf_restart}) + 6x6 block Jacobi, tol {args.tol}",
+        "dof": ndof, "nnz": nnz, "fp64": out["fp64"], "mixed": out["mixed"],
+        "mixed_over_fp64": out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"],
+        "assembly_wall_ms": 1e3 * t_asm, "time_steps_s": t_steps,
+        "picard_iterations": st.picard_iterations if st is not None else None, "steps_note": note,
+        "mesh_dual_gc_s": t_setup,
+    }
+
+
 def run_flow(vfvm, args):
     """Lid-driven cavity at Re = 100: Picard iterations of the coupled velocity-pressure system
     with GMRES(100) + 4x4 block Jacobi; the linear solve of the system after a few Picard
@@ -196,42 +351,7 @@ def run_flow(vfvm, args):
     ctx.assemble(ops)
     torch.cuda.synchronize()
     t_asm = time.time() - ta
-    x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
-    xp = x.data_ptr()
-    out = {}
-    for var in ("fp64", "mixed"):
-        prec = vfvm.FP64 if var == "fp64" else vfvm.FP32_REFINED
-        kw = dict(tol=args.tol, restart=args.flow_restart, max_outer=2000, precision=prec, precond=vfvm.PRECOND_BJACOBI,
-                  inner_tol=args.inner_tol, ortho=1)
-        ctx.timing_enable(0)
-        x.zero_()
-        torch.cuda.synchronize()
-        ctx.solve_device(xp, **kw)
-        ts = []
-        for _ in range(args.flow_steps):
-            x.zero_()
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            st = ctx.solve_device(xp, **kw)
-            torch.cuda.synchronize()
-            ts.append(time.perf_counter() - t1)
-        ctx.timing_enable(-1)
-        ctx.timing_reset()
-        x.zero_()
-        torch.cuda.synchronize()
-        ctx.solve_device(xp, **kw)
-        torch.cuda.synchronize()
-        ctx.timing_enable(0)
-        kern = {}
-        for name, k in (("spmv", vfvm.K_SPMV), ("mdot", vfvm.K_MDOT), ("update", vfvm.K_UPDATE), ("bjacobi", vfvm.K_PRECOND)):
-            nl, ms, by = ctx.timing_get(k)
-            if nl:
-                kern[name] = dict(launches=nl, avg_us=1e3 * ms / nl, total_ms=ms, bytes_per_launch=by / nl,
-                                  gbs=by / (ms * 1e-3) / 1e9)
-        ms = 1e3 * sum(ts) / len(ts)
-        progress(f"lid cavity {var}: {ms:.1f} ms per solve")
-        out[var] = dict(ms_per_step=ms, dof_per_s=ndof / (ms * 1e-3), iters=st.iterations, refinements=st.refinements,
-                        rel_residual=st.rel_residual, setup_s=st.setup_seconds, kernels=kern)
+    out = timed_block_solves(vfvm, ctx, ndof, args.tol, args.flow_restart, args.inner_tol, args.flow_steps, "lid cavity")
     ctx.close()
     return {
         "workload": f"steady lid-driven cavity Re=100 (rho=1, mu=0.01, lid (1,0,0)), {args.flow_n}^3 jittered lattice "
@@ -288,6 +408,14 @@ def main():
     ap.add_argument("--flow-restart", type=int, default=100)
     ap.add_argument("--flow-steps", type=int, default=2)
     ap.add_argument("--no-flow", action="store_true")
+    ap.add_argument("--hmf-n", type=int, default=100, help="lattice points per axis of the heat-mass-flow system")
+    ap.add_argument("--hmf-ts", type=int, default=10, help="implicit-Euler steps before the timed heat-mass-flow solves")
+    ap.add_argument("--hmf-steps", type=int, default=1)
+    ap.add_argument("--hmf-restart", type=int, default=100, help="GMRES restart of the heat-mass-flow solves")
+    ap.add_argument("--hmf-inner-tol", type=float, default=1e-6, help="inner tolerance of the mixed heat-mass-flow solve")
+    ap.add_argument("--no-hmf", action="store_true")
+    ap.add_argument("--hmf-only", action="store_true",
+                    help="run only the heat-mass-flow line and print it (profiling aid; not the headline line)")
     ap.add_argument("--flow-only", action="store_true",
                     help="run only the lid-driven cavity line and print it (profiling aid; not the headline line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -307,6 +435,9 @@ def main():
 
     if args.flow_only:
         print(json.dumps({"flow": run_flow(vfvm, args)}))
+        return
+    if args.hmf_only:
+        print(json.dumps({"heat_mass_flow": run_heat_mass_flow(vfvm, args)}))
         return
     if args.copy_calibration > 0:
         src = torch.ones(args.copy_calibration * (1 << 20) // 4, dtype=torch.float32, device="cuda")
@@ -402,10 +533,20 @@ def main():
                "cpu": cpu_model(), "nproc": os.cpu_count()}
 
     ctx.close()
-    progress("adjoint workload")
-    adjoint = None if args.no_adjoint else run_adjoint(vfvm, args)
-    progress("lid-driven cavity workload")
-    flow = None if args.no_flow else run_flow(vfvm, args)
+    def side_line(name, skip, fn):
+        # a failing side workload is reported in its own field; the headline line still prints
+        if skip:
+            return None
+        progress(f"{name} workload")
+        try:
+            return fn(vfvm, args)
+        except Exception as e:  # noqa: BLE001 - the error text is the report
+            progress(f"{name} workload failed: {e}")
+            return {"error": f"{type(e).__name__}: {e}"}
+
+    adjoint = side_line("adjoint", args.no_adjoint, run_adjoint)
+    flow = side_line("lid-driven cavity", args.no_flow, run_flow)
+    hmf = side_line("heat-mass-flow", args.no_hmf, run_heat_mass_flow)
 
     line = {
         "metric": "solved DOF/s at 1e-10 rel residual (FP32-inner/FP64-refined GMRES; FP64 GMRES beside it); "
@@ -436,6 +577,7 @@ def main():
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "flow": flow,
+        "heat_mass_flow": hmf,
     }
     print(json.dumps(line))
 
