@@ -180,6 +180,8 @@ int vfvm_sclflw_add(vfvm_ctx* ctx, int32_t vel_id, int32_t pres_id, int32_t den_
  * are 3 x count doubles SoA (x[count], y[count], z[count]) in the boundary's node order
  * (NULL -> every node takes cnst3[0..2]). */
 int vfvm_vecbnd_add(vfvm_ctx* ctx, int32_t bnd_id, const double* cnst3, const double* values, int32_t* id);
+/* New values of a boundary velocity (VectorBoundary::update_func, vec_bnd.rs), same layout. */
+int vfvm_vecbnd_set(vfvm_ctx* ctx, int32_t id, const double* values);
 /* flow_helper::update_flow (flow_helper.rs:3-179) for a flow with an unknown pressure: the
  * gradients of its fields, rc = V / a_P, the momentum term rc (grad p - fce_c) and the
  * Rhie-Chow face mass flux, from the current field values. */

@@ -148,3 +148,76 @@ def test_transient_boussinesq_vs_oracle(ctx, species):
     assert st.picard_iterations == its, (st.picard_iterations, its)
     assert np.max(np.abs(vel - ovel)) <= 1e-8 * np.max(np.abs(ovel))
     assert np.max(np.abs(S - oS)) <= 1e-8 * np.max(np.abs(oS))
+
+
+def _mirror_cavity(mesh, transient, species):
+    """The Boussinesq cavity of _cavity built through the mirrored reference API
+    (Variables / SteadyHeatFlow / TransientHeatMassFlow, a body-force closure of T)."""
+    vars = vfvm.Variables(mesh)
+    dom = vars.add_dom("cube")
+    bnds = [vars.add_bnd(dom, name) for name in ("x0", "x1", "y0", "y1", "z0", "z1")]
+    vel = vars.add_vecdom_unkn(dom, (0.0, 0.0, 0.0))
+    pres = vars.add_scldom_unkn(dom, 0.0)
+    den = vars.add_scldom_cnst(dom, 1.0)
+    visc = vars.add_scldom_cnst(dom, 0.1)
+    temp = vars.add_scldom_unkn(dom, 0.0)
+    fce = vars.add_vecdom_func(dom, lambda t, pos, u: np.stack([0.0 * u[0], 0.0 * u[0], 20.0 * (u[0] - 0.5)]), [temp])
+    vlcp = vars.add_scldom_cnst(dom, 1.0)
+    cond = vars.add_scldom_cnst(dom, 0.1)
+    zero = vars.add_scldom_cnst(dom, 0.0)
+    phys = (vfvm.TransientHeatMassFlow.new(1) if species else vfvm.TransientHeatFlow.new()) if transient else vfvm.SteadyHeatFlow.new()
+    phys.add_flow_dom(dom, vel, pres, den, visc, fce)
+    for b in bnds:
+        phys.add_vel_bnd(b, vars.add_vecbnd_cnst(b, (0.0, 0.0, 0.0)))
+    phys.set_pres_ref(dom, 0, 0.0)
+    phys.add_heat_dom(dom, temp, vlcp, cond, zero, zero)
+    phys.add_temp_bnd(bnds[0], vars.add_sclbnd_cnst(bnds[0], 1.0))
+    phys.add_temp_bnd(bnds[1], vars.add_sclbnd_cnst(bnds[1], 0.0))
+    for b in bnds[2:]:
+        phys.add_hflx_bnd(b, vars.add_sclbnd_cnst(b, 0.0))
+    conc = None
+    if species:
+        conc = vars.add_scldom_unkn(dom, 0.0)
+        dif = vars.add_scldom_cnst(dom, 0.07)
+        phys.add_mass_dom(0, dom, conc, {0: dif}, zero, zero)
+        phys.add_conc_bnd(0, bnds[0], vars.add_sclbnd_cnst(bnds[0], 1.0))
+        phys.add_conc_bnd(0, bnds[1], vars.add_sclbnd_cnst(bnds[1], 0.0))
+        for b in bnds[2:]:
+            phys.add_mflx_bnd(0, b, vars.add_sclbnd_cnst(b, 0.0))
+    return vars, phys, vel, pres, temp, conc
+
+
+def test_reference_api_mirror_steady_heatflow(cavity_oracle):
+    """SteadyHeatFlow through the mirrored API (the Boussinesq force a vecdom closure of T,
+    refreshed by the update hook) gives the oracle cavity's solution."""
+    print("[heat-flow api mirror steady]", flush=True)
+    mesh, res = cavity_oracle
+    ovel, opres, oS, its = res[False]
+    mesh.build_grad_coeff()
+    vars, phys, vel, pres, temp, _ = _mirror_cavity(mesh, False, False)
+    lin = vfvm.LinearGmres.new_with_options(1, 1e-13, 256, 400, precond=vfvm.PRECOND_BJACOBI)
+    st = phys.solve(vars, lin, 60, 1e-9, 0.8)
+    assert st.picard_iterations == its
+    assert np.max(np.abs(vars.value_vec(vel) - ovel)) <= 1e-8 * np.max(np.abs(ovel))
+    assert np.max(np.abs(vars.value(temp) - oS)) <= 1e-8 * np.max(np.abs(oS))
+
+
+def test_reference_api_mirror_transient_heatmassflow():
+    """TransientHeatMassFlow through the mirrored API (reference operator order: the species
+    time operator after diffusion / production / source) against the oracle's stepping of
+    the same physics (species time operator first: equal up to rounding)."""
+    print("[heat-mass-flow api mirror transient]", flush=True)
+    mesh = vfvm.Mesh.lattice(5, 0, 0.3)
+    mesh.build_grad_coeff()
+    vars, phys, vel, pres, temp, conc = _mirror_cavity(mesh, True, True)
+    lin = vfvm.LinearGmres.new_with_options(1, 1e-13, 256, 400, precond=vfvm.PRECOND_BJACOBI)
+    dt, steps = 0.05, 3
+    st = phys.solve(vars, lin, dt, steps, 1, 40, 1e-9, 1.0)
+    g = O.dual(mesh.get("pos"), mesh.get("tets"), mesh.num_pts)
+    gc = O.grad_coeff(g)
+    ovel, opres, oS, its = _oracle_solve(mesh, _cavity(mesh, transient=True, species=True), g, gc, 40, 1e-9, 1.0,
+                                         dt=dt, steps=steps)
+    S = np.concatenate([vars.value(temp), vars.value(conc)])
+    assert st.picard_iterations == its
+    assert np.max(np.abs(vars.value_vec(vel) - ovel)) <= 1e-8 * np.max(np.abs(ovel))
+    assert np.max(np.abs(S - oS)) <= 1e-8 * np.max(np.abs(oS))

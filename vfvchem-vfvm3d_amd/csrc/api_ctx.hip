@@ -488,6 +488,16 @@ extern "C" int vfvm_vecbnd_add(vfvm_ctx* ctx, int32_t bnd_id, const double* cnst
     return clear_error();
 }
 
+extern "C" int vfvm_vecbnd_set(vfvm_ctx* ctx, int32_t id, const double* values) {
+    CTX_GUARD(ctx);
+    if (id < 0 || id >= (int32_t)c.vbndv.size() || !values) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_vecbnd_set: invalid argument");
+    const int32_t b = c.vbndv[id].bnd;
+    const int64_t cnt = c.g.bnd_ptr[b + 1] - c.g.bnd_ptr[b];
+    if (cnt > 0) VFVM_HIP_TRY(hipMemcpyAsync(c.vbndv[id].value.p, values, 3 * cnt * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (int st = sync_err(c)) return st;
+    return clear_error();
+}
+
 // ------------------------------------------------------------------ pattern (LinearBase::update_sparse)
 extern "C" int vfvm_pattern_build(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, int64_t* nnz) {
     CTX_GUARD(ctx);

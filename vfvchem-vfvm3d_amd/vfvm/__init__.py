@@ -16,6 +16,7 @@ from .core import (FLW_FLUX, FLW_MOM, FLW_RC, OP_SCL_BND_DIVERGENCE, OP_SCL_BND_
                    OP_SCL_DOM_DIVERGENCE, OP_SCL_DOM_PRESSURE_REFERENCE, OP_SCL_DOM_PRESSURE_STABILIZER, OP_VEC_BND_DIRICHLET,
                    OP_VEC_DOM_ADVECTION, OP_VEC_DOM_DIFFUSION, OP_VEC_DOM_DILATATORY, OP_VEC_DOM_GRADIENT, OP_VEC_DOM_SOURCE,
                    OP_VEC_DOM_TRANSPOSE, PRECOND_BJACOBI)
-from .api import (LinearGmres, SteadyFlow, SteadyHeat, SteadyMass, SteadyMassFlow, TransientHeat, Variables)  # noqa: F401
+from .api import (LinearGmres, SteadyFlow, SteadyHeat, SteadyHeatFlow, SteadyMass, SteadyMassFlow, TransientFlow,  # noqa: F401
+                  TransientHeat, TransientHeatFlow, TransientHeatMassFlow, Variables)
 
 __version__ = "0.1.0"

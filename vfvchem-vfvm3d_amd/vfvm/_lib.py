@@ -136,6 +136,7 @@ SIGNATURES = [
     ("vfvm_vecdom_get", _I, [_P, _I, _PD]),
     ("vfvm_sclbnd_add", _I, [_P, _I, _D, _PD, _PI]),
     ("vfvm_sclbnd_set", _I, [_P, _I, _PD]),
+    ("vfvm_vecbnd_set", _I, [_P, _I, _PD]),
     ("vfvm_sclflw_add", _I, [_P, _I, _I, _I, _I, _I, _PI]),
     ("vfvm_vecbnd_add", _I, [_P, _I, _PD, _PD, _PI]),
     ("vfvm_update_flow", _I, [_P, _I]),

@@ -11,6 +11,10 @@ Names, argument order and error behaviour follow the reference:
   SteadyFlow           src/steady/core/steady_flow.rs:9-321 (velocity boundaries and
                        a reference pressure; pressure/traction boundaries and
                        interfaces are outside the built path)
+  TransientFlow        src/transient/core/transient_flow.rs (the same, implicit Euler)
+  SteadyHeatFlow       src/steady/core/steady_heatflow.rs:9-536 (one domain)
+  TransientHeatMassFlow src/transient/core/transient_heatmassflow.rs (one domain, one
+                       species component with self-diffusion)
   LinearGmres          src/linear/linear_gmres.rs:16-50
 The operator lists these classes build are applied in the order of each
 reference solver's assemble_matrix; the Picard loops run in the library
@@ -22,10 +26,11 @@ from __future__ import annotations
 import numpy as np
 
 from . import _lib as L
-from .core import (OP_SCL_BND_DIVERGENCE, OP_SCL_BND_PRESSURE_STABILIZER, OP_SCL_DOM_DIVERGENCE,
-                   OP_SCL_DOM_PRESSURE_REFERENCE, OP_SCL_DOM_PRESSURE_STABILIZER, OP_VEC_BND_DIRICHLET,
-                   OP_VEC_DOM_ADVECTION, OP_VEC_DOM_DIFFUSION, OP_VEC_DOM_DILATATORY, OP_VEC_DOM_GRADIENT,
-                   OP_VEC_DOM_SOURCE, OP_VEC_DOM_TRANSPOSE, PRECOND_BJACOBI)
+from .core import (OP_SCL_BND_ADVECTION, OP_SCL_BND_DIVERGENCE, OP_SCL_BND_PRESSURE_STABILIZER,
+                   OP_SCL_BND_TIME_PREVIOUS, OP_SCL_DOM_ADVECTION, OP_SCL_DOM_DIVERGENCE, OP_SCL_DOM_PRESSURE_REFERENCE,
+                   OP_SCL_DOM_PRESSURE_STABILIZER, OP_SCL_DOM_TIME_PREVIOUS, OP_VEC_BND_DIRICHLET, OP_VEC_DOM_ADVECTION,
+                   OP_VEC_DOM_DIFFUSION, OP_VEC_DOM_DILATATORY, OP_VEC_DOM_GRADIENT, OP_VEC_DOM_SOURCE, OP_VEC_DOM_TIME,
+                   OP_VEC_DOM_TRANSPOSE, PRECOND_BJACOBI)
 from .core import (FIELD_CONSTANT, FIELD_FUNCTION, FIELD_UNKNOWN, FP32_REFINED, FP64, PRECOND_ILU0, PRECOND_JACOBI,
                    PRECOND_NONE, OP_SCL_BND_ADVECTION_UNITY, OP_SCL_BND_DIFFUSION, OP_SCL_BND_DIRICHLET,
                    OP_SCL_BND_NEUMANN, OP_SCL_BND_PRODUCTION, OP_SCL_BND_SOURCE, OP_SCL_BND_TIME,
@@ -96,8 +101,13 @@ class Variables:
         self.sclbnd.append((bnd_id, None, []))
         return i
 
+    def _bnd_unkn(self, bnd_id, unkn):
+        nodes = self.mesh.num_pts + self.mesh.bnd_nodes(self.bnd[bnd_id]).astype(np.int64)
+        return [self.ctx.scldom_get(u)[nodes] for u in unkn]
+
     def add_sclbnd_func(self, bnd_id: int, func, unkn_id=(), file_path: str = "") -> int:
-        vals = np.asarray(func(0.0, self._bnd_pos(bnd_id), []), np.float64)
+        """func(t, pos[count,3], [values of each unknown on the boundary's nodes]) -> values[count]."""
+        vals = np.asarray(func(0.0, self._bnd_pos(bnd_id), self._bnd_unkn(bnd_id, unkn_id)), np.float64)
         i = self.ctx.sclbnd_add(self.bnd[bnd_id], 0.0, vals)
         self.sclbnd.append((bnd_id, func, list(unkn_id)))
         return i
@@ -109,7 +119,8 @@ class Variables:
         return i
 
     def add_vecdom_func(self, dom_id: int, func, unkn_id=(), file_path: str = "") -> int:
-        vals = np.asarray(func(0.0, self.pos, []), np.float64).reshape(3, -1)
+        """func(t, pos[N,3], [values of each scalar unknown]) -> values (3, N)."""
+        vals = np.asarray(func(0.0, self.pos, [self.ctx.scldom_get(u) for u in unkn_id]), np.float64).reshape(3, -1)
         i = self.ctx.vecdom_add(FIELD_FUNCTION, values=vals)
         self.vecdom.append((FIELD_FUNCTION, func, list(unkn_id)))
         return i
@@ -126,7 +137,7 @@ class Variables:
         return i
 
     def add_vecbnd_func(self, bnd_id: int, func, unkn_id=(), file_path: str = "") -> int:
-        vals = np.asarray(func(0.0, self._bnd_pos(bnd_id), []), np.float64).reshape(3, -1)
+        vals = np.asarray(func(0.0, self._bnd_pos(bnd_id), self._bnd_unkn(bnd_id, unkn_id)), np.float64).reshape(3, -1)
         i = self.ctx.vecbnd_add(self.bnd[bnd_id], values=vals)
         self.vecbnd.append((bnd_id, func, list(unkn_id)))
         return i
@@ -137,16 +148,26 @@ class Variables:
         return i
 
     def update_func(self, t: float) -> None:
-        """_update_func (steady_base.rs:242-261): re-evaluate function fields on the host."""
+        """_update_func (steady_base.rs:242-261): re-evaluate function fields on the host, in
+        the reference's order (scalar domain, scalar boundary, vector domain, vector boundary)."""
         for i, (kind, func, unkn) in enumerate(self.scldom):
             if kind == FIELD_FUNCTION:
                 self.ctx.scldom_set(i, func(t, self.pos, [self.ctx.scldom_get(u) for u in unkn]))
         for i, (bnd_id, func, unkn) in enumerate(self.sclbnd):
             if func is not None:
-                self.ctx.sclbnd_set(i, func(t, self._bnd_pos(bnd_id), []))
+                self.ctx.sclbnd_set(i, func(t, self._bnd_pos(bnd_id), self._bnd_unkn(bnd_id, unkn)))
         for i, (kind, func, unkn) in enumerate(self.vecdom):
             if kind == FIELD_FUNCTION:
-                self.ctx.vecdom_set(i, np.asarray(func(t, self.pos, []), np.float64).reshape(3, -1))
+                self.ctx.vecdom_set(i, np.asarray(func(t, self.pos, [self.ctx.scldom_get(u) for u in unkn]),
+                                                  np.float64).reshape(3, -1))
+        for i, (bnd_id, func, unkn) in enumerate(self.vecbnd):
+            if func is not None:
+                self.ctx.vecbnd_set(i, np.asarray(func(t, self._bnd_pos(bnd_id), self._bnd_unkn(bnd_id, unkn)),
+                                                  np.float64).reshape(3, -1))
+
+    def has_func(self) -> bool:
+        return (any(k == FIELD_FUNCTION for k, _, _ in self.scldom) or any(f is not None for _, f, _ in self.sclbnd)
+                or any(k == FIELD_FUNCTION for k, _, _ in self.vecdom) or any(f is not None for _, f, _ in self.vecbnd))
 
     def value(self, scldom_id: int) -> np.ndarray:
         return self.ctx.scldom_get(scldom_id)
@@ -185,8 +206,22 @@ class LinearGmres:
         return LinearGmres(tol=tol, restart=restart, max_outer=max_outer, **kw)
 
 
-def _depends_on_unknowns(vars: Variables) -> bool:
-    return any(kind == FIELD_FUNCTION and unkn for kind, _, unkn in vars.scldom)
+class _UpdateHook:
+    """Registers Variables.update_func as the library's update hook for one driver call: the
+    library calls it with the time before every assembly (the reference's _update_func,
+    steady_base.rs:50-54, 100; transient_base.rs:71-100)."""
+
+    def __init__(self, vars: Variables):
+        self.vars = vars
+
+    def __enter__(self):
+        if self.vars.has_func():
+            self.vars.ctx.set_update_hook(self.vars.update_func)
+        return self
+
+    def __exit__(self, *exc):
+        self.vars.ctx.set_update_hook(None)
+        return False
 
 
 class SteadyBase:
@@ -196,10 +231,10 @@ class SteadyBase:
         raise NotImplementedError
 
     def solve(self, vars: Variables, lin: LinearGmres, max_iter: int, tol: float, damp: float):
-        if _depends_on_unknowns(vars):
-            raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "function fields of unknowns need the host Picard loop (not on the built path)")
         vars.update_func(0.0)
-        self.stats = vars.ctx.steady_solve(self.operators(vars), max_iter, tol, damp, lin=lin.opts)
+        ops = self.operators(vars)
+        with _UpdateHook(vars):
+            self.stats = vars.ctx.steady_solve(ops, max_iter, tol, damp, lin=lin.opts)
         return self.stats
 
 
@@ -213,10 +248,10 @@ class TransientBase:
               damp: float):
         if num_ts_write == 0:
             raise VFVChemError(L.ERR_INVALID_ARGUMENT, "SolverNonPositiveWriteFrequency")
-        if _depends_on_unknowns(vars) or any(k == FIELD_FUNCTION for k, _, _ in vars.scldom):
-            # time-dependent closures: step from the host
-            raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "time-dependent function fields need the host time loop (not on the built path)")
-        self.stats = vars.ctx.transient_solve(self.operators(vars), dt, num_ts, max_iter, tol, damp, lin=lin.opts)
+        vars.update_func(0.0)
+        ops = self.operators(vars)
+        with _UpdateHook(vars):
+            self.stats = vars.ctx.transient_solve(ops, dt, num_ts, max_iter, tol, damp, lin=lin.opts)
         return self.stats
 
 
@@ -404,4 +439,207 @@ class SteadyFlow(SteadyBase):
                     (OP_SCL_BND_PRESSURE_STABILIZER, b, pres, den, fce, f)]
         for _, nid, p in self.pref:
             ops.append((OP_SCL_DOM_PRESSURE_REFERENCE, 0, pres, nid, -1, -1, p))
+        return ops
+
+
+class TransientFlow(TransientBase):
+    """TransientFlow (transient_flow.rs): SteadyFlow's operators with OpVecDomTime first and the
+    divergence operators' time part, implicit Euler."""
+
+    def __init__(self):
+        self.flow = SteadyFlow()
+
+    @staticmethod
+    def new():
+        return TransientFlow()
+
+    def add_flow_dom(self, dom_id, vel_id, pres_id, den_id, visc_id, fce_id):
+        self.flow.add_flow_dom(dom_id, vel_id, pres_id, den_id, visc_id, fce_id)
+
+    def add_vel_bnd(self, bnd_id, vel_id):
+        self.flow.add_vel_bnd(bnd_id, vel_id)
+
+    def add_pres_bnd(self, bnd_id, pres_id):
+        self.flow.add_pres_bnd(bnd_id, pres_id)
+
+    def add_trac_bnd(self, bnd_id, pres_id):
+        self.flow.add_trac_bnd(bnd_id, pres_id)
+
+    def set_pres_ref(self, dom_id, nid, pres):
+        self.flow.set_pres_ref(dom_id, nid, pres)
+
+    def operators(self, vars: Variables):
+        ops = self.flow.operators(vars)
+        _, vel, _, den, _, _ = self.flow.itr[0]
+        return [(OP_VEC_DOM_TIME, 0, vel, den, -1)] + ops
+
+
+def _split_pref(ops):
+    pref = [o for o in ops if o[0] == OP_SCL_DOM_PRESSURE_REFERENCE]
+    return [o for o in ops if o[0] != OP_SCL_DOM_PRESSURE_REFERENCE], pref
+
+
+class SteadyHeatFlow(SteadyBase):
+    """SteadyHeatFlow (steady_heatflow.rs:9-536), one domain: the flow's operators, then the
+    heat operators in assemble_matrix order (steady_heatflow.rs:466-512), then the reference
+    pressure.  Unknowns [u, v, w, p, T] (initial_start, :154-186); the heat advection reads
+    the flow's own Rhie-Chow flux."""
+
+    transient = False
+
+    def __init__(self):
+        self.flow = SteadyFlow()
+        self.itr_heat = []   # (dom, temp, vlcp, cond, hprd, hsrc)
+        self.temp_bnd = []   # (bnd, temp)
+        self.hflx_bnd = []   # (bnd, hflx)
+        self.hout_bnd = []   # bnd
+
+    @staticmethod
+    def new():
+        return SteadyHeatFlow()
+
+    def add_heat_dom(self, dom_id, temp_id, vlcp_id, cond_id, hprd_id, hsrc_id):
+        self.itr_heat.append((dom_id, temp_id, vlcp_id, cond_id, hprd_id, hsrc_id))
+
+    def add_temp_bnd(self, bnd_id, temp_id):
+        self.temp_bnd.append((bnd_id, temp_id))
+
+    def add_hflx_bnd(self, bnd_id, hflx_id):
+        self.hflx_bnd.append((bnd_id, hflx_id))
+
+    def add_hout_bnd(self, bnd_id):
+        self.hout_bnd.append(bnd_id)
+
+    def add_hcnt_itf(self, itf_id):
+        raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "interfaces are outside the built path")
+
+    def add_flow_dom(self, dom_id, vel_id, pres_id, den_id, visc_id, fce_id):
+        self.flow.add_flow_dom(dom_id, vel_id, pres_id, den_id, visc_id, fce_id)
+
+    def add_vel_bnd(self, bnd_id, vel_id):
+        self.flow.add_vel_bnd(bnd_id, vel_id)
+
+    def add_pres_bnd(self, bnd_id, pres_id):
+        self.flow.add_pres_bnd(bnd_id, pres_id)
+
+    def add_trac_bnd(self, bnd_id, pres_id):
+        self.flow.add_trac_bnd(bnd_id, pres_id)
+
+    def add_fcnt_itf(self, itf_id):
+        raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "interfaces are outside the built path")
+
+    def set_pres_ref(self, dom_id, nid, pres):
+        self.flow.set_pres_ref(dom_id, nid, pres)
+
+    def heat_operators(self, vars: Variables):
+        if len(self.itr_heat) != 1:
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, "one heat domain on the built path")
+        _, temp, vlcp, cond, hprd, hsrc = self.itr_heat[0]
+        flw = self.flow.flw
+        tp = self.transient
+        ops = [(OP_SCL_DOM_TIME_PREVIOUS, 0, temp, vlcp, -1)] if tp else []
+        ops += [(OP_SCL_DOM_DIFFUSION, 0, temp, cond, temp), (OP_SCL_DOM_PRODUCTION, 0, temp, hprd, -1),
+                (OP_SCL_DOM_SOURCE, 0, temp, hsrc, -1), (OP_SCL_DOM_ADVECTION, 0, temp, vlcp, cond, flw)]
+        for bnd_id, tv in self.temp_bnd:
+            ops.append((OP_SCL_BND_DIRICHLET, vars.bnd[bnd_id], temp, tv, -1))
+        for bnd_id, hf in self.hflx_bnd:
+            b = vars.bnd[bnd_id]
+            ops += [(OP_SCL_BND_TIME_PREVIOUS, b, temp, vlcp, -1)] if tp else []
+            ops += [(OP_SCL_BND_DIFFUSION, b, temp, cond, temp), (OP_SCL_BND_PRODUCTION, b, temp, hprd, -1),
+                    (OP_SCL_BND_SOURCE, b, temp, hsrc, -1), (OP_SCL_BND_NEUMANN, b, temp, hf, -1)]
+        for bnd_id, _ in self.hflx_bnd:
+            ops.append((OP_SCL_BND_ADVECTION, vars.bnd[bnd_id], temp, vlcp, cond, flw))
+        for bnd_id in self.hout_bnd:
+            b = vars.bnd[bnd_id]
+            ops += [(OP_SCL_BND_TIME_PREVIOUS, b, temp, vlcp, -1)] if tp else []
+            ops += [(OP_SCL_BND_DIFFUSION, b, temp, cond, temp), (OP_SCL_BND_ADVECTION, b, temp, vlcp, cond, flw),
+                    (OP_SCL_BND_PRODUCTION, b, temp, hprd, -1), (OP_SCL_BND_SOURCE, b, temp, hsrc, -1)]
+        return ops
+
+    def mass_operators(self, vars: Variables):
+        return []
+
+    def operators(self, vars: Variables):
+        flow_ops, pref = _split_pref(self.flow.operators(vars))
+        if self.transient:
+            _, vel, _, den, _, _ = self.flow.itr[0]
+            flow_ops = [(OP_VEC_DOM_TIME, 0, vel, den, -1)] + flow_ops
+        return flow_ops + self.heat_operators(vars) + self.mass_operators(vars) + pref
+
+
+class TransientHeatFlow(SteadyHeatFlow):
+    """TransientHeatFlow (transient_heatflow.rs): SteadyHeatFlow's composition with the
+    velocity time operator and OpSclDomTimePrevious / OpSclBndTimePrevious (weight rho c_p)
+    ahead of each heat row class, implicit Euler (TransientBase::solve)."""
+
+    transient = True
+    solve = TransientBase.solve
+
+    @staticmethod
+    def new():
+        return TransientHeatFlow()
+
+
+class TransientHeatMassFlow(TransientHeatFlow):
+    """TransientHeatMassFlow (transient_heatmassflow.rs:9-900), one domain and one species
+    component: TransientHeatFlow plus the species operators in assemble_matrix order
+    (diffusion, production, source, OpSclDomTimePrevious weighted by the density,
+    OpSclDomAdvectionUnity with the self-diffusivity; :785-840).  Unknowns
+    [u, v, w, p, T, C] (initial_start, :248-268).  Cross-diffusion (another component
+    driving the flux) is outside the built path."""
+
+    def __init__(self, num_comp: int = 1):
+        super().__init__()
+        if num_comp != 1:
+            raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "one species component on the built path (six unknowns per vertex)")
+        self.itr_mass = []   # (comp, dom, conc, diff_ids, mprd, msrc)
+        self.conc_bnd = []   # (comp, bnd, conc)
+        self.mflx_bnd = []   # (comp, bnd, mflx)
+        self.mout_bnd = []   # (comp, bnd)
+
+    @staticmethod
+    def new(num_comp: int = 1):
+        return TransientHeatMassFlow(num_comp)
+
+    def add_mass_dom(self, comp_id, dom_id, conc_id, diff_ids: dict, mprd_id, msrc_id):
+        self.itr_mass.append((comp_id, dom_id, conc_id, dict(diff_ids), mprd_id, msrc_id))
+
+    def add_conc_bnd(self, comp_id, bnd_id, conc_id):
+        self.conc_bnd.append((comp_id, bnd_id, conc_id))
+
+    def add_mflx_bnd(self, comp_id, bnd_id, mflx_id):
+        self.mflx_bnd.append((comp_id, bnd_id, mflx_id))
+
+    def add_mout_bnd(self, comp_id, bnd_id):
+        self.mout_bnd.append((comp_id, bnd_id))
+
+    def mass_operators(self, vars: Variables):
+        if len(self.itr_mass) != 1:
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, "one species domain on the built path")
+        comp, _, conc, diff_ids, mprd, msrc = self.itr_mass[0]
+        if set(diff_ids) - {comp}:
+            raise VFVChemError(L.ERR_OPERATOR_UNSUPPORTED, "cross-diffusion into another unknown is outside the built path")
+        diff = diff_ids.get(comp, -1)
+        _, _, _, den, _, _ = self.flow.itr[0]
+        flw = self.flow.flw
+        ops = [(OP_SCL_DOM_DIFFUSION, 0, conc, diff, conc)] if diff >= 0 else []
+        ops += [(OP_SCL_DOM_PRODUCTION, 0, conc, mprd, -1), (OP_SCL_DOM_SOURCE, 0, conc, msrc, -1),
+                (OP_SCL_DOM_TIME_PREVIOUS, 0, conc, den, -1), (OP_SCL_DOM_ADVECTION_UNITY, 0, conc, diff, flw)]
+        for _, bnd_id, cv in self.conc_bnd:
+            ops.append((OP_SCL_BND_DIRICHLET, vars.bnd[bnd_id], conc, cv, -1))
+        for _, bnd_id, mf in self.mflx_bnd:
+            b = vars.bnd[bnd_id]
+            ops += [(OP_SCL_BND_DIFFUSION, b, conc, diff, conc)] if diff >= 0 else []
+            ops += [(OP_SCL_BND_PRODUCTION, b, conc, mprd, -1), (OP_SCL_BND_SOURCE, b, conc, msrc, -1),
+                    (OP_SCL_BND_NEUMANN, b, conc, mf, -1)]
+        for _, bnd_id, _ in self.mflx_bnd:
+            ops.append((OP_SCL_BND_TIME_PREVIOUS, vars.bnd[bnd_id], conc, den, -1))
+        for _, bnd_id, _ in self.mflx_bnd:
+            ops.append((OP_SCL_BND_ADVECTION_UNITY, vars.bnd[bnd_id], conc, diff, flw))
+        for _, bnd_id in self.mout_bnd:
+            b = vars.bnd[bnd_id]
+            ops += [(OP_SCL_BND_TIME_PREVIOUS, b, conc, den, -1)]
+            ops += [(OP_SCL_BND_DIFFUSION, b, conc, diff, conc)] if diff >= 0 else []
+            ops += [(OP_SCL_BND_ADVECTION_UNITY, b, conc, diff, flw), (OP_SCL_BND_PRODUCTION, b, conc, mprd, -1),
+                    (OP_SCL_BND_SOURCE, b, conc, msrc, -1)]
         return ops

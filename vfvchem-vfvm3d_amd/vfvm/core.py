@@ -231,6 +231,11 @@ class Context:
         v = np.ascontiguousarray(values, np.float64)
         check(lib.vfvm_sclbnd_set(self._h, int(fid), ptr(v)))
 
+    def vecbnd_set(self, fid, values):
+        """New boundary velocity values, SoA (3, count)."""
+        v = np.ascontiguousarray(values, np.float64)
+        check(lib.vfvm_vecbnd_set(self._h, int(fid), ptr(v)))
+
     def sclflw_add(self, vel, pres, den, visc, fce) -> int:
         i = C.c_int32()
         check(lib.vfvm_sclflw_add(self._h, int(vel), int(pres), int(den), int(visc), int(fce), C.byref(i)))
