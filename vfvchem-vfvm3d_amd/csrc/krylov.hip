@@ -126,7 +126,7 @@ __device__ inline int64_t fidx(int64_t N, int c, int64_t m) { return IL ? (int64
 // the row sums of vertex v (partials of its L lanes, not yet reduced): the four flow rows, and
 // the rows of the NS coupled scalar unknowns (rows (4 + k) N + v, columns in their own block:
 // {v} u ngh(v), or {v} alone)
-template <class T, bool IL, int L = WAVE, int NS = 0>
+template <class T, bool IL, int L = WAVE, int NS = 0, int UR = 4, bool UG = true>
 __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                  const T* __restrict__ val, const T* __restrict__ x, int64_t v, int lane, T& su, T& sv,
                                  T& sw, T& sq, T* st) {
@@ -147,15 +147,39 @@ __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restri
             sw += val[rw + k] * x[fidx<IL, S>(N, 2, m)];
         }
         // p part of the velocity rows and velocity part of the continuity row: grad_coeff(v)
-        for (int32_t q = lane; q < G; q += L) {
+        // (a point's continuity row is always full: the reference-pressure row is a node's).
+        // Two entries per lane and trip, every load issued before the first product; each
+        // lane still sums its entries in ascending order.
+        int32_t q = lane;
+        if (UG) for (; q + L < G; q += 2 * L) {
+            const int32_t m0 = col[ru + d1 + q] - pN3, m1 = col[ru + d1 + q + L] - pN3;
+            const T a0 = val[ru + d1 + q], a1 = val[ru + d1 + q + L];
+            const T b0 = val[rv + d1 + q], b1 = val[rv + d1 + q + L];
+            const T c0 = val[rw + d1 + q], c1 = val[rw + d1 + q + L];
+            const T e0 = val[rq + q], e1 = val[rq + q + L];
+            const T f0 = val[rq + G + q], f1 = val[rq + G + q + L];
+            const T g0 = val[rq + 2 * G + q], g1 = val[rq + 2 * G + q + L];
+            const T p0 = x[fidx<IL, S>(N, 3, m0)], p1 = x[fidx<IL, S>(N, 3, m1)];
+            const T u0 = x[fidx<IL, S>(N, 0, m0)], u1 = x[fidx<IL, S>(N, 0, m1)];
+            const T v0 = x[fidx<IL, S>(N, 1, m0)], v1 = x[fidx<IL, S>(N, 1, m1)];
+            const T w0 = x[fidx<IL, S>(N, 2, m0)], w1 = x[fidx<IL, S>(N, 2, m1)];
+            su += a0 * p0;
+            sv += b0 * p0;
+            sw += c0 * p0;
+            sq += e0 * u0 + f0 * v0 + g0 * w0;
+            su += a1 * p1;
+            sv += b1 * p1;
+            sw += c1 * p1;
+            sq += e1 * u1 + f1 * v1 + g1 * w1;
+        }
+        for (; q < G; q += L) {
             const int32_t m = col[ru + d1 + q] - pN3;
             const T pm = x[fidx<IL, S>(N, 3, m)];
             su += val[ru + d1 + q] * pm;
             sv += val[rv + d1 + q] * pm;
             sw += val[rw + d1 + q] * pm;
-            if (lq > 1)
-                sq += val[rq + q] * x[fidx<IL, S>(N, 0, m)] + val[rq + G + q] * x[fidx<IL, S>(N, 1, m)] +
-                      val[rq + 2 * G + q] * x[fidx<IL, S>(N, 2, m)];
+            sq += val[rq + q] * x[fidx<IL, S>(N, 0, m)] + val[rq + G + q] * x[fidx<IL, S>(N, 1, m)] +
+                  val[rq + 2 * G + q] * x[fidx<IL, S>(N, 2, m)];
         }
     } else {
         if (lane == 0) {
@@ -171,7 +195,23 @@ __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restri
             }
     }
     if (lq > 1) {
-        for (int32_t i = 3 * G + lane; i < lq; i += L) sq += val[rq + i] * x[fidx<IL, S>(N, 3, col[rq + i] - pN3)];
+        // p part of the continuity row (the 3-ring): four entries per lane and trip, loads
+        // first, products summed in ascending order
+        int32_t i = 3 * G + lane;
+        if (UR > 1) for (; i + (UR - 1) * L < lq; i += UR * L) {
+            int32_t cc[UR];
+            T vv[UR], xx[UR];
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                cc[u] = col[rq + i + u * L];
+                vv[u] = val[rq + i + u * L];
+            }
+#pragma unroll
+            for (int u = 0; u < UR; ++u) xx[u] = x[fidx<IL, S>(N, 3, cc[u] - pN3)];
+#pragma unroll
+            for (int u = 0; u < UR; ++u) sq += vv[u] * xx[u];
+        }
+        for (; i < lq; i += L) sq += val[rq + i] * x[fidx<IL, S>(N, 3, col[rq + i] - pN3)];
     } else if (lane == 0) {
         sq = val[rq] * x[fidx<IL, S>(N, 3, v)];
     }
@@ -196,7 +236,7 @@ __device__ inline void flow_rows(const FlowSpmvGeom& fg, const int32_t* __restri
 // products.  With IL the vectors are vertex-interleaved, so the four unknowns of a gathered
 // vertex share one 32-byte (FP64) segment.  Rows of length 1 (velocity Dirichlet rows, the
 // reference-pressure row) are diagonal.
-template <class T, bool IL, int L, int NS>
+template <class T, bool IL, int L, int NS, int UR = 4, bool UG = true>
 __global__ void __launch_bounds__(BLK) k_spmv_flow(FlowSpmvGeom fg, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                    const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
                                                    const double* __restrict__ scale, const int* __restrict__ ctrl) {
@@ -207,7 +247,7 @@ __global__ void __launch_bounds__(BLK) k_spmv_flow(FlowSpmvGeom fg, const int32_
     const int lane = threadIdx.x % L;
     if (v >= N) return;
     T su, sv, sw, sq, st[NS + 1];
-    flow_rows<T, IL, L, NS>(fg, rp, col, val, x, v, lane, su, sv, sw, sq, st);
+    flow_rows<T, IL, L, NS, UR, UG>(fg, rp, col, val, x, v, lane, su, sv, sw, sq, st);
     su = wave_sum<T, L>(su);
     sv = wave_sum<T, L>(sv);
     sw = wave_sum<T, L>(sw);
@@ -906,12 +946,25 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
             const char* e = getenv("VFVM_FLOW_LPV");
             return (e && atoi(e) == 64) ? 64 : 32;
         }();
+        // VFVM_FLOW_UNROLL=10: one entry per lane and trip in every loop (the default issues
+        // four 3-ring and two grad_coeff entries per trip: at 80^3 FP64 958 -> 912 us,
+        // FP32 624 -> 592 us; eight per trip lose occupancy: 1004 us); read on every launch
+        // so that one process can compare them with eager launches
+        const int unroll = [] {
+            const char* e = getenv("VFVM_FLOW_UNROLL");
+            return e ? atoi(e) : 41;
+        }();
 #define VFVM_FLOW_SPMV(NS)                                                                                                   \
     if (sy.flow_layout == 2) {                                                                                               \
-        if (lpv == 32)                                                                                                       \
-            launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 32, NS>, dim3(grid_for(c.g.N * 32)), dim3(BLK), fg, rp, cl, \
-                         val, x, y, scale, ctrl);                                                                            \
-        else                                                                                                                 \
+        if (lpv == 32) {                                                                                                     \
+            const dim3 g32(grid_for(c.g.N * 32));                                                                            \
+            switch (unroll) {                                                                                                \
+                case 10: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 32, NS, 1, false>, g32, dim3(BLK), fg, rp, \
+                                      cl, val, x, y, scale, ctrl); break;                                                   \
+                default: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 32, NS, 4, true>, g32, dim3(BLK), fg, rp,  \
+                                      cl, val, x, y, scale, ctrl); break;                                                   \
+            }                                                                                                                \
+        } else                                                                                                               \
             launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 64, NS>, dim3(grid_for(c.g.N * 64)), dim3(BLK), fg, rp, cl, \
                          val, x, y, scale, ctrl);                                                                            \
     } else {                                                                                                                 \
