@@ -84,8 +84,32 @@ __device__ inline T row_dot(const int32_t* __restrict__ col, const T* __restrict
     for (; k < e1; k += step) s += val[k] * x[col[k]];
     return s;
 }
+// the same sum with two entries per trip, both gathers issued before the first product
+// (the lane's entries are still added in ascending order)
+template <class T>
+__device__ inline T row_dot2(const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ x, int32_t k,
+                             int32_t e1, int step) {
+    T s = 0;
+    for (; k + step < e1; k += 2 * step) {
+        const int32_t c0 = col[k], c1 = col[k + step];
+        const T a0 = val[k], a1 = val[k + step];
+        const T x0 = x[c0], x1 = x[c1];
+        s += a0 * x0;
+        s += a1 * x1;
+    }
+    if (k < e1) s += val[k] * x[col[k]];
+    return s;
+}
+// rows of the scalar SpMV and the ILU sweeps take two entries per trip unless
+// VFVM_ROW_UNROLL=0 (read on every launch, so one process can compare both; at 100^3
+// convection-diffusion: SpMV FP64 78.0 -> 71.7 us, FP32 54.4 -> 49.2 us, ILU sweep
+// launch FP32 6.41 -> 6.05 us)
+inline bool row_unroll() {
+    const char* e = getenv("VFVM_ROW_UNROLL");
+    return !(e && atoi(e) == 0);
+}
 
-template <class T, int LPR>
+template <class T, int LPR, bool U2 = true>
 __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                               const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
                                               const double* __restrict__ scale, const int* __restrict__ ctrl) {
@@ -94,7 +118,7 @@ __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restri
     const int64_t row = tid / LPR;
     const int lane = (int)(tid % LPR);
     if (row >= n) return;
-    T s = row_dot<T>(col, val, x, rp[row] + lane, rp[row + 1], LPR);
+    T s = U2 ? row_dot2<T>(col, val, x, rp[row] + lane, rp[row + 1], LPR) : row_dot<T>(col, val, x, rp[row] + lane, rp[row + 1], LPR);
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
     if (lane == 0) y[row] = scale ? (T)(s * (T)scale[0]) : s;
 }
@@ -322,7 +346,7 @@ __global__ void __launch_bounds__(BLK) k_residual(int64_t n, const int32_t* __re
     const int64_t rows_per_pass = (int64_t)gridDim.x * (BLK / LPR);
     double rr = 0.0, bb = 0.0;
     for (int64_t row = ((int64_t)xcd_block() * BLK + threadIdx.x) / LPR; row < n; row += rows_per_pass) {
-        double s = row_dot<double>(col, val, x, rp[row] + lane, rp[row + 1], LPR);
+        double s = row_dot2<double>(col, val, x, rp[row] + lane, rp[row + 1], LPR);
         for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
         if (lane == 0) {
             const double bi = b[row];
@@ -665,7 +689,7 @@ __global__ void __launch_bounds__(BLK) k_ilu_factor(int32_t t0, int32_t t1, cons
 // Forward sweep over the rows [t0, t1) of one colour of the colour-major system
 // (L has a unit diagonal), in place in z:  z_t = c rhs_t - sum_L l_tj z_j  (c = *rscale or 1)
 // The last colour has no U entries: with FINAL its backward sweep is folded in.
-template <class T, int LPR, bool FINAL>
+template <class T, int LPR, bool FINAL, bool U2 = true>
 __global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
                                                    const T* __restrict__ lu, const T* __restrict__ rhs,
@@ -678,13 +702,12 @@ __global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const
     if (t >= t1) return;
     const int32_t k0 = rp[t], d = diag[t];
     const T r = rscale ? (T)(rhs[t] * (T)rscale[0]) : rhs[t];
-    T s = 0;
-    for (int32_t k = k0 + lane; k < d; k += LPR) s += lu[k] * z[col[k]];
+    T s = U2 ? row_dot2<T>(col, lu, z, k0 + lane, d, LPR) : row_dot<T>(col, lu, z, k0 + lane, d, LPR);
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
     if (lane == 0) z[t] = FINAL ? (r - s) / lu[d] : r - s;
 }
 // backward sweep over one colour below the last, in place: z_t = (z_t - sum_U u_tj z_j) / u_tt
-template <class T, int LPR>
+template <class T, int LPR, bool U2 = true>
 __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
                                                    const T* __restrict__ lu, T* __restrict__ z, const int* __restrict__ ctrl) {
@@ -694,8 +717,7 @@ __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const
     const int lane = (int)(tid % LPR);
     if (t >= t1) return;
     const int32_t d = diag[t], e = rp[t + 1];
-    T s = 0;
-    for (int32_t k = d + 1 + lane; k < e; k += LPR) s += lu[k] * z[col[k]];
+    T s = U2 ? row_dot2<T>(col, lu, z, d + 1 + lane, e, LPR) : row_dot<T>(col, lu, z, d + 1 + lane, e, LPR);
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
     if (lane == 0) z[t] = (z[t] - s) / lu[d];
 }
@@ -980,12 +1002,18 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }
+    const bool u2 = row_unroll();
     switch (lpr) {
-        case 4: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 4>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
-        case 8: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 8>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
-        case 32: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 32>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
-        case 64: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 64>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
-        default: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 16>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
+        case 4: launch_timed(c, VFVM_K_SPMV, bytes, u2 ? k_spmv<T, 4, true> : k_spmv<T, 4, false>, dim3(grid), dim3(BLK), n, rp, cl,
+                             val, x, y, scale, ctrl); break;
+        case 8: launch_timed(c, VFVM_K_SPMV, bytes, u2 ? k_spmv<T, 8, true> : k_spmv<T, 8, false>, dim3(grid), dim3(BLK), n, rp, cl,
+                             val, x, y, scale, ctrl); break;
+        case 32: launch_timed(c, VFVM_K_SPMV, bytes, u2 ? k_spmv<T, 32, true> : k_spmv<T, 32, false>, dim3(grid), dim3(BLK), n, rp,
+                              cl, val, x, y, scale, ctrl); break;
+        case 64: launch_timed(c, VFVM_K_SPMV, bytes, u2 ? k_spmv<T, 64, true> : k_spmv<T, 64, false>, dim3(grid), dim3(BLK), n, rp,
+                              cl, val, x, y, scale, ctrl); break;
+        default: launch_timed(c, VFVM_K_SPMV, bytes, u2 ? k_spmv<T, 16, true> : k_spmv<T, 16, false>, dim3(grid), dim3(BLK), n, rp,
+                              cl, val, x, y, scale, ctrl); break;
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
@@ -1186,24 +1214,25 @@ int ilu_apply_lpr(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const do
     // algorithmic bytes of one sweep launch over r rows: the triangle's entries (value +
     // column, about half of each row), rp/diag (8 B), the row's rhs/z reads and writes
     const double per_row_entries = 0.5 * (double)sy.nnz / (double)sy.n * (sizeof(T) + 4.0);
+    const bool u2 = row_unroll();
     for (int q = 0; q < ncol; ++q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
         const dim3 grid(grid_for((int64_t)(e - s) * LPR));
         const double bytes = (double)(e - s) * (per_row_entries + 8.0 + 2.0 * sizeof(T));
         if (q == ncol - 1)
-            launch_timed(c, VFVM_K_PRECOND, bytes, k_ilu_lower<T, LPR, true>, grid, dim3(BLK), s, e, I.prp.p, I.pcol.p,
-                         I.pdiag.p, lu, rhs, rscale, z, ctrl);
+            launch_timed(c, VFVM_K_PRECOND, bytes, u2 ? k_ilu_lower<T, LPR, true, true> : k_ilu_lower<T, LPR, true, false>, grid,
+                         dim3(BLK), s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, rscale, z, ctrl);
         else
-            launch_timed(c, VFVM_K_PRECOND, bytes, k_ilu_lower<T, LPR, false>, grid, dim3(BLK), s, e, I.prp.p, I.pcol.p,
-                         I.pdiag.p, lu, rhs, rscale, z, ctrl);
+            launch_timed(c, VFVM_K_PRECOND, bytes, u2 ? k_ilu_lower<T, LPR, false, true> : k_ilu_lower<T, LPR, false, false>,
+                         grid, dim3(BLK), s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, rscale, z, ctrl);
     }
     for (int q = ncol - 2; q >= 0; --q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
         launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 8.0 + 2.0 * sizeof(T)),
-                     k_ilu_upper<T, LPR>, dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK),
-                     s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, z, ctrl);
+                     u2 ? k_ilu_upper<T, LPR, true> : k_ilu_upper<T, LPR, false>, dim3(grid_for((int64_t)(e - s) * LPR)),
+                     dim3(BLK), s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, z, ctrl);
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
