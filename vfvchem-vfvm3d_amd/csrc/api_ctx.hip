@@ -618,8 +618,10 @@ int vfvm::prepare_scalar_fields(Ctx& c, const vfvm_op* ops, int32_t nops, int32_
                 need_s[op.a] = 1;
         }
     }
+    // a flow's flux needs its velocity and density gradients; the flow whose data is
+    // current (skip_flw) had them computed with it
     for (size_t f = 0; f < c.flw.size(); ++f)
-        if (need_f[f]) { need_s[c.flw[f].den] = 1; need_v[c.flw[f].vel] = 1; }
+        if (need_f[f] && (int32_t)f != skip_flw) { need_s[c.flw[f].den] = 1; need_v[c.flw[f].vel] = 1; }
     // a function field's gradient is kept until its values are set again; an unknown's
     // values change with every solve
     for (size_t i = 0; i < c.scl.size(); ++i) {
