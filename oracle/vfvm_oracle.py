@@ -21,8 +21,26 @@ Pinning (how each function is checked independently of itself):
                     uniform velocity (closed form).
   assemble()        row sums, symmetry of diffusion, exactness on linear fields,
                     1D closed form T = x + q x(1-x)/(2k) of heat_analytic.rs:9,
-                    power-law limits; pattern {v} u ngh(v) / {v}.
+                    power-law limits; pattern {v} u ngh(v) / {v}; weighted advection,
+                    time-previous and time-unity reduce bitwise to advection-unity and
+                    time; production integrates linear fields exactly; time + production
+                    give the implicit-Euler map of m dc/dt = s c.
   solve (C, oracle/gmres_ref.c) and scipy SuperLU: residual and cross-check.
+  ilu0()            LU = A on the pattern; exact where ILU(0) drops no fill.
+  linear_map / adjoint_*   central finite differences of forward re-solves; the
+                    affine identity of the map.
+  grad_coeff()      sum_m c_m p_m = V grad(p) for linear p at every vertex.
+  update_flow()     flux antisymmetry; hydrostatic balance (linear p with matching body
+                    force gives no Rhie-Chow correction); rc > 0.
+  assemble_flow()   velocity-block row sums 0; gradient block on linear p = -V grad(p);
+                    divergence block on linear u = -rho V div(u); stabiliser annihilates
+                    linear p; transpose / dilatatory b vanish for linear u; time terms.
+  scaled_residual_flow / scaled_residual_scalar   hand-computed cases.
+  picard_flow / transient_flow   exercised by the GPU drivers' parity (same iteration
+                    counts) and by heatflow_solve's decoupled limit.
+  heatflow_solve()  conduction limit T = C = 1 - x exactly; decoupled limit equals
+                    picard_flow and a direct scalar solve; buoyancy sign.
+  transient_adaptive()   fixed mode, output-time landing, error falls with rtol.
 """
 from __future__ import annotations
 
