@@ -270,3 +270,33 @@ def test_adaptive_heatmassflow(ctx):
     assert st.t_final == 0.4 and st.accepted >= 3
     assert np.all(np.isfinite(S)) and np.min(S) >= -1e-6 and np.max(S) <= 1.0 + 1e-6
     assert np.all(np.isfinite(ctx.vecdom_get(hp.fp.ids[0])))
+
+
+def test_coupled_list_errors(ctx):
+    """Rejected coupled lists name the reason: a third scalar unknown (seven per vertex), a
+    scalar operator on the flow's pressure, a scalar operator on a non-unknown field, and
+    cross-diffusion into another unknown."""
+    print("[heat-flow list errors]", flush=True)
+    mesh = vfvm.Mesh.lattice(4, 0, 0.3)
+    hp = _cavity(mesh, species=True)
+    ops = hp.to_device(ctx)
+    T, C = hp.hid["T"], hp.hid["C"]
+    extra = ctx.scldom_add(vfvm.FIELD_UNKNOWN, 0.0)
+    k = hp.hid["k"]
+    cases = [
+        (ops + [(vfvm.OP_SCL_DOM_DIFFUSION, 0, extra, k, extra)], vfvm.VFVChemError, "at most two scalar unknowns"),
+        (ops + [(vfvm.OP_SCL_DOM_DIFFUSION, 0, k, k, k)], vfvm.VFVChemError, "unknown scalar field"),
+        ([o if not (o[0] == vfvm.OP_SCL_DOM_DIFFUSION and o[2] == C) else (o[0], o[1], C, o[3], T) for o in ops],
+         vfvm.VFVChemError, "cross-diffusion"),
+    ]
+    for bad, exc, msg in cases:
+        with pytest.raises(exc, match=msg):
+            ctx.pattern_build(bad)
+    # the valid list still builds afterwards
+    assert ctx.pattern_build(ops) > 0
+    # a scalar operator on the pressure (with one scalar block, so the block count holds)
+    hp = _cavity(mesh)
+    ops = hp.to_device(ctx)
+    pres, k = hp.fp.ids[1], hp.hid["k"]
+    with pytest.raises(vfvm.VFVChemError, match="pressure"):
+        ctx.pattern_build(ops + [(vfvm.OP_SCL_DOM_DIFFUSION, 0, pres, k, pres)])

@@ -389,16 +389,70 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
     const double vol_v = g.vol[v];
     double fcv[3];
     fce_centroid(g, f, v, fcv);
+    // Per-face data, lane q holding face q (deg_v <= 64): the face geometry, the face density,
+    // the neighbour's rc / volume / list offsets, its pressure slot and the stabiliser's
+    // body-force term, loaded and computed in one parallel pass and broadcast face by face
+    // (every value is the expression of the sequential walk, so the sums are unchanged).
+    int64_t fn = 0;
+    double f_area = 0.0, f_dist = 1.0, f_nx = 0.0, f_ny = 0.0, f_nz = 0.0, f_sx = 0.0, f_sy = 0.0, f_sz = 0.0;
+    double f_den = 0.0, f_rcn = 0.0, f_voln = 1.0, f_bterm = 0.0, gwx = 0.0, gwy = 0.0, gwz = 0.0;
+    int32_t f_rp = 0, f_deg = 0, f_gc = 0, f_glen = 0, f_psn = 0;
+    if (lane < deg_v) {
+        const int64_t e = es + lane;
+        fn = g.ngh[e];
+        f_area = g.area[e];
+        f_dist = g.dist[e];
+        f_nx = g.norm[e]; f_ny = g.norm[E + e]; f_nz = g.norm[2 * E + e];
+        f_sx = g.skew[e]; f_sy = g.skew[E + e]; f_sz = g.skew[2 * E + e];
+        gwx = g.gw[e]; gwy = g.gw[E + e]; gwz = g.gw[2 * E + e];
+        f_den = face_mean(f.den, f.deng, N, v, fn, f_sx, f_sy, f_sz);
+        f_rcn = f.rc[fn];
+        f_voln = g.vol[fn];
+        f_rp = g.rp[fn];
+        f_deg = g.rp[fn + 1] - f_rp;
+        f_gc = g.gc_ptr[fn];
+        f_glen = g.gc_ptr[fn + 1] - f_gc;
+        f_psn = find_slot(pc, plen, (int32_t)fn);
+        double fcn[3];
+        fce_centroid(g, f, fn, fcn);
+        double frc[3];
+        for (int c = 0; c < 3; ++c)
+            frc[c] = 0.5 * (rc_v + f_rcn) * 0.5 * (f.fce[c * N + v] + f.fce[c * N + fn]) - 0.5 * (rc_v * fcv[c] + f_rcn * fcn[c]);
+        f_bterm = f_den * (frc[0] * f_nx + frc[1] * f_ny + frc[2] * f_nz) * f_area;
+    }
     double bacc = 0.0;
     // divergence apply_time, before the stabiliser's pushes (opscl_dom_div.rs:40-66)
     if (a.transient) bacc += (den_v - f.den_prev[v]) * vol_v / a.dt;
-    for (int32_t e = es; e < ee; ++e) {
-        const int64_t n = g.ngh[e];
-        const double face = g.area[e], dist = g.dist[e];
-        const double nx = g.norm[e], ny = g.norm[E + e], nz = g.norm[2 * E + e];
-        const double sx = g.skew[e], sy = g.skew[E + e], sz = g.skew[2 * E + e];
-        const double den_f = face_mean(f.den, f.deng, N, v, n, sx, sy, sz);
-        const int sn = find_slot(gcv, glen, (int32_t)n);
+    for (int fi = 0; fi < deg_v; ++fi) {
+        const double face = __shfl(f_area, fi), dist = __shfl(f_dist, fi);
+        const double nx = __shfl(f_nx, fi), ny = __shfl(f_ny, fi), nz = __shfl(f_nz, fi);
+        const double sx = __shfl(f_sx, fi), sy = __shfl(f_sy, fi), sz = __shfl(f_sz, fi);
+        const double den_f = __shfl(f_den, fi);
+        const double rc_n = __shfl(f_rcn, fi), vol_n = __shfl(f_voln, fi);
+        const int32_t ns = __shfl(f_rp, fi), deg_n = __shfl(f_deg, fi);
+        const int32_t gn = __shfl(f_gc, fi), glen_n = __shfl(f_glen, fi);
+        const int psn = __shfl(f_psn, fi);
+        const double bterm = __shfl(f_bterm, fi);
+        const int sn = nsl[fi];
+        // the neighbour's lists, loaded ahead of the LDS phases: ngh(n) with its gradient
+        // weights, and the first 128 entries of grad_coeff(n)
+        double nw0 = 0.0, nw1 = 0.0, nw2 = 0.0;
+        int32_t nnb = 0;
+        if (lane < deg_n) {
+            const int32_t q = ns + lane;
+            nw0 = g.gw[q]; nw1 = g.gw[E + q]; nw2 = g.gw[2 * E + q];
+            nnb = g.ngh[q];
+        }
+        int32_t gv0 = 0, gv1 = 0;
+        double gx0 = 0.0, gy0 = 0.0, gz0 = 0.0, gx1 = 0.0, gy1 = 0.0, gz1 = 0.0;
+        if (lane < glen_n) {
+            const int64_t q = gn + lane;
+            gv0 = g.gc_vid[q]; gx0 = g.gc_c[q]; gy0 = g.gc_c[G + q]; gz0 = g.gc_c[2 * G + q];
+        }
+        if (lane + WAVE < glen_n) {
+            const int64_t q = gn + lane + WAVE;
+            gv1 = g.gc_vid[q]; gx1 = g.gc_c[q]; gy1 = g.gc_c[G + q]; gz1 = g.gc_c[2 * G + q];
+        }
         // ---- divergence: -rho_f vel_f . n A with the skew-corrected face velocity
         const double coeff = 0.5 * den_f * face;
         if (lane < 3) {
@@ -409,8 +463,7 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
         __syncthreads();
         // vertex gradient contribution: (vel_m - vel_v) terms over ngh(v)
         if (lane < deg_v) {
-            const int32_t q = es + lane;
-            const double w_skew = g.gw[q] * sx + g.gw[E + q] * sy + g.gw[2 * E + q] * sz;
+            const double w_skew = gwx * sx + gwy * sy + gwz * sz;
             const int sl = nsl[lane];
             for (int c = 0; c < 3; ++c) {
                 const double val = -coeff * w_skew * comp3(c, nx, ny, nz);
@@ -419,16 +472,17 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
             }
         }
         __syncthreads();
-        if (lane < 3)
-            for (int q = 0; q < deg_v; ++q) us[lane * gm + sv] += -tq[lane * WAVE + q];
+        if (lane < 3) {
+            // the chain into the v slot, summed in a register in push order
+            double acc = us[lane * gm + sv];
+            for (int q = 0; q < deg_v; ++q) acc += -tq[lane * WAVE + q];
+            us[lane * gm + sv] = acc;
+        }
         __syncthreads();
         // neighbour gradient contribution: (vel_m - vel_n) terms over ngh(n)
-        const int32_t ns = g.rp[n];
-        const int deg_n = g.rp[n + 1] - ns;
         if (lane < deg_n) {
-            const int32_t q = ns + lane;
-            const double w_skew = g.gw[q] * sx + g.gw[E + q] * sy + g.gw[2 * E + q] * sz;
-            const int sl = find_slot(gcv, glen, g.ngh[q]);
+            const double w_skew = nw0 * sx + nw1 * sy + nw2 * sz;
+            const int sl = find_slot(gcv, glen, nnb);
             for (int c = 0; c < 3; ++c) {
                 const double val = -coeff * w_skew * comp3(c, nx, ny, nz);
                 us[c * gm + sl] += val;
@@ -436,14 +490,15 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
             }
         }
         __syncthreads();
-        if (lane < 3)
-            for (int q = 0; q < deg_n; ++q) us[lane * gm + sn] += -tq[lane * WAVE + q];
+        if (lane < 3) {
+            double acc = us[lane * gm + sn];
+            for (int q = 0; q < deg_n; ++q) acc += -tq[lane * WAVE + q];
+            us[lane * gm + sn] = acc;
+        }
         // ---- pressure stabiliser: Rhie-Chow pressure terms and the body-force correction
-        const double rc_n = f.rc[n];
-        const double vol_n = g.vol[n];
         const double cpf = den_f * 0.5 * (rc_v + rc_n) * face;
         if (lane == 0) {
-            ps[find_slot(pc, plen, (int32_t)n)] += cpf / dist;
+            ps[psn] += cpf / dist;
             ps[psv] += -cpf / dist;
         }
         __syncthreads();
@@ -454,21 +509,14 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
         }
         __syncthreads();
         const double coeff_n = den_f * 0.5 * rc_n / vol_n * face;
-        const int32_t gn = g.gc_ptr[n];
-        const int glen_n = g.gc_ptr[n + 1] - gn;
-        for (int i = lane; i < glen_n; i += WAVE) {
+        if (lane < glen_n) ps[find_slot(pc, plen, gv0)] += -coeff_n * (gx0 * nx + gy0 * ny + gz0 * nz);
+        if (lane + WAVE < glen_n) ps[find_slot(pc, plen, gv1)] += -coeff_n * (gx1 * nx + gy1 * ny + gz1 * nz);
+        for (int i = lane + 2 * WAVE; i < glen_n; i += WAVE) {
             const int64_t q = gn + i;
             ps[find_slot(pc, plen, g.gc_vid[q])] += -coeff_n * (g.gc_c[q] * nx + g.gc_c[G + q] * ny + g.gc_c[2 * G + q] * nz);
         }
         __syncthreads();
-        if (lane == 0) {
-            double fcn[3];
-            fce_centroid(g, f, n, fcn);
-            double frc[3];
-            for (int c = 0; c < 3; ++c)
-                frc[c] = 0.5 * (rc_v + rc_n) * 0.5 * (f.fce[c * N + v] + f.fce[c * N + n]) - 0.5 * (rc_v * fcv[c] + rc_n * fcn[c]);
-            bacc += den_f * (frc[0] * nx + frc[1] * ny + frc[2] * nz) * face;
-        }
+        bacc += bterm;
     }
     // boundary sub-faces of a node: -(w_m rho_m A) n_c on vertices v, j, k
     if (v >= g.NP && lane < 3) {
