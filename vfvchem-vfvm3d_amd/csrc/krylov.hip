@@ -100,6 +100,18 @@ __device__ inline T row_dot2(const int32_t* __restrict__ col, const T* __restric
     if (k < e1) s += val[k] * x[col[k]];
     return s;
 }
+// Stop test of a row kernel after convergence (ctrl[0] == 0).  The control word is loaded
+// first (ctrl_go) and tested together with a condition on the row's first loads that always
+// holds (e.g. rp[t] <= rp[t + 1]): the control word and every load that depends only on the
+// row index are then one round trip to memory, and the column / value loads the next
+// (the test alone would be waited for before the row's loads issue).
+__device__ const int kCtrlGo = 1;
+__device__ const double kUnitScale = 1.0;
+// pointer selects instead of null-test branches, so that the loads issue without a wait
+__device__ inline int ctrl_go(const int* __restrict__ ctrl) { return *(ctrl ? ctrl : &kCtrlGo); }
+// rhs scale factor (1 when absent: x * 1 == x exactly)
+__device__ inline double scale_of(const double* __restrict__ scale) { return *(scale ? scale : &kUnitScale); }
+__device__ inline bool ctrl_stop(int go, bool loaded) { return (go == 0) & loaded; }
 // rows of the scalar SpMV and the ILU sweeps take two entries per trip unless
 // VFVM_ROW_UNROLL=0 (read on every launch, so one process can compare both; at 100^3
 // convection-diffusion: SpMV FP64 78.0 -> 71.7 us, FP32 54.4 -> 49.2 us, ILU sweep
@@ -113,14 +125,17 @@ template <class T, int LPR, bool U2 = true>
 __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                               const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
                                               const double* __restrict__ scale, const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[0] == 0) return;
     const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
     const int64_t row = tid / LPR;
     const int lane = (int)(tid % LPR);
     if (row >= n) return;
-    T s = U2 ? row_dot2<T>(col, val, x, rp[row] + lane, rp[row + 1], LPR) : row_dot<T>(col, val, x, rp[row] + lane, rp[row + 1], LPR);
+    const int go = ctrl_go(ctrl);
+    const int32_t k0 = rp[row], k1 = rp[row + 1];
+    const T sc = (T)scale_of(scale);
+    if (ctrl_stop(go, (k0 <= k1) | (sc != sc))) return;
+    T s = U2 ? row_dot2<T>(col, val, x, k0 + lane, k1, LPR) : row_dot<T>(col, val, x, k0 + lane, k1, LPR);
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-    if (lane == 0) y[row] = scale ? (T)(s * (T)scale[0]) : s;
+    if (lane == 0) y[row] = (T)(s * sc);
 }
 
 // SpMV of the coupled flow system [u, v, w, p] (rows and columns comp-major), one wave per
@@ -427,11 +442,11 @@ __global__ void __launch_bounds__(BLK) k_mdot(int64_t n, int nv, const T* __rest
 // out[k] = sig_k sum_blk part[k][blk]; one block per k
 __global__ void __launch_bounds__(BLK) k_reduce(int nblk, const double* __restrict__ part, const double* __restrict__ sig,
                                                 double* __restrict__ out, const int* __restrict__ ctrl) {
-    if (ctrl[0] == 0) return;
     __shared__ double sh[BLK / WAVE];
     const int k = blockIdx.x;
     double s = 0.0;
     for (int i = threadIdx.x; i < nblk; i += BLK) s += part[(int64_t)k * nblk + i];
+    if (ctrl[0] == 0) return;  // tested after the partials' loads issue: one round trip, not two
     s = block_sum(s, sh);
     if (threadIdx.x == 0) out[k] = s * sig[k];
 }
@@ -442,7 +457,6 @@ __global__ void __launch_bounds__(BLK) k_reduce(int nblk, const double* __restri
 // rotations in order.
 __global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int reorth, int nblk, const double* __restrict__ part_nrm,
                                                       double* __restrict__ S, StateLayout L, int* __restrict__ ctrl) {
-    if (ctrl[0] == 0) return;
     __shared__ double sh[BLK / WAVE];
     __shared__ double hc[MAX_RESTART + 1], cs_s[MAX_RESTART], sn_s[MAX_RESTART];
     const int m = L.m;
@@ -454,6 +468,7 @@ __global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int reorth, int nbl
     }
     double s = 0.0;
     for (int i = threadIdx.x; i < nblk; i += BLK) s += part_nrm[i];
+    if (ctrl[0] == 0) return;  // tested after the loads above issue (nothing is written before)
     s = block_sum(s, sh);  // contains the barrier that publishes hc/cs_s/sn_s
     if (threadIdx.x == 0) {
         const double hn = sqrt(s);
@@ -495,12 +510,13 @@ __global__ void __launch_bounds__(BLK) k_update(int64_t n, int nv, const T* __re
                                                 const double* __restrict__ h, const double* __restrict__ sig, const T* win,
                                                 T* wout, double* __restrict__ part_nrm, int64_t rows_per_blk,
                                                 const int* __restrict__ ctrl) {
-    if (ctrl[0] == 0) return;
     typedef typename Vec16<T>::type VT;
     constexpr int VW = Vec16<T>::N;
     __shared__ double hs[MAX_RESTART + 1];
     __shared__ double sh[BLK / WAVE];
+    const int go = ctrl[0];  // waited for with the coefficients' loads
     for (int k = threadIdx.x; k < nv; k += BLK) hs[k] = h[k] * sig[k];
+    if (go == 0) return;
     __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_blk;
     const int64_t r1 = min(n, r0 + rows_per_blk);
@@ -695,31 +711,80 @@ __global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const
                                                    const T* __restrict__ lu, const T* __restrict__ rhs,
                                                    const double* __restrict__ rscale, T* __restrict__ z,
                                                    const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[0] == 0) return;
     const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
     const int32_t t = t0 + (int32_t)(tid / LPR);
     const int lane = (int)(tid % LPR);
     if (t >= t1) return;
+    const int go = ctrl_go(ctrl);
     const int32_t k0 = rp[t], d = diag[t];
-    const T r = rscale ? (T)(rhs[t] * (T)rscale[0]) : rhs[t];
+    const T r = (T)(rhs[t] * (T)scale_of(rscale));
+    if (ctrl_stop(go, (k0 <= d) | (r != r))) return;
+    const T dd = FINAL ? lu[d] : (T)1;  // issued with the row's entries
     T s = U2 ? row_dot2<T>(col, lu, z, k0 + lane, d, LPR) : row_dot<T>(col, lu, z, k0 + lane, d, LPR);
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-    if (lane == 0) z[t] = FINAL ? (r - s) / lu[d] : r - s;
+    if (lane == 0) z[t] = FINAL ? (r - s) / dd : r - s;
+}
+// The first two colours' forward sweeps in one launch.  Colour 0 = rows [0, t0) has no L
+// entries (z_t = c rhs_t), and every L entry of colour 1 = rows [t0, t1) lies in colour 0,
+// so colour 1 reads c rhs_j where the colour-by-colour sweep reads z_j: the same value, so
+// the result is bitwise that of the two launches.  The colour-0 rows are written by a
+// grid-stride copy over the launch's threads.  (c = 1 without a scale: x * 1 == x.)
+template <class T, int LPR, bool FINAL, bool U2 = true>
+__global__ void __launch_bounds__(BLK) k_ilu_lower01(int32_t t0, int32_t t1, const int32_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
+                                                     const T* __restrict__ lu, const T* __restrict__ rhs,
+                                                     const double* __restrict__ rscale, T* __restrict__ z,
+                                                     const int* __restrict__ ctrl) {
+    const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
+    const int64_t nthr = (int64_t)gridDim.x * BLK;
+    const T c = (T)scale_of(rscale);
+    const int32_t t = t0 + (int32_t)(tid / LPR);
+    const int lane = (int)(tid % LPR);
+    const bool has_row = t < t1;
+    const int32_t tc = has_row ? t : t0;  // threads past the colour read row t0 (in bounds: t0 < t1)
+    const int go = ctrl_go(ctrl);
+    const int32_t k0 = rp[tc], d = diag[tc];
+    const T r = (T)(rhs[tc] * c);
+    const bool cp = tid < t0;  // this thread's first colour-0 row, loaded with the row bounds
+    const T cv = (T)(rhs[cp ? tid : 0] * c);
+    if (ctrl_stop(go, (k0 <= d) | (r != r) | (cv != cv))) return;
+    if (has_row) {
+        const T dd = FINAL ? lu[d] : (T)1;
+        T s = 0;
+        int32_t k = k0 + lane;
+        if (U2) {
+            for (; k + LPR < d; k += 2 * LPR) {
+                const int32_t c0 = col[k], c1 = col[k + LPR];
+                const T a0 = lu[k], a1 = lu[k + LPR];
+                const T x0 = (T)(rhs[c0] * c), x1 = (T)(rhs[c1] * c);
+                s += a0 * x0;
+                s += a1 * x1;
+            }
+        }
+        for (; k < d; k += LPR) s += lu[k] * (T)(rhs[col[k]] * c);
+        for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
+        if (lane == 0) z[t] = FINAL ? (r - s) / dd : r - s;
+    }
+    if (cp) z[tid] = cv;
+    for (int64_t i = tid + nthr; i < t0; i += nthr) z[i] = (T)(rhs[i] * c);
 }
 // backward sweep over one colour below the last, in place: z_t = (z_t - sum_U u_tj z_j) / u_tt
 template <class T, int LPR, bool U2 = true>
 __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
                                                    const T* __restrict__ lu, T* __restrict__ z, const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[0] == 0) return;
     const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
     const int32_t t = t0 + (int32_t)(tid / LPR);
     const int lane = (int)(tid % LPR);
     if (t >= t1) return;
+    const int go = ctrl_go(ctrl);
+    const T zt = z[t];  // row t's own entry: written below by this row only
     const int32_t d = diag[t], e = rp[t + 1];
+    if (ctrl_stop(go, (d < e) | (zt != zt))) return;
+    const T dd = lu[d];  // issued with the row's entries
     T s = U2 ? row_dot2<T>(col, lu, z, d + 1 + lane, e, LPR) : row_dot<T>(col, lu, z, d + 1 + lane, e, LPR);
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
-    if (lane == 0) z[t] = (z[t] - s) / lu[d];
+    if (lane == 0) z[t] = (zt - s) / dd;
 }
 
 // out[t] = in[perm[t]]
@@ -1204,9 +1269,16 @@ int ilu_factor(Ctx& c, const SysView& sy, T* lu, const double* val) {
     return VFVM_OK;
 }
 
+// colours 0 and 1 of the forward sweep in one launch unless VFVM_ILU_FUSE01=0
+inline bool ilu_fuse01() {
+    const char* e = getenv("VFVM_ILU_FUSE01");
+    return !(e && atoi(e) == 0);
+}
+
 // z = (LU)^-1 rhs in the colour-major space: forward sweep over the colours ascending (the
-// last one also finishing its backward sweep), backward sweep over the other colours
-// descending, in place in z (z must not alias rhs): 2 colours - 1 launches.
+// last one also finishing its backward sweep; colours 0 and 1 share a launch), backward
+// sweep over the other colours descending, in place in z (z must not alias rhs):
+// 2 colours - 2 launches.
 template <class T, int LPR>
 int ilu_apply_lpr(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double* rscale, T* z, const int* ctrl) {
     const IluData& I = *sy.ilu;
@@ -1215,7 +1287,21 @@ int ilu_apply_lpr(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const do
     // column, about half of each row), rp/diag (8 B), the row's rhs/z reads and writes
     const double per_row_entries = 0.5 * (double)sy.nnz / (double)sy.n * (sizeof(T) + 4.0);
     const bool u2 = row_unroll();
-    for (int q = 0; q < ncol; ++q) {
+    int q0 = 0;
+    if (ncol >= 2 && ilu_fuse01() && I.h_color_ptr[0] == 0 && I.h_color_ptr[2] > I.h_color_ptr[1]) {
+        // colours 0 and 1 in one launch (colour 0 is a scaled copy)
+        const int32_t s = I.h_color_ptr[1], e = I.h_color_ptr[2];
+        const dim3 grid(grid_for((int64_t)(e - s) * LPR));
+        const double bytes = (double)s * 2.0 * sizeof(T) + (double)(e - s) * (per_row_entries + 8.0 + 2.0 * sizeof(T));
+        if (ncol == 2)
+            launch_timed(c, VFVM_K_PRECOND, bytes, u2 ? k_ilu_lower01<T, LPR, true, true> : k_ilu_lower01<T, LPR, true, false>,
+                         grid, dim3(BLK), s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, rscale, z, ctrl);
+        else
+            launch_timed(c, VFVM_K_PRECOND, bytes, u2 ? k_ilu_lower01<T, LPR, false, true> : k_ilu_lower01<T, LPR, false, false>,
+                         grid, dim3(BLK), s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, rscale, z, ctrl);
+        q0 = 2;
+    }
+    for (int q = q0; q < ncol; ++q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
         const dim3 grid(grid_for((int64_t)(e - s) * LPR));
@@ -1379,7 +1465,7 @@ CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv
                                     (uint64_t)(uintptr_t)aval, (uint64_t)(uintptr_t)lu, (uint64_t)(uintptr_t)w.S,
                                     (uint64_t)(uintptr_t)w.ctrl, (uint64_t)(uintptr_t)c.part.p,
                                     (uint64_t)(uintptr_t)sy.row, (uint64_t)(uintptr_t)sy.col, (uint64_t)w.n,
-                                    (uint64_t)w.nnz, (uint64_t)w.m, (uint64_t)reorth, (uint64_t)sizeof(T),
+                                    (uint64_t)w.nnz, (uint64_t)w.m, (uint64_t)reorth + 2u * row_unroll() + 4u * ilu_fuse01(), (uint64_t)sizeof(T),
                                     (uint64_t)(uintptr_t)I, I ? I->generation : 0, (uint64_t)w.nblk, (uint64_t)w.nwav, (uint64_t)w.ldv};
     for (auto& g : c.graphs)
         if (g.key == key) { g.last_use = ++c.graph_clock; return &g; }
