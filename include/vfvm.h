@@ -105,6 +105,17 @@ int vfvm_mesh_from_tets(int64_t num_pts, int64_t num_nds, const double* pos,
                         int32_t num_bnd, const int64_t* bnd_ptr, const int32_t* bnd_nid,
                         vfvm_mesh** out);
 
+/* The same two constructors with the Voronoi dual built on GPU `device` (topology,
+ * face pieces, per-edge and per-vertex sums, least-squares weights; the boundary sub-faces
+ * on the host): every array is bitwise equal to the host build's.  Replaces
+ * Tetrahedron::update_tet (src/base/geom_tet.rs:58-293) like the host build; Delaunay
+ * stays on the host. */
+int vfvm_mesh_lattice_dev(int32_t n, uint64_t seed, double jitter, int32_t device, vfvm_mesh** out);
+int vfvm_mesh_from_tets_dev(int64_t num_pts, int64_t num_nds, const double* pos,
+                            int64_t num_tets, const int32_t* tets,
+                            int32_t num_bnd, const int64_t* bnd_ptr, const int32_t* bnd_nid,
+                            int32_t device, vfvm_mesh** out);
+
 int vfvm_mesh_info_get(const vfvm_mesh* mesh, vfvm_mesh_info* info);
 
 /* Builds the pressure-gradient coefficients of the flow operators: for every vertex v,

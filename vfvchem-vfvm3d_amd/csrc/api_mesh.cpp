@@ -12,22 +12,34 @@ struct vfvm_mesh {
     Mesh m;
 };
 
-extern "C" int vfvm_mesh_lattice(int32_t n, uint64_t seed, double jitter, vfvm_mesh** out) {
+// the dual on the host (dual_device < 0) or on that GPU
+static int make_dual(Mesh& m, int32_t dual_device, std::string& err) {
+    return dual_device < 0 ? build_dual(m, 0, err) : build_dual_device(m, dual_device, err);
+}
+
+static int mesh_lattice(int32_t n, uint64_t seed, double jitter, int32_t dual_device, vfvm_mesh** out) {
     if (!out) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_lattice: out is NULL");
     *out = nullptr;
     auto* h = new vfvm_mesh();
     std::string err;
     int st = lattice_mesh(n, seed, jitter, h->m, err);
-    if (st == VFVM_OK) st = build_dual(h->m, 0, err);
+    if (st == VFVM_OK) st = make_dual(h->m, dual_device, err);
     if (st != VFVM_OK) { delete h; return set_error(st, err); }
     *out = h;
     return clear_error();
 }
 
-extern "C" int vfvm_mesh_from_tets(int64_t num_pts, int64_t num_nds, const double* pos,
-                                   int64_t num_tets, const int32_t* tets,
-                                   int32_t num_bnd, const int64_t* bnd_ptr, const int32_t* bnd_nid,
-                                   vfvm_mesh** out) {
+extern "C" int vfvm_mesh_lattice(int32_t n, uint64_t seed, double jitter, vfvm_mesh** out) {
+    return mesh_lattice(n, seed, jitter, -1, out);
+}
+extern "C" int vfvm_mesh_lattice_dev(int32_t n, uint64_t seed, double jitter, int32_t device, vfvm_mesh** out) {
+    if (device < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_lattice_dev: negative device");
+    return mesh_lattice(n, seed, jitter, device, out);
+}
+
+static int mesh_from_tets(int64_t num_pts, int64_t num_nds, const double* pos, int64_t num_tets, const int32_t* tets,
+                          int32_t num_bnd, const int64_t* bnd_ptr, const int32_t* bnd_nid, int32_t dual_device,
+                          vfvm_mesh** out) {
     if (!out || !pos || !tets || num_pts < 0 || num_nds < 0 || num_tets <= 0 || num_bnd < 0 || (num_bnd > 0 && (!bnd_ptr || !bnd_nid)))
         return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_from_tets: invalid argument");
     *out = nullptr;
@@ -63,10 +75,24 @@ extern "C" int vfvm_mesh_from_tets(int64_t num_pts, int64_t num_nds, const doubl
         }
     }
     std::string err;
-    int st = build_dual(m, 0, err);
+    int st = make_dual(m, dual_device, err);
     if (st != VFVM_OK) { delete h; return set_error(st, err); }
     *out = h;
     return clear_error();
+}
+
+extern "C" int vfvm_mesh_from_tets(int64_t num_pts, int64_t num_nds, const double* pos,
+                                   int64_t num_tets, const int32_t* tets,
+                                   int32_t num_bnd, const int64_t* bnd_ptr, const int32_t* bnd_nid,
+                                   vfvm_mesh** out) {
+    return mesh_from_tets(num_pts, num_nds, pos, num_tets, tets, num_bnd, bnd_ptr, bnd_nid, -1, out);
+}
+extern "C" int vfvm_mesh_from_tets_dev(int64_t num_pts, int64_t num_nds, const double* pos,
+                                       int64_t num_tets, const int32_t* tets,
+                                       int32_t num_bnd, const int64_t* bnd_ptr, const int32_t* bnd_nid,
+                                       int32_t device, vfvm_mesh** out) {
+    if (device < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_from_tets_dev: negative device");
+    return mesh_from_tets(num_pts, num_nds, pos, num_tets, tets, num_bnd, bnd_ptr, bnd_nid, device, out);
 }
 
 extern "C" int vfvm_mesh_info_get(const vfvm_mesh* mesh, vfvm_mesh_info* info) {

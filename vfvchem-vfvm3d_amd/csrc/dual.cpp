@@ -50,9 +50,89 @@ inline V3 circumsphere3(V3 a, V3 b, V3 c, V3 d) {
 
 }  // namespace
 
+// Surface triangles (faces of one tetrahedron, flagged per (t, k)) in tetrahedron order,
+// counterclockwise seen from outside (geom_tet.rs:99-120): 3 vids per triangle.
+std::vector<int32_t> dual_surface_triangles(const Mesh& m, const std::vector<uint8_t>& is_surf) {
+    static const int SURF[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1}, {0, 1, 2}};
+    const int64_t T = (int64_t)m.tets.size() / 4;
+    const int32_t* tv = m.tets.data();
+    std::vector<int32_t> surf;
+    for (int64_t t = 0; t < T; ++t)
+        for (int k = 0; k < 4; ++k)
+            if (is_surf[4 * t + k]) {
+                surf.push_back(tv[4 * t + SURF[k][0]]);
+                surf.push_back(tv[4 * t + SURF[k][1]]);
+                surf.push_back(tv[4 * t + SURF[k][2]]);
+            }
+    return surf;
+}
+
+// Points (interior vertices) must not lie on the surface (geom_tet.rs:118-120).
+int dual_check_surface(const Mesh& m, const std::vector<int32_t>& surf, std::string& err) {
+    const int64_t NP = m.num_pts;
+    std::vector<uint8_t> on(NP, 0);
+    int64_t num_bnd = 0;
+    for (int32_t v : surf) if (v < NP && !on[v]) { on[v] = 1; ++num_bnd; }
+    if (num_bnd > 0) { err = "dual: " + std::to_string(num_bnd) + " points lie on the domain boundary"; return VFVM_ERR_TETRA_POINTS_ON_BOUNDARY; }
+    return VFVM_OK;
+}
+
+// Boundary sub-faces (geom_tet.rs:208-259), in surface-triangle order.
+void dual_subfaces(Mesh& m, const std::vector<int32_t>& surf) {
+    const int64_t N = m.num_tot();
+    auto P = [&](int64_t v) { return V3{m.pos[3 * v], m.pos[3 * v + 1], m.pos[3 * v + 2]}; };
+    const int64_t NS = (int64_t)surf.size() / 3;
+    m.bsf_ptr.assign(N + 1, 0);
+    for (int64_t k = 0; k < 3 * NS; ++k) m.bsf_ptr[surf[k] + 1] += 1;
+    for (int64_t v = 0; v < N; ++v) m.bsf_ptr[v + 1] += m.bsf_ptr[v];
+    const int64_t S = m.bsf_ptr[N];
+    m.bsf_vj.assign(S, 0); m.bsf_vk.assign(S, 0); m.bsf_area.assign(S, 0.0);
+    m.bsf_norm.assign(3 * S, 0.0); m.bsf_wgt.assign(3 * S, 0.0);
+    m.bnd_sum.assign(N, 0.0);
+    {
+        std::vector<int32_t> fill(m.bsf_ptr.begin(), m.bsf_ptr.end() - 1);
+        for (int64_t f = 0; f < NS; ++f) {
+            const int32_t* t = &surf[3 * f];
+            V3 p0 = P(t[0]), p1 = P(t[1]), p2 = P(t[2]);
+            V3 nrm = cross3(sub3(p1, p0), sub3(p2, p0));
+            nrm = scale3(1.0 / len3(nrm), nrm);
+            V3 c = circumcircle3(p0, p1, p2);
+            double sq[3] = {dot3(sub3(p2, p1), sub3(p2, p1)), dot3(sub3(p0, p2), sub3(p0, p2)), dot3(sub3(p1, p0), sub3(p1, p0))};
+            double lam_raw[3] = {sq[0] * (sq[1] + sq[2] - sq[0]), sq[1] * (sq[2] + sq[0] - sq[1]), sq[2] * (sq[0] + sq[1] - sq[2])};
+            double lam_sum = lam_raw[0] + lam_raw[1] + lam_raw[2];
+            double lam[3] = {lam_raw[0] / lam_sum, lam_raw[1] / lam_sum, lam_raw[2] / lam_sum};
+            for (int loc = 0; loc < 3; ++loc) {
+                int32_t vi = t[loc], vj = t[(loc + 1) % 3], vk = t[(loc + 2) % 3];
+                V3 pi = P(vi), pj = P(vj), pk = P(vk);
+                double ar[2];
+                for (int side = 0; side < 2; ++side) {
+                    V3 pa = side == 0 ? pj : pk, pb = side == 0 ? pk : pj;
+                    V3 mm = scale3(0.5, add3(pi, pa));
+                    V3 e = sub3(pa, pi);
+                    double len = len3(e);
+                    V3 u = sub3(sub3(pb, mm), scale3(dot3(sub3(pb, mm), e) / (len * len), e));
+                    ar[side] = 0.25 * len * dot3(sub3(c, mm), scale3(1.0 / len3(u), u));
+                }
+                double area_sum = ar[0] + ar[1];
+                double l0 = lam[loc], l1 = lam[(loc + 1) % 3], l2 = lam[(loc + 2) % 3];
+                V3 cent_j = {(1.5 + l0) / 3.0, (0.5 + l1) / 3.0, l2 / 3.0};
+                V3 cent_k = {(1.5 + l0) / 3.0, l1 / 3.0, (0.5 + l2) / 3.0};
+                V3 w = {1.0, 0.0, 0.0};
+                if (std::fabs(area_sum) > 1e-12 * len3(sub3(pj, pi)) * len3(sub3(pk, pi)))
+                    w = scale3(1.0 / area_sum, add3(scale3(ar[0], cent_j), scale3(ar[1], cent_k)));
+                m.bnd_sum[vi] += area_sum;
+                int32_t x = fill[vi]++;
+                m.bsf_vj[x] = vj; m.bsf_vk[x] = vk; m.bsf_area[x] = area_sum;
+                m.bsf_norm[x] = nrm.x; m.bsf_norm[S + x] = nrm.y; m.bsf_norm[2 * S + x] = nrm.z;
+                m.bsf_wgt[x] = w.x; m.bsf_wgt[S + x] = w.y; m.bsf_wgt[2 * S + x] = w.z;
+            }
+        }
+    }
+
+}
+
 int build_dual(Mesh& m, int /*nthreads*/, std::string& err) {
     const int64_t N = m.num_tot();
-    const int64_t NP = m.num_pts;
     const int64_t T = (int64_t)m.tets.size() / 4;
     if (T == 0) { err = "dual: no tetrahedra"; return VFVM_ERR_TETRA_DEGENERATE_INPUT; }
     const int32_t* tv = m.tets.data();
@@ -84,22 +164,9 @@ int build_dual(Mesh& m, int /*nthreads*/, std::string& err) {
         s = e;
     }
     std::vector<FaceRec>().swap(faces);
-    static const int SURF[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1}, {0, 1, 2}};
-    std::vector<int32_t> surf;  // 3 per surface triangle, counterclockwise seen from outside
-    for (int64_t t = 0; t < T; ++t)
-        for (int k = 0; k < 4; ++k)
-            if (is_surf[4 * t + k]) {
-                surf.push_back(tv[4 * t + SURF[k][0]]);
-                surf.push_back(tv[4 * t + SURF[k][1]]);
-                surf.push_back(tv[4 * t + SURF[k][2]]);
-            }
+    std::vector<int32_t> surf = dual_surface_triangles(m, is_surf);
     std::vector<uint8_t>().swap(is_surf);
-    {
-        std::vector<uint8_t> on(NP, 0);
-        int64_t num_bnd = 0;
-        for (int32_t v : surf) if (v < NP && !on[v]) { on[v] = 1; ++num_bnd; }
-        if (num_bnd > 0) { err = "dual: " + std::to_string(num_bnd) + " points lie on the domain boundary"; return VFVM_ERR_TETRA_POINTS_ON_BOUNDARY; }
-    }
+    if (int st = dual_check_surface(m, surf, err)) return st;
 
     // ---- neighbour CSR, ascending per row
     std::vector<int64_t> cnt(N + 1, 0);
@@ -214,54 +281,8 @@ int build_dual(Mesh& m, int /*nthreads*/, std::string& err) {
     }
     std::vector<V3>().swap(cent_sum);
 
-    // ---- step 4: boundary sub-faces (geom_tet.rs:208-259)
-    const int64_t NS = (int64_t)surf.size() / 3;
-    m.bsf_ptr.assign(N + 1, 0);
-    for (int64_t k = 0; k < 3 * NS; ++k) m.bsf_ptr[surf[k] + 1] += 1;
-    for (int64_t v = 0; v < N; ++v) m.bsf_ptr[v + 1] += m.bsf_ptr[v];
-    const int64_t S = m.bsf_ptr[N];
-    m.bsf_vj.assign(S, 0); m.bsf_vk.assign(S, 0); m.bsf_area.assign(S, 0.0);
-    m.bsf_norm.assign(3 * S, 0.0); m.bsf_wgt.assign(3 * S, 0.0);
-    m.bnd_sum.assign(N, 0.0);
-    {
-        std::vector<int32_t> fill(m.bsf_ptr.begin(), m.bsf_ptr.end() - 1);
-        for (int64_t f = 0; f < NS; ++f) {
-            const int32_t* t = &surf[3 * f];
-            V3 p0 = P(t[0]), p1 = P(t[1]), p2 = P(t[2]);
-            V3 nrm = cross3(sub3(p1, p0), sub3(p2, p0));
-            nrm = scale3(1.0 / len3(nrm), nrm);
-            V3 c = circumcircle3(p0, p1, p2);
-            double sq[3] = {dot3(sub3(p2, p1), sub3(p2, p1)), dot3(sub3(p0, p2), sub3(p0, p2)), dot3(sub3(p1, p0), sub3(p1, p0))};
-            double lam_raw[3] = {sq[0] * (sq[1] + sq[2] - sq[0]), sq[1] * (sq[2] + sq[0] - sq[1]), sq[2] * (sq[0] + sq[1] - sq[2])};
-            double lam_sum = lam_raw[0] + lam_raw[1] + lam_raw[2];
-            double lam[3] = {lam_raw[0] / lam_sum, lam_raw[1] / lam_sum, lam_raw[2] / lam_sum};
-            for (int loc = 0; loc < 3; ++loc) {
-                int32_t vi = t[loc], vj = t[(loc + 1) % 3], vk = t[(loc + 2) % 3];
-                V3 pi = P(vi), pj = P(vj), pk = P(vk);
-                double ar[2];
-                for (int side = 0; side < 2; ++side) {
-                    V3 pa = side == 0 ? pj : pk, pb = side == 0 ? pk : pj;
-                    V3 mm = scale3(0.5, add3(pi, pa));
-                    V3 e = sub3(pa, pi);
-                    double len = len3(e);
-                    V3 u = sub3(sub3(pb, mm), scale3(dot3(sub3(pb, mm), e) / (len * len), e));
-                    ar[side] = 0.25 * len * dot3(sub3(c, mm), scale3(1.0 / len3(u), u));
-                }
-                double area_sum = ar[0] + ar[1];
-                double l0 = lam[loc], l1 = lam[(loc + 1) % 3], l2 = lam[(loc + 2) % 3];
-                V3 cent_j = {(1.5 + l0) / 3.0, (0.5 + l1) / 3.0, l2 / 3.0};
-                V3 cent_k = {(1.5 + l0) / 3.0, l1 / 3.0, (0.5 + l2) / 3.0};
-                V3 w = {1.0, 0.0, 0.0};
-                if (std::fabs(area_sum) > 1e-12 * len3(sub3(pj, pi)) * len3(sub3(pk, pi)))
-                    w = scale3(1.0 / area_sum, add3(scale3(ar[0], cent_j), scale3(ar[1], cent_k)));
-                m.bnd_sum[vi] += area_sum;
-                int32_t x = fill[vi]++;
-                m.bsf_vj[x] = vj; m.bsf_vk[x] = vk; m.bsf_area[x] = area_sum;
-                m.bsf_norm[x] = nrm.x; m.bsf_norm[S + x] = nrm.y; m.bsf_norm[2 * S + x] = nrm.z;
-                m.bsf_wgt[x] = w.x; m.bsf_wgt[S + x] = w.y; m.bsf_wgt[2 * S + x] = w.z;
-            }
-        }
-    }
+    dual_subfaces(m, surf);
+    std::vector<int32_t>().swap(surf);
 
     // ---- step 5: least-squares gradient weights (geom_tet.rs:261-293)
     m.gw.assign(3 * E, 0.0);

@@ -61,6 +61,12 @@ int lattice_mesh(int n, uint64_t seed, double jitter, Mesh& m, std::string& err)
 int delaunay_int(const std::vector<int64_t>& ixyz, const std::vector<int32_t>& order,
                  std::vector<int32_t>& tets_out, std::string& err);
 int build_dual(Mesh& m, int nthreads, std::string& err);
+// The same dual built on GPU `device` (bitwise equal to build_dual; dual_gpu.hip).
+int build_dual_device(Mesh& m, int device, std::string& err);
+// Pieces of build_dual shared with the device build.
+std::vector<int32_t> dual_surface_triangles(const Mesh& m, const std::vector<uint8_t>& is_surf);
+int dual_check_surface(const Mesh& m, const std::vector<int32_t>& surf, std::string& err);
+void dual_subfaces(Mesh& m, const std::vector<int32_t>& surf);
 int build_grad_coeff(Mesh& m, std::string& err);
 
 }  // namespace vfvm

@@ -120,6 +120,8 @@ SIGNATURES = [
     ("vfvm_version", C.c_char_p, []),
     ("vfvm_mesh_lattice", _I, [_I, C.c_uint64, _D, C.POINTER(_P)]),
     ("vfvm_mesh_from_tets", _I, [_L, _L, _PD, _L, _PI, _I, _PL, _PI, C.POINTER(_P)]),
+    ("vfvm_mesh_lattice_dev", _I, [_I, C.c_uint64, _D, _I, C.POINTER(_P)]),
+    ("vfvm_mesh_from_tets_dev", _I, [_L, _L, _PD, _L, _PI, _I, _PL, _PI, _I, C.POINTER(_P)]),
     ("vfvm_mesh_info_get", _I, [_P, C.POINTER(MeshInfo)]),
     ("vfvm_mesh_build_grad_coeff", _I, [_P]),
     ("vfvm_mesh_get", _I, [_P, _I, _P]),

@@ -86,13 +86,17 @@ class Mesh:
         return self
 
     @classmethod
-    def lattice(cls, n: int, seed: int = 0, jitter: float = 0.3) -> "Mesh":
+    def lattice(cls, n: int, seed: int = 0, jitter: float = 0.3, device=None) -> "Mesh":
+        """Jittered lattice mesh; the dual is built on GPU `device` when given (bitwise equal)."""
         h = C.c_void_p()
-        check(lib.vfvm_mesh_lattice(int(n), int(seed), float(jitter), C.byref(h)))
+        if device is None:
+            check(lib.vfvm_mesh_lattice(int(n), int(seed), float(jitter), C.byref(h)))
+        else:
+            check(lib.vfvm_mesh_lattice_dev(int(n), int(seed), float(jitter), int(device), C.byref(h)))
         return cls(h)
 
     @classmethod
-    def from_tets(cls, pos, tets, num_pts: int, bnd_nids) -> "Mesh":
+    def from_tets(cls, pos, tets, num_pts: int, bnd_nids, device=None) -> "Mesh":
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
         bptr = np.zeros(len(bnd_nids) + 1, np.int64)
@@ -100,8 +104,13 @@ class Mesh:
             bptr[b + 1] = bptr[b] + len(ids)
         bnid = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in bnd_nids]) if bnd_nids else np.zeros(0, np.int32), np.int32)
         h = C.c_void_p()
-        check(lib.vfvm_mesh_from_tets(int(num_pts), int(pos.shape[0] - num_pts), ptr(pos), int(tets.shape[0]), ptr(tets, C.c_int32),
-                                      len(bnd_nids), ptr(bptr, C.c_int64), ptr(bnid, C.c_int32), C.byref(h)))
+        if device is None:
+            check(lib.vfvm_mesh_from_tets(int(num_pts), int(pos.shape[0] - num_pts), ptr(pos), int(tets.shape[0]), ptr(tets, C.c_int32),
+                                          len(bnd_nids), ptr(bptr, C.c_int64), ptr(bnid, C.c_int32), C.byref(h)))
+        else:
+            check(lib.vfvm_mesh_from_tets_dev(int(num_pts), int(pos.shape[0] - num_pts), ptr(pos), int(tets.shape[0]),
+                                              ptr(tets, C.c_int32), len(bnd_nids), ptr(bptr, C.c_int64), ptr(bnid, C.c_int32),
+                                              int(device), C.byref(h)))
         return cls(h)
 
     def _size(self, name):
