@@ -13,6 +13,8 @@
  *
  * Reference interfaces replaced (paths relative to the reference repository):
  *   Tetrahedron::update_tet            src/base/geom_tet.rs:58      -> vfvm_mesh_from_tets / vfvm_mesh_lattice
+ *                                                                      (dual on a GPU: vfvm_mesh_from_tets_dev / vfvm_mesh_lattice_dev)
+ *   Tetrahedron::grad_coeff            src/base/geom_tet.rs:295     -> vfvm_mesh_build_grad_coeff (_dev on a GPU)
  *   Variables::add_scldom_* / add_sclbnd_* / add_sclflw / add_vecdom_*
  *                                      src/base/vars.rs:80-183      -> vfvm_scldom_add / vfvm_sclbnd_add / vfvm_vecdom_add / vfvm_sclflw_add
  *   OperatorBase::apply / apply_time   src/operator/operator_base.rs:5-7,
@@ -123,6 +125,9 @@ int vfvm_mesh_info_get(const vfvm_mesh* mesh, vfvm_mesh_info* info);
  * correction of the face values fully implicit (Tetrahedron::grad_coeff,
  * src/base/geom_tet.rs:295-347).  Needed before uploading a mesh for the flow operators. */
 int vfvm_mesh_build_grad_coeff(vfvm_mesh* mesh);
+/* The same coefficients built on GPU `device`, bitwise equal to the host build
+ * (Tetrahedron::grad_coeff, src/base/geom_tet.rs:295-347). */
+int vfvm_mesh_build_grad_coeff_dev(vfvm_mesh* mesh, int32_t device);
 
 /* Array ids for vfvm_mesh_get (element type and length in brackets). */
 enum vfvm_mesh_array {

@@ -68,3 +68,32 @@ def test_device_dual_large_lattice_and_speed():
     t2 = time.perf_counter()
     assert_same_dual(h, d)
     print(f"lattice {n}^3 with host dual {t1 - t0:.2f} s, with device dual {t2 - t1:.2f} s", flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,seed", [(4, 0), (7, 3), (12, 0)])
+def test_device_grad_coeff_equals_host(n, seed):
+    print(f"device grad_coeff: lattice n={n} seed={seed}", flush=True)
+    h = vfvm.Mesh.lattice(n, seed, 0.3).build_grad_coeff()
+    d = vfvm.Mesh.lattice(n, seed, 0.3).build_grad_coeff(device=0)
+    assert h.num_grad_coeff == d.num_grad_coeff
+    for name in ("gc_ptr", "gc_vid", "gc_c"):
+        assert np.array_equal(bits(h.get(name)), bits(d.get(name))), f"{name} differs"
+
+
+@pytest.mark.gpu
+def test_device_grad_coeff_large_and_speed():
+    n = 40
+    print(f"device grad_coeff: lattice n={n}", flush=True)
+    m = vfvm.Mesh.lattice(n, 0, 0.3, device=0)
+    pos, tets = m.get("pos"), m.get("tets")
+    bnd = [m.bnd_nodes(b) for b in range(6)]
+    h = vfvm.Mesh.from_tets(pos, tets, m.num_pts, bnd)
+    t0 = time.perf_counter()
+    h.build_grad_coeff()
+    t1 = time.perf_counter()
+    m.build_grad_coeff(device=0)
+    t2 = time.perf_counter()
+    for name in ("gc_ptr", "gc_vid", "gc_c"):
+        assert np.array_equal(bits(h.get(name)), bits(m.get(name))), f"{name} differs"
+    print(f"grad_coeff {n}^3: host {t1 - t0:.2f} s, device {t2 - t1:.2f} s", flush=True)

@@ -118,6 +118,13 @@ extern "C" int vfvm_mesh_build_grad_coeff(vfvm_mesh* mesh) {
     if (st != VFVM_OK) return set_error(st, err);
     return clear_error();
 }
+extern "C" int vfvm_mesh_build_grad_coeff_dev(vfvm_mesh* mesh, int32_t device) {
+    if (!mesh || device < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_build_grad_coeff_dev: NULL mesh or negative device");
+    std::string err;
+    int st = build_grad_coeff_device(mesh->m, device, err);
+    if (st != VFVM_OK) return set_error(st, err);
+    return clear_error();
+}
 
 template <class T>
 static int copy_out(const std::vector<T>& v, void* dst) {

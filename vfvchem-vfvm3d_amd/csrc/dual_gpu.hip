@@ -14,7 +14,9 @@
 // Data on the device: tets (4T), vertex -> tetrahedra lists (CSR, 4T entries, ascending),
 // face pieces per (tetrahedron, edge, side) = (area, centroid) (48 doubles per tetrahedron),
 // the neighbour CSR and the per-edge / per-vertex results, copied back into the host mesh.
-// The boundary sub-faces (surface only) are built on the host (dual_subfaces).
+// The boundary sub-faces (surface only) are built on the host (dual_subfaces).  The flow
+// operators' pressure-gradient coefficients (grad_coeff, geom_tet.rs:295-347) are built the
+// same way (build_grad_coeff_device), bitwise equal to grad_coeff.cpp.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -330,6 +332,98 @@ __global__ void k_lsq(int64_t n, const int32_t* __restrict__ rp, const int32_t* 
     }
 }
 
+
+// ---- pressure-gradient coefficients (geom_tet.rs:295-347), restating grad_coeff.cpp as a
+// gather.  The coefficients of vertex v are listed for every vertex of its 2-ring
+// R2(v) = {v} u ngh(v) u ngh(ngh(v)), ascending.  Coefficient (v, m) is the sum, in the
+// reference's push order, of the terms pushed for m: per face e = (v, n) the arithmetic
+// means (to v, to n), then for s = v and s = n the skew-correction pairs (to ngh(s)[q], to s)
+// in q order, then per boundary sub-face the three interpolation terms.  For m != s only
+// the pair whose neighbour is m contributes (found by binary search in the ascending row of
+// s); for m == s every pair's second term does.  The first term starts the sum (no 0 + x).
+__global__ void k_r2_build(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ngh,
+                           const int32_t* __restrict__ cap, int32_t* __restrict__ scr, int32_t* __restrict__ cnt) {
+    const int64_t v = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (v >= n) return;
+    int32_t* s = scr + cap[v];
+    int32_t m = 0;
+    auto insert = [&](int32_t u) {
+        int32_t j = m - 1;
+        while (j >= 0 && s[j] > u) --j;
+        if (j >= 0 && s[j] == u) return;
+        for (int32_t r = m; r > j + 1; --r) s[r] = s[r - 1];
+        s[j + 1] = u;
+        ++m;
+    };
+    insert((int32_t)v);
+    for (int32_t e = rp[v]; e < rp[v + 1]; ++e) {
+        const int32_t nb = ngh[e];
+        insert(nb);
+        for (int32_t q = rp[nb]; q < rp[nb + 1]; ++q) insert(ngh[q]);
+    }
+    cnt[v] = m;
+}
+__global__ void k_r2_copy(int64_t n, const int32_t* __restrict__ cap, const int32_t* __restrict__ scr,
+                          const int32_t* __restrict__ gp, int32_t* __restrict__ gvid, int32_t* __restrict__ grow) {
+    const int64_t v = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (v >= n) return;
+    for (int32_t k = gp[v]; k < gp[v + 1]; ++k) {
+        gvid[k] = scr[cap[v] + (k - gp[v])];
+        grow[k] = (int32_t)v;
+    }
+}
+struct GcAcc {
+    double x = 0.0, y = 0.0, z = 0.0;
+    bool first = true;
+    __device__ void add(double a, double b, double c) {
+        if (first) { x = a; y = b; z = c; first = false; }
+        else { x += a; y += b; z += c; }
+    }
+};
+__global__ void k_gc_coeff(int64_t G, const int32_t* __restrict__ grow, const int32_t* __restrict__ gvid,
+                           const int32_t* __restrict__ rp, const int32_t* __restrict__ ngh, int64_t E,
+                           const double* __restrict__ area_a, const double* __restrict__ norm, const double* __restrict__ skew,
+                           const double* __restrict__ gw, const int32_t* __restrict__ bsp, const int32_t* __restrict__ bvj,
+                           const int32_t* __restrict__ bvk, int64_t S, const double* __restrict__ barea,
+                           const double* __restrict__ bnorm, const double* __restrict__ bwgt, double* __restrict__ gc) {
+    const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (k >= G) return;
+    const int32_t v = grow[k], m = gvid[k];
+    GcAcc acc;
+    for (int32_t e = rp[v]; e < rp[v + 1]; ++e) {
+        const int32_t nb = ngh[e];
+        const double area = area_a[e];
+        const double nx = norm[e], ny = norm[E + e], nz = norm[2 * E + e];
+        const double sx = skew[e], sy = skew[E + e], sz = skew[2 * E + e];
+        if (m == v) acc.add(0.5 * area * nx, 0.5 * area * ny, 0.5 * area * nz);
+        if (m == nb) acc.add(0.5 * area * nx, 0.5 * area * ny, 0.5 * area * nz);
+        for (int side = 0; side < 2; ++side) {
+            const int32_t sv = side == 0 ? v : nb;
+            if (m == sv) {
+                for (int32_t q = rp[sv]; q < rp[sv + 1]; ++q) {
+                    const double w_skew = 0.5 * (gw[q] * sx + gw[E + q] * sy + gw[2 * E + q] * sz) * area;
+                    acc.add(-w_skew * nx, -w_skew * ny, -w_skew * nz);
+                }
+            } else {
+                const int32_t q = slot_of(rp, ngh, sv, m);
+                if (q < rp[sv + 1] && ngh[q] == m) {
+                    const double w_skew = 0.5 * (gw[q] * sx + gw[E + q] * sy + gw[2 * E + q] * sz) * area;
+                    acc.add(w_skew * nx, w_skew * ny, w_skew * nz);
+                }
+            }
+        }
+    }
+    for (int32_t b = bsp[v]; b < bsp[v + 1]; ++b) {
+        const double area = barea[b];
+        const double nx = bnorm[b], ny = bnorm[S + b], nz = bnorm[2 * S + b];
+        const double wv = bwgt[b], wj = bwgt[S + b], wk = bwgt[2 * S + b];
+        if (m == v) acc.add(wv * area * nx, wv * area * ny, wv * area * nz);
+        if (m == bvj[b]) acc.add(wj * area * nx, wj * area * ny, wj * area * nz);
+        if (m == bvk[b]) acc.add(wk * area * nx, wk * area * ny, wk * area * nz);
+    }
+    gc[k] = acc.x; gc[G + k] = acc.y; gc[2 * G + k] = acc.z;
+}
+
 template <class T>
 struct Buf {
     T* p = nullptr;
@@ -473,6 +567,90 @@ int build_dual_device(Mesh& m, int device, std::string& err) {
         return VFVM_ERR_TETRA_SINGULAR_GRADIENT;
     }
     m.has_dual = true;
+    return VFVM_OK;
+}
+
+int build_grad_coeff_device(Mesh& m, int device, std::string& err) {
+    if (!m.has_dual) { err = "grad_coeff: mesh without dual"; return VFVM_ERR_INVALID_ARGUMENT; }
+    if (m.has_grad_coeff) return VFVM_OK;
+    const int64_t N = m.num_tot(), E = m.num_edges(), S = m.num_subfaces();
+    // scratch capacity of each vertex's 2-ring: 1 + deg(v) + sum of its neighbours' degrees
+    std::vector<int32_t> h_cap(N + 1, 0);
+    {
+        int64_t acc = 0;
+        for (int64_t v = 0; v < N; ++v) {
+            int64_t c = 1;
+            for (int32_t e = m.row_ptr[v]; e < m.row_ptr[v + 1]; ++e) c += 1 + (m.row_ptr[m.ngh[e] + 1] - m.row_ptr[m.ngh[e]]);
+            h_cap[v] = (int32_t)acc;
+            acc += c;
+            if (acc > INT32_MAX) { err = "grad_coeff: 2-ring scratch exceeds 2^31 entries"; return VFVM_ERR_INVALID_ARGUMENT; }
+        }
+        h_cap[N] = (int32_t)acc;
+    }
+    int prev = 0;
+    DUAL_TRY(hipGetDevice(&prev));
+    DUAL_TRY(hipSetDevice(device));
+    struct Restore { int d; ~Restore() { (void)hipSetDevice(d); } } restore{prev};
+    hipStream_t st = nullptr;
+    DUAL_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard { hipStream_t s; ~StreamGuard() { if (s) (void)hipStreamDestroy(s); } } sg{st};
+
+    Buf<int32_t> rp, ngh, cap, scr, cnt;
+    DUAL_TRY(rp.alloc(N + 1));
+    DUAL_TRY(ngh.alloc(E));
+    DUAL_TRY(cap.alloc(N + 1));
+    DUAL_TRY(scr.alloc(h_cap[N]));
+    DUAL_TRY(cnt.alloc(N));
+    DUAL_TRY(hipMemcpyAsync(rp.p, m.row_ptr.data(), (N + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    DUAL_TRY(hipMemcpyAsync(ngh.p, m.ngh.data(), E * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    DUAL_TRY(hipMemcpyAsync(cap.p, h_cap.data(), (N + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_r2_build, dim3(grid_of(N)), dim3(BLK), 0, st, N, rp.p, ngh.p, cap.p, scr.p, cnt.p);
+    std::vector<int32_t> h_cnt(N);
+    DUAL_TRY(hipMemcpyAsync(h_cnt.data(), cnt.p, N * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    DUAL_TRY(hipStreamSynchronize(st));
+    std::vector<int32_t> gptr;
+    if (!scan_counts(h_cnt, gptr)) { err = "grad_coeff: more than 2^31 coefficients"; return VFVM_ERR_INVALID_ARGUMENT; }
+    const int64_t G = gptr[N];
+    Buf<int32_t> gp, gvid, grow, bsp, bvj, bvk;
+    Buf<double> area, norm, skew, gw, barea, bnorm, bwgt, gc;
+    DUAL_TRY(gp.alloc(N + 1));
+    DUAL_TRY(gvid.alloc(G));
+    DUAL_TRY(grow.alloc(G));
+    DUAL_TRY(hipMemcpyAsync(gp.p, gptr.data(), (N + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_r2_copy, dim3(grid_of(N)), dim3(BLK), 0, st, N, cap.p, scr.p, gp.p, gvid.p, grow.p);
+    DUAL_TRY(area.alloc(E));
+    DUAL_TRY(norm.alloc(3 * E));
+    DUAL_TRY(skew.alloc(3 * E));
+    DUAL_TRY(gw.alloc(3 * E));
+    DUAL_TRY(bsp.alloc(N + 1));
+    DUAL_TRY(bvj.alloc(S));
+    DUAL_TRY(bvk.alloc(S));
+    DUAL_TRY(barea.alloc(S));
+    DUAL_TRY(bnorm.alloc(3 * S));
+    DUAL_TRY(bwgt.alloc(3 * S));
+    DUAL_TRY(gc.alloc(3 * G));
+    DUAL_TRY(hipMemcpyAsync(area.p, m.area.data(), E * sizeof(double), hipMemcpyHostToDevice, st));
+    DUAL_TRY(hipMemcpyAsync(norm.p, m.norm.data(), 3 * E * sizeof(double), hipMemcpyHostToDevice, st));
+    DUAL_TRY(hipMemcpyAsync(skew.p, m.skew.data(), 3 * E * sizeof(double), hipMemcpyHostToDevice, st));
+    DUAL_TRY(hipMemcpyAsync(gw.p, m.gw.data(), 3 * E * sizeof(double), hipMemcpyHostToDevice, st));
+    DUAL_TRY(hipMemcpyAsync(bsp.p, m.bsf_ptr.data(), (N + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    if (S > 0) {
+        DUAL_TRY(hipMemcpyAsync(bvj.p, m.bsf_vj.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        DUAL_TRY(hipMemcpyAsync(bvk.p, m.bsf_vk.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        DUAL_TRY(hipMemcpyAsync(barea.p, m.bsf_area.data(), S * sizeof(double), hipMemcpyHostToDevice, st));
+        DUAL_TRY(hipMemcpyAsync(bnorm.p, m.bsf_norm.data(), 3 * S * sizeof(double), hipMemcpyHostToDevice, st));
+        DUAL_TRY(hipMemcpyAsync(bwgt.p, m.bsf_wgt.data(), 3 * S * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    hipLaunchKernelGGL(k_gc_coeff, dim3(grid_of(G)), dim3(BLK), 0, st, G, grow.p, gvid.p, rp.p, ngh.p, E, area.p, norm.p, skew.p,
+                       gw.p, bsp.p, bvj.p, bvk.p, S, barea.p, bnorm.p, bwgt.p, gc.p);
+    DUAL_TRY(hipGetLastError());
+    m.gc_ptr = std::move(gptr);
+    m.gc_vid.resize(G);
+    m.gc_c.resize(3 * G);
+    DUAL_TRY(hipMemcpyAsync(m.gc_vid.data(), gvid.p, G * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    DUAL_TRY(hipMemcpyAsync(m.gc_c.data(), gc.p, 3 * G * sizeof(double), hipMemcpyDeviceToHost, st));
+    DUAL_TRY(hipStreamSynchronize(st));
+    m.has_grad_coeff = true;
     return VFVM_OK;
 }
 

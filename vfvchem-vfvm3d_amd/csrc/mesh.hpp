@@ -68,5 +68,7 @@ std::vector<int32_t> dual_surface_triangles(const Mesh& m, const std::vector<uin
 int dual_check_surface(const Mesh& m, const std::vector<int32_t>& surf, std::string& err);
 void dual_subfaces(Mesh& m, const std::vector<int32_t>& surf);
 int build_grad_coeff(Mesh& m, std::string& err);
+// The same coefficients built on GPU `device`, bitwise equal (dual_gpu.hip).
+int build_grad_coeff_device(Mesh& m, int device, std::string& err);
 
 }  // namespace vfvm

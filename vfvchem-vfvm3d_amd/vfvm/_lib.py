@@ -124,6 +124,7 @@ SIGNATURES = [
     ("vfvm_mesh_from_tets_dev", _I, [_L, _L, _PD, _L, _PI, _I, _PL, _PI, _I, C.POINTER(_P)]),
     ("vfvm_mesh_info_get", _I, [_P, C.POINTER(MeshInfo)]),
     ("vfvm_mesh_build_grad_coeff", _I, [_P]),
+    ("vfvm_mesh_build_grad_coeff_dev", _I, [_P, _I]),
     ("vfvm_mesh_get", _I, [_P, _I, _P]),
     ("vfvm_mesh_destroy", None, [_P]),
     ("vfvm_create", _I, [_I, C.POINTER(_P)]),

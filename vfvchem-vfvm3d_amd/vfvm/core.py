@@ -77,9 +77,13 @@ class Mesh:
         self.num_grad_coeff = info.num_grad_coeff
         self._cache = {}
 
-    def build_grad_coeff(self):
-        """Pressure-gradient coefficients of the flow operators (geom_tet.rs:295-347)."""
-        check(lib.vfvm_mesh_build_grad_coeff(self._h))
+    def build_grad_coeff(self, device=None):
+        """Pressure-gradient coefficients of the flow operators (geom_tet.rs:295-347); built
+        on GPU `device` when given (bitwise equal to the host build)."""
+        if device is None:
+            check(lib.vfvm_mesh_build_grad_coeff(self._h))
+        else:
+            check(lib.vfvm_mesh_build_grad_coeff_dev(self._h, int(device)))
         info = L.MeshInfo()
         check(lib.vfvm_mesh_info_get(self._h, C.byref(info)))
         self.num_grad_coeff = info.num_grad_coeff
