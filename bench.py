@@ -277,21 +277,60 @@ def build_heat_mass_flow(vfvm, n, Ra, Pr, Sc):
     return mesh, ctx, ops, hook
 
 
+def hmf_temperature_id(ops, vfvm):
+    """Field id of the temperature: the unknown of the first scalar time-previous operator."""
+    for op in ops:
+        if op[0] == vfvm.OP_SCL_DOM_TIME_PREVIOUS:
+            return op[2]
+    raise ValueError("no temperature operator")
+
+
 def run_heat_mass_flow(vfvm, args):
     """Config-5 system (natural convection with a passive species, six unknowns per vertex):
     implicit-Euler steps from rest with Picard per step, then the linear solve of the
     transient system at the last state timed FP64 vs FP32-inner/FP64-refined with GMRES(100)
     + 6x6 block Jacobi (solved DOF/s at 1e-10)."""
     import torch
+    import numpy as np
     t0 = time.time()
     mesh, ctx, ops, hook = build_heat_mass_flow(vfvm, args.hmf_n, 1e4, 0.71, 1.0)
-    ctx.set_update_hook(hook)
     t_setup = time.time() - t0
     N = mesh.num_tot
     ndof = 6 * N
     dt = 1e-2
+    # Hot-wall Nusselt number: the conductive heat flux out of the x = 0 nodes' control
+    # volumes through their faces, sum (A / d)(T_v - T_n), over k dT / L times the wall area
+    # (both 1); exactly 1 for the conduction profile T = 1 - x (the faces of a wall node's
+    # control volume close with its boundary face).  J = mean of Nu over the time steps.
+    rp, ngh = mesh.get("row_ptr"), mesh.get("ngh")
+    wall = mesh.num_pts + mesh.bnd_nodes(0).astype(np.int64)
+    deg = rp[wall + 1] - rp[wall]
+    eidx = np.concatenate([np.arange(rp[v], rp[v + 1]) for v in wall])
+    w_src, w_dst = np.repeat(wall, deg), ngh[eidx]
+    w_coef = mesh.get("area")[eidx] / mesh.get("dist")[eidx]
+    nus = {}
+
+    def nusselt():
+        Tv = ctx.scldom_get(T_field[0])
+        return float(np.sum(w_coef * (Tv[w_src] - Tv[w_dst])))
+
+    T_field = [None]
+    last_t = [None]
+    offset = [0]  # steps done by earlier driver calls
+
+    def hook_nu(t):
+        # called before every assembly with the step's time: when the time moves on, the
+        # temperature holds the converged state of the previous step
+        if last_t[0] is not None and t != last_t[0] and last_t[0] > 0.0:
+            nus[offset[0] + round(last_t[0] / dt)] = nusselt()
+            if len(nus) % 10 == 0:
+                progress(f"heat-mass-flow: step {len(nus)}, Nu = {nus[max(nus)]:.4f}")
+        last_t[0] = t
+        hook(t)
     lin = dict(tol=1e-8, restart=args.hmf_restart, max_outer=100, precision=vfvm.FP32_REFINED,
                precond=vfvm.PRECOND_BJACOBI, inner_tol=1e-4, ortho=1)
+    T_field[0] = hmf_temperature_id(ops, vfvm)
+    ctx.set_update_hook(hook_nu)
     tp = time.time()
     note = None
     st = None
@@ -300,6 +339,9 @@ def run_heat_mass_flow(vfvm, args):
     except vfvm.VFVChemError as e:
         note = str(e)
     t_steps = time.time() - tp
+    if st is not None:
+        nus[args.hmf_ts] = nusselt()  # the last step's converged state
+    ctx.set_update_hook(hook)
     progress(f"heat-mass-flow {args.hmf_n}^3: {args.hmf_ts} time steps in {t_steps:.1f} s")
     _, nnz = ctx.system_info()
     ta = time.time()
@@ -308,6 +350,20 @@ def run_heat_mass_flow(vfvm, args):
     t_asm = time.time() - ta
     out = timed_block_solves(vfvm, ctx, ndof, args.tol, args.hmf_restart, args.hmf_inner_tol, args.hmf_steps,
                              "heat-mass-flow", max_outer=100, events_first=True)
+    # the remaining steps of the run, for the objective
+    t_more = 0.0
+    more = max(0, args.hmf_ts_total - args.hmf_ts)
+    if st is not None and more > 0:
+        offset[0], last_t[0] = args.hmf_ts, None
+        ctx.set_update_hook(hook_nu)
+        tm = time.time()
+        try:
+            ctx.transient_solve(ops, dt, more, 8, 1e-6, 1.0, lin=lin)
+            nus[args.hmf_ts_total] = nusselt()
+        except vfvm.VFVChemError as e:
+            note = str(e)
+        t_more = time.time() - tm
+        progress(f"heat-mass-flow {args.hmf_n}^3: {more} more time steps in {t_more:.1f} s")
     ctx.set_update_hook(None)
     ctx.close()
     return {
@@ -319,6 +375,12 @@ def run_heat_mass_flow(vfvm, args):
         "assembly_wall_ms": 1e3 * t_asm, "time_steps_s": t_steps,
         "picard_iterations": st.picard_iterations if st is not None else None, "steps_note": note,
         "mesh_dual_gc_s": t_setup,
+        "objective": {"J_time_averaged_hot_wall_nusselt": float(np.mean([nus[k] for k in sorted(nus)])) if nus else None,
+                      "steps": len(nus), "t_end": dt * max(nus) if nus else None,
+                      "nu_first": nus[min(nus)] if nus else None, "nu_last": nus[max(nus)] if nus else None,
+                      "steps_s": t_steps + t_more,
+                      "note": "J = mean hot-wall Nusselt number over the implicit-Euler steps (rectangle rule); "
+                              "its adjoint w.r.t. the hot-wall profile is not built"},
     }
 
 
@@ -410,6 +472,7 @@ def main():
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--hmf-n", type=int, default=100, help="lattice points per axis of the heat-mass-flow system")
     ap.add_argument("--hmf-ts", type=int, default=10, help="implicit-Euler steps before the timed heat-mass-flow solves")
+    ap.add_argument("--hmf-ts-total", type=int, default=100, help="implicit-Euler steps of the heat-mass-flow run (objective J)")
     ap.add_argument("--hmf-steps", type=int, default=1)
     ap.add_argument("--hmf-restart", type=int, default=100, help="GMRES restart of the heat-mass-flow solves")
     ap.add_argument("--hmf-inner-tol", type=float, default=1e-6, help="inner tolerance of the mixed heat-mass-flow solve")
