@@ -34,7 +34,6 @@ namespace {
 
 constexpr int BLK = 256;
 constexpr int WAVE = 64;
-constexpr int CHUNK = 8;  // basis vectors per pass of the multi-dot
 constexpr int MAX_RESTART = 256;
 
 // ---------------------------------------------------------------- state layout (doubles)
@@ -386,33 +385,33 @@ template <> __device__ inline double vget<double>(const double2& v, int e) { ret
 
 // ---------------------------------------------------------------- multi-dot: part[k][blk] = V[k] . w over a block's rows
 // 2-D grid: x = row ranges (BLK / WAVE waves per block, one range per wave, a multiple of
-// the vector width), y = groups of CHUNK basis vectors.  Each lane streams 16-byte vectors
-// of w and of CHUNK basis vectors (ldv a multiple of the vector width) and keeps CHUNK dot
+// the vector width), y = groups of CH basis vectors.  Each lane streams 16-byte vectors
+// of w and of CH basis vectors (ldv a multiple of the vector width) and keeps CH dot
 // products in registers; the block's waves are summed in LDS into one partial per vector.
-template <class T>
+template <class T, int CH>
 __global__ void __launch_bounds__(BLK) k_mdot(int64_t n, int nv, const T* __restrict__ V, int64_t ldv,
                                               const T* __restrict__ w, double* __restrict__ part, int64_t rows_per_wave,
                                               const int* __restrict__ ctrl) {
     if (ctrl[0] == 0) return;
     typedef typename Vec16<T>::type VT;
     constexpr int VW = Vec16<T>::N;
-    __shared__ double wsum[BLK / WAVE][CHUNK];
+    __shared__ double wsum[BLK / WAVE][CH];
     const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
     const int nbx = gridDim.x;
     const int wv = blockIdx.x * (BLK / WAVE) + wid;
-    const int k0 = blockIdx.y * CHUNK;
+    const int k0 = blockIdx.y * CH;
     const int64_t r0 = (int64_t)wv * rows_per_wave;
     const int64_t r1 = min(n, r0 + rows_per_wave);
     const int64_t rv = r1 > r0 ? r0 + (r1 - r0) / VW * VW : r0;  // end of the vector part
-    const int kq = min(CHUNK, nv - k0);
+    const int kq = min(CH, nv - k0);
     const T* Vk = V + (int64_t)k0 * ldv;
-    double acc[CHUNK];
+    double acc[CH];
 #pragma unroll
-    for (int q = 0; q < CHUNK; ++q) acc[q] = 0.0;
+    for (int q = 0; q < CH; ++q) acc[q] = 0.0;
     for (int64_t i = r0 + (int64_t)lane * VW; i < rv; i += (int64_t)WAVE * VW) {
         const VT w4 = *reinterpret_cast<const VT*>(w + i);
 #pragma unroll
-        for (int q = 0; q < CHUNK; ++q) {
+        for (int q = 0; q < CH; ++q) {
             if (q < kq) {
                 const VT v4 = *reinterpret_cast<const VT*>(Vk + (int64_t)q * ldv + i);
 #pragma unroll
@@ -423,11 +422,11 @@ __global__ void __launch_bounds__(BLK) k_mdot(int64_t n, int nv, const T* __rest
     for (int64_t i = rv + lane; i < r1; i += WAVE) {
         const double wi = (double)w[i];
 #pragma unroll
-        for (int q = 0; q < CHUNK; ++q)
+        for (int q = 0; q < CH; ++q)
             if (q < kq) acc[q] += (double)Vk[(int64_t)q * ldv + i] * wi;
     }
 #pragma unroll
-    for (int q = 0; q < CHUNK; ++q) {
+    for (int q = 0; q < CH; ++q) {
         const double v = wave_sum(acc[q]);
         if (lane == 0) wsum[wid][q] = v;
     }
@@ -1118,14 +1117,35 @@ int residual_launch(Ctx& c, const SysView& sy, int lpr, const double* x, double*
 
 // hout[k] = sig_k V[k] . wv over the basis vectors [0, nv): per-block partials, then one
 // reduction block per vector
+// basis vectors per multi-dot pass (VFVM_MDOT_CHUNK = 2, 4, 8 or 16; default 2); each
+// vector's partial sums are the same for all (same lane, rows and order).  At 100^3 the
+// average multi-dot takes FP32 21.4 / 22.4 / 24.4 / 34.5 us and FP64 40.3 / 42.9 / 47.4 /
+// 65.5 us for 2 / 4 / 8 / 16: fewer accumulators per lane keep more waves in flight, and
+// the re-reads of w hit L2
+inline int mdot_chunk() {
+    const char* e = getenv("VFVM_MDOT_CHUNK");
+    const int v = e ? atoi(e) : 2;
+    return (v == 4 || v == 8 || v == 16) ? v : 2;
+}
 template <class T>
 void mdot_launch(Ctx& c, Work& w, int nv, const T* V, const T* wv, double* part, double* hout) {
     const int nbx = w.nwav / (BLK / WAVE);
-    const dim3 grid(nbx, (nv + CHUNK - 1) / CHUNK), block(BLK);
     // algorithmic bytes: the nv basis vectors and w once
     const double bytes = (double)sizeof(T) * w.n * (nv + 1);
-    launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
-                 (const int*)w.ctrl);
+    const int ch = mdot_chunk();
+    const dim3 grid(nbx, (nv + ch - 1) / ch), block(BLK);
+    if (ch == 16)
+        launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T, 16>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
+                     (const int*)w.ctrl);
+    else if (ch == 8)
+        launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T, 8>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
+                     (const int*)w.ctrl);
+    else if (ch == 2)
+        launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T, 2>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
+                     (const int*)w.ctrl);
+    else
+        launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T, 4>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
+                     (const int*)w.ctrl);
     hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, nbx, (const double*)part, (const double*)(w.S + w.L.sig()),
                        hout, (const int*)w.ctrl);
 }
@@ -1554,7 +1574,7 @@ CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv
                                     (uint64_t)(uintptr_t)aval, (uint64_t)(uintptr_t)lu, (uint64_t)(uintptr_t)w.S,
                                     (uint64_t)(uintptr_t)w.ctrl, (uint64_t)(uintptr_t)c.part.p,
                                     (uint64_t)(uintptr_t)sy.row, (uint64_t)(uintptr_t)sy.col, (uint64_t)w.n,
-                                    (uint64_t)w.nnz, (uint64_t)w.m, (uint64_t)reorth + 2u * row_unroll() + 4u * ilu_fuse01(), (uint64_t)sizeof(T),
+                                    (uint64_t)w.nnz, (uint64_t)w.m, (uint64_t)reorth + 2u * row_unroll() + 4u * ilu_fuse01() + 8u * (uint64_t)mdot_chunk(), (uint64_t)sizeof(T),
                                     (uint64_t)(uintptr_t)I, I ? I->generation : 0, (uint64_t)w.nblk, (uint64_t)w.nwav, (uint64_t)w.ldv};
     for (auto& g : c.graphs)
         if (g.key == key) { g.last_use = ++c.graph_clock; return &g; }
@@ -1803,7 +1823,13 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     w.m = m;
     w.nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 4 * BLK - 1) / (4 * BLK)));
     w.rows_per_blk = ((n + w.nblk - 1) / w.nblk + 3) / 4 * 4;
-    w.nwav = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + WAVE * 8 - 1) / (WAVE * 8)));
+    // multi-dot waves per group of basis vectors (VFVM_MDOT_WAVES, default 1024)
+    static const int64_t mdot_waves = [] {
+        const char* e = getenv("VFVM_MDOT_WAVES");
+        const int64_t v = e ? atoll(e) : 1024;
+        return std::min<int64_t>(8192, std::max<int64_t>(64, v));
+    }();
+    w.nwav = (int)std::min<int64_t>(mdot_waves, std::max<int64_t>(1, (n + WAVE * 8 - 1) / (WAVE * 8)));
     w.nwav = (w.nwav + BLK / WAVE - 1) / (BLK / WAVE) * (BLK / WAVE);
     w.rows_per_wave = ((n + w.nwav - 1) / w.nwav + 3) / 4 * 4;
     w.ldv = (n + 63) / 64 * 64;
