@@ -599,12 +599,15 @@ def _liboracle():
     PD = C.POINTER(C.c_double)
     lib.orc_gmres.restype = C.c_int
     lib.orc_gmres.argtypes = [C.c_int64, P64, P64, PD, PD, PD, C.c_int, C.c_int, C.c_double, C.c_int64, P64, PD]
+    lib.orc_gmres_threads.restype = C.c_int
+    lib.orc_gmres_threads.argtypes = [C.c_int64, P64, P64, PD, PD, PD, C.c_int, C.c_int, C.c_double, C.c_int64, C.c_int, P64, PD]
     return lib
 
 
-def gmres(rp, col, val, b, x0=None, restart=50, max_outer=200, tol=1e-6, max_steps=0):
-    """Jacobi-preconditioned restarted GMRES in FP64, single thread
-    (oracle/gmres_ref.c; LinearGmres::solve_matrix, linear_gmres.rs:76-84).
+def gmres(rp, col, val, b, x0=None, restart=50, max_outer=200, tol=1e-6, max_steps=0, threads=1):
+    """Jacobi-preconditioned restarted GMRES in FP64 (oracle/gmres_ref.c;
+    LinearGmres::solve_matrix, linear_gmres.rs:76-84), on one thread unless `threads` > 1
+    (OpenMP vector loops; used to time the CPU baseline on all host cores).
     Returns (x, converged, steps, rel_residual)."""
     import ctypes as C
     lib = _liboracle()
@@ -618,8 +621,13 @@ def gmres(rp, col, val, b, x0=None, restart=50, max_outer=200, tol=1e-6, max_ste
     rel = C.c_double()
     P64 = C.POINTER(C.c_int64)
     PD = C.POINTER(C.c_double)
-    st = lib.orc_gmres(n, rp.ctypes.data_as(P64), col.ctypes.data_as(P64), val.ctypes.data_as(PD), b.ctypes.data_as(PD),
-                       x.ctypes.data_as(PD), int(restart), int(max_outer), float(tol), int(max_steps), C.byref(it), C.byref(rel))
+    if threads > 1:
+        st = lib.orc_gmres_threads(n, rp.ctypes.data_as(P64), col.ctypes.data_as(P64), val.ctypes.data_as(PD),
+                                   b.ctypes.data_as(PD), x.ctypes.data_as(PD), int(restart), int(max_outer), float(tol),
+                                   int(max_steps), int(threads), C.byref(it), C.byref(rel))
+    else:
+        st = lib.orc_gmres(n, rp.ctypes.data_as(P64), col.ctypes.data_as(P64), val.ctypes.data_as(PD), b.ctypes.data_as(PD),
+                           x.ctypes.data_as(PD), int(restart), int(max_outer), float(tol), int(max_steps), C.byref(it), C.byref(rel))
     return x, st == 0, it.value, rel.value
 
 
