@@ -15,6 +15,7 @@
  *   Tetrahedron::update_tet            src/base/geom_tet.rs:58      -> vfvm_mesh_from_tets / vfvm_mesh_lattice
  *                                                                      (dual on a GPU: vfvm_mesh_from_tets_dev / vfvm_mesh_lattice_dev)
  *   Tetrahedron::grad_coeff            src/base/geom_tet.rs:295     -> vfvm_mesh_build_grad_coeff (_dev on a GPU)
+ *   write_{scl,vec}{dom,bnd}_csv       src/base/write_csv.rs:5-100  -> vfvm_write_csv
  *   Variables::add_scldom_* / add_sclbnd_* / add_sclflw / add_vecdom_*
  *                                      src/base/vars.rs:80-183      -> vfvm_scldom_add / vfvm_sclbnd_add / vfvm_vecdom_add / vfvm_sclflw_add
  *   OperatorBase::apply / apply_time   src/operator/operator_base.rs:5-7,
@@ -66,7 +67,8 @@ enum vfvm_status {
     VFVM_ERR_STEADY_DID_NOT_CONVERGE = 45,       /* SteadyDidNotConverge       steady_base.rs:116     */
     VFVM_ERR_STEADY_NON_FINITE_RESIDUAL = 46,    /* SteadyNonFiniteResidual    steady_base.rs:101     */
     VFVM_ERR_TRANSIENT_DID_NOT_CONVERGE = 47,    /* TransientDidNotConverge    transient_base.rs:127  */
-    VFVM_ERR_TRANSIENT_NON_FINITE_RESIDUAL = 48  /* TransientNonFiniteResidual transient_base.rs:119  */
+    VFVM_ERR_TRANSIENT_NON_FINITE_RESIDUAL = 48, /* TransientNonFiniteResidual transient_base.rs:119  */
+    VFVM_ERR_WRITE_CSV = 50                      /* WriteCsv                   error.rs:116            */
 };
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -155,6 +157,16 @@ enum vfvm_mesh_array {
     VFVM_MESH_GC_C = 21      /* f64 [3G] SoA (c_x, c_y, c_z)                              */
 };
 int vfvm_mesh_get(const vfvm_mesh* mesh, int32_t array_id, void* dst);
+
+/* CSV output of a field (write_scldom_csv / write_vecdom_csv / write_sclbnd_csv /
+ * write_vecbnd_csv, src/base/write_csv.rs:5-100): file "<file_name>_<ts, 4 digits>.<file_type>",
+ * header "x,y,z,value" (ncomp = 1) or "x,y,z,value_x,value_y,value_z" (ncomp = 3), one line per
+ * vertex in vertex order (points, then nodes), or per node of boundary `bnd` (bnd >= 0, values
+ * in the boundary's node order).  values: ncomp arrays of that length, back to back.
+ * digits = 9 writes the reference's fixed 9-decimal format; digits = 0 writes every number
+ * with 17 significant digits (reads back to the same double).  Host only. */
+int vfvm_write_csv(const vfvm_mesh* mesh, int32_t bnd, int32_t ncomp, const double* values, const char* file_name,
+                   const char* file_type, int32_t ts, int32_t digits);
 void vfvm_mesh_destroy(vfvm_mesh* mesh);
 
 /* ============================ device context ============================== */

@@ -56,6 +56,8 @@ class Variables:
         self.vecdom = []
         self.vecbnd = []
         self.sclflw = []
+        self.outputs = []       # (kind, field id, file_path) of fields written by the drivers
+        self.t_offset = 0.0     # time of the driver call's start (transient output in chunks)
 
     @staticmethod
     def new_lattice(n: int, seed: int = 0, jitter: float = 0.3, device: int = 0) -> "Variables":
@@ -79,9 +81,21 @@ class Variables:
         return self.pos[self.mesh.num_pts + nids]
 
     # scalar domain fields (vars.rs:80-102)
+    def _output(self, kind: str, i: int, file_path: str) -> None:
+        # file_path "" = no output; "<name>.csv" (scl_dom.rs:56-66); vtu output is not built
+        if not file_path:
+            return
+        name, sep, typ = file_path.rpartition(".")
+        if not sep:
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, f"UnknownOutputType: {file_path}")
+        if typ != "csv":
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, f"UnsupportedOutputType: {typ} (csv output only)")
+        self.outputs.append((kind, i, file_path))
+
     def add_scldom_cnst(self, dom_id: int, value: float, file_path: str = "") -> int:
         i = self.ctx.scldom_add(FIELD_CONSTANT, value)
         self.scldom.append((FIELD_CONSTANT, None, []))
+        self._output("scldom", i, file_path)
         return i
 
     def add_scldom_func(self, dom_id: int, func, unkn_id=(), file_path: str = "") -> int:
@@ -89,16 +103,19 @@ class Variables:
         vals = np.asarray(func(0.0, self.pos, [self.ctx.scldom_get(u) for u in unkn_id]), np.float64)
         i = self.ctx.scldom_add(FIELD_FUNCTION, 0.0, vals)
         self.scldom.append((FIELD_FUNCTION, func, list(unkn_id)))
+        self._output("scldom", i, file_path)
         return i
 
     def add_scldom_unkn(self, dom_id: int, value_init: float, file_path: str = "") -> int:
         i = self.ctx.scldom_add(FIELD_UNKNOWN, value_init)
         self.scldom.append((FIELD_UNKNOWN, None, []))
+        self._output("scldom", i, file_path)
         return i
 
     def add_sclbnd_cnst(self, bnd_id: int, value: float, file_path: str = "") -> int:
         i = self.ctx.sclbnd_add(self.bnd[bnd_id], value)
-        self.sclbnd.append((bnd_id, None, []))
+        self.sclbnd.append((bnd_id, None, [], float(value)))
+        self._output("sclbnd", i, file_path)
         return i
 
     def _bnd_unkn(self, bnd_id, unkn):
@@ -109,13 +126,15 @@ class Variables:
         """func(t, pos[count,3], [values of each unknown on the boundary's nodes]) -> values[count]."""
         vals = np.asarray(func(0.0, self._bnd_pos(bnd_id), self._bnd_unkn(bnd_id, unkn_id)), np.float64)
         i = self.ctx.sclbnd_add(self.bnd[bnd_id], 0.0, vals)
-        self.sclbnd.append((bnd_id, func, list(unkn_id)))
+        self.sclbnd.append((bnd_id, func, list(unkn_id), None))
+        self._output("sclbnd", i, file_path)
         return i
 
     # vector domain fields (vars.rs:145-167); values SoA (3, N)
     def add_vecdom_cnst(self, dom_id: int, value, file_path: str = "") -> int:
         i = self.ctx.vecdom_add(FIELD_CONSTANT, cnst3=np.asarray(value, np.float64))
         self.vecdom.append((FIELD_CONSTANT, None, []))
+        self._output("vecdom", i, file_path)
         return i
 
     def add_vecdom_func(self, dom_id: int, func, unkn_id=(), file_path: str = "") -> int:
@@ -123,23 +142,27 @@ class Variables:
         vals = np.asarray(func(0.0, self.pos, [self.ctx.scldom_get(u) for u in unkn_id]), np.float64).reshape(3, -1)
         i = self.ctx.vecdom_add(FIELD_FUNCTION, values=vals)
         self.vecdom.append((FIELD_FUNCTION, func, list(unkn_id)))
+        self._output("vecdom", i, file_path)
         return i
 
     def add_vecdom_unkn(self, dom_id: int, value_init, file_path: str = "") -> int:
         i = self.ctx.vecdom_add(FIELD_UNKNOWN, cnst3=np.asarray(value_init, np.float64))
         self.vecdom.append((FIELD_UNKNOWN, None, []))
+        self._output("vecdom", i, file_path)
         return i
 
     # vector boundary fields (vars.rs:169-184); values SoA (3, count)
     def add_vecbnd_cnst(self, bnd_id: int, value, file_path: str = "") -> int:
         i = self.ctx.vecbnd_add(self.bnd[bnd_id], cnst3=np.asarray(value, np.float64))
-        self.vecbnd.append((bnd_id, None, []))
+        self.vecbnd.append((bnd_id, None, [], np.asarray(value, np.float64)))
+        self._output("vecbnd", i, file_path)
         return i
 
     def add_vecbnd_func(self, bnd_id: int, func, unkn_id=(), file_path: str = "") -> int:
         vals = np.asarray(func(0.0, self._bnd_pos(bnd_id), self._bnd_unkn(bnd_id, unkn_id)), np.float64).reshape(3, -1)
         i = self.ctx.vecbnd_add(self.bnd[bnd_id], values=vals)
-        self.vecbnd.append((bnd_id, func, list(unkn_id)))
+        self.vecbnd.append((bnd_id, func, list(unkn_id), None))
+        self._output("vecbnd", i, file_path)
         return i
 
     def add_sclflw(self, dom_id: int, vel_id: int, pres_id: int, den_id: int, visc_id: int, fce_id: int) -> int:
@@ -153,21 +176,41 @@ class Variables:
         for i, (kind, func, unkn) in enumerate(self.scldom):
             if kind == FIELD_FUNCTION:
                 self.ctx.scldom_set(i, func(t, self.pos, [self.ctx.scldom_get(u) for u in unkn]))
-        for i, (bnd_id, func, unkn) in enumerate(self.sclbnd):
+        for i, (bnd_id, func, unkn, _) in enumerate(self.sclbnd):
             if func is not None:
                 self.ctx.sclbnd_set(i, func(t, self._bnd_pos(bnd_id), self._bnd_unkn(bnd_id, unkn)))
         for i, (kind, func, unkn) in enumerate(self.vecdom):
             if kind == FIELD_FUNCTION:
                 self.ctx.vecdom_set(i, np.asarray(func(t, self.pos, [self.ctx.scldom_get(u) for u in unkn]),
                                                   np.float64).reshape(3, -1))
-        for i, (bnd_id, func, unkn) in enumerate(self.vecbnd):
+        for i, (bnd_id, func, unkn, _) in enumerate(self.vecbnd):
             if func is not None:
                 self.ctx.vecbnd_set(i, np.asarray(func(t, self._bnd_pos(bnd_id), self._bnd_unkn(bnd_id, unkn)),
                                                   np.float64).reshape(3, -1))
 
     def has_func(self) -> bool:
-        return (any(k == FIELD_FUNCTION for k, _, _ in self.scldom) or any(f is not None for _, f, _ in self.sclbnd)
-                or any(k == FIELD_FUNCTION for k, _, _ in self.vecdom) or any(f is not None for _, f, _ in self.vecbnd))
+        return (any(k == FIELD_FUNCTION for k, _, _ in self.scldom) or any(e[1] is not None for e in self.sclbnd)
+                or any(k == FIELD_FUNCTION for k, _, _ in self.vecdom) or any(e[1] is not None for e in self.vecbnd))
+
+    def write(self, ts: int, t: float = 0.0) -> None:
+        """The drivers' _write (steady_base.rs:213-233): every field with an output path, as
+        CSV (write_csv.rs:5-100) numbered ts; boundary fields on their boundary's nodes (function
+        fields evaluated at time t)."""
+        for kind, i, path in self.outputs:
+            if kind == "scldom":
+                self.mesh.write_csv(self.ctx.scldom_get(i), path, ts)
+            elif kind == "vecdom":
+                self.mesh.write_csv(self.ctx.vecdom_get(i), path, ts)
+            else:
+                bnd_id, func, unkn, cnst = (self.sclbnd if kind == "sclbnd" else self.vecbnd)[i]
+                count = len(self.mesh.bnd_nodes(self.bnd[bnd_id]))
+                if func is not None:
+                    vals = np.asarray(func(t, self._bnd_pos(bnd_id), self._bnd_unkn(bnd_id, unkn)), np.float64)
+                elif kind == "sclbnd":
+                    vals = np.full(count, cnst)
+                else:
+                    vals = np.repeat(np.asarray(cnst, np.float64).reshape(3, 1), count, axis=1)
+                self.mesh.write_csv(vals.reshape(-1, count) if kind == "vecbnd" else vals, path, ts, bnd=self.bnd[bnd_id])
 
     def value(self, scldom_id: int) -> np.ndarray:
         return self.ctx.scldom_get(scldom_id)
@@ -216,7 +259,7 @@ class _UpdateHook:
 
     def __enter__(self):
         if self.vars.has_func():
-            self.vars.ctx.set_update_hook(self.vars.update_func)
+            self.vars.ctx.set_update_hook(lambda t: self.vars.update_func(t + self.vars.t_offset))
         return self
 
     def __exit__(self, *exc):
@@ -233,8 +276,10 @@ class SteadyBase:
     def solve(self, vars: Variables, lin: LinearGmres, max_iter: int, tol: float, damp: float):
         vars.update_func(0.0)
         ops = self.operators(vars)
+        vars.t_offset = 0.0
         with _UpdateHook(vars):
             self.stats = vars.ctx.steady_solve(ops, max_iter, tol, damp, lin=lin.opts)
+        vars.write(0)  # steady_base.rs:119-121
         return self.stats
 
 
@@ -250,8 +295,29 @@ class TransientBase:
             raise VFVChemError(L.ERR_INVALID_ARGUMENT, "SolverNonPositiveWriteFrequency")
         vars.update_func(0.0)
         ops = self.operators(vars)
-        with _UpdateHook(vars):
-            self.stats = vars.ctx.transient_solve(ops, dt, num_ts, max_iter, tol, damp, lin=lin.opts)
+        vars.t_offset = 0.0
+        vars.write(0)  # transient_base.rs:67
+        if not vars.outputs:
+            with _UpdateHook(vars):
+                self.stats = vars.ctx.transient_solve(ops, dt, num_ts, max_iter, tol, damp, lin=lin.opts)
+            return self.stats
+        # with outputs: the steps in chunks that end at the output steps ((ts + 1) % num_ts_write
+        # == 0, transient_base.rs:141-143); each call continues from the current state, and the
+        # hook's time is offset by the steps already done
+        done, picard = 0, 0
+        try:
+            while done < num_ts:
+                chunk = min(num_ts_write - done % num_ts_write, num_ts - done)
+                vars.t_offset = done * dt
+                with _UpdateHook(vars):
+                    self.stats = vars.ctx.transient_solve(ops, dt, chunk, max_iter, tol, damp, lin=lin.opts)
+                picard += self.stats.picard_iterations
+                done += chunk
+                if done % num_ts_write == 0:
+                    vars.write(done // num_ts_write, done * dt)
+        finally:
+            vars.t_offset = 0.0
+        self.stats.picard_iterations = picard
         return self.stats
 
 
