@@ -16,6 +16,7 @@
  *                                                                      (dual on a GPU: vfvm_mesh_from_tets_dev / vfvm_mesh_lattice_dev)
  *   Tetrahedron::grad_coeff            src/base/geom_tet.rs:295     -> vfvm_mesh_build_grad_coeff (_dev on a GPU)
  *   write_{scl,vec}{dom,bnd}_csv       src/base/write_csv.rs:5-100  -> vfvm_write_csv
+ *   write_{scl,vec}{dom,bnd}_vtu       src/base/write_vtu.rs        -> vfvm_write_vtu
  *   Variables::add_scldom_* / add_sclbnd_* / add_sclflw / add_vecdom_*
  *                                      src/base/vars.rs:80-183      -> vfvm_scldom_add / vfvm_sclbnd_add / vfvm_vecdom_add / vfvm_sclflw_add
  *   OperatorBase::apply / apply_time   src/operator/operator_base.rs:5-7,
@@ -68,7 +69,8 @@ enum vfvm_status {
     VFVM_ERR_STEADY_NON_FINITE_RESIDUAL = 46,    /* SteadyNonFiniteResidual    steady_base.rs:101     */
     VFVM_ERR_TRANSIENT_DID_NOT_CONVERGE = 47,    /* TransientDidNotConverge    transient_base.rs:127  */
     VFVM_ERR_TRANSIENT_NON_FINITE_RESIDUAL = 48, /* TransientNonFiniteResidual transient_base.rs:119  */
-    VFVM_ERR_WRITE_CSV = 50                      /* WriteCsv                   error.rs:116            */
+    VFVM_ERR_WRITE_CSV = 50,                     /* WriteCsv                   error.rs:116            */
+    VFVM_ERR_WRITE_VTU = 51                      /* WriteVtu                   error.rs:122            */
 };
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -166,6 +168,11 @@ int vfvm_mesh_get(const vfvm_mesh* mesh, int32_t array_id, void* dst);
  * digits = 9 writes the reference's fixed 9-decimal format; digits = 0 writes every number
  * with 17 significant digits (reads back to the same double).  Host only. */
 int vfvm_write_csv(const vfvm_mesh* mesh, int32_t bnd, int32_t ncomp, const double* values, const char* file_name,
+                   const char* file_type, int32_t ts, int32_t digits);
+/* The same field as an ASCII VTU file (write_{scl,vec}{dom,bnd}_vtu, src/base/write_vtu.rs):
+ * a domain field on the mesh's tetrahedra (VTK type 10), a boundary field on one vertex cell
+ * per node (type 1), point data "value" with ncomp components; numbers as vfvm_write_csv. */
+int vfvm_write_vtu(const vfvm_mesh* mesh, int32_t bnd, int32_t ncomp, const double* values, const char* file_name,
                    const char* file_type, int32_t ts, int32_t digits);
 void vfvm_mesh_destroy(vfvm_mesh* mesh);
 

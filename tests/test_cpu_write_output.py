@@ -1,4 +1,5 @@
-"""CSV output (vfvm_write_csv, restating src/base/write_csv.rs:5-100) on the CPU: the
+"""CSV and VTU output (vfvm_write_csv / vfvm_write_vtu, restating src/base/write_csv.rs:5-100
+and src/base/write_vtu.rs) on the CPU: the
 reference's fixed 9-decimal format checked against Python's correctly rounded formatting of
 the same doubles, the full-precision mode reading back bit for bit, vertex and boundary-node
 order, vector fields and the WriteCsv error."""
@@ -56,3 +57,35 @@ def test_boundary_nodes_and_errors(tmp_path):
     with pytest.raises(vfvm.VFVChemError) as e:
         m.write_csv(vals, str(tmp_path / "y.csv"), bnd=9)
     assert e.value.code == 1
+
+
+def _vtu_arrays(path):
+    import xml.etree.ElementTree as ET
+    root = ET.parse(path).getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    arr = {}
+    for da in piece.iter("DataArray"):
+        arr[da.get("Name") or "points"] = np.array(da.text.split(), float)
+    return piece, arr
+
+
+def test_vtu_domain_and_boundary(tmp_path):
+    """VTU layout of write_vtu.rs: points, the tetrahedra (type 10) or one vertex cell per
+    boundary node (type 1), point data "value" in the reference's 9-decimal format."""
+    m = vfvm.Mesh.lattice(4, 0, 0.3)
+    vals = np.random.default_rng(4).standard_normal((3, m.num_tot))
+    path = m.write_csv(vals, str(tmp_path / "vel.vtu"), ts=3)
+    assert path.endswith("vel_0003.vtu")
+    piece, a = _vtu_arrays(path)
+    tets = m.get("tets")
+    assert int(piece.get("NumberOfPoints")) == m.num_tot and int(piece.get("NumberOfCells")) == len(tets)
+    assert np.abs(a["points"].reshape(-1, 3) - m.get("pos")).max() <= 5e-10
+    assert np.array_equal(a["connectivity"].reshape(-1, 4), tets)
+    assert np.array_equal(a["offsets"], 4 * np.arange(1, len(tets) + 1)) and np.all(a["types"] == 10)
+    assert np.abs(a["value"].reshape(-1, 3) - vals.T).max() <= 5e-10
+    nodes = m.num_pts + m.bnd_nodes(4)
+    bv = np.linspace(0.0, 1.0, len(nodes))
+    piece, a = _vtu_arrays(m.write_csv(bv, str(tmp_path / "qb.vtu"), bnd=4, digits=0))
+    assert int(piece.get("NumberOfCells")) == len(nodes) and np.all(a["types"] == 1)
+    assert np.array_equal(a["connectivity"], np.arange(len(nodes))) and np.array_equal(a["value"], bv)
+    assert np.array_equal(a["points"].reshape(-1, 3), m.get("pos")[nodes])

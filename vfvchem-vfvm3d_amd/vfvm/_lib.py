@@ -37,6 +37,7 @@ ERR_STEADY_NON_FINITE_RESIDUAL = 46
 ERR_TRANSIENT_DID_NOT_CONVERGE = 47
 ERR_TRANSIENT_NON_FINITE_RESIDUAL = 48
 ERR_WRITE_CSV = 50
+ERR_WRITE_VTU = 51
 
 STATUS_NAMES = {v: k for k, v in globals().items() if k.startswith("ERR_") or k == "OK"}
 
@@ -128,6 +129,7 @@ SIGNATURES = [
     ("vfvm_mesh_build_grad_coeff_dev", _I, [_P, _I]),
     ("vfvm_mesh_get", _I, [_P, _I, _P]),
     ("vfvm_write_csv", _I, [_P, _I, _I, _PD, C.c_char_p, C.c_char_p, _I, _I]),
+    ("vfvm_write_vtu", _I, [_P, _I, _I, _PD, C.c_char_p, C.c_char_p, _I, _I]),
     ("vfvm_mesh_destroy", None, [_P]),
     ("vfvm_create", _I, [_I, C.POINTER(_P)]),
     ("vfvm_destroy", None, [_P]),

@@ -82,14 +82,14 @@ class Variables:
 
     # scalar domain fields (vars.rs:80-102)
     def _output(self, kind: str, i: int, file_path: str) -> None:
-        # file_path "" = no output; "<name>.csv" (scl_dom.rs:56-66); vtu output is not built
+        # file_path "" = no output; "<name>.csv" or "<name>.vtu" (scl_dom.rs:56-66, 132-138)
         if not file_path:
             return
         name, sep, typ = file_path.rpartition(".")
         if not sep:
             raise VFVChemError(L.ERR_INVALID_ARGUMENT, f"UnknownOutputType: {file_path}")
-        if typ != "csv":
-            raise VFVChemError(L.ERR_INVALID_ARGUMENT, f"UnsupportedOutputType: {typ} (csv output only)")
+        if typ not in ("csv", "vtu"):
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, f"UnsupportedOutputType: {typ}")
         self.outputs.append((kind, i, file_path))
 
     def add_scldom_cnst(self, dom_id: int, value: float, file_path: str = "") -> int:
@@ -194,7 +194,7 @@ class Variables:
 
     def write(self, ts: int, t: float = 0.0) -> None:
         """The drivers' _write (steady_base.rs:213-233): every field with an output path, as
-        CSV (write_csv.rs:5-100) numbered ts; boundary fields on their boundary's nodes (function
+        CSV or VTU (write_csv.rs:5-100, write_vtu.rs) numbered ts; boundary fields on their boundary's nodes (function
         fields evaluated at time t)."""
         for kind, i, path in self.outputs:
             if kind == "scldom":

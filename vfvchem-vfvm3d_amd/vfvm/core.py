@@ -144,11 +144,11 @@ class Mesh:
         return out
 
     def write_csv(self, values, file_path: str, ts: int = 0, bnd: int = -1, digits: int = 9) -> str:
-        """CSV output of a field (write_{scl,vec}{dom,bnd}_csv, src/base/write_csv.rs:5-100):
-        `values` of shape (n,) or (3, n) over the vertices (bnd < 0) or the nodes of boundary
-        `bnd`; `file_path` is split at its last '.' into name and type as the reference does
-        (scl_dom.rs:60); digits = 9 is the reference's format, 0 writes 17 significant digits.
-        Returns the path written."""
+        """Output of a field as CSV or VTU by the type of `file_path` (write_csv.rs:5-100,
+        write_vtu.rs): `values` of shape (n,) or (3, n) over the vertices (bnd < 0) or the nodes
+        of boundary `bnd`; `file_path` is split at its last '.' into name and type as the
+        reference does (scl_dom.rs:60); digits = 9 is the reference's format, 0 writes 17
+        significant digits.  Returns the path written."""
         v = np.ascontiguousarray(values, np.float64)
         ncomp = 1 if v.ndim == 1 else v.shape[0]
         nb = len(self.get("bnd_ptr")) - 1
@@ -158,8 +158,9 @@ class Mesh:
         name, sep, typ = file_path.rpartition(".")
         if not sep:
             raise ValueError(f"output path without a type: {file_path}")
-        check(lib.vfvm_write_csv(self._h, int(bnd), int(ncomp), v.ctypes.data_as(C.POINTER(C.c_double)), name.encode(),
-                                 typ.encode(), int(ts), int(digits)))
+        fn = lib.vfvm_write_vtu if typ == "vtu" else lib.vfvm_write_csv
+        check(fn(self._h, int(bnd), int(ncomp), v.ctypes.data_as(C.POINTER(C.c_double)), name.encode(), typ.encode(), int(ts),
+                 int(digits)))
         return f"{name}_{ts:04d}.{typ}"
 
     def bnd_nodes(self, b: int) -> np.ndarray:
