@@ -256,14 +256,10 @@ __device__ inline void vertex_op(const DevOp& op, const AsmArgs& a, int64_t v, i
 // in the reference's push order (operator, then neighbour) by walking the lanes' edge
 // coefficients with shuffles, so the result is bit-identical to a sequential row walk.
 // Rows of higher degree take the sequential walk in their first lane.
+// The row of vertex v for the operator class cls (shared by the wave-uniform and the
+// per-lane paths of k_assemble).
 template <int LPR>
-__global__ void __launch_bounds__(BLK) k_assemble(AsmArgs a) {
-    const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
-    const int64_t v = tid / LPR;
-    const int lane = (int)(tid % LPR);
-    if (v >= a.N) return;  // whole row groups leave together (BLK is a multiple of LPR)
-    const int32_t vb = a.vbnd[v];
-    const int cls = vb < 0 ? 0 : 1 + vb;
+__device__ inline void assemble_row(const AsmArgs& a, int64_t v, int lane, int32_t vb, int cls) {
     const int nops = a.prog->nops[cls];
     const int32_t es = a.rp[v], ee = a.rp[v + 1];
     const int32_t deg = ee - es;
@@ -313,6 +309,29 @@ __global__ void __launch_bounds__(BLK) k_assemble(AsmArgs a) {
         if (ds >= 0) a.vals[ds] = dacc;
         a.b[v] = bacc;
     }
+}
+
+// One row per LPR lanes: lane l handles the directed edges es + l, es + l + LPR (rows of
+// degree <= 2 LPR), so the per-edge geometry and coefficient loads of a wave are
+// contiguous.  Each matrix entry is summed in operator order in its lane's registers
+// (starting from 0, as the reference's duplicate summation does); the diagonal is summed
+// in the reference's push order (operator, then neighbour) by walking the lanes' edge
+// coefficients with shuffles, so the result is bit-identical to a sequential row walk.
+// Rows of higher degree take the sequential walk in their first lane.  Rows are ordered
+// points first, then the nodes of each boundary, so almost every wave holds one operator
+// class: the class is then taken from the first lane (wave-uniform), and the operator
+// descriptors are read once per wave through the scalar cache instead of once per lane.
+template <int LPR>
+__global__ void __launch_bounds__(BLK) k_assemble(AsmArgs a) {
+    const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
+    const int64_t v = tid / LPR;
+    const int lane = (int)(tid % LPR);
+    if (v >= a.N) return;  // whole row groups leave together (BLK is a multiple of LPR)
+    const int32_t vb = a.vbnd[v];
+    const int cls = vb < 0 ? 0 : 1 + vb;
+    const int cls0 = __builtin_amdgcn_readfirstlane(cls);
+    if (__all(cls == cls0)) assemble_row<LPR>(a, v, lane, vb, cls0);
+    else assemble_row<LPR>(a, v, lane, vb, cls);
 }
 
 }  // namespace
