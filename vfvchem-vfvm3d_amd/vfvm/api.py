@@ -61,8 +61,9 @@ class Variables:
 
     @staticmethod
     def new_lattice(n: int, seed: int = 0, jitter: float = 0.3, device: int = 0) -> "Variables":
-        """Jittered unit-cube lattice (replaces Variables::new_bounds, vars.rs:45, for the lattice meshes)."""
-        return Variables(Mesh.lattice(n, seed, jitter), device)
+        """Jittered unit-cube lattice (replaces Variables::new_bounds, vars.rs:45, for the lattice meshes);
+        the Voronoi dual is built on the context's GPU (bitwise equal to the host build)."""
+        return Variables(Mesh.lattice(n, seed, jitter, device=device), device)
 
     def add_dom(self, grp3d_id: str = "cube") -> int:
         if self.dom:
