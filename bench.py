@@ -298,21 +298,15 @@ def run_heat_mass_flow(vfvm, args):
     N = mesh.num_tot
     ndof = 6 * N
     dt = 1e-2
-    # Hot-wall Nusselt number: the conductive heat flux out of the x = 0 nodes' control
-    # volumes through their faces, sum (A / d)(T_v - T_n), over k dT / L times the wall area
-    # (both 1); exactly 1 for the conduction profile T = 1 - x (the faces of a wall node's
-    # control volume close with its boundary face).  J = mean of Nu over the time steps.
-    rp, ngh = mesh.get("row_ptr"), mesh.get("ngh")
-    wall = mesh.num_pts + mesh.bnd_nodes(0).astype(np.int64)
-    deg = rp[wall + 1] - rp[wall]
-    eidx = np.concatenate([np.arange(rp[v], rp[v + 1]) for v in wall])
-    w_src, w_dst = np.repeat(wall, deg), ngh[eidx]
-    w_coef = mesh.get("area")[eidx] / mesh.get("dist")[eidx]
+    # Hot-wall Nusselt number (vfvm.objectives.WallFlux: conductive flux out of the x = 0
+    # nodes' control volumes over k dT / L times the wall area, both 1).  J = mean of Nu over
+    # the time steps.
+    from vfvm.objectives import WallFlux
+    wall_flux = WallFlux(mesh, 0)
     nus = {}
 
     def nusselt():
-        Tv = ctx.scldom_get(T_field[0])
-        return float(np.sum(w_coef * (Tv[w_src] - Tv[w_dst])))
+        return wall_flux(ctx.scldom_get(T_field[0]))
 
     T_field = [None]
     last_t = [None]
