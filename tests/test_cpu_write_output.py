@@ -89,3 +89,19 @@ def test_vtu_domain_and_boundary(tmp_path):
     assert int(piece.get("NumberOfCells")) == len(nodes) and np.all(a["types"] == 1)
     assert np.array_equal(a["connectivity"], np.arange(len(nodes))) and np.array_equal(a["value"], bv)
     assert np.array_equal(a["points"].reshape(-1, 3), m.get("pos")[nodes])
+
+
+def test_huge_values_are_written_in_full(tmp_path):
+    """Fixed-point expansions longer than any small buffer (1e100, -1e300 with 9 decimals) are
+    written whole, as Rust's "{:.9}" writes them; the full-precision mode reads them back."""
+    m = vfvm.Mesh.lattice(3, 0, 0.3)
+    vals = np.zeros(m.num_tot)
+    vals[0], vals[1], vals[2] = 1e100, -1e300, np.finfo(np.float64).max
+    path = m.write_csv(vals, str(tmp_path / "big.csv"), ts=0)
+    lines = open(path).read().splitlines()
+    for v in range(3):
+        assert lines[1 + v].split(",")[3] == ref_fmt(vals[v])
+    assert len(lines[2].split(",")[3]) == len(f"{-1e300:.9f}") == 312
+    path = m.write_csv(vals, str(tmp_path / "bigfull.csv"), ts=0, digits=0)
+    data = np.loadtxt(path, delimiter=",", skiprows=1)
+    assert np.array_equal(data[:, 3], vals)

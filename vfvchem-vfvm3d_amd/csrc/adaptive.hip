@@ -97,12 +97,12 @@ int error_norm(System& sy, const std::vector<ErrField>& fields, const double* xn
     for (const ErrField& f : fields) {
         const int64_t cnt = f.kind == 1 ? 3 * f.len : f.len;
         const unsigned nb = grid_for(cnt);
-        DevBuf<double> part;
-        VFVM_HIP_TRY(part.alloc(2 * (size_t)nb));
+        VFVM_HIP_TRY(c.red.grow(2 * (size_t)nb));
+        double* part = c.red.p;
         hipLaunchKernelGGL(k_lte, dim3(nb), dim3(BLK), 0, c.stream, cnt, xn + f.start, xm ? xm + f.start : xn + f.start,
-                           x + f.start, ratio, fac, has_prev ? 1 : 0, part.p);
+                           x + f.start, ratio, fac, has_prev ? 1 : 0, part);
         std::vector<double> h(2 * (size_t)nb);
-        VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         double emax = 0.0, ymax = 0.0;
         for (unsigned k = 0; k < nb; ++k) { emax = std::max(emax, h[2 * k]); ymax = std::max(ymax, h[2 * k + 1]); }
@@ -209,7 +209,9 @@ extern "C" int vfvm_transient_adaptive(vfvm_ctx* ctx, const vfvm_op* ops, int32_
     DevBuf<double> xn, xm, xi, xu, xd;
     for (DevBuf<double>* b : {&xn, &xm, &xi, &xu, &xd}) VFVM_HIP_TRY(b->alloc(n));
     if (int st = sy.pack(xn.p)) return st;
-    if (int st = flow_update_prev(c)) return st;
+    // _update_func at the initial state, then _update_prev (transient_adaptive.rs:138-140)
+    if (int st = call_update_hook(c, 0.0)) return st;
+    if (int st = update_prev_all(c)) return st;
 
     int out_idx = 0;
     double t = 0.0;
@@ -309,7 +311,7 @@ extern "C" int vfvm_transient_adaptive(vfvm_ctx* ctx, const vfvm_op* ops, int32_
         // accept: the hook sees the new state with value_prev still at t - dt
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         if (hook) hook(user, t_new, dt_step);
-        if (int st = flow_update_prev(c)) return st;
+        if (int st = update_prev_all(c)) return st;
         if (steps_out && as.accepted < max_steps) {
             steps_out[3 * as.accepted] = t_new;
             steps_out[3 * as.accepted + 1] = dt_step;

@@ -327,11 +327,10 @@ int copy_d2d(Ctx& c, double* dst, const double* src, int64_t n) {
 
 int count_neq(Ctx& c, const double* a, const double* b, int64_t n, int64_t* out) {
     const unsigned nb = grid_for(n);
-    DevBuf<int> part;
-    VFVM_HIP_TRY(part.alloc(nb));
-    hipLaunchKernelGGL(k_count_neq, dim3(nb), dim3(BLK), 0, c.stream, n, a, b, part.p);
+    VFVM_HIP_TRY(c.red_i.grow(nb));
+    hipLaunchKernelGGL(k_count_neq, dim3(nb), dim3(BLK), 0, c.stream, n, a, b, c.red_i.p);
     std::vector<int> h(nb);
-    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, nb * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), c.red_i.p, nb * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     int64_t s = 0;
     for (int x : h) s += x;

@@ -318,6 +318,7 @@ extern "C" int vfvm_mesh_upload(vfvm_ctx* ctx, const vfvm_mesh* mesh) {
     g.bnd_nid = m.bnd_nid;
     g.h_row_ptr = m.row_ptr;
     g.h_ngh = m.ngh;
+    g.h_pos = m.pos;
     if (int st = sync_err(c)) return st;
     c.has_mesh = true;
     return clear_error();
@@ -806,11 +807,10 @@ namespace {
 // scaled Picard residual for one scalar field of n entries (scaling.rs:88-135)
 int picard_residual(Ctx& c, const double* xo, const double* xn, double* res, int64_t n) {
     const unsigned nb = grid_for(n);
-    DevBuf<double> part;
-    VFVM_HIP_TRY(part.alloc(4 * (size_t)nb));
-    hipLaunchKernelGGL(k_picard, dim3(nb), dim3(BLK), 0, c.stream, n, xo, xn, part.p);
+    VFVM_HIP_TRY(c.red.grow(4 * (size_t)nb));
+    hipLaunchKernelGGL(k_picard, dim3(nb), dim3(BLK), 0, c.stream, n, xo, xn, c.red.p);
     std::vector<double> h(4 * (size_t)nb);
-    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), c.red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     double sum = 0.0, mo = 0.0, mn = 0.0, nf = 0.0;
     for (unsigned k = 0; k < nb; ++k) { sum += h[4 * k]; mo = std::max(mo, h[4 * k + 1]); mn = std::max(mn, h[4 * k + 2]); nf += h[4 * k + 3]; }
@@ -829,9 +829,10 @@ int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts*
     const double t_now = transient ? dt * (double)(ts + 1) : 0.0;  // end of the step (transient_base.rs:71)
     const int64_t n = c.n;
     SclDom& u = c.scl[c.unk];
-    DevBuf<double> xi, xu;
-    VFVM_HIP_TRY(xi.alloc(n));
-    VFVM_HIP_TRY(xu.alloc(n));
+    DevBuf<double>& xi = c.pic_xi;
+    DevBuf<double>& xu = c.pic_xu;
+    VFVM_HIP_TRY(xi.grow(n));
+    VFVM_HIP_TRY(xu.grow(n));
     auto t0 = clk::now();
     // x_iter = current unknown values (_unkn_to_solu, steady_base.rs:137)
     VFVM_HIP_TRY(hipMemcpyAsync(xi.p, u.value.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
@@ -927,29 +928,34 @@ extern "C" int vfvm_transient_solve(vfvm_ctx* ctx, const vfvm_op* ops, int32_t n
     vfvm_driver_stats ds;
     std::memset(&ds, 0, sizeof(ds));
     if (int st = vfvm_pattern_build(ctx, ops, nops, nullptr)) return st;
-    if (int st = call_update_hook(c, 0.0)) return st;  // _update_func before initial_dirichlet (steady_base.rs:50-54)
+    // transient_base.rs:55-64: _update_func, the Dirichlet values at t = 0, _update_func again
+    // at the initial state, and _update_prev of every field (the initial state is the previous
+    // state of the first step)
+    if (int st = call_update_hook(c, 0.0)) return st;
     if (int st = initial_dirichlet(c, ops, nops)) return st;
+    if (int st = call_update_hook(c, 0.0)) return st;
+    if (c.flow && !c.fs.transient)
+        return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "transient flow needs the velocity time operator (VEC_DOM_TIME) first");
+    if (int st = update_prev_all(c)) return st;
     if (c.flow) {
-        // TransientFlow (transient_flow.rs:248-303 with transient_base.rs:16-160): the initial
-        // state is the previous state of the first step; Picard per step, then _update_prev
-        if (!c.fs.transient) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "transient flow needs the velocity time operator (VEC_DOM_TIME) first");
-        if (int st = flow_update_prev(c)) return st;
+        // TransientFlow (transient_flow.rs:248-303 with transient_base.rs:16-160): Picard per
+        // step, then _update_prev
         int r = VFVM_OK;
         for (int32_t ts = 0; ts < num_ts && r == VFVM_OK; ++ts) {
             r = flow_picard(c, ops, nops, *opts, max_iter, tol, damp, ds, dt, ts);
-            if (r == VFVM_OK) r = flow_update_prev(c);
+            if (r == VFVM_OK) r = update_prev_all(c);
         }
         if (stats) *stats = ds;
         if (r != VFVM_OK) return r;
         return clear_error();
     }
-    // the initial state is the previous state of the first step (transient_base.rs:59-62)
-    SclDom& u = c.scl[c.unk];
-    VFVM_HIP_TRY(hipMemcpyAsync(u.value_prev.p, u.value.p, c.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    // scalar list: Picard per step, then _update_prev of every field (transient_base.rs:131,
+    // :297-311), so a weight that the update hook changes enters the next step's
+    // TIME_PREVIOUS terms with its value at the end of this step
     int r = VFVM_OK;
     for (int32_t ts = 0; ts < num_ts && r == VFVM_OK; ++ts) {
         r = picard_loop(c, ops, nops, opts, dt, max_iter, tol, damp, ds, ts, true);
-        if (r == VFVM_OK) VFVM_HIP_TRY(hipMemcpyAsync(u.value_prev.p, u.value.p, c.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        if (r == VFVM_OK) r = update_prev_all(c);
     }
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     if (stats) *stats = ds;

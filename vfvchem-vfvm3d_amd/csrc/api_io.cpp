@@ -17,13 +17,19 @@ using namespace vfvm;
 
 namespace {
 
+// Appends v in the chosen format.  A fixed-point expansion of a large value can run to
+// hundreds of characters (1e300 with 9 decimals: 311), so the text is formatted at the
+// length snprintf reports, never cut to a fixed buffer.
 void put_num(std::string& line, double v, int digits) {
+    if (std::isnan(v)) { line += "NaN"; return; }
+    if (std::isinf(v)) { line += v > 0 ? "inf" : "-inf"; return; }
     char buf[64];
-    if (std::isnan(v)) std::snprintf(buf, sizeof buf, "NaN");
-    else if (std::isinf(v)) std::snprintf(buf, sizeof buf, v > 0 ? "inf" : "-inf");
-    else if (digits > 0) std::snprintf(buf, sizeof buf, "%.*f", digits, v);
-    else std::snprintf(buf, sizeof buf, "%.17g", v);
-    line += buf;
+    int len = digits > 0 ? std::snprintf(buf, sizeof buf, "%.*f", digits, v) : std::snprintf(buf, sizeof buf, "%.17g", v);
+    if (len < 0) return;
+    if ((size_t)len < sizeof buf) { line.append(buf, (size_t)len); return; }
+    std::string big((size_t)len + 1, '\0');
+    len = digits > 0 ? std::snprintf(&big[0], big.size(), "%.*f", digits, v) : std::snprintf(&big[0], big.size(), "%.17g", v);
+    if (len > 0) line.append(big.data(), (size_t)len);
 }
 
 }  // namespace

@@ -39,6 +39,11 @@ struct DevBuf {
         if (e == hipSuccess) n = count; else p = nullptr;
         return e;
     }
+    // keeps the allocation when it already holds `count` entries (scratch reused across calls)
+    hipError_t grow(size_t count) {
+        if (p && n >= count) return hipSuccess;
+        return alloc(count);
+    }
     hipError_t upload(const T* h, size_t count, hipStream_t s) {
         hipError_t e = alloc(count);
         if (e != hipSuccess || count == 0) return e;
@@ -60,6 +65,7 @@ struct DevGeom {
     std::vector<int64_t> bnd_ptr;              // host copy
     std::vector<int32_t> bnd_nid;
     std::vector<int32_t> h_row_ptr, h_ngh;     // host copies for the pattern build
+    std::vector<double> h_pos;                 // vertex coordinates (3N AoS): row order of the ILU(0) colours
     // pressure-gradient coefficients (flow operators), when the mesh has them
     int64_t G = 0;
     DevBuf<int32_t> gc_ptr, gc_vid;
@@ -116,10 +122,15 @@ struct DevProgram {
 };
 
 // Multicolour ILU(0) ordering of one pattern, in colour-major storage: row t of the
-// permuted system is row perm[t] of the matrix, rows sorted by (colour, index); entries
-// [L | diag | U] with permuted columns; src[k] = CSR slot of permuted entry k.
+// permuted system is row perm[t] of the matrix, rows sorted by (colour, Morton code of the
+// vertex); entries [L | diag | U] with permuted columns; src[k] = CSR slot of permuted entry
+// k.  The factor is stored split by triangle, [L (nnz_l) | U (nnz_u) | diag (n)] with the
+// index arrays lptr / lcol and uptr / ucol, so that each sweep streams only its triangle;
+// lpos / upos give the entries' positions in the colour-major CSR.
 struct IluData {
     DevBuf<int32_t> perm, prp, src, pcol, pdiag;
+    DevBuf<int32_t> lptr, lcol, lpos, uptr, ucol, upos;
+    int64_t nnz_l = 0, nnz_u = 0;
     std::vector<int32_t> h_color_ptr;
     bool ready = false;
     uint64_t generation = 0;  // bumped by every set-up (graphs bake the layout in)
@@ -249,9 +260,16 @@ struct Ctx {
     // solver workspace (shared by both systems)
     DevBuf<double> V64, w64, z64, x64, r64, dinv64, avals64, part, scal;
     DevBuf<float> V32, w32, avals32;
-    DevBuf<double> ilu64, ap64, bp64, xp64;  // ILU factor; A, b, x in the colour-major order
+    DevBuf<double> ilu64, ap64, bp64, xp64;  // ILU factor (split); A, b, x in the colour-major order
     DevBuf<float> ilu32;
+    DevBuf<double> iluf64;  // factor in the colour-major [L | diag | U] layout while it is computed
+    DevBuf<float> iluf32;
     double* h_pinned = nullptr;  // small pinned readback buffer
+    // driver scratch, kept across Picard iterations and time steps: per-block partial sums of
+    // the residual and statistics reductions, and the Picard iterates
+    DevBuf<double> red;
+    DevBuf<int> red_i;
+    DevBuf<double> pic_xi, pic_xu, pic_xd;
 
     AdjTape tape;
 
@@ -303,7 +321,7 @@ int flow_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops);
 int flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts& o, int32_t max_iter, double tol,
                 double damp, vfvm_driver_stats& ds, double dt = 0.0, int ts = 0);
 int flow_assemble_dt(Ctx& c, const vfvm_op* ops, int32_t nops, double dt);
-int flow_update_prev(Ctx& c);
+int update_prev_all(Ctx& c);
 // x = [vel | pres] and back (the unknowns of the flow system)
 int flow_pack(Ctx& c, double* x);
 int flow_unpack(Ctx& c, const double* x);

@@ -1124,10 +1124,12 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
     const int64_t N = c.g.N, n = c.n;
     double* vel = c.vec[c.fs.vel].value.p;
     double* pres = c.scl[c.fs.pres].value.p;
-    DevBuf<double> xi, xu, xd;
-    VFVM_HIP_TRY(xi.alloc(n));
-    VFVM_HIP_TRY(xu.alloc(n));
-    VFVM_HIP_TRY(xd.alloc(n));
+    DevBuf<double>& xi = c.pic_xi;
+    DevBuf<double>& xu = c.pic_xu;
+    DevBuf<double>& xd = c.pic_xd;
+    VFVM_HIP_TRY(xi.grow(n));
+    VFVM_HIP_TRY(xu.grow(n));
+    VFVM_HIP_TRY(xd.grow(n));
     auto t0 = clk::now();
     const double t_now = transient ? dt * (double)(ts + 1) : 0.0;  // end of the step (transient_base.rs:71)
     if (int st = flow_pack(c, xi.p)) return st;
@@ -1170,7 +1172,7 @@ int vfvm::flow_picard(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve
 }
 
 // _update_prev (transient_base.rs:63, :110): every field's previous values := current values
-int vfvm::flow_update_prev(Ctx& c) {
+int vfvm::update_prev_all(Ctx& c) {
     const int64_t N = c.g.N;
     for (auto& f : c.scl)
         VFVM_HIP_TRY(hipMemcpyAsync(f.value_prev.p, f.value.p, N * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
@@ -1202,11 +1204,10 @@ int vfvm::flow_unpack(Ctx& c, const double* x) {
 int vfvm::flow_stats(Ctx& c, const double* xo, const double* xn, double out[7]) {
     const int64_t N = c.g.N;
     const unsigned nb = grid_for(N);
-    DevBuf<double> part;
-    VFVM_HIP_TRY(part.alloc(7 * (size_t)nb));
-    hipLaunchKernelGGL(k_flow_stats, dim3(nb), dim3(BLK), 0, c.stream, N, xo, xn, (const double*)c.scl[c.fs.den].value.p, part.p);
+    VFVM_HIP_TRY(c.red.grow(7 * (size_t)nb));
+    hipLaunchKernelGGL(k_flow_stats, dim3(nb), dim3(BLK), 0, c.stream, N, xo, xn, (const double*)c.scl[c.fs.den].value.p, c.red.p);
     std::vector<double> h(7 * (size_t)nb);
-    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), c.red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     out[0] = 0.0; out[1] = 0.0; out[2] = 0.0; out[3] = INFINITY; out[4] = -INFINITY; out[5] = 0.0; out[6] = 0.0;
     for (unsigned k = 0; k < nb; ++k) {
@@ -1231,11 +1232,10 @@ int vfvm::flow_residual(Ctx& c, const double* xo, const double* xn, double* res)
     if (st[5] > 0.0) { *res = NAN; return VFVM_OK; }
     const double dead = 1e-11 * st[2];  // PRES_ROUNDOFF (scaling.rs:28)
     const unsigned nb = grid_for(N);
-    DevBuf<double> part;
-    VFVM_HIP_TRY(part.alloc(nb));
-    hipLaunchKernelGGL(k_flow_pres_sq, dim3(nb), dim3(BLK), 0, c.stream, N, xo + 3 * N, xn + 3 * N, dead, part.p);
+    VFVM_HIP_TRY(c.red.grow(nb));
+    hipLaunchKernelGGL(k_flow_pres_sq, dim3(nb), dim3(BLK), 0, c.stream, N, xo + 3 * N, xn + 3 * N, dead, c.red.p);
     std::vector<double> h(nb);
-    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), part.p, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    VFVM_HIP_TRY(hipMemcpyAsync(h.data(), c.red.p, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     double sp = 0.0;
     for (unsigned k = 0; k < nb; ++k) sp += h[k];

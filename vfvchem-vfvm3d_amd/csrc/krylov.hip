@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "order.hpp"
 #include "xcd.hpp"
 
 
@@ -76,13 +77,6 @@ __device__ inline double block_sum(double v, double* sh) {
 
 // ---------------------------------------------------------------- SpMV
 // LPR lanes per row (scalar rows carry ~17 entries, coupled flow rows ~100-480)
-template <class T>
-__device__ inline T row_dot(const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ x, int32_t k,
-                            int32_t e1, int step) {
-    T s = 0;
-    for (; k < e1; k += step) s += val[k] * x[col[k]];
-    return s;
-}
 // the same sum with two entries per trip, both gathers issued before the first product
 // (the lane's entries are still added in ascending order)
 template <class T>
@@ -111,16 +105,7 @@ __device__ inline int ctrl_go(const int* __restrict__ ctrl) { return *(ctrl ? ct
 // rhs scale factor (1 when absent: x * 1 == x exactly)
 __device__ inline double scale_of(const double* __restrict__ scale) { return *(scale ? scale : &kUnitScale); }
 __device__ inline bool ctrl_stop(int go, bool loaded) { return (go == 0) & loaded; }
-// rows of the scalar SpMV and the ILU sweeps take two entries per trip unless
-// VFVM_ROW_UNROLL=0 (read on every launch, so one process can compare both; at 100^3
-// convection-diffusion: SpMV FP64 78.0 -> 71.7 us, FP32 54.4 -> 49.2 us, ILU sweep
-// launch FP32 6.41 -> 6.05 us)
-inline bool row_unroll() {
-    const char* e = getenv("VFVM_ROW_UNROLL");
-    return !(e && atoi(e) == 0);
-}
-
-template <class T, int LPR, bool U2 = true>
+template <class T, int LPR>
 __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                               const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
                                               const double* __restrict__ scale, const int* __restrict__ ctrl) {
@@ -132,7 +117,7 @@ __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restri
     const int32_t k0 = rp[row], k1 = rp[row + 1];
     const T sc = (T)scale_of(scale);
     if (ctrl_stop(go, (k0 <= k1) | (sc != sc))) return;
-    T s = U2 ? row_dot2<T>(col, val, x, k0 + lane, k1, LPR) : row_dot<T>(col, val, x, k0 + lane, k1, LPR);
+    T s = row_dot2<T>(col, val, x, k0 + lane, k1, LPR);
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
     if (lane == 0) y[row] = (T)(s * sc);
 }
@@ -701,13 +686,27 @@ __global__ void __launch_bounds__(BLK) k_ilu_factor(int32_t t0, int32_t t1, cons
     }
 }
 
-// Forward sweep over the rows [t0, t1) of one colour of the colour-major system
-// (L has a unit diagonal), in place in z:  z_t = c rhs_t - sum_L l_tj z_j  (c = *rscale or 1)
-// The last colour has no U entries: with FINAL its backward sweep is folded in.
-template <class T, int LPR, bool FINAL, bool U2 = true>
-__global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const int32_t* __restrict__ rp,
-                                                   const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
-                                                   const T* __restrict__ lu, const T* __restrict__ rhs,
+// The factor in the split layout [L (nnz_l) | U (nnz_u) | diag (n)] from the colour-major
+// [L | diag | U] rows it was computed in.
+template <class T>
+__global__ void __launch_bounds__(BLK) k_ilu_split(int64_t nnz_l, int64_t nnz_u, int64_t n, const int32_t* __restrict__ lpos,
+                                                   const int32_t* __restrict__ upos, const int32_t* __restrict__ diag,
+                                                   const T* __restrict__ full, T* __restrict__ split) {
+    const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (k < nnz_l) split[k] = full[lpos[k]];
+    else if (k < nnz_l + nnz_u) split[k] = full[upos[k - nnz_l]];
+    else if (k < nnz_l + nnz_u + n) split[k] = full[diag[k - nnz_l - nnz_u]];
+}
+
+// Forward sweep over the rows [t0, t1) of one colour of the colour-major system (L has a
+// unit diagonal), in place in z:  z_t = c rhs_t - sum_L l_tj z_j  (c = *rscale or 1).  The L
+// entries of row t are [lptr[t], lptr[t+1]) of (lcol, lval): the sweep streams only its own
+// triangle.  The last colour has no U entries: with FINAL its backward sweep is folded in
+// (division by the diagonal u_tt).
+template <class T, int LPR, bool FINAL>
+__global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const int32_t* __restrict__ lptr,
+                                                   const int32_t* __restrict__ lcol, const T* __restrict__ lval,
+                                                   const T* __restrict__ dval, const T* __restrict__ rhs,
                                                    const double* __restrict__ rscale, T* __restrict__ z,
                                                    const int* __restrict__ ctrl) {
     const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
@@ -715,11 +714,11 @@ __global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const
     const int lane = (int)(tid % LPR);
     if (t >= t1) return;
     const int go = ctrl_go(ctrl);
-    const int32_t k0 = rp[t], d = diag[t];
+    const int32_t k0 = lptr[t], k1 = lptr[t + 1];
     const T r = (T)(rhs[t] * (T)scale_of(rscale));
-    if (ctrl_stop(go, (k0 <= d) | (r != r))) return;
-    const T dd = FINAL ? lu[d] : (T)1;  // issued with the row's entries
-    T s = U2 ? row_dot2<T>(col, lu, z, k0 + lane, d, LPR) : row_dot<T>(col, lu, z, k0 + lane, d, LPR);
+    if (ctrl_stop(go, (k0 <= k1) | (r != r))) return;
+    const T dd = FINAL ? dval[t] : (T)1;  // issued with the row's entries
+    T s = row_dot2<T>(lcol, lval, z, k0 + lane, k1, LPR);
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
     if (lane == 0) z[t] = FINAL ? (r - s) / dd : r - s;
 }
@@ -728,10 +727,10 @@ __global__ void __launch_bounds__(BLK) k_ilu_lower(int32_t t0, int32_t t1, const
 // so colour 1 reads c rhs_j where the colour-by-colour sweep reads z_j: the same value, so
 // the result is bitwise that of the two launches.  The colour-0 rows are written by a
 // grid-stride copy over the launch's threads.  (c = 1 without a scale: x * 1 == x.)
-template <class T, int LPR, bool FINAL, bool U2 = true>
-__global__ void __launch_bounds__(BLK) k_ilu_lower01(int32_t t0, int32_t t1, const int32_t* __restrict__ rp,
-                                                     const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
-                                                     const T* __restrict__ lu, const T* __restrict__ rhs,
+template <class T, int LPR, bool FINAL>
+__global__ void __launch_bounds__(BLK) k_ilu_lower01(int32_t t0, int32_t t1, const int32_t* __restrict__ lptr,
+                                                     const int32_t* __restrict__ lcol, const T* __restrict__ lval,
+                                                     const T* __restrict__ dval, const T* __restrict__ rhs,
                                                      const double* __restrict__ rscale, T* __restrict__ z,
                                                      const int* __restrict__ ctrl) {
     const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
@@ -742,25 +741,23 @@ __global__ void __launch_bounds__(BLK) k_ilu_lower01(int32_t t0, int32_t t1, con
     const bool has_row = t < t1;
     const int32_t tc = has_row ? t : t0;  // threads past the colour read row t0 (in bounds: t0 < t1)
     const int go = ctrl_go(ctrl);
-    const int32_t k0 = rp[tc], d = diag[tc];
+    const int32_t k0 = lptr[tc], k1 = lptr[tc + 1];
     const T r = (T)(rhs[tc] * c);
     const bool cp = tid < t0;  // this thread's first colour-0 row, loaded with the row bounds
     const T cv = (T)(rhs[cp ? tid : 0] * c);
-    if (ctrl_stop(go, (k0 <= d) | (r != r) | (cv != cv))) return;
+    if (ctrl_stop(go, (k0 <= k1) | (r != r) | (cv != cv))) return;
     if (has_row) {
-        const T dd = FINAL ? lu[d] : (T)1;
+        const T dd = FINAL ? dval[t] : (T)1;
         T s = 0;
         int32_t k = k0 + lane;
-        if (U2) {
-            for (; k + LPR < d; k += 2 * LPR) {
-                const int32_t c0 = col[k], c1 = col[k + LPR];
-                const T a0 = lu[k], a1 = lu[k + LPR];
-                const T x0 = (T)(rhs[c0] * c), x1 = (T)(rhs[c1] * c);
-                s += a0 * x0;
-                s += a1 * x1;
-            }
+        for (; k + LPR < k1; k += 2 * LPR) {
+            const int32_t c0 = lcol[k], c1 = lcol[k + LPR];
+            const T a0 = lval[k], a1 = lval[k + LPR];
+            const T x0 = (T)(rhs[c0] * c), x1 = (T)(rhs[c1] * c);
+            s += a0 * x0;
+            s += a1 * x1;
         }
-        for (; k < d; k += LPR) s += lu[k] * (T)(rhs[col[k]] * c);
+        for (; k < k1; k += LPR) s += lval[k] * (T)(rhs[lcol[k]] * c);
         for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
         if (lane == 0) z[t] = FINAL ? (r - s) / dd : r - s;
     }
@@ -768,20 +765,21 @@ __global__ void __launch_bounds__(BLK) k_ilu_lower01(int32_t t0, int32_t t1, con
     for (int64_t i = tid + nthr; i < t0; i += nthr) z[i] = (T)(rhs[i] * c);
 }
 // backward sweep over one colour below the last, in place: z_t = (z_t - sum_U u_tj z_j) / u_tt
-template <class T, int LPR, bool U2 = true>
-__global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const int32_t* __restrict__ rp,
-                                                   const int32_t* __restrict__ col, const int32_t* __restrict__ diag,
-                                                   const T* __restrict__ lu, T* __restrict__ z, const int* __restrict__ ctrl) {
+// (the U entries of row t are [uptr[t], uptr[t+1]) of (ucol, uval))
+template <class T, int LPR>
+__global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const int32_t* __restrict__ uptr,
+                                                   const int32_t* __restrict__ ucol, const T* __restrict__ uval,
+                                                   const T* __restrict__ dval, T* __restrict__ z, const int* __restrict__ ctrl) {
     const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
     const int32_t t = t0 + (int32_t)(tid / LPR);
     const int lane = (int)(tid % LPR);
     if (t >= t1) return;
     const int go = ctrl_go(ctrl);
     const T zt = z[t];  // row t's own entry: written below by this row only
-    const int32_t d = diag[t], e = rp[t + 1];
-    if (ctrl_stop(go, (d < e) | (zt != zt))) return;
-    const T dd = lu[d];  // issued with the row's entries
-    T s = U2 ? row_dot2<T>(col, lu, z, d + 1 + lane, e, LPR) : row_dot<T>(col, lu, z, d + 1 + lane, e, LPR);
+    const int32_t k0 = uptr[t], k1 = uptr[t + 1];
+    if (ctrl_stop(go, (k0 <= k1) | (zt != zt))) return;
+    const T dd = dval[t];  // issued with the row's entries
+    T s = row_dot2<T>(ucol, uval, z, k0 + lane, k1, LPR);
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LPR);
     if (lane == 0) z[t] = (zt - s) / dd;
 }
@@ -1007,13 +1005,9 @@ struct Work {
     double* S;    // device state
 };
 
+// lanes per CSR row from the average row length (scalar rows carry ~17 entries, the generic
+// coupled rows ~100-480)
 inline int pick_lpr(int64_t n, int64_t nnz) {
-    static const int forced = [] {
-        const char* e = getenv("VFVM_SPMV_LPR");
-        const int v = e ? atoi(e) : 0;
-        return (v == 4 || v == 8 || v == 16 || v == 32 || v == 64) ? v : 0;
-    }();
-    if (forced) return forced;
     double avg = n > 0 ? (double)nnz / (double)n : 1.0;
     return avg > 96.0 ? 32 : (avg > 24.0 ? 16 : (avg > 10.0 ? 8 : 4));
 }
@@ -1025,38 +1019,16 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
     const int32_t* rp = sy.row;
     const int32_t* cl = sy.col;
     if (sy.flow_layout) {
-        // the coupled flow system, comp-major or vertex-interleaved vectors
+        // the coupled flow system: vertex-interleaved vectors take 32 lanes per vertex (four
+        // 3-ring and two grad_coeff entries per lane and trip), comp-major vectors one wave
         FlowSpmvGeom fg{c.g.N, c.g.NP, c.g.row_ptr.p, c.g.gc_ptr.p};
-        // lanes per vertex (VFVM_FLOW_LPV=64 selects one vertex per wave)
-        static const int lpv = [] {
-            const char* e = getenv("VFVM_FLOW_LPV");
-            return (e && atoi(e) == 64) ? 64 : 32;
-        }();
-        // VFVM_FLOW_UNROLL=10: one entry per lane and trip in every loop (the default issues
-        // four 3-ring and two grad_coeff entries per trip: at 80^3 FP64 958 -> 912 us,
-        // FP32 624 -> 592 us; eight per trip lose occupancy: 1004 us); read on every launch
-        // so that one process can compare them with eager launches
-        const int unroll = [] {
-            const char* e = getenv("VFVM_FLOW_UNROLL");
-            return e ? atoi(e) : 41;
-        }();
 #define VFVM_FLOW_SPMV(NS)                                                                                                   \
-    if (sy.flow_layout == 2) {                                                                                               \
-        if (lpv == 32) {                                                                                                     \
-            const dim3 g32(grid_for(c.g.N * 32));                                                                            \
-            switch (unroll) {                                                                                                \
-                case 10: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 32, NS, 1, false>, g32, dim3(BLK), fg, rp, \
-                                      cl, val, x, y, scale, ctrl); break;                                                   \
-                default: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 32, NS, 4, true>, g32, dim3(BLK), fg, rp,  \
-                                      cl, val, x, y, scale, ctrl); break;                                                   \
-            }                                                                                                                \
-        } else                                                                                                               \
-            launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 64, NS>, dim3(grid_for(c.g.N * 64)), dim3(BLK), fg, rp, cl, \
-                         val, x, y, scale, ctrl);                                                                            \
-    } else {                                                                                                                 \
+    if (sy.flow_layout == 2)                                                                                                 \
+        launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, true, 32, NS>, dim3(grid_for(c.g.N * 32)), dim3(BLK), fg, rp, cl, \
+                     val, x, y, scale, ctrl);                                                                                \
+    else                                                                                                                     \
         launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_flow<T, false, 64, NS>, dim3(grid_for(c.g.N * WAVE)), dim3(BLK), fg, rp,  \
-                     cl, val, x, y, scale, ctrl);                                                                            \
-    }
+                     cl, val, x, y, scale, ctrl);
         switch (sy.flow_ns) {
             case 0: VFVM_FLOW_SPMV(0) break;
             case 1: VFVM_FLOW_SPMV(1) break;
@@ -1066,18 +1038,11 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }
-    const bool u2 = row_unroll();
     switch (lpr) {
-        case 4: launch_timed(c, VFVM_K_SPMV, bytes, u2 ? k_spmv<T, 4, true> : k_spmv<T, 4, false>, dim3(grid), dim3(BLK), n, rp, cl,
-                             val, x, y, scale, ctrl); break;
-        case 8: launch_timed(c, VFVM_K_SPMV, bytes, u2 ? k_spmv<T, 8, true> : k_spmv<T, 8, false>, dim3(grid), dim3(BLK), n, rp, cl,
-                             val, x, y, scale, ctrl); break;
-        case 32: launch_timed(c, VFVM_K_SPMV, bytes, u2 ? k_spmv<T, 32, true> : k_spmv<T, 32, false>, dim3(grid), dim3(BLK), n, rp,
-                              cl, val, x, y, scale, ctrl); break;
-        case 64: launch_timed(c, VFVM_K_SPMV, bytes, u2 ? k_spmv<T, 64, true> : k_spmv<T, 64, false>, dim3(grid), dim3(BLK), n, rp,
-                              cl, val, x, y, scale, ctrl); break;
-        default: launch_timed(c, VFVM_K_SPMV, bytes, u2 ? k_spmv<T, 16, true> : k_spmv<T, 16, false>, dim3(grid), dim3(BLK), n, rp,
-                              cl, val, x, y, scale, ctrl); break;
+        case 4: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 4>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
+        case 8: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 8>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
+        case 32: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 32>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
+        default: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 16>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
@@ -1108,7 +1073,6 @@ int residual_launch(Ctx& c, const SysView& sy, int lpr, const double* x, double*
         case 4: hipLaunchKernelGGL((k_residual<4>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
         case 8: hipLaunchKernelGGL((k_residual<8>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
         case 32: hipLaunchKernelGGL((k_residual<32>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
-        case 64: hipLaunchKernelGGL((k_residual<64>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
         default: hipLaunchKernelGGL((k_residual<16>), dim3(grid), dim3(BLK), 0, c.stream, n, sy.row, sy.col, sy.vals, x, sy.b, r, part, nblk_r); break;
     }
     VFVM_HIP_TRY(hipGetLastError());
@@ -1116,341 +1080,129 @@ int residual_launch(Ctx& c, const SysView& sy, int lpr, const double* x, double*
 }
 
 // hout[k] = sig_k V[k] . wv over the basis vectors [0, nv): per-block partials, then one
-// reduction block per vector
-// basis vectors per multi-dot pass (VFVM_MDOT_CHUNK = 2, 4, 8 or 16; default 2); each
-// vector's partial sums are the same for all (same lane, rows and order).  At 100^3 the
-// average multi-dot takes FP32 21.4 / 22.4 / 24.4 / 34.5 us and FP64 40.3 / 42.9 / 47.4 /
-// 65.5 us for 2 / 4 / 8 / 16: fewer accumulators per lane keep more waves in flight, and
-// the re-reads of w hit L2
-inline int mdot_chunk() {
-    const char* e = getenv("VFVM_MDOT_CHUNK");
-    const int v = e ? atoi(e) : 2;
-    return (v == 4 || v == 8 || v == 16) ? v : 2;
-}
+// reduction block per vector.  Two basis vectors per pass: at 100^3 the average multi-dot
+// takes FP32 21.4 us and FP64 40.3 us against 24.4 / 47.4 us for eight per pass (fewer
+// accumulators per lane keep more waves in flight, and the re-reads of w hit L2)
+constexpr int MDOT_CHUNK = 2;
 template <class T>
 void mdot_launch(Ctx& c, Work& w, int nv, const T* V, const T* wv, double* part, double* hout) {
     const int nbx = w.nwav / (BLK / WAVE);
     // algorithmic bytes: the nv basis vectors and w once
     const double bytes = (double)sizeof(T) * w.n * (nv + 1);
-    const int ch = mdot_chunk();
-    const dim3 grid(nbx, (nv + ch - 1) / ch), block(BLK);
-    if (ch == 16)
-        launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T, 16>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
-                     (const int*)w.ctrl);
-    else if (ch == 8)
-        launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T, 8>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
-                     (const int*)w.ctrl);
-    else if (ch == 2)
-        launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T, 2>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
-                     (const int*)w.ctrl);
-    else
-        launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T, 4>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
-                     (const int*)w.ctrl);
+    const dim3 grid(nbx, (nv + MDOT_CHUNK - 1) / MDOT_CHUNK), block(BLK);
+    launch_timed(c, VFVM_K_MDOT, bytes, k_mdot<T, MDOT_CHUNK>, grid, block, w.n, nv, V, w.ldv, wv, part, w.rows_per_wave,
+                 (const int*)w.ctrl);
     hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(BLK), 0, c.stream, nbx, (const double*)part, (const double*)(w.S + w.L.sig()),
                        hout, (const int*)w.ctrl);
 }
 
-// Multicolour ordering of the matrix pattern and the colour-major [L | diag | U] storage.
+// Multicolour ordering of the matrix pattern (host, ilu_order.cpp), the colour-major CSR
+// and the split triangle index arrays, uploaded once per pattern.
 int ilu_setup(Ctx& c, const SysView& sy) {
     IluData& I = *sy.ilu;
     if (I.ready) return VFVM_OK;
-    const int64_t n = sy.n;
-    const std::vector<int32_t>& rp = *sy.h_row;
-    const std::vector<int32_t>& cl = *sy.h_col;
-    // symmetrised adjacency (rows of one colour must not touch each other in A or A^T)
-    std::vector<int32_t> tcnt(n + 1, 0);
-    for (int64_t i = 0; i < n; ++i)
-        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) tcnt[cl[k] + 1]++;
-    for (int64_t i = 0; i < n; ++i) tcnt[i + 1] += tcnt[i];
-    std::vector<int32_t> trows(tcnt[n]);
-    {
-        std::vector<int32_t> fill(tcnt.begin(), tcnt.end() - 1);
-        for (int64_t i = 0; i < n; ++i)
-            for (int32_t k = rp[i]; k < rp[i + 1]; ++k) trows[fill[cl[k]]++] = (int32_t)i;
-    }
-    // deduplicated symmetrised adjacency without the diagonal
-    std::vector<int32_t> aptr(n + 1, 0), adj;
-    adj.reserve((size_t)rp[n] * 2);
-    for (int64_t i = 0; i < n; ++i) {
-        int32_t a = rp[i], ae = rp[i + 1], b = tcnt[i], be = tcnt[i + 1];
-        int32_t last = -1;
-        while (a < ae || b < be) {
-            int32_t j;
-            if (b >= be || (a < ae && cl[a] <= trows[b])) j = cl[a++];
-            else j = trows[b++];
-            if (j != (int32_t)i && j != last) { adj.push_back(j); last = j; }
-        }
-        aptr[i + 1] = (int32_t)adj.size();
-    }
-    // colouring order: smallest-last (Matula-Beck degeneracy order, reversed), which needs
-    // fewer colours than the natural order on Delaunay graphs; VFVM_ILU_ORDER=natural keeps
-    // the natural order
-    std::vector<int32_t> order(n);
-    const char* ord_env = getenv("VFVM_ILU_ORDER");
-    if (ord_env && std::string(ord_env) == "natural") {
-        for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
-    } else {
-        std::vector<int32_t> deg(n);
-        int32_t maxd = 0;
-        for (int64_t i = 0; i < n; ++i) { deg[i] = aptr[i + 1] - aptr[i]; maxd = std::max(maxd, deg[i]); }
-        std::vector<std::vector<int32_t>> bucket(maxd + 1);
-        for (int64_t i = n - 1; i >= 0; --i) bucket[deg[i]].push_back((int32_t)i);
-        std::vector<uint8_t> gone(n, 0);
-        int32_t cur = 0;
-        for (int64_t cnt = n - 1; cnt >= 0; --cnt) {
-            int32_t v = -1;
-            for (;;) {
-                while (bucket[cur].empty()) ++cur;
-                v = bucket[cur].back();
-                bucket[cur].pop_back();
-                if (!gone[v] && deg[v] == cur) break;  // skip stale entries
-            }
-            gone[v] = 1;
-            order[cnt] = v;
-            for (int32_t k = aptr[v]; k < aptr[v + 1]; ++k) {
-                const int32_t u = adj[k];
-                if (!gone[u]) bucket[--deg[u]].push_back(u);
-            }
-            cur = std::max(0, cur - 1);
-        }
-    }
-    // greedy colouring in that order
-    std::vector<int32_t> color(n, -1);
-    std::vector<int64_t> mark(64, -1);
-    int32_t ncol = 0;
-    for (int64_t oi = 0; oi < n; ++oi) {
-        const int32_t i = order[oi];
-        for (int32_t k = aptr[i]; k < aptr[i + 1]; ++k) {
-            const int32_t cj = color[adj[k]];
-            if (cj >= 0) {
-                if (cj >= (int32_t)mark.size()) mark.resize(cj + 1, -1);
-                mark[cj] = oi;
-            }
-        }
-        int32_t cc = 0;
-        while (cc < (int32_t)mark.size() && mark[cc] == oi) ++cc;
-        color[i] = cc;
-        ncol = std::max(ncol, cc + 1);
-    }
-    // Trim a small last colour class: every colour costs two sweep launches per application,
-    // and greedy colourings end with a few rows in their last class (16 of 1M on the 100^3
-    // lattice).  Each row of the last class moves to a lower colour: directly when one is free
-    // among its neighbours, after moving its only neighbour of some colour to another free
-    // colour, or by swapping the two colours of a bounded two-colour (Kempe) component around
-    // it.  The class is dropped only if all its rows move (else the colouring is restored);
-    // classes above VFVM_ILU_TRIM_MAX rows or swaps above 4096 rows (VFVM_ILU_KEMPE_MAX) are
-    // not attempted, so the ordering changes only locally.  Off by default: on the 100^3
-    // convection system VFVM_ILU_TRIM_MAX=3906 gives 9 colours (16 instead of 18 launches per
-    // application, 6 us less per step), but the changed ordering needed 3324 instead of 3273
-    // FP64 steps, so the solve time did not change.
-    {
-        const char* tm = getenv("VFVM_ILU_TRIM_MAX");
-        const char* km = getenv("VFVM_ILU_KEMPE_MAX");
-        const int64_t trim_max = tm ? atoll(tm) : 0;
-        const int64_t kempe_max = km ? atoll(km) : 4096;
-        std::vector<int32_t> cnt_c, saved, stack, comp;
-        std::vector<uint8_t> seen;
-        auto colour_counts = [&](int32_t v, int32_t nc) {
-            cnt_c.assign(nc, 0);
-            for (int32_t k = aptr[v]; k < aptr[v + 1]; ++k) cnt_c[color[adj[k]]]++;
-        };
-        auto move_row = [&](int32_t v, int32_t last) -> bool {
-            colour_counts(v, last + 1);
-            for (int32_t c = 0; c < last; ++c)
-                if (cnt_c[c] == 0) { color[v] = c; return true; }
-            for (int32_t c = 0; c < last; ++c) {
-                if (cnt_c[c] != 1) continue;
-                int32_t u = -1;
-                for (int32_t k = aptr[v]; k < aptr[v + 1]; ++k)
-                    if (color[adj[k]] == c) { u = adj[k]; break; }
-                std::vector<int32_t> cu(last + 1, 0);
-                for (int32_t k = aptr[u]; k < aptr[u + 1]; ++k) cu[color[adj[k]]]++;
-                for (int32_t c2 = 0; c2 < last; ++c2)
-                    if (c2 != c && cu[c2] == 0) { color[u] = c2; color[v] = c; return true; }
-            }
-            if (seen.size() != (size_t)n) seen.assign(n, 0);
-            for (int32_t a = 0; a < last; ++a)
-                for (int32_t b = 0; b < last; ++b) {
-                    if (a == b) continue;
-                    comp.clear();
-                    bool bad = false;
-                    for (int32_t k = aptr[v]; k < aptr[v + 1] && !bad; ++k) {
-                        const int32_t u = adj[k];
-                        if (color[u] != a || seen[u]) continue;
-                        stack.assign(1, u);
-                        seen[u] = 1;
-                        while (!stack.empty() && !bad) {
-                            const int32_t x = stack.back();
-                            stack.pop_back();
-                            comp.push_back(x);
-                            if ((int64_t)comp.size() > kempe_max) { bad = true; break; }
-                            for (int32_t q = aptr[x]; q < aptr[x + 1]; ++q) {
-                                const int32_t y = adj[q];
-                                if (seen[y] || (color[y] != a && color[y] != b)) continue;
-                                seen[y] = 1;
-                                stack.push_back(y);
-                            }
-                        }
-                    }
-                    for (int32_t k = aptr[v]; k < aptr[v + 1] && !bad; ++k)
-                        if (color[adj[k]] == b && seen[adj[k]]) bad = true;
-                    // clear the marks (component plus the stack left by an early stop)
-                    for (int32_t x : comp) seen[x] = 0;
-                    for (int32_t x : stack) seen[x] = 0;
-                    for (int32_t k = aptr[v]; k < aptr[v + 1]; ++k) seen[adj[k]] = 0;
-                    if (bad) continue;
-                    for (int32_t x : comp) color[x] = color[x] == a ? b : a;
-                    color[v] = a;
-                    return true;
-                }
-            return false;
-        };
-        while (trim_max > 0 && ncol > 2) {
-            const int32_t last = ncol - 1;
-            int64_t sz = 0;
-            for (int64_t i = 0; i < n; ++i) sz += color[i] == last;
-            if (sz > trim_max) break;
-            saved = color;
-            bool ok = true;
-            for (int64_t i = 0; i < n && ok; ++i)
-                if (color[i] == last) ok = move_row((int32_t)i, last);
-            for (int64_t i = 0; i < n && ok; ++i)  // the result must still be a colouring
-                for (int32_t k = aptr[i]; k < aptr[i + 1]; ++k)
-                    if (color[adj[k]] == color[i]) { ok = false; break; }
-            if (!ok) { color.swap(saved); break; }
-            ncol = last;
-        }
-    }
-    // permutation: rows sorted by (colour, index)
-    std::vector<int32_t> cptr(ncol + 1, 0), perm(n), iperm(n);
-    for (int64_t i = 0; i < n; ++i) cptr[color[i] + 1]++;
-    for (int32_t q = 0; q < ncol; ++q) cptr[q + 1] += cptr[q];
-    {
-        std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
-        for (int64_t i = 0; i < n; ++i) {
-            const int32_t t = fill[color[i]]++;
-            perm[t] = (int32_t)i;
-            iperm[i] = t;
-        }
-    }
-    // colour-major CSR, columns permuted and ascending
-    std::vector<int32_t> prp(n + 1, 0), src(sy.nnz), pcol(sy.nnz), pdiag(n, -1);
-    std::vector<std::pair<int32_t, int32_t>> tmp;
-    for (int64_t t = 0; t < n; ++t) {
-        const int32_t i = perm[t];
-        tmp.clear();
-        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) tmp.push_back({iperm[cl[k]], k});
-        std::sort(tmp.begin(), tmp.end());
-        const int32_t base = prp[t];
-        for (size_t q = 0; q < tmp.size(); ++q) {
-            pcol[base + q] = tmp[q].first;
-            src[base + q] = tmp[q].second;
-            if (tmp[q].first == t) pdiag[t] = base + (int32_t)q;
-        }
-        prp[t + 1] = base + (int32_t)tmp.size();
-        if (pdiag[t] < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "ILU(0): a row has no diagonal entry");
-    }
-    VFVM_HIP_TRY(I.perm.upload(perm.data(), perm.size(), c.stream));
-    VFVM_HIP_TRY(I.prp.upload(prp.data(), prp.size(), c.stream));
-    VFVM_HIP_TRY(I.src.upload(src.data(), src.size(), c.stream));
-    VFVM_HIP_TRY(I.pcol.upload(pcol.data(), pcol.size(), c.stream));
-    VFVM_HIP_TRY(I.pdiag.upload(pdiag.data(), pdiag.size(), c.stream));
+    IluOrder o;
+    std::string err;
+    const double* pos = c.g.h_pos.empty() ? nullptr : c.g.h_pos.data();
+    if (ilu_order_build(sy.n, *sy.h_row, *sy.h_col, pos, c.g.N, o, err)) return set_error(VFVM_ERR_INVALID_ARGUMENT, err);
+    VFVM_HIP_TRY(I.perm.upload(o.perm.data(), o.perm.size(), c.stream));
+    VFVM_HIP_TRY(I.prp.upload(o.prp.data(), o.prp.size(), c.stream));
+    VFVM_HIP_TRY(I.src.upload(o.src.data(), o.src.size(), c.stream));
+    VFVM_HIP_TRY(I.pcol.upload(o.pcol.data(), o.pcol.size(), c.stream));
+    VFVM_HIP_TRY(I.pdiag.upload(o.pdiag.data(), o.pdiag.size(), c.stream));
+    VFVM_HIP_TRY(I.lptr.upload(o.lptr.data(), o.lptr.size(), c.stream));
+    VFVM_HIP_TRY(I.lcol.upload(o.lcol.data(), std::max<size_t>(o.lcol.size(), 1), c.stream));
+    VFVM_HIP_TRY(I.lpos.upload(o.lpos.data(), std::max<size_t>(o.lpos.size(), 1), c.stream));
+    VFVM_HIP_TRY(I.uptr.upload(o.uptr.data(), o.uptr.size(), c.stream));
+    VFVM_HIP_TRY(I.ucol.upload(o.ucol.data(), std::max<size_t>(o.ucol.size(), 1), c.stream));
+    VFVM_HIP_TRY(I.upos.upload(o.upos.data(), std::max<size_t>(o.upos.size(), 1), c.stream));
+    I.nnz_l = (int64_t)o.lpos.size();
+    I.nnz_u = (int64_t)o.upos.size();
     ++I.generation;
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
-    I.h_color_ptr = cptr;
+    I.h_color_ptr = o.color_ptr;
     I.ready = true;
     return VFVM_OK;
 }
 
-// lu <- ILU(0) of A in multicolour order, values gathered from the CSR values
+// ILU(0) of A in multicolour order (IKJ, Saad Alg. 10.4, one launch per colour) in the
+// colour-major [L | diag | U] rows of `full` (nnz entries), then split into lu = [L | U | diag]
 template <class T>
-int ilu_factor(Ctx& c, const SysView& sy, T* lu, const double* val) {
+int ilu_factor(Ctx& c, const SysView& sy, T* lu, T* full, const double* val) {
     const IluData& I = *sy.ilu;
-    hipLaunchKernelGGL((k_ilu_gather<T>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, I.src.p, val, lu);
+    hipLaunchKernelGGL((k_ilu_gather<T>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, I.src.p, val, full);
     const int ncol = (int)I.h_color_ptr.size() - 1;
     for (int q = 0; q < ncol; ++q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e > s)
             hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, s, e, I.prp.p,
-                               I.pcol.p, I.pdiag.p, lu);
+                               I.pcol.p, I.pdiag.p, full);
     }
+    hipLaunchKernelGGL((k_ilu_split<T>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, I.nnz_l, I.nnz_u, sy.n, I.lpos.p,
+                       I.upos.p, I.pdiag.p, (const T*)full, lu);
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
 }
 
-// colours 0 and 1 of the forward sweep in one launch unless VFVM_ILU_FUSE01=0
-inline bool ilu_fuse01() {
-    const char* e = getenv("VFVM_ILU_FUSE01");
-    return !(e && atoi(e) == 0);
-}
+// Lanes per row of the sweeps: four (a row holds ~8 entries of each triangle)
+constexpr int ILU_LPR = 4;
 
 // z = (LU)^-1 rhs in the colour-major space: forward sweep over the colours ascending (the
 // last one also finishing its backward sweep; colours 0 and 1 share a launch), backward
 // sweep over the other colours descending, in place in z (z must not alias rhs):
-// 2 colours - 2 launches.
-template <class T, int LPR>
-int ilu_apply_lpr(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double* rscale, T* z, const int* ctrl) {
+// 2 colours - 2 launches.  lu is the split factor [L | U | diag].
+template <class T>
+int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double* rscale, T* z, const int* ctrl) {
+    constexpr int LPR = ILU_LPR;
     const IluData& I = *sy.ilu;
     const int ncol = (int)I.h_color_ptr.size() - 1;
-    // algorithmic bytes of one sweep launch over r rows: the triangle's entries (value +
-    // column, about half of each row), rp/diag (8 B), the row's rhs/z reads and writes
-    const double per_row_entries = 0.5 * (double)sy.nnz / (double)sy.n * (sizeof(T) + 4.0);
-    const bool u2 = row_unroll();
+    const T* lval = lu;
+    const T* uval = lu + I.nnz_l;
+    const T* dval = lu + I.nnz_l + I.nnz_u;
+    // algorithmic bytes of one sweep launch over the rows [s, e): the triangle's entries (value
+    // + column), its row pointers (8 B per row; the last colour also reads the diagonal), the
+    // row's rhs / z read and z write
+    auto sweep_bytes = [&](int32_t s, int32_t e, bool lower, bool final) -> double {
+        const double ent = lower ? (double)I.nnz_l : (double)I.nnz_u;
+        // entries of the colour in proportion to its rows (the exact count needs the host
+        // pointers; rows of one colour carry the same share of a triangle on average)
+        const double per_row = ent / (double)std::max<int64_t>(sy.n, 1);
+        return (double)(e - s) * (per_row * (sizeof(T) + 4.0) + 8.0 + 2.0 * sizeof(T) + ((final || !lower) ? sizeof(T) : 0.0));
+    };
     int q0 = 0;
-    if (ncol >= 2 && ilu_fuse01() && I.h_color_ptr[0] == 0 && I.h_color_ptr[2] > I.h_color_ptr[1]) {
+    if (ncol >= 2 && I.h_color_ptr[0] == 0 && I.h_color_ptr[2] > I.h_color_ptr[1]) {
         // colours 0 and 1 in one launch (colour 0 is a scaled copy)
         const int32_t s = I.h_color_ptr[1], e = I.h_color_ptr[2];
         const dim3 grid(grid_for((int64_t)(e - s) * LPR));
-        const double bytes = (double)s * 2.0 * sizeof(T) + (double)(e - s) * (per_row_entries + 8.0 + 2.0 * sizeof(T));
+        const double bytes = (double)s * 2.0 * sizeof(T) + sweep_bytes(s, e, true, ncol == 2);
         if (ncol == 2)
-            launch_timed(c, VFVM_K_PRECOND, bytes, u2 ? k_ilu_lower01<T, LPR, true, true> : k_ilu_lower01<T, LPR, true, false>,
-                         grid, dim3(BLK), s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, rscale, z, ctrl);
+            launch_timed(c, VFVM_K_PRECOND, bytes, k_ilu_lower01<T, LPR, true>, grid, dim3(BLK), s, e, I.lptr.p, I.lcol.p, lval,
+                         dval, rhs, rscale, z, ctrl);
         else
-            launch_timed(c, VFVM_K_PRECOND, bytes, u2 ? k_ilu_lower01<T, LPR, false, true> : k_ilu_lower01<T, LPR, false, false>,
-                         grid, dim3(BLK), s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, rscale, z, ctrl);
+            launch_timed(c, VFVM_K_PRECOND, bytes, k_ilu_lower01<T, LPR, false>, grid, dim3(BLK), s, e, I.lptr.p, I.lcol.p, lval,
+                         dval, rhs, rscale, z, ctrl);
         q0 = 2;
     }
     for (int q = q0; q < ncol; ++q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
         const dim3 grid(grid_for((int64_t)(e - s) * LPR));
-        const double bytes = (double)(e - s) * (per_row_entries + 8.0 + 2.0 * sizeof(T));
+        const double bytes = sweep_bytes(s, e, true, q == ncol - 1);
         if (q == ncol - 1)
-            launch_timed(c, VFVM_K_PRECOND, bytes, u2 ? k_ilu_lower<T, LPR, true, true> : k_ilu_lower<T, LPR, true, false>, grid,
-                         dim3(BLK), s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, rscale, z, ctrl);
+            launch_timed(c, VFVM_K_PRECOND, bytes, k_ilu_lower<T, LPR, true>, grid, dim3(BLK), s, e, I.lptr.p, I.lcol.p, lval,
+                         dval, rhs, rscale, z, ctrl);
         else
-            launch_timed(c, VFVM_K_PRECOND, bytes, u2 ? k_ilu_lower<T, LPR, false, true> : k_ilu_lower<T, LPR, false, false>,
-                         grid, dim3(BLK), s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, rhs, rscale, z, ctrl);
+            launch_timed(c, VFVM_K_PRECOND, bytes, k_ilu_lower<T, LPR, false>, grid, dim3(BLK), s, e, I.lptr.p, I.lcol.p, lval,
+                         dval, rhs, rscale, z, ctrl);
     }
     for (int q = ncol - 2; q >= 0; --q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e == s) continue;
-        launch_timed(c, VFVM_K_PRECOND, (double)(e - s) * (per_row_entries + 8.0 + 2.0 * sizeof(T)),
-                     u2 ? k_ilu_upper<T, LPR, true> : k_ilu_upper<T, LPR, false>, dim3(grid_for((int64_t)(e - s) * LPR)),
-                     dim3(BLK), s, e, I.prp.p, I.pcol.p, I.pdiag.p, lu, z, ctrl);
+        launch_timed(c, VFVM_K_PRECOND, sweep_bytes(s, e, false, false), k_ilu_upper<T, LPR>,
+                     dim3(grid_for((int64_t)(e - s) * LPR)), dim3(BLK), s, e, I.uptr.p, I.ucol.p, uval, dval, z, ctrl);
     }
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
-}
-
-// lanes per row of the sweeps (VFVM_ILU_LPR = 1, 2, 4 or 8; default 4)
-inline int ilu_lpr() {
-    static const int v = [] {
-        const char* e = getenv("VFVM_ILU_LPR");
-        const int k = e ? atoi(e) : 4;
-        return (k == 1 || k == 2 || k == 4 || k == 8) ? k : 4;
-    }();
-    return v;
-}
-
-template <class T>
-int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double* rscale, T* z, const int* ctrl) {
-    switch (ilu_lpr()) {
-        case 1: return ilu_apply_lpr<T, 1>(c, sy, lu, rhs, rscale, z, ctrl);
-        case 2: return ilu_apply_lpr<T, 2>(c, sy, lu, rhs, rscale, z, ctrl);
-        case 8: return ilu_apply_lpr<T, 8>(c, sy, lu, rhs, rscale, z, ctrl);
-        default: return ilu_apply_lpr<T, 4>(c, sy, lu, rhs, rscale, z, ctrl);
-    }
 }
 
 // Explicitly applied preconditioner of a cycle: the ILU(0) factor or the block-Jacobi
@@ -1574,7 +1326,7 @@ CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv
                                     (uint64_t)(uintptr_t)aval, (uint64_t)(uintptr_t)lu, (uint64_t)(uintptr_t)w.S,
                                     (uint64_t)(uintptr_t)w.ctrl, (uint64_t)(uintptr_t)c.part.p,
                                     (uint64_t)(uintptr_t)sy.row, (uint64_t)(uintptr_t)sy.col, (uint64_t)w.n,
-                                    (uint64_t)w.nnz, (uint64_t)w.m, (uint64_t)reorth + 2u * row_unroll() + 4u * ilu_fuse01() + 8u * (uint64_t)mdot_chunk(), (uint64_t)sizeof(T),
+                                    (uint64_t)w.nnz, (uint64_t)w.m, (uint64_t)reorth, (uint64_t)sizeof(T),
                                     (uint64_t)(uintptr_t)I, I ? I->generation : 0, (uint64_t)w.nblk, (uint64_t)w.nwav, (uint64_t)w.ldv};
     for (auto& g : c.graphs)
         if (g.key == key) { g.last_use = ++c.graph_clock; return &g; }
@@ -1691,7 +1443,8 @@ int precond_apply(Ctx& c, const vfvm_solve_opts& o, const double* r_host, double
         if (int s = ilu_setup(c, sy)) return s;
         if (o.precision == VFVM_FP64) {
             VFVM_HIP_TRY(c.ilu64.alloc(sy.nnz));
-            if (int s = ilu_factor<double>(c, sy, c.ilu64.p, sy.vals)) return s;
+            VFVM_HIP_TRY(c.iluf64.alloc(sy.nnz));
+            if (int s = ilu_factor<double>(c, sy, c.ilu64.p, c.iluf64.p, sy.vals)) return s;
             DevBuf<double> rp_, zp_;
             VFVM_HIP_TRY(rp_.alloc(n));
             VFVM_HIP_TRY(zp_.alloc(n));
@@ -1703,7 +1456,8 @@ int precond_apply(Ctx& c, const vfvm_solve_opts& o, const double* r_host, double
             VFVM_HIP_TRY(r32.alloc(n));
             VFVM_HIP_TRY(z32.alloc(n));
             VFVM_HIP_TRY(c.ilu32.alloc(sy.nnz));
-            if (int s = ilu_factor<float>(c, sy, c.ilu32.p, sy.vals)) return s;
+            VFVM_HIP_TRY(c.iluf32.alloc(sy.nnz));
+            if (int s = ilu_factor<float>(c, sy, c.ilu32.p, c.iluf32.p, sy.vals)) return s;
             DevBuf<double> zp_;
             VFVM_HIP_TRY(zp_.alloc(n));
             hipLaunchKernelGGL((k_gather_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.ilu->perm.p, (const double*)r.p, zp_.p);
@@ -1823,12 +1577,8 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     w.m = m;
     w.nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 4 * BLK - 1) / (4 * BLK)));
     w.rows_per_blk = ((n + w.nblk - 1) / w.nblk + 3) / 4 * 4;
-    // multi-dot waves per group of basis vectors (VFVM_MDOT_WAVES, default 1024)
-    static const int64_t mdot_waves = [] {
-        const char* e = getenv("VFVM_MDOT_WAVES");
-        const int64_t v = e ? atoll(e) : 1024;
-        return std::min<int64_t>(8192, std::max<int64_t>(64, v));
-    }();
+    // multi-dot: up to 1024 waves per group of basis vectors, 8+ rows per lane
+    constexpr int64_t mdot_waves = 1024;
     w.nwav = (int)std::min<int64_t>(mdot_waves, std::max<int64_t>(1, (n + WAVE * 8 - 1) / (WAVE * 8)));
     w.nwav = (w.nwav + BLK / WAVE - 1) / (BLK / WAVE) * (BLK / WAVE);
     w.rows_per_wave = ((n + w.nwav - 1) / w.nwav + 3) / 4 * 4;
@@ -1884,7 +1634,8 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
         }
         if (ilu) {
             VFVM_HIP_TRY(c.ilu64.alloc(sy.nnz));
-            if (int s = ilu_factor<double>(c, sy0, c.ilu64.p, sy0.vals)) return s;
+            VFVM_HIP_TRY(c.iluf64.alloc(sy.nnz));
+            if (int s = ilu_factor<double>(c, sy0, c.ilu64.p, c.iluf64.p, sy0.vals)) return s;
             lu = c.ilu64.p;
         }
     } else {
@@ -1894,7 +1645,8 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
         aval = c.avals32.p;
         if (ilu) {
             VFVM_HIP_TRY(c.ilu32.alloc(sy.nnz));
-            if (int s = ilu_factor<float>(c, sy0, c.ilu32.p, sy0.vals)) return s;
+            VFVM_HIP_TRY(c.iluf32.alloc(sy.nnz));
+            if (int s = ilu_factor<float>(c, sy0, c.ilu32.p, c.iluf32.p, sy0.vals)) return s;
             lu = c.ilu32.p;
         }
     }
