@@ -32,6 +32,11 @@
  *   flow_helper::update_flow           src/operator/core/flow_helper.rs:3-179 -> vfvm_update_flow
  *   Variables::add_vecbnd_{cnst,func}  src/base/vars.rs:169-184           -> vfvm_vecbnd_add
  *   TransientAdaptive::solve_adaptive_with src/transient/transient_adaptive.rs:81-350 -> vfvm_transient_adaptive
+ *   LinearBase::{update_sparse, build_matrix, solve_matrix} for caller-assembled triplets
+ *                                      src/linear/linear_base.rs:5-10, linear_gmres.rs:53-84
+ *                                      -> vfvm_linear_update_sparse / vfvm_linear_build_matrix / vfvm_linear_solve_matrix
+ *   Variables::set_scldom_scale / set_vecdom_scale / scale_off
+ *                                      src/base/vars.rs:28-31,203-213 -> vfvm_set_scldom_scale / vfvm_set_vecdom_scale / vfvm_set_scale_off
  */
 #ifndef VFVM_H
 #define VFVM_H
@@ -70,7 +75,8 @@ enum vfvm_status {
     VFVM_ERR_TRANSIENT_DID_NOT_CONVERGE = 47,    /* TransientDidNotConverge    transient_base.rs:127  */
     VFVM_ERR_TRANSIENT_NON_FINITE_RESIDUAL = 48, /* TransientNonFiniteResidual transient_base.rs:119  */
     VFVM_ERR_WRITE_CSV = 50,                     /* WriteCsv                   error.rs:116            */
-    VFVM_ERR_WRITE_VTU = 51                      /* WriteVtu                   error.rs:122            */
+    VFVM_ERR_WRITE_VTU = 51,                     /* WriteVtu                   error.rs:122            */
+    VFVM_ERR_VARIABLES_NON_POSITIVE_SCALE = 60   /* VariablesNonPositiveScale  vars.rs:204,210         */
 };
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -350,6 +356,32 @@ int vfvm_precond_apply(vfvm_ctx* ctx, const vfvm_solve_opts* opts, const double*
  * by (colour, index)), color_ptr[ncolors + 1] = first position of each colour. */
 int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, int32_t* ncolors);
 
+/* ---- caller-assembled systems (LinearBase, src/linear/linear_base.rs:5-10) ----
+ * For a solver whose operators are not on the device: it hands its triplet list, right-hand
+ * side and initial guess to the device GMRES, as the reference's drivers hand theirs to
+ * LinearGmres.  The system replaces the context's operator-assembled one (vfvm_assemble then
+ * fails until vfvm_pattern_build is called again); vfvm_solve, vfvm_residual and
+ * vfvm_precond_apply work on it.  No mesh is needed (block Jacobi needs one: its blocks are the
+ * unknowns of a vertex). */
+/* LinearBase::update_sparse (linear_gmres.rs:53-64): the pattern of the triplets (rows[k],
+ * cols[k]), k < ntrip, of a mat_size x mat_size matrix (0-based, as usize in the reference),
+ * duplicates merged, rows sorted by column; nnz: the number of distinct entries. */
+int vfvm_linear_update_sparse(vfvm_ctx* ctx, int64_t mat_size, int64_t ntrip, const int64_t* rows, const int64_t* cols,
+                              int64_t* nnz);
+/* LinearBase::build_matrix (linear_gmres.rs:66-74): the matrix of a triplet list with the
+ * (row, col) sequence given to update_sparse (its argsort is reused, as new_from_argsort
+ * does): vals[k] of triplet k, duplicates summed in list order.  Host array; the _device
+ * variant reads a device array of ntrip doubles. */
+int vfvm_linear_build_matrix(vfvm_ctx* ctx, int64_t ntrip, const double* vals);
+int vfvm_linear_build_matrix_device(vfvm_ctx* ctx, int64_t ntrip, const double* vals_dev);
+/* LinearBase::solve_matrix (linear_gmres.rs:76-84): solves A x = b from x_init (NULL: zero)
+ * with the options' GMRES; host arrays of mat_size doubles.  The _device variant reads b_dev
+ * and takes x_dev as initial guess in, solution out (device arrays). */
+int vfvm_linear_solve_matrix(vfvm_ctx* ctx, const vfvm_solve_opts* opts, const double* b, const double* x_init, double* x,
+                             vfvm_solve_stats* stats);
+int vfvm_linear_solve_matrix_device(vfvm_ctx* ctx, const vfvm_solve_opts* opts, const double* b_dev, double* x_dev,
+                                    vfvm_solve_stats* stats);
+
 /* ---- drivers ---- */
 /* Host closure of function fields that depend on unknowns or time (Variables::_update_func,
  * steady_base.rs:242-261): called with the time t before every assembly of the drivers
@@ -358,6 +390,18 @@ int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, int32_t* nc
  * (vfvm_scldom_set, vfvm_vecdom_set).  NULL removes it. */
 typedef void (*vfvm_update_hook)(void* user, double t);
 int vfvm_set_update_hook(vfvm_ctx* ctx, vfvm_update_hook hook, void* user);
+/* Convergence scaling (src/base/scaling.rs): a reference scale of an unknown scalar or vector
+ * field, a lower bound of the field's scale in the scaled Picard residual (scaling.rs:88-111)
+ * and the adaptive error norm (Variables::set_scldom_scale / set_vecdom_scale, vars.rs:203-213;
+ * VFVM_ERR_VARIABLES_NON_POSITIVE_SCALE for scale <= 0 or NaN).  For a pressure it is the
+ * reference pressure difference. */
+int vfvm_set_scldom_scale(vfvm_ctx* ctx, int32_t id, double scale);
+int vfvm_set_vecdom_scale(vfvm_ctx* ctx, int32_t id, double scale);
+/* Variables::scale_off (vars.rs:31): off != 0 selects the unscaled Picard residual
+ * |x_new - x_old|_2 / (|x_old|_2 + 1e-10) over the whole system vector (scaling.rs:116-122) and
+ * the per-block error norm of the adaptive driver (transient_adaptive.rs:257-270). */
+int vfvm_set_scale_off(vfvm_ctx* ctx, int32_t off);
+
 typedef struct vfvm_driver_stats {
     int32_t picard_iterations;   /* total Picard iterations                   */
     int32_t linear_iterations;   /* total Arnoldi steps                       */

@@ -420,6 +420,16 @@ def _pow5(x):
     return x4 * x
 
 
+def _adv_coeff(cond, F):
+    """Power-law upwind coefficient of a face (opscl_dom_advunity.rs:96-108, opscl_dom_adv.rs:
+    106-118): Pe = F / cond (0 when cond = 0), A = max(0, 1 - 0.1 |Pe|)^5 and
+    coeff = cond (A - 1) + max(-F, 0), added on (v, n) and subtracted on (v, v)."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        pe = np.where(cond != 0.0, F / cond, 0.0)
+    pl = _pow5(np.maximum(1.0 - 0.1 * np.abs(pe), 0.0))
+    return cond * (pl - 1.0) + np.maximum(-F, 0.0)
+
+
 def assemble(g: dict, ops, scl: dict, sclbnd: dict, flux: dict, bnd_nid: list, dt: float = 0.0,
              value_prev: dict | None = None, flow_den: dict | None = None):
     """Triplets and b of assemble_matrix for the scalar operators, in push order.
@@ -480,12 +490,7 @@ def assemble(g: dict, ops, scl: dict, sclbnd: dict, flux: dict, bnd_nid: list, d
             else:
                 df = np.zeros(e.size)
             cond = df * g["area"][e] / g["dist"][e]
-            F = flux[op.b][e]
-            with np.errstate(divide="ignore", invalid="ignore"):
-                pe = np.where(cond != 0.0, F / cond, 0.0)
-            pl = _pow5(np.maximum(1.0 - 0.1 * np.abs(pe), 0.0))
-            coeff = cond * (pl - 1.0) + np.maximum(-F, 0.0)
-            push_pairs(e, coeff)
+            push_pairs(e, _adv_coeff(cond, flux[op.b][e]))
         elif k in (Op.SCL_DOM_ADVECTION, Op.SCL_BND_ADVECTION):
             # opscl_dom_adv.rs:49-125, opscl_bnd_adv.rs: flux = m_f / rho_f * F
             e = edges_of(emask)
@@ -500,9 +505,7 @@ def assemble(g: dict, ops, scl: dict, sclbnd: dict, flux: dict, bnd_nid: list, d
             cond = df * g["area"][e] / g["dist"][e]
             with np.errstate(divide="ignore", invalid="ignore"):
                 F = np.where(den_f != 0.0, wgt_f / den_f * flux[op.c][e], 0.0)
-                pe = np.where(cond != 0.0, F / cond, 0.0)
-            pl = _pow5(np.maximum(1.0 - 0.1 * np.abs(pe), 0.0))
-            push_pairs(e, cond * (pl - 1.0) + np.maximum(-F, 0.0))
+            push_pairs(e, _adv_coeff(cond, F))
         elif k in (Op.SCL_DOM_TIME_PREVIOUS, Op.SCL_BND_TIME_PREVIOUS, Op.SCL_DOM_TIME_UNITY, Op.SCL_BND_TIME_UNITY):
             # opscl_dom_timeprev.rs:36-52, opscl_dom_timeunity.rs:35-50 (apply_time)
             if not dt > 0.0:
@@ -1024,11 +1027,10 @@ def assemble_flow(g: dict, gc, fields: dict, bnd_vel: list, bnd_nid: list, pref=
         R.append(rr); C.append(cc); V.append(vv)
 
     # OpVecDomAdvection
+    # (opvec_dom_adv.rs:47-116: the scalar power-law coefficient with the viscosity as the
+    # diffusivity, on each component block)
     cond = visc_f * A / d
-    F = flux[e_pts]
-    with np.errstate(divide="ignore", invalid="ignore"):
-        pe = np.where(cond != 0.0, F / cond, 0.0)
-    push_vec_pairs(cond * (_pow5(np.maximum(1.0 - 0.1 * np.abs(pe), 0.0)) - 1.0) + np.maximum(-F, 0.0))
+    push_vec_pairs(_adv_coeff(cond, flux[e_pts]))
     # OpVecDomGradient
     grow = np.repeat(np.arange(N), np.diff(gc_ptr))
     gsel = np.nonzero(grow < NP)[0]
@@ -1181,23 +1183,35 @@ def flow_fields(g, gc, vel, pres, den, visc, fce, vel_const=False, den_const=Tru
                 mom=mom, flux=flux)
 
 
-def scaled_residual_flow(N, x_old, x_new, den):
+def unscaled_residual(x_old, x_new):
+    """The Picard residual with Variables::scale_off (src/base/scaling.rs:116-122):
+    |x_new - x_old|_2 / (|x_old|_2 + 1e-10) over the whole vector, NaN if x_new is not finite."""
+    if not np.all(np.isfinite(x_new)):
+        return float("nan")
+    return float(np.sqrt(((x_new - x_old) ** 2).sum()) / (np.sqrt((x_old * x_old).sum()) + 1e-10))
+
+
+def scaled_residual_flow(N, x_old, x_new, den, scale_vel=0.0, scale_pres=0.0, scale_off=False):
     """Picard residual of the flow unknowns [vel (vector field), pres (pressure field)]
     (src/base/scaling.rs:88-135): max over the two fields of rms(x_new - x_old) / scale, the
     rms over N entries (per component for the vector), pressure changes below the round-off
-    dead band 1e-11 max|p| not counted, scales max|vel| and max(range p, rho_max U_max^2)."""
+    dead band 1e-11 max|p| not counted, scales max(max|vel|, scale_vel) and max(range p,
+    rho_max U_max^2, scale_pres) (user reference scales, vars.rs:203-213, are lower bounds);
+    with scale_off the unscaled residual of the whole vector."""
+    if scale_off:
+        return unscaled_residual(x_old, x_new)
     if not np.all(np.isfinite(x_new)):
         return float("nan")
     vo, vn = x_old[:3 * N].reshape(3, N), x_new[:3 * N].reshape(3, N)
     po, pn = x_old[3 * N:], x_new[3 * N:]
     vmax = max(np.sqrt((vo * vo).sum(axis=0)).max(), np.sqrt((vn * vn).sum(axis=0)).max())
     floor = 1e-30
-    r_v = np.sqrt(((np.abs(vn - vo)) ** 2).sum() / max(N, 1)) / max(vmax, floor)
+    r_v = np.sqrt(((np.abs(vn - vo)) ** 2).sum() / max(N, 1)) / max(vmax, scale_vel, floor)
     dead = 1e-11 * max(np.abs(po).max(), np.abs(pn).max())
     r_p = np.sqrt((np.maximum(np.abs(pn - po) - dead, 0.0) ** 2).sum() / max(N, 1))
     rng = max(po.max(), pn.max()) - min(po.min(), pn.min())
     rho = np.abs(den).max()
-    r_p /= max(rng, rho * vmax * vmax, floor)
+    r_p /= max(rng, rho * vmax * vmax, scale_pres, floor)
     return float(max(r_v, r_p))
 
 
@@ -1242,12 +1256,12 @@ def transient_flow(g, gc, vel0, pres0, den, visc, fce, bnd_vel, bnd_nid, pref, d
     return vel, pres, iters
 
 
-def picard_flow(g, gc, vel0, pres0, den, visc, fce, bnd_vel, bnd_nid, pref, max_iter, tol, damp):
+def picard_flow(g, gc, vel0, pres0, den, visc, fce, bnd_vel, bnd_nid, pref, max_iter, tol, damp, scales=None):
     """SteadyBase::solve (src/steady/steady_base.rs:16-135) for SteadyFlow with constant
     density, viscosity and body force: Dirichlet velocities into the initial state
     (opvec_bnd_dir.rs:34-53), then Picard iterations solving each linear system with
-    SuperLU, damping, re-assembly and the scaled residual.  Returns vel, pres, iterations,
-    residuals."""
+    SuperLU, damping, re-assembly and the scaled residual (scales: keyword arguments of
+    scaled_residual_flow).  Returns vel, pres, iterations, residuals."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
     N, NP = g["N"], g["num_pts"]
@@ -1271,7 +1285,7 @@ def picard_flow(g, gc, vel0, pres0, den, visc, fce, bnd_vel, bnd_nid, pref, max_
         x_d = (1.0 - damp) * x_iter + damp * x_u
         vel, pres = x_d[:3 * N].reshape(3, N).copy(), x_d[3 * N:].copy()
         A, b = system(vel, pres)
-        res = scaled_residual_flow(N, x_iter, x_d, den)
+        res = scaled_residual_flow(N, x_iter, x_d, den, **(scales or {}))
         res_hist.append(res)
         if not np.isfinite(res):
             raise FloatingPointError("SteadyNonFiniteResidual")
@@ -1281,14 +1295,17 @@ def picard_flow(g, gc, vel0, pres0, den, visc, fce, bnd_vel, bnd_nid, pref, max_
     raise RuntimeError("SteadyDidNotConverge")
 
 
-def scaled_residual_scalar(x_old, x_new):
+def scaled_residual_scalar(x_old, x_new, scale=0.0, scale_off=False):
     """Picard residual of one scalar field (src/base/scaling.rs:88-135): rms(x_new - x_old)
-    over its entries divided by max |x| over both states (floor 1e-30)."""
+    over its entries divided by max(|x| over both states, the user scale, 1e-30); with
+    scale_off the unscaled residual."""
+    if scale_off:
+        return unscaled_residual(x_old, x_new)
     if not np.all(np.isfinite(x_new)):
         return float("nan")
     n = max(x_new.size, 1)
     rms = np.sqrt(((np.abs(x_new - x_old)) ** 2).sum() / n)
-    return float(rms / max(np.abs(x_old).max(), np.abs(x_new).max(), 1e-30))
+    return float(rms / max(np.abs(x_old).max(), np.abs(x_new).max(), scale, 1e-30))
 
 
 def heatflow_solve(g, gc, vel0, pres0, T0, den, visc, fce_of, bnd_vel, bnd_nid, pref, heat_rows, t_dirichlet,

@@ -73,7 +73,7 @@ class Problem:
             vk, vv = self.vec[vel]
             vg = O.update_grad_vec(g, vv) if vk != vfvm.FIELD_CONSTANT else np.zeros((9, self.N))
             flux[i] = O.flux_prescribed(g, vv, vg, scl[den][0], scl[den][1])
-        bnd_nid = [self.mesh.bnd_nodes(b) for b in range(6)]
+        bnd_nid = [self.mesh.bnd_nodes(b) for b in range(self.mesh.num_bnd)]
         sb = {i: v for i, (b, v) in enumerate(self.sbnd)}
         ops = [O.Op(*o) for o in self.ops]
         self.flow_den = {i: scl[den] for i, (vel, pres, den, visc, fce) in enumerate(self.flw)}
@@ -473,3 +473,168 @@ class HeatFlowProblem:
             return S
         return load
 
+
+
+# --------------------------------------------------------------------------- closed forms
+def advdiff_1d(mesh, Pe):
+    """Uniform flow u = (Pe, 0, 0) with unit diffusivity (global Peclet number Pe), c = 0 at
+    x = 0 and c = 1 at x = 1, zero diffusive flux on the four other faces, in the SteadyMassFlow
+    operator order (steady_massflow.rs:455-545).  The exact solution is the 1D profile
+    c = (exp(Pe x) - 1) / (exp(Pe) - 1) of D c'' = U c'."""
+    p = Problem(mesh)
+    c = p.scldom(vfvm.FIELD_UNKNOWN, 0.0)
+    D = p.scldom(vfvm.FIELD_CONSTANT, 1.0)
+    den = p.scldom(vfvm.FIELD_CONSTANT, 1.0)
+    visc = p.scldom(vfvm.FIELD_CONSTANT, 1.0)
+    pres = p.scldom(vfvm.FIELD_CONSTANT, 0.0)
+    vel = p.vecdom(vfvm.FIELD_CONSTANT, np.stack([np.full(p.N, float(Pe)), np.zeros(p.N), np.zeros(p.N)]))
+    fce = p.vecdom(vfvm.FIELD_CONSTANT, np.zeros((3, p.N)))
+    f = p.sclflw(vel, pres, den, visc, fce)
+    c0, c1 = p.sclbnd(X0, 0.0), p.sclbnd(X1, 1.0)
+    q = {b: p.sclbnd(b, 0.0) for b in (Y0, Y1, Z0, Z1)}
+    p.ops = [(vfvm.OP_SCL_DOM_DIFFUSION, 0, c, D, c), (vfvm.OP_SCL_DOM_ADVECTION_UNITY, 0, c, D, f),
+             (vfvm.OP_SCL_BND_DIRICHLET, X0, c, c0, -1), (vfvm.OP_SCL_BND_DIRICHLET, X1, c, c1, -1)]
+    for b in (Y0, Y1, Z0, Z1):
+        p.ops += [(vfvm.OP_SCL_BND_DIFFUSION, b, c, D, c), (vfvm.OP_SCL_BND_NEUMANN, b, c, q[b], -1),
+                  (vfvm.OP_SCL_BND_ADVECTION_UNITY, b, c, D, f)]
+    p.unknown = c
+    p.exact = np.expm1(Pe * p.pos[:, 0]) / np.expm1(Pe)
+    return p
+
+
+def regroup_lattice(n, groups_of, seed=0, jitter=0.3, device=None):
+    """The jittered lattice mesh with its nodes regrouped into boundaries: groups_of(pos of the
+    nodes, lattice boundary of each node) returns a group index per node."""
+    m = vfvm.Mesh.lattice(n, seed, jitter)
+    pos = m.get("pos")
+    nb = np.full(m.num_nds, -1)
+    for b in range(m.num_bnd):
+        nb[m.bnd_nodes(b)] = b
+    gid = np.asarray(groups_of(pos[m.num_pts:], nb))
+    groups = [np.nonzero(gid == k)[0].astype(np.int32) for k in range(int(gid.max()) + 1)]
+    return vfvm.Mesh.from_tets(pos, m.get("tets"), m.num_pts, groups, device=device)
+
+
+def neumann_linear_profile(n, k=2.0, flux=0.7):
+    """T = 0 at x = 0, Neumann flux `flux` on the interior of the x = 1 face, zero flux on the
+    four lateral faces, and the border of the x = 1 face held at the exact value: the solution
+    of k lap T = 0 is T = -flux x / k (OpSclBndNeumann adds +A flux to b, so the flux is
+    -k dT/dn), exact on the Voronoi dual because the scheme is exact for linear fields.
+    Groups: 0 x0 (Dirichlet 0), 1 border of x = 1 (Dirichlet -flux/k), 2 interior of x = 1
+    (Neumann flux), 3-6 y0, y1, z0, z1 (Neumann 0)."""
+    def groups(pn, nb):
+        g = np.where(nb == X0, 0, nb + 1)
+        on_x1 = nb == X1
+        border = on_x1 & ((pn[:, 1] == 0.0) | (pn[:, 1] == 1.0) | (pn[:, 2] == 0.0) | (pn[:, 2] == 1.0))
+        g = np.where(on_x1, np.where(border, 1, 2), g)
+        return g
+    mesh = regroup_lattice(n, groups)
+    p = Problem(mesh)
+    T = p.scldom(vfvm.FIELD_UNKNOWN, 0.0)
+    kk = p.scldom(vfvm.FIELD_CONSTANT, k)
+    d0, d1 = p.sclbnd(0, 0.0), p.sclbnd(1, -flux / k)
+    p.ops = [(vfvm.OP_SCL_DOM_DIFFUSION, 0, T, kk, T), (vfvm.OP_SCL_BND_DIRICHLET, 0, T, d0, -1),
+             (vfvm.OP_SCL_BND_DIRICHLET, 1, T, d1, -1)]
+    for b, q in ((2, flux), (3, 0.0), (4, 0.0), (5, 0.0), (6, 0.0)):
+        p.ops += [(vfvm.OP_SCL_BND_DIFFUSION, b, T, kk, T), (vfvm.OP_SCL_BND_NEUMANN, b, T, p.sclbnd(b, q), -1)]
+    p.unknown = T
+    p.exact = -flux * p.pos[:, 0] / k
+    return p
+
+
+def neumann_surface_integral(n, coef):
+    """Every node in one boundary group with a Neumann flux N = a + bx x + by y + bz z (linear):
+    the boundary right-hand sides sum to the integral of N over the cube's surface,
+    6 a + 3 (bx + by + bz), exactly (each sub-face weight is the barycentric coordinate of
+    the sub-face centroid, so a sub-face adds the exact integral of a linear N over it, and
+    the sub-faces of the three corners of a surface triangle tile it)."""
+    def groups(pn, nb):
+        return np.zeros(len(pn), np.int64)
+    mesh = regroup_lattice(n, groups)
+    p = Problem(mesh)
+    T = p.scldom(vfvm.FIELD_UNKNOWN, 0.0)
+    kk = p.scldom(vfvm.FIELD_CONSTANT, 1.0)
+    a, bx, by, bz = coef
+    pn = p.pos[mesh.num_pts + mesh.bnd_nodes(0)]
+    q = p.sclbnd(0, a + bx * pn[:, 0] + by * pn[:, 1] + bz * pn[:, 2])
+    p.ops = [(vfvm.OP_SCL_DOM_DIFFUSION, 0, T, kk, T), (vfvm.OP_SCL_BND_DIFFUSION, 0, T, kk, T),
+             (vfvm.OP_SCL_BND_NEUMANN, 0, T, q, -1)]
+    p.unknown = T
+    p.exact_integral = 6.0 * a + 3.0 * (bx + by + bz)
+    return p
+
+
+REG_TET = np.array([[1.0, 1.0, 1.0], [1.0, -1.0, -1.0], [-1.0, 1.0, -1.0], [-1.0, -1.0, 1.0]])
+
+
+def neumann_tetra(flux=(0.3, -1.1, 2.5, 0.7), groups=((0, 1), (2,), (3,))):
+    """A single regular tetrahedron (edge 2 sqrt 2, face area 2 sqrt 3), all four vertices
+    nodes, in boundary groups `groups`, with the Neumann fluxes `flux` per vertex.  Each face
+    is equilateral, so its circumcentre is its centroid: every corner's sub-face has a third
+    of the face area and the weights (11/18, 7/36, 7/36).  A neighbour on another boundary
+    takes the vertex's own flux (opscl_bnd_neu.rs:73-80), so
+    b_i = (A_f / 3) sum over the faces of i of (11/18 N_i + 7/36 (N~_j + N~_k))."""
+    gid = np.zeros(4, np.int64)
+    for k, gr in enumerate(groups):
+        gid[list(gr)] = k
+    mesh = vfvm.Mesh.from_tets(REG_TET, np.array([[0, 1, 2, 3]], np.int32), 0,
+                               [np.array(gr, np.int32) for gr in groups])
+    p = Problem(mesh)
+    T = p.scldom(vfvm.FIELD_UNKNOWN, 0.0)
+    kk = p.scldom(vfvm.FIELD_CONSTANT, 1.0)
+    fl = np.asarray(flux, np.float64)
+    p.ops = [(vfvm.OP_SCL_DOM_DIFFUSION, 0, T, kk, T)]
+    for k, gr in enumerate(groups):
+        nodes = mesh.bnd_nodes(k)
+        p.ops += [(vfvm.OP_SCL_BND_DIFFUSION, k, T, kk, T), (vfvm.OP_SCL_BND_NEUMANN, k, T, p.sclbnd(k, fl[nodes]), -1)]
+    p.unknown = T
+    area_f = 2.0 * np.sqrt(3.0)
+    exact = np.zeros(4)
+    for i in range(4):
+        for face in ([0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]):
+            if i not in face:
+                continue
+            j, k2 = [x for x in face if x != i]
+            nj = fl[j] if gid[j] == gid[i] else fl[i]
+            nk = fl[k2] if gid[k2] == gid[i] else fl[i]
+            exact[i] += area_f / 3.0 * (11.0 / 18.0 * fl[i] + 7.0 / 36.0 * (nj + nk))
+    p.exact_b = exact
+    return p
+
+
+def poiseuille(mesh, U=1.0, mu=1.0, rho=1.0):
+    """Plane Poiseuille flow between z = 0 and z = 1 through the whole Navier-Stokes operator
+    set (the closed form of examples/flow/flow_analytic.rs:10): u = (4 U z (1 - z), 0, 0) held
+    on every face, no body force; the exact fields are that velocity and p = -8 mu U x (zero at
+    x = 0), the reference pressure pinned at node 0 to its exact value."""
+    pos = mesh.get("pos")
+    N = mesh.num_tot
+    uex = np.stack([4.0 * U * pos[:, 2] * (1.0 - pos[:, 2]), np.zeros(N), np.zeros(N)])
+    pex = -8.0 * mu * U * pos[:, 0]
+    bnd = []
+    for b in range(6):
+        nodes = mesh.num_pts + mesh.bnd_nodes(b).astype(np.int64)
+        bnd.append((b, uex[:, nodes]))
+    nid0 = mesh.num_pts
+    p = FlowProblem(mesh, np.zeros((3, N)), 0.0, rho, mu, np.zeros((3, N)), bnd, pref=(0, float(pex[nid0])))
+    p.uex, p.pex = uex, pex
+    return p
+
+
+def suction_profile(mesh, U=1.0, V=1.0, mu=0.2, rho=1.0):
+    """Asymptotic suction profile: a shear flow u(z) between z = 0 (u = 0) and z = 1 (u = U)
+    with a uniform cross-flow w = -V through both walls; the momentum balance rho w u' = mu u''
+    gives u = U (1 - exp(-V z / nu)) / (1 - exp(-V / nu)), nu = mu / rho, at constant pressure.
+    The exact velocity is held on every face.  Pins the upwind side of the momentum advection
+    (OpVecDomAdvection, opvec_dom_adv.rs:47-116): a flipped sign convects the layer the wrong way."""
+    pos = mesh.get("pos")
+    N = mesh.num_tot
+    nu = mu / rho
+    uex = np.stack([U * -np.expm1(-V * pos[:, 2] / nu) / -np.expm1(-V / nu), np.zeros(N), np.full(N, -V)])
+    bnd = []
+    for b in range(6):
+        nodes = mesh.num_pts + mesh.bnd_nodes(b).astype(np.int64)
+        bnd.append((b, uex[:, nodes]))
+    p = FlowProblem(mesh, np.zeros((3, N)), 0.0, rho, mu, np.zeros((3, N)), bnd, pref=(0, 0.0))
+    p.uex, p.pex = uex, np.zeros(N)
+    return p

@@ -180,3 +180,31 @@ def test_transient_lid_cavity_vs_oracle(ctx):
     assert st.picard_iterations == sum(iters), (st.picard_iterations, iters)
     assert np.max(np.abs(vel - ovel)) <= 1e-8 * np.max(np.abs(ovel))
     assert np.max(np.abs(pres - opres)) <= 1e-8 * np.max(np.abs(opres))
+
+
+@pytest.fixture(scope="module")
+def lid_oracle_re100():
+    mesh = vfvm.Mesh.lattice(6, 0, 0.3)
+    p = lid_cavity(mesh, rho=1.0, mu=0.01, lid=1.0)
+    g = O.dual(mesh.get("pos"), mesh.get("tets"), mesh.num_pts)
+    gc = O.grad_coeff(g)
+    return mesh, O.picard_flow(g, gc, p.vel, p.pres, p.den, p.visc, p.fce, p.bnd_vel,
+                               [mesh.bnd_nodes(k) for k in range(6)], p.pref, 100, 1e-6, 0.8)
+
+
+@pytest.mark.parametrize("precision,bar", [(vfvm.FP64, 1e-8), (vfvm.FP32_REFINED, 1e-7)])
+def test_lid_cavity_re100_picard_vs_oracle(ctx, lid_oracle_re100, precision, bar):
+    """The lid-driven cavity at its configured Reynolds number (Re = 100: rho = 1, mu = 0.01, lid
+    speed 1, unit cavity) through the device Picard driver (steady_base.rs:74-117, damping 0.8)
+    to a scaled residual of 1e-6: the oracle's number of Picard iterations and its fields."""
+    print(f"[flow lid cavity Re=100 precision={precision}]", flush=True)
+    mesh, (ovel, opres, its, hist) = lid_oracle_re100
+    p = lid_cavity(mesh, rho=1.0, mu=0.01, lid=1.0)
+    ops = p.to_device(ctx)
+    st = ctx.steady_solve(ops, 100, 1e-6, 0.8, lin=dict(tol=1e-13, restart=250, max_outer=400, precision=precision,
+                                                       precond=vfvm.PRECOND_BJACOBI, inner_tol=1e-6))
+    vel = ctx.vecdom_get(p.ids[0])
+    pres = ctx.scldom_get(p.ids[1])
+    assert st.picard_iterations == its, (st.picard_iterations, its, hist)
+    assert np.max(np.abs(vel - ovel)) <= bar * np.max(np.abs(ovel))
+    assert np.max(np.abs(pres - opres)) <= bar * max(np.max(np.abs(opres)), 1e-30)

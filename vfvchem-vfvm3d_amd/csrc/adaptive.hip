@@ -58,10 +58,11 @@ using clk = std::chrono::steady_clock;
 double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
 // one unknown field of the error norm: entries [start, start + len), kind 0 scalar, 1 vector
-// (3 components of `len` vertices), 2 pressure
+// (3 components of `len` vertices), 2 pressure; id: its scldom / vecdom id (user scale)
 struct ErrField {
     int64_t start, len;
     int kind;
+    int32_t id;
 };
 
 struct System {
@@ -106,12 +107,17 @@ int error_norm(System& sy, const std::vector<ErrField>& fields, const double* xn
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         double emax = 0.0, ymax = 0.0;
         for (unsigned k = 0; k < nb; ++k) { emax = std::max(emax, h[2 * k]); ymax = std::max(ymax, h[2 * k + 1]); }
-        // field scale (scaling.rs:88-111) and the pressure round-off (scaling.rs:30-41)
+        // field scale (scaling.rs:88-111; a user scale is a lower bound) and the pressure
+        // round-off (scaling.rs:30-41); with scale_off every block is measured against its own
+        // max(|y_n|, |y|) with no round-off term (transient_adaptive.rs:257-270)
         double scale = ymax, round = 0.0;
-        if (f.kind == 1) scale = st[1];
-        if (f.kind == 2) {
-            scale = std::max(st[4] - st[3], st[6] * st[1] * st[1]);
-            round = 1e-11 * st[2];
+        if (!c.scale_off) {
+            if (f.kind == 0) scale = std::max(ymax, c.user_scale_scl(f.id));
+            if (f.kind == 1) scale = std::max(st[1], c.user_scale_vec(f.id));
+            if (f.kind == 2) {
+                scale = std::max(std::max(st[4] - st[3], st[6] * st[1] * st[1]), c.user_scale_scl(f.id));
+                round = 1e-11 * st[2];
+            }
         }
         scale = std::max(scale, 1e-30);
         e = std::max(e, emax / (atol + round + rtol * scale));
@@ -197,13 +203,18 @@ extern "C" int vfvm_transient_adaptive(vfvm_ctx* ctx, const vfvm_op* ops, int32_
             if (opt->scl_exclude && opt->scl_exclude[k] == id) return true;
         return false;
     };
+    // the error norm's fields (field_norms, scaling.rs:43-60), or with scale_off its blocks: every
+    // scalar unknown and every velocity component on its own (_field_blocks, transient_adaptive.rs:355-370)
     if (c.flow) {
-        fields.push_back({0, c.g.N, 1});
-        if (!excluded(c.fs.pres)) fields.push_back({3 * c.g.N, c.g.N, 2});
+        if (c.scale_off)
+            for (int k = 0; k < 3; ++k) fields.push_back({k * c.g.N, c.g.N, 0, -1});
+        else
+            fields.push_back({0, c.g.N, 1, c.fs.vel});
+        if (!excluded(c.fs.pres)) fields.push_back({3 * c.g.N, c.g.N, 2, c.fs.pres});
         for (size_t k = 0; k < c.sblk.size(); ++k)
-            if (!excluded(c.sblk[k].unk)) fields.push_back({(int64_t)(4 + k) * c.g.N, c.g.N, 0});
+            if (!excluded(c.sblk[k].unk)) fields.push_back({(int64_t)(4 + k) * c.g.N, c.g.N, 0, c.sblk[k].unk});
     } else if (!excluded(c.unk)) {
-        fields.push_back({0, c.g.N, 0});
+        fields.push_back({0, c.g.N, 0, c.unk});
     }
     const int64_t n = sy.n;
     DevBuf<double> xn, xm, xi, xu, xd;

@@ -47,11 +47,13 @@ __global__ void k_damp(int64_t n, double d, const double* __restrict__ xi, const
     int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i < n) out[i] = (1.0 - d) * xi[i] + d * xu[i];
 }
-// partials of sum (x_new - x_old)^2, max |x_old|, max |x_new|, non-finite count (scaling.rs:113-135)
+// partials of sum (x_new - x_old)^2, max |x_old|, max |x_new|, non-finite count and
+// sum x_old^2 (scaling.rs:113-135)
+constexpr int PIC_PARTS = 5;
 __global__ void k_picard(int64_t n, const double* __restrict__ xo, const double* __restrict__ xn, double* __restrict__ part) {
-    __shared__ double s0[BLK], s1[BLK], s2[BLK], s3[BLK];
+    __shared__ double s0[BLK], s1[BLK], s2[BLK], s3[BLK], s4[BLK];
     int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    double d2 = 0.0, mo = 0.0, mn = 0.0, nf = 0.0;
+    double d2 = 0.0, mo = 0.0, mn = 0.0, nf = 0.0, o2 = 0.0;
     if (i < n) {
         double a = xo[i], b = xn[i];
         if (!isfinite(b)) nf = 1.0;
@@ -59,8 +61,9 @@ __global__ void k_picard(int64_t n, const double* __restrict__ xo, const double*
         d2 = d * d;
         mo = fabs(a);
         mn = fabs(b);
+        o2 = a * a;
     }
-    s0[threadIdx.x] = d2; s1[threadIdx.x] = mo; s2[threadIdx.x] = mn; s3[threadIdx.x] = nf;
+    s0[threadIdx.x] = d2; s1[threadIdx.x] = mo; s2[threadIdx.x] = mn; s3[threadIdx.x] = nf; s4[threadIdx.x] = o2;
     __syncthreads();
     for (int off = BLK / 2; off > 0; off >>= 1) {
         if (threadIdx.x < off) {
@@ -68,12 +71,13 @@ __global__ void k_picard(int64_t n, const double* __restrict__ xo, const double*
             s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + off]);
             s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + off]);
             s3[threadIdx.x] += s3[threadIdx.x + off];
+            s4[threadIdx.x] += s4[threadIdx.x + off];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[4 * blockIdx.x] = s0[0]; part[4 * blockIdx.x + 1] = s1[0];
-        part[4 * blockIdx.x + 2] = s2[0]; part[4 * blockIdx.x + 3] = s3[0];
+        double* p = part + PIC_PARTS * blockIdx.x;
+        p[0] = s0[0]; p[1] = s1[0]; p[2] = s2[0]; p[3] = s3[0]; p[4] = s4[0];
     }
 }
 
@@ -243,6 +247,9 @@ extern "C" int vfvm_create(int32_t device, vfvm_ctx** out) {
     h->c.graphs_enabled = !(gr && gr[0] == '0');
     e = hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete h; return set_error(VFVM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    // pinned readback buffer of the solver's control word and norms
+    e = hipHostMalloc((void**)&h->c.h_pinned, 4096, hipHostMallocDefault);
+    if (e != hipSuccess) { vfvm_destroy(h); return set_error(VFVM_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e)); }
     *out = h;
     return clear_error();
 }
@@ -268,6 +275,8 @@ extern "C" int vfvm_mesh_upload(vfvm_ctx* ctx, const vfvm_mesh* mesh) {
     const Mesh& m = *mp;
     // reset everything that depends on the mesh
     c.scl.clear(); c.vec.clear(); c.sbnd.clear(); c.flw.clear(); c.vbndv.clear();
+    c.scale_scl.clear(); c.scale_vec.clear();
+    c.lin.active = false;
     c.has_pattern = false; c.has_matrix = false; c.ilu.ready = false; c.flow = false;
     c.has_transpose = false; c.tilu.ready = false; c.tape.valid = false;
     DevGeom& g = c.g;
@@ -502,6 +511,7 @@ extern "C" int vfvm_vecbnd_set(vfvm_ctx* ctx, int32_t id, const double* values) 
 // ------------------------------------------------------------------ pattern (LinearBase::update_sparse)
 extern "C" int vfvm_pattern_build(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, int64_t* nnz) {
     CTX_GUARD(ctx);
+    c.lin.active = false;
     c.has_pattern = false;
     c.has_matrix = false;
     c.ilu.ready = false;
@@ -652,6 +662,8 @@ int vfvm::prepare_scalar_fields(Ctx& c, const vfvm_op* ops, int32_t nops, int32_
 
 int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
     if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "assemble: no pattern built (call vfvm_pattern_build)");
+    if (c.lin.active)
+        return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: the system was built from triplets (vfvm_linear_update_sparse)");
     if (c.flow || is_flow_list(ops, nops)) return flow_assemble_dt(c, ops, nops, dt);
     Compiled cp;
     if (int st = compile_ops(c, ops, nops, cp)) return st;
@@ -804,19 +816,33 @@ int vfvm::scalar_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
 
 namespace {
 
-// scaled Picard residual for one scalar field of n entries (scaling.rs:88-135)
-int picard_residual(Ctx& c, const double* xo, const double* xn, double* res, int64_t n) {
+struct PicardSums {
+    double d2 = 0.0, mo = 0.0, mn = 0.0, nf = 0.0, o2 = 0.0;
+};
+int picard_sums(Ctx& c, const double* xo, const double* xn, int64_t n, PicardSums& s) {
     const unsigned nb = grid_for(n);
-    VFVM_HIP_TRY(c.red.grow(4 * (size_t)nb));
+    VFVM_HIP_TRY(c.red.grow(PIC_PARTS * (size_t)nb));
     hipLaunchKernelGGL(k_picard, dim3(nb), dim3(BLK), 0, c.stream, n, xo, xn, c.red.p);
-    std::vector<double> h(4 * (size_t)nb);
+    std::vector<double> h(PIC_PARTS * (size_t)nb);
     VFVM_HIP_TRY(hipMemcpyAsync(h.data(), c.red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
-    double sum = 0.0, mo = 0.0, mn = 0.0, nf = 0.0;
-    for (unsigned k = 0; k < nb; ++k) { sum += h[4 * k]; mo = std::max(mo, h[4 * k + 1]); mn = std::max(mn, h[4 * k + 2]); nf += h[4 * k + 3]; }
-    if (nf > 0.0) { *res = NAN; return VFVM_OK; }
-    const double rms = std::sqrt(sum / (double)std::max<int64_t>(n, 1));
-    const double scale = std::max(std::max(mo, mn), 1e-30);
+    for (unsigned k = 0; k < nb; ++k) {
+        const double* p = h.data() + PIC_PARTS * (size_t)k;
+        s.d2 += p[0]; s.mo = std::max(s.mo, p[1]); s.mn = std::max(s.mn, p[2]); s.nf += p[3]; s.o2 += p[4];
+    }
+    return VFVM_OK;
+}
+
+// scaled Picard residual for one scalar field of n entries (scaling.rs:88-135): rms of the
+// change over the field scale max(|x| over both states, the field's user scale, 1e-30); with
+// scale_off the unscaled |x_new - x_old| / (|x_old| + 1e-10)
+int picard_residual(Ctx& c, const double* xo, const double* xn, double* res, int64_t n, int32_t fid) {
+    PicardSums s;
+    if (int st = picard_sums(c, xo, xn, n, s)) return st;
+    if (s.nf > 0.0) { *res = NAN; return VFVM_OK; }
+    if (c.scale_off) { *res = std::sqrt(s.d2) / (std::sqrt(s.o2) + 1e-10); return VFVM_OK; }
+    const double rms = std::sqrt(s.d2 / (double)std::max<int64_t>(n, 1));
+    const double scale = std::max(std::max(std::max(s.mo, s.mn), c.user_scale_scl(fid)), 1e-30);
     *res = rms / scale;
     return VFVM_OK;
 }
@@ -856,7 +882,7 @@ int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts*
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         ds.time_assembly += secs(ti1, clk::now());
         double res = 0.0;
-        if (int st3 = picard_residual(c, xi.p, u.value.p, &res, n)) return st3;
+        if (int st3 = picard_residual(c, xi.p, u.value.p, &res, n, c.unk)) return st3;
         ds.residual = res;
         ds.picard_iterations += 1;
         if (!std::isfinite(res))
@@ -873,11 +899,48 @@ int picard_loop(Ctx& c, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts*
 }  // namespace
 
 int vfvm::scalar_picard_residual(Ctx& c, const double* xo, const double* xn, double* res) {
-    return picard_residual(c, xo, xn, res, c.n);
+    return picard_residual(c, xo, xn, res, c.n, c.unk);
 }
 
-int vfvm::scalar_field_residual(Ctx& c, const double* xo, const double* xn, int64_t n, double* res) {
-    return picard_residual(c, xo, xn, res, n);
+int vfvm::scalar_field_residual(Ctx& c, const double* xo, const double* xn, int64_t n, int32_t fid, double* res) {
+    return picard_residual(c, xo, xn, res, n, fid);
+}
+
+int vfvm::unscaled_residual(Ctx& c, const double* xo, const double* xn, int64_t n, double* res) {
+    PicardSums s;
+    if (int st = picard_sums(c, xo, xn, n, s)) return st;
+    *res = s.nf > 0.0 ? NAN : std::sqrt(s.d2) / (std::sqrt(s.o2) + 1e-10);
+    return VFVM_OK;
+}
+
+// ------------------------------------------------------------------ convergence scaling
+// Variables::set_scldom_scale / set_vecdom_scale (vars.rs:203-213): a reference scale of an
+// unknown field, a lower bound of its scale in the Picard residual and the adaptive error norm
+extern "C" int vfvm_set_scldom_scale(vfvm_ctx* ctx, int32_t id, double scale) {
+    CTX_GUARD(ctx);
+    if (std::isnan(scale) || scale <= 0.0)
+        return set_error(VFVM_ERR_VARIABLES_NON_POSITIVE_SCALE, "reference scale must be greater than zero (VariablesNonPositiveScale)");
+    if (!check_scl(c, id)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_set_scldom_scale: id out of range");
+    if ((int32_t)c.scale_scl.size() <= id) c.scale_scl.resize(id + 1, 0.0);
+    c.scale_scl[id] = scale;
+    return clear_error();
+}
+
+extern "C" int vfvm_set_vecdom_scale(vfvm_ctx* ctx, int32_t id, double scale) {
+    CTX_GUARD(ctx);
+    if (std::isnan(scale) || scale <= 0.0)
+        return set_error(VFVM_ERR_VARIABLES_NON_POSITIVE_SCALE, "reference scale must be greater than zero (VariablesNonPositiveScale)");
+    if (id < 0 || id >= (int32_t)c.vec.size()) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_set_vecdom_scale: id out of range");
+    if ((int32_t)c.scale_vec.size() <= id) c.scale_vec.resize(id + 1, 0.0);
+    c.scale_vec[id] = scale;
+    return clear_error();
+}
+
+// Variables::scale_off (vars.rs:31)
+extern "C" int vfvm_set_scale_off(vfvm_ctx* ctx, int32_t off) {
+    CTX_GUARD(ctx);
+    c.scale_off = off != 0;
+    return clear_error();
 }
 
 int vfvm::call_update_hook(Ctx& c, double t) {

@@ -208,6 +208,15 @@ struct ScalarBlock {
     DevBuf<DevProgram> prog;
 };
 
+// A system assembled by the caller from triplets (LinearBase): the triplets of CSR slot s are
+// trip[sptr[s] .. sptr[s+1]) in list order; tv holds the uploaded triplet values.
+struct LinearSys {
+    bool active = false;
+    int64_t ntrip = 0;
+    DevBuf<int64_t> sptr, trip;
+    DevBuf<double> tv;
+};
+
 struct KTimer {
     int64_t launches = 0;
     double ms = 0.0;
@@ -236,6 +245,7 @@ struct Ctx {
     DevBuf<double> vals, b;
     bool has_matrix = false;
 
+    LinearSys lin;  // set when the pattern came from vfvm_linear_update_sparse
     IluData ilu;  // ILU(0) ordering of the pattern above
     BlockJacobi bjac;
 
@@ -243,6 +253,15 @@ struct Ctx {
     bool flow = false;
     FlowSys fs;
     std::vector<ScalarBlock> sblk;
+
+    // convergence scaling (Variables::scale_scldom / scale_vecdom / scale_off, vars.rs:28-31):
+    // user reference scales per field id (0: none), lower bounds of the field scales of the
+    // Picard residual and the adaptive error norm (scaling.rs:88-111); scale_off selects the
+    // unscaled residual |x_new - x_old| / |x_old| (scaling.rs:120-121)
+    std::vector<double> scale_scl, scale_vec;
+    bool scale_off = false;
+    double user_scale_scl(int32_t id) const { return id >= 0 && id < (int32_t)scale_scl.size() ? scale_scl[id] : 0.0; }
+    double user_scale_vec(int32_t id) const { return id >= 0 && id < (int32_t)scale_vec.size() ? scale_vec[id] : 0.0; }
 
     // host closure of function fields that depend on unknowns (Variables::_update_func),
     // called before every assembly of the drivers
@@ -332,7 +351,11 @@ int flow_residual(Ctx& c, const double* xo, const double* xn, double* res);
 int flow_stats(Ctx& c, const double* xo, const double* xn, double out[7]);
 // scaled Picard residual of the scalar unknown (api_ctx.hip)
 int scalar_picard_residual(Ctx& c, const double* xo, const double* xn, double* res);
-int scalar_field_residual(Ctx& c, const double* xo, const double* xn, int64_t n, double* res);
+// scaled residual of one scalar field of n entries (scale lower bound: field fid's user scale)
+int scalar_field_residual(Ctx& c, const double* xo, const double* xn, int64_t n, int32_t fid, double* res);
+// |x_new - x_old|_2 / (|x_old|_2 + 1e-10) over n entries, NaN if x_new is not finite (the
+// residual with scale_off, scaling.rs:116-122)
+int unscaled_residual(Ctx& c, const double* xo, const double* xn, int64_t n, double* res);
 // scalar operator lists inside a coupled system (api_ctx.hip)
 int prepare_scalar_fields(Ctx& c, const vfvm_op* ops, int32_t nops, int32_t skip_flw);
 int scalar_classes(Ctx& c, const vfvm_op* ops, int32_t nops, int32_t* unk, std::vector<uint8_t>& has_edges,

@@ -1227,6 +1227,7 @@ int vfvm::flow_stats(Ctx& c, const double* xo, const double* xn, double out[7]) 
 // (scaling.rs:113-135)
 int vfvm::flow_residual(Ctx& c, const double* xo, const double* xn, double* res) {
     const int64_t N = c.g.N;
+    if (c.scale_off) return unscaled_residual(c, xo, xn, c.n, res);
     double st[7];
     if (int r = flow_stats(c, xo, xn, st)) return r;
     if (st[5] > 0.0) { *res = NAN; return VFVM_OK; }
@@ -1242,12 +1243,13 @@ int vfvm::flow_residual(Ctx& c, const double* xo, const double* xn, double* res)
     const double floor = 1e-30;  // SCALE_FLOOR (scaling.rs:27)
     const double nn = (double)std::max<int64_t>(N, 1);
     const double vmax = st[1];
-    const double scale_p = std::max(std::max(st[4] - st[3], st[6] * vmax * vmax), floor);
-    const double scale_v = std::max(vmax, floor);
+    // user reference scales are lower bounds (scaling.rs:95-107)
+    const double scale_p = std::max(std::max(std::max(st[4] - st[3], st[6] * vmax * vmax), c.user_scale_scl(c.fs.pres)), floor);
+    const double scale_v = std::max(std::max(vmax, c.user_scale_vec(c.fs.vel)), floor);
     double r = std::max(std::sqrt(sp / nn) / scale_p, std::sqrt(st[0] / nn) / scale_v);
     for (size_t k = 0; k < c.sblk.size(); ++k) {
         double rs = 0.0;
-        if (int e = scalar_field_residual(c, xo + (4 + k) * N, xn + (4 + k) * N, N, &rs)) return e;
+        if (int e = scalar_field_residual(c, xo + (4 + k) * N, xn + (4 + k) * N, N, c.sblk[k].unk, &rs)) return e;
         if (std::isnan(rs)) { *res = NAN; return VFVM_OK; }
         r = std::max(r, rs);
     }

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -1003,6 +1004,7 @@ struct Work {
     int* ctrl;    // device control word
     int* h_ctrl;  // pinned
     double* S;    // device state
+    double* part; // reduction partials
 };
 
 // lanes per CSR row from the average row length (scalar rows carry ~17 entries, the generic
@@ -1267,7 +1269,7 @@ int issue_step(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* 
     const T* lu = pc.lu;
     const int64_t n = w.n;
     const int lpr = pick_lpr(n, w.nnz);
-    double* part = c.part.p;
+    double* part = w.part;
     double* part_nrm = part + (int64_t)(MAX_RESTART + 1) * w.nwav;
     // algorithmic bytes: values, the column indices the kernel must read, row pointers, x, y
     const bool flow_spmv = sy.flow_layout != 0;
@@ -1324,7 +1326,7 @@ CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv
                                     (uint64_t)(uintptr_t)V,
                                     (uint64_t)(uintptr_t)wv, (uint64_t)(uintptr_t)zv,
                                     (uint64_t)(uintptr_t)aval, (uint64_t)(uintptr_t)lu, (uint64_t)(uintptr_t)w.S,
-                                    (uint64_t)(uintptr_t)w.ctrl, (uint64_t)(uintptr_t)c.part.p,
+                                    (uint64_t)(uintptr_t)w.ctrl, (uint64_t)(uintptr_t)w.part,
                                     (uint64_t)(uintptr_t)sy.row, (uint64_t)(uintptr_t)sy.col, (uint64_t)w.n,
                                     (uint64_t)w.nnz, (uint64_t)w.m, (uint64_t)reorth, (uint64_t)sizeof(T),
                                     (uint64_t)(uintptr_t)I, I ? I->generation : 0, (uint64_t)w.nblk, (uint64_t)w.nwav, (uint64_t)w.ldv};
@@ -1367,10 +1369,10 @@ CycleGraphs* cycle_graphs(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv
 // need the eager path) or graphs are off (VFVM_GRAPHS=0).
 template <class T>
 int arnoldi_cycle(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const T* aval, const Pc<T>& pc, bool reorth,
-                  int* steps_out) {
+                  bool graphs, int* steps_out) {
     int steps = 0;
     CycleGraphs* cg = nullptr;
-    if (c.timing_mask == 0 && c.graphs_enabled) {
+    if (graphs && c.timing_mask == 0 && c.graphs_enabled) {
         int status = VFVM_OK;
         cg = cycle_graphs<T>(c, sy, w, V, wv, zv, aval, pc, reorth, &status);
         if (status != VFVM_OK) return status;
@@ -1526,7 +1528,76 @@ int residual_norm(Ctx& c, const SysView& sy, const double* x_dev, double* rel, d
     return VFVM_OK;
 }
 
+namespace {
+
+// Device workspace of one solve: pointers into the context's buffers, or into a caller-provided
+// region (the no-allocation entry point).  In the latter mode the solve launches eagerly (no
+// graph capture or instantiation, no events) and supports FP64 with Jacobi or no preconditioner.
+struct GmresBufs {
+    bool noalloc = false;
+    double* S = nullptr;
+    int* ctrl = nullptr;
+    double *part = nullptr, *r64 = nullptr, *dinv64 = nullptr, *w64 = nullptr, *z64 = nullptr, *V64 = nullptr,
+           *avals64 = nullptr, *bp64 = nullptr, *xp64 = nullptr;
+    float *V32 = nullptr, *w32 = nullptr, *avals32 = nullptr;
+};
+
+// sizes of the workspace (entries) for a solve of n rows, nnz entries, restart m
+struct GmresSizes {
+    int64_t ldv, nwav, nblk, nblk_r, state, part;
+};
+GmresSizes gmres_sizes(int64_t n, int64_t nnz, int m) {
+    GmresSizes z;
+    z.nblk = std::min<int64_t>(1024, std::max<int64_t>(1, (n + 4 * BLK - 1) / (4 * BLK)));
+    // multi-dot: up to 1024 waves per group of basis vectors, 8+ rows per lane
+    constexpr int64_t mdot_waves = 1024;
+    z.nwav = std::min<int64_t>(mdot_waves, std::max<int64_t>(1, (n + WAVE * 8 - 1) / (WAVE * 8)));
+    z.nwav = (z.nwav + BLK / WAVE - 1) / (BLK / WAVE) * (BLK / WAVE);
+    z.ldv = (n + 63) / 64 * 64;
+    const int lpr = pick_lpr(n, nnz);
+    z.nblk_r = std::min<int64_t>(1024, grid_for(n * lpr));
+    StateLayout L;
+    L.m = m;
+    z.state = L.total() + 64;
+    z.part = (int64_t)(MAX_RESTART + 1) * z.nwav + z.nblk + 2 * z.nblk_r + 16;
+    return z;
+}
+
+int gmres_run(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, vfvm_solve_stats& st, GmresBufs& B);
+
+}  // namespace
+
 int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, vfvm_solve_stats& st) {
+    std::memset(&st, 0, sizeof(st));
+    if (o.restart <= 0 || o.restart > MAX_RESTART) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_RESTART, "GMRES: restart must be in [1, 256]");
+    const bool mixed = o.precision == VFVM_FP32_REFINED;
+    const int64_t n = sy0.n;
+    const GmresSizes z = gmres_sizes(n, sy0.nnz, o.restart);
+    GmresBufs B;
+    VFVM_HIP_TRY(c.scal.alloc((size_t)z.state));
+    B.S = c.scal.p;
+    B.ctrl = reinterpret_cast<int*>(c.scal.p + (z.state - 64));
+    VFVM_HIP_TRY(c.part.alloc((size_t)z.part));
+    B.part = c.part.p;
+    VFVM_HIP_TRY(c.r64.alloc(n));
+    VFVM_HIP_TRY(c.dinv64.alloc(n));
+    VFVM_HIP_TRY(c.w64.alloc(n));
+    VFVM_HIP_TRY(c.z64.alloc(n));
+    B.r64 = c.r64.p; B.dinv64 = c.dinv64.p; B.w64 = c.w64.p; B.z64 = c.z64.p;
+    if (!mixed) {
+        VFVM_HIP_TRY(c.V64.alloc((size_t)z.ldv * (o.restart + 1)));
+        B.V64 = c.V64.p;
+    } else {
+        VFVM_HIP_TRY(c.V32.alloc((size_t)z.ldv * (o.restart + 1)));
+        VFVM_HIP_TRY(c.w32.alloc(2 * (size_t)n));
+        B.V32 = c.V32.p; B.w32 = c.w32.p;
+    }
+    return gmres_run(c, sy0, o, x0, st, B);
+}
+
+namespace {
+
+int gmres_run(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, vfvm_solve_stats& st, GmresBufs& B) {
     std::memset(&st, 0, sizeof(st));
     if (!(o.tol > 0.0)) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_TOLERANCE, "GMRES: tolerance must be positive");
     if (o.restart <= 0 || o.restart > MAX_RESTART) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_RESTART, "GMRES: restart must be in [1, 256]");
@@ -1548,11 +1619,15 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     // vectors: P A P^T with the four unknowns of a vertex adjacent
     const bool il = sy0.flow_layout != 0 && (bjac || o.precond == VFVM_PRECOND_NONE);
     if (il) {
-        VFVM_HIP_TRY(c.bp64.alloc(sy0.n));
-        VFVM_HIP_TRY(c.xp64.alloc(sy0.n));
+        if (!B.noalloc) {
+            VFVM_HIP_TRY(c.bp64.alloc(sy0.n));
+            VFVM_HIP_TRY(c.xp64.alloc(sy0.n));
+            B.bp64 = c.bp64.p;
+            B.xp64 = c.xp64.p;
+        }
         sy.flow_layout = 2;
-        sy.b = c.bp64.p;
-        x = c.xp64.p;
+        sy.b = B.bp64;
+        x = B.xp64;
     } else if (sy0.flow_layout && ilu) {
         sy.flow_layout = 0;
     }
@@ -1571,48 +1646,43 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     }
     const int64_t n = sy.n;
     const int m = o.restart;
+    const GmresSizes z = gmres_sizes(n, sy.nnz, m);
     Work w;
     w.n = n;
     w.nnz = sy.nnz;
     w.m = m;
-    w.nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 4 * BLK - 1) / (4 * BLK)));
+    w.nblk = (int)z.nblk;
     w.rows_per_blk = ((n + w.nblk - 1) / w.nblk + 3) / 4 * 4;
-    // multi-dot: up to 1024 waves per group of basis vectors, 8+ rows per lane
-    constexpr int64_t mdot_waves = 1024;
-    w.nwav = (int)std::min<int64_t>(mdot_waves, std::max<int64_t>(1, (n + WAVE * 8 - 1) / (WAVE * 8)));
-    w.nwav = (w.nwav + BLK / WAVE - 1) / (BLK / WAVE) * (BLK / WAVE);
+    w.nwav = (int)z.nwav;
     w.rows_per_wave = ((n + w.nwav - 1) / w.nwav + 3) / 4 * 4;
-    w.ldv = (n + 63) / 64 * 64;
+    w.ldv = z.ldv;
     w.L.m = m;
     const int lpr = pick_lpr(n, sy.nnz);
-    const int nblk_r = (int)std::min<int64_t>(1024, grid_for(n * lpr));
-
-    // workspace
-    VFVM_HIP_TRY(c.scal.alloc((size_t)w.L.total() + 64));
-    w.S = c.scal.p;
-    w.ctrl = reinterpret_cast<int*>(c.scal.p + w.L.total());
-    if (!c.h_pinned) VFVM_HIP_TRY(hipHostMalloc((void**)&c.h_pinned, 4096, hipHostMallocDefault));
+    const int nblk_r = (int)z.nblk_r;
+    w.S = B.S;
+    w.ctrl = B.ctrl;
+    w.part = B.part;
+    if (!c.h_pinned) return set_error(VFVM_ERR_HIP, "GMRES: the context has no pinned readback buffer");
     w.h_ctrl = reinterpret_cast<int*>(c.h_pinned);
-    VFVM_HIP_TRY(c.part.alloc((size_t)(MAX_RESTART + 1) * w.nwav + w.nblk + 2 * (size_t)nblk_r + 16));
-    VFVM_HIP_TRY(c.r64.alloc(n));
-    VFVM_HIP_TRY(c.dinv64.alloc(n));
-    VFVM_HIP_TRY(c.w64.alloc(n));
-    VFVM_HIP_TRY(c.z64.alloc(n));
-    if (!mixed) VFVM_HIP_TRY(c.V64.alloc((size_t)w.ldv * (m + 1)));
-    else {
-        VFVM_HIP_TRY(c.V32.alloc((size_t)w.ldv * (m + 1)));
-        VFVM_HIP_TRY(c.w32.alloc(2 * (size_t)n));
-    }
+    double* const part = B.part;
+    double* const r64 = B.r64;
+    double* const dinv64 = B.dinv64;
+    double* const w64 = B.w64;
+    double* const z64 = B.z64;
 
     HipEvents ev;
-    VFVM_HIP_TRY(ev.create());
-    VFVM_HIP_TRY(hipEventRecord(ev.e[0], c.stream));
+    const auto host_t0 = std::chrono::steady_clock::now();
+    auto host_t1 = host_t0;
+    if (!B.noalloc) {
+        VFVM_HIP_TRY(ev.create());
+        VFVM_HIP_TRY(hipEventRecord(ev.e[0], c.stream));
+    }
 
     // permuted system and preconditioner set-up (inside the timed solve)
     if (il) {
         const int S = 4 + sy0.flow_ns;
-        hipLaunchKernelGGL((k_to_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, S, sy0.b, c.bp64.p);
-        hipLaunchKernelGGL((k_to_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, S, (const double*)x0, c.xp64.p);
+        hipLaunchKernelGGL((k_to_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, S, sy0.b, B.bp64);
+        hipLaunchKernelGGL((k_to_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, S, (const double*)x0, B.xp64);
     }
     if (ilu) {
         const IluData& I = *sy0.ilu;
@@ -1620,15 +1690,18 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
         hipLaunchKernelGGL((k_gather_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, I.perm.p, sy0.b, c.bp64.p);
         hipLaunchKernelGGL((k_gather_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, I.perm.p, (const double*)x0, c.xp64.p);
     }
-    hipLaunchKernelGGL(k_dinv, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.dslot, sy.vals, c.dinv64.p, jac ? 1 : 0);
-    const double* dinv_x = jac ? c.dinv64.p : nullptr;
+    hipLaunchKernelGGL(k_dinv, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy.dslot, sy.vals, dinv64, jac ? 1 : 0);
+    const double* dinv_x = jac ? dinv64 : nullptr;
     const void* aval = nullptr;
     const void* lu = nullptr;
     if (!mixed) {
         if (jac) {
-            VFVM_HIP_TRY(c.avals64.alloc(sy.nnz));
-            hipLaunchKernelGGL((k_scale_cols<double>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, sy.col, sy.vals, c.dinv64.p, c.avals64.p);
-            aval = c.avals64.p;
+            if (!B.noalloc) {
+                VFVM_HIP_TRY(c.avals64.alloc(sy.nnz));
+                B.avals64 = c.avals64.p;
+            }
+            hipLaunchKernelGGL((k_scale_cols<double>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, sy.col, sy.vals, dinv64, B.avals64);
+            aval = B.avals64;
         } else {
             aval = sy.vals;
         }
@@ -1641,8 +1714,9 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     } else {
         // FP32 copy of A D^-1 (Jacobi) or of A (ILU(0), none); dinv = 1 unless Jacobi
         VFVM_HIP_TRY(c.avals32.alloc(sy.nnz));
-        hipLaunchKernelGGL((k_scale_cols<float>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, sy.col, sy.vals, c.dinv64.p, c.avals32.p);
-        aval = c.avals32.p;
+        B.avals32 = c.avals32.p;
+        hipLaunchKernelGGL((k_scale_cols<float>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, sy.col, sy.vals, dinv64, B.avals32);
+        aval = B.avals32;
         if (ilu) {
             VFVM_HIP_TRY(c.ilu32.alloc(sy.nnz));
             VFVM_HIP_TRY(c.iluf32.alloc(sy.nnz));
@@ -1661,16 +1735,21 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
         pc32.bj = c.bjac.inv32.p; pc32.nb = c.bjac.nb; pc32.nv = c.g.N; pc32.il = il;
     }
     VFVM_HIP_TRY(hipGetLastError());
-    VFVM_HIP_TRY(hipEventRecord(ev.e[1], c.stream));
+    if (!B.noalloc) VFVM_HIP_TRY(hipEventRecord(ev.e[1], c.stream));
+    else {
+        VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+        host_t1 = std::chrono::steady_clock::now();
+    }
 
-    double* part_r = c.part.p + (size_t)(MAX_RESTART + 1) * w.nwav + w.nblk;
+    double* part_r = part + (size_t)(MAX_RESTART + 1) * w.nwav + w.nblk;
     int total_steps = 0, cycles = 0, refinements = 0;
     double rel = 0.0, bnorm = 0.0;
     int status = VFVM_ERR_LINEAR_DID_NOT_CONVERGE;
     const int max_ref = o.max_refine > 0 ? o.max_refine : o.max_outer;
+    const bool graphs = !B.noalloc;
     for (;;) {
         // true FP64 residual r = b - A x, beta = ||r||
-        if (int s = residual_launch(c, sy, lpr, x, c.r64.p, part_r, nblk_r)) return s;
+        if (int s = residual_launch(c, sy, lpr, x, r64, part_r, nblk_r)) return s;
         hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(BLK), 0, c.stream, nblk_r, part_r, w.S, w.L, w.ctrl, 0.0, 0);
         VFVM_HIP_TRY(hipMemcpyAsync(c.h_pinned + 8, w.S + w.L.misc(), 8 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
@@ -1686,41 +1765,41 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
         hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(BLK), 0, c.stream, nblk_r, part_r, w.S, w.L, w.ctrl, target_abs, 1);
         int steps = 0;
         if (!mixed) {
-            hipLaunchKernelGGL((k_scale<double, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V64.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<double>(c, sy, w, c.V64.p, c.w64.p, c.z64.p, (const double*)aval, pc64, reorth, &steps)) return s;
+            hipLaunchKernelGGL((k_scale<double, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, r64, B.V64, w.S, w.L.misc() + 1, w.ctrl);
+            if (int s = arnoldi_cycle<double>(c, sy, w, B.V64, w64, z64, (const double*)aval, pc64, reorth, graphs, &steps)) return s;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y)
-            hipLaunchKernelGGL((k_combine<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V64.p, w.ldv, w.S, w.L, w.ctrl, c.w64.p);
+            hipLaunchKernelGGL((k_combine<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, B.V64, w.ldv, w.S, w.L, w.ctrl, w64);
             if (lu) {
-                if (int s = ilu_apply<double>(c, sy, (const double*)lu, (const double*)c.w64.p, nullptr, c.z64.p, (const int*)nullptr)) return s;
-                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
+                if (int s = ilu_apply<double>(c, sy, (const double*)lu, (const double*)w64, nullptr, z64, (const int*)nullptr)) return s;
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z64, (const double*)nullptr, x);
             } else if (bjac) {
-                if (int s = bjac_apply<double>(c, pc64, (const double*)c.w64.p, nullptr, c.z64.p, (const int*)nullptr)) return s;
-                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
+                if (int s = bjac_apply<double>(c, pc64, (const double*)w64, nullptr, z64, (const int*)nullptr)) return s;
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z64, (const double*)nullptr, x);
             } else {
-                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.w64.p, dinv_x, x);
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, w64, dinv_x, x);
             }
         } else {
             // inner FP32 solve of A M^-1 u = r from u = 0, V0 = r / beta in FP32
-            hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.r64.p, c.V32.p, w.S, w.L.misc() + 1, w.ctrl);
-            if (int s = arnoldi_cycle<float>(c, sy, w, c.V32.p, c.w32.p, c.w32.p + n, (const float*)aval, pc32, reorth, &steps)) return s;
+            hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, r64, B.V32, w.S, w.L.misc() + 1, w.ctrl);
+            if (int s = arnoldi_cycle<float>(c, sy, w, B.V32, B.w32, B.w32 + n, (const float*)aval, pc32, reorth, graphs, &steps)) return s;
             ++refinements;
             if (w.h_ctrl[2]) { status = VFVM_ERR_LINEAR_NON_FINITE; break; }
             // x += M^-1 (V y), V y accumulated in FP64
-            hipLaunchKernelGGL((k_combine<float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.V32.p, w.ldv, w.S, w.L, w.ctrl, c.z64.p);
+            hipLaunchKernelGGL((k_combine<float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, B.V32, w.ldv, w.S, w.L, w.ctrl, z64);
             if (lu) {
-                float* t32 = c.w32.p;
-                float* z32 = c.w32.p + n;
-                hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, t32, w.S, w.L.misc() + 4, (const int*)nullptr);
+                float* t32 = B.w32;
+                float* z32 = B.w32 + n;
+                hipLaunchKernelGGL((k_scale<double, float>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z64, t32, w.S, w.L.misc() + 4, (const int*)nullptr);
                 if (int s = ilu_apply<float>(c, sy, (const float*)lu, (const float*)t32, nullptr, z32, (const int*)nullptr)) return s;
-                hipLaunchKernelGGL((k_scale<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z32, c.z64.p, w.S, w.L.misc() + 4, (const int*)nullptr);
-                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, (const double*)nullptr, x);
+                hipLaunchKernelGGL((k_scale<float, double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z32, z64, w.S, w.L.misc() + 4, (const int*)nullptr);
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z64, (const double*)nullptr, x);
             } else if (bjac) {
                 // the FP64 inverse blocks on the FP64 combination
-                if (int s = bjac_apply<double>(c, pc64, (const double*)c.z64.p, nullptr, c.w64.p, (const int*)nullptr)) return s;
-                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.w64.p, (const double*)nullptr, x);
+                if (int s = bjac_apply<double>(c, pc64, (const double*)z64, nullptr, w64, (const int*)nullptr)) return s;
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, w64, (const double*)nullptr, x);
             } else {
-                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, c.z64.p, dinv_x, x);
+                hipLaunchKernelGGL(k_xupdate, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, z64, dinv_x, x);
             }
         }
         VFVM_HIP_TRY(hipGetLastError());
@@ -1730,11 +1809,18 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     if (ilu) hipLaunchKernelGGL((k_scatter_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy0.ilu->perm.p, (const double*)x, x0);
     if (il)
         hipLaunchKernelGGL((k_from_interleaved<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, c.g.N, 4 + sy0.flow_ns, (const double*)x, x0);
-    VFVM_HIP_TRY(hipEventRecord(ev.e[2], c.stream));
-    VFVM_HIP_TRY(hipEventSynchronize(ev.e[2]));
     float ms_all = 0.f, ms_setup = 0.f;
-    VFVM_HIP_TRY(hipEventElapsedTime(&ms_all, ev.e[0], ev.e[2]));
-    VFVM_HIP_TRY(hipEventElapsedTime(&ms_setup, ev.e[0], ev.e[1]));
+    if (!B.noalloc) {
+        VFVM_HIP_TRY(hipEventRecord(ev.e[2], c.stream));
+        VFVM_HIP_TRY(hipEventSynchronize(ev.e[2]));
+        VFVM_HIP_TRY(hipEventElapsedTime(&ms_all, ev.e[0], ev.e[2]));
+        VFVM_HIP_TRY(hipEventElapsedTime(&ms_setup, ev.e[0], ev.e[1]));
+    } else {
+        VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+        const auto host_t2 = std::chrono::steady_clock::now();
+        ms_all = (float)(1e3 * std::chrono::duration<double>(host_t2 - host_t0).count());
+        ms_setup = (float)(1e3 * std::chrono::duration<double>(host_t1 - host_t0).count());
+    }
     if (int s = ktime_flush(c)) return s;
     st.iterations = total_steps;
     st.restarts = cycles;
@@ -1750,5 +1836,7 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     }
     return VFVM_OK;
 }
+
+}  // namespace
 
 }  // namespace vfvm

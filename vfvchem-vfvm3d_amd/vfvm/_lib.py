@@ -38,6 +38,7 @@ ERR_TRANSIENT_DID_NOT_CONVERGE = 47
 ERR_TRANSIENT_NON_FINITE_RESIDUAL = 48
 ERR_WRITE_CSV = 50
 ERR_WRITE_VTU = 51
+ERR_VARIABLES_NON_POSITIVE_SCALE = 60
 
 STATUS_NAMES = {v: k for k, v in globals().items() if k.startswith("ERR_") or k == "OK"}
 
@@ -167,6 +168,14 @@ SIGNATURES = [
     ("vfvm_transient_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, _D, _D, C.POINTER(DriverStats)]),
     ("vfvm_adjoint_forward", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, C.POINTER(AdjointStats)]),
     ("vfvm_adjoint_backward", _I, [_P, C.POINTER(SolveOpts), _PD, _I, C.POINTER(AdjointWrt), C.POINTER(AdjointStats)]),
+    ("vfvm_set_scldom_scale", _I, [_P, _I, _D]),
+    ("vfvm_set_vecdom_scale", _I, [_P, _I, _D]),
+    ("vfvm_set_scale_off", _I, [_P, _I]),
+    ("vfvm_linear_update_sparse", _I, [_P, _L, _L, _PL, _PL, _PL]),
+    ("vfvm_linear_build_matrix", _I, [_P, _L, _PD]),
+    ("vfvm_linear_build_matrix_device", _I, [_P, _L, _P]),
+    ("vfvm_linear_solve_matrix", _I, [_P, C.POINTER(SolveOpts), _PD, _PD, _PD, C.POINTER(SolveStats)]),
+    ("vfvm_linear_solve_matrix_device", _I, [_P, C.POINTER(SolveOpts), _P, _P, C.POINTER(SolveStats)]),
     ("vfvm_timing_enable", _I, [_P, _I]),
     ("vfvm_timing_get", _I, [_P, _I, _PL, _PD, _PD]),
     ("vfvm_timing_reset", _I, [_P]),

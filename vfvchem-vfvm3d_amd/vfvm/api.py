@@ -59,6 +59,23 @@ class Variables:
         self.outputs = []       # (kind, field id, file_path) of fields written by the drivers
         self.t_offset = 0.0     # time of the driver call's start (transient output in chunks)
 
+    # convergence scaling (vars.rs:28-31, 203-213): reference scales of unknown fields (lower
+    # bounds of their scales in the Picard residual and the adaptive error norm) and scale_off
+    def set_scldom_scale(self, scldom_id: int, scale: float) -> None:
+        self.ctx.set_scldom_scale(scldom_id, scale)
+
+    def set_vecdom_scale(self, vecdom_id: int, scale: float) -> None:
+        self.ctx.set_vecdom_scale(vecdom_id, scale)
+
+    @property
+    def scale_off(self) -> bool:
+        return getattr(self, "_scale_off", False)
+
+    @scale_off.setter
+    def scale_off(self, off: bool) -> None:
+        self._scale_off = bool(off)
+        self.ctx.set_scale_off(self._scale_off)
+
     @staticmethod
     def new_lattice(n: int, seed: int = 0, jitter: float = 0.3, device: int = 0) -> "Variables":
         """Jittered unit-cube lattice (replaces Variables::new_bounds, vars.rs:45, for the lattice meshes);
@@ -248,6 +265,27 @@ class LinearGmres:
         if max_outer == 0:
             raise VFVChemError(L.ERR_LINEAR_NON_POSITIVE_MAX_OUTER, "LinearGmresNonPositiveMaxOuter")
         return LinearGmres(tol=tol, restart=restart, max_outer=max_outer, **kw)
+
+    # LinearBase (linear_base.rs:5-10) for a caller's own triplet list: the device GMRES on a
+    # private context of device 0.  Triplets are (rows, cols, vals) arrays.
+    def _context(self) -> Context:
+        if getattr(self, "_ctx", None) is None:
+            self._ctx = Context(0)
+        return self._ctx
+
+    def update_sparse(self, a_triplet, mat_size: int) -> None:
+        rows, cols, _ = a_triplet
+        self._context().linear_update_sparse(mat_size, rows, cols)
+
+    def build_matrix(self, a_triplet) -> "LinearGmres":
+        self._context().linear_build_matrix(a_triplet[2])
+        return self
+
+    def solve_matrix(self, a_mat: "LinearGmres", b_vec, x_init, mat_size: int) -> np.ndarray:
+        if a_mat is not self or len(b_vec) != mat_size:
+            raise VFVChemError(L.ERR_INVALID_ARGUMENT, "solve_matrix: matrix of another solver or size mismatch")
+        x, self.stats = self._context().linear_solve_matrix(b_vec, x_init, **self.opts)
+        return x
 
 
 class _UpdateHook:

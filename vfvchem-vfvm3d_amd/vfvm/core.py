@@ -314,7 +314,8 @@ class Context:
         return n.value, nnz.value
 
     def _size(self):
-        return getattr(self, "n", self.mesh.num_tot)
+        n = getattr(self, "n", None)
+        return n if n is not None else self.mesh.num_tot
 
     def pattern_get(self):
         rp = np.empty(self._size() + 1, np.int32)
@@ -345,6 +346,44 @@ class Context:
         o = solve_opts(**kw)
         check(lib.vfvm_solve_device(self._h, C.byref(o), C.c_void_p(x_dev_ptr), C.byref(st)))
         return st
+
+    # LinearBase for caller-assembled triplets (linear_base.rs:5-10)
+    def linear_update_sparse(self, mat_size: int, rows, cols) -> int:
+        """update_sparse: the pattern of the triplets (rows[k], cols[k]); returns nnz."""
+        rows = np.ascontiguousarray(rows, np.int64)
+        cols = np.ascontiguousarray(cols, np.int64)
+        if rows.shape != cols.shape:
+            raise ValueError("rows and cols differ in length")
+        nnz = C.c_int64()
+        check(lib.vfvm_linear_update_sparse(self._h, int(mat_size), int(rows.size), ptr(rows, C.c_int64), ptr(cols, C.c_int64),
+                                            C.byref(nnz)))
+        self.system_info()
+        return self.nnz
+
+    def linear_build_matrix(self, vals) -> None:
+        """build_matrix: values of the triplets in the update_sparse order, duplicates summed."""
+        vals = np.ascontiguousarray(vals, np.float64)
+        check(lib.vfvm_linear_build_matrix(self._h, int(vals.size), ptr(vals)))
+
+    def linear_solve_matrix(self, b, x_init=None, **kw):
+        """solve_matrix: x of A x = b from x_init (zero if None) with the GMRES options kw."""
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.empty_like(b)
+        xi = None if x_init is None else np.ascontiguousarray(x_init, np.float64)
+        st = L.SolveStats()
+        o = solve_opts(**kw)
+        check(lib.vfvm_linear_solve_matrix(self._h, C.byref(o), ptr(b), None if xi is None else ptr(xi), ptr(x), C.byref(st)))
+        return x, st
+
+    # convergence scaling (vars.rs:28-31,203-213)
+    def set_scldom_scale(self, fid: int, scale: float) -> None:
+        check(lib.vfvm_set_scldom_scale(self._h, int(fid), float(scale)))
+
+    def set_vecdom_scale(self, fid: int, scale: float) -> None:
+        check(lib.vfvm_set_vecdom_scale(self._h, int(fid), float(scale)))
+
+    def set_scale_off(self, off: bool) -> None:
+        check(lib.vfvm_set_scale_off(self._h, 1 if off else 0))
 
     def residual(self, x):
         x = np.ascontiguousarray(x, np.float64)
