@@ -176,6 +176,7 @@ def timed_block_solves(vfvm, ctx, ndof, tol, restart, inner_tol, steps, label, m
     x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
     xp = x.data_ptr()
     out = {}
+    sols = {}
 
     def kernel_pass(kw):
         ctx.timing_enable(-1)
@@ -218,6 +219,10 @@ def timed_block_solves(vfvm, ctx, ndof, tol, restart, inner_tol, steps, label, m
         progress(f"{label} {var}: {ms:.1f} ms per solve ({st.iterations} steps)")
         out[var] = dict(ms_per_step=ms, dof_per_s=ndof / (ms * 1e-3), iters=st.iterations, refinements=st.refinements,
                         rel_residual=st.rel_residual, setup_s=st.setup_seconds, kernels=kern)
+        sols[var] = x.clone()
+    # error of the refined mixed-precision result against the FP64 solve of the same system
+    out["mixed"]["rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(sols["mixed"] - sols["fp64"]).item() /
+                                           torch.linalg.vector_norm(sols["fp64"]).item())
     return out
 
 
@@ -388,18 +393,20 @@ def run_flow(vfvm, args):
     t_setup = time.time() - t0
     N = mesh.num_tot
     ndof = 4 * N
-    lin = dict(tol=1e-8, restart=args.flow_restart, max_outer=400, precision=vfvm.FP32_REFINED,
+    lin = dict(tol=args.tol, restart=args.flow_restart, max_outer=400, precision=vfvm.FP32_REFINED,
                precond=vfvm.PRECOND_BJACOBI, inner_tol=1e-4, ortho=1)
-    # a few damped Picard iterations (tol unreachable: the driver stops at max_iter) so the
-    # timed system carries the advection of a developed flow
+    # SteadyBase::solve (steady_base.rs:74-117) to steady state: damped Picard (0.8) until the
+    # scaled residual is below 1e-6, each linearised system solved to the metric's tolerance
     tp = time.time()
     picard = None
+    pst = None
     try:
-        ctx.steady_solve(ops, args.flow_picard, 1e-300, 0.8, lin=lin)
+        pst = ctx.steady_solve(ops, args.flow_picard, 1e-6, 0.8, lin=lin)
     except vfvm.VFVChemError as e:
         picard = str(e)
     t_picard = time.time() - tp
-    progress(f"lid cavity: {args.flow_picard} Picard iterations in {t_picard:.1f} s")
+    progress(f"lid cavity: Picard {'converged' if pst is not None else 'stopped'} in {t_picard:.1f} s "
+             f"({pst.picard_iterations if pst is not None else '?'} iterations)")
     _, nnz = ctx.system_info()
     ctx.timing_enable(-1)
     ctx.timing_reset()
@@ -411,26 +418,47 @@ def run_flow(vfvm, args):
     ctx.close()
     return {
         "workload": f"steady lid-driven cavity Re=100 (rho=1, mu=0.01, lid (1,0,0)), {args.flow_n}^3 jittered lattice "
-                    f"({N} vertices, {ndof} DOF, {nnz} nnz), system after {args.flow_picard} damped Picard iterations, "
-                    f"GMRES({args.flow_restart}) + 4x4 block Jacobi, tol {args.tol}",
+                    f"({N} vertices, {ndof} DOF, {nnz} nnz), Picard to steady state (damping 0.8, scaled residual 1e-6), "
+                    f"then the converged system's linear solve timed, GMRES({args.flow_restart}) + 4x4 block Jacobi, tol {args.tol}",
         "dof": ndof, "nnz": nnz, "fp64": out["fp64"], "mixed": out["mixed"],
         "mixed_over_fp64": out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"],
-        "assembly_wall_ms": 1e3 * t_asm, "picard_s": t_picard, "picard_note": picard, "mesh_dual_gc_s": t_setup,
+        "assembly_wall_ms": 1e3 * t_asm, "mesh_dual_gc_s": t_setup,
+        "steady_state": {"converged": pst is not None, "picard_iterations": pst.picard_iterations if pst else None,
+                         "linear_iterations": pst.linear_iterations if pst else None,
+                         "scaled_residual": pst.residual if pst else None, "time_to_steady_s": t_picard,
+                         "assembly_s": pst.time_assembly if pst else None, "solve_s": pst.time_solve if pst else None,
+                         "note": picard},
     }
+
+
+def kernel_source_hash():
+    """sha256 of the Krylov kernel source whose launches the traffic figure describes."""
+    import hashlib
+    with open(os.path.join(ROOT, "vfvchem-vfvm3d_amd", "csrc", "krylov.hip"), "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()
 
 
 def measured_traffic(kernel, variant):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary of this workload (scaled by a copy kernel of known size), or None."""
+    summary of this workload (scaled by a copy kernel of known size), or None; only a summary
+    measured on the current kernel source (its recorded sha256 equal to krylov.hip's) counts.
+    Returns (bytes or None, note)."""
     path = os.path.join(ROOT, "profiles", "ilu_sweep_traffic_convdiff_100cubed.md")
     if variant != "mixed" or not os.path.exists(path):
-        return None
+        return None, "no counter summary for this variant"
+    value, src = None, None
     with open(path) as fh:
         for line in fh:
             parts = line.split()
             if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                return float(parts[2])
-    return None
+                value = float(parts[2])
+            if len(parts) == 3 and parts[0] == "source_sha256" and parts[1] == "krylov.hip":
+                src = parts[2]
+    if value is None:
+        return None, "the counter summary has no figure for this kernel"
+    if src != kernel_source_hash():
+        return None, "the counter summary was measured on another kernel source"
+    return value, "rocprofv3 FETCH_SIZE + WRITE_SIZE of the current kernel source, " + os.path.relpath(path, ROOT)
 
 
 def cpu_model():
@@ -460,7 +488,7 @@ def main():
     ap.add_argument("--adjoint-every", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--flow-n", type=int, default=80, help="lattice points per axis of the lid-driven cavity")
-    ap.add_argument("--flow-picard", type=int, default=3, help="Picard iterations before the timed flow solves")
+    ap.add_argument("--flow-picard", type=int, default=100, help="Picard iterations allowed to reach the steady state")
     ap.add_argument("--flow-restart", type=int, default=100)
     ap.add_argument("--flow-steps", type=int, default=2)
     ap.add_argument("--no-flow", action="store_true")
@@ -520,6 +548,7 @@ def main():
     xp = x.data_ptr()
 
     results = {}
+    solutions = {}
     variants = [v for v in args.variants.split(",") if v]
     for var in variants:
         prec = vfvm.FP64 if var == "fp64" else vfvm.FP32_REFINED
@@ -541,7 +570,11 @@ def main():
             stats.append((st.iterations, st.refinements, st.rel_residual, st.seconds, st.setup_seconds))
         torch.cuda.synchronize()
         t2 = time.perf_counter()
+        # error of the result against the exact solution c = 1 (the inlet value carried by a
+        # divergence-free flow with zero diffusive flux on the other walls)
         err = float((x - 1.0).abs().max().item())
+        err_l2 = float(torch.linalg.vector_norm(x - 1.0).item() / N ** 0.5)
+        solutions[var] = x.clone()
         # kernel pass: one more solve with HIP start/stop events on every Krylov kernel launch
         ctx.timing_enable(-1)
         ctx.timing_reset()
@@ -559,7 +592,11 @@ def main():
         progress(f"convection-diffusion {var}: {1e3 * (t2 - t1) / args.steps:.1f} ms per solve")
         results[var] = dict(ms_per_step=1e3 * (t2 - t1) / args.steps, iters=stats[-1][0], refinements=stats[-1][1],
                             rel_residual=stats[-1][2], device_s=stats[-1][3], setup_s=stats[-1][4], kernels=kern,
-                            max_abs_err_vs_exact=err)
+                            max_abs_err_vs_exact=err, rel_l2_err_vs_exact=err_l2)
+    if "fp64" in solutions and "mixed" in solutions:
+        a, b = solutions["fp64"], solutions["mixed"]
+        results["mixed"]["rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(b - a).item() / torch.linalg.vector_norm(a).item())
+    solutions.clear()
 
     head = "mixed" if "mixed" in results else variants[0]
     r = results[head]
@@ -615,6 +652,7 @@ def main():
     flow = side_line("lid-driven cavity", args.no_flow, run_flow)
     hmf = side_line("heat-mass-flow", args.no_hmf, run_heat_mass_flow)
 
+    traffic = measured_traffic(dom_name, head)
     line = {
         "metric": "solved DOF/s at 1e-10 rel residual (FP32-inner/FP64-refined GMRES; FP64 GMRES beside it); "
                   "adjoint/forward time ratio in 'adjoint'",
@@ -640,7 +678,7 @@ def main():
         "mesh_and_dual_s": t_mesh,
         "mesh_note": "jittered lattice and exact Delaunay on the host, Voronoi dual on the GPU (bitwise equal to the host build)",
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_name, head),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic[0], "traffic_note": traffic[1],
                      "algorithmic_bytes_per_launch": dom["bytes_per_launch"]},
         "cpu_baseline": cpu,
         "adjoint": adjoint,

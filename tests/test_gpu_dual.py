@@ -97,3 +97,16 @@ def test_device_grad_coeff_large_and_speed():
     for name in ("gc_ptr", "gc_vid", "gc_c"):
         assert np.array_equal(bits(h.get(name)), bits(m.get(name))), f"{name} differs"
     print(f"grad_coeff {n}^3: host {t1 - t0:.2f} s, device {t2 - t1:.2f} s", flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,seed", [(5, 0), (8, 3)])
+def test_device_dual_equals_oracle(n, seed):
+    """The device-built dual against the oracle's restatement of geom_tet.rs:58-293 directly
+    (not through the host build): every array bit for bit."""
+    import vfvm_oracle as O
+    print(f"device dual vs oracle: lattice n={n} seed={seed}", flush=True)
+    m = vfvm.Mesh.lattice(n, seed, 0.3, device=0)
+    g = O.dual(m.get("pos"), m.get("tets"), m.num_pts)
+    for name in ARRAYS:
+        assert np.array_equal(bits(m.get(name)), bits(np.asarray(g[name], m.get(name).dtype))), name
