@@ -346,6 +346,16 @@ void vfvm_solve_opts_default(vfvm_solve_opts* opts);
 int vfvm_solve(vfvm_ctx* ctx, const vfvm_solve_opts* opts, double* x, vfvm_solve_stats* stats);
 /* Same, on a device-resident x (hipMalloc'd, mat_size doubles) on the context stream. */
 int vfvm_solve_device(vfvm_ctx* ctx, const vfvm_solve_opts* opts, double* x_dev, vfvm_solve_stats* stats);
+/* No reference counterpart.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): vfvm_solve_device with
+ * no allocation of any kind.  The Krylov workspace lives in a fixed array of 128 MiB per GPU
+ * (held from the library's load until the process ends); a system needs about
+ * 8 (n (restart + 5) + nnz) bytes of it (nnz only with Jacobi) plus a few MB of partial sums,
+ * and larger ones are rejected with VFVM_ERR_INVALID_ARGUMENT.  FP64 with the Jacobi
+ * preconditioner or none; launches are eager (no graphs), stats.seconds is host wall time.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (two threads,
+ * two contexts) race and must be serialised by the caller. */
+int vfvm_solve_device_noalloc(vfvm_ctx* ctx, const vfvm_solve_opts* opts, double* x_dev, vfvm_solve_stats* stats);
 /* r = b - A x in FP64 for a host x; returns ||r||_2 / ||b||_2. */
 int vfvm_residual(vfvm_ctx* ctx, const double* x, double* rel_residual);
 /* z = M^-1 r for the assembled matrix with the preconditioner of opts (precond, and

@@ -250,6 +250,8 @@ extern "C" int vfvm_create(int32_t device, vfvm_ctx** out) {
     // pinned readback buffer of the solver's control word and norms
     e = hipHostMalloc((void**)&h->c.h_pinned, 4096, hipHostMallocDefault);
     if (e != hipSuccess) { vfvm_destroy(h); return set_error(VFVM_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e)); }
+    // device address of this GPU's no-allocation scratch array (once per device)
+    if (int st = noalloc_setup(device)) { vfvm_destroy(h); return st; }
     *out = h;
     return clear_error();
 }

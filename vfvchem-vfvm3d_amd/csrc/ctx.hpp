@@ -324,6 +324,14 @@ int launch_assemble_block(Ctx& c, double dt, const int32_t* mrow, const int32_t*
 SysView primal_view(Ctx& c);
 SysView transpose_view(Ctx& c);
 int gmres_solve(Ctx& c, const SysView& s, const vfvm_solve_opts& o, double* x_dev, vfvm_solve_stats& st);
+// The same solve (FP64, Jacobi or no preconditioner) with its whole workspace carved from the
+// caller's device region of `entries` doubles: no allocation, eager launches, host-clock timing.
+// gmres_noalloc_entries gives the doubles a system needs.
+int gmres_solve_noalloc(Ctx& c, const SysView& s, const vfvm_solve_opts& o, double* x_dev, vfvm_solve_stats& st,
+                        double* scratch, int64_t entries);
+int64_t gmres_noalloc_entries(int64_t n, int64_t nnz, int m, bool jacobi, bool interleaved);
+// Looks up the device address of the current device's no-allocation scratch array (api_noalloc.hip)
+int noalloc_setup(int device);
 // host-side preconditioner ordering of a pattern (ILU(0) colouring), done once per pattern
 int solver_setup(Ctx& c, const SysView& s, const vfvm_solve_opts& o);
 // z = M^-1 r for the assembled matrix (host arrays); the ILU(0) row order

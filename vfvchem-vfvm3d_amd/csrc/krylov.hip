@@ -1595,6 +1595,51 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     return gmres_run(c, sy0, o, x0, st, B);
 }
 
+// ---------------------------------------------------------------- no-allocation solve
+int64_t gmres_noalloc_entries(int64_t n, int64_t nnz, int m, bool jacobi, bool interleaved) {
+    const GmresSizes z = gmres_sizes(n, nnz, m);
+    auto up = [](int64_t k) { return (k + 31) / 32 * 32; };  // 256-byte aligned pieces
+    int64_t e = up(z.state) + up(z.part) + 4 * up(n) + up(z.ldv * (m + 1));
+    if (jacobi) e += up(nnz);
+    if (interleaved) e += 2 * up(n);
+    return e;
+}
+
+int gmres_solve_noalloc(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, vfvm_solve_stats& st, double* scratch,
+                        int64_t scratch_entries) {
+    std::memset(&st, 0, sizeof(st));
+    if (o.restart <= 0 || o.restart > MAX_RESTART) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_RESTART, "GMRES: restart must be in [1, 256]");
+    if (o.precision != VFVM_FP64 || (o.precond != VFVM_PRECOND_NONE && o.precond != VFVM_PRECOND_JACOBI))
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "no-allocation solve: FP64 with the Jacobi preconditioner or none only");
+    const int64_t n = sy0.n;
+    const bool jac = o.precond == VFVM_PRECOND_JACOBI;
+    const bool il = sy0.flow_layout != 0 && o.precond == VFVM_PRECOND_NONE;
+    const int64_t need = gmres_noalloc_entries(n, sy0.nnz, o.restart, jac, il);
+    if (need > scratch_entries)
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "no-allocation solve: the system needs " + std::to_string(need * 8) +
+                                                        " bytes of workspace, the scratch array holds " + std::to_string(scratch_entries * 8) +
+                                                        " (n (restart + 5) + nnz doubles and a few MB of partials)");
+    const GmresSizes z = gmres_sizes(n, sy0.nnz, o.restart);
+    auto up = [](int64_t k) { return (k + 31) / 32 * 32; };
+    double* p = scratch;
+    auto take = [&](int64_t k) { double* q = p; p += up(k); return q; };
+    GmresBufs B;
+    B.noalloc = true;
+    B.S = take(z.state);
+    B.ctrl = reinterpret_cast<int*>(B.S + (z.state - 64));
+    B.part = take(z.part);
+    B.r64 = take(n); B.dinv64 = take(n); B.w64 = take(n); B.z64 = take(n);
+    B.V64 = take(z.ldv * (o.restart + 1));
+    if (jac) B.avals64 = take(sy0.nnz);
+    if (il) { B.bp64 = take(n); B.xp64 = take(n); }
+    // no events: kernel timing is off for the call
+    const uint32_t mask = c.timing_mask;
+    c.timing_mask = 0;
+    int r = gmres_run(c, sy0, o, x0, st, B);
+    c.timing_mask = mask;
+    return r;
+}
+
 namespace {
 
 int gmres_run(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, vfvm_solve_stats& st, GmresBufs& B) {

@@ -176,6 +176,7 @@ SIGNATURES = [
     ("vfvm_linear_build_matrix_device", _I, [_P, _L, _P]),
     ("vfvm_linear_solve_matrix", _I, [_P, C.POINTER(SolveOpts), _PD, _PD, _PD, C.POINTER(SolveStats)]),
     ("vfvm_linear_solve_matrix_device", _I, [_P, C.POINTER(SolveOpts), _P, _P, C.POINTER(SolveStats)]),
+    ("vfvm_solve_device_noalloc", _I, [_P, C.POINTER(SolveOpts), _P, C.POINTER(SolveStats)]),
     ("vfvm_timing_enable", _I, [_P, _I]),
     ("vfvm_timing_get", _I, [_P, _I, _PL, _PD, _PD]),
     ("vfvm_timing_reset", _I, [_P]),

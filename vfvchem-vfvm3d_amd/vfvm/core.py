@@ -385,6 +385,14 @@ class Context:
     def set_scale_off(self, off: bool) -> None:
         check(lib.vfvm_set_scale_off(self._h, 1 if off else 0))
 
+    def solve_device_noalloc(self, x_dev_ptr: int, **kw):
+        """vfvm_solve_device_noalloc: the same solve with no device allocation (FP64, Jacobi or
+        no preconditioner; workspace in the library's fixed per-GPU scratch array)."""
+        st = L.SolveStats()
+        o = solve_opts(**kw)
+        check(lib.vfvm_solve_device_noalloc(self._h, C.byref(o), C.c_void_p(x_dev_ptr), C.byref(st)))
+        return st
+
     def residual(self, x):
         x = np.ascontiguousarray(x, np.float64)
         r = C.c_double()
