@@ -519,6 +519,73 @@ typedef struct vfvm_adjoint_wrt {
 int vfvm_adjoint_backward(vfvm_ctx* ctx, const vfvm_solve_opts* opts, const double* seed, int32_t nwrt,
                           const vfvm_adjoint_wrt* wrt, vfvm_adjoint_stats* stats);
 
+/* ---- affine function fields ----
+ * Variables::add_vecdom_func (src/base/vars.rs:153) for a closure affine in one scalar field:
+ * vec := f0 + df * scl per component (f0, df: 3N SoA host arrays), evaluated on the device
+ * where the drivers evaluate function fields (_update_func, before every assembly; before
+ * the host update hook when both are set) and once at registration.  The vector field must
+ * be a FUNCTION field.  f0 = df = NULL removes the closure.  The Boussinesq body force of a
+ * natural-convection list is fce = (0, 0, gamma (T - T_ref)): f0 = (0, 0, -gamma T_ref),
+ * df = (0, 0, gamma). */
+int vfvm_vecdom_func_affine(vfvm_ctx* ctx, int32_t vec_id, int32_t scl_id, const double* f0, const double* df);
+
+/* ---- adjoint of the coupled flow system (no reference counterpart) ----
+ * Forward map: the Picard-converged coupled flow list (SteadyFlow / TransientFlow /
+ * TransientHeat(Mass)Flow operator lists, as vfvm_steady_solve / vfvm_transient_solve run
+ * them), i.e. the states x^n = [u, v, w, p, s_1, s_2] with
+ *     R(x^n; x^(n-1), theta) = A(c(x^n)) x^n - b(c(x^n), x^(n-1), theta) = 0,  n = 1..num_ts
+ * (num_ts = 0: one steady state), where c(x) are the state-dependent coefficients: the
+ * Rhie-Chow face mass flux, rc = V / a_P, the velocity gradients and the body force (an
+ * affine closure of vfvm_vecdom_func_affine; a body force that a host hook sets is taken as
+ * a fixed field).  Objective J = sum_n w_n seed . x^n (steady: seed . x).
+ * Backward: J(x)^T l^n = w_n seed - D l^(n+1) with the full Jacobian J = A + E (E: the
+ * derivative through c, applied by vector-Jacobian kernels of the flux, rc, the pressure
+ * stabiliser, the advection coefficients, the transpose / dilatatory / source right-hand
+ * sides and the least-squares gradients) and D = dR^(n+1)/dx^n (the time operators'
+ * rho V / dt, w V / dt), solved by GCR on J^T preconditioned with the transposed GMRES on
+ * A^T; each checkpoint segment is replayed to recover x^n.  Coefficient fields must not
+ * depend on the unknowns (constant or function fields); a PRODUCTION operator with a
+ * non-zero coefficient is rejected. */
+typedef struct vfvm_flow_adjoint_opts {
+    vfvm_solve_opts inner;  /* transposed solves on A^T (the outer preconditioner)       */
+    double tol;             /* outer: ||s - J^T l|| <= tol ||s|| per state               */
+    int32_t max_iter;       /* outer iterations per state                                */
+    int32_t restart;        /* outer directions kept before a restart (1..64)            */
+} vfvm_flow_adjoint_opts;
+typedef struct vfvm_flow_adjoint_stats {
+    vfvm_adjoint_stats base;          /* steps, checkpoints, Arnoldi steps and times as the
+                                         linear adjoint reports them (adjoint_iterations:
+                                         the inner transposed solves)                     */
+    int32_t picard_iterations;        /* forward pass                                     */
+    int32_t recompute_picard_iterations;
+    int32_t outer_iterations;         /* GCR iterations over every state                  */
+    int32_t reserved;
+    double objective;                 /* J of the forward pass                            */
+    double max_outer_residual;        /* largest final ||s - J^T l|| / ||s|| of the states */
+} vfvm_flow_adjoint_stats;
+/* Runs the driver of the flow list from the fields' current values (as vfvm_steady_solve
+ * for num_ts = 0, vfvm_transient_solve otherwise: Picard max_iter / tol / damp per step),
+ * keeps x^0, x^K, x^2K, ... (K = checkpoint_every), and evaluates J: seed (n entries, may be
+ * NULL: J = 0), weights (num_ts entries, w_n of step n; NULL: all 1; ignored when steady). */
+int vfvm_flow_adjoint_forward(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, const vfvm_solve_opts* opts, double dt,
+                              int32_t num_ts, int32_t max_iter, double tol, double damp, int32_t checkpoint_every,
+                              const double* seed, const double* weights, vfvm_flow_adjoint_stats* stats);
+/* Backward pass of the last vfvm_flow_adjoint_forward with the same seed / weights.  wrt:
+ * VFVM_WRT_SCLBND (Dirichlet values of a coupled scalar's boundary field) and
+ * VFVM_WRT_VECBND (velocity Dirichlet values, 3 x nodes SoA).  The fields hold the final
+ * state again on return. */
+enum { VFVM_WRT_VECBND = 3 };
+int vfvm_flow_adjoint_backward(vfvm_ctx* ctx, const vfvm_flow_adjoint_opts* opts, const double* seed, const double* weights,
+                               int32_t nwrt, const vfvm_adjoint_wrt* wrt, vfvm_flow_adjoint_stats* stats);
+/* The residual r = A(c(x)) x - b(c(x), x_prev) of the flow list's system at the state x
+ * (host arrays of n entries; x_prev: previous state of a transient list, NULL when dt = 0),
+ * and y = J(x)^T l (J = A + E as above).  The pattern must have been built for the list; the
+ * fields hold x on return. */
+int vfvm_flow_residual_eval(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, double dt, const double* x, const double* x_prev,
+                            double* r);
+int vfvm_flow_jt_apply(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nops, double dt, const double* x, const double* x_prev,
+                       const double* lam, double* y);
+
 /* ---- measurement ---- */
 /* Per-launch HIP-event timing on the context stream: bit k of `mask` times kernel class k
  * (-1: all classes, 0: off).  Each timed launch is bracketed by two events. */

@@ -41,6 +41,8 @@ Pinning (how each function is checked independently of itself):
   heatflow_solve()  conduction limit T = C = 1 - x exactly; decoupled limit equals
                     picard_flow and a direct scalar solve; buoyancy sign.
   transient_adaptive()   fixed mode, output-time landing, error falls with rtol.
+  coupled_residual / flow_jt   J^T of the Picard fixed point against central finite
+                    differences of coupled_residual in random directions.
 """
 from __future__ import annotations
 
@@ -1380,6 +1382,200 @@ def heatflow_solve(g, gc, vel0, pres0, T0, den, visc, fce_of, bnd_vel, bnd_nid, 
             raise RuntimeError("DidNotConverge")
         vel_prev, T_prev = vel.copy(), T.copy()
     return vel, pres, T, total
+
+
+# --------------------------------------------------------------------------- flow adjoint
+# The Jacobian of the Picard fixed point R(x) = A(c(x)) x - b(c(x), x_prev) of the coupled
+# flow system, J = A + E with E the derivative through the state-dependent coefficients c
+# (the Rhie-Chow flux, rc, the velocity and body-force gradients, the body force).  There is
+# no reference counterpart; flow_jt() is pinned by central finite differences of
+# coupled_residual() in random directions (tests/test_cpu_flow_adjoint_oracle.py): any
+# dropped, mis-signed or mis-indexed term of E fails that identity.
+
+def coupled_residual(g, gc, x, x_prev, den, visc, fce_of, bnd_vel, bnd_nid, pref, heat_rows, dt=0.0, den_const=True,
+                     visc_const=True):
+    """R(x) = A(x) x - b(x) of the system heatflow_solve iterates (its system() at the state
+    x = [vel (3N), pres (N), S], previous state x_prev for dt > 0).  Returns R, A (CSR), the
+    flow fields f."""
+    import scipy.sparse as sp
+    N = g["N"]
+    x = np.asarray(x, np.float64)
+    vel, pres, S = x[:3 * N].reshape(3, N), x[3 * N:4 * N], x[4 * N:]
+    f = flow_fields(g, gc, vel, pres, den, visc, fce_of(S), den_const=den_const, visc_const=visc_const, fce_const=False)
+    transient = dt > 0.0
+    vp = np.asarray(x_prev, np.float64)[:3 * N].reshape(3, N) if transient else None
+    r, c, v, b = assemble_flow(g, gc, f, bnd_vel, bnd_nid, pref, dt, vp, den if transient else None)
+    Sp = np.asarray(x_prev, np.float64)[4 * N:] if transient else S
+    rh, ch, vh, bh = heat_rows(S, Sp, f)
+    K = S.size // N
+    rows = np.concatenate([np.asarray(r, np.int64), 4 * N + np.asarray(rh, np.int64)])
+    cols = np.concatenate([np.asarray(c, np.int64), 4 * N + np.asarray(ch, np.int64)])
+    vals = np.concatenate([np.asarray(v, np.float64), np.asarray(vh, np.float64)])
+    n = (4 + K) * N
+    rp, col, val = coo_to_csr(rows, cols, vals, n)
+    A = sp.csr_matrix((val, col, rp), shape=(n, n))
+    return A @ x - np.concatenate([b, bh]), A, f
+
+
+def scalar_adv_terms(g, ops, scl, flow_den, bnd_nid):
+    """Per advection operator of a scalar list: (directed edges of its rows, kappa, cond) with
+    the operator's coefficient phi(kappa F) = cond (P(kappa F / cond) - 1) + max(-kappa F, 0)
+    (opscl_dom_advunity.rs:70-112: kappa = 1; opscl_dom_adv.rs:86-118: kappa = m_f / rho_f)."""
+    N = g["N"]
+    rowsE, ngh = g["rows"].astype(np.int64), g["ngh"].astype(np.int64)
+    out = []
+    for op in ops:
+        if op.kind not in (Op.SCL_DOM_ADVECTION_UNITY, Op.SCL_BND_ADVECTION_UNITY, Op.SCL_DOM_ADVECTION, Op.SCL_BND_ADVECTION):
+            continue
+        m = np.zeros(N, bool)
+        m[_op_rows(g, op, bnd_nid)] = True
+        e = np.nonzero(m[rowsE])[0]
+        v_, n_ = rowsE[e], ngh[e]
+        sk = g["skew"][:, e]
+        if op.kind in (Op.SCL_DOM_ADVECTION_UNITY, Op.SCL_BND_ADVECTION_UNITY):
+            df = _diff_face(*scl[op.a], v_, n_, sk) if op.a >= 0 else np.zeros(e.size)
+            kap = np.ones(e.size)
+        else:
+            df = _diff_face(*scl[op.b], v_, n_, sk)
+            W, Wg = scl[op.a]
+            Rr, Rg = flow_den[op.c]
+            wgt_f = _face_mean(W, Wg, v_, n_, *sk)
+            den_f = _face_mean(Rr, Rg, v_, n_, *sk)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                kap = np.where(den_f != 0.0, wgt_f / den_f, 0.0)
+        out.append((e, kap, df * g["area"][e] / g["dist"][e]))
+    return out
+
+
+def _dphi_adv(F, cond):
+    """d/dF of cond (P(F / cond) - 1) + max(-F, 0), P(x) = max(0, 1 - 0.1|x|)^5:
+    P'(Pe) - H(-F) with P'(Pe) = -0.5 a^4 sign(Pe), Pe = F / cond (cond < 0 on dual faces of
+    negative area); -0.5 at F = 0 (the derivative for cond > 0, the mean of the one-sided
+    ones for cond < 0); max(-F, 0) alone when cond = 0."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        pe = np.where(cond != 0.0, F / cond, 0.0)
+    a4 = np.maximum(1.0 - 0.1 * np.abs(pe), 0.0) ** 4
+    d = -0.5 * a4 * np.sign(pe) - (F < 0.0)
+    return np.where(cond != 0.0, np.where(F == 0.0, -0.5, d), np.where(F < 0.0, -1.0, 0.0))
+
+
+def _dh_ap(F, cond):
+    """d/dF of cond P(F / cond) + max(F, 0), the face term of a_P (flow_helper.rs:3-70):
+    P'(Pe) + H(F), 0.5 at F = 0."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        pe = np.where(cond != 0.0, F / cond, 0.0)
+    a4 = np.maximum(1.0 - 0.1 * np.abs(pe), 0.0) ** 4
+    d = -0.5 * a4 * np.sign(pe) + (F > 0.0)
+    return np.where(cond != 0.0, np.where(F == 0.0, 0.5, d), np.where(F > 0.0, 1.0, 0.0))
+
+
+def flow_jt(g, gc, f, x, lam, A, pref_row, adv_blocks, body_blk=-1, dfce=None):
+    """y = J(x)^T lam = A^T lam + E^T lam for the coupled system of coupled_residual (A and f
+    from it).  adv_blocks[k]: scalar_adv_terms of scalar block k; body_blk, dfce (3, N): the
+    block the body force depends on pointwise, fce = f0 + dfce s.  E^T is written as scatters
+    over the directed edges e = (v, n), following the chain
+        F_e = rho_f (u_f . n + vel_rc) A  (flow_helper.rs:114-171),
+        vel_rc = -(rc_v + rc_n)/2 ((p_n - p_v)/d - fce_e) + n . (mom_v + mom_n)/2,
+        mom = rc (grad p - fce_c), rc = V / a_P, a_P = sum_e h(Fc_e)  (flow_helper.rs:3-112),
+    through the rows that read them: momentum advection (opvec_dom_adv.rs:72-108) and the
+    transpose / dilatatory / source right-hand sides (opvec_dom_dtrn.rs, opvec_dom_ddil.rs,
+    opvec_dom_src.rs), the scalar advection rows, and the continuity rows, which equal
+    -sum_e F_e (their velocity and pressure columns at fixed rc are in A)."""
+    N, NP = g["N"], g["num_pts"]
+    x = np.asarray(x, np.float64)
+    lam = np.asarray(lam, np.float64)
+    rows, ngh = g["rows"].astype(np.int64), g["ngh"].astype(np.int64)
+    Ae, d, nv, sk, gw = g["area"], g["dist"], g["norm"], g["skew"], g["gw"]
+    vol, cs = g["vol"], g["cskew"]
+    vel, pres, rc, F = f["vel"], f["pres"], f["rc"], f["flux"]
+    fce, fceg = f["fce"], f["fceg"]
+    lu = lam[:3 * N].reshape(3, N)
+    lp = lam[3 * N:4 * N]
+    K = (x.size - 4 * N) // N
+    S = [x[(4 + k) * N:(5 + k) * N] for k in range(K)]
+    Ls = [lam[(4 + k) * N:(5 + k) * N] for k in range(K)]
+    den_f = _face_mean(f["den"], f["deng"], rows, ngh, *sk)
+    visc_f = _visc_face(f["visc"], f["viscg"], rows, ngh, *sk)
+    # adjoint of the face mass flux
+    gF = np.zeros(F.size)
+    pm = rows < NP
+    dph = _dphi_adv(F, visc_f * Ae / d)
+    for c in range(3):
+        gF += np.where(pm, lu[c][rows] * dph * (vel[c][ngh] - vel[c][rows]), 0.0)
+    for k, terms in enumerate(adv_blocks):
+        for e, kap, cond in terms:
+            gF[e] += Ls[k][rows[e]] * kap * _dphi_adv(kap * F[e], cond) * (S[k][ngh[e]] - S[k][rows[e]])
+    wF = gF * den_f * Ae
+    wC = -lp[rows] * den_f * Ae
+    wC[3 * N + rows == pref_row] = 0.0
+    wT = wF + wC
+    gu, gvelg = np.zeros((3, N)), np.zeros((9, N))
+    gp, ggp, grc = np.zeros(N), np.zeros((3, N)), np.zeros(N)
+    gfce, gfceg = np.zeros((3, N)), np.zeros((9, N))
+    both = lambda arr, val_v, val_n: (np.add.at(arr, rows, val_v), np.add.at(arr, ngh, val_n))
+    # u_f . n
+    for c in range(3):
+        both(gu[c], wF * 0.5 * nv[c], wF * 0.5 * nv[c])
+        for q in range(3):
+            both(gvelg[3 * c + q], wF * 0.5 * nv[c] * sk[q], wF * 0.5 * nv[c] * sk[q])
+    # vel_rc: pressure (flux path only), grad p through mom, rc, body force
+    rcs = rc[rows] + rc[ngh]
+    both(gp, wF * 0.5 * rcs / d, -wF * 0.5 * rcs / d)
+    for c in range(3):
+        both(ggp[c], wF * 0.5 * rc[rows] * nv[c], wF * 0.5 * rc[ngh] * nv[c])
+    gm = np.repeat(np.arange(N), np.diff(gc[0]))
+    pg = []
+    for c in range(3):
+        acc = np.zeros(N)
+        np.add.at(acc, gm, gc[2][c] * pres[gc[1]] / vol[gm])
+        pg.append(acc)
+    fcc = _fce_centroid(g, fce, fceg, np.arange(N))
+    pg = np.stack([pg[c] - fcc[c] for c in range(3)])
+    fce_e = 0.5 * sum(nv[c] * (fce[c][rows] + fce[c][ngh]) for c in range(3))
+    dp = (pres[ngh] - pres[rows]) / d
+    both(grc, wT * (-0.5 * (dp - fce_e) + 0.5 * sum(nv[c] * pg[c][rows] for c in range(3))),
+         wT * (-0.5 * (dp - fce_e) + 0.5 * sum(nv[c] * pg[c][ngh] for c in range(3))))
+    for c in range(3):
+        both(gfce[c], wT * (0.25 * rcs - 0.5 * rc[rows]) * nv[c], wT * (0.25 * rcs - 0.5 * rc[ngh]) * nv[c])
+        for q in range(3):
+            both(gfceg[3 * c + q], wT * (-0.5 * rc[rows] * nv[c] * cs[q][rows]), wT * (-0.5 * rc[ngh] * nv[c] * cs[q][ngh]))
+    # momentum right-hand sides of the points: +sum st + sum sd + fce_c V
+    gdiv = np.zeros(N)
+    for c in range(3):
+        lc = np.where(pm, lu[c][rows], 0.0)
+        for q in range(3):
+            t = lc * visc_f * Ae * nv[q] * 0.5
+            both(gvelg[3 * q + c], t, t)
+        t = lc * (-2.0 / 3.0) * visc_f * 0.5 * Ae * nv[c]
+        both(gdiv, t, t)
+    for q in (0, 4, 8):
+        gvelg[q] += gdiv
+    for c in range(3):
+        gfce[c][:NP] += lu[c][:NP] * vol[:NP]
+        for q in range(3):
+            gfceg[3 * c + q][:NP] += lu[c][:NP] * vol[:NP] * cs[q][:NP]
+    # rc = V / a_P
+    gaP = np.where(rc > 0.0, -grc * rc * rc / vol, 0.0)
+    Fc = den_f * 0.5 * sum((vel[c][rows] + vel[c][ngh]) * nv[c] for c in range(3)) * Ae
+    w = gaP[rows] * _dh_ap(Fc, visc_f * Ae / d) * den_f * 0.5 * Ae
+    for c in range(3):
+        both(gu[c], w * nv[c], w * nv[c])
+    # least-squares gradients grad_q x_c,v = sum_e gw_q (x_c,n - x_c,v)
+    for c in range(3):
+        for q in range(3):
+            t = gvelg[3 * c + q][rows] * gw[q]
+            both(gu[c], -t, t)
+            t = gfceg[3 * c + q][rows] * gw[q]
+            both(gfce[c], -t, t)
+    # grad p_v = sum_q gc_q p_q / V_v
+    for c in range(3):
+        np.add.at(gp, gc[1], ggp[c][gm] * gc[2][c] / vol[gm])
+    y = A.T @ lam
+    y[:3 * N] += gu.ravel()
+    y[3 * N:4 * N] += gp
+    if body_blk >= 0:
+        y[(4 + body_blk) * N:(5 + body_blk) * N] += sum(dfce[c] * gfce[c] for c in range(3))
+    return y
 
 
 def bjacobi_apply(rp, col, val, r, N):

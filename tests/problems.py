@@ -458,6 +458,51 @@ class HeatFlowProblem:
                     np.concatenate([np.asarray(v), np.asarray(v2)]), np.concatenate([b, b2]))
         return rows
 
+    def affine_closure(self, ctx):
+        """The body force as a device closure fce = (0, 0, gamma (T - t_ref)) of the temperature
+        (in place of the host hook)."""
+        N = self.N
+        f0 = np.zeros((3, N))
+        f0[2] = -self.gamma * self.t_ref
+        df = np.zeros((3, N))
+        df[2] = self.gamma
+        ctx.vecdom_func_affine(self.fp.ids[4], self.hid["T"], f0, df)
+
+    def dfce(self):
+        """d fce / d T of the Boussinesq closure (3, N)."""
+        df = np.zeros((3, self.N))
+        df[2] = self.gamma
+        return df
+
+    def oracle_residual(self, g, gc, dt=0.0):
+        """R(x, x_prev) of the coupled system (O.coupled_residual) -> (R, A, f)."""
+        fp = self.fp
+        bnd_nid = [self.mesh.bnd_nodes(bb) for bb in range(self.mesh.num_bnd)]
+        rows = self.oracle_heat_rows(g, dt)
+        return lambda x, xp=None: O.coupled_residual(g, gc, x, xp, fp.den, fp.visc, self.fce_of, fp.bnd_vel, bnd_nid, fp.pref,
+                                                      rows, dt, den_const="den" in fp.const, visc_const="visc" in fp.const)
+
+    def oracle_jt(self, g, gc, dt=0.0):
+        """(x, x_prev, lam) -> J(x)^T lam of the coupled system (O.flow_jt)."""
+        res = self.oracle_residual(g, gc, dt)
+        h = self.hid
+        N = self.N
+        bnd_nid = [self.mesh.bnd_nodes(bb) for bb in range(self.mesh.num_bnd)]
+        gr = lambda v: (v, O.update_grad(g, v))
+        scl = {h["rcp"]: gr(self.rcp), h["k"]: gr(self.k), h["q"]: gr(self.q)}
+        if self.species:
+            scl[h["D"]] = gr(self.D)
+        pref_row = 3 * N + self.mesh.num_pts + int(self.fp.pref[0]) if self.fp.pref is not None else -1
+
+        def jt(x, xp, lam):
+            _, A, f = res(x, xp)
+            fd = {h["flw"]: (f["den"], f["deng"])}
+            blocks = [O.scalar_adv_terms(g, [O.Op(*o) for o in self.heat_ops(h)], scl, fd, bnd_nid)]
+            if self.species:
+                blocks.append(O.scalar_adv_terms(g, [O.Op(*o) for o in self.species_ops(h)], scl, fd, bnd_nid))
+            return O.flow_jt(g, gc, f, x, lam, A, pref_row, blocks, 0, self.dfce())
+        return jt
+
     def oracle_t_dirichlet(self):
         NP, N = self.mesh.num_pts, self.N
         n0 = NP + self.mesh.bnd_nodes(X0).astype(np.int64)

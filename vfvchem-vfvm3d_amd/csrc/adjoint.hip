@@ -268,9 +268,11 @@ int adjoint_check_ops(Ctx& c, const vfvm_op* ops, int32_t nops) {
     return VFVM_OK;
 }
 
+}  // namespace
+
 // CSR of A^T from the pattern of A: rows of A^T ascending in the column index of A, their
 // entries ascending in the row index of A; tsrc maps each entry to its slot in A's values.
-int build_transpose(Ctx& c) {
+int vfvm::build_transpose(Ctx& c) {
     if (c.has_transpose) return VFVM_OK;
     const int64_t n = c.n, nnz = c.nnz;
     const std::vector<int32_t>& rp = c.h_mrow;
@@ -302,7 +304,7 @@ int build_transpose(Ctx& c) {
 }
 
 // rev[e] for e = (v, n): the slot of v in n's (ascending) neighbour list
-int build_rev(Ctx& c) {
+int vfvm::build_rev(Ctx& c) {
     DevGeom& g = c.g;
     if (g.rev.p || g.E == 0) return VFVM_OK;
     std::vector<int32_t> rev(g.E);
@@ -319,6 +321,8 @@ int build_rev(Ctx& c) {
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     return VFVM_OK;
 }
+
+namespace {
 
 int copy_d2d(Ctx& c, double* dst, const double* src, int64_t n) {
     VFVM_HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));

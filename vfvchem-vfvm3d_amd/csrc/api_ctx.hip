@@ -37,6 +37,13 @@ __global__ void k_fill(int64_t n, double v, double* __restrict__ out) {
     if (i < n) out[i] = v;
 }
 // u[NP + nid[k]] = val[k]  (OpSclBndDirichlet::apply_initial, opscl_bnd_dir.rs:34-53)
+// vec = f0 + df s per component (3N SoA), the affine closure of a vector function field
+__global__ void k_affine(int64_t N, const double* __restrict__ f0, const double* __restrict__ df, const double* __restrict__ s,
+                         double* __restrict__ vec) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= 3 * N) return;
+    vec[i] = f0[i] + df[i] * s[i % N];
+}
 __global__ void k_scatter_bnd(int64_t cnt, int64_t NP, const int32_t* __restrict__ nid, const double* __restrict__ val,
                               double* __restrict__ u) {
     int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
@@ -277,12 +284,13 @@ extern "C" int vfvm_mesh_upload(vfvm_ctx* ctx, const vfvm_mesh* mesh) {
     const Mesh& m = *mp;
     // reset everything that depends on the mesh
     c.scl.clear(); c.vec.clear(); c.sbnd.clear(); c.flw.clear(); c.vbndv.clear();
-    c.scale_scl.clear(); c.scale_vec.clear();
+    c.scale_scl.clear(); c.scale_vec.clear(); c.vaff.clear();
     c.lin.active = false;
     c.has_pattern = false; c.has_matrix = false; c.ilu.ready = false; c.flow = false;
     c.has_transpose = false; c.tilu.ready = false; c.tape.valid = false;
     DevGeom& g = c.g;
     g.rev.reset();
+    g.gct_ptr.reset(); g.gct_slot.reset(); g.gct_own.reset();
     g.N = m.num_tot(); g.NP = m.num_pts; g.E = m.num_edges(); g.S = m.num_subfaces();
     g.num_bnd = (int32_t)m.bnd_name.size();
     hipStream_t s = c.stream;
@@ -945,11 +953,51 @@ extern "C" int vfvm_set_scale_off(vfvm_ctx* ctx, int32_t off) {
     return clear_error();
 }
 
+int vfvm::eval_affine_closures(Ctx& c) {
+    const int64_t N = c.g.N;
+    for (const Ctx::VecAffine& a : c.vaff) {
+        VecDom& f = c.vec[a.vec];
+        hipLaunchKernelGGL(k_affine, dim3(grid_for(3 * N)), dim3(BLK), 0, c.stream, N, (const double*)a.f0.p, (const double*)a.df.p,
+                           (const double*)c.scl[a.scl].value.p, f.value.p);
+        VFVM_HIP_TRY(hipGetLastError());
+        f.grad_valid = false;
+    }
+    return VFVM_OK;
+}
+
 int vfvm::call_update_hook(Ctx& c, double t) {
+    if (int st = eval_affine_closures(c)) return st;
     if (!c.update_hook) return VFVM_OK;
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
     c.update_hook(c.update_user, t);
     return VFVM_OK;
+}
+
+// Variables::add_vecdom_func with an affine closure of one scalar field (vars.rs:153):
+// vec := f0 + df s, component-wise (3N SoA arrays), re-evaluated on the device wherever the
+// drivers evaluate function fields (_update_func); df is what the flow adjoint differentiates
+extern "C" int vfvm_vecdom_func_affine(vfvm_ctx* ctx, int32_t vec_id, int32_t scl_id, const double* f0, const double* df) {
+    CTX_GUARD(ctx);
+    if (vec_id < 0 || vec_id >= (int32_t)c.vec.size() || scl_id < 0 || scl_id >= (int32_t)c.scl.size())
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_vecdom_func_affine: field id out of range");
+    if (c.vec[vec_id].kind != VFVM_FIELD_FUNCTION)
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_vecdom_func_affine: the vector field must be a function field");
+    size_t k = 0;
+    while (k < c.vaff.size() && c.vaff[k].vec != vec_id) ++k;
+    if (!f0 || !df) {  // removes the closure
+        if (k < c.vaff.size()) c.vaff.erase(c.vaff.begin() + k);
+        return clear_error();
+    }
+    if (k == c.vaff.size()) c.vaff.emplace_back();
+    Ctx::VecAffine& a = c.vaff[k];
+    a.vec = vec_id;
+    a.scl = scl_id;
+    const int64_t N = c.g.N;
+    VFVM_HIP_TRY(a.f0.upload(f0, 3 * N, c.stream));
+    VFVM_HIP_TRY(a.df.upload(df, 3 * N, c.stream));
+    if (int st = eval_affine_closures(c)) return st;
+    if (int st = sync_err(c)) return st;
+    return clear_error();
 }
 
 extern "C" int vfvm_set_update_hook(vfvm_ctx* ctx, vfvm_update_hook hook, void* user) {

@@ -71,6 +71,9 @@ struct DevGeom {
     DevBuf<int32_t> gc_ptr, gc_vid;
     DevBuf<double> gc_c;                       // 3G SoA
     std::vector<int32_t> h_gc_ptr, h_gc_vid;
+    // transpose of the coefficient lists, built on demand (flow adjoint): the slots q with
+    // gc_vid[q] = u are gct_slot[gct_ptr[u] .. gct_ptr[u+1]), owner vertex gct_own (ascending)
+    DevBuf<int32_t> gct_ptr, gct_slot, gct_own;
 };
 
 struct SclDom {
@@ -167,6 +170,13 @@ struct AdjTape {
     int32_t seg_first = -1, seg_count = 0;
     DevBuf<double> final_state;
     vfvm_adjoint_stats stats{};
+    // coupled flow system (vfvm_flow_adjoint_forward): the Picard controls of every step,
+    // replayed exactly by the backward pass, and the forward pass's own totals
+    bool flow = false;
+    int32_t picard_max = 0;
+    double picard_tol = 0.0, picard_damp = 1.0;
+    int32_t picard_iterations = 0;
+    double objective = 0.0;
 };
 
 // Captured launches of one restart cycle (one executable graph per chunk of steps).
@@ -267,6 +277,14 @@ struct Ctx {
     // called before every assembly of the drivers
     vfvm_update_hook update_hook = nullptr;
     void* update_user = nullptr;
+    // device closures of vector function fields affine in one scalar field,
+    // vec = f0 + df s (Variables::add_vecdom_func for an affine func, vars.rs:153), evaluated
+    // at the same points as the host hook, before it
+    struct VecAffine {
+        int32_t vec = -1, scl = -1;
+        DevBuf<double> f0, df;  // 3N SoA each
+    };
+    std::vector<VecAffine> vaff;
 
     // transposed system A^T (adjoint solves): CSR of the columns of A, tsrc[k] = slot of
     // entry k in vals; values gathered on demand
@@ -370,8 +388,15 @@ int scalar_classes(Ctx& c, const vfvm_op* ops, int32_t nops, int32_t* unk, std::
                    std::vector<uint8_t>& has_diag, bool* has_time);
 int scalar_block_assemble(Ctx& c, ScalarBlock& blk, int64_t row_off, double dt, int32_t skip_flw);
 int scalar_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops);
-// calls the host update hook (if any) with time t
+// evaluates the device closures of affine function fields, then calls the host update hook
+// (if any) with time t
 int call_update_hook(Ctx& c, double t);
+int eval_affine_closures(Ctx& c);
+// y = A x over the view's CSR in FP64 (krylov.hip)
+int spmv_apply(Ctx& c, const SysView& s, const double* x, double* y);
+// transposed pattern of the context's system and the reverse edge map (adjoint.hip)
+int build_transpose(Ctx& c);
+int build_rev(Ctx& c);
 
 // drivers and adjoint set-up shared across translation units (api_ctx.hip)
 int assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt);

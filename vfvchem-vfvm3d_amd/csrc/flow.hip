@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "flow.hpp"
 #include "xcd.hpp"
 
 using namespace vfvm;
@@ -34,94 +35,6 @@ constexpr int WAVE = 64;
 constexpr int MAX_BND = MAX_CLASSES;
 constexpr double NEG_FRAC_TWO_THREE = -2.0 / 3.0;  // opvec_dom_ddil.rs:6
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
-
-struct FlowGeom {
-    int64_t N, NP, E, S, G;
-    const int32_t* rp;
-    const int32_t* ngh;
-    const int32_t* erow;
-    const double* dist;
-    const double* area;
-    const double* norm;  // 3E
-    const double* skew;  // 3E
-    const double* gw;    // 3E
-    const double* vol;
-    const double* cskew;  // 3N
-    const int32_t* bsf_ptr;
-    const int32_t* bsf_vj;
-    const int32_t* bsf_vk;
-    const double* bsf_area;
-    const double* bsf_norm;  // 3S
-    const double* bsf_wgt;   // 3S
-    const int32_t* vbnd;     // vertex -> boundary (-1: point)
-    const int32_t* bidx;     // index of a node in its boundary
-    const int32_t* gc_ptr;
-    const int32_t* gc_vid;
-    const double* gc_c;  // 3G
-};
-
-struct FlowFields {
-    const double* vel;    // 3N
-    const double* velg;   // 9N (u_x, u_y, u_z, v_x, ...)
-    const double* pres;   // N
-    const double* den;    // N
-    const double* deng;   // 3N
-    const double* visc;   // N
-    const double* viscg;  // 3N
-    const double* fce;    // 3N
-    const double* fceg;   // 9N
-    double* rc;           // N
-    double* mom;          // 3N
-    double* flux;         // E
-    const double* vel_prev;  // 3N (transient)
-    const double* den_prev;  // N (transient)
-};
-
-struct FlowAsm {
-    FlowGeom g;
-    FlowFields f;
-    const int32_t* mrow;
-    const int32_t* mcol;
-    double* vals;
-    double* b;
-    const double* vbnd_val[MAX_BND];  // per boundary: velocity values (3 x count SoA)
-    int64_t vbnd_cnt[MAX_BND];
-    int64_t pref_row;  // row of the reference pressure (-1: none)
-    double pref_val;
-    int32_t gm, rm;  // LDS capacities of a continuity row
-    int32_t transient;
-    double dt;
-};
-
-// skew-corrected arithmetic mean of a scalar at the face of edge (v, n)
-__device__ inline double face_mean(const double* f, const double* fg, int64_t N, int64_t v, int64_t n, double sx, double sy,
-                                   double sz) {
-    const double m = 0.5 * (f[v] + f[n]);
-    return m + 0.5 * (sx * (fg[v] + fg[n]) + sy * (fg[N + v] + fg[N + n]) + sz * (fg[2 * N + v] + fg[2 * N + n]));
-}
-// harmonic mean with skew correction, guarded against a non-positive result (viscosity)
-__device__ inline double face_visc(const double* f, const double* fg, int64_t N, int64_t v, int64_t n, double sx, double sy,
-                                   double sz) {
-    const double fv = f[v], fn = f[n];
-    const double m = (fv + fn != 0.0) ? 2.0 * fv * fn / (fv + fn) : 0.0;
-    const double ff = m + 0.5 * (sx * (fg[v] + fg[n]) + sy * (fg[N + v] + fg[N + n]) + sz * (fg[2 * N + v] + fg[2 * N + n]));
-    return ff > 0.0 ? ff : m;
-}
-// x^5 as powi(5) evaluates it (binary exponentiation)
-__device__ inline double pow5(double x) {
-    const double x2 = x * x;
-    const double x4 = x2 * x2;
-    return x4 * x;
-}
-// body force at the control-volume centroid: fce_v + grad(fce_v) . cskew_v
-__device__ inline void fce_centroid(const FlowGeom& g, const FlowFields& f, int64_t v, double out[3]) {
-    const int64_t N = g.N;
-    const double cx = g.cskew[v], cy = g.cskew[N + v], cz = g.cskew[2 * N + v];
-    const double* F = f.fceg;
-    out[0] = f.fce[v] + F[v] * cx + F[N + v] * cy + F[2 * N + v] * cz;
-    out[1] = f.fce[N + v] + F[3 * N + v] * cx + F[4 * N + v] * cy + F[5 * N + v] * cz;
-    out[2] = f.fce[2 * N + v] + F[6 * N + v] * cx + F[7 * N + v] * cy + F[8 * N + v] * cz;
-}
 
 // flow_helper::update_flow (flow_helper.rs:3-70): rc_v = V_v / a_P with
 // a_P = sum_f cond A(|Pe|) + max(F_c, 0) and the central face mass flux F_c; every vertex
@@ -760,7 +673,61 @@ int flow_compile(Ctx& c, const vfvm_op* ops, int32_t nops, FlowSys& fs) {
     return VFVM_OK;
 }
 
-FlowAsm make_args(Ctx& c) {
+size_t continuity_lds(const FlowSys& fs) {
+    return sizeof(double) * (3 * (size_t)fs.gmax + fs.rmax + 3 * WAVE) + sizeof(int32_t) * (2 * (size_t)fs.gmax + fs.rmax + WAVE);
+}
+
+// gradients of the fields the flow operators read (ScalarDomain/VectorDomain::update_grad);
+// a function field keeps its gradient until its values change
+int flow_grads(Ctx& c, const FlowSys& fs) {
+    for (int32_t id : {fs.den, fs.visc}) {
+        SclDom& f = c.scl[id];
+        if (f.kind == VFVM_FIELD_CONSTANT || (f.kind == VFVM_FIELD_FUNCTION && f.grad_valid)) continue;
+        if (int st = launch_grad(c, f.value.p, f.grad.p)) return st;
+        f.grad_valid = f.kind == VFVM_FIELD_FUNCTION;
+    }
+    for (int32_t id : {fs.vel, fs.fce}) {
+        VecDom& f = c.vec[id];
+        if (f.kind == VFVM_FIELD_CONSTANT || (f.kind == VFVM_FIELD_FUNCTION && f.grad_valid)) continue;
+        if (int st = launch_grad_vec(c, f.value.p, f.grad.p)) return st;
+        f.grad_valid = f.kind == VFVM_FIELD_FUNCTION;
+    }
+    return VFVM_OK;
+}
+
+int flow_update_impl(Ctx& c) {
+    SclFlw& fl = c.flw[c.fs.flw];
+    VFVM_HIP_TRY(fl.rc.alloc(c.g.N));
+    VFVM_HIP_TRY(fl.mom.alloc(3 * c.g.N));
+    VFVM_HIP_TRY(fl.flux.alloc(std::max<int64_t>(c.g.E, 1)));
+    if (int st = flow_grads(c, c.fs)) return st;
+    FlowAsm a = flow_make_args(c);
+    hipLaunchKernelGGL(k_flow_rc, dim3(grid_for(c.g.N)), dim3(BLK), 0, c.stream, a.g, a.f);
+    hipLaunchKernelGGL(k_flow_mom, dim3(grid_for(c.g.N)), dim3(BLK), 0, c.stream, a.g, a.f);
+    if (c.g.E > 0) hipLaunchKernelGGL(k_flow_flux, dim3(grid_for(c.g.E)), dim3(BLK), 0, c.stream, a.g, a.f);
+    VFVM_HIP_TRY(hipGetLastError());
+    return VFVM_OK;
+}
+
+int flow_assemble_impl(Ctx& c) {
+    if (int st = flow_update_impl(c)) return st;
+    FlowAsm a = flow_make_args(c);
+    const int64_t N = c.g.N;
+    if (c.fs.lpr == 16)
+        hipLaunchKernelGGL((k_flow_momentum_rows<16>), dim3(grid_for(N * 16)), dim3(BLK), 0, c.stream, a);
+    else
+        hipLaunchKernelGGL((k_flow_momentum_rows<32>), dim3(grid_for(N * 32)), dim3(BLK), 0, c.stream, a);
+    hipLaunchKernelGGL(k_flow_continuity_rows, dim3((unsigned)N), dim3(WAVE), continuity_lds(c.fs), c.stream, a);
+    VFVM_HIP_TRY(hipGetLastError());
+    return VFVM_OK;
+}
+
+using clk = std::chrono::steady_clock;
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+}  // namespace
+
+FlowAsm vfvm::flow_make_args(Ctx& c) {
     DevGeom& g = c.g;
     FlowAsm a;
     std::memset(&a, 0, sizeof(a));
@@ -790,60 +757,6 @@ FlowAsm make_args(Ctx& c) {
     a.dt = fs.dt;
     return a;
 }
-
-size_t continuity_lds(const FlowSys& fs) {
-    return sizeof(double) * (3 * (size_t)fs.gmax + fs.rmax + 3 * WAVE) + sizeof(int32_t) * (2 * (size_t)fs.gmax + fs.rmax + WAVE);
-}
-
-// gradients of the fields the flow operators read (ScalarDomain/VectorDomain::update_grad);
-// a function field keeps its gradient until its values change
-int flow_grads(Ctx& c, const FlowSys& fs) {
-    for (int32_t id : {fs.den, fs.visc}) {
-        SclDom& f = c.scl[id];
-        if (f.kind == VFVM_FIELD_CONSTANT || (f.kind == VFVM_FIELD_FUNCTION && f.grad_valid)) continue;
-        if (int st = launch_grad(c, f.value.p, f.grad.p)) return st;
-        f.grad_valid = f.kind == VFVM_FIELD_FUNCTION;
-    }
-    for (int32_t id : {fs.vel, fs.fce}) {
-        VecDom& f = c.vec[id];
-        if (f.kind == VFVM_FIELD_CONSTANT || (f.kind == VFVM_FIELD_FUNCTION && f.grad_valid)) continue;
-        if (int st = launch_grad_vec(c, f.value.p, f.grad.p)) return st;
-        f.grad_valid = f.kind == VFVM_FIELD_FUNCTION;
-    }
-    return VFVM_OK;
-}
-
-int flow_update_impl(Ctx& c) {
-    SclFlw& fl = c.flw[c.fs.flw];
-    VFVM_HIP_TRY(fl.rc.alloc(c.g.N));
-    VFVM_HIP_TRY(fl.mom.alloc(3 * c.g.N));
-    VFVM_HIP_TRY(fl.flux.alloc(std::max<int64_t>(c.g.E, 1)));
-    if (int st = flow_grads(c, c.fs)) return st;
-    FlowAsm a = make_args(c);
-    hipLaunchKernelGGL(k_flow_rc, dim3(grid_for(c.g.N)), dim3(BLK), 0, c.stream, a.g, a.f);
-    hipLaunchKernelGGL(k_flow_mom, dim3(grid_for(c.g.N)), dim3(BLK), 0, c.stream, a.g, a.f);
-    if (c.g.E > 0) hipLaunchKernelGGL(k_flow_flux, dim3(grid_for(c.g.E)), dim3(BLK), 0, c.stream, a.g, a.f);
-    VFVM_HIP_TRY(hipGetLastError());
-    return VFVM_OK;
-}
-
-int flow_assemble_impl(Ctx& c) {
-    if (int st = flow_update_impl(c)) return st;
-    FlowAsm a = make_args(c);
-    const int64_t N = c.g.N;
-    if (c.fs.lpr == 16)
-        hipLaunchKernelGGL((k_flow_momentum_rows<16>), dim3(grid_for(N * 16)), dim3(BLK), 0, c.stream, a);
-    else
-        hipLaunchKernelGGL((k_flow_momentum_rows<32>), dim3(grid_for(N * 32)), dim3(BLK), 0, c.stream, a);
-    hipLaunchKernelGGL(k_flow_continuity_rows, dim3((unsigned)N), dim3(WAVE), continuity_lds(c.fs), c.stream, a);
-    VFVM_HIP_TRY(hipGetLastError());
-    return VFVM_OK;
-}
-
-using clk = std::chrono::steady_clock;
-double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
-
-}  // namespace
 
 bool vfvm::is_flow_list(const vfvm_op* ops, int32_t nops) {
     if (!ops) return false;
@@ -1103,7 +1016,7 @@ int vfvm::flow_initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
     SclFlw& fl = c.flw[fs.flw];
     VFVM_HIP_TRY(fl.rc.alloc(c.g.N));
     VFVM_HIP_TRY(fl.mom.alloc(3 * c.g.N));
-    FlowAsm a = make_args(c);
+    FlowAsm a = flow_make_args(c);
     c.fs = saved;
     const int64_t nn = c.g.N - c.g.NP;
     if (nn > 0)

@@ -1510,6 +1510,12 @@ int ilu_order(Ctx& c, int32_t* perm, int32_t* color_ptr, int32_t* ncolors) {
     return VFVM_OK;
 }
 
+int spmv_apply(Ctx& c, const SysView& sy, const double* x, double* y) {
+    const int lpr = pick_lpr(sy.n, sy.nnz);
+    const double bytes = (double)sy.nnz * 12.0 + 4.0 * (sy.n + 1) + 16.0 * sy.n;
+    return spmv_launch<double>(c, sy, lpr, sy.n, sy.vals, x, y, nullptr, nullptr, bytes);
+}
+
 int residual_norm(Ctx& c, const SysView& sy, const double* x_dev, double* rel, double* bnorm) {
     const int64_t n = sy.n;
     const int lpr = pick_lpr(n, sy.nnz);

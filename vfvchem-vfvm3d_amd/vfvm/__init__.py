@@ -6,7 +6,7 @@ include/vfvm.h); this package is the host-side mirror of the reference's API.
 from ._lib import LIB_PATH, VFVChemError, lib  # noqa: F401  (fails loudly without the library)
 from .core import (FIELD_CONSTANT, FIELD_FUNCTION, FIELD_UNKNOWN, FP32_REFINED, FP64, K_ASSEMBLE, K_MDOT, K_PRECOND, K_UPDATE,  # noqa: F401
                    K_SPMV, ORTHO_AUTO, ORTHO_CGS, ORTHO_CGS2, PRECOND_ILU0, PRECOND_JACOBI, PRECOND_NONE, WRT_INITIAL, WRT_SCLBND,
-                   WRT_SCLDOM, Context, Mesh, solve_opts)
+                   WRT_SCLDOM, WRT_VECBND, Context, Mesh, solve_opts)
 from .core import (OP_SCL_BND_ADVECTION_UNITY, OP_SCL_BND_DIFFUSION, OP_SCL_BND_DIRICHLET, OP_SCL_BND_NEUMANN,  # noqa: F401
                    OP_SCL_BND_PRODUCTION, OP_SCL_BND_SOURCE, OP_SCL_BND_TIME, OP_SCL_DOM_ADVECTION_UNITY,
                    OP_SCL_DOM_DIFFUSION, OP_SCL_DOM_PRODUCTION, OP_SCL_DOM_SOURCE, OP_SCL_DOM_TIME)

@@ -109,6 +109,16 @@ class AdjointWrt(C.Structure):
     _fields_ = [("kind", C.c_int32), ("id", C.c_int32), ("grad", C.POINTER(C.c_double))]
 
 
+class FlowAdjointOpts(C.Structure):
+    _fields_ = [("inner", SolveOpts), ("tol", C.c_double), ("max_iter", C.c_int32), ("restart", C.c_int32)]
+
+
+class FlowAdjointStats(C.Structure):
+    _fields_ = [("base", AdjointStats), ("picard_iterations", C.c_int32), ("recompute_picard_iterations", C.c_int32),
+                ("outer_iterations", C.c_int32), ("reserved", C.c_int32), ("objective", C.c_double),
+                ("max_outer_residual", C.c_double)]
+
+
 _P = C.c_void_p
 _I = C.c_int32
 _L = C.c_int64
@@ -168,6 +178,13 @@ SIGNATURES = [
     ("vfvm_transient_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, _D, _D, C.POINTER(DriverStats)]),
     ("vfvm_adjoint_forward", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, C.POINTER(AdjointStats)]),
     ("vfvm_adjoint_backward", _I, [_P, C.POINTER(SolveOpts), _PD, _I, C.POINTER(AdjointWrt), C.POINTER(AdjointStats)]),
+    ("vfvm_vecdom_func_affine", _I, [_P, _I, _I, _PD, _PD]),
+    ("vfvm_flow_adjoint_forward", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _D, _I, _I, _D, _D, _I, _PD, _PD,
+                                       C.POINTER(FlowAdjointStats)]),
+    ("vfvm_flow_adjoint_backward", _I, [_P, C.POINTER(FlowAdjointOpts), _PD, _PD, _I, C.POINTER(AdjointWrt),
+                                        C.POINTER(FlowAdjointStats)]),
+    ("vfvm_flow_residual_eval", _I, [_P, C.POINTER(OpStruct), _I, _D, _PD, _PD, _PD]),
+    ("vfvm_flow_jt_apply", _I, [_P, C.POINTER(OpStruct), _I, _D, _PD, _PD, _PD, _PD]),
     ("vfvm_set_scldom_scale", _I, [_P, _I, _D]),
     ("vfvm_set_vecdom_scale", _I, [_P, _I, _D]),
     ("vfvm_set_scale_off", _I, [_P, _I]),

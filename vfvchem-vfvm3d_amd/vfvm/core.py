@@ -47,7 +47,7 @@ FP64, FP32_REFINED = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_ILU0, PRECOND_BJACOBI = 0, 1, 2, 3
 K_SPMV, K_MDOT, K_ASSEMBLE, K_PRECOND, K_UPDATE = 0, 1, 2, 3, 4
 ORTHO_AUTO, ORTHO_CGS, ORTHO_CGS2 = 0, 1, 2
-WRT_SCLDOM, WRT_SCLBND, WRT_INITIAL = 0, 1, 2
+WRT_SCLDOM, WRT_SCLBND, WRT_INITIAL, WRT_VECBND = 0, 1, 2, 3
 
 _MESH_ARRAYS = {
     "pos": (0, np.float64), "tets": (1, np.int32), "bnd_ptr": (2, np.int64), "bnd_nid": (3, np.int32),
@@ -206,6 +206,7 @@ class Context:
         check(lib.vfvm_create(int(device), C.byref(self._h)))
         self.mesh = None
         self._sclbnd_size = []
+        self._vecbnd_size = []
 
     def close(self):
         if self._h:
@@ -222,6 +223,7 @@ class Context:
         check(lib.vfvm_mesh_upload(self._h, mesh._h))
         self.mesh = mesh
         self._sclbnd_size = []
+        self._vecbnd_size = []
 
     # fields
     def scldom_add(self, kind, cnst=0.0, values=None) -> int:
@@ -285,7 +287,19 @@ class Context:
         c3 = None if cnst3 is None else np.ascontiguousarray(cnst3, np.float64)
         v = None if values is None else np.ascontiguousarray(values, np.float64)
         check(lib.vfvm_vecbnd_add(self._h, int(bnd), None if c3 is None else ptr(c3), None if v is None else ptr(v), C.byref(i)))
+        self._vecbnd_size.append(len(self.mesh.bnd_nodes(bnd)))
         return i.value
+
+    def vecdom_func_affine(self, vec, scl, f0, df):
+        """vec := f0 + df * scl on the device wherever the drivers evaluate function fields
+        (Variables::add_vecdom_func for an affine closure); f0, df: (3, N).  None, None removes
+        the closure."""
+        if f0 is None or df is None:
+            check(lib.vfvm_vecdom_func_affine(self._h, int(vec), int(scl), None, None))
+            return
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(f0, np.float64).reshape(3, -1), (3, self.mesh.num_tot)))
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(df, np.float64).reshape(3, -1), (3, self.mesh.num_tot)))
+        check(lib.vfvm_vecdom_func_affine(self._h, int(vec), int(scl), ptr(a), ptr(b)))
 
     def update_flow(self, flw):
         check(lib.vfvm_update_flow(self._h, int(flw)))
@@ -504,6 +518,68 @@ class Context:
         o = solve_opts(**(lin or {}))
         check(lib.vfvm_adjoint_backward(self._h, C.byref(o), ptr(seed), len(wrt), arr, C.byref(st)))
         return outs, st
+
+    # adjoint of the coupled flow system
+    def flow_adjoint_forward(self, ops, dt=0.0, num_ts=0, max_iter=20, tol=1e-8, damp=1.0, checkpoint_every=10, lin=None,
+                             seed=None, weights=None):
+        """Driver of the flow list (steady for num_ts = 0, else num_ts implicit-Euler steps) with
+        checkpoints every checkpoint_every states; J = sum_n w_n seed . x^n is returned in
+        stats.objective."""
+        arr = make_ops(ops)
+        st = L.FlowAdjointStats()
+        o = solve_opts(**(lin or {}))
+        sd = None if seed is None else np.ascontiguousarray(seed, np.float64)
+        wt = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        if wt is not None and wt.shape != (int(num_ts),):
+            raise ValueError(f"weights must have num_ts = {num_ts} entries")
+        check(lib.vfvm_flow_adjoint_forward(self._h, arr, len(ops), C.byref(o), float(dt), int(num_ts), int(max_iter), float(tol),
+                                            float(damp), int(checkpoint_every), None if sd is None else ptr(sd),
+                                            None if wt is None else ptr(wt), C.byref(st)))
+        self.system_info()
+        return st
+
+    def flow_adjoint_backward(self, seed, wrt, weights=None, inner=None, tol=1e-10, max_iter=100, restart=20):
+        """dJ/dtheta for the last flow_adjoint_forward: wrt is a list of (WRT_SCLBND, id) or
+        (WRT_VECBND, id); returns (list of gradients, stats).  inner: options of the transposed
+        solves that precondition the outer GCR on J^T."""
+        n = self._size()
+        seed = np.ascontiguousarray(seed, np.float64)
+        if seed.shape != (n,):
+            raise ValueError(f"seed must have {n} entries")
+        wt = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        outs = []
+        arr = (L.AdjointWrt * max(len(wrt), 1))()
+        for i, (kind, fid) in enumerate(wrt):
+            size = self._sclbnd_size[fid] if kind == WRT_SCLBND else 3 * self._vecbnd_size[fid]
+            g = np.zeros(size)
+            outs.append(g)
+            arr[i].kind, arr[i].id, arr[i].grad = int(kind), int(fid), ptr(g)
+        o = L.FlowAdjointOpts()
+        o.inner = solve_opts(**(inner or {}))
+        o.tol, o.max_iter, o.restart = float(tol), int(max_iter), int(restart)
+        st = L.FlowAdjointStats()
+        check(lib.vfvm_flow_adjoint_backward(self._h, C.byref(o), ptr(seed), None if wt is None else ptr(wt), len(wrt), arr,
+                                             C.byref(st)))
+        return [g.reshape(3, -1) if kind == WRT_VECBND else g for g, (kind, _) in zip(outs, wrt)], st
+
+    def flow_residual(self, ops, x, x_prev=None, dt=0.0):
+        """r = A(c(x)) x - b(c(x), x_prev) of the built flow pattern at the state x."""
+        arr = make_ops(ops)
+        x = np.ascontiguousarray(x, np.float64)
+        xp = None if x_prev is None else np.ascontiguousarray(x_prev, np.float64)
+        r = np.empty(self._size())
+        check(lib.vfvm_flow_residual_eval(self._h, arr, len(ops), float(dt), ptr(x), None if xp is None else ptr(xp), ptr(r)))
+        return r
+
+    def flow_jt_apply(self, ops, x, lam, x_prev=None, dt=0.0):
+        """y = J(x)^T lam, J the full Jacobian of flow_residual at x."""
+        arr = make_ops(ops)
+        x = np.ascontiguousarray(x, np.float64)
+        lam = np.ascontiguousarray(lam, np.float64)
+        xp = None if x_prev is None else np.ascontiguousarray(x_prev, np.float64)
+        y = np.empty(self._size())
+        check(lib.vfvm_flow_jt_apply(self._h, arr, len(ops), float(dt), ptr(x), None if xp is None else ptr(xp), ptr(lam), ptr(y)))
+        return y
 
     # timing
     def timing_enable(self, mask=-1):
