@@ -230,9 +230,10 @@ def build_heat_mass_flow(vfvm, n, Ra, Pr, Sc):
     """Natural convection in the unit cube (TransientHeatMassFlow composition,
     transient_heatmassflow.rs:248-600) in free-fall units (velocity sqrt(g beta dT L), time
     L / that velocity): rho = 1, mu = sqrt(Pr / Ra), rho c_p = 1, k = 1 / sqrt(Ra Pr), species
-    diffusivity mu / Sc, body force (0, 0, T - 1/2) set from T before every assembly
-    (Boussinesq), T = C = 1 on x = 0 and 0 on x = 1, zero heat / species flux and no-slip on
-    every wall, reference pressure 0 at node 0; fluid at rest, T = C = 0."""
+    diffusivity mu / Sc, body force (0, 0, T - 1/2) evaluated on the device from T before every
+    assembly (Boussinesq, an affine closure of the temperature), T = C = 1 on x = 0 and 0 on
+    x = 1, zero heat / species flux and no-slip on every wall, reference pressure 0 at node 0;
+    fluid at rest, T = C = 0.  Returns mesh, ctx, ops and the field ids."""
     import numpy as np
     mesh = vfvm.Mesh.lattice(n, 0, 0.3, device=0)  # Voronoi dual built on the GPU
     mesh.build_grad_coeff(device=0)
@@ -262,12 +263,14 @@ def build_heat_mass_flow(vfvm, n, Ra, Pr, Sc):
         ops += [(vfvm.OP_VEC_BND_DIRICHLET, b, vel, vb, -1), (vfvm.OP_SCL_BND_DIVERGENCE, b, pres, vel, den),
                 (vfvm.OP_SCL_BND_PRESSURE_STABILIZER, b, pres, den, fce, flw)]
     # heat, then species (the reference's unknown order [vel, pres, temp, conc])
+    hot = {}
     for u, w, dif, src, adv in ((T, rcp, k, q, vfvm.OP_SCL_DOM_ADVECTION), (C, den, D, -1, vfvm.OP_SCL_DOM_ADVECTION_UNITY)):
         ops += [(vfvm.OP_SCL_DOM_TIME_PREVIOUS, 0, u, w, -1), (vfvm.OP_SCL_DOM_DIFFUSION, 0, u, dif, u)]
         if src >= 0:
             ops.append((vfvm.OP_SCL_DOM_SOURCE, 0, u, src, -1))
         ops.append((adv, 0, u, w, dif, flw) if u == T else (adv, 0, u, dif, flw))
-        ops += [(vfvm.OP_SCL_BND_DIRICHLET, 0, u, ctx.sclbnd_add(0, 1.0), -1),
+        hot[u] = ctx.sclbnd_add(0, 1.0)
+        ops += [(vfvm.OP_SCL_BND_DIRICHLET, 0, u, hot[u], -1),
                 (vfvm.OP_SCL_BND_DIRICHLET, 1, u, ctx.sclbnd_add(1, 0.0), -1)]
         for b in range(2, 6):
             ops += [(vfvm.OP_SCL_BND_TIME_PREVIOUS, b, u, w, -1), (vfvm.OP_SCL_BND_DIFFUSION, b, u, dif, u),
@@ -276,60 +279,36 @@ def build_heat_mass_flow(vfvm, n, Ra, Pr, Sc):
                        (vfvm.OP_SCL_BND_ADVECTION_UNITY, b, u, dif, flw))
     ops.append((vfvm.OP_SCL_DOM_PRESSURE_REFERENCE, 0, pres, 0, -1, -1, 0.0))
     zero = np.zeros(N)
+    ctx.vecdom_func_affine(fce, T, np.stack([zero, zero, zero - 0.5]), np.stack([zero, zero, zero + 1.0]))
+    ids = dict(vel=vel, pres=pres, T=T, C=C, fce=fce, hot_T=hot[T])
+    return mesh, ctx, ops, ids
 
-    def hook(t):
-        ctx.vecdom_set(fce, np.stack([zero, zero, ctx.scldom_get(T) - 0.5]))
-    return mesh, ctx, ops, hook
 
-
-def hmf_temperature_id(ops, vfvm):
-    """Field id of the temperature: the unknown of the first scalar time-previous operator."""
-    for op in ops:
-        if op[0] == vfvm.OP_SCL_DOM_TIME_PREVIOUS:
-            return op[2]
-    raise ValueError("no temperature operator")
+def hmf_rest(ctx, ids, N):
+    """The heat-mass-flow fields back to the initial state (rest, T = C = 0)."""
+    import numpy as np
+    ctx.vecdom_set(ids["vel"], np.zeros((3, N)))
+    for f in ("pres", "T", "C"):
+        ctx.scldom_set(ids[f], np.zeros(N))
 
 
 def run_heat_mass_flow(vfvm, args):
     """Config-5 system (natural convection with a passive species, six unknowns per vertex):
     implicit-Euler steps from rest with Picard per step, then the linear solve of the
     transient system at the last state timed FP64 vs FP32-inner/FP64-refined with GMRES(100)
-    + 6x6 block Jacobi (solved DOF/s at 1e-10)."""
+    + 6x6 block Jacobi (solved DOF/s at 1e-10); then the adjoint of the run: J = time-averaged
+    hot-wall Nusselt number over hmf_ts_total steps, dJ/dtheta of a 32 x 32 bilinear profile of
+    the hot wall's temperature, with every-10-step checkpoints and segment replays, and a
+    central-difference check of the gradient on a short horizon of the same mesh."""
     import torch
-    import numpy as np
     t0 = time.time()
-    mesh, ctx, ops, hook = build_heat_mass_flow(vfvm, args.hmf_n, 1e4, 0.71, 1.0)
+    mesh, ctx, ops, ids = build_heat_mass_flow(vfvm, args.hmf_n, 1e4, 0.71, 1.0)
     t_setup = time.time() - t0
     N = mesh.num_tot
     ndof = 6 * N
     dt = 1e-2
-    # Hot-wall Nusselt number (vfvm.objectives.WallFlux: conductive flux out of the x = 0
-    # nodes' control volumes over k dT / L times the wall area, both 1).  J = mean of Nu over
-    # the time steps.
-    from vfvm.objectives import WallFlux
-    wall_flux = WallFlux(mesh, 0)
-    nus = {}
-
-    def nusselt():
-        return wall_flux(ctx.scldom_get(T_field[0]))
-
-    T_field = [None]
-    last_t = [None]
-    offset = [0]  # steps done by earlier driver calls
-
-    def hook_nu(t):
-        # called before every assembly with the step's time: when the time moves on, the
-        # temperature holds the converged state of the previous step
-        if last_t[0] is not None and t != last_t[0] and last_t[0] > 0.0:
-            nus[offset[0] + round(last_t[0] / dt)] = nusselt()
-            if len(nus) % 10 == 0:
-                progress(f"heat-mass-flow: step {len(nus)}, Nu = {nus[max(nus)]:.4f}")
-        last_t[0] = t
-        hook(t)
     lin = dict(tol=1e-8, restart=args.hmf_restart, max_outer=100, precision=vfvm.FP32_REFINED,
                precond=vfvm.PRECOND_BJACOBI, inner_tol=1e-4, ortho=1)
-    T_field[0] = hmf_temperature_id(ops, vfvm)
-    ctx.set_update_hook(hook_nu)
     tp = time.time()
     note = None
     st = None
@@ -338,9 +317,6 @@ def run_heat_mass_flow(vfvm, args):
     except vfvm.VFVChemError as e:
         note = str(e)
     t_steps = time.time() - tp
-    if st is not None:
-        nus[args.hmf_ts] = nusselt()  # the last step's converged state
-    ctx.set_update_hook(hook)
     progress(f"heat-mass-flow {args.hmf_n}^3: {args.hmf_ts} time steps in {t_steps:.1f} s")
     _, nnz = ctx.system_info()
     ta = time.time()
@@ -349,21 +325,9 @@ def run_heat_mass_flow(vfvm, args):
     t_asm = time.time() - ta
     out = timed_block_solves(vfvm, ctx, ndof, args.tol, args.hmf_restart, args.hmf_inner_tol, args.hmf_steps,
                              "heat-mass-flow", max_outer=100, events_first=True)
-    # the remaining steps of the run, for the objective
-    t_more = 0.0
-    more = max(0, args.hmf_ts_total - args.hmf_ts)
-    if st is not None and more > 0:
-        offset[0], last_t[0] = args.hmf_ts, None
-        ctx.set_update_hook(hook_nu)
-        tm = time.time()
-        try:
-            ctx.transient_solve(ops, dt, more, 8, 1e-6, 1.0, lin=lin)
-            nus[args.hmf_ts_total] = nusselt()
-        except vfvm.VFVChemError as e:
-            note = str(e)
-        t_more = time.time() - tm
-        progress(f"heat-mass-flow {args.hmf_n}^3: {more} more time steps in {t_more:.1f} s")
-    ctx.set_update_hook(None)
+    adj = None
+    if st is not None and args.hmf_ts_total > 0:
+        adj = hmf_adjoint(vfvm, args, mesh, ctx, ops, ids, lin, dt)
     ctx.close()
     return {
         "workload": f"natural convection Ra=1e4, Pr=0.71, Sc=1 (free-fall units) with a passive species (u, v, w, p, T, C), {args.hmf_n}^3 "
@@ -374,13 +338,86 @@ def run_heat_mass_flow(vfvm, args):
         "assembly_wall_ms": 1e3 * t_asm, "time_steps_s": t_steps,
         "picard_iterations": st.picard_iterations if st is not None else None, "steps_note": note,
         "mesh_dual_gc_s": t_setup,
-        "objective": {"J_time_averaged_hot_wall_nusselt": float(np.mean([nus[k] for k in sorted(nus)])) if nus else None,
-                      "steps": len(nus), "t_end": dt * max(nus) if nus else None,
-                      "nu_first": nus[min(nus)] if nus else None, "nu_last": nus[max(nus)] if nus else None,
-                      "steps_s": t_steps + t_more,
-                      "note": "J = mean hot-wall Nusselt number over the implicit-Euler steps (rectangle rule); "
-                              "its adjoint w.r.t. the hot-wall profile is not built"},
+        "adjoint": adj,
     }
+
+
+def hmf_adjoint(vfvm, args, mesh, ctx, ops, ids, lin, dt):
+    """Forward pass with checkpoints, backward pass and the profile gradient of J = mean
+    hot-wall Nusselt number; a central-difference check on args.hmf_fd_steps steps."""
+    import numpy as np
+    from vfvm.objectives import WallProfile, wall_flux_seed
+    N = mesh.num_tot
+    n = 6 * N
+    M, K = args.hmf_ts_total, args.hmf_every
+    # Nu of the x = 0 wall: sum over its nodes' faces of (A / d)(T_v - T_n), the conductive flux
+    # over k dT / L (dT = L = wall area = 1); 1 for the conduction profile T = 1 - x
+    seed = wall_flux_seed(mesh, 0, 4 * N, n)
+    prof = WallProfile(mesh, 0, (32, 32))
+    # the transposed solves precondition the outer GCR: one FP32 cycle of GMRES(30) each (the
+    # outer iteration corrects what it leaves)
+    inner = dict(tol=args.hmf_adj_inner_tol, restart=args.hmf_adj_inner_restart, max_outer=1, precision=vfvm.FP32_REFINED,
+                 precond=vfvm.PRECOND_BJACOBI, inner_tol=1e-4, ortho=1)
+
+    def forward(steps, theta, tol, ln):
+        hmf_rest(ctx, ids, N)
+        ctx.sclbnd_set(ids["hot_T"], prof.values(theta, base=1.0))
+        return ctx.flow_adjoint_forward(ops, dt, steps, max_iter=20, tol=tol, damp=1.0, checkpoint_every=K, lin=ln,
+                                        seed=seed, weights=np.full(steps, 1.0 / steps))
+
+    def on_state(step, outer, inner_steps, rel):
+        if step % 10 == 0 or step <= 2:
+            progress(f"heat-mass-flow adjoint: state {step}, {outer} outer iterations, {inner_steps} inner steps, "
+                     f"residual {rel:.2e}")
+
+    def backward(steps, tol):
+        g, bst = ctx.flow_adjoint_backward(seed, [(vfvm.WRT_SCLBND, ids["hot_T"])], weights=np.full(steps, 1.0 / steps),
+                                           inner=inner, tol=tol, max_iter=200, restart=20, on_state=on_state)
+        return prof.grad(g[0]), bst
+
+    theta0 = np.zeros((32, 32))
+    fst = forward(M, theta0, 1e-6, lin)
+    progress(f"heat-mass-flow adjoint: forward {M} steps in {fst.base.forward_seconds:.1f} s, J = {fst.objective:.6f}")
+    grad, bst = backward(M, args.hmf_adj_tol)
+    progress(f"heat-mass-flow adjoint: backward in {bst.base.backward_seconds:.1f} s "
+             f"(replay {bst.base.recompute_seconds:.1f} s, {bst.outer_iterations} outer iterations)")
+    res = {
+        "workload": f"J = time-averaged hot-wall Nusselt number over {M} implicit-Euler steps (dt 1e-2) from rest, "
+                    f"dJ/dtheta of a 32 x 32 bilinear hot-wall temperature profile, checkpoint every {K}, forward "
+                    f"Picard tol 1e-6 with GMRES({args.hmf_restart}) + 6x6 block Jacobi FP32-inner/FP64-refined at 1e-8; "
+                    f"backward: GCR on J^T (tol {args.hmf_adj_tol}) preconditioned by one FP32 cycle of transposed "
+                    f"GMRES({args.hmf_adj_inner_restart}) + 6x6 block Jacobi on A^T with the reference-pressure mode deflated",
+        "J": fst.objective, "forward_s": fst.base.forward_seconds, "backward_s": bst.base.backward_seconds,
+        "adjoint_over_forward": bst.base.backward_seconds / fst.base.forward_seconds,
+        "recompute_s": bst.base.recompute_seconds, "transposed_solve_s": bst.base.adjoint_solve_seconds,
+        "setup_s": fst.base.setup_seconds + bst.base.setup_seconds,
+        "forward_picard_iterations": fst.picard_iterations, "forward_linear_iterations": fst.base.forward_iterations,
+        "recompute_picard_iterations": bst.recompute_picard_iterations, "recomputed_steps": bst.base.recomputed_steps,
+        "replay_mismatches": bst.base.replay_mismatches, "outer_iterations": bst.outer_iterations,
+        "inner_arnoldi_steps": bst.base.adjoint_iterations, "max_outer_residual": bst.max_outer_residual,
+        "checkpoints": fst.base.checkpoints, "grad_abs_max": float(np.abs(grad).max()),
+        "grad_l2": float(np.linalg.norm(grad)), "grad_sum": float(grad.sum()),
+    }
+    if args.hmf_fd_steps > 0:
+        # central differences of the device forward pass on a short horizon of the same mesh,
+        # Picard and linear solves tightened (1e-10, 1e-11) so that J carries ~1e-11 of relative
+        # error, against the adjoint gradient of the same horizon (outer tolerance 1e-12)
+        S = args.hmf_fd_steps
+        tight = dict(lin, tol=1e-11, max_outer=400)
+        forward(S, theta0, 1e-10, tight)
+        g_s, bs = backward(S, 1e-12)
+        yy, zz = np.meshgrid(np.linspace(0, 1, 32), np.linspace(0, 1, 32), indexing="ij")
+        delta = np.sin(np.pi * yy) * np.sin(2 * np.pi * zz) + 0.5 * np.cos(3 * np.pi * zz)
+        eps = 1e-3
+        jp = forward(S, eps * delta, 1e-10, tight).objective
+        jm = forward(S, -eps * delta, 1e-10, tight).objective
+        fd = (jp - jm) / (2 * eps)
+        ad = float(np.sum(g_s * delta))
+        res["fd_check"] = {"steps": S, "direction": "sin(pi y) sin(2 pi z) + 0.5 cos(3 pi z) on the 32 x 32 grid",
+                           "eps": eps, "adjoint": ad, "central_difference": fd, "rel_err": abs(ad - fd) / abs(fd),
+                           "max_outer_residual": bs.max_outer_residual}
+        progress(f"heat-mass-flow adjoint: {S}-step gradient check, adjoint {ad:.8e}, fd {fd:.8e}")
+    return res
 
 
 def run_flow(vfvm, args):
@@ -494,10 +531,15 @@ def main():
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--hmf-n", type=int, default=100, help="lattice points per axis of the heat-mass-flow system")
     ap.add_argument("--hmf-ts", type=int, default=10, help="implicit-Euler steps before the timed heat-mass-flow solves")
-    ap.add_argument("--hmf-ts-total", type=int, default=100, help="implicit-Euler steps of the heat-mass-flow run (objective J)")
+    ap.add_argument("--hmf-ts-total", type=int, default=100, help="implicit-Euler steps of the heat-mass-flow adjoint run (objective J; 0: skip)")
     ap.add_argument("--hmf-steps", type=int, default=1)
     ap.add_argument("--hmf-restart", type=int, default=100, help="GMRES restart of the heat-mass-flow solves")
     ap.add_argument("--hmf-inner-tol", type=float, default=1e-6, help="inner tolerance of the mixed heat-mass-flow solve")
+    ap.add_argument("--hmf-every", type=int, default=10, help="checkpoint interval of the heat-mass-flow adjoint")
+    ap.add_argument("--hmf-adj-tol", type=float, default=1e-8, help="outer tolerance of the heat-mass-flow adjoint solves")
+    ap.add_argument("--hmf-adj-inner-tol", type=float, default=1e-2, help="tolerance of the transposed solves (outer preconditioner)")
+    ap.add_argument("--hmf-adj-inner-restart", type=int, default=30, help="GMRES restart of the transposed solves")
+    ap.add_argument("--hmf-fd-steps", type=int, default=3, help="steps of the central-difference gradient check (0: skip)")
     ap.add_argument("--no-hmf", action="store_true")
     ap.add_argument("--hmf-only", action="store_true",
                     help="run only the heat-mass-flow line and print it (profiling aid; not the headline line)")

@@ -546,11 +546,16 @@ int vfvm_vecdom_func_affine(vfvm_ctx* ctx, int32_t vec_id, int32_t scl_id, const
  * A^T; each checkpoint segment is replayed to recover x^n.  Coefficient fields must not
  * depend on the unknowns (constant or function fields); a PRODUCTION operator with a
  * non-zero coefficient is rejected. */
+/* Called after each adjoint state with its step (0: steady), the outer iterations and inner
+ * Arnoldi steps it took and its final ||s - J^T l|| / ||s||. */
+typedef void (*vfvm_adjoint_hook)(void* user, int32_t step, int32_t outer, int32_t inner_steps, double rel_residual);
 typedef struct vfvm_flow_adjoint_opts {
     vfvm_solve_opts inner;  /* transposed solves on A^T (the outer preconditioner)       */
     double tol;             /* outer: ||s - J^T l|| <= tol ||s|| per state               */
     int32_t max_iter;       /* outer iterations per state                                */
     int32_t restart;        /* outer directions kept before a restart (1..64)            */
+    vfvm_adjoint_hook on_state;  /* may be NULL                                          */
+    void* user;
 } vfvm_flow_adjoint_opts;
 typedef struct vfvm_flow_adjoint_stats {
     vfvm_adjoint_stats base;          /* steps, checkpoints, Arnoldi steps and times as the

@@ -272,24 +272,31 @@ int adjoint_check_ops(Ctx& c, const vfvm_op* ops, int32_t nops) {
 
 // CSR of A^T from the pattern of A: rows of A^T ascending in the column index of A, their
 // entries ascending in the row index of A; tsrc maps each entry to its slot in A's values.
-int vfvm::build_transpose(Ctx& c) {
-    if (c.has_transpose) return VFVM_OK;
-    const int64_t n = c.n, nnz = c.nnz;
+int vfvm::build_transpose(Ctx& c, bool interleaved) {
+    const int64_t n = c.n, nnz = c.nnz, N = c.g.N;
+    const int64_t S = interleaved && N > 0 && n % N == 0 ? n / N : 1;
+    if (c.has_transpose && c.t_interleaved == (S > 1)) return VFVM_OK;
     const std::vector<int32_t>& rp = c.h_mrow;
     const std::vector<int32_t>& cl = c.h_mcol;
+    // the numbering of the transposed system: identity, or vertex-interleaved (entry (k, v) of
+    // the comp-major order at S v + k)
+    auto perm = [&](int64_t i) -> int64_t { return S == 1 ? i : (i % N) * S + i / N; };
     c.h_trow.assign(n + 1, 0);
-    for (int64_t k = 0; k < nnz; ++k) c.h_trow[cl[k] + 1]++;
+    for (int64_t k = 0; k < nnz; ++k) c.h_trow[perm(cl[k]) + 1]++;
     for (int64_t i = 0; i < n; ++i) c.h_trow[i + 1] += c.h_trow[i];
     c.h_tcol.assign(nnz, 0);
     std::vector<int32_t> tsrc(nnz), tdslot(n, -1), fill(c.h_trow.begin(), c.h_trow.end() - 1);
-    for (int64_t i = 0; i < n; ++i)
+    // rows of A in increasing new number, so that each row of A^T receives its columns ascending
+    for (int64_t t = 0; t < n; ++t) {
+        const int64_t i = S == 1 ? t : (t % S) * N + t / S;
         for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-            const int32_t j = cl[k];
+            const int64_t j = perm(cl[k]);
             const int32_t pos = fill[j]++;
-            c.h_tcol[pos] = (int32_t)i;
+            c.h_tcol[pos] = (int32_t)t;
             tsrc[pos] = k;
-            if (j == i) tdslot[j] = pos;
+            if (j == t) tdslot[j] = pos;
         }
+    }
     hipStream_t s = c.stream;
     VFVM_HIP_TRY(c.trow.upload(c.h_trow.data(), c.h_trow.size(), s));
     VFVM_HIP_TRY(c.tcol.upload(c.h_tcol.data(), c.h_tcol.size(), s));
@@ -300,6 +307,7 @@ int vfvm::build_transpose(Ctx& c) {
     VFVM_HIP_TRY(hipStreamSynchronize(s));
     c.tilu.ready = false;
     c.has_transpose = true;
+    c.t_interleaved = S > 1;
     return VFVM_OK;
 }
 

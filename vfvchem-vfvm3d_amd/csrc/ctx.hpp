@@ -154,6 +154,9 @@ struct SysView {
     // 2 structured rows with the vectors vertex-interleaved (entry (c, v) at (4 + flow_ns) v + c)
     int flow_layout = 0;
     int flow_ns = 0;  // scalar unknowns coupled to the flow (rows (4 + k) N + v), at most 2
+    // generic CSR whose rows, columns and vectors are numbered vertex-interleaved (unknown c of
+    // vertex v at (n / N) v + c): block Jacobi reads and writes its blocks there
+    bool interleaved = false;
 };
 
 // Checkpoint tape of the adjoint forward pass: the operator list, step data and the
@@ -289,6 +292,7 @@ struct Ctx {
     // transposed system A^T (adjoint solves): CSR of the columns of A, tsrc[k] = slot of
     // entry k in vals; values gathered on demand
     bool has_transpose = false;
+    bool t_interleaved = false;  // A^T numbered vertex-interleaved (coupled flow systems)
     std::vector<int32_t> h_trow, h_tcol;
     DevBuf<int32_t> trow, tcol, tsrc, tdslot;
     DevBuf<double> tvals, tb;
@@ -342,6 +346,10 @@ int launch_assemble_block(Ctx& c, double dt, const int32_t* mrow, const int32_t*
 SysView primal_view(Ctx& c);
 SysView transpose_view(Ctx& c);
 int gmres_solve(Ctx& c, const SysView& s, const vfvm_solve_opts& o, double* x_dev, vfvm_solve_stats& st);
+// An inner solve of an outer iteration: x from 0, the FP32 matrix and the preconditioner of the
+// previous solve reused when reuse_setup (the caller guarantees the same values), and no residual
+// after the last allowed cycle (status VFVM_ERR_LINEAR_DID_NOT_CONVERGE, rel_residual -1)
+int gmres_solve_inner(Ctx& c, const SysView& s, const vfvm_solve_opts& o, double* x_dev, vfvm_solve_stats& st, bool reuse_setup);
 // The same solve (FP64, Jacobi or no preconditioner) with its whole workspace carved from the
 // caller's device region of `entries` doubles: no allocation, eager launches, host-clock timing.
 // gmres_noalloc_entries gives the doubles a system needs.
@@ -394,8 +402,9 @@ int call_update_hook(Ctx& c, double t);
 int eval_affine_closures(Ctx& c);
 // y = A x over the view's CSR in FP64 (krylov.hip)
 int spmv_apply(Ctx& c, const SysView& s, const double* x, double* y);
-// transposed pattern of the context's system and the reverse edge map (adjoint.hip)
-int build_transpose(Ctx& c);
+// transposed pattern of the context's system and the reverse edge map (adjoint.hip); with
+// interleaved, A^T is numbered vertex-interleaved (unknown c of vertex v at S v + c, S = n / N)
+int build_transpose(Ctx& c, bool interleaved = false);
 int build_rev(Ctx& c);
 
 // drivers and adjoint set-up shared across translation units (api_ctx.hip)

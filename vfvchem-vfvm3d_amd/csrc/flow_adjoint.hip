@@ -91,13 +91,17 @@ __device__ inline double dh_ap(double F, double cond) {
 struct AdjBlk {
     const DevProgram* prog;  // the scalar unknown's operators per vertex class
     const double* s;         // its values
-    const double* lam;       // its adjoint rows
 };
 
+// The adjoint vectors (l, y, the outer solve's directions) and the transposed system are
+// numbered vertex-interleaved: unknown c of vertex v at S v + c, S = 4 + coupled scalars, so
+// that a row's gathers of a neighbour's unknowns share cache lines.  The work arrays of one
+// product are vertex-major (the 3 or 9 entries of a vertex adjacent).
 struct FlowAdjArgs {
     FlowGeom g;
     FlowFields f;
-    int64_t pref_row;
+    int64_t pref_v;  // vertex of the reference-pressure row (-1: none)
+    int32_t S;       // unknowns per vertex
     int32_t nblk;
     int32_t body_blk;  // block whose unknown drives the body force (-1: the force is fixed)
     AdjBlk blk[2];
@@ -106,8 +110,13 @@ struct FlowAdjArgs {
     const int32_t* gct_ptr;
     const int32_t* gct_slot;
     const int32_t* gct_own;
-    const double* lam;  // [u v w p s1 s2] adjoint vector
-    const double* pg;   // 3N: grad p - fce_c
+    // state-only factors (k_adj_coeffs): per directed edge, 7E SoA
+    double* ec;
+    // per vertex: pg = grad p - fce_c (3N, vertex-major), sgw = sum of the edges' gradient
+    // weights (3N, vertex-major)
+    double* pg;
+    double* sgw;
+    const double* lam;  // adjoint vector, entries (u v w p s1 s2) of each vertex adjacent
     double* wF;         // E
     double* wC;         // E
     double* gu;         // 3N
@@ -121,31 +130,25 @@ struct FlowAdjArgs {
     double* y;          // n
 };
 
-// pg_v = grad p_v - fce_c,v with grad p_v = sum_q gc_q p_q / V_v (the factors of mom_v / rc_v,
-// flow_helper.rs:85-112)
-__global__ void __launch_bounds__(BLK) k_adj_pgrad(FlowAdjArgs p) {
-    const int64_t v = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    const FlowGeom& g = p.g;
-    if (v >= g.N) return;
-    const int64_t N = g.N, G = g.G;
-    const double vol = g.vol[v];
-    double px = 0.0, py = 0.0, pz = 0.0;
-    for (int32_t q = g.gc_ptr[v]; q < g.gc_ptr[v + 1]; ++q) {
-        const double pr = p.f.pres[g.gc_vid[q]];
-        px += g.gc_c[q] * pr / vol;
-        py += g.gc_c[G + q] * pr / vol;
-        pz += g.gc_c[2 * G + q] * pr / vol;
-    }
-    double fc[3];
-    fce_centroid(g, p.f, v, fc);
-    double* pg = const_cast<double*>(p.pg);
-    pg[v] = px - fc[0];
-    pg[N + v] = py - fc[1];
-    pg[2 * N + v] = pz - fc[2];
+// rows of the per-edge factors in ec (entry k E + e)
+enum { EC_RHOA = 0, EC_MPHI = 1, EC_SPHI0 = 2, EC_SPHI1 = 3, EC_MU2A = 4, EC_APW = 5, EC_TFAC = 6, EC_ROWS = 7 };
+
+__device__ inline int64_t ix(const FlowAdjArgs& p, int c, int64_t v) { return (int64_t)p.S * v + c; }
+
+constexpr int LPV = 16;  // lanes per vertex of the gathers
+
+// sum over the LPV lanes of a vertex, in a fixed tree order (the total in every lane)
+__device__ inline double lane_sum(double x) {
+#pragma unroll
+    for (int off = LPV / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, LPV);
+    return x;
 }
 
-// per directed edge e = (v, n): wF_e, wC_e
-__global__ void __launch_bounds__(BLK) k_adj_edge(FlowAdjArgs p) {
+// Per directed edge e = (v, n), once per state: rho_f A; the momentum advection's slope
+// d phi / dF (rows of the points); per coupled scalar the slope sum of its advection operators
+// kappa phi'(kappa F); mu_f A / 2 of the transpose / dilatatory sums (points); the a_P factor
+// h'(Fc) rho_f A / 2; the Rhie-Chow term -((p_n - p_v) / d - fce_e) / 2.
+__global__ void __launch_bounds__(BLK) k_adj_coeffs(FlowAdjArgs p) {
     const int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
     const FlowGeom& g = p.g;
     const FlowFields& f = p.f;
@@ -153,21 +156,17 @@ __global__ void __launch_bounds__(BLK) k_adj_edge(FlowAdjArgs p) {
     const int64_t N = g.N, E = g.E;
     const int64_t v = g.erow[e], n = g.ngh[e];
     const double A = g.area[e], d = g.dist[e];
+    const double nx = g.norm[e], ny = g.norm[E + e], nz = g.norm[2 * E + e];
     const double sx = g.skew[e], sy = g.skew[E + e], sz = g.skew[2 * E + e];
     const double den_f = face_mean(f.den, f.deng, N, v, n, sx, sy, sz);
+    const double visc_f = face_visc(f.visc, f.viscg, N, v, n, sx, sy, sz);
+    const double cond = visc_f * A / d;
     const double F = f.flux[e];
-    double gF = 0.0;
-    if (v < g.NP) {
-        const double visc_f = face_visc(f.visc, f.viscg, N, v, n, sx, sy, sz);
-        const double dp = dphi_adv(F, visc_f * A / d);
-        for (int c = 0; c < 3; ++c) gF += p.lam[c * N + v] * dp * (f.vel[c * N + n] - f.vel[c * N + v]);
-    }
     const int32_t vb = g.vbnd[v];
     const int cls = vb < 0 ? 0 : 1 + vb;
+    double sphi[2] = {0.0, 0.0};
     for (int k = 0; k < p.nblk; ++k) {
         const DevProgram* prog = p.blk[k].prog;
-        const double* s = p.blk[k].s;
-        const double lv = p.blk[k].lam[v];
         for (int o = 0; o < prog->nops[cls]; ++o) {
             const DevOp& op = prog->ops[cls][o];
             if (!is_adv(op.kind)) continue;
@@ -178,147 +177,215 @@ __global__ void __launch_bounds__(BLK) k_adj_edge(FlowAdjArgs p) {
                 const double rho_f = face_mean(op.r_val, op.r_grad, N, v, n, sx, sy, sz);
                 kap = rho_f != 0.0 ? wgt_f / rho_f : 0.0;
             }
-            gF += lv * kap * dphi_adv(kap * op.flux[e], df * A / d) * (s[n] - s[v]);
+            sphi[k] += kap * dphi_adv(kap * op.flux[e], df * A / d);
         }
     }
-    p.wF[e] = gF * den_f * A;
-    p.wC[e] = (3 * N + v == p.pref_row) ? 0.0 : -p.lam[3 * N + v] * den_f * A;
+    const double Fc = den_f * 0.5 * ((f.vel[v] + f.vel[n]) * nx + (f.vel[N + v] + f.vel[N + n]) * ny + (f.vel[2 * N + v] + f.vel[2 * N + n]) * nz) * A;
+    const double fce_e = 0.5 * (nx * (f.fce[v] + f.fce[n]) + ny * (f.fce[N + v] + f.fce[N + n]) + nz * (f.fce[2 * N + v] + f.fce[2 * N + n]));
+    double* ec = p.ec;
+    ec[EC_RHOA * E + e] = den_f * A;
+    ec[EC_MPHI * E + e] = v < g.NP ? dphi_adv(F, cond) : 0.0;
+    ec[EC_SPHI0 * E + e] = sphi[0];
+    ec[EC_SPHI1 * E + e] = sphi[1];
+    ec[EC_MU2A * E + e] = v < g.NP ? 0.5 * visc_f * A : 0.0;
+    ec[EC_APW * E + e] = dh_ap(Fc, cond) * den_f * 0.5 * A;
+    ec[EC_TFAC * E + e] = -0.5 * ((f.pres[n] - f.pres[v]) / d - fce_e);
 }
 
-// per vertex u, over its edges e = (u, n) and their reverses r = (n, u)
-__global__ void __launch_bounds__(BLK) k_adj_vertex(FlowAdjArgs p) {
-    const int64_t u = (int64_t)blockIdx.x * BLK + threadIdx.x;
+// Per vertex, once per state: pg_v = grad p_v - fce_c,v with grad p_v = sum_q gc_q p_q / V_v
+// (the factors of mom_v / rc_v, flow_helper.rs:85-112), and the sums of the edges' gradient
+// weights (the vertex's own term of the transposed least-squares gradient)
+__global__ void __launch_bounds__(BLK) k_adj_vcoeffs(FlowAdjArgs p) {
+    const int64_t v = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const FlowGeom& g = p.g;
+    if (v >= g.N) return;
+    const int64_t E = g.E, G = g.G;
+    const double vol = g.vol[v];
+    double px = 0.0, py = 0.0, pz = 0.0;
+    for (int32_t q = g.gc_ptr[v]; q < g.gc_ptr[v + 1]; ++q) {
+        const double pr = p.f.pres[g.gc_vid[q]];
+        px += g.gc_c[q] * pr / vol;
+        py += g.gc_c[G + q] * pr / vol;
+        pz += g.gc_c[2 * G + q] * pr / vol;
+    }
+    double fc[3];
+    fce_centroid(g, p.f, v, fc);
+    p.pg[3 * v] = px - fc[0];
+    p.pg[3 * v + 1] = py - fc[1];
+    p.pg[3 * v + 2] = pz - fc[2];
+    double wx = 0.0, wy = 0.0, wz = 0.0;
+    for (int32_t e = g.rp[v]; e < g.rp[v + 1]; ++e) {
+        wx += g.gw[e];
+        wy += g.gw[E + e];
+        wz += g.gw[2 * E + e];
+    }
+    p.sgw[3 * v] = wx;
+    p.sgw[3 * v + 1] = wy;
+    p.sgw[3 * v + 2] = wz;
+}
+
+// per directed edge e = (v, n): wF_e, wC_e
+__global__ void __launch_bounds__(BLK) k_adj_edge(FlowAdjArgs p) {
+    const int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
     const FlowGeom& g = p.g;
     const FlowFields& f = p.f;
-    if (u >= g.N) return;
+    if (e >= g.E) return;
     const int64_t N = g.N, E = g.E;
-    const double rc_u = f.rc[u], vol_u = g.vol[u], p_u = f.pres[u];
-    const double pgu[3] = {p.pg[u], p.pg[N + u], p.pg[2 * N + u]};
-    const double cs[3] = {g.cskew[u], g.cskew[N + u], g.cskew[2 * N + u]};
-    const bool upt = u < g.NP;
-    double gu[3] = {0.0, 0.0, 0.0}, ggp[3] = {0.0, 0.0, 0.0}, gfce[3] = {0.0, 0.0, 0.0};
-    double gvelg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, gfceg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int64_t v = g.erow[e], n = g.ngh[e];
+    const double* ec = p.ec;
+    double gF = 0.0;
+    const double mphi = ec[EC_MPHI * E + e];
+    if (mphi != 0.0) {
+        double t = 0.0;
+        for (int c = 0; c < 3; ++c) t += p.lam[ix(p, c, v)] * (f.vel[c * N + n] - f.vel[c * N + v]);
+        gF += mphi * t;
+    }
+    for (int k = 0; k < p.nblk; ++k) {
+        const double sphi = ec[(EC_SPHI0 + k) * E + e];
+        if (sphi != 0.0) gF += sphi * p.lam[ix(p, 4 + k, v)] * (p.blk[k].s[n] - p.blk[k].s[v]);
+    }
+    const double rhoA = ec[EC_RHOA * E + e];
+    p.wF[e] = gF * rhoA;
+    p.wC[e] = (v == p.pref_v) ? 0.0 : -p.lam[ix(p, 3, v)] * rhoA;
+}
+
+// Per vertex u (LPV lanes, each over a stride of u's edges e = (u, n) and their reverses
+// r = (n, u)): the adjoint of the face velocity uf . n and its skew term, of vel_rc (the
+// pressure difference and grad p through mom_u on the flux path; rc_u and the body force on
+// both paths) and of the momentum rows' transpose / dilatatory / source right-hand sides.
+__global__ void __launch_bounds__(BLK) k_adj_vertex(FlowAdjArgs p) {
+    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t u = tid / LPV;
+    const int lane = (int)(tid % LPV);
+    const FlowGeom& g = p.g;
+    const FlowFields& f = p.f;
+    if (u >= g.N) return;  // whole vertex groups leave together (BLK is a multiple of LPV)
+    const int64_t E = g.E;
+    const double* ec = p.ec;
+    const double rc_u = f.rc[u];
+    const double pgu[3] = {p.pg[3 * u], p.pg[3 * u + 1], p.pg[3 * u + 2]};
+    double gu[3] = {0.0, 0.0, 0.0}, ggp[3] = {0.0, 0.0, 0.0}, gfn[3] = {0.0, 0.0, 0.0}, wn[3] = {0.0, 0.0, 0.0};
+    double gvelg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double gp = 0.0, grc = 0.0, gdiv = 0.0;
-    for (int32_t e = g.rp[u]; e < g.rp[u + 1]; ++e) {
+    for (int32_t e = g.rp[u] + lane; e < g.rp[u + 1]; e += LPV) {
         const int64_t n = g.ngh[e];
         const int64_t r = p.rev[e];
-        const double rc_n = f.rc[n], p_n = f.pres[n];
+        const double rc_n = f.rc[n];
         for (int side = 0; side < 2; ++side) {
             // side 0: edge e = (u, n), u its first vertex; side 1: r = (n, u), u its second
             const int64_t k = side == 0 ? e : r;
-            const double A = g.area[k], d = g.dist[k];
+            const double d = g.dist[k];
             const double nv[3] = {g.norm[k], g.norm[E + k], g.norm[2 * E + k]};
             const double sk[3] = {g.skew[k], g.skew[E + k], g.skew[2 * E + k]};
             const double wf = p.wF[k], wt = wf + p.wC[k];
-            double fce_k = 0.0;
-            for (int c = 0; c < 3; ++c) fce_k += nv[c] * (f.fce[c * N + u] + f.fce[c * N + n]);
-            fce_k *= 0.5;
-            double npg = 0.0;
-            for (int c = 0; c < 3; ++c) npg += nv[c] * pgu[c];
-            // face velocity uf . n (both directions read u with weight 1/2)
             for (int c = 0; c < 3; ++c) {
-                gu[c] += wf * 0.5 * nv[c];
-                for (int q = 0; q < 3; ++q) gvelg[3 * c + q] += wf * 0.5 * nv[c] * sk[q];
+                const double t = wf * 0.5 * nv[c];
+                gu[c] += t;
+                for (int q = 0; q < 3; ++q) gvelg[3 * c + q] += t * sk[q];
+                ggp[c] += wf * 0.5 * rc_u * nv[c];
+                gfn[c] += wt * (0.25 * (rc_u + rc_n) - 0.5 * rc_u) * nv[c];
+                wn[c] += wt * nv[c];
             }
-            // vel_rc: the pressure difference, grad p through mom_u, rc_u, the body force
-            const double dpdd = side == 0 ? (p_n - p_u) / d : (p_u - p_n) / d;
             gp += wf * (side == 0 ? 0.5 : -0.5) * (rc_u + rc_n) / d;
-            for (int c = 0; c < 3; ++c) ggp[c] += wf * 0.5 * rc_u * nv[c];
-            grc += wt * (-0.5 * (dpdd - fce_k) + 0.5 * npg);
-            for (int c = 0; c < 3; ++c) {
-                gfce[c] += wt * (0.25 * (rc_u + rc_n) * nv[c] - 0.5 * rc_u * nv[c]);
-                for (int q = 0; q < 3; ++q) gfceg[3 * c + q] += wt * (-0.5 * rc_u * nv[c] * cs[q]);
-            }
-            // transpose and dilatatory right-hand sides of the momentum row of the edge's first
-            // vertex (a point) read the face mean of u's velocity gradient
-            const int64_t row_v = side == 0 ? u : n;
-            if (row_v < g.NP) {
-                const int64_t other = side == 0 ? n : u;
-                const double visc_f = face_visc(f.visc, f.viscg, N, row_v, other, sk[0], sk[1], sk[2]);
+            grc += wt * (ec[EC_TFAC * E + k] + 0.5 * (nv[0] * pgu[0] + nv[1] * pgu[1] + nv[2] * pgu[2]));
+            // transpose and dilatatory sums of the momentum row of the edge's first vertex (a
+            // point) read the face mean of u's velocity gradient
+            const double mu2A = ec[EC_MU2A * E + k];
+            if (mu2A != 0.0) {
+                const int64_t row_v = side == 0 ? u : n;
                 for (int c = 0; c < 3; ++c) {
-                    const double lc = p.lam[c * N + row_v];
-                    for (int q = 0; q < 3; ++q) gvelg[3 * q + c] += lc * visc_f * A * nv[q] * 0.5;
-                    gdiv += lc * (NEG_FRAC_TWO_THREE * visc_f * 0.5 * A * nv[c]);
+                    const double lc = p.lam[ix(p, c, row_v)];
+                    for (int q = 0; q < 3; ++q) gvelg[3 * q + c] += lc * mu2A * nv[q];
+                    gdiv += lc * (NEG_FRAC_TWO_THREE * mu2A * nv[c]);
                 }
             }
         }
     }
-    if (upt) {
-        // source: b -= fce_c V
-        for (int c = 0; c < 3; ++c) {
-            const double lc = p.lam[c * N + u];
-            gfce[c] += lc * vol_u;
-            for (int q = 0; q < 3; ++q) gfceg[3 * c + q] += lc * vol_u * cs[q];
-        }
+    for (int c = 0; c < 3; ++c) {
+        gu[c] = lane_sum(gu[c]);
+        ggp[c] = lane_sum(ggp[c]);
+        gfn[c] = lane_sum(gfn[c]);
+        wn[c] = lane_sum(wn[c]);
+    }
+    for (int q = 0; q < 9; ++q) gvelg[q] = lane_sum(gvelg[q]);
+    gp = lane_sum(gp);
+    grc = lane_sum(grc);
+    gdiv = lane_sum(gdiv);
+    if (lane != 0) return;
+    const double vol_u = g.vol[u];
+    const double cs[3] = {g.cskew[u], g.cskew[g.N + u], g.cskew[2 * g.N + u]};
+    // body force: vel_rc reads fce_e and the centroid values fc = fce + grad fce . cskew
+    // (-rc_u / 2 n_c cskew_q per face); the point rows' source b -= fc V
+    const double lv = u < g.NP ? 1.0 : 0.0;
+    for (int c = 0; c < 3; ++c) {
+        const double lc = lv * p.lam[ix(p, c, u)];
+        p.gfce[3 * u + c] = gfn[c] + lc * vol_u;
+        for (int q = 0; q < 3; ++q) p.gfceg[9 * u + 3 * c + q] = -0.5 * rc_u * wn[c] * cs[q] + lc * vol_u * cs[q];
+        p.gu[3 * u + c] = gu[c];
+        p.ggp[3 * u + c] = ggp[c];
     }
     gvelg[0] += gdiv;
     gvelg[4] += gdiv;
     gvelg[8] += gdiv;
-    for (int c = 0; c < 3; ++c) {
-        p.gu[c * N + u] = gu[c];
-        p.ggp[c * N + u] = ggp[c];
-        p.gfce[c * N + u] = gfce[c];
-    }
-    for (int q = 0; q < 9; ++q) {
-        p.gvelg[q * N + u] = gvelg[q];
-        p.gfceg[q * N + u] = gfceg[q];
-    }
+    for (int q = 0; q < 9; ++q) p.gvelg[9 * u + q] = gvelg[q];
     p.gp1[u] = gp;
     p.gaP[u] = rc_u > 0.0 ? -grc * rc_u * rc_u / vol_u : 0.0;
 }
 
-// per vertex u: a_P chain, transposed gradients, y = A^T l + E^T l
+// Per vertex u (LPV lanes): the a_P chain dFc/du through h'(Fc) (of u's own a_P over e and of
+// each neighbour's over r), the transposed least-squares gradients of the velocity and the
+// body force (grad_q x_c,m = sum over m's edges gw_q (x_c,o - x_c,m)), the transposed
+// pressure gradient (grad p_v = sum_q gc_q p_q / V_v, over the coefficient lists' transpose),
+// the body force's closure; y = A^T l + E^T l.
 __global__ void __launch_bounds__(BLK) k_adj_final(FlowAdjArgs p) {
-    const int64_t u = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t tid = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t u = tid / LPV;
+    const int lane = (int)(tid % LPV);
     const FlowGeom& g = p.g;
-    const FlowFields& f = p.f;
     if (u >= g.N) return;
     const int64_t N = g.N, E = g.E, G = g.G;
-    double gu[3] = {p.gu[u], p.gu[N + u], p.gu[2 * N + u]};
-    double gf[3] = {p.gfce[u], p.gfce[N + u], p.gfce[2 * N + u]};
+    const double* ec = p.ec;
     const double gaP_u = p.gaP[u];
-    for (int32_t e = g.rp[u]; e < g.rp[u + 1]; ++e) {
+    double gu[3] = {0.0, 0.0, 0.0}, gf[3] = {0.0, 0.0, 0.0}, gpv = 0.0;
+    for (int32_t e = g.rp[u] + lane; e < g.rp[u + 1]; e += LPV) {
         const int64_t n = g.ngh[e];
         const int64_t r = p.rev[e];
-        const double gaP_n = p.gaP[n];
-        for (int side = 0; side < 2; ++side) {
-            // a_P of the edge's first vertex: u (side 0, edge e) or n (side 1, edge r)
-            const double ga = side == 0 ? gaP_u : gaP_n;
-            if (ga == 0.0) continue;
-            const int64_t k = side == 0 ? e : r;
-            const int64_t a = side == 0 ? u : n, b = side == 0 ? n : u;
-            const double A = g.area[k], d = g.dist[k];
-            const double nx = g.norm[k], ny = g.norm[E + k], nz = g.norm[2 * E + k];
-            const double sx = g.skew[k], sy = g.skew[E + k], sz = g.skew[2 * E + k];
-            const double den_f = face_mean(f.den, f.deng, N, a, b, sx, sy, sz);
-            const double visc_f = face_visc(f.visc, f.viscg, N, a, b, sx, sy, sz);
-            const double Fc = den_f * 0.5 * ((f.vel[a] + f.vel[b]) * nx + (f.vel[N + a] + f.vel[N + b]) * ny +
-                                             (f.vel[2 * N + a] + f.vel[2 * N + b]) * nz) * A;
-            const double w = ga * dh_ap(Fc, visc_f * A / d) * den_f * 0.5 * A;
-            gu[0] += w * nx;
-            gu[1] += w * ny;
-            gu[2] += w * nz;
-        }
-        // least-squares gradients: grad_q x_c,m = sum over m's edges gw_q (x_c,o - x_c,m)
-        for (int c = 0; c < 3; ++c)
+        const double w0 = gaP_u * ec[EC_APW * E + e], w1 = p.gaP[n] * ec[EC_APW * E + r];
+        const double wr[3] = {g.gw[r], g.gw[E + r], g.gw[2 * E + r]};
+        for (int c = 0; c < 3; ++c) {
+            gu[c] += w0 * g.norm[(int64_t)c * E + e] + w1 * g.norm[(int64_t)c * E + r];
             for (int q = 0; q < 3; ++q) {
-                const double wr = g.gw[(int64_t)q * E + r], we = g.gw[(int64_t)q * E + e];
-                gu[c] += wr * p.gvelg[(3 * c + q) * N + n] - we * p.gvelg[(3 * c + q) * N + u];
-                gf[c] += wr * p.gfceg[(3 * c + q) * N + n] - we * p.gfceg[(3 * c + q) * N + u];
+                gu[c] += wr[q] * p.gvelg[9 * n + 3 * c + q];
+                gf[c] += wr[q] * p.gfceg[9 * n + 3 * c + q];
             }
+        }
     }
-    double gpu = p.gp1[u];
-    for (int32_t t = p.gct_ptr[u]; t < p.gct_ptr[u + 1]; ++t) {
+    for (int32_t t = p.gct_ptr[u] + lane; t < p.gct_ptr[u + 1]; t += LPV) {
         const int32_t q = p.gct_slot[t];
         const int64_t v = p.gct_own[t];
-        gpu += (p.ggp[v] * g.gc_c[q] + p.ggp[N + v] * g.gc_c[G + q] + p.ggp[2 * N + v] * g.gc_c[2 * G + q]) / g.vol[v];
+        gpv += (p.ggp[3 * v] * g.gc_c[q] + p.ggp[3 * v + 1] * g.gc_c[G + q] + p.ggp[3 * v + 2] * g.gc_c[2 * G + q]) / g.vol[v];
     }
-    for (int c = 0; c < 3; ++c) p.y[c * N + u] = p.base[c * N + u] + gu[c];
-    p.y[3 * N + u] = p.base[3 * N + u] + gpu;
+    for (int c = 0; c < 3; ++c) {
+        gu[c] = lane_sum(gu[c]);
+        gf[c] = lane_sum(gf[c]);
+    }
+    gpv = lane_sum(gpv);
+    if (lane != 0) return;
+    const double sw[3] = {p.sgw[3 * u], p.sgw[3 * u + 1], p.sgw[3 * u + 2]};
+    for (int c = 0; c < 3; ++c) {
+        gu[c] += p.gu[3 * u + c];
+        gf[c] += p.gfce[3 * u + c];
+        for (int q = 0; q < 3; ++q) {
+            gu[c] -= sw[q] * p.gvelg[9 * u + 3 * c + q];
+            gf[c] -= sw[q] * p.gfceg[9 * u + 3 * c + q];
+        }
+    }
+    for (int c = 0; c < 3; ++c) p.y[ix(p, c, u)] = p.base[ix(p, c, u)] + gu[c];
+    p.y[ix(p, 3, u)] = p.base[ix(p, 3, u)] + gpv + p.gp1[u];
     for (int k = 0; k < p.nblk; ++k) {
         double s = 0.0;
         if (k == p.body_blk) s = p.dfce[u] * gf[0] + p.dfce[N + u] * gf[1] + p.dfce[2 * N + u] * gf[2];
-        p.y[(4 + k) * N + u] = p.base[(4 + k) * N + u] + s;
+        p.y[ix(p, 4 + k, u)] = p.base[ix(p, 4 + k, u)] + s;
     }
 }
 
@@ -329,11 +396,10 @@ __global__ void __launch_bounds__(BLK) k_adj_time_diag(FlowAdjArgs p, double dt,
     const int64_t v = (int64_t)blockIdx.x * BLK + threadIdx.x;
     const FlowGeom& g = p.g;
     if (v >= g.N) return;
-    const int64_t N = g.N;
     const double vol = g.vol[v];
     const double dm = v < g.NP ? p.f.den[v] * vol / dt : 0.0;
-    for (int c = 0; c < 3; ++c) D[c * N + v] = dm;
-    D[3 * N + v] = 0.0;
+    for (int c = 0; c < 3; ++c) D[ix(p, c, v)] = dm;
+    D[ix(p, 3, v)] = 0.0;
     const int32_t vb = g.vbnd[v];
     const int cls = vb < 0 ? 0 : 1 + vb;
     for (int k = 0; k < p.nblk; ++k) {
@@ -347,7 +413,7 @@ __global__ void __launch_bounds__(BLK) k_adj_time_diag(FlowAdjArgs p, double dt,
             else if (op.kind == VFVM_OP_SCL_DOM_TIME_UNITY || op.kind == VFVM_OP_SCL_BND_TIME_UNITY)
                 acc += vol / dt;
         }
-        D[(4 + k) * N + v] = acc;
+        D[ix(p, 4 + k, v)] = acc;
     }
 }
 
@@ -364,6 +430,43 @@ __global__ void __launch_bounds__(BLK) k_dot_part(int64_t n, const double* __res
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+// per-block partial sums of x over the pressure entries but the reference vertex's
+__global__ void __launch_bounds__(BLK) k_psum_part(int64_t N, int S, int64_t pref_v, const double* __restrict__ x,
+                                                   double* __restrict__ part) {
+    __shared__ double s[BLK];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLK)
+        if (i != pref_v) acc += x[(int64_t)S * i + 3];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = BLK / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+// y = indicator of the continuity rows but the reference one
+__global__ void __launch_bounds__(BLK) k_cont_indicator(int64_t n, int S, int64_t pref_v, double* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) y[i] = (i % S == 3 && i / S != pref_v) ? 1.0 : 0.0;
+}
+// W -= t on the velocity entries of the nodes (their rows are identity rows of A)
+__global__ void __launch_bounds__(BLK) k_mass_correct(int64_t N, int64_t NP, int S, const double* __restrict__ t,
+                                                      double* __restrict__ W) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= 3 * (N - NP)) return;
+    const int64_t k = (int64_t)S * (NP + i / 3) + i % 3;
+    W[k] -= t[k];
+}
+// comp-major (entry (c, v) at c N + v) to vertex-interleaved (at S v + c), and back
+__global__ void __launch_bounds__(BLK) k_to_il(int64_t N, int S, const double* __restrict__ in, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < (int64_t)S * N) out[(int64_t)S * (i % N) + i / N] = in[i];
+}
+__global__ void __launch_bounds__(BLK) k_from_il(int64_t N, int S, const double* __restrict__ in, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < (int64_t)S * N) out[i] = in[(int64_t)S * (i % N) + i / N];
 }
 // y += a x
 __global__ void __launch_bounds__(BLK) k_axpy(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
@@ -483,9 +586,10 @@ int build_gc_transpose(Ctx& c) {
 
 // Work buffers of J^T products and of the outer solve
 struct AdjWork {
-    DevBuf<double> pg, wF, wC, gu, gp1, ggp, gaP, gvelg, gfce, gfceg, atl;
+    DevBuf<double> ec, pg, sgw, wF, wC, gu, gp1, ggp, gaP, gvelg, gfce, gfceg, atl;
     int alloc(Ctx& c) {
         const int64_t N = c.g.N, E = std::max<int64_t>(c.g.E, 1), n = c.n;
+        VFVM_HIP_TRY(ec.grow(EC_ROWS * E)); VFVM_HIP_TRY(sgw.grow(3 * N));
         VFVM_HIP_TRY(pg.grow(3 * N)); VFVM_HIP_TRY(wF.grow(E)); VFVM_HIP_TRY(wC.grow(E));
         VFVM_HIP_TRY(gu.grow(3 * N)); VFVM_HIP_TRY(gp1.grow(N)); VFVM_HIP_TRY(ggp.grow(3 * N)); VFVM_HIP_TRY(gaP.grow(N));
         VFVM_HIP_TRY(gvelg.grow(9 * N)); VFVM_HIP_TRY(gfce.grow(3 * N)); VFVM_HIP_TRY(gfceg.grow(9 * N));
@@ -500,8 +604,9 @@ int adj_args(Ctx& c, AdjWork& w, FlowAdjArgs& p) {
     const FlowAsm a = flow_make_args(c);
     p.g = a.g;
     p.f = a.f;
-    p.pref_row = c.fs.pref_row;
+    p.pref_v = c.fs.pref_row >= 0 ? c.fs.pref_row - 3 * c.g.N : -1;
     p.nblk = (int32_t)c.sblk.size();
+    p.S = 4 + p.nblk;
     if (p.nblk > 2) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "flow adjoint: at most two coupled scalars");
     for (int k = 0; k < p.nblk; ++k) {
         p.blk[k].prog = c.sblk[k].prog.p;
@@ -519,7 +624,9 @@ int adj_args(Ctx& c, AdjWork& w, FlowAdjArgs& p) {
     p.gct_ptr = c.g.gct_ptr.p;
     p.gct_slot = c.g.gct_slot.p;
     p.gct_own = c.g.gct_own.p;
+    p.ec = w.ec.p;
     p.pg = w.pg.p;
+    p.sgw = w.sgw.p;
     p.wF = w.wF.p; p.wC = w.wC.p;
     p.gu = w.gu.p; p.gp1 = w.gp1.p; p.ggp = w.ggp.p; p.gaP = w.gaP.p;
     p.gvelg = w.gvelg.p; p.gfce = w.gfce.p; p.gfceg = w.gfceg.p;
@@ -530,13 +637,12 @@ int adj_args(Ctx& c, AdjWork& w, FlowAdjArgs& p) {
 int jt_apply(Ctx& c, AdjWork& w, FlowAdjArgs p, const double* lam, double* y) {
     if (int st = spmv_apply(c, transpose_view(c), lam, w.atl.p)) return st;
     p.lam = lam;
-    for (int k = 0; k < p.nblk; ++k) p.blk[k].lam = lam + (int64_t)(4 + k) * c.g.N;
     p.base = w.atl.p;
     p.y = y;
     const int64_t N = c.g.N, E = c.g.E;
     if (E > 0) hipLaunchKernelGGL(k_adj_edge, dim3(grid_for(E)), dim3(BLK), 0, c.stream, p);
-    hipLaunchKernelGGL(k_adj_vertex, dim3(grid_for(N)), dim3(BLK), 0, c.stream, p);
-    hipLaunchKernelGGL(k_adj_final, dim3(grid_for(N)), dim3(BLK), 0, c.stream, p);
+    hipLaunchKernelGGL(k_adj_vertex, dim3(grid_for(N * LPV)), dim3(BLK), 0, c.stream, p);
+    hipLaunchKernelGGL(k_adj_final, dim3(grid_for(N * LPV)), dim3(BLK), 0, c.stream, p);
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
 }
@@ -553,7 +659,8 @@ int load_state(Ctx& c, const std::vector<vfvm_op>& ops, double dt, const double*
     if (int st = flow_assemble_dt(c, ops.data(), (int32_t)ops.size(), dt)) return st;
     if (c.nnz > 0) hipLaunchKernelGGL(k_gather, dim3(grid_for(c.nnz)), dim3(BLK), 0, c.stream, c.nnz, c.tsrc.p, c.vals.p, c.tvals.p);
     if (int st = adj_args(c, w, p)) return st;
-    hipLaunchKernelGGL(k_adj_pgrad, dim3(grid_for(c.g.N)), dim3(BLK), 0, c.stream, p);
+    hipLaunchKernelGGL(k_adj_vcoeffs, dim3(grid_for(c.g.N)), dim3(BLK), 0, c.stream, p);
+    if (c.g.E > 0) hipLaunchKernelGGL(k_adj_coeffs, dim3(grid_for(c.g.E)), dim3(BLK), 0, c.stream, p);
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
 }
@@ -561,6 +668,21 @@ int load_state(Ctx& c, const std::vector<vfvm_op>& ops, double dt, const double*
 int dot(Ctx& c, const double* a, const double* b, int64_t n, double* out) {
     VFVM_HIP_TRY(c.red.grow(DOT_BLOCKS));
     hipLaunchKernelGGL(k_dot_part, dim3(DOT_BLOCKS), dim3(BLK), 0, c.stream, n, a, b, c.red.p);
+    double h[DOT_BLOCKS];
+    VFVM_HIP_TRY(hipMemcpyAsync(h, c.red.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+    double s = 0.0;
+    for (int k = 0; k < DOT_BLOCKS; ++k) s += h[k];
+    *out = s;
+    return VFVM_OK;
+}
+
+// sum of x over the pressure rows but the reference one
+int psum(Ctx& c, const double* x, double* out) {
+    VFVM_HIP_TRY(c.red.grow(DOT_BLOCKS));
+    const int S = (int)(c.n / c.g.N);
+    const int64_t pref_v = c.fs.pref_row >= 0 ? c.fs.pref_row - 3 * c.g.N : -1;
+    hipLaunchKernelGGL(k_psum_part, dim3(DOT_BLOCKS), dim3(BLK), 0, c.stream, c.g.N, S, pref_v, x, c.red.p);
     double h[DOT_BLOCKS];
     VFVM_HIP_TRY(hipMemcpyAsync(h, c.red.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
     VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
@@ -592,10 +714,45 @@ struct OuterStats {
     double rel = 0.0, t_inner = 0.0;
 };
 
+// Coarse correction of the transposed solves for the reference-pressure mode.  The pressure
+// is fixed by one identity row, so A 1_p = e_pref and the system restricted to the other
+// unknowns is nearly singular: its near-null left vector is the constant pressure Y (the
+// pressure rows but the reference one) and its near-null right vector W is the mass
+// balance, the sum of the continuity rows but the reference one (the face fluxes of two
+// neighbours cancel), corrected on the nodes' velocity rows (identity rows of A), so that
+// W^T A is non-zero only next to the reference vertex.  A transposed solve A^T z = r then
+// takes z = a W + z~, a = (Y . r) / (Y . A^T W), A^T z~ = r - a A^T W, whose right-hand side
+// has no component along Y: the Krylov solve never has to build the global component a W.
+struct Deflation {
+    bool on = false;
+    DevBuf<double> W, qW;
+    double E = 0.0;
+};
+
+int deflation_setup(Ctx& c, Deflation& d) {
+    d.on = false;
+    const int64_t n = c.n, N = c.g.N, NP = c.g.NP;
+    if (c.fs.pref_row < 0) return VFVM_OK;
+    VFVM_HIP_TRY(d.W.grow(n));
+    VFVM_HIP_TRY(d.qW.grow(n));
+    const SysView tv = transpose_view(c);
+    const int S = (int)(n / N);
+    hipLaunchKernelGGL(k_cont_indicator, dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, S, c.fs.pref_row - 3 * N, d.W.p);
+    if (int st = spmv_apply(c, tv, d.W.p, d.qW.p)) return st;  // column sums of the continuity rows
+    if (N > NP)
+        hipLaunchKernelGGL(k_mass_correct, dim3(grid_for(3 * (N - NP))), dim3(BLK), 0, c.stream, N, NP, S, (const double*)d.qW.p, d.W.p);
+    if (int st = spmv_apply(c, tv, d.W.p, d.qW.p)) return st;
+    VFVM_HIP_TRY(hipGetLastError());
+    if (int st = psum(c, d.qW.p, &d.E)) return st;
+    d.on = d.E != 0.0 && std::isfinite(d.E);
+    return VFVM_OK;
+}
+
 // GCR on J^T l = s, preconditioned by the transposed GMRES on A^T (flexible: each direction
 // z = M^-1 r is its own inexact solve).  l holds the initial guess on entry.
-int outer_solve(Ctx& c, AdjWork& w, const FlowAdjArgs& p, const vfvm_flow_adjoint_opts& o, const double* s, double* l,
-                DevBuf<double>& ZQ, DevBuf<double>& r, DevBuf<double>& q, OuterStats& os) {
+int outer_solve(Ctx& c, AdjWork& w, const FlowAdjArgs& p, const vfvm_flow_adjoint_opts& o, const Deflation& dfl, const double* s,
+                double* l, DevBuf<double>& ZQ, DevBuf<double>& r, DevBuf<double>& q, OuterStats& os) {
+    bool setup_done = false;  // the transposed system's FP32 values and preconditioner of this state
     const int64_t n = c.n;
     const unsigned gn = grid_for(n);
     const int m = o.restart;
@@ -620,18 +777,27 @@ int outer_solve(Ctx& c, AdjWork& w, const FlowAdjArgs& p, const vfvm_flow_adjoin
         if (!std::isfinite(os.rel)) return set_error(VFVM_ERR_LINEAR_NON_FINITE, "flow adjoint: non-finite outer residual");
         if (os.rel <= o.tol) return VFVM_OK;
         if (os.outer >= o.max_iter) break;
-        // z = M^-1 r: A^T z = r from z = 0
+        // z = M^-1 r: A^T z = r from z = 0, with the coarse correction of the reference mode
         double* z = Z + (size_t)kept * n;
         double* qk = Q + (size_t)kept * n;
         VFVM_HIP_TRY(hipMemcpyAsync(c.tb.p, r.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        double a = 0.0;
+        if (dfl.on) {
+            double yr = 0.0;
+            if (int st = psum(c, r.p, &yr)) return st;
+            a = yr / dfl.E;
+            hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(BLK), 0, c.stream, n, -a, (const double*)dfl.qW.p, c.tb.p);
+        }
         hipLaunchKernelGGL(k_zero_n, dim3(gn), dim3(BLK), 0, c.stream, n, z);
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         auto t0 = clk::now();
         vfvm_solve_stats ss;
-        const int gs = gmres_solve(c, transpose_view(c), o.inner, z, ss);
+        const int gs = gmres_solve_inner(c, transpose_view(c), o.inner, z, ss, setup_done);
+        setup_done = true;
         os.t_inner += secs(t0, clk::now());
         os.inner_steps += ss.iterations;
         if (gs != VFVM_OK && gs != VFVM_ERR_LINEAR_DID_NOT_CONVERGE) return gs;  // an inexact direction still serves
+        if (dfl.on) hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(BLK), 0, c.stream, n, a, (const double*)dfl.W.p, z);
         // q_k = J^T z, orthogonalised against the kept directions (modified Gram-Schmidt)
         if (int st = jt_apply(c, w, p, z, qk)) return st;
         for (int j = 0; j < kept; ++j) {
@@ -710,20 +876,25 @@ extern "C" int vfvm_flow_jt_apply(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nop
     CTX_GUARD(ctx);
     if (!x || !lam || !y || (dt > 0.0 && !x_prev)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "flow J^T apply: NULL argument");
     if (int st = check_state_args(c, ops, nops, dt)) return st;
-    if (int st = build_transpose(c)) return st;
+    if (int st = build_transpose(c, true)) return st;
     if (int st = build_rev(c)) return st;
     if (int st = build_gc_transpose(c)) return st;
-    DevBuf<double> dx, dxp, dl, dy;
+    DevBuf<double> dx, dxp, dl, dy, tmp;
     if (int st = upload_states(c, x, dt > 0.0 ? x_prev : nullptr, dx, dxp)) return st;
-    VFVM_HIP_TRY(dl.upload(lam, c.n, c.stream));
+    const int S = (int)(c.n / c.g.N);
+    VFVM_HIP_TRY(tmp.upload(lam, c.n, c.stream));
+    VFVM_HIP_TRY(dl.alloc(c.n));
     VFVM_HIP_TRY(dy.alloc(c.n));
+    hipLaunchKernelGGL(k_to_il, dim3(grid_for(c.n)), dim3(BLK), 0, c.stream, c.g.N, S, (const double*)tmp.p, dl.p);
     AdjWork w;
     if (int st = w.alloc(c)) return st;
     FlowAdjArgs p;
     const std::vector<vfvm_op> ol(ops, ops + nops);
     if (int st = load_state(c, ol, dt, dt > 0.0 ? dxp.p : nullptr, dx.p, w, p)) return st;
     if (int st = jt_apply(c, w, p, dl.p, dy.p)) return st;
-    VFVM_HIP_TRY(hipMemcpyAsync(y, dy.p, c.n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    hipLaunchKernelGGL(k_from_il, dim3(grid_for(c.n)), dim3(BLK), 0, c.stream, c.g.N, S, (const double*)dy.p, tmp.p);
+    VFVM_HIP_TRY(hipGetLastError());
+    VFVM_HIP_TRY(hipMemcpyAsync(y, tmp.p, c.n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     if (int st = sync_err(c)) return st;
     return clear_error();
 }
@@ -857,14 +1028,17 @@ extern "C" int vfvm_flow_adjoint_backward(vfvm_ctx* ctx, const vfvm_flow_adjoint
         }
     }
     auto ts0 = clk::now();
-    if (int st = build_transpose(c)) return st;
+    if (int st = build_transpose(c, true)) return st;
     if (int st = build_rev(c)) return st;
     if (int st = build_gc_transpose(c)) return st;
     AdjWork w;
     if (int st = w.alloc(c)) return st;
     const int m = opts->restart;
+    const int S = (int)(n / N);
     DevBuf<double> dseed, lam, lamn, lam2, s, r, q, ZQ, D, lsum;
-    VFVM_HIP_TRY(dseed.upload(seed, n, c.stream));
+    VFVM_HIP_TRY(q.upload(seed, n, c.stream));
+    VFVM_HIP_TRY(dseed.alloc(n));
+    hipLaunchKernelGGL(k_to_il, dim3(grid_for(n)), dim3(BLK), 0, c.stream, N, S, (const double*)q.p, dseed.p);
     VFVM_HIP_TRY(lam.alloc(n)); VFVM_HIP_TRY(lamn.alloc(n)); VFVM_HIP_TRY(lam2.alloc(n));
     VFVM_HIP_TRY(s.alloc(n)); VFVM_HIP_TRY(r.alloc(n)); VFVM_HIP_TRY(q.alloc(n));
     VFVM_HIP_TRY(ZQ.alloc((size_t)2 * m * n));
@@ -876,6 +1050,7 @@ extern "C" int vfvm_flow_adjoint_backward(vfvm_ctx* ctx, const vfvm_flow_adjoint
     auto t0 = clk::now();
     tp.stats.setup_seconds += secs(ts0, t0);
 
+    Deflation dfl;
     double t_rec = 0.0, t_adj = 0.0, max_rel = 0.0;
     int rec_iters = 0, rec_picard = 0, rec_steps = 0, outer_total = 0, inner_total = 0;
     int64_t mismatches = 0;
@@ -885,6 +1060,7 @@ extern "C" int vfvm_flow_adjoint_backward(vfvm_ctx* ctx, const vfvm_flow_adjoint
     auto adjoint_state = [&](const double* xp, const double* x, int32_t step) -> int {
         FlowAdjArgs p;
         if (int st = load_state(c, ops, tp.dt, xp, x, w, p)) return st;
+        if (int st = deflation_setup(c, dfl)) return st;
         if (tp.num_ts > 0 && !have_D) {
             hipLaunchKernelGGL(k_adj_time_diag, dim3(grid_for(N)), dim3(BLK), 0, c.stream, p, tp.dt, D.p);
             have_D = true;
@@ -899,11 +1075,12 @@ extern "C" int vfvm_flow_adjoint_backward(vfvm_ctx* ctx, const vfvm_flow_adjoint
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         auto a0 = clk::now();
         OuterStats os;
-        if (int st = outer_solve(c, w, p, *opts, s.p, lam.p, ZQ, r, q, os)) return st;
+        if (int st = outer_solve(c, w, p, *opts, dfl, s.p, lam.p, ZQ, r, q, os)) return st;
         t_adj += secs(a0, clk::now());
         outer_total += os.outer;
         inner_total += os.inner_steps;
         max_rel = std::max(max_rel, os.rel);
+        if (opts->on_state) opts->on_state(opts->user, step, os.outer, os.inner_steps, os.rel);
         hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(BLK), 0, c.stream, n, 1.0, (const double*)lam.p, lsum.p);
         VFVM_HIP_TRY(hipGetLastError());
         std::swap(lam2.p, lamn.p);
@@ -966,14 +1143,14 @@ extern "C" int vfvm_flow_adjoint_backward(vfvm_ctx* ctx, const vfvm_flow_adjoint
             for (size_t k = 0; k < c.sblk.size(); ++k)
                 for (const vfvm_op& op : c.sblk[k].ops)
                     if (op.kind == VFVM_OP_SCL_BND_DIRICHLET && op.a == wr.id)
-                        for (int64_t j = 0; j < cnt; ++j) wr.grad[j] += h_ls[(4 + k) * N + NP + g.bnd_nid[off + j]];
+                        for (int64_t j = 0; j < cnt; ++j) wr.grad[j] += h_ls[(int64_t)S * (NP + g.bnd_nid[off + j]) + 4 + k];
         } else {
             const int32_t b = c.vbndv[wr.id].bnd;
             const int64_t off = g.bnd_ptr[b], cnt = g.bnd_ptr[b + 1] - off;
             std::fill(wr.grad, wr.grad + 3 * cnt, 0.0);
             if (b < (int32_t)c.fs.vbnd_val.size() && c.fs.vbnd_val[b] == wr.id)
                 for (int cc = 0; cc < 3; ++cc)
-                    for (int64_t j = 0; j < cnt; ++j) wr.grad[cc * cnt + j] += h_ls[cc * N + NP + g.bnd_nid[off + j]];
+                    for (int64_t j = 0; j < cnt; ++j) wr.grad[cc * cnt + j] += h_ls[(int64_t)S * (NP + g.bnd_nid[off + j]) + cc];
         }
     }
     // leave the final state in the fields

@@ -142,6 +142,55 @@ __device__ inline T wave_sum(T s) {
     for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
     return s;
 }
+
+// SpMV of a generic CSR numbered vertex-interleaved (rows S v .. S v + S - 1 of vertex v
+// contiguous in the CSR, e.g. the transposed coupled flow system): one wave per vertex streams
+// the vertex's whole entry range [rp[S v], rp[S v + S]) with coalesced column / value loads
+// (each lane every 64th entry, two in flight) and adds each product to the accumulator of its
+// row; the S sums are then reduced across the wave in a fixed butterfly order.  Rows of very
+// different lengths (velocity, pressure and scalar rows of one vertex) share one wave instead
+// of each leaving lanes idle.
+template <class T, int S>
+__global__ void __launch_bounds__(BLK) k_spmv_vblock(int64_t nv, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                     const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
+                                                     const double* __restrict__ scale, const int* __restrict__ ctrl) {
+    const int64_t tid = (int64_t)xcd_block() * BLK + threadIdx.x;
+    const int64_t v = tid / WAVE;
+    const int lane = (int)(tid % WAVE);
+    if (v >= nv) return;
+    const int go = ctrl_go(ctrl);
+    int32_t b[S + 1];
+#pragma unroll
+    for (int c = 0; c <= S; ++c) b[c] = rp[S * v + c];
+    const T sc = (T)scale_of(scale);
+    if (ctrl_stop(go, (b[0] <= b[S]) | (sc != sc))) return;
+    T acc[S];
+#pragma unroll
+    for (int c = 0; c < S; ++c) acc[c] = 0;
+    int32_t k = b[0] + lane;
+    for (; k + WAVE < b[S]; k += 2 * WAVE) {
+        const int32_t c0 = col[k], c1 = col[k + WAVE];
+        const T a0 = val[k], a1 = val[k + WAVE];
+        const T p0 = a0 * x[c0], p1 = a1 * x[c1];
+#pragma unroll
+        for (int c = 0; c < S; ++c) {
+            acc[c] += (k >= b[c] && k < b[c + 1]) ? p0 : (T)0;
+            acc[c] += (k + WAVE >= b[c] && k + WAVE < b[c + 1]) ? p1 : (T)0;
+        }
+    }
+    if (k < b[S]) {
+        const T p0 = val[k] * x[col[k]];
+#pragma unroll
+        for (int c = 0; c < S; ++c) acc[c] += (k >= b[c] && k < b[c + 1]) ? p0 : (T)0;
+    }
+#pragma unroll
+    for (int c = 0; c < S; ++c) acc[c] = wave_sum<T>(acc[c]);
+    T out = acc[0];
+#pragma unroll
+    for (int c = 1; c < S; ++c) out = lane == c ? acc[c] : out;
+    if (lane < S) y[S * v + lane] = (T)(out * sc);
+}
+
 // position of unknown (component c, vertex m) in a system of S unknowns per vertex:
 // comp-major c N + m, or vertex-interleaved S m + c
 template <bool IL, int S>
@@ -359,6 +408,20 @@ __global__ void __launch_bounds__(BLK) k_residual(int64_t n, const int32_t* __re
     double t1 = block_sum(rr, sh);
     double t2 = block_sum(bb, sh);
     if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[nblk + blockIdx.x] = t2; }
+}
+
+// r = b (the residual of x = 0) and the partial sums of r^2 and b^2 in k_residual's layout
+__global__ void __launch_bounds__(BLK) k_residual_zero(int64_t n, const double* __restrict__ b, double* __restrict__ r,
+                                                       double* __restrict__ part, int nblk) {
+    __shared__ double sh[BLK / WAVE];
+    double rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+        const double bi = b[i];
+        r[i] = bi;
+        rr += bi * bi;
+    }
+    const double t = block_sum(rr, sh);
+    if (threadIdx.x == 0) { part[blockIdx.x] = t; part[nblk + blockIdx.x] = t; }
 }
 
 // 16-byte vector of T (4 floats or 2 doubles) for the basis-vector streams
@@ -818,7 +881,7 @@ __global__ void __launch_bounds__(BLK) k_xupdate_set(int64_t n, const double* __
 // gathered from the CSR rows by binary search (columns ascending), inverted by Gauss-Jordan
 // elimination with partial pivoting in registers (compile-time indices only), and stored as
 // inv[(i NB + j) N + v]; a singular or non-finite block becomes the identity.
-template <int NB>
+template <int NB, bool IL>
 __global__ void __launch_bounds__(BLK) k_bjac_setup(int64_t N, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                     const double* __restrict__ val, double* __restrict__ inv,
                                                     float* __restrict__ inv32) {
@@ -827,11 +890,11 @@ __global__ void __launch_bounds__(BLK) k_bjac_setup(int64_t N, const int32_t* __
     double a[NB][2 * NB];
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-        const int64_t row = (int64_t)i * N + v;
+        const int64_t row = IL ? (int64_t)NB * v + i : (int64_t)i * N + v;
         const int32_t s = rp[row], e = rp[row + 1];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            const int32_t target = (int32_t)((int64_t)j * N + v);
+            const int32_t target = (int32_t)(IL ? (int64_t)NB * v + j : (int64_t)j * N + v);
             int32_t lo = s, hi = e;
             while (lo < hi) {
                 const int32_t mid = (lo + hi) >> 1;
@@ -1037,6 +1100,21 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
             default: VFVM_FLOW_SPMV(2) break;
         }
 #undef VFVM_FLOW_SPMV
+        VFVM_HIP_TRY(hipGetLastError());
+        return VFVM_OK;
+    }
+    if (sy.interleaved && c.g.N > 0 && n % c.g.N == 0) {
+        const int64_t nv = c.g.N;
+        const dim3 vg(grid_for(nv * WAVE));
+        switch (n / nv) {
+            case 1: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_vblock<T, 1>, vg, dim3(BLK), nv, rp, cl, val, x, y, scale, ctrl); break;
+            case 2: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_vblock<T, 2>, vg, dim3(BLK), nv, rp, cl, val, x, y, scale, ctrl); break;
+            case 3: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_vblock<T, 3>, vg, dim3(BLK), nv, rp, cl, val, x, y, scale, ctrl); break;
+            case 4: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_vblock<T, 4>, vg, dim3(BLK), nv, rp, cl, val, x, y, scale, ctrl); break;
+            case 5: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_vblock<T, 5>, vg, dim3(BLK), nv, rp, cl, val, x, y, scale, ctrl); break;
+            case 6: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv_vblock<T, 6>, vg, dim3(BLK), nv, rp, cl, val, x, y, scale, ctrl); break;
+            default: return set_error(VFVM_ERR_INVALID_ARGUMENT, "SpMV: 1 to 6 unknowns per vertex");
+        }
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }
@@ -1250,14 +1328,18 @@ int bjac_setup(Ctx& c, const SysView& sy, bool with32) {
         i32 = c.bjac.inv32.p;
     }
     const dim3 grid(grid_for(N));
+#define VFVM_BJS(NB)                                                                                                             \
+    case NB:                                                                                                                     \
+        if (sy.interleaved) hipLaunchKernelGGL((k_bjac_setup<NB, true>), grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, \
+                                               c.bjac.inv64.p, i32);                                                             \
+        else hipLaunchKernelGGL((k_bjac_setup<NB, false>), grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, \
+                                i32);                                                                                            \
+        break;
     switch (nb) {
-        case 1: hipLaunchKernelGGL(k_bjac_setup<1>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
-        case 2: hipLaunchKernelGGL(k_bjac_setup<2>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
-        case 3: hipLaunchKernelGGL(k_bjac_setup<3>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
-        case 4: hipLaunchKernelGGL(k_bjac_setup<4>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
-        case 5: hipLaunchKernelGGL(k_bjac_setup<5>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
-        default: hipLaunchKernelGGL(k_bjac_setup<6>, grid, dim3(BLK), 0, c.stream, N, sy.row, sy.col, sy.vals, c.bjac.inv64.p, i32); break;
+        VFVM_BJS(1) VFVM_BJS(2) VFVM_BJS(3) VFVM_BJS(4) VFVM_BJS(5)
+        default: VFVM_BJS(6)
     }
+#undef VFVM_BJS
     VFVM_HIP_TRY(hipGetLastError());
     return VFVM_OK;
 }
@@ -1424,6 +1506,7 @@ SysView transpose_view(Ctx& c) {
     s.vals = c.tvals.p; s.b = c.tb.p;
     s.h_row = &c.h_trow; s.h_col = &c.h_tcol;
     s.ilu = &c.tilu;
+    s.interleaved = c.t_interleaved;
     return s;
 }
 
@@ -1541,6 +1624,10 @@ namespace {
 // graph capture or instantiation, no events) and supports FP64 with Jacobi or no preconditioner.
 struct GmresBufs {
     bool noalloc = false;
+    // an inner solve of an outer iteration: x0 = 0 (the first residual is b), the FP32 matrix
+    // and the preconditioner kept from the previous solve of the same values, and no residual
+    // after the last allowed cycle (the caller takes the iterate as it is)
+    bool x0_zero = false, reuse_setup = false, no_final_residual = false;
     double* S = nullptr;
     int* ctrl = nullptr;
     double *part = nullptr, *r64 = nullptr, *dinv64 = nullptr, *w64 = nullptr, *z64 = nullptr, *V64 = nullptr,
@@ -1580,6 +1667,37 @@ int gmres_solve(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0
     const int64_t n = sy0.n;
     const GmresSizes z = gmres_sizes(n, sy0.nnz, o.restart);
     GmresBufs B;
+    VFVM_HIP_TRY(c.scal.alloc((size_t)z.state));
+    B.S = c.scal.p;
+    B.ctrl = reinterpret_cast<int*>(c.scal.p + (z.state - 64));
+    VFVM_HIP_TRY(c.part.alloc((size_t)z.part));
+    B.part = c.part.p;
+    VFVM_HIP_TRY(c.r64.alloc(n));
+    VFVM_HIP_TRY(c.dinv64.alloc(n));
+    VFVM_HIP_TRY(c.w64.alloc(n));
+    VFVM_HIP_TRY(c.z64.alloc(n));
+    B.r64 = c.r64.p; B.dinv64 = c.dinv64.p; B.w64 = c.w64.p; B.z64 = c.z64.p;
+    if (!mixed) {
+        VFVM_HIP_TRY(c.V64.alloc((size_t)z.ldv * (o.restart + 1)));
+        B.V64 = c.V64.p;
+    } else {
+        VFVM_HIP_TRY(c.V32.alloc((size_t)z.ldv * (o.restart + 1)));
+        VFVM_HIP_TRY(c.w32.alloc(2 * (size_t)n));
+        B.V32 = c.V32.p; B.w32 = c.w32.p;
+    }
+    return gmres_run(c, sy0, o, x0, st, B);
+}
+
+int gmres_solve_inner(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, vfvm_solve_stats& st, bool reuse_setup) {
+    std::memset(&st, 0, sizeof(st));
+    if (o.restart <= 0 || o.restart > MAX_RESTART) return set_error(VFVM_ERR_LINEAR_NON_POSITIVE_RESTART, "GMRES: restart must be in [1, 256]");
+    const bool mixed = o.precision == VFVM_FP32_REFINED;
+    const int64_t n = sy0.n;
+    const GmresSizes z = gmres_sizes(n, sy0.nnz, o.restart);
+    GmresBufs B;
+    B.x0_zero = true;
+    B.reuse_setup = reuse_setup;
+    B.no_final_residual = true;
     VFVM_HIP_TRY(c.scal.alloc((size_t)z.state));
     B.S = c.scal.p;
     B.ctrl = reinterpret_cast<int*>(c.scal.p + (z.state - 64));
@@ -1751,7 +1869,8 @@ int gmres_run(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, 
                 VFVM_HIP_TRY(c.avals64.alloc(sy.nnz));
                 B.avals64 = c.avals64.p;
             }
-            hipLaunchKernelGGL((k_scale_cols<double>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, sy.col, sy.vals, dinv64, B.avals64);
+            if (!B.reuse_setup)
+                hipLaunchKernelGGL((k_scale_cols<double>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, sy.col, sy.vals, dinv64, B.avals64);
             aval = B.avals64;
         } else {
             aval = sy.vals;
@@ -1759,31 +1878,34 @@ int gmres_run(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, 
         if (ilu) {
             VFVM_HIP_TRY(c.ilu64.alloc(sy.nnz));
             VFVM_HIP_TRY(c.iluf64.alloc(sy.nnz));
-            if (int s = ilu_factor<double>(c, sy0, c.ilu64.p, c.iluf64.p, sy0.vals)) return s;
+            if (!B.reuse_setup)
+                if (int s = ilu_factor<double>(c, sy0, c.ilu64.p, c.iluf64.p, sy0.vals)) return s;
             lu = c.ilu64.p;
         }
     } else {
         // FP32 copy of A D^-1 (Jacobi) or of A (ILU(0), none); dinv = 1 unless Jacobi
         VFVM_HIP_TRY(c.avals32.alloc(sy.nnz));
         B.avals32 = c.avals32.p;
-        hipLaunchKernelGGL((k_scale_cols<float>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, sy.col, sy.vals, dinv64, B.avals32);
+        if (!B.reuse_setup)
+            hipLaunchKernelGGL((k_scale_cols<float>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, sy.col, sy.vals, dinv64, B.avals32);
         aval = B.avals32;
         if (ilu) {
             VFVM_HIP_TRY(c.ilu32.alloc(sy.nnz));
             VFVM_HIP_TRY(c.iluf32.alloc(sy.nnz));
-            if (int s = ilu_factor<float>(c, sy0, c.ilu32.p, c.iluf32.p, sy0.vals)) return s;
+            if (!B.reuse_setup)
+                if (int s = ilu_factor<float>(c, sy0, c.ilu32.p, c.iluf32.p, sy0.vals)) return s;
             lu = c.ilu32.p;
         }
     }
-    if (bjac)
+    if (bjac && !B.reuse_setup)
         if (int s = bjac_setup(c, sy, mixed)) return s;
     Pc<double> pc64;
     Pc<float> pc32;
     pc64.lu = (const double*)(mixed ? nullptr : lu);
     pc32.lu = (const float*)(mixed ? lu : nullptr);
     if (bjac) {
-        pc64.bj = c.bjac.inv64.p; pc64.nb = c.bjac.nb; pc64.nv = c.g.N; pc64.il = il;
-        pc32.bj = c.bjac.inv32.p; pc32.nb = c.bjac.nb; pc32.nv = c.g.N; pc32.il = il;
+        pc64.bj = c.bjac.inv64.p; pc64.nb = c.bjac.nb; pc64.nv = c.g.N; pc64.il = il || sy0.interleaved;
+        pc32.bj = c.bjac.inv32.p; pc32.nb = c.bjac.nb; pc32.nv = c.g.N; pc32.il = il || sy0.interleaved;
     }
     VFVM_HIP_TRY(hipGetLastError());
     if (!B.noalloc) VFVM_HIP_TRY(hipEventRecord(ev.e[1], c.stream));
@@ -1799,8 +1921,10 @@ int gmres_run(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, 
     const int max_ref = o.max_refine > 0 ? o.max_refine : o.max_outer;
     const bool graphs = !B.noalloc;
     for (;;) {
-        // true FP64 residual r = b - A x, beta = ||r||
-        if (int s = residual_launch(c, sy, lpr, x, r64, part_r, nblk_r)) return s;
+        // true FP64 residual r = b - A x, beta = ||r|| (r = b for a zero initial guess)
+        if (B.x0_zero && cycles == 0)
+            hipLaunchKernelGGL(k_residual_zero, dim3(nblk_r), dim3(BLK), 0, c.stream, n, (const double*)sy.b, r64, part_r, nblk_r);
+        else if (int s = residual_launch(c, sy, lpr, x, r64, part_r, nblk_r)) return s;
         hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(BLK), 0, c.stream, nblk_r, part_r, w.S, w.L, w.ctrl, 0.0, 0);
         VFVM_HIP_TRY(hipMemcpyAsync(c.h_pinned + 8, w.S + w.L.misc(), 8 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
@@ -1856,6 +1980,10 @@ int gmres_run(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, 
         VFVM_HIP_TRY(hipGetLastError());
         total_steps += steps;
         ++cycles;
+        if (B.no_final_residual && (cycles >= o.max_outer || (mixed && refinements >= max_ref))) {
+            rel = -1.0;  // not evaluated
+            break;
+        }
     }
     if (ilu) hipLaunchKernelGGL((k_scatter_perm<double>), dim3(grid_for(n)), dim3(BLK), 0, c.stream, n, sy0.ilu->perm.p, (const double*)x, x0);
     if (il)

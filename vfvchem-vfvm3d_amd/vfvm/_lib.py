@@ -109,8 +109,12 @@ class AdjointWrt(C.Structure):
     _fields_ = [("kind", C.c_int32), ("id", C.c_int32), ("grad", C.POINTER(C.c_double))]
 
 
+ADJOINT_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double)
+
+
 class FlowAdjointOpts(C.Structure):
-    _fields_ = [("inner", SolveOpts), ("tol", C.c_double), ("max_iter", C.c_int32), ("restart", C.c_int32)]
+    _fields_ = [("inner", SolveOpts), ("tol", C.c_double), ("max_iter", C.c_int32), ("restart", C.c_int32),
+                ("on_state", ADJOINT_HOOK), ("user", C.c_void_p)]
 
 
 class FlowAdjointStats(C.Structure):

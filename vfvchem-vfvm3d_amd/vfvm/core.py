@@ -538,10 +538,11 @@ class Context:
         self.system_info()
         return st
 
-    def flow_adjoint_backward(self, seed, wrt, weights=None, inner=None, tol=1e-10, max_iter=100, restart=20):
+    def flow_adjoint_backward(self, seed, wrt, weights=None, inner=None, tol=1e-10, max_iter=100, restart=20, on_state=None):
         """dJ/dtheta for the last flow_adjoint_forward: wrt is a list of (WRT_SCLBND, id) or
         (WRT_VECBND, id); returns (list of gradients, stats).  inner: options of the transposed
-        solves that precondition the outer GCR on J^T."""
+        solves that precondition the outer GCR on J^T.  on_state(step, outer, inner_steps,
+        rel_residual) is called after each adjoint state."""
         n = self._size()
         seed = np.ascontiguousarray(seed, np.float64)
         if seed.shape != (n,):
@@ -557,6 +558,8 @@ class Context:
         o = L.FlowAdjointOpts()
         o.inner = solve_opts(**(inner or {}))
         o.tol, o.max_iter, o.restart = float(tol), int(max_iter), int(restart)
+        cb = L.ADJOINT_HOOK(lambda _u, s_, a_, b_, r_: on_state(s_, a_, b_, r_)) if on_state else L.ADJOINT_HOOK()
+        o.on_state = cb
         st = L.FlowAdjointStats()
         check(lib.vfvm_flow_adjoint_backward(self._h, C.byref(o), ptr(seed), None if wt is None else ptr(wt), len(wrt), arr,
                                              C.byref(st)))
