@@ -97,6 +97,9 @@ typedef struct vfvm_mesh_info {
     int32_t num_bnd;      /* boundary groups                                         */
     int32_t max_degree;
     int64_t num_grad_coeff; /* pressure-gradient coefficients (0 until built)         */
+    int32_t num_dom;      /* domains (1 unless joined by vfvm_mesh_join)                 */
+    int32_t num_itf;      /* interfaces                                                  */
+    int64_t num_itf_nodes; /* interface nodes over all interfaces                         */
 } vfvm_mesh_info;
 
 /* Jittered unit-cube lattice of n^3 vertices (h = 1/(n-1)), jitter uniform in
@@ -127,6 +130,27 @@ int vfvm_mesh_from_tets_dev(int64_t num_pts, int64_t num_nds, const double* pos,
                             int64_t num_tets, const int32_t* tets,
                             int32_t num_bnd, const int64_t* bnd_ptr, const int32_t* bnd_nid,
                             int32_t device, vfvm_mesh** out);
+
+/* The vertices (i,j,k) of the n^3 jittered lattice above with lo <= (i,j,k) <= hi (a box of the
+ * unit cube, at least 3 vertices per axis), Voronoi dual on GPU `device` (host if device < 0).
+ * A coordinate on a face of the box is the unjittered lattice plane, so boxes that share a
+ * face place its vertices identically; boundaries "x0".."z1" are the box's faces with the
+ * ownership of vfvm_mesh_lattice.  lo = 0, hi = n - 1 gives vfvm_mesh_lattice's mesh.  Builds
+ * the domains of a multi-domain lattice (Variables::new_obj + add_dom, src/base/vars.rs:50-69,
+ * for box-shaped domains). */
+int vfvm_mesh_lattice_box(int32_t n, uint64_t seed, double jitter, const int32_t* lo, const int32_t* hi,
+                          int32_t device, vfvm_mesh** out);
+/* Joins num_dom meshes (each with its dual) into one multi-domain mesh: domain d's vertices are
+ * [dom_ptr[d], dom_ptr[d+1]) in the domain's own order (points, then nodes), so an unknown over
+ * the joined vertices is the reference's per-domain unknown scalars with their blocks in domain
+ * order (SteadyBase::initial_start); each domain keeps its own dual (block-diagonal neighbour
+ * CSR).  itf: num_itf quadruples (dom_a, bnd_a, dom_b, bnd_b): boundary bnd_a of dom_a and bnd_b
+ * of dom_b hold the same nodes (equal coordinates) and become interface itf (Interface::new,
+ * src/base/geom_itf.rs:19-49; side a = dom_a, whose node order the interface keeps); the other
+ * boundaries keep their order, domain by domain.  In the joined mesh num_pts = 0 and node ids
+ * are vertex ids.  Flow operators and pressure-gradient coefficients are not supported on it. */
+int vfvm_mesh_join(int32_t num_dom, const vfvm_mesh* const* doms, int32_t num_itf, const int32_t* itf,
+                   vfvm_mesh** out);
 
 int vfvm_mesh_info_get(const vfvm_mesh* mesh, vfvm_mesh_info* info);
 
@@ -162,7 +186,13 @@ enum vfvm_mesh_array {
     VFVM_MESH_BSF_WGT = 18,  /* f64 [3S] SoA                         */
     VFVM_MESH_GC_PTR = 19,   /* i32 [N+1] pressure-gradient coefficients, CSR over vertices */
     VFVM_MESH_GC_VID = 20,   /* i32 [G] vertex m (the 2-ring of v, ascending)            */
-    VFVM_MESH_GC_C = 21      /* f64 [3G] SoA (c_x, c_y, c_z)                              */
+    VFVM_MESH_GC_C = 21,     /* f64 [3G] SoA (c_x, c_y, c_z)                              */
+    VFVM_MESH_DOM_PTR = 22,  /* i64 [num_dom+1] first vertex of each domain                  */
+    VFVM_MESH_DOM_NPTS = 23, /* i64 [num_dom] points of each domain (its first vertices)     */
+    VFVM_MESH_ITF_PTR = 24,  /* i64 [num_itf+1] interface node ranges                        */
+    VFVM_MESH_ITF_VA = 25,   /* i32 [num_itf_nodes] vertex of the node's side-a copy          */
+    VFVM_MESH_ITF_VB = 26,   /* i32 [num_itf_nodes] vertex of the node's side-b copy          */
+    VFVM_MESH_ITF_DOM = 27   /* i32 [2 num_itf] domains of side a and side b                  */
 };
 int vfvm_mesh_get(const vfvm_mesh* mesh, int32_t array_id, void* dst);
 
@@ -271,7 +301,23 @@ enum vfvm_op_kind {
     /* flow boundary operators; geom_id = bnd_id */
     VFVM_OP_VEC_BND_DIRICHLET = 60,       /* OpVecBndDirichlet{vel,val}           opvec_bnd_dir.rs:56     unk = vel, a = vecbnd value */
     VFVM_OP_SCL_BND_DIVERGENCE = 61,      /* OpSclBndDivergence{vel,pres,den}     opscl_bnd_div.rs:73     unk = pres, a = vel, b = den */
-    VFVM_OP_SCL_BND_PRESSURE_STABILIZER = 62 /* OpSclBndPressureStabilizer{pres,den,fce,flw} opscl_bnd_pstab.rs:49 unk = pres, a = den, b = fce, c = flw */
+    VFVM_OP_SCL_BND_PRESSURE_STABILIZER = 62, /* OpSclBndPressureStabilizer{pres,den,fce,flw} opscl_bnd_pstab.rs:49 unk = pres, a = den, b = fce, c = flw */
+    /* scalar interface operators on a joined mesh (vfvm_mesh_join); geom_id = itf.  The fields
+     * are joined-mesh fields: `unk` holds the reference's (unk_a, unk_b) pair (its values on
+     * side a's and side b's domains), `a` the (coef_a, coef_b) pair.  The continuity operator
+     * fills the rows of side a's node copies, every other one the rows of side b's copies (the
+     * combined control volume); no domain or boundary operator touches an interface node. */
+    VFVM_OP_SCL_ITF_CONTINUITY = 70,      /* OpSclItfContinuity{itf,unk_a,unk_b}   opscl_itf_cont.rs:38-58                         */
+    VFVM_OP_SCL_ITF_DIFFUSION = 71,       /* OpSclItfDiffusion{itf,unk,diff,drv}   opscl_itf_diff.rs:49-113  a = diff, b = drv (= unk) */
+    VFVM_OP_SCL_ITF_PRODUCTION = 72,      /* OpSclItfProduction{itf,unk,prd}       opscl_itf_prod.rs:46-96   a = prd               */
+    VFVM_OP_SCL_ITF_SOURCE = 73,          /* OpSclItfSource{itf,unk,src}           opscl_itf_src.rs:44-87    a = src               */
+    VFVM_OP_SCL_ITF_TIME = 74,            /* OpSclItfTime{itf,unk,wgt}             opscl_itf_time.rs:42-80   a = wgt (apply_time)  */
+    VFVM_OP_SCL_ITF_TIME_PREVIOUS = 75,   /* OpSclItfTimePrevious{itf,unk,wgt}     opscl_itf_timeprev.rs:42-82 a = wgt (its previous values) */
+    VFVM_OP_SCL_ITF_TIME_UNITY = 76,      /* OpSclItfTimeUnity{itf,unk}            opscl_itf_timeunity.rs:35-73                    */
+    VFVM_OP_SCL_ITF_ADVECTION = 77,       /* OpSclItfAdvection{itf,unk,wgt,diff,den,flw} opscl_itf_adv.rs:61-150 a = wgt, b = diff, c = flw
+                                             (den = the flow's), value = flowing side (0: a, 1: b) */
+    VFVM_OP_SCL_ITF_ADVECTION_UNITY = 78  /* OpSclItfAdvectionUnity{itf,unk,diff?,flw} opscl_itf_advunity.rs:59-137 a = diff or -1,
+                                             b = flw, value = flowing side (0: a, 1: b) */
 };
 typedef struct vfvm_op {
     int32_t kind;     /* vfvm_op_kind                                         */

@@ -50,7 +50,7 @@ import numpy as np
 
 __all__ = [
     "dual", "update_grad", "update_grad_vec", "flux_prescribed", "assemble",
-    "coo_to_csr", "Op",
+    "coo_to_csr", "Op", "assemble_interface", "assemble_domains",
 ]
 
 
@@ -592,6 +592,145 @@ def coo_to_csr(rows, cols, vals, n):
     row_ptr = np.zeros(n + 1, np.int64)
     np.add.at(row_ptr, ur + 1, 1)
     return np.cumsum(row_ptr), uc.astype(np.int64), out
+
+
+# --------------------------------------------------------------------------- interfaces
+# Multi-domain systems (Variables with several domains, src/base/vars.rs:7-34): each domain
+# has its own dual (its own Tetrahedron) and its own unknown scalar, whose block starts where
+# the previous domain's ends (SteadyBase::initial_start); an interface node has one vertex in
+# each of its two domains.  The interface operators (src/operator/core/opscl_itf_*.rs) load
+# the continuity row into side a's rows and the combined control volume into side b's rows.
+
+ITF_CONTINUITY, ITF_DIFFUSION, ITF_PRODUCTION, ITF_SOURCE = 70, 71, 72, 73
+ITF_TIME, ITF_TIME_PREVIOUS, ITF_TIME_UNITY, ITF_ADVECTION, ITF_ADVECTION_UNITY = 74, 75, 76, 77, 78
+
+
+def assemble_interface(doms, starts, itf, ops, fields, dt=0.0, value_prev=None, flux=None, flow_den=None):
+    """Triplets (push order) and b contributions of the scalar interface operators.
+
+    doms: per-domain dual() dicts; starts: first row of each domain's unknown block;
+    itf: (dom_a, dom_b, va, vb) with va[q], vb[q] the local vertex of node q in domain a / b
+    (the interface's node order, geom_itf.rs:37-44); ops: Op list (kinds 70-78, value = flowing
+    side 0/1 of the advection operators); fields[id][d] = (value, grad[3]) on domain d;
+    value_prev[id][d]; flux[flw][d] = face flux of domain d's directed edges; flow_den[flw][d] =
+    (rho, grad rho).  Returns rows, cols, vals, b_rows, b_vals in global numbering."""
+    da, db, va, vb = itf
+    va = np.asarray(va, np.int64)
+    vb = np.asarray(vb, np.int64)
+    sides = ((da, va), (db, vb))
+    row_a = starts[da] + va
+    row_b = starts[db] + vb
+    R, C, V, BR, BV = [], [], [], [], []
+
+    def face_edges(g, vids):
+        # directed edges of the vertices vids in (node, loc) order, with the node index of each
+        rp = g["row_ptr"]
+        cnt = rp[vids + 1] - rp[vids]
+        q = np.repeat(np.arange(vids.size), cnt)
+        e = np.concatenate([np.arange(rp[v], rp[v + 1]) for v in vids]) if vids.size else np.zeros(0, np.int64)
+        return q, e.astype(np.int64)
+
+    for op in ops:
+        k = op.kind
+        if k == ITF_CONTINUITY:
+            # opscl_itf_cont.rs:44-56: (a_v, a_v) +1, (a_v, b_v) -1 per node
+            R.append(np.stack([row_a, row_a], 1).ravel())
+            C.append(np.stack([row_a, starts[db] + vb], 1).ravel())
+            V.append(np.tile([1.0, -1.0], va.size))
+            continue
+        for si, (d, vids) in enumerate(sides):
+            g = doms[d]
+            s0 = starts[d]
+            if k in (ITF_DIFFUSION, ITF_ADVECTION, ITF_ADVECTION_UNITY):
+                if k != ITF_DIFFUSION and si != int(op.value):
+                    continue
+                q, e = face_edges(g, vids)
+                v_, n_ = g["rows"][e], g["ngh"][e]
+                skew = g["skew"][:, e]
+                cond_of = lambda fid: (_diff_face(*fields[fid][d], v_, n_, skew) * g["area"][e] / g["dist"][e])
+                if k == ITF_DIFFUSION:
+                    # opscl_itf_diff.rs:73-108: both sides' faces, rows of side b
+                    coeff = cond_of(op.a)
+                elif k == ITF_ADVECTION_UNITY:
+                    # opscl_itf_advunity.rs:85-135: the flowing side's faces
+                    cond = cond_of(op.a) if op.a >= 0 else np.zeros(e.size)
+                    coeff = _adv_coeff(cond, flux[op.b][d][e])
+                else:
+                    # opscl_itf_adv.rs:86-148: flux = m_f / rho_f F
+                    sx, sy, sz = skew
+                    W, Wg = fields[op.a][d]
+                    Rr, Rg = flow_den[op.c][d]
+                    wgt_f = 0.5 * (W[v_] + W[n_]) + 0.5 * (sx * (Wg[0][v_] + Wg[0][n_]) + sy * (Wg[1][v_] + Wg[1][n_]) + sz * (Wg[2][v_] + Wg[2][n_]))
+                    den_f = 0.5 * (Rr[v_] + Rr[n_]) + 0.5 * (sx * (Rg[0][v_] + Rg[0][n_]) + sy * (Rg[1][v_] + Rg[1][n_]) + sz * (Rg[2][v_] + Rg[2][n_]))
+                    with np.errstate(divide="ignore", invalid="ignore"):
+                        F = np.where(den_f != 0.0, wgt_f / den_f * flux[op.c][d][e], 0.0)
+                    coeff = _adv_coeff(cond_of(op.b), F)
+                rb = row_b[q]
+                R.append(np.stack([rb, rb], 1).ravel())
+                C.append(np.stack([s0 + n_, s0 + v_], 1).ravel())
+                V.append(np.stack([coeff, -coeff], 1).ravel())
+            elif k == ITF_PRODUCTION:
+                # opscl_itf_prod.rs:60-92: both halves on (b_v, b_v)
+                P, Pg = fields[op.a][d]
+                _, Ug = fields[op.unk_id][d]
+                sx, sy, sz = g["cskew"][:, vids]
+                vol = g["vol"][vids]
+                prd_c = P[vids] + Pg[0][vids] * sx + Pg[1][vids] * sy + Pg[2][vids] * sz
+                unk_skew = Ug[0][vids] * sx + Ug[1][vids] * sy + Ug[2][vids] * sz
+                R.append(row_b); C.append(row_b); V.append(prd_c * vol)
+                BR.append(row_b); BV.append(-P[vids] * unk_skew * vol)
+            elif k == ITF_SOURCE:
+                # opscl_itf_src.rs:57-83
+                S, Sg = fields[op.a][d]
+                sx, sy, sz = g["cskew"][:, vids]
+                src_c = S[vids] + Sg[0][vids] * sx + Sg[1][vids] * sy + Sg[2][vids] * sz
+                BR.append(row_b); BV.append(-src_c * g["vol"][vids])
+            elif k in (ITF_TIME, ITF_TIME_PREVIOUS, ITF_TIME_UNITY):
+                # opscl_itf_time.rs:50-76, opscl_itf_timeprev.rs:50-78, opscl_itf_timeunity.rs:47-70:
+                # each side's term on that side's copy of the unknown
+                if not dt > 0.0:
+                    raise ValueError("OperatorUseApplyTime")
+                vol = g["vol"][vids]
+                prev = value_prev[op.unk_id][d][vids]
+                if k == ITF_TIME_UNITY:
+                    w = np.ones(vids.size)
+                elif k == ITF_TIME:
+                    w = fields[op.a][d][0][vids]
+                else:
+                    w = value_prev[op.a][d][vids]
+                R.append(row_b); C.append(s0 + vids); V.append(-w * vol / dt)
+                BR.append(row_b); BV.append(-w * prev * vol / dt)
+            else:
+                raise ValueError(f"unsupported interface operator kind {k}")
+    cat = lambda xs, dt_: np.concatenate(xs).astype(dt_) if xs else np.zeros(0, dt_)
+    return cat(R, np.int64), cat(C, np.int64), cat(V, np.float64), cat(BR, np.int64), cat(BV, np.float64)
+
+
+def assemble_domains(doms, bnd_nids, dom_ops, itfs, itf_ops, fields, sclbnd, dt=0.0, value_prev=None, flux=None,
+                     flow_den=None):
+    """The whole multi-domain system: per domain d, assemble() of its domain and boundary
+    operators dom_ops[d] on its own dual with its fields (fields[id][d]), shifted to its block;
+    then the interface operators itf_ops[k] of every interface itfs[k].  Returns the CSR
+    (row_ptr, col, val), b and the block starts."""
+    starts = np.concatenate([[0], np.cumsum([g["N"] for g in doms])]).astype(np.int64)
+    Ntot = int(starts[-1])
+    R, C, V = [], [], []
+    b = np.zeros(Ntot)
+    for d, g in enumerate(doms):
+        scl_d = {fid: per[d] for fid, per in fields.items() if d in per}
+        prev_d = {fid: per[d] for fid, per in (value_prev or {}).items() if d in per}
+        flux_d = {f: per[d] for f, per in (flux or {}).items() if d in per}
+        den_d = {f: per[d] for f, per in (flow_den or {}).items() if d in per}
+        r, c, v, bd = assemble(g, dom_ops[d], scl_d, sclbnd[d], flux_d, bnd_nids[d], dt=dt, value_prev=prev_d,
+                               flow_den=den_d)
+        R.append(r + starts[d]); C.append(c + starts[d]); V.append(v)
+        b[starts[d]:starts[d + 1]] += bd
+    for itf, ops in zip(itfs, itf_ops):
+        r, c, v, br, bv = assemble_interface(doms, starts, itf, ops, fields, dt, value_prev, flux, flow_den)
+        R.append(r); C.append(c); V.append(v)
+        np.add.at(b, br, bv)
+    rp, col, val = coo_to_csr(np.concatenate(R), np.concatenate(C), np.concatenate(V), Ntot)
+    return rp, col, val, b, starts
 
 
 # --------------------------------------------------------------------------- CPU GMRES (C)

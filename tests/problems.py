@@ -683,3 +683,252 @@ def suction_profile(mesh, U=1.0, V=1.0, mu=0.2, rho=1.0):
     p = FlowProblem(mesh, np.zeros((3, N)), 0.0, rho, mu, np.zeros((3, N)), bnd, pref=(0, 0.0))
     p.uex, p.pex = uex, np.zeros(N)
     return p
+
+
+# ------------------------------------------------------------------ multi-domain systems
+class MultiProblem:
+    """Fields and operators of a multi-domain system (several domains joined by interfaces):
+    the joined mesh and joined-mesh fields for the device path, the per-domain duals, fields and
+    operator lists for the oracle (assemble_domains), which restates the reference's per-domain
+    variables and its interface operators."""
+
+    def __init__(self, parts, itfs):
+        self.parts = list(parts)
+        self.itfs = [tuple(f) for f in itfs]
+        self.mesh = vfvm.Mesh.join(self.parts, self.itfs)
+        self.N = self.mesh.num_tot
+        self.pos = self.mesh.get("pos")
+        self.dom_ptr = self.mesh.get("dom_ptr")
+        side = {(f[0], f[1]) for f in self.itfs} | {(f[2], f[3]) for f in self.itfs}
+        self.jbnd = {}  # (domain, local boundary) -> joined boundary id
+        for d, m in enumerate(self.parts):
+            for b in range(m.num_bnd):
+                if (d, b) not in side:
+                    self.jbnd[(d, b)] = len(self.jbnd)
+        self.lbnd = {j: db for db, j in self.jbnd.items()}
+        self.scl, self.sbnd, self.vec, self.flw, self.ops = [], [], [], [], []
+
+    def dom_slice(self, d):
+        return slice(int(self.dom_ptr[d]), int(self.dom_ptr[d + 1]))
+
+    def scldom(self, kind, per_dom):
+        """per_dom: one value (or array over the domain's vertices) per domain, or a callable
+        of the joined positions."""
+        if callable(per_dom):
+            v = np.asarray(per_dom(self.pos), np.float64)
+        else:
+            v = np.concatenate([np.broadcast_to(np.asarray(x, np.float64), (m.num_tot,)) for x, m in zip(per_dom, self.parts)])
+        self.scl.append((kind, v.copy()))
+        return len(self.scl) - 1
+
+    def sclbnd(self, d, b, values):
+        j = self.jbnd[(d, b)]
+        cnt = len(self.parts[d].bnd_nodes(b))
+        self.sbnd.append((j, np.broadcast_to(np.asarray(values, np.float64), (cnt,)).copy()))
+        return len(self.sbnd) - 1
+
+    def vecdom(self, kind, values):
+        self.vec.append((kind, np.asarray(values, np.float64).reshape(3, self.N).copy()))
+        return len(self.vec) - 1
+
+    def sclflw(self, vel, pres, den, visc, fce):
+        self.flw.append((vel, pres, den, visc, fce))
+        return len(self.flw) - 1
+
+    def to_device(self, ctx):
+        ctx.upload(self.mesh)
+        for kind, v in self.scl:
+            ctx.scldom_add(kind, 0.0, v)
+        for j, v in self.sbnd:
+            ctx.sclbnd_add(j, 0.0, v)
+        for kind, v in self.vec:
+            ctx.vecdom_add(kind, values=v)
+        for f in self.flw:
+            ctx.sclflw_add(*f)
+        return ctx
+
+    # ---- oracle
+    def oracle_system(self, dt=0.0, value_prev=None, op_filter=None, unknown_value=None):
+        """CSR (row_ptr, col, val), b and block starts of the oracle's per-domain assembly.
+        value_prev: field id -> joined-mesh previous values; unknown_value: the unknown's
+        values (joined mesh) the operators read (lagged terms of a Picard iteration)."""
+        if getattr(self, "doms", None) is None:
+            self.doms = [O.dual(m.get("pos"), m.get("tets"), m.num_pts) for m in self.parts]
+        doms = self.doms
+        sl = [self.dom_slice(d) for d in range(len(self.parts))]
+        fields = {}
+        for i, (kind, v) in enumerate(self.scl):
+            if unknown_value is not None and kind == vfvm.FIELD_UNKNOWN:
+                v = np.asarray(unknown_value, np.float64)
+            fields[i] = {}
+            for d, g in enumerate(doms):
+                vd = v[sl[d]]
+                gr = O.update_grad(g, vd) if kind != vfvm.FIELD_CONSTANT else np.zeros((3, g["N"]))
+                fields[i][d] = (vd, gr)
+        flux, den = {}, {}
+        for i, (vel, pres, dn, visc, fce) in enumerate(self.flw):
+            vk, vv = self.vec[vel]
+            flux[i], den[i] = {}, {}
+            for d, g in enumerate(doms):
+                vd = vv[:, sl[d]]
+                vg = O.update_grad_vec(g, vd) if vk != vfvm.FIELD_CONSTANT else np.zeros((9, g["N"]))
+                flux[i][d] = O.flux_prescribed(g, vd, vg, fields[dn][d][0], fields[dn][d][1])
+                den[i][d] = fields[dn][d]
+        prev = None
+        if value_prev is not None:
+            prev = {fid: {d: np.asarray(v, np.float64)[sl[d]] for d in range(len(doms))} for fid, v in value_prev.items()}
+        sclbnd = [{i: v for i, (j, v) in enumerate(self.sbnd) if self.lbnd[j][0] == d} for d in range(len(doms))]
+        bnd_nids = [[m.bnd_nodes(b) for b in range(m.num_bnd)] for m in self.parts]
+        dom_ops = [[] for _ in doms]
+        itf_ops = [[] for _ in self.itfs]
+        for o in self.ops:
+            if op_filter is not None and not op_filter(o):
+                continue
+            kind, geom, unk, a, b, c = o[:6]
+            value = o[6] if len(o) > 6 else 0.0
+            if kind < 20:
+                dom_ops[geom].append(O.Op(kind, 0, unk, a, b, c, value))
+            elif kind < 40:
+                d, lb = self.lbnd[geom]
+                dom_ops[d].append(O.Op(kind, lb, unk, a, b, c, value))
+            elif kind >= 70:
+                itf_ops[geom].append(O.Op(kind, geom, unk, a, b, c, value))
+            else:
+                raise ValueError(f"operator kind {kind} on a joined mesh")
+        va, vb = self.mesh.get("itf_va"), self.mesh.get("itf_vb")
+        iptr = self.mesh.get("itf_ptr")
+        itfs = []
+        for k, (da, _, db, _) in enumerate(self.itfs):
+            q = slice(int(iptr[k]), int(iptr[k + 1]))
+            itfs.append((da, db, va[q].astype(np.int64) - self.dom_ptr[da], vb[q].astype(np.int64) - self.dom_ptr[db]))
+        return O.assemble_domains(doms, bnd_nids, dom_ops, itfs, itf_ops, fields, sclbnd, dt=dt, value_prev=prev,
+                                  flux=flux, flow_den=den)
+
+
+def slab_domains(n, splits, seed=0, jitter=0.3, device=None):
+    """The n^3 lattice cut at the x lattice planes `splits` into boxes joined by interfaces:
+    domain d spans i in [s_d, s_(d+1)]; interface k joins domain k's x1 face (side a) to
+    domain k+1's x0 face (side b)."""
+    cuts = [0] + list(splits) + [n - 1]
+    parts = [vfvm.Mesh.lattice_box(n, (cuts[d], 0, 0), (cuts[d + 1], n - 1, n - 1), seed, jitter, device=device)
+             for d in range(len(cuts) - 1)]
+    itfs = [(k, X1, k + 1, X0) for k in range(len(cuts) - 2)]
+    return parts, itfs
+
+
+def itf_heat(n, splits=None, k=(1.0, 3.0), q=0.0, prd=0.0, seed=0, device=None):
+    """Steady heat through slabs of different conductivity (SteadyHeat with add_hcnt_itf,
+    steady_heat.rs:100-171, as feat_itf.rs sets it up): T = 0 on the first slab's x = 0 face,
+    T = 1 on the last slab's x = 1 face, zero flux on the other outer faces, temperature and
+    flux continuity at every interface.  k, q, prd: one value per slab or callables of the
+    joined positions."""
+    if splits is None:
+        splits = [(n - 1) // 2]
+    parts, itfs = slab_domains(n, splits, seed, device=device)
+    p = MultiProblem(parts, itfs)
+    nd = len(parts)
+    per = lambda x: x if callable(x) else (list(x) if np.ndim(x) else [x] * nd)
+    kind = lambda x: vfvm.FIELD_FUNCTION if callable(x) else vfvm.FIELD_CONSTANT
+    T = p.scldom(vfvm.FIELD_UNKNOWN, [0.0] * nd)
+    kk = p.scldom(kind(k), per(k))
+    hp = p.scldom(kind(prd), per(prd))
+    hs = p.scldom(kind(q), per(q))
+    ops = [(vfvm.OP_SCL_DOM_DIFFUSION, d, T, kk, T, -1) for d in range(nd)]
+    for d in range(nd):
+        ops += [(vfvm.OP_SCL_DOM_PRODUCTION, d, T, hp, -1, -1), (vfvm.OP_SCL_DOM_SOURCE, d, T, hs, -1, -1)]
+    ops.append((vfvm.OP_SCL_BND_DIRICHLET, p.jbnd[(0, X0)], T, p.sclbnd(0, X0, 0.0), -1, -1))
+    ops.append((vfvm.OP_SCL_BND_DIRICHLET, p.jbnd[(nd - 1, X1)], T, p.sclbnd(nd - 1, X1, 1.0), -1, -1))
+    for d in range(nd):
+        for b in (Y0, Y1, Z0, Z1):
+            j = p.jbnd[(d, b)]
+            ops += [(vfvm.OP_SCL_BND_DIFFUSION, j, T, kk, T, -1), (vfvm.OP_SCL_BND_PRODUCTION, j, T, hp, -1, -1),
+                    (vfvm.OP_SCL_BND_SOURCE, j, T, hs, -1, -1), (vfvm.OP_SCL_BND_NEUMANN, j, T, p.sclbnd(d, b, 0.0), -1, -1)]
+    for i in range(len(itfs)):
+        ops += [(vfvm.OP_SCL_ITF_CONTINUITY, i, T, -1, -1, -1), (vfvm.OP_SCL_ITF_DIFFUSION, i, T, kk, T, -1),
+                (vfvm.OP_SCL_ITF_PRODUCTION, i, T, hp, -1, -1), (vfvm.OP_SCL_ITF_SOURCE, i, T, hs, -1, -1)]
+    p.ops = ops
+    p.unknown = T
+    p.fields = dict(k=kk, prd=hp, src=hs)
+    return p
+
+
+def itf_advdiff(n, Pe, splits=None, seed=0, device=None, weighted=False):
+    """The 1D convection-diffusion profile c = (exp(Pe x) - 1) / (exp(Pe) - 1) through slabs
+    joined by interfaces (SteadyMassFlow with add_mcnt_itf and a prescribed flow in every slab,
+    steady_massflow.rs:300-345): uniform u = (Pe, 0, 0), unit diffusivity, the interface
+    advection on both sides (the flow crosses the interface).  weighted: the temperature form
+    (OpSclDomAdvection / OpSclItfAdvection with weight = density, SteadyHeatFlow)."""
+    if splits is None:
+        splits = [(n - 1) // 2]
+    parts, itfs = slab_domains(n, splits, seed, device=device)
+    p = MultiProblem(parts, itfs)
+    nd = len(parts)
+    c = p.scldom(vfvm.FIELD_UNKNOWN, [0.0] * nd)
+    D = p.scldom(vfvm.FIELD_CONSTANT, [1.0] * nd)
+    den = p.scldom(vfvm.FIELD_CONSTANT, [1.0] * nd)
+    visc = p.scldom(vfvm.FIELD_CONSTANT, [1.0] * nd)
+    pres = p.scldom(vfvm.FIELD_CONSTANT, [0.0] * nd)
+    vel = p.vecdom(vfvm.FIELD_CONSTANT, np.stack([np.full(p.N, float(Pe)), np.zeros(p.N), np.zeros(p.N)]))
+    fce = p.vecdom(vfvm.FIELD_CONSTANT, np.zeros((3, p.N)))
+    f = p.sclflw(vel, pres, den, visc, fce)
+    if weighted:
+        adv = lambda kind, g: (kind, g, c, den, D, f)
+        dom_adv, bnd_adv, itf_adv = vfvm.OP_SCL_DOM_ADVECTION, vfvm.OP_SCL_BND_ADVECTION, vfvm.OP_SCL_ITF_ADVECTION
+    else:
+        adv = lambda kind, g: (kind, g, c, D, f, -1)
+        dom_adv, bnd_adv, itf_adv = vfvm.OP_SCL_DOM_ADVECTION_UNITY, vfvm.OP_SCL_BND_ADVECTION_UNITY, vfvm.OP_SCL_ITF_ADVECTION_UNITY
+    ops = []
+    for d in range(nd):
+        ops += [(vfvm.OP_SCL_DOM_DIFFUSION, d, c, D, c, -1), adv(dom_adv, d)]
+    ops.append((vfvm.OP_SCL_BND_DIRICHLET, p.jbnd[(0, X0)], c, p.sclbnd(0, X0, 0.0), -1, -1))
+    ops.append((vfvm.OP_SCL_BND_DIRICHLET, p.jbnd[(nd - 1, X1)], c, p.sclbnd(nd - 1, X1, 1.0), -1, -1))
+    for d in range(nd):
+        for b in (Y0, Y1, Z0, Z1):
+            j = p.jbnd[(d, b)]
+            ops += [(vfvm.OP_SCL_BND_DIFFUSION, j, c, D, c, -1), (vfvm.OP_SCL_BND_NEUMANN, j, c, p.sclbnd(d, b, 0.0), -1, -1),
+                    adv(bnd_adv, j)]
+    for i in range(len(itfs)):
+        ops += [(vfvm.OP_SCL_ITF_CONTINUITY, i, c, -1, -1, -1), (vfvm.OP_SCL_ITF_DIFFUSION, i, c, D, c, -1)]
+        for side in (0, 1):
+            ops.append(adv(itf_adv, i) + (float(side),))
+    p.ops = ops
+    p.unknown = c
+    p.exact = np.expm1(Pe * p.pos[:, 0]) / np.expm1(Pe)
+    return p
+
+
+def itf_transient(n, m=(1.0, 2.5), s=(-0.5, -2.0), dt=0.1, splits=None, seed=0, device=None, previous=False):
+    """One implicit-Euler step of m dc/dt = s c on slabs with different m and s per slab and no
+    exchange between control volumes (zero conductivity): every control volume integrates the
+    ODE by itself, so an
+    interface node's combined volume gives c = c_prev / (1 - dt (s_a V_a + s_b V_b) /
+    (m_a V_a + m_b V_b)) (TransientHeat with add_hcnt_itf: time, diffusion, production per
+    slab and interface).  previous: the time-previous operators (the weight's previous values)."""
+    if splits is None:
+        splits = [(n - 1) // 2]
+    parts, itfs = slab_domains(n, splits, seed, device=device)
+    p = MultiProblem(parts, itfs)
+    nd = len(parts)
+    c = p.scldom(vfvm.FIELD_UNKNOWN, [1.0] * nd)
+    w = p.scldom(vfvm.FIELD_CONSTANT, list(m))
+    k = p.scldom(vfvm.FIELD_CONSTANT, [0.0] * nd)  # no exchange between control volumes
+    sp_ = p.scldom(vfvm.FIELD_CONSTANT, list(s))
+    tk = (vfvm.OP_SCL_DOM_TIME_PREVIOUS, vfvm.OP_SCL_BND_TIME_PREVIOUS, vfvm.OP_SCL_ITF_TIME_PREVIOUS) if previous else \
+         (vfvm.OP_SCL_DOM_TIME, vfvm.OP_SCL_BND_TIME, vfvm.OP_SCL_ITF_TIME)
+    ops = []
+    for d in range(nd):
+        ops += [(tk[0], d, c, w, -1, -1), (vfvm.OP_SCL_DOM_DIFFUSION, d, c, k, c, -1), (vfvm.OP_SCL_DOM_PRODUCTION, d, c, sp_, -1, -1)]
+    for d in range(nd):
+        for b in range(6):
+            if (d, b) in p.jbnd:
+                j = p.jbnd[(d, b)]
+                ops += [(tk[1], j, c, w, -1, -1), (vfvm.OP_SCL_BND_DIFFUSION, j, c, k, c, -1),
+                        (vfvm.OP_SCL_BND_PRODUCTION, j, c, sp_, -1, -1), (vfvm.OP_SCL_BND_NEUMANN, j, c, p.sclbnd(d, b, 0.0), -1, -1)]
+    for i in range(len(itfs)):
+        ops += [(tk[2], i, c, w, -1, -1), (vfvm.OP_SCL_ITF_CONTINUITY, i, c, -1, -1, -1),
+                (vfvm.OP_SCL_ITF_DIFFUSION, i, c, k, c, -1), (vfvm.OP_SCL_ITF_PRODUCTION, i, c, sp_, -1, -1)]
+    p.ops = ops
+    p.unknown = c
+    p.dt = dt
+    p.m, p.s = tuple(m), tuple(s)
+    return p

@@ -244,6 +244,7 @@ bool is_kind(int k, int a, int b) { return k == a || k == b; }
 int adjoint_check_ops(Ctx& c, const vfvm_op* ops, int32_t nops) {
     if (!ops || nops <= 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "adjoint: empty operator list");
     if (is_flow_list(ops, nops)) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "adjoint: the flow system is nonlinear (outside the linear adjoint)");
+    if (c.g.multi) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "adjoint: joined meshes (interface operators) are outside the built adjoint");
     for (int32_t i = 0; i < nops; ++i) {
         const vfvm_op& op = ops[i];
         const std::string where = "adjoint: operator " + std::to_string(i) + ": ";

@@ -90,12 +90,14 @@ __global__ void k_picard(int64_t n, const double* __restrict__ xo, const double*
 
 bool is_dom_op(int k) { return k >= 1 && k < 20; }
 bool is_bnd_op(int k) { return k >= 20 && k < 40; }
+bool is_itf_op(int k) { return k >= VFVM_OP_SCL_ITF_CONTINUITY && k <= VFVM_OP_SCL_ITF_ADVECTION_UNITY; }
 
 struct Compiled {
     int nclass = 0;
     int32_t unk = -1;
     std::vector<std::vector<vfvm_op>> per_class;
     std::vector<uint8_t> has_edges, has_diag;
+    std::vector<uint8_t> itf_cols;  // interface classes: ITF_* bits of the columns their rows hold
     bool has_time = false;
 };
 
@@ -106,11 +108,13 @@ int check_scl(Ctx& c, int32_t id) { return id >= 0 && id < (int32_t)c.scl.size()
 int compile_ops(Ctx& c, const vfvm_op* ops, int32_t nops, Compiled& out) {
     if (!c.has_mesh) return set_error(VFVM_ERR_INVALID_ARGUMENT, "no mesh uploaded");
     if (nops <= 0 || !ops) return set_error(VFVM_ERR_INVALID_ARGUMENT, "empty operator list");
-    out.nclass = 1 + c.g.num_bnd;
+    const DevGeom& g = c.g;
+    out.nclass = g.cls_itf0 + 2 * g.num_itf;
     if (out.nclass > MAX_CLASSES) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "too many boundaries");
     out.per_class.assign(out.nclass, {});
     out.has_edges.assign(out.nclass, 0);
     out.has_diag.assign(out.nclass, 0);
+    out.itf_cols.assign(out.nclass, 0);
     for (int32_t i = 0; i < nops; ++i) {
         const vfvm_op& op = ops[i];
         if (!check_scl(c, op.unk_id)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "operator " + std::to_string(i) + ": unknown id out of range");
@@ -120,11 +124,14 @@ int compile_ops(Ctx& c, const vfvm_op* ops, int32_t nops, Compiled& out) {
         else if (out.unk != op.unk_id) return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "operators on more than one unknown are outside the built path");
         int cls;
         if (is_dom_op(op.kind)) {
-            if (op.geom_id != 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "domain operator: only domain 0 exists");
-            cls = 0;
+            if (op.geom_id < 0 || op.geom_id >= g.num_dom) return set_error(VFVM_ERR_INVALID_ARGUMENT, "domain operator: domain id out of range");
+            cls = op.geom_id;
         } else if (is_bnd_op(op.kind)) {
-            if (op.geom_id < 0 || op.geom_id >= c.g.num_bnd) return set_error(VFVM_ERR_INVALID_ARGUMENT, "boundary operator: boundary id out of range");
-            cls = 1 + op.geom_id;
+            if (op.geom_id < 0 || op.geom_id >= g.num_bnd) return set_error(VFVM_ERR_INVALID_ARGUMENT, "boundary operator: boundary id out of range");
+            cls = g.num_dom + op.geom_id;
+        } else if (is_itf_op(op.kind)) {
+            if (op.geom_id < 0 || op.geom_id >= g.num_itf) return set_error(VFVM_ERR_INVALID_ARGUMENT, "interface operator: interface id out of range");
+            cls = g.cls_itf0 + 2 * op.geom_id + (op.kind == VFVM_OP_SCL_ITF_CONTINUITY ? 0 : 1);
         } else {
             return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "unknown operator kind " + std::to_string(op.kind));
         }
@@ -176,6 +183,47 @@ int compile_ops(Ctx& c, const vfvm_op* ops, int32_t nops, Compiled& out) {
                     return set_error(VFVM_ERR_INVALID_ARGUMENT, "boundary value id out of range or on another boundary");
                 if (op.kind == VFVM_OP_SCL_BND_DIRICHLET) out.has_diag[cls] = 1;
                 break;
+            // interface operators (opscl_itf_*.rs): the columns their rows receive
+            case VFVM_OP_SCL_ITF_CONTINUITY:
+                out.itf_cols[cls] |= ITF_CA | ITF_CB;
+                break;
+            case VFVM_OP_SCL_ITF_DIFFUSION:
+                if (!check_scl(c, op.a)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "interface diffusion: coefficient id out of range");
+                if (op.b != op.unk_id) {
+                    if (check_scl(c, op.b) && c.scl[op.b].kind != VFVM_FIELD_UNKNOWN)
+                        return set_error(VFVM_ERR_OPERATOR_NON_UNKNOWN, "interface diffusion: driving field is not an unknown (OperatorNonUnknownInLinearSystem)");
+                    return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "interface diffusion: cross-diffusion into another unknown is outside the built path");
+                }
+                out.itf_cols[cls] |= ITF_EA | ITF_EB | ITF_CA | ITF_CB;
+                break;
+            case VFVM_OP_SCL_ITF_PRODUCTION:
+                if (!check_scl(c, op.a)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "interface production: coefficient id out of range");
+                out.itf_cols[cls] |= ITF_CB;
+                break;
+            case VFVM_OP_SCL_ITF_SOURCE:
+                if (!check_scl(c, op.a)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "interface source: id out of range");
+                break;
+            case VFVM_OP_SCL_ITF_TIME:
+            case VFVM_OP_SCL_ITF_TIME_PREVIOUS:
+                if (!check_scl(c, op.a)) return set_error(VFVM_ERR_INVALID_ARGUMENT, "interface time: weight id out of range");
+                out.itf_cols[cls] |= ITF_CA | ITF_CB;
+                out.has_time = true;
+                break;
+            case VFVM_OP_SCL_ITF_TIME_UNITY:
+                out.itf_cols[cls] |= ITF_CA | ITF_CB;
+                out.has_time = true;
+                break;
+            case VFVM_OP_SCL_ITF_ADVECTION:
+            case VFVM_OP_SCL_ITF_ADVECTION_UNITY: {
+                const bool w = op.kind == VFVM_OP_SCL_ITF_ADVECTION;
+                const int32_t fl = w ? op.c : op.b;
+                if (w ? (!check_scl(c, op.a) || !check_scl(c, op.b)) : (op.a >= 0 && !check_scl(c, op.a)))
+                    return set_error(VFVM_ERR_INVALID_ARGUMENT, "interface advection: weight or diffusion id out of range");
+                if (fl < 0 || fl >= (int32_t)c.flw.size()) return set_error(VFVM_ERR_INVALID_ARGUMENT, "interface advection: flow id out of range");
+                if (op.value != 0.0 && op.value != 1.0)
+                    return set_error(VFVM_ERR_INVALID_ARGUMENT, "interface advection: value must name the flowing side (0: a, 1: b)");
+                out.itf_cols[cls] |= op.value == 0.0 ? (ITF_EA | ITF_CA) : (ITF_EB | ITF_CB);
+            } break;
             default:
                 return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "unknown operator kind " + std::to_string(op.kind));
         }
@@ -222,6 +270,26 @@ int build_program(Ctx& c, const Compiled& cp, DevProgram& p) {
                     break;
                 case VFVM_OP_SCL_DOM_TIME_UNITY:
                 case VFVM_OP_SCL_BND_TIME_UNITY:
+                case VFVM_OP_SCL_ITF_TIME_UNITY:
+                case VFVM_OP_SCL_ITF_CONTINUITY:
+                    break;
+                case VFVM_OP_SCL_ITF_TIME_PREVIOUS:
+                    d.f_val = c.scl[op.a].value_prev.p;
+                    break;
+                case VFVM_OP_SCL_ITF_ADVECTION: {
+                    const SclFlw& fl = c.flw[op.c];
+                    d.has_diff = 1;
+                    d.f_val = c.scl[op.b].value.p; d.f_grad = c.scl[op.b].grad.p;
+                    d.w_val = c.scl[op.a].value.p; d.w_grad = c.scl[op.a].grad.p;
+                    d.r_val = c.scl[fl.den].value.p; d.r_grad = c.scl[fl.den].grad.p;
+                    d.flux = fl.flux.p;
+                    d.side = op.value == 0.0 ? 0 : 1;
+                } break;
+                case VFVM_OP_SCL_ITF_ADVECTION_UNITY:
+                    d.has_diff = op.a >= 0;
+                    if (op.a >= 0) { d.f_val = c.scl[op.a].value.p; d.f_grad = c.scl[op.a].grad.p; }
+                    d.flux = c.flw[op.b].flux.p;
+                    d.side = op.value == 0.0 ? 0 : 1;
                     break;
                 default:
                     d.f_val = c.scl[op.a].value.p;
@@ -333,6 +401,41 @@ extern "C" int vfvm_mesh_upload(vfvm_ctx* ctx, const vfvm_mesh* mesh) {
         }
     VFVM_HIP_TRY(g.vbnd.upload(vbnd.data(), vbnd.size(), s));
     VFVM_HIP_TRY(g.bidx.upload(bidx.data(), bidx.size(), s));
+    // operator classes: points of domain d -> d, boundary b -> num_dom + b, interface k's
+    // side-a / side-b copies -> cls_itf0 + 2k / + 2k + 1
+    g.num_dom = m.num_dom();
+    g.num_itf = m.num_itf();
+    g.multi = g.num_dom > 1 || g.num_itf > 0;
+    g.cls_itf0 = g.num_dom + g.num_bnd;
+    if (g.cls_itf0 + 2 * g.num_itf > MAX_CLASSES)
+        return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "vfvm_mesh_upload: more than " + std::to_string(MAX_CLASSES) +
+                                                        " domains + boundaries + 2 x interfaces");
+    // (a single-domain mesh treats a node on no boundary as a point, as the class of its rows
+    // always did)
+    g.h_vcls.assign(g.N, g.multi ? -1 : 0);
+    for (int32_t d = 0; d < g.num_dom; ++d) {
+        const int64_t v0 = m.dom_ptr.empty() ? 0 : m.dom_ptr[d];
+        const int64_t np = m.dom_npts.empty() ? m.num_pts : m.dom_npts[d];
+        for (int64_t v = v0; v < v0 + np; ++v) g.h_vcls[v] = d;
+    }
+    for (int64_t v = 0; v < g.N; ++v)
+        if (vbnd[v] >= 0) g.h_vcls[v] = g.num_dom + vbnd[v];
+    g.K = (int64_t)m.itf_va.size();
+    g.h_itf_va = m.itf_va;
+    g.h_itf_vb = m.itf_vb;
+    g.h_itf_of.assign(g.K, 0);
+    for (int32_t k = 0; k < g.num_itf; ++k)
+        for (int64_t q = m.itf_ptr[k]; q < m.itf_ptr[k + 1]; ++q) {
+            g.h_itf_of[q] = k;
+            g.h_vcls[m.itf_va[q]] = g.cls_itf0 + 2 * k;
+            g.h_vcls[m.itf_vb[q]] = g.cls_itf0 + 2 * k + 1;
+        }
+    for (int64_t v = 0; v < g.N; ++v)
+        if (g.h_vcls[v] < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_upload: node " + std::to_string(v) + " is on no boundary or interface");
+    VFVM_HIP_TRY(g.vcls.upload(g.h_vcls.data(), g.h_vcls.size(), s));
+    VFVM_HIP_TRY(g.itf_va.upload(g.h_itf_va.data(), g.h_itf_va.size(), s));
+    VFVM_HIP_TRY(g.itf_vb.upload(g.h_itf_vb.data(), g.h_itf_vb.size(), s));
+    VFVM_HIP_TRY(g.itf_of.upload(g.h_itf_of.data(), g.h_itf_of.size(), s));
     g.bnd_ptr = m.bnd_ptr;
     g.bnd_nid = m.bnd_nid;
     g.h_row_ptr = m.row_ptr;
@@ -543,16 +646,54 @@ extern "C" int vfvm_pattern_build(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nop
     // single unknown: block start 0 (SteadyHeat::initial_start, steady_heat.rs:68-78)
     c.unk = cp.unk;
     c.n = N;
-    std::vector<int32_t> vbnd_h(N, -1);
-    for (int32_t b = 0; b < g.num_bnd; ++b)
-        for (int64_t k = g.bnd_ptr[b]; k < g.bnd_ptr[b + 1]; ++k) vbnd_h[g.NP + g.bnd_nid[k]] = b;
     c.h_mrow.assign(N + 1, 0);
     c.h_mcol.clear();
     std::vector<int32_t> eslot(g.E, -1), dslot(N, -1);
+    // interface node of each copy (-1: none)
+    std::vector<int32_t> itf_q(g.multi ? N : 0, -1);
+    for (int64_t q = 0; q < g.K; ++q) itf_q[g.h_itf_va[q]] = itf_q[g.h_itf_vb[q]] = (int32_t)q;
+    std::vector<int32_t> islot(4 * g.K, -1);
+    std::vector<int32_t> cols;
     for (int64_t v = 0; v < N; ++v) {
-        const int cls = vbnd_h[v] < 0 ? 0 : 1 + vbnd_h[v];
+        const int cls = g.h_vcls[v];
         const int32_t es = g.h_row_ptr[v], ee = g.h_row_ptr[v + 1];
-        if (cp.has_edges[cls]) {
+        if (cls >= g.cls_itf0) {
+            // interface rows: columns of the operators on this side's class, ascending
+            // (opscl_itf_cont.rs:50-53 for side a; opscl_itf_diff.rs:73-108 and the other
+            // operators load into side b's rows)
+            const int32_t q = itf_q[v];
+            const int32_t va = g.h_itf_va[q], vb = g.h_itf_vb[q];
+            const uint8_t m = cp.itf_cols[cls];
+            const bool side_b = ((cls - g.cls_itf0) & 1) != 0;
+            cols.clear();
+            if (side_b) {
+                if (m & ITF_EA)
+                    for (int32_t e = g.h_row_ptr[va]; e < g.h_row_ptr[va + 1]; ++e) cols.push_back(g.h_ngh[e]);
+                if (m & ITF_EB)
+                    for (int32_t e = g.h_row_ptr[vb]; e < g.h_row_ptr[vb + 1]; ++e) cols.push_back(g.h_ngh[e]);
+            }
+            if (m & ITF_CA) cols.push_back(va);
+            if (m & ITF_CB) cols.push_back(vb);
+            std::sort(cols.begin(), cols.end());
+            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+            const int32_t base = (int32_t)c.h_mcol.size();
+            c.h_mcol.insert(c.h_mcol.end(), cols.begin(), cols.end());
+            auto slot = [&](int32_t col) -> int32_t {
+                auto it = std::lower_bound(cols.begin(), cols.end(), col);
+                return (it != cols.end() && *it == col) ? base + (int32_t)(it - cols.begin()) : -1;
+            };
+            if (side_b) {
+                islot[4 * q + 2] = slot(va);
+                islot[4 * q + 3] = slot(vb);
+                if (m & ITF_EA)
+                    for (int32_t e = g.h_row_ptr[va]; e < g.h_row_ptr[va + 1]; ++e) eslot[e] = slot(g.h_ngh[e]);
+                if (m & ITF_EB)
+                    for (int32_t e = g.h_row_ptr[vb]; e < g.h_row_ptr[vb + 1]; ++e) eslot[e] = slot(g.h_ngh[e]);
+            } else {
+                islot[4 * q + 0] = slot(va);
+                islot[4 * q + 1] = slot(vb);
+            }
+        } else if (cp.has_edges[cls]) {
             // {v} u ngh(v), ascending (opscl_dom_diff.rs:84-85)
             bool placed = false;
             for (int32_t e = es; e < ee; ++e) {
@@ -576,6 +717,7 @@ extern "C" int vfvm_pattern_build(vfvm_ctx* ctx, const vfvm_op* ops, int32_t nop
     VFVM_HIP_TRY(c.mcol.upload(c.h_mcol.data(), c.h_mcol.size(), s));
     VFVM_HIP_TRY(c.eslot.upload(eslot.data(), eslot.size(), s));
     VFVM_HIP_TRY(c.dslot.upload(dslot.data(), dslot.size(), s));
+    VFVM_HIP_TRY(c.islot.upload(islot.data(), islot.size(), s));
     VFVM_HIP_TRY(c.vals.alloc(std::max<int64_t>(c.nnz, 1)));
     VFVM_HIP_TRY(c.b.alloc(N));
     DevProgram p;
@@ -627,7 +769,16 @@ int vfvm::prepare_scalar_fields(Ctx& c, const vfvm_op* ops, int32_t nops, int32_
                 need_f[op.b] = 1; break;
             case VFVM_OP_SCL_DOM_ADVECTION:
             case VFVM_OP_SCL_BND_ADVECTION:
+            case VFVM_OP_SCL_ITF_ADVECTION:
                 need_s[op.a] = 1; need_s[op.b] = 1; need_f[op.c] = 1; break;
+            case VFVM_OP_SCL_ITF_PRODUCTION:
+                need_s[op.a] = 1; need_s[op.unk_id] = 1; break;
+            case VFVM_OP_SCL_ITF_ADVECTION_UNITY:
+                if (op.a >= 0) need_s[op.a] = 1;
+                need_f[op.b] = 1; break;
+            case VFVM_OP_SCL_ITF_CONTINUITY:
+            case VFVM_OP_SCL_ITF_TIME_PREVIOUS:
+            case VFVM_OP_SCL_ITF_TIME_UNITY:
             case VFVM_OP_SCL_BND_DIRICHLET:
             case VFVM_OP_SCL_BND_NEUMANN:
             case VFVM_OP_SCL_DOM_TIME_PREVIOUS:
@@ -681,7 +832,7 @@ int vfvm::assemble_impl(Ctx& c, const vfvm_op* ops, int32_t nops, double dt) {
     {
         Compiled pp;
         if (int st = compile_ops(c, c.pattern_ops.data(), (int32_t)c.pattern_ops.size(), pp)) return st;
-        if (pp.has_edges != cp.has_edges || pp.has_diag != cp.has_diag)
+        if (pp.has_edges != cp.has_edges || pp.has_diag != cp.has_diag || pp.itf_cols != cp.itf_cols)
             return set_error(VFVM_ERR_PATTERN_MISMATCH, "assemble: operators do not fit the built pattern");
     }
     if (cp.has_time && !(dt > 0.0)) return set_error(VFVM_ERR_OPERATOR_USE_APPLY_TIME, "time operators need dt > 0 (OperatorUseApplyTime)");

@@ -37,6 +37,36 @@ extern "C" int vfvm_mesh_lattice_dev(int32_t n, uint64_t seed, double jitter, in
     return mesh_lattice(n, seed, jitter, device, out);
 }
 
+extern "C" int vfvm_mesh_lattice_box(int32_t n, uint64_t seed, double jitter, const int32_t* lo, const int32_t* hi,
+                                     int32_t device, vfvm_mesh** out) {
+    if (!out || !lo || !hi) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_lattice_box: NULL argument");
+    *out = nullptr;
+    auto* h = new vfvm_mesh();
+    std::string err;
+    int st = lattice_box_mesh(n, seed, jitter, lo, hi, h->m, err);
+    if (st == VFVM_OK) st = make_dual(h->m, device, err);
+    if (st != VFVM_OK) { delete h; return set_error(st, err); }
+    *out = h;
+    return clear_error();
+}
+
+extern "C" int vfvm_mesh_join(int32_t num_dom, const vfvm_mesh* const* doms, int32_t num_itf, const int32_t* itf,
+                              vfvm_mesh** out) {
+    if (!out || !doms || num_dom < 1 || num_itf < 0 || (num_itf > 0 && !itf))
+        return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_join: invalid argument");
+    *out = nullptr;
+    std::vector<const Mesh*> parts;
+    for (int32_t d = 0; d < num_dom; ++d) parts.push_back(doms[d] ? &doms[d]->m : nullptr);
+    std::vector<JoinItf> fs;
+    for (int32_t k = 0; k < num_itf; ++k) fs.push_back({itf[4 * k], itf[4 * k + 1], itf[4 * k + 2], itf[4 * k + 3]});
+    auto* h = new vfvm_mesh();
+    std::string err;
+    int st = join_meshes(parts, fs, h->m, err);
+    if (st != VFVM_OK) { delete h; return set_error(st, err); }
+    *out = h;
+    return clear_error();
+}
+
 static int mesh_from_tets(int64_t num_pts, int64_t num_nds, const double* pos, int64_t num_tets, const int32_t* tets,
                           int32_t num_bnd, const int64_t* bnd_ptr, const int32_t* bnd_nid, int32_t dual_device,
                           vfvm_mesh** out) {
@@ -108,11 +138,16 @@ extern "C" int vfvm_mesh_info_get(const vfvm_mesh* mesh, vfvm_mesh_info* info) {
     for (int64_t v = 0; v + 1 < (int64_t)m.row_ptr.size(); ++v) md = std::max(md, m.row_ptr[v + 1] - m.row_ptr[v]);
     info->max_degree = md;
     info->num_grad_coeff = m.has_grad_coeff ? (int64_t)m.gc_vid.size() : 0;
+    info->num_dom = m.num_dom();
+    info->num_itf = m.num_itf();
+    info->num_itf_nodes = (int64_t)m.itf_va.size();
     return clear_error();
 }
 
 extern "C" int vfvm_mesh_build_grad_coeff(vfvm_mesh* mesh) {
     if (!mesh) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_build_grad_coeff: NULL mesh");
+    if (mesh->m.num_dom() > 1 || mesh->m.num_itf() > 0)
+        return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "pressure-gradient coefficients of a joined mesh (flow across interfaces) are outside the built path");
     std::string err;
     int st = build_grad_coeff(mesh->m, err);
     if (st != VFVM_OK) return set_error(st, err);
@@ -120,6 +155,8 @@ extern "C" int vfvm_mesh_build_grad_coeff(vfvm_mesh* mesh) {
 }
 extern "C" int vfvm_mesh_build_grad_coeff_dev(vfvm_mesh* mesh, int32_t device) {
     if (!mesh || device < 0) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_build_grad_coeff_dev: NULL mesh or negative device");
+    if (mesh->m.num_dom() > 1 || mesh->m.num_itf() > 0)
+        return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "pressure-gradient coefficients of a joined mesh (flow across interfaces) are outside the built path");
     std::string err;
     int st = build_grad_coeff_device(mesh->m, device, err);
     if (st != VFVM_OK) return set_error(st, err);
@@ -158,6 +195,12 @@ extern "C" int vfvm_mesh_get(const vfvm_mesh* mesh, int32_t id, void* dst) {
         case VFVM_MESH_GC_PTR: return copy_out(m.gc_ptr, dst);
         case VFVM_MESH_GC_VID: return copy_out(m.gc_vid, dst);
         case VFVM_MESH_GC_C: return copy_out(m.gc_c, dst);
+        case VFVM_MESH_DOM_PTR: return copy_out(m.dom_ptr.empty() ? std::vector<int64_t>{0, m.num_tot()} : m.dom_ptr, dst);
+        case VFVM_MESH_DOM_NPTS: return copy_out(m.dom_npts.empty() ? std::vector<int64_t>{m.num_pts} : m.dom_npts, dst);
+        case VFVM_MESH_ITF_PTR: return copy_out(m.itf_ptr.empty() ? std::vector<int64_t>{0} : m.itf_ptr, dst);
+        case VFVM_MESH_ITF_VA: return copy_out(m.itf_va, dst);
+        case VFVM_MESH_ITF_VB: return copy_out(m.itf_vb, dst);
+        case VFVM_MESH_ITF_DOM: return copy_out(m.itf_dom, dst);
         default: return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_mesh_get: unknown array id");
     }
 }

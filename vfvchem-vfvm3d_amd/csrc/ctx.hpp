@@ -61,6 +61,16 @@ struct DevGeom {
     DevBuf<int32_t> bsf_ptr, bsf_vj, bsf_vk;
     DevBuf<double> bsf_area, bsf_wgt, bsf_norm; // bsf_wgt, bsf_norm: 3S SoA
     DevBuf<int32_t> vbnd, bidx;                // vertex -> boundary (-1: point), index in boundary
+    // operator class of every vertex: the points of domain d are class d, boundary b is class
+    // num_dom + b, interface k's side-a / side-b node copies are classes cls_itf0 + 2k / + 2k + 1
+    DevBuf<int32_t> vcls;
+    std::vector<int32_t> h_vcls;
+    int32_t num_dom = 1, num_itf = 0, cls_itf0 = 1;
+    bool multi = false;                        // joined mesh (several domains or interfaces)
+    // interface nodes over all interfaces (q = 0 .. K-1): the two copies and the interface
+    int64_t K = 0;
+    DevBuf<int32_t> itf_va, itf_vb, itf_of;
+    std::vector<int32_t> h_itf_va, h_itf_vb, h_itf_of;
     DevBuf<int32_t> rev;                       // rev[e] = directed edge (n, v) of e = (v, n); built on demand
     std::vector<int64_t> bnd_ptr;              // host copy
     std::vector<int32_t> bnd_nid;
@@ -116,8 +126,14 @@ struct DevOp {
     const double* w_grad;
     const double* r_val;   // density value and gradient (advection: rho)
     const double* r_grad;
+    int32_t side;          // interface advection: flowing side (0: a, 1: b)
+    int32_t pad;
 };
-constexpr int MAX_CLASSES = 16;  // class 0 = points, class 1 + b = boundary b
+// class d = points of domain d, then one class per boundary, then two per interface
+constexpr int MAX_CLASSES = 32;
+// columns of an interface row (per interface class): the faces of the side-a / side-b copy of
+// the node (ngh(a_v) / ngh(b_v)) and the two copies themselves
+constexpr uint8_t ITF_EA = 1, ITF_EB = 2, ITF_CA = 4, ITF_CB = 8;
 constexpr int MAX_OPS = 8;
 struct DevProgram {
     int32_t nops[MAX_CLASSES];
@@ -253,6 +269,10 @@ struct Ctx {
     int64_t n = 0, nnz = 0;
     std::vector<int32_t> h_mrow, h_mcol;
     DevBuf<int32_t> mrow, mcol, eslot, dslot;
+    // interface rows (joined meshes): per interface node q the slots (a_v, a_v), (a_v, b_v) of
+    // the side-a row and (b_v, a_v), (b_v, b_v) of the side-b row (-1: absent), 4 per node;
+    // eslot of a face of either copy points into the side-b row
+    DevBuf<int32_t> islot;
     std::vector<vfvm_op> pattern_ops;
     DevBuf<DevProgram> prog;
     DevBuf<double> vals, b;

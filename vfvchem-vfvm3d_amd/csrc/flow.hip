@@ -32,7 +32,7 @@ namespace {
 
 constexpr int BLK = 256;
 constexpr int WAVE = 64;
-constexpr int MAX_BND = MAX_CLASSES;
+constexpr int MAX_BND = MAX_FLOW_BND;
 constexpr double NEG_FRAC_TWO_THREE = -2.0 / 3.0;  // opvec_dom_ddil.rs:6
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 
@@ -771,6 +771,8 @@ bool vfvm::is_flow_list(const vfvm_op* ops, int32_t nops) {
 // R3(v) = union of grad_coeff(u) over u in {v} u ngh(v) (which holds {v} u ngh(v)), and the
 // reference-pressure row {3N + v}; all ascending.
 int vfvm::flow_pattern_build(Ctx& c, const vfvm_op* ops, int32_t nops) {
+    if (c.g.multi)
+        return set_error(VFVM_ERR_OPERATOR_UNSUPPORTED, "flow operators on a joined mesh (flow across interfaces) are outside the built path");
     FlowSys fs;
     if (int st = flow_compile(c, ops, nops, fs)) return st;
     const DevGeom& g = c.g;

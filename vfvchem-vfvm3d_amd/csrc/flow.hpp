@@ -52,6 +52,9 @@ struct FlowFields {
     const double* den_prev;  // N (transient)
 };
 
+// boundaries of a flow system (one domain)
+constexpr int MAX_FLOW_BND = 16;
+
 struct FlowAsm {
     FlowGeom g;
     FlowFields f;
@@ -59,8 +62,8 @@ struct FlowAsm {
     const int32_t* mcol;
     double* vals;
     double* b;
-    const double* vbnd_val[MAX_CLASSES];  // per boundary: velocity values (3 x count SoA)
-    int64_t vbnd_cnt[MAX_CLASSES];
+    const double* vbnd_val[MAX_FLOW_BND];  // per boundary: velocity values (3 x count SoA)
+    int64_t vbnd_cnt[MAX_FLOW_BND];
     int64_t pref_row;  // row of the reference pressure (-1: none)
     double pref_val;
     int32_t gm, rm;  // LDS capacities of a continuity row

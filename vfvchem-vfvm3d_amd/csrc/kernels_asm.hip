@@ -130,6 +130,8 @@ struct AsmArgs {
     const double* bsf_wgt;  // 3S
     const int32_t* vbnd;
     const int32_t* bidx;
+    const int32_t* vcls;
+    int32_t cls_itf0;  // first interface class (rows of k_assemble_itf)
     const int32_t* mrow;
     const int32_t* eslot;
     const int32_t* dslot;
@@ -185,7 +187,7 @@ __device__ inline bool is_edge_op(int k) {
 }
 
 // Diagonal and right-hand-side contributions of a per-vertex operator.
-__device__ inline void vertex_op(const DevOp& op, const AsmArgs& a, int64_t v, int32_t vb, double& dacc, double& bacc) {
+__device__ inline void vertex_op(const DevOp& op, const AsmArgs& a, int64_t v, double& dacc, double& bacc) {
     const int64_t N = a.N, S = a.S;
     switch (op.kind) {
         case VFVM_OP_SCL_DOM_PRODUCTION:
@@ -237,6 +239,7 @@ __device__ inline void vertex_op(const DevOp& op, const AsmArgs& a, int64_t v, i
         case VFVM_OP_SCL_BND_NEUMANN: {
             // opscl_bnd_neu.rs:54-83
             const double fv = op.f_val[a.bidx[v]];
+            const int32_t vb = a.vbnd[v];
             for (int32_t s = a.bsf_ptr[v]; s < a.bsf_ptr[v + 1]; ++s) {
                 const int32_t j = a.bsf_vj[s], k = a.bsf_vk[s];
                 const double fj = a.vbnd[j] == vb ? op.f_val[a.bidx[j]] : fv;
@@ -259,7 +262,7 @@ __device__ inline void vertex_op(const DevOp& op, const AsmArgs& a, int64_t v, i
 // The row of vertex v for the operator class cls (shared by the wave-uniform and the
 // per-lane paths of k_assemble).
 template <int LPR>
-__device__ inline void assemble_row(const AsmArgs& a, int64_t v, int lane, int32_t vb, int cls) {
+__device__ inline void assemble_row(const AsmArgs& a, int64_t v, int lane, int cls) {
     const int nops = a.prog->nops[cls];
     const int32_t es = a.rp[v], ee = a.rp[v + 1];
     const int32_t deg = ee - es;
@@ -281,7 +284,7 @@ __device__ inline void assemble_row(const AsmArgs& a, int64_t v, int lane, int32
                 for (int k = 0; k < n0; ++k) dacc += -__shfl(c0, k, LPR);
                 for (int k = 0; k < deg - LPR; ++k) dacc += -__shfl(c1, k, LPR);
             } else if (lane == 0) {
-                vertex_op(op, a, v, vb, dacc, bacc);
+                vertex_op(op, a, v, dacc, bacc);
             }
         }
         if (any_edge) {
@@ -300,7 +303,7 @@ __device__ inline void assemble_row(const AsmArgs& a, int64_t v, int lane, int32
                     dacc += -coeff;
                 }
             } else {
-                vertex_op(op, a, v, vb, dacc, bacc);
+                vertex_op(op, a, v, dacc, bacc);
             }
         }
     }
@@ -327,11 +330,106 @@ __global__ void __launch_bounds__(BLK) k_assemble(AsmArgs a) {
     const int64_t v = tid / LPR;
     const int lane = (int)(tid % LPR);
     if (v >= a.N) return;  // whole row groups leave together (BLK is a multiple of LPR)
-    const int32_t vb = a.vbnd[v];
-    const int cls = vb < 0 ? 0 : 1 + vb;
+    const int cls = a.vcls[v];
+    if (cls >= a.cls_itf0) return;  // interface rows: k_assemble_itf
     const int cls0 = __builtin_amdgcn_readfirstlane(cls);
-    if (__all(cls == cls0)) assemble_row<LPR>(a, v, lane, vb, cls0);
-    else assemble_row<LPR>(a, v, lane, vb, cls);
+    if (__all(cls == cls0)) assemble_row<LPR>(a, v, lane, cls0);
+    else assemble_row<LPR>(a, v, lane, cls);
+}
+
+// The rows of the interface nodes' copies (joined meshes), one thread per node q: every push
+// of the interface operators (opscl_itf_*.rs) is added into its slot in the reference's push
+// order (operator, then side a before side b, then face order), so each entry and b are the
+// reference's sequential sums.  Side a's row takes the continuity operator, side b's row every
+// other one; a face coefficient is the domain kernel's (edge_coeff) on the face of that side's
+// copy of the node.
+struct ItfArgs {
+    int64_t K;
+    const int32_t* va;
+    const int32_t* vb;
+    const int32_t* itf;
+    const int32_t* islot;  // 4 per node: (a,a), (a,b) of row a_v; (b,a), (b,b) of row b_v
+};
+__device__ inline int itf_face_kind(int k) {
+    return k == VFVM_OP_SCL_ITF_DIFFUSION ? VFVM_OP_SCL_DOM_DIFFUSION
+         : k == VFVM_OP_SCL_ITF_ADVECTION ? VFVM_OP_SCL_DOM_ADVECTION
+                                           : VFVM_OP_SCL_DOM_ADVECTION_UNITY;
+}
+__global__ void __launch_bounds__(BLK) k_assemble_itf(AsmArgs a, ItfArgs f) {
+    const int64_t q = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (q >= f.K) return;
+    const int64_t N = a.N;
+    const int32_t va = f.va[q], vb = f.vb[q];
+    const int clsA = a.cls_itf0 + 2 * f.itf[q], clsB = clsA + 1;
+    const int32_t saa = f.islot[4 * q], sab = f.islot[4 * q + 1], sba = f.islot[4 * q + 2], sbb = f.islot[4 * q + 3];
+    // side a's row (opscl_itf_cont.rs:44-56): c_a - c_b = 0
+    for (int32_t k = a.mrow[va]; k < a.mrow[va + 1]; ++k) a.vals[k] = 0.0;
+    for (int o = 0; o < a.prog->nops[clsA]; ++o) {
+        a.vals[saa] += 1.0;
+        a.vals[sab] += -1.0;
+    }
+    a.b[va] = 0.0;
+    // side b's row: the combined control volume
+    for (int32_t k = a.mrow[vb]; k < a.mrow[vb + 1]; ++k) a.vals[k] = 0.0;
+    double bacc = 0.0;
+    for (int o = 0; o < a.prog->nops[clsB]; ++o) {
+        DevOp op = a.prog->ops[clsB][o];
+        const int kind = op.kind;
+        for (int side = 0; side < 2; ++side) {
+            const int32_t sv = side ? vb : va;
+            const int32_t sdiag = side ? sbb : sba;
+            switch (kind) {
+                case VFVM_OP_SCL_ITF_DIFFUSION:
+                case VFVM_OP_SCL_ITF_ADVECTION:
+                case VFVM_OP_SCL_ITF_ADVECTION_UNITY: {
+                    // opscl_itf_diff.rs:73-108 (both sides); opscl_itf_adv.rs:86-148 and
+                    // opscl_itf_advunity.rs:85-135 (the flowing side only)
+                    if (kind != VFVM_OP_SCL_ITF_DIFFUSION && side != op.side) break;
+                    op.kind = itf_face_kind(kind);
+                    for (int32_t e = a.rp[sv]; e < a.rp[sv + 1]; ++e) {
+                        const double coeff = edge_coeff(op, a, sv, e);
+                        a.vals[a.eslot[e]] += coeff;
+                        a.vals[sdiag] += -coeff;
+                    }
+                    op.kind = kind;
+                } break;
+                case VFVM_OP_SCL_ITF_PRODUCTION: {
+                    // opscl_itf_prod.rs:60-92: both halves on (b_v, b_v)
+                    const double sx = a.cskew[sv], sy = a.cskew[N + sv], sz = a.cskew[2 * N + sv];
+                    const double vol = a.vol[sv];
+                    const double pv = op.f_val[sv];
+                    const double prd_c = pv + op.f_grad[sv] * sx + op.f_grad[N + sv] * sy + op.f_grad[2 * N + sv] * sz;
+                    const double unk_skew = op.u_grad[sv] * sx + op.u_grad[N + sv] * sy + op.u_grad[2 * N + sv] * sz;
+                    a.vals[sbb] += prd_c * vol;
+                    bacc += -pv * unk_skew * vol;
+                } break;
+                case VFVM_OP_SCL_ITF_SOURCE: {
+                    // opscl_itf_src.rs:57-83
+                    const double sx = a.cskew[sv], sy = a.cskew[N + sv], sz = a.cskew[2 * N + sv];
+                    const double src_c = op.f_val[sv] + op.f_grad[sv] * sx + op.f_grad[N + sv] * sy + op.f_grad[2 * N + sv] * sz;
+                    bacc += -src_c * a.vol[sv];
+                } break;
+                case VFVM_OP_SCL_ITF_TIME:
+                case VFVM_OP_SCL_ITF_TIME_PREVIOUS: {
+                    // opscl_itf_time.rs:50-76, opscl_itf_timeprev.rs:50-78: each side's term on
+                    // that side's copy of the unknown
+                    const double vol = a.vol[sv];
+                    const double w = op.f_val[sv];
+                    a.vals[sdiag] += -w * vol / a.dt;
+                    bacc += -w * op.u_prev[sv] * vol / a.dt;
+                } break;
+                case VFVM_OP_SCL_ITF_TIME_UNITY: {
+                    // opscl_itf_timeunity.rs:47-70
+                    const double vol = a.vol[sv];
+                    a.vals[sdiag] += -vol / a.dt;
+                    bacc += -op.u_prev[sv] * vol / a.dt;
+                } break;
+                default:
+                    break;
+            }
+        }
+    }
+    a.b[vb] = bacc;
 }
 
 }  // namespace
@@ -377,6 +475,7 @@ int launch_assemble_block(Ctx& c, double dt, const int32_t* mrow, const int32_t*
     a.vol = g.vol.p; a.cskew = g.cskew.p;
     a.bsf_ptr = g.bsf_ptr.p; a.bsf_vj = g.bsf_vj.p; a.bsf_vk = g.bsf_vk.p; a.bsf_area = g.bsf_area.p; a.bsf_wgt = g.bsf_wgt.p;
     a.vbnd = g.vbnd.p; a.bidx = g.bidx.p;
+    a.vcls = g.vcls.p; a.cls_itf0 = g.cls_itf0;
     a.mrow = mrow; a.eslot = eslot; a.dslot = dslot;
     a.prog = prog;
     a.dt = dt;
@@ -390,6 +489,11 @@ int launch_assemble_block(Ctx& c, double dt, const int32_t* mrow, const int32_t*
     constexpr int LPR = 16;
     hipLaunchKernelGGL((k_assemble<LPR>), dim3(grid_for(g.N * LPR)), dim3(BLK), 0, c.stream, a);
     VFVM_HIP_TRY(hipGetLastError());
+    if (g.K > 0 && mrow == c.mrow.p) {
+        ItfArgs f{g.K, g.itf_va.p, g.itf_vb.p, g.itf_of.p, c.islot.p};
+        hipLaunchKernelGGL(k_assemble_itf, dim3(grid_for(g.K)), dim3(BLK), 0, c.stream, a, f);
+        VFVM_HIP_TRY(hipGetLastError());
+    }
     ktime_end(c, VFVM_K_ASSEMBLE, bytes, ev);
     return VFVM_OK;
 }

@@ -52,12 +52,33 @@ struct Mesh {
     std::vector<int32_t> gc_vid;    // G
     std::vector<double> gc_c;       // 3G SoA
 
+    // multi-domain meshes (join_meshes): domain d holds the vertices [dom_ptr[d], dom_ptr[d+1]),
+    // the first dom_npts[d] of them its points, then its nodes; num_pts is 0 and node ids are
+    // vertex ids.  Interface k pairs the two copies itf_va[q] (side a) and itf_vb[q] (side b)
+    // of one node, q in [itf_ptr[k], itf_ptr[k+1]), in the order of side a's node group.
+    // Empty for a single-domain mesh.
+    std::vector<int64_t> dom_ptr, dom_npts;
+    std::vector<int64_t> itf_ptr;
+    std::vector<int32_t> itf_va, itf_vb;
+    std::vector<int32_t> itf_dom;   // 2 per interface: domain of side a, of side b
+    int32_t num_dom() const { return dom_ptr.empty() ? 1 : (int32_t)dom_ptr.size() - 1; }
+    int32_t num_itf() const { return itf_ptr.empty() ? 0 : (int32_t)itf_ptr.size() - 1; }
+
     int64_t num_edges() const { return (int64_t)ngh.size(); }
     int64_t num_subfaces() const { return (int64_t)bsf_vj.size(); }
 };
 
 // Status codes shared with the C ABI (mirrors of VFVChemError variants on this path).
 int lattice_mesh(int n, uint64_t seed, double jitter, Mesh& m, std::string& err);
+int lattice_box_mesh(int n, uint64_t seed, double jitter, const int32_t lo[3], const int32_t hi[3], Mesh& m,
+                     std::string& err);
+// Interface of a join: boundary bnd_a of domain dom_a and boundary bnd_b of domain dom_b hold
+// the same nodes (equal coordinates).
+struct JoinItf {
+    int32_t dom_a, bnd_a, dom_b, bnd_b;
+};
+// One mesh of several domains whose duals are built (mesh_join.cpp).
+int join_meshes(const std::vector<const Mesh*>& doms, const std::vector<JoinItf>& itfs, Mesh& out, std::string& err);
 int delaunay_int(const std::vector<int64_t>& ixyz, const std::vector<int32_t>& order,
                  std::vector<int32_t>& tets_out, std::string& err);
 int build_dual(Mesh& m, int nthreads, std::string& err);

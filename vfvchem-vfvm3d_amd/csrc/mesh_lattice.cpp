@@ -12,6 +12,7 @@
 // Free axes: all three inside, the two in-plane axes on a face, the edge axis on
 // a cube edge, none at a corner.  Coordinates are then rounded to multiples of
 // 2^-22 so that the Delaunay predicates are exact.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -40,24 +41,46 @@ double lattice_jitter_unit(uint64_t seed, int64_t i, int64_t j, int64_t k, int a
 }
 
 int lattice_mesh(int n, uint64_t seed, double jitter, Mesh& m, std::string& err) {
+    const int32_t lo[3] = {0, 0, 0}, hi[3] = {n - 1, n - 1, n - 1};
+    return lattice_box_mesh(n, seed, jitter, lo, hi, m, err);
+}
+
+// The vertices of the n^3 lattice with lo <= (i, j, k) <= hi (a box of the unit cube).  A
+// coordinate on a face of the box is the unjittered lattice value (0 and 1 exactly on the
+// cube's faces), so each face is a plane and two boxes sharing a face place their common
+// vertices identically (the same hash of the global lattice index jitters the free axes).
+int lattice_box_mesh(int n, uint64_t seed, double jitter, const int32_t lo[3], const int32_t hi[3], Mesh& m,
+                     std::string& err) {
     if (n < 3 || n > 1024) { err = "lattice: n must be in [3, 1024]"; return VFVM_ERR_INVALID_ARGUMENT; }
     if (!(jitter >= 0.0 && jitter < 0.5)) { err = "lattice: jitter must be in [0, 0.5)"; return VFVM_ERR_INVALID_ARGUMENT; }
-    const int64_t nn = n;
-    const int64_t ntot = nn * nn * nn;
+    for (int ax = 0; ax < 3; ++ax)
+        if (lo[ax] < 0 || hi[ax] > n - 1 || hi[ax] - lo[ax] < 2) {
+            err = "lattice box: need 0 <= lo < hi <= n - 1 with at least 3 vertices per axis";
+            return VFVM_ERR_INVALID_ARGUMENT;
+        }
+    const int64_t nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    const int64_t ntot = nx * ny * nz;
     const double h = 1.0 / (double)(n - 1);
     const double Q = 4194304.0;  // 2^22
     const int64_t QI = int64_t(1) << 22;
+    const int64_t ext[3] = {nx, ny, nz};
+    // quantised coordinate of lattice plane idx (0 and 2^22 exactly on the cube's faces)
+    auto plane = [&](int64_t idx) -> int64_t {
+        if (idx == 0) return 0;
+        if (idx == n - 1) return QI;
+        return (int64_t)std::llround((double)idx / (double)(n - 1) * Q);
+    };
 
-    auto on_b = [&](int64_t a) { return a == 0 || a == nn - 1; };
-    // vid of every lattice index
+    // box-relative index r = idx - lo; on the box's boundary when r is 0 or ext - 1
+    auto on_b = [&](int64_t r, int ax) { return r == 0 || r == ext[ax] - 1; };
     std::vector<int32_t> vid_of(ntot);
-    int64_t num_pts = (nn - 2) * (nn - 2) * (nn - 2);
+    int64_t num_pts = (nx - 2) * (ny - 2) * (nz - 2);
     int64_t p = 0, q = num_pts;
-    for (int64_t k = 0; k < nn; ++k)
-        for (int64_t j = 0; j < nn; ++j)
-            for (int64_t i = 0; i < nn; ++i) {
-                int64_t li = i + nn * (j + nn * k);
-                bool b = on_b(i) || on_b(j) || on_b(k);
+    for (int64_t k = 0; k < nz; ++k)
+        for (int64_t j = 0; j < ny; ++j)
+            for (int64_t i = 0; i < nx; ++i) {
+                int64_t li = i + nx * (j + ny * k);
+                bool b = on_b(i, 0) || on_b(j, 1) || on_b(k, 2);
                 vid_of[li] = (int32_t)(b ? q++ : p++);
             }
     m = Mesh();
@@ -68,28 +91,30 @@ int lattice_mesh(int n, uint64_t seed, double jitter, Mesh& m, std::string& err)
     m.bnd_name = {"x0", "x1", "y0", "y1", "z0", "z1"};
     std::vector<std::vector<int32_t>> bl(6);
     m.nd_bnd.assign(m.num_nds, -1);
-    for (int64_t k = 0; k < nn; ++k)
-        for (int64_t j = 0; j < nn; ++j)
-            for (int64_t i = 0; i < nn; ++i) {
-                int64_t li = i + nn * (j + nn * k);
+    for (int64_t k = 0; k < nz; ++k)
+        for (int64_t j = 0; j < ny; ++j)
+            for (int64_t i = 0; i < nx; ++i) {
+                int64_t li = i + nx * (j + ny * k);
                 int64_t v = vid_of[li];
-                int64_t idx[3] = {i, j, k};
+                const int64_t rel[3] = {i, j, k};
+                const int64_t idx[3] = {lo[0] + i, lo[1] + j, lo[2] + k};
                 for (int ax = 0; ax < 3; ++ax) {
-                    double x = (double)idx[ax] / (double)(n - 1);
                     int64_t X;
-                    if (idx[ax] == 0) X = 0;
-                    else if (idx[ax] == nn - 1) X = QI;
-                    else {
-                        x += jitter * h * lattice_jitter_unit(seed, i, j, k, ax);
+                    if (on_b(rel[ax], ax)) {
+                        X = plane(idx[ax]);
+                    } else {
+                        double x = (double)idx[ax] / (double)(n - 1);
+                        x += jitter * h * lattice_jitter_unit(seed, idx[0], idx[1], idx[2], ax);
                         X = (int64_t)std::llround(x * Q);
-                        if (X < 1) X = 1;
-                        if (X > QI - 1) X = QI - 1;
+                        const int64_t xlo = plane(lo[ax]) + 1, xhi = plane(hi[ax]) - 1;
+                        if (X < xlo) X = xlo;
+                        if (X > xhi) X = xhi;
                     }
                     ixyz[3 * v + ax] = X;
                     m.pos[3 * v + ax] = (double)X / Q;
                 }
                 if (v >= num_pts) {
-                    int bid = (i == 0) ? 0 : (i == nn - 1) ? 1 : (j == 0) ? 2 : (j == nn - 1) ? 3 : (k == 0) ? 4 : 5;
+                    int bid = on_b(i, 0) ? (i == 0 ? 0 : 1) : on_b(j, 1) ? (j == 0 ? 2 : 3) : (k == 0 ? 4 : 5);
                     int32_t nid = (int32_t)(v - num_pts);
                     bl[bid].push_back(nid);
                     m.nd_bnd[nid] = bid;
@@ -107,18 +132,19 @@ int lattice_mesh(int n, uint64_t seed, double jitter, Mesh& m, std::string& err)
     order.reserve(ntot);
     std::vector<uint8_t> done(ntot, 0);
     auto push = [&](int64_t i, int64_t j, int64_t k) {
-        int64_t li = i + nn * (j + nn * k);
+        int64_t li = i + nx * (j + ny * k);
         if (!done[li]) { done[li] = 1; order.push_back(vid_of[li]); }
     };
-    const int64_t e = nn - 1;
-    push(0, 0, 0); push(e, 0, 0); push(0, e, 0); push(0, 0, e);
-    push(e, e, 0); push(e, 0, e); push(0, e, e); push(e, e, e);
+    const int64_t ex = nx - 1, ey = ny - 1, ez = nz - 1;
+    push(0, 0, 0); push(ex, 0, 0); push(0, ey, 0); push(0, 0, ez);
+    push(ex, ey, 0); push(ex, 0, ez); push(0, ey, ez); push(ex, ey, ez);
+    const int64_t emax = std::max(ex, std::max(ey, ez));
     int64_t stride = 1;
-    while (stride * 2 <= e) stride *= 2;
+    while (stride * 2 <= emax) stride *= 2;
     for (; stride >= 1; stride /= 2)
-        for (int64_t k = 0; k < nn; k += stride)
-            for (int64_t j = 0; j < nn; j += stride)
-                for (int64_t i = 0; i < nn; i += stride) push(i, j, k);
+        for (int64_t k = 0; k < nz; k += stride)
+            for (int64_t j = 0; j < ny; j += stride)
+                for (int64_t i = 0; i < nx; i += stride) push(i, j, k);
 
     int st = delaunay_int(ixyz, order, m.tets, err);
     if (st != VFVM_OK) return st;

@@ -16,6 +16,8 @@ from .core import (FLW_FLUX, FLW_MOM, FLW_RC, OP_SCL_BND_DIVERGENCE, OP_SCL_BND_
                    OP_SCL_DOM_DIVERGENCE, OP_SCL_DOM_PRESSURE_REFERENCE, OP_SCL_DOM_PRESSURE_STABILIZER, OP_VEC_BND_DIRICHLET,
                    OP_VEC_DOM_ADVECTION, OP_VEC_DOM_DIFFUSION, OP_VEC_DOM_DILATATORY, OP_VEC_DOM_GRADIENT, OP_VEC_DOM_SOURCE,
                    OP_VEC_DOM_TRANSPOSE, PRECOND_BJACOBI)
+from .core import (OP_SCL_ITF_ADVECTION, OP_SCL_ITF_ADVECTION_UNITY, OP_SCL_ITF_CONTINUITY, OP_SCL_ITF_DIFFUSION,  # noqa: F401
+                   OP_SCL_ITF_PRODUCTION, OP_SCL_ITF_SOURCE, OP_SCL_ITF_TIME, OP_SCL_ITF_TIME_PREVIOUS, OP_SCL_ITF_TIME_UNITY)
 from .api import (LinearGmres, SteadyFlow, SteadyHeat, SteadyHeatFlow, SteadyMass, SteadyMassFlow, TransientFlow,  # noqa: F401
                   TransientHeat, TransientHeatFlow, TransientHeatMassFlow, Variables)
 

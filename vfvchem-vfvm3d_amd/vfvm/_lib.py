@@ -55,7 +55,8 @@ class VFVChemError(RuntimeError):
 class MeshInfo(C.Structure):
     _fields_ = [("num_pts", C.c_int64), ("num_nds", C.c_int64), ("num_tets", C.c_int64),
                 ("num_edges", C.c_int64), ("num_subfaces", C.c_int64), ("num_bnd", C.c_int32),
-                ("max_degree", C.c_int32), ("num_grad_coeff", C.c_int64)]
+                ("max_degree", C.c_int32), ("num_grad_coeff", C.c_int64), ("num_dom", C.c_int32), ("num_itf", C.c_int32),
+                ("num_itf_nodes", C.c_int64)]
 
 
 class OpStruct(C.Structure):
@@ -138,6 +139,8 @@ SIGNATURES = [
     ("vfvm_mesh_lattice", _I, [_I, C.c_uint64, _D, C.POINTER(_P)]),
     ("vfvm_mesh_from_tets", _I, [_L, _L, _PD, _L, _PI, _I, _PL, _PI, C.POINTER(_P)]),
     ("vfvm_mesh_lattice_dev", _I, [_I, C.c_uint64, _D, _I, C.POINTER(_P)]),
+    ("vfvm_mesh_lattice_box", _I, [_I, C.c_uint64, _D, _PI, _PI, _I, C.POINTER(_P)]),
+    ("vfvm_mesh_join", _I, [_I, C.POINTER(_P), _I, _PI, C.POINTER(_P)]),
     ("vfvm_mesh_from_tets_dev", _I, [_L, _L, _PD, _L, _PI, _I, _PL, _PI, _I, C.POINTER(_P)]),
     ("vfvm_mesh_info_get", _I, [_P, C.POINTER(MeshInfo)]),
     ("vfvm_mesh_build_grad_coeff", _I, [_P]),

@@ -40,6 +40,15 @@ OP_VEC_DOM_TIME = 49
 OP_VEC_BND_DIRICHLET = 60
 OP_SCL_BND_DIVERGENCE = 61
 OP_SCL_BND_PRESSURE_STABILIZER = 62
+OP_SCL_ITF_CONTINUITY = 70
+OP_SCL_ITF_DIFFUSION = 71
+OP_SCL_ITF_PRODUCTION = 72
+OP_SCL_ITF_SOURCE = 73
+OP_SCL_ITF_TIME = 74
+OP_SCL_ITF_TIME_PREVIOUS = 75
+OP_SCL_ITF_TIME_UNITY = 76
+OP_SCL_ITF_ADVECTION = 77
+OP_SCL_ITF_ADVECTION_UNITY = 78
 FLW_FLUX, FLW_RC, FLW_MOM = 0, 1, 2
 
 FIELD_CONSTANT, FIELD_FUNCTION, FIELD_UNKNOWN = 0, 1, 2
@@ -56,6 +65,8 @@ _MESH_ARRAYS = {
     "cskew": (12, np.float64), "bsf_ptr": (13, np.int32), "bsf_vj": (14, np.int32), "bsf_vk": (15, np.int32),
     "bsf_area": (16, np.float64), "bsf_norm": (17, np.float64), "bsf_wgt": (18, np.float64),
     "gc_ptr": (19, np.int32), "gc_vid": (20, np.int32), "gc_c": (21, np.float64),
+    "dom_ptr": (22, np.int64), "dom_npts": (23, np.int64), "itf_ptr": (24, np.int64), "itf_va": (25, np.int32),
+    "itf_vb": (26, np.int32), "itf_dom": (27, np.int32),
 }
 
 
@@ -75,7 +86,11 @@ class Mesh:
         self.num_bnd = info.num_bnd
         self.max_degree = info.max_degree
         self.num_grad_coeff = info.num_grad_coeff
+        self.num_dom = info.num_dom
+        self.num_itf = info.num_itf
+        self.num_itf_nodes = info.num_itf_nodes
         self._cache = {}
+        self._parts = ()
 
     def build_grad_coeff(self, device=None):
         """Pressure-gradient coefficients of the flow operators (geom_tet.rs:295-347); built
@@ -100,6 +115,29 @@ class Mesh:
         return cls(h)
 
     @classmethod
+    def lattice_box(cls, n: int, lo, hi, seed: int = 0, jitter: float = 0.3, device=None) -> "Mesh":
+        """The vertices lo <= (i, j, k) <= hi of the n^3 jittered lattice (a box of the unit cube;
+        boxes sharing a face place its vertices identically); dual on GPU `device` when given."""
+        lo = np.ascontiguousarray(lo, np.int32)
+        hi = np.ascontiguousarray(hi, np.int32)
+        h = C.c_void_p()
+        check(lib.vfvm_mesh_lattice_box(int(n), int(seed), float(jitter), ptr(lo, C.c_int32), ptr(hi, C.c_int32),
+                                        -1 if device is None else int(device), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def join(cls, doms, itfs) -> "Mesh":
+        """Multi-domain mesh of the meshes `doms` joined by interfaces `itfs` = [(dom_a, bnd_a,
+        dom_b, bnd_b), ...] (boundary bnd_a of dom_a and bnd_b of dom_b hold the same nodes)."""
+        hs = (C.c_void_p * len(doms))(*[d._h for d in doms])
+        q = np.ascontiguousarray(np.asarray(itfs, np.int32).reshape(-1), np.int32)
+        h = C.c_void_p()
+        check(lib.vfvm_mesh_join(len(doms), hs, len(itfs), ptr(q, C.c_int32), C.byref(h)))
+        m = cls(h)
+        m._parts = tuple(doms)
+        return m
+
+    @classmethod
     def from_tets(cls, pos, tets, num_pts: int, bnd_nids, device=None) -> "Mesh":
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
@@ -122,7 +160,9 @@ class Mesh:
         return {"pos": 3 * N, "tets": 4 * T, "bnd_ptr": self.num_bnd + 1, "bnd_nid": None, "row_ptr": N + 1, "ngh": E,
                 "dist": E, "area": E, "norm": 3 * E, "skew": 3 * E, "gw": 3 * E, "vol": N, "cskew": 3 * N,
                 "bsf_ptr": N + 1, "bsf_vj": S, "bsf_vk": S, "bsf_area": S, "bsf_norm": 3 * S, "bsf_wgt": 3 * S,
-                "gc_ptr": N + 1, "gc_vid": self.num_grad_coeff, "gc_c": 3 * self.num_grad_coeff}[name]
+                "gc_ptr": N + 1, "gc_vid": self.num_grad_coeff, "gc_c": 3 * self.num_grad_coeff,
+                "dom_ptr": self.num_dom + 1, "dom_npts": self.num_dom, "itf_ptr": self.num_itf + 1,
+                "itf_va": self.num_itf_nodes, "itf_vb": self.num_itf_nodes, "itf_dom": 2 * self.num_itf}[name]
 
     def get(self, name: str) -> np.ndarray:
         if name in self._cache:
