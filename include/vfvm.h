@@ -408,9 +408,17 @@ int vfvm_residual(vfvm_ctx* ctx, const double* x, double* rel_residual);
  * precision: FP32_REFINED factors and applies ILU(0) in FP32), host arrays.  No reference
  * counterpart (the reference's Jacobi lives inside faer_gmres); exposed for parity tests. */
 int vfvm_precond_apply(vfvm_ctx* ctx, const vfvm_solve_opts* opts, const double* r, double* z);
-/* The ILU(0) row order of the built pattern: perm[t] = row of A at position t (rows sorted
- * by (colour, index)), color_ptr[ncolors + 1] = first position of each colour. */
+/* The ILU(0) row order of the built pattern: perm[t] = row of A at position t, color_ptr[ncolors
+ * + 1] = first position of each colour.  Multicolour order: rows sorted by (colour, Morton code),
+ * no two rows of one colour coupled.  Tiled order (scalar systems on a mesh): rows sorted by
+ * (tile colour, tile, level, Morton code); the "colours" are the tile colours, whose rows couple
+ * only inside one tile (vfvm_ilu_tiles).  For the parity tests; no reference counterpart. */
 int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, int32_t* ncolors);
+/* The tiles of the tiled ILU(0) order (ntiles = 0 for the multicolour order): tile k holds the
+ * positions [tile_ptr[k], tile_ptr[k+1]) (tile_ptr: ntiles + 1 entries, at most n + 1), level[t]
+ * = level of position t inside its tile (n entries); rows of one tile and one level are not
+ * coupled.  For the parity tests; no reference counterpart. */
+int vfvm_ilu_tiles(vfvm_ctx* ctx, int32_t* tile_ptr, int32_t* ntiles, int32_t* level);
 
 /* ---- caller-assembled systems (LinearBase, src/linear/linear_base.rs:5-10) ----
  * For a solver whose operators are not on the device: it hands its triplet list, right-hand

@@ -173,26 +173,41 @@ def test_c2_full_size_uniform_solution(ctx):
         assert np.abs(x - 1.0).max() <= 1e-3
 
 
-@pytest.mark.parametrize("prob", ["heat", "mass"])
-def test_ilu0_application_vs_oracle(ctx, prob):
+@pytest.mark.parametrize("prob,n", [("heat", 8), ("mass", 8), ("mass", 13)])
+def test_ilu0_application_vs_oracle(ctx, prob, n):
     """M^-1 r of the device ILU(0) against the oracle's ILU(0) (Saad Alg. 10.4, IKJ) of the
-    same matrix in the device's multicolour row order; the oracle's factors are pinned by
-    (LU)_ij = A_ij on the pattern."""
-    print(f"[ilu0 {prob}]", flush=True)
-    mesh = vfvm.Mesh.lattice(8)
+    same matrix in the device's row order (tiled: tile colours, tiles, levels); the oracle's
+    factors are pinned by (LU)_ij = A_ij on the pattern."""
+    print(f"[ilu0 {prob} n={n}]", flush=True)
+    mesh = vfvm.Mesh.lattice(n)
     p = heat_c1(mesh) if prob == "heat" else mass_c2(mesh, c_out=0.0, prod=0.3)
     rp, col, vals, b = _assembled(ctx, p)
     perm, cptr = ctx.ilu_order()
+    tptr, level = ctx.ilu_tiles()
     n = len(rp) - 1
     assert np.array_equal(np.sort(perm), np.arange(n)) and cptr[0] == 0 and cptr[-1] == n
     A = sp.csr_matrix((vals, col, rp), shape=(n, n))
-    # no two rows of one colour are coupled (in A or A^T)
+    # the launches' independence: rows of one colour are not coupled (in A or A^T), except, in
+    # the tiled order, rows of one tile at different levels
     colour = np.empty(n, np.int64)
     for q in range(len(cptr) - 1):
         colour[perm[cptr[q]:cptr[q + 1]]] = q
+    tile = np.zeros(n, np.int64)
+    lev = np.zeros(n, np.int64)
+    if len(tptr):
+        assert tptr[0] == 0 and tptr[-1] == n and np.all(np.diff(tptr) > 0)
+        tile[perm] = np.repeat(np.arange(len(tptr) - 1), np.diff(tptr))
+        lev[perm] = level
+        # tiles do not straddle tile colours
+        assert np.all(np.isin(cptr, tptr))
+    else:
+        tile[:] = np.arange(n)
     coo = A.tocoo()
     off = coo.row != coo.col
-    assert np.all(colour[coo.row[off]] != colour[coo.col[off]])
+    r, c = coo.row[off], coo.col[off]
+    same_colour = colour[r] == colour[c]
+    assert np.all(tile[r][same_colour] == tile[c][same_colour])
+    assert np.all(lev[r][same_colour] != lev[c][same_colour])
     L, U = O.ilu0(A, perm)
     P = sp.csr_matrix((np.ones(n), (np.arange(n), perm)), shape=(n, n))
     Ap = (P @ A @ P.T).tocsr()

@@ -954,6 +954,13 @@ extern "C" int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, 
     return clear_error();
 }
 
+extern "C" int vfvm_ilu_tiles(vfvm_ctx* ctx, int32_t* tile_ptr, int32_t* ntiles, int32_t* level) {
+    CTX_GUARD(ctx);
+    if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_ilu_tiles: no pattern built");
+    if (int st = ilu_tiles(c, tile_ptr, ntiles, level)) return st;
+    return clear_error();
+}
+
 // ------------------------------------------------------------------ drivers
 int vfvm::initial_dirichlet(Ctx& c, const vfvm_op* ops, int32_t nops) {
     if (is_flow_list(ops, nops)) return flow_initial_dirichlet(c, ops, nops);

@@ -151,6 +151,14 @@ struct IluData {
     DevBuf<int32_t> lptr, lcol, lpos, uptr, ucol, upos;
     int64_t nnz_l = 0, nnz_u = 0;
     std::vector<int32_t> h_color_ptr;
+    // tiled order (the rows of a tile contiguous, its levels inside): per tile 8 ints
+    // {r0, r1, L entries begin, end, U entries begin, end, 0, 0}, the first row of every level of
+    // every tile, the factorisation's row groups; the sweeps take one launch per tile colour
+    bool tiled = false;
+    DevBuf<int32_t> tinfo, lev_row, fac_rows;
+    std::vector<int32_t> h_tc_ptr, h_fac_ptr, h_tile_row, h_lev_row;
+    std::vector<double> h_tc_rows, h_tc_lent, h_tc_uent;  // per tile colour: rows, L and U entries
+    int32_t nlev = 0, rmax = 0, emax_l = 0, emax_u = 0;
     bool ready = false;
     uint64_t generation = 0;  // bumped by every set-up (graphs bake the layout in)
 };
@@ -383,6 +391,7 @@ int solver_setup(Ctx& c, const SysView& s, const vfvm_solve_opts& o);
 // z = M^-1 r for the assembled matrix (host arrays); the ILU(0) row order
 int precond_apply(Ctx& c, const vfvm_solve_opts& o, const double* r_host, double* z_host);
 int ilu_order(Ctx& c, int32_t* perm, int32_t* color_ptr, int32_t* ncolors);
+int ilu_tiles(Ctx& c, int32_t* tile_ptr, int32_t* ntiles, int32_t* level);
 int residual_norm(Ctx& c, const SysView& s, const double* x_dev, double* rel, double* bnorm);
 
 // flow operators (flow.hip)

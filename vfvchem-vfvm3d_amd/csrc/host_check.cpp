@@ -1,5 +1,5 @@
 // Host-only exercise of the library's C++ host code — lattice generation, exact Delaunay
-// insertion, the Voronoi dual, grad_coeff, the multicolour ILU(0) ordering and the CSV / VTU
+// insertion, the Voronoi dual, grad_coeff, the tiled and multicolour ILU(0) orderings and the CSV / VTU
 // writers — for a build with AddressSanitizer and UndefinedBehaviorSanitizer (make sanitize).
 // It needs no GPU: the device builders are replaced by functions that report their absence.
 // Exits non-zero (or the sanitizers abort) on the first failure.
@@ -64,26 +64,39 @@ static void scalar_pattern(const vfvm_mesh_info& info, const std::vector<int32_t
 static void check_ilu_order(const std::vector<int32_t>& mrow, const std::vector<int32_t>& mcol, const std::vector<double>& pos,
                             int64_t nvert) {
     const int64_t n = (int64_t)mrow.size() - 1;
-    vfvm::IluOrder o;
-    std::string err;
-    check(vfvm::ilu_order_build(n, mrow, mcol, pos.data(), nvert, o, err) == 0, "ilu_order_build");
-    std::vector<int> seen(n, 0), colour(n, -1);
-    for (int64_t t = 0; t < n; ++t) seen[o.perm[t]]++;
-    bool bij = true;
-    for (int64_t i = 0; i < n; ++i) bij = bij && seen[i] == 1;
-    check(bij, "ILU permutation is a bijection");
-    for (size_t q = 0; q + 1 < o.color_ptr.size(); ++q)
-        for (int32_t t = o.color_ptr[q]; t < o.color_ptr[q + 1]; ++t) colour[o.perm[t]] = (int)q;
-    bool indep = true;
-    for (int64_t i = 0; i < n; ++i)
-        for (int32_t k = mrow[i]; k < mrow[i + 1]; ++k)
-            if (mcol[k] != i && colour[mcol[k]] == colour[i]) indep = false;
-    check(indep, "no two rows of one colour are coupled");
-    check((int64_t)o.lpos.size() + (int64_t)o.upos.size() + n == (int64_t)mcol.size(), "split triangles cover the pattern");
-    bool lower = true;
-    for (int64_t t = 0; t < n; ++t)
-        for (int32_t k = o.lptr[t]; k < o.lptr[t + 1]; ++k) lower = lower && o.lcol[k] < t;
-    check(lower, "L entries lie below the diagonal");
+    for (bool tiled : {true, false}) {
+        vfvm::IluOrder o;
+        std::string err;
+        check(vfvm::ilu_order_build(n, mrow, mcol, pos.data(), nvert, o, err, tiled) == 0, "ilu_order_build");
+        std::vector<int> seen(n, 0), colour(n, -1), tile(n, -1), level(n, 0);
+        for (int64_t t = 0; t < n; ++t) seen[o.perm[t]]++;
+        bool bij = true;
+        for (int64_t i = 0; i < n; ++i) bij = bij && seen[i] == 1;
+        check(bij, "ILU permutation is a bijection");
+        for (size_t q = 0; q + 1 < o.color_ptr.size(); ++q)
+            for (int32_t t = o.color_ptr[q]; t < o.color_ptr[q + 1]; ++t) colour[o.perm[t]] = (int)q;
+        // tiled order: a tile's rows and levels (a multicolour row is a tile of its own)
+        for (int64_t t = 0; t < n; ++t) tile[o.perm[t]] = (int)t;
+        for (size_t k = 0; k + 1 < o.tile_row.size(); ++k)
+            for (int l = 0; l < vfvm::ILU_MAXLEV; ++l)
+                for (int32_t t = o.lev_row[k * (vfvm::ILU_MAXLEV + 1) + l]; t < o.lev_row[k * (vfvm::ILU_MAXLEV + 1) + l + 1]; ++t) {
+                    tile[o.perm[t]] = (int)k;
+                    level[o.perm[t]] = l;
+                }
+        check(!tiled || nvert != n || !o.tile_row.empty(), "the tiled order is taken for a scalar system");
+        bool indep = true;
+        for (int64_t i = 0; i < n; ++i)
+            for (int32_t k = mrow[i]; k < mrow[i + 1]; ++k) {
+                const int32_t j = mcol[k];
+                if (j != i && colour[j] == colour[i] && (tile[j] != tile[i] || level[j] == level[i])) indep = false;
+            }
+        check(indep, "rows of one colour are coupled only inside a tile, at different levels");
+        check((int64_t)o.lpos.size() + (int64_t)o.upos.size() + n == (int64_t)mcol.size(), "split triangles cover the pattern");
+        bool lower = true;
+        for (int64_t t = 0; t < n; ++t)
+            for (int32_t k = o.lptr[t]; k < o.lptr[t + 1]; ++k) lower = lower && o.lcol[k] < t;
+        check(lower, "L entries lie below the diagonal");
+    }
 }
 
 int main() {

@@ -848,6 +848,191 @@ __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const
     if (lane == 0) z[t] = (zt - s) / dd;
 }
 
+// ---- tiled order: one workgroup per tile, the tile's rows in LDS
+// A tile's rows are contiguous (r0 .. r1) and sorted by level; the tiles of one tile colour do
+// not touch, so each launch sweeps a whole tile colour.  Phase 1 stages the tile's triangle
+// entries in LDS: an entry whose column lies outside the tile (a finished tile colour) is
+// multiplied by its z at once, an entry inside keeps its value and local column.  Phase 2 walks
+// the levels (barrier between levels; the rows of one level do not touch), each row summing its
+// staged entries in column order, so only the tile's own z round-trips, through LDS.  Phase 3
+// writes the tile's z.  tinfo: per tile {r0, r1, L begin, L end, U begin, U end, -, -};
+// lev: per tile the first row of each of the ILU_MAXLEV levels and the tile's end.
+// LDS: T sv[emax], T zl[rmax], int16 sc[emax], int16 lp[rmax + 1].  One thread per row in
+// phase 2 (a tile holds at most TBLK rows).
+// A workgroup of TBLK threads stages TSTAGE entries per thread and round: all of a round's
+// column / value loads are issued before its gathers, so a tile of up to TBLK * TSTAGE entries
+// costs one round trip for its entries and one for their z.
+constexpr int TBLK = 512;
+constexpr int TSTAGE = 10;
+static_assert(ILU_TILE_RMAX <= TBLK, "a tile's rows take one thread each");
+template <class T>
+__device__ inline void tile_stage(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                  const T* __restrict__ val, const T* __restrict__ z, int32_t r0, int32_t r1,
+                                  int32_t base, int32_t ne, bool lower, T* sv, int16_t* sc, int16_t* lp) {
+    const int nr = r1 - r0;
+    for (int i = threadIdx.x; i < nr; i += TBLK) lp[i] = (int16_t)(ptr[r0 + i] - base);
+    if (threadIdx.x == 0) lp[nr] = (int16_t)ne;
+    for (int e0 = 0; e0 < ne; e0 += TBLK * TSTAGE) {
+        int32_t j[TSTAGE];
+        T v[TSTAGE];
+#pragma unroll
+        for (int u = 0; u < TSTAGE; ++u) {
+            const int e = e0 + u * TBLK + threadIdx.x;
+            const bool in = e < ne;
+            j[u] = in ? col[base + e] : r0;
+            v[u] = in ? val[base + e] : (T)0;
+        }
+        T zx[TSTAGE];
+#pragma unroll
+        for (int u = 0; u < TSTAGE; ++u) {
+            const bool outside = lower ? j[u] < r0 : j[u] >= r1;
+            zx[u] = z[outside ? j[u] : r0];
+        }
+#pragma unroll
+        for (int u = 0; u < TSTAGE; ++u) {
+            const int e = e0 + u * TBLK + threadIdx.x;
+            if (e < ne) {
+                const bool outside = lower ? j[u] < r0 : j[u] >= r1;
+                sv[e] = outside ? v[u] * zx[u] : v[u];
+                sc[e] = outside ? (int16_t)-1 : (int16_t)(j[u] - r0);
+            }
+        }
+    }
+}
+// Phase 2 of a row: its staged entries outside the tile summed (a prefix of the row for the
+// forward sweep, a suffix for the backward one: columns ascending) and the range of the entries
+// inside returned, packed as (begin << 16) | end
+template <class T>
+__device__ inline int tile_row_outside(const T* sv, const int16_t* sc, const int16_t* lp, int i, bool lower, T& acc) {
+    const int q0 = lp[i], q1 = lp[i + 1];
+    T s = 0;
+    int nin = 0;
+#pragma unroll 4
+    for (int q = q0; q < q1; ++q) {
+        const int16_t j = sc[q];
+        const T v = sv[q];
+        s += j < 0 ? v : (T)0;
+        nin += j >= 0;
+    }
+    acc = s;
+    return lower ? ((q1 - nin) << 16) | q1 : (q0 << 16) | (q0 + nin);
+}
+// the entries inside the tile of a row (range packed as above) against the tile's z
+template <class T>
+__device__ inline T tile_row_inside(const T* sv, const int16_t* sc, const T* zl, int range) {
+    const int q0 = range >> 16, q1 = range & 0xffff;
+    T s = 0;
+    int q = q0;
+    for (; q + 1 < q1; q += 2) {
+        const T a0 = sv[q], a1 = sv[q + 1];
+        const T z0 = zl[sc[q]], z1 = zl[sc[q + 1]];
+        s += a0 * z0;
+        s += a1 * z1;
+    }
+    if (q < q1) s += sv[q] * zl[sc[q]];
+    return s;
+}
+// forward sweep of the tiles [k0, k0 + gridDim.x): z_t = c rhs_t - sum_L l_tj z_j
+template <class T>
+__global__ void __launch_bounds__(TBLK) k_ilu_tile_lower(int32_t k0, const int32_t* __restrict__ tinfo,
+                                                         const int32_t* __restrict__ lev, const int32_t* __restrict__ lptr,
+                                                         const int32_t* __restrict__ lcol, const T* __restrict__ lval,
+                                                         const T* __restrict__ rhs, const double* __restrict__ rscale,
+                                                         T* __restrict__ z, const int* __restrict__ ctrl, int32_t nlev,
+                                                         int32_t emax, int32_t rmax) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int32_t lv[ILU_MAXLEV + 1];
+    const int64_t k = k0 + (int64_t)xcd_block();
+    const int go = ctrl_go(ctrl);
+    const int32_t r0 = tinfo[8 * k], r1 = tinfo[8 * k + 1], base = tinfo[8 * k + 2], end = tinfo[8 * k + 3];
+    if (ctrl_stop(go, r0 <= r1)) return;
+    T* sv = reinterpret_cast<T*>(lds);
+    T* zl = sv + emax;
+    int16_t* sc = reinterpret_cast<int16_t*>(zl + rmax);
+    int16_t* lp = sc + emax;
+    const int nr = r1 - r0;
+    const T c = (T)scale_of(rscale);
+    if (threadIdx.x <= ILU_MAXLEV) lv[threadIdx.x] = lev[k * (ILU_MAXLEV + 1) + threadIdx.x] - r0;
+    T rv = 0;
+    if ((int)threadIdx.x < nr) rv = (T)(rhs[r0 + threadIdx.x] * c);  // one row per thread (nr <= TBLK)
+    tile_stage<T>(lptr, lcol, lval, z, r0, r1, base, end - base, true, sv, sc, lp);
+    __syncthreads();
+    int range = 0;
+    if ((int)threadIdx.x < nr) {
+        T acc;
+        range = tile_row_outside<T>(sv, sc, lp, threadIdx.x, true, acc);
+        zl[threadIdx.x] = rv - acc;
+    }
+    __syncthreads();
+    for (int l = 0; l < nlev; ++l) {
+        const int i = threadIdx.x;
+        if (i >= lv[l] && i < lv[l + 1]) zl[i] -= tile_row_inside<T>(sv, sc, zl, range);
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < nr) z[r0 + threadIdx.x] = zl[threadIdx.x];
+}
+// backward sweep of the tiles [k0, k0 + gridDim.x), in place: z_t = (z_t - sum_U u_tj z_j) / u_tt
+template <class T>
+__global__ void __launch_bounds__(TBLK) k_ilu_tile_upper(int32_t k0, const int32_t* __restrict__ tinfo,
+                                                         const int32_t* __restrict__ lev, const int32_t* __restrict__ uptr,
+                                                         const int32_t* __restrict__ ucol, const T* __restrict__ uval,
+                                                         const T* __restrict__ dval, T* __restrict__ z,
+                                                         const int* __restrict__ ctrl, int32_t nlev, int32_t emax, int32_t rmax) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int32_t lv[ILU_MAXLEV + 1];
+    const int64_t k = k0 + (int64_t)xcd_block();
+    const int go = ctrl_go(ctrl);
+    const int32_t r0 = tinfo[8 * k], r1 = tinfo[8 * k + 1], base = tinfo[8 * k + 4], end = tinfo[8 * k + 5];
+    if (ctrl_stop(go, r0 <= r1)) return;
+    T* sv = reinterpret_cast<T*>(lds);
+    T* zl = sv + emax;
+    int16_t* sc = reinterpret_cast<int16_t*>(zl + rmax);
+    int16_t* lp = sc + emax;
+    const int nr = r1 - r0;
+    if (threadIdx.x <= ILU_MAXLEV) lv[threadIdx.x] = lev[k * (ILU_MAXLEV + 1) + threadIdx.x] - r0;
+    T zv = 0, dv = 1;
+    if ((int)threadIdx.x < nr) {  // one row per thread (nr <= TBLK)
+        zv = z[r0 + threadIdx.x];
+        dv = dval[r0 + threadIdx.x];
+    }
+    tile_stage<T>(uptr, ucol, uval, z, r0, r1, base, end - base, false, sv, sc, lp);
+    __syncthreads();
+    int range = 0;
+    if ((int)threadIdx.x < nr) {
+        T acc;
+        range = tile_row_outside<T>(sv, sc, lp, threadIdx.x, false, acc);
+        zl[threadIdx.x] = zv - acc;
+    }
+    __syncthreads();
+    for (int l = nlev - 1; l >= 0; --l) {
+        const int i = threadIdx.x;
+        if (i >= lv[l] && i < lv[l + 1]) zl[i] = (zl[i] - tile_row_inside<T>(sv, sc, zl, range)) / dv;
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < nr) z[r0 + threadIdx.x] = zl[threadIdx.x];
+}
+// IKJ ILU(0) rows of one (tile colour, level) group: the same update as k_ilu_factor
+template <class T>
+__global__ void __launch_bounds__(BLK) k_ilu_factor_rows(const int32_t* __restrict__ rows, int32_t cnt,
+                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                         const int32_t* __restrict__ diag, T* __restrict__ lu) {
+    const int32_t q = blockIdx.x * BLK + threadIdx.x;
+    if (q >= cnt) return;
+    const int32_t t = rows[q];
+    const int32_t e = rp[t + 1], dt = diag[t];
+    for (int32_t kk = rp[t]; kk < dt; ++kk) {
+        const int32_t k = col[kk];
+        const T lik = lu[kk] / lu[diag[k]];
+        lu[kk] = lik;
+        int32_t p = kk + 1;
+        for (int32_t qq = diag[k] + 1; qq < rp[k + 1]; ++qq) {
+            const int32_t j = col[qq];
+            while (p < e && col[p] < j) ++p;
+            if (p < e && col[p] == j) lu[p] -= lik * lu[qq];
+        }
+    }
+}
+
 // out[t] = in[perm[t]]
 template <class T>
 __global__ void __launch_bounds__(BLK) k_gather_perm(int64_t n, const int32_t* __restrict__ perm, const T* __restrict__ in,
@@ -1055,6 +1240,31 @@ void launch_timed(Ctx& c, int kclass, double bytes, K kernel, dim3 grid, dim3 bl
     }
 }
 
+// the same with `shmem` bytes of dynamic LDS
+template <typename K, typename... Args>
+void launch_timed_lds(Ctx& c, int kclass, double bytes, K kernel, dim3 grid, dim3 block, size_t shmem, Args... args) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c.timing_mask & (1u << kclass)) {
+        if (c.ev_next + 2 > c.ev_pool.size()) {
+            for (int k = 0; k < 256; ++k) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) break;
+                c.ev_pool.push_back(e);
+            }
+        }
+        if (c.ev_next + 2 <= c.ev_pool.size()) {
+            a = c.ev_pool[c.ev_next++];
+            b = c.ev_pool[c.ev_next++];
+        }
+    }
+    if (a) {
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, c.stream, a, b, 0, args...);
+        c.pending.push_back({kclass, bytes, a, b, c.cur_step});
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, c.stream, args...);
+    }
+}
+
 struct Work {
     int64_t n, nnz;
     int64_t ldv;  // basis stride: n rounded up to 64 entries (16-byte aligned columns)
@@ -1176,8 +1386,16 @@ void mdot_launch(Ctx& c, Work& w, int nv, const T* V, const T* wv, double* part,
                        hout, (const int*)w.ctrl);
 }
 
-// Multicolour ordering of the matrix pattern (host, ilu_order.cpp), the colour-major CSR
-// and the split triangle index arrays, uploaded once per pattern.
+// LDS of a tile sweep workgroup (value size vb): staged entries and their 16-bit local columns,
+// the tile's z (and the diagonal for the backward sweep), 16-bit row offsets
+constexpr int64_t ILU_TILE_LDS = 64 * 1024 - 256;
+inline int64_t tile_lds_bytes(int vb, int64_t emax, int64_t rmax, bool upper) {
+    (void)upper;
+    return vb * (emax + rmax) + 2 * (emax + rmax + 1);
+}
+
+// Row order of the matrix pattern (host, ilu_order.cpp: tiled, or multicolour), the permuted
+// CSR and the split triangle index arrays, uploaded once per pattern.
 int ilu_setup(Ctx& c, const SysView& sy) {
     IluData& I = *sy.ilu;
     if (I.ready) return VFVM_OK;
@@ -1185,6 +1403,38 @@ int ilu_setup(Ctx& c, const SysView& sy) {
     std::string err;
     const double* pos = c.g.h_pos.empty() ? nullptr : c.g.h_pos.data();
     if (ilu_order_build(sy.n, *sy.h_row, *sy.h_col, pos, c.g.N, o, err)) return set_error(VFVM_ERR_INVALID_ARGUMENT, err);
+    // the tiled order only when the tiles' FP64 entries fit the LDS of one workgroup (16-bit
+    // offsets inside a tile); else the multicolour order
+    if (!o.tile_row.empty() && (tile_lds_bytes(8, o.emax_l, o.rmax, false) > ILU_TILE_LDS ||
+                                tile_lds_bytes(8, o.emax_u, o.rmax, true) > ILU_TILE_LDS || o.emax_l > 32767 || o.emax_u > 32767))
+        if (ilu_order_build(sy.n, *sy.h_row, *sy.h_col, pos, c.g.N, o, err, false)) return set_error(VFVM_ERR_INVALID_ARGUMENT, err);
+    I.tiled = !o.tile_row.empty();
+    if (I.tiled) {
+        const int64_t nt = (int64_t)o.tile_row.size() - 1;
+        std::vector<int32_t> ti(8 * nt, 0);
+        for (int64_t k = 0; k < nt; ++k) {
+            const int32_t r0 = o.tile_row[k], r1 = o.tile_row[k + 1];
+            ti[8 * k] = r0; ti[8 * k + 1] = r1;
+            ti[8 * k + 2] = o.lptr[r0]; ti[8 * k + 3] = o.lptr[r1];
+            ti[8 * k + 4] = o.uptr[r0]; ti[8 * k + 5] = o.uptr[r1];
+        }
+        VFVM_HIP_TRY(I.tinfo.upload(ti.data(), ti.size(), c.stream));
+        VFVM_HIP_TRY(I.lev_row.upload(o.lev_row.data(), o.lev_row.size(), c.stream));
+        VFVM_HIP_TRY(I.fac_rows.upload(o.fac_rows.data(), o.fac_rows.size(), c.stream));
+        I.h_tc_ptr = o.tc_ptr;
+        I.h_fac_ptr = o.fac_ptr;
+        I.h_tile_row = o.tile_row;
+        I.h_lev_row = o.lev_row;
+        const int ntc = (int)o.tc_ptr.size() - 1;
+        I.h_tc_rows.assign(ntc, 0.0); I.h_tc_lent.assign(ntc, 0.0); I.h_tc_uent.assign(ntc, 0.0);
+        for (int q = 0; q < ntc; ++q)
+            for (int32_t k = o.tc_ptr[q]; k < o.tc_ptr[q + 1]; ++k) {
+                I.h_tc_rows[q] += ti[8 * k + 1] - ti[8 * k];
+                I.h_tc_lent[q] += ti[8 * k + 3] - ti[8 * k + 2];
+                I.h_tc_uent[q] += ti[8 * k + 5] - ti[8 * k + 4];
+            }
+        I.nlev = o.nlev; I.rmax = o.rmax; I.emax_l = o.emax_l; I.emax_u = o.emax_u;
+    }
     VFVM_HIP_TRY(I.perm.upload(o.perm.data(), o.perm.size(), c.stream));
     VFVM_HIP_TRY(I.prp.upload(o.prp.data(), o.prp.size(), c.stream));
     VFVM_HIP_TRY(I.src.upload(o.src.data(), o.src.size(), c.stream));
@@ -1211,7 +1461,14 @@ template <class T>
 int ilu_factor(Ctx& c, const SysView& sy, T* lu, T* full, const double* val) {
     const IluData& I = *sy.ilu;
     hipLaunchKernelGGL((k_ilu_gather<T>), dim3(grid_for(sy.nnz)), dim3(BLK), 0, c.stream, sy.nnz, I.src.p, val, full);
-    const int ncol = (int)I.h_color_ptr.size() - 1;
+    const int ncol = I.tiled ? 0 : (int)I.h_color_ptr.size() - 1;
+    // tiled order: the rows of every (tile colour, level) group are independent
+    for (size_t g = 0; I.tiled && g + 1 < I.h_fac_ptr.size(); ++g) {
+        const int32_t s = I.h_fac_ptr[g], e = I.h_fac_ptr[g + 1];
+        if (e > s)
+            hipLaunchKernelGGL((k_ilu_factor_rows<T>), dim3(grid_for(e - s)), dim3(BLK), 0, c.stream, I.fac_rows.p + s, e - s,
+                               I.prp.p, I.pcol.p, I.pdiag.p, full);
+    }
     for (int q = 0; q < ncol; ++q) {
         int32_t s = I.h_color_ptr[q], e = I.h_color_ptr[q + 1];
         if (e > s)
@@ -1249,6 +1506,26 @@ int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double
         const double per_row = ent / (double)std::max<int64_t>(sy.n, 1);
         return (double)(e - s) * (per_row * (sizeof(T) + 4.0) + 8.0 + 2.0 * sizeof(T) + ((final || !lower) ? sizeof(T) : 0.0));
     };
+    if (I.tiled) {
+        // one launch per tile colour and sweep (ascending forward, descending backward)
+        const int ntc = (int)I.h_tc_ptr.size() - 1;
+        const size_t lds_l = (size_t)tile_lds_bytes(sizeof(T), I.emax_l, I.rmax, false);
+        const size_t lds_u = (size_t)tile_lds_bytes(sizeof(T), I.emax_u, I.rmax, true);
+        for (int q = 0; q < ntc; ++q) {
+            const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
+            const double bytes = I.h_tc_lent[q] * (sizeof(T) + 4.0) + I.h_tc_rows[q] * (4.0 + 2.0 * sizeof(T));
+            launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_lower<T>, dim3(nt), dim3(TBLK), lds_l, k0, I.tinfo.p,
+                             I.lev_row.p, I.lptr.p, I.lcol.p, lval, rhs, rscale, z, ctrl, I.nlev, I.emax_l, I.rmax);
+        }
+        for (int q = ntc - 1; q >= 0; --q) {
+            const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
+            const double bytes = I.h_tc_uent[q] * (sizeof(T) + 4.0) + I.h_tc_rows[q] * (4.0 + 3.0 * sizeof(T));
+            launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_upper<T>, dim3(nt), dim3(TBLK), lds_u, k0, I.tinfo.p,
+                             I.lev_row.p, I.uptr.p, I.ucol.p, uval, dval, z, ctrl, I.nlev, I.emax_u, I.rmax);
+        }
+        VFVM_HIP_TRY(hipGetLastError());
+        return VFVM_OK;
+    }
     int q0 = 0;
     if (ncol >= 2 && I.h_color_ptr[0] == 0 && I.h_color_ptr[2] > I.h_color_ptr[1]) {
         // colours 0 and 1 in one launch (colour 0 is a scaled copy)
@@ -1590,6 +1867,23 @@ int ilu_order(Ctx& c, int32_t* perm, int32_t* color_ptr, int32_t* ncolors) {
     if (ncolors) *ncolors = nc;
     if (color_ptr) std::memcpy(color_ptr, I.h_color_ptr.data(), I.h_color_ptr.size() * sizeof(int32_t));
     if (perm) VFVM_HIP_TRY(hipMemcpy(perm, I.perm.p, sy.n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return VFVM_OK;
+}
+
+int ilu_tiles(Ctx& c, int32_t* tile_ptr, int32_t* ntiles, int32_t* level) {
+    const SysView sy = primal_view(c);
+    if (int s = ilu_setup(c, sy)) return s;
+    const IluData& I = *sy.ilu;
+    const int32_t nt = I.tiled ? (int32_t)I.h_tile_row.size() - 1 : 0;
+    if (ntiles) *ntiles = nt;
+    if (tile_ptr && nt > 0) std::memcpy(tile_ptr, I.h_tile_row.data(), I.h_tile_row.size() * sizeof(int32_t));
+    if (level) {
+        for (int64_t t = 0; t < sy.n; ++t) level[t] = 0;
+        for (int32_t k = 0; k < nt; ++k)
+            for (int l = 0; l < ILU_MAXLEV; ++l)
+                for (int32_t r = I.h_lev_row[(size_t)k * (ILU_MAXLEV + 1) + l]; r < I.h_lev_row[(size_t)k * (ILU_MAXLEV + 1) + l + 1]; ++r)
+                    level[r] = l;
+    }
     return VFVM_OK;
 }
 

@@ -470,6 +470,16 @@ class Context:
         check(lib.vfvm_ilu_order(self._h, ptr(perm, C.c_int32), ptr(cptr, C.c_int32), C.byref(nc)))
         return perm, cptr[:nc.value + 1].copy()
 
+    def ilu_tiles(self):
+        """(tile_ptr, level): the tiles of the tiled ILU(0) order over the permuted rows and the
+        level of each position inside its tile (tile_ptr empty for the multicolour order)."""
+        n = self._size()
+        tp = np.empty(n + 1, np.int32)
+        lev = np.empty(n, np.int32)
+        nt = C.c_int32()
+        check(lib.vfvm_ilu_tiles(self._h, ptr(tp, C.c_int32), C.byref(nt), ptr(lev, C.c_int32)))
+        return (tp[:nt.value + 1].copy() if nt.value > 0 else np.zeros(0, np.int32)), lev
+
     def steady_solve(self, ops, max_iter, tol, damp, lin=None):
         """SteadyBase::solve; lin: dict of linear-solver options (solve_opts keywords)."""
         arr = make_ops(ops)
