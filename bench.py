@@ -517,7 +517,7 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="lattice points per axis")
     ap.add_argument("--precond", default="ilu0", choices=["jacobi", "ilu0", "none"])
     ap.add_argument("--restart", type=int, default=50)
-    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--inner-tol", type=float, default=1e-6)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--cpu-sample-steps", type=int, default=50, help="Arnoldi steps timed on the CPU (0: skip)")
     ap.add_argument("--adjoint-n", type=int, default=126, help="lattice points per axis of the adjoint workload")

@@ -1011,6 +1011,82 @@ __global__ void __launch_bounds__(TBLK) k_ilu_tile_upper(int32_t k0, const int32
     }
     if ((int)threadIdx.x < nr) z[r0 + threadIdx.x] = zl[threadIdx.x];
 }
+// Both sweeps of the last tile colour in one launch: its U entries all lie inside the tile (no
+// later tile colour), so after the forward levels the same LDS holds the U entries, loaded into
+// registers while the forward sweep runs, and the levels are walked back.  LDS as the forward
+// sweep with emax = max(L, U entries of a tile).
+template <class T>
+__global__ void __launch_bounds__(TBLK) k_ilu_tile_last(int32_t k0, const int32_t* __restrict__ tinfo,
+                                                        const int32_t* __restrict__ lev, const int32_t* __restrict__ lptr,
+                                                        const int32_t* __restrict__ lcol, const T* __restrict__ lval,
+                                                        const int32_t* __restrict__ uptr, const int32_t* __restrict__ ucol,
+                                                        const T* __restrict__ uval, const T* __restrict__ dval,
+                                                        const T* __restrict__ rhs, const double* __restrict__ rscale,
+                                                        T* __restrict__ z, const int* __restrict__ ctrl, int32_t nlev,
+                                                        int32_t emax, int32_t rmax) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int32_t lv[ILU_MAXLEV + 1];
+    const int64_t k = k0 + (int64_t)xcd_block();
+    const int go = ctrl_go(ctrl);
+    const int32_t r0 = tinfo[8 * k], r1 = tinfo[8 * k + 1], base = tinfo[8 * k + 2], end = tinfo[8 * k + 3];
+    const int32_t ubase = tinfo[8 * k + 4], uend = tinfo[8 * k + 5];
+    if (ctrl_stop(go, r0 <= r1)) return;
+    T* sv = reinterpret_cast<T*>(lds);
+    T* zl = sv + emax;
+    int16_t* sc = reinterpret_cast<int16_t*>(zl + rmax);
+    int16_t* lp = sc + emax;
+    const int nr = r1 - r0;
+    const T c = (T)scale_of(rscale);
+    if (threadIdx.x <= ILU_MAXLEV) lv[threadIdx.x] = lev[k * (ILU_MAXLEV + 1) + threadIdx.x] - r0;
+    T rv = 0, dv = 1;
+    int urange = 0;
+    if ((int)threadIdx.x < nr) {  // one row per thread (nr <= TBLK)
+        rv = (T)(rhs[r0 + threadIdx.x] * c);
+        dv = dval[r0 + threadIdx.x];
+        urange = ((uptr[r0 + threadIdx.x] - ubase) << 16) | (uptr[r0 + threadIdx.x + 1] - ubase);
+    }
+    // the tile's U entries, held in registers through the forward sweep
+    const int32_t nu = uend - ubase;
+    int32_t uj[TSTAGE];
+    T uv[TSTAGE];
+#pragma unroll
+    for (int u = 0; u < TSTAGE; ++u) {
+        const int e = u * TBLK + threadIdx.x;
+        uj[u] = e < nu ? ucol[ubase + e] : r0;
+        uv[u] = e < nu ? uval[ubase + e] : (T)0;
+    }
+    tile_stage<T>(lptr, lcol, lval, z, r0, r1, base, end - base, true, sv, sc, lp);
+    __syncthreads();
+    int range = 0;
+    if ((int)threadIdx.x < nr) {
+        T acc;
+        range = tile_row_outside<T>(sv, sc, lp, threadIdx.x, true, acc);
+        zl[threadIdx.x] = rv - acc;
+    }
+    __syncthreads();
+    for (int l = 0; l < nlev; ++l) {
+        const int i = threadIdx.x;
+        if (i >= lv[l] && i < lv[l + 1]) zl[i] -= tile_row_inside<T>(sv, sc, zl, range);
+        __syncthreads();
+    }
+    // backward: the U entries (all inside the tile) replace the L entries in LDS
+#pragma unroll
+    for (int u = 0; u < TSTAGE; ++u) {
+        const int e = u * TBLK + threadIdx.x;
+        if (e < nu) {
+            sv[e] = uv[u];
+            sc[e] = (int16_t)(uj[u] - r0);
+        }
+    }
+    __syncthreads();
+    for (int l = nlev - 1; l >= 0; --l) {
+        const int i = threadIdx.x;
+        if (i >= lv[l] && i < lv[l + 1]) zl[i] = (zl[i] - tile_row_inside<T>(sv, sc, zl, urange)) / dv;
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < nr) z[r0 + threadIdx.x] = zl[threadIdx.x];
+}
+
 // IKJ ILU(0) rows of one (tile colour, level) group: the same update as k_ilu_factor
 template <class T>
 __global__ void __launch_bounds__(BLK) k_ilu_factor_rows(const int32_t* __restrict__ rows, int32_t cnt,
@@ -1406,7 +1482,8 @@ int ilu_setup(Ctx& c, const SysView& sy) {
     // the tiled order only when the tiles' FP64 entries fit the LDS of one workgroup (16-bit
     // offsets inside a tile); else the multicolour order
     if (!o.tile_row.empty() && (tile_lds_bytes(8, o.emax_l, o.rmax, false) > ILU_TILE_LDS ||
-                                tile_lds_bytes(8, o.emax_u, o.rmax, true) > ILU_TILE_LDS || o.emax_l > 32767 || o.emax_u > 32767))
+                                tile_lds_bytes(8, o.emax_u, o.rmax, true) > ILU_TILE_LDS || o.emax_l > 32767 || o.emax_u > 32767 ||
+                                o.emax_u > TBLK * TSTAGE))
         if (ilu_order_build(sy.n, *sy.h_row, *sy.h_col, pos, c.g.N, o, err, false)) return set_error(VFVM_ERR_INVALID_ARGUMENT, err);
     I.tiled = !o.tile_row.empty();
     if (I.tiled) {
@@ -1511,13 +1588,23 @@ int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double
         const int ntc = (int)I.h_tc_ptr.size() - 1;
         const size_t lds_l = (size_t)tile_lds_bytes(sizeof(T), I.emax_l, I.rmax, false);
         const size_t lds_u = (size_t)tile_lds_bytes(sizeof(T), I.emax_u, I.rmax, true);
-        for (int q = 0; q < ntc; ++q) {
+        for (int q = 0; q + 1 < ntc; ++q) {
             const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
             const double bytes = I.h_tc_lent[q] * (sizeof(T) + 4.0) + I.h_tc_rows[q] * (4.0 + 2.0 * sizeof(T));
             launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_lower<T>, dim3(nt), dim3(TBLK), lds_l, k0, I.tinfo.p,
                              I.lev_row.p, I.lptr.p, I.lcol.p, lval, rhs, rscale, z, ctrl, I.nlev, I.emax_l, I.rmax);
         }
-        for (int q = ntc - 1; q >= 0; --q) {
+        {
+            // the last tile colour: both sweeps in one launch
+            const int q = ntc - 1;
+            const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
+            const int32_t emax = std::max(I.emax_l, I.emax_u);
+            const size_t lds = (size_t)tile_lds_bytes(sizeof(T), emax, I.rmax, false);
+            const double bytes = (I.h_tc_lent[q] + I.h_tc_uent[q]) * (sizeof(T) + 4.0) + I.h_tc_rows[q] * (8.0 + 3.0 * sizeof(T));
+            launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_last<T>, dim3(nt), dim3(TBLK), lds, k0, I.tinfo.p, I.lev_row.p,
+                             I.lptr.p, I.lcol.p, lval, I.uptr.p, I.ucol.p, uval, dval, rhs, rscale, z, ctrl, I.nlev, emax, I.rmax);
+        }
+        for (int q = ntc - 2; q >= 0; --q) {
             const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
             const double bytes = I.h_tc_uent[q] * (sizeof(T) + 4.0) + I.h_tc_rows[q] * (4.0 + 3.0 * sizeof(T));
             launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_upper<T>, dim3(nt), dim3(TBLK), lds_u, k0, I.tinfo.p,
