@@ -498,6 +498,68 @@ def measured_traffic(kernel, variant):
     return value, "rocprofv3 FETCH_SIZE + WRITE_SIZE of the current kernel source, " + os.path.relpath(path, ROOT)
 
 
+def kernel_class(name):
+    """Kernel class of a rocprofv3 kernel name (the classes bench.py times with events)."""
+    if "k_ilu_tile" in name or "k_ilu_lower" in name or "k_ilu_upper" in name:
+        return "ilu_sweep"
+    if "k_spmv" in name:
+        return "spmv"
+    if "copyBuffer" in name or "elementwise" in name or "copy_kernel" in name:
+        return "copy"
+    return None
+
+
+def pmc_summary(fetch_dir, write_dir, copy_mb):
+    """Markdown summary of two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch) of
+    the same command: per kernel the average counters, scaled to bytes by what the same counters
+    read for the plain device copy of copy_mb MiB in that command; the per-launch HBM traffic of
+    the ILU(0) sweep and SpMV classes in the lines measured_traffic() reads, with the sha256 of the
+    kernel source they were measured on."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        per = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name = row["Kernel_Name"]
+                    per.setdefault(name, []).append(float(row["Counter_Value"]))
+        return per
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    copy_bytes = copy_mb * float(1 << 20)
+    cf = [max(v) for k, v in fetch.items() if kernel_class(k) == "copy"]
+    cw = [max(v) for k, v in write.items() if kernel_class(k) == "copy"]
+    f_scale = copy_bytes / (max(cf) * 1024.0) if cf else 1.0
+    w_scale = copy_bytes / (max(cw) * 1024.0) if cw else 1.0
+    out = [f"Calibration: the {copy_mb} MiB device copy reads {max(cf) if cf else float('nan'):.0f} KiB by FETCH_SIZE and "
+           f"writes {max(cw) if cw else float('nan'):.0f} KiB by WRITE_SIZE: scale factors {f_scale:.3f} (fetch) and "
+           f"{w_scale:.3f} (write) to bytes.", "",
+           "| kernel | dispatches | FETCH_SIZE avg KiB | WRITE_SIZE avg KiB | scaled HBM bytes per launch |",
+           "|---|---:|---:|---:|---:|"]
+    cls_tot = {}
+    for name in sorted(fetch, key=lambda k: -sum(fetch[k])):
+        c = kernel_class(name)
+        if c not in ("ilu_sweep", "spmv"):
+            continue
+        fv = fetch[name]
+        wv = write.get(name, [0.0])
+        fa, wa = sum(fv) / len(fv), sum(wv) / len(wv)
+        b = fa * 1024.0 * f_scale + wa * 1024.0 * w_scale
+        out.append(f"| `{name[:90]}` | {len(fv)} | {fa:.1f} | {wa:.1f} | {b:.4e} |")
+        t = cls_tot.setdefault(c, [0, 0.0])
+        t[0] += len(fv)
+        t[1] += b * len(fv)
+    out.append("")
+    for c, (cnt, tot) in sorted(cls_tot.items()):
+        out.append(f"traffic_bytes_per_launch {c} {tot / cnt:.6e}")
+    out.append(f"source_sha256 krylov.hip {kernel_source_hash()}")
+    return "\n".join(out)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -548,9 +610,15 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first copy MB megabytes device-to-device 5 times (a plain copy kernel of known size, "
                          "used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="print the traffic summary of two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of a command "
+                         "with --copy-calibration MB, and exit (no GPU work)")
     ap.add_argument("--variants", default="fp64,mixed")
     ap.add_argument("--ortho", default="cgs", choices=["auto", "cgs", "cgs2"])
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.copy_calibration or 256))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design only)", file=sys.stderr)

@@ -157,6 +157,7 @@ struct IluData {
     bool tiled = false;
     DevBuf<int32_t> tinfo, lev_row, fac_rows;
     std::vector<int32_t> h_tc_ptr, h_fac_ptr, h_tile_row, h_lev_row;
+    std::vector<int32_t> h_prp;  // host copy of the permuted CSR's row pointers
     std::vector<double> h_tc_rows, h_tc_lent, h_tc_uent;  // per tile colour: rows, L and U entries
     int32_t nlev = 0, rmax = 0, emax_l = 0, emax_u = 0;
     bool ready = false;
@@ -181,6 +182,8 @@ struct SysView {
     // generic CSR whose rows, columns and vectors are numbered vertex-interleaved (unknown c of
     // vertex v at (n / N) v + c): block Jacobi reads and writes its blocks there
     bool interleaved = false;
+    // most entries held by one block of SPMV_ROWS consecutive rows (0: the per-row SpMV kernel)
+    int32_t spmv_emax = 0;
 };
 
 // Checkpoint tape of the adjoint forward pass: the operator list, step data and the

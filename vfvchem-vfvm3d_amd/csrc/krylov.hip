@@ -123,6 +123,70 @@ __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restri
     if (lane == 0) y[row] = (T)(s * sc);
 }
 
+// SpMV over blocks of SPMV_ROWS consecutive rows, one workgroup each: the block's entries are
+// streamed with SPMV_STAGE independent column / value loads and x gathers in flight per thread
+// (the per-row kernel keeps two per lane), their products kept in LDS, and each row then sums
+// its products in column order.  emax: most entries of a block (the LDS capacity).
+constexpr int SPMV_ROWS = 128;
+constexpr int SPMV_BLK = 256;
+constexpr int SPMV_STAGE = 8;
+template <class T>
+__global__ void __launch_bounds__(SPMV_BLK) k_spmv_chunk(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                         const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
+                                                         const double* __restrict__ scale, const int* __restrict__ ctrl,
+                                                         int32_t emax) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    T* prod = reinterpret_cast<T*>(lds);
+    int32_t* lrp = reinterpret_cast<int32_t*>(prod + emax);
+    const int64_t r0 = (int64_t)xcd_block() * SPMV_ROWS;
+    const int64_t r1 = r0 + SPMV_ROWS < n ? r0 + SPMV_ROWS : n;
+    const int nr = (int)(r1 - r0);
+    const int go = ctrl_go(ctrl);
+    const int32_t base = rp[r0], ne = rp[r1] - base;
+    const T sc = (T)scale_of(scale);
+    if (ctrl_stop(go, (base <= base + ne) | (sc != sc))) return;
+    for (int i = threadIdx.x; i <= nr; i += SPMV_BLK) lrp[i] = rp[r0 + i] - base;
+    for (int e0 = 0; e0 < ne; e0 += SPMV_BLK * SPMV_STAGE) {
+        int32_t j[SPMV_STAGE];
+        T a[SPMV_STAGE];
+#pragma unroll
+        for (int u = 0; u < SPMV_STAGE; ++u) {
+            const int e = e0 + u * SPMV_BLK + threadIdx.x;
+            j[u] = e < ne ? col[base + e] : 0;
+            a[u] = e < ne ? val[base + e] : (T)0;
+        }
+        T xv[SPMV_STAGE];
+#pragma unroll
+        for (int u = 0; u < SPMV_STAGE; ++u) xv[u] = x[j[u]];
+#pragma unroll
+        for (int u = 0; u < SPMV_STAGE; ++u) {
+            const int e = e0 + u * SPMV_BLK + threadIdx.x;
+            if (e < ne) prod[e] = a[u] * xv[u];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nr; i += SPMV_BLK) {
+        const int q1 = lrp[i + 1];
+        T s = 0;
+#pragma unroll 4
+        for (int q = lrp[i]; q < q1; ++q) s += prod[q];
+        y[r0 + i] = (T)(s * sc);
+    }
+}
+// most entries of a block of SPMV_ROWS rows of the CSR row pointers rp
+inline int32_t spmv_block_emax(const std::vector<int32_t>& rp) {
+    const int64_t n = (int64_t)rp.size() - 1;
+    int32_t m = 0;
+    for (int64_t r0 = 0; r0 < n; r0 += SPMV_ROWS) {
+        const int64_t r1 = std::min<int64_t>(n, r0 + SPMV_ROWS);
+        m = std::max(m, rp[r1] - rp[r0]);
+    }
+    return m;
+}
+// LDS of a k_spmv_chunk block, and the most it may take
+inline size_t spmv_chunk_lds(int vb, int64_t emax) { return (size_t)(vb * emax + 4 * (SPMV_ROWS + 1)); }
+constexpr size_t SPMV_CHUNK_LDS = 48 * 1024;
+
 // SpMV of the coupled flow system [u, v, w, p] (rows and columns comp-major), one wave per
 // vertex v computing its four rows together.  The three velocity rows of a point share their
 // column sets ({v} u ngh(v) in their own component, grad_coeff(v) in p) and the continuity
@@ -1404,6 +1468,12 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }
+    if (sy.spmv_emax > 0 && lpr <= 8 && spmv_chunk_lds(sizeof(T), sy.spmv_emax) <= SPMV_CHUNK_LDS) {
+        launch_timed_lds(c, VFVM_K_SPMV, bytes, k_spmv_chunk<T>, dim3((unsigned)((n + SPMV_ROWS - 1) / SPMV_ROWS)), dim3(SPMV_BLK),
+                         spmv_chunk_lds(sizeof(T), sy.spmv_emax), n, rp, cl, val, x, y, scale, ctrl, sy.spmv_emax);
+        VFVM_HIP_TRY(hipGetLastError());
+        return VFVM_OK;
+    }
     switch (lpr) {
         case 4: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 4>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
         case 8: launch_timed(c, VFVM_K_SPMV, bytes, k_spmv<T, 8>, dim3(grid), dim3(BLK), n, rp, cl, val, x, y, scale, ctrl); break;
@@ -1514,6 +1584,7 @@ int ilu_setup(Ctx& c, const SysView& sy) {
     }
     VFVM_HIP_TRY(I.perm.upload(o.perm.data(), o.perm.size(), c.stream));
     VFVM_HIP_TRY(I.prp.upload(o.prp.data(), o.prp.size(), c.stream));
+    I.h_prp = o.prp;
     VFVM_HIP_TRY(I.src.upload(o.src.data(), o.src.size(), c.stream));
     VFVM_HIP_TRY(I.pcol.upload(o.pcol.data(), o.pcol.size(), c.stream));
     VFVM_HIP_TRY(I.pdiag.upload(o.pdiag.data(), o.pdiag.size(), c.stream));
@@ -1588,13 +1659,16 @@ int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double
         const int ntc = (int)I.h_tc_ptr.size() - 1;
         const size_t lds_l = (size_t)tile_lds_bytes(sizeof(T), I.emax_l, I.rmax, false);
         const size_t lds_u = (size_t)tile_lds_bytes(sizeof(T), I.emax_u, I.rmax, true);
-        for (int q = 0; q + 1 < ntc; ++q) {
+        // FP32: the last tile colour's two sweeps share a launch (FP64: the fused kernel's U entries
+        // held in registers cost more than the launch it saves, 35 against 14 + 16 us at 100^3)
+        const bool fuse = sizeof(T) == 4;
+        for (int q = 0; q + (fuse ? 1 : 0) < ntc; ++q) {
             const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
             const double bytes = I.h_tc_lent[q] * (sizeof(T) + 4.0) + I.h_tc_rows[q] * (4.0 + 2.0 * sizeof(T));
             launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_lower<T>, dim3(nt), dim3(TBLK), lds_l, k0, I.tinfo.p,
                              I.lev_row.p, I.lptr.p, I.lcol.p, lval, rhs, rscale, z, ctrl, I.nlev, I.emax_l, I.rmax);
         }
-        {
+        if (fuse) {
             // the last tile colour: both sweeps in one launch
             const int q = ntc - 1;
             const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
@@ -1604,7 +1678,7 @@ int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double
             launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_last<T>, dim3(nt), dim3(TBLK), lds, k0, I.tinfo.p, I.lev_row.p,
                              I.lptr.p, I.lcol.p, lval, I.uptr.p, I.ucol.p, uval, dval, rhs, rscale, z, ctrl, I.nlev, emax, I.rmax);
         }
-        for (int q = ntc - 2; q >= 0; --q) {
+        for (int q = ntc - (fuse ? 2 : 1); q >= 0; --q) {
             const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
             const double bytes = I.h_tc_uent[q] * (sizeof(T) + 4.0) + I.h_tc_rows[q] * (4.0 + 3.0 * sizeof(T));
             launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_upper<T>, dim3(nt), dim3(TBLK), lds_u, k0, I.tinfo.p,
@@ -2193,6 +2267,12 @@ int gmres_run(Ctx& c, const SysView& sy0, const vfvm_solve_opts& o, double* x0, 
         sy.vals = c.ap64.p;
         sy.b = c.bp64.p;
         x = c.xp64.p;
+    }
+    // the block SpMV of a generic CSR: its blocks' largest entry count, from the host row
+    // pointers of the rows the solve multiplies by
+    if (!sy.flow_layout && !sy.interleaved) {
+        const std::vector<int32_t>* hr = ilu ? &sy0.ilu->h_prp : sy0.h_row;
+        sy.spmv_emax = (hr && (int64_t)hr->size() == sy.n + 1) ? spmv_block_emax(*hr) : 0;
     }
     const int64_t n = sy.n;
     const int m = o.restart;
