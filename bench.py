@@ -480,7 +480,7 @@ def measured_traffic(kernel, variant):
     summary of this workload (scaled by a copy kernel of known size), or None; only a summary
     measured on the current kernel source (its recorded sha256 equal to krylov.hip's) counts.
     Returns (bytes or None, note)."""
-    path = os.path.join(ROOT, "profiles", "ilu_sweep_traffic_convdiff_100cubed.md")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_convdiff_100cubed.md")
     if variant != "mixed" or not os.path.exists(path):
         return None, "no counter summary for this variant"
     value, src = None, None
