@@ -979,6 +979,42 @@ def ilu0(A, perm):
     return L, U
 
 
+def ilu0_rows(A, perm):
+    """ilu0() with sparse rows, for n where a dense copy does not fit: the same IKJ updates in
+    the same order (pivot columns ascending, each pivot row's final U entries), so the factors
+    equal ilu0()'s bit for bit (pinned by that in the tests)."""
+    import scipy.sparse as sp
+    n = A.shape[0]
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), perm)), shape=(n, n))
+    Ap = (P @ A @ P.T).tocsr()
+    Ap.sort_indices()
+    ip, ix, dv = Ap.indptr, Ap.indices.tolist(), Ap.data.tolist()
+    urow = [None] * n  # row k's U entries right of the diagonal: list of (j, u_kj)
+    diag = [0.0] * n
+    lr, lc, lv, ur, uc, uv = [], [], [], [], [], []
+    for i in range(n):
+        cols = ix[ip[i]:ip[i + 1]]
+        w = dict(zip(cols, dv[ip[i]:ip[i + 1]]))
+        for k in cols:
+            if k >= i:
+                break
+            w[k] /= diag[k]
+            wk = w[k]
+            for j, ukj in urow[k]:
+                if j in w:
+                    w[j] -= wk * ukj
+        diag[i] = w[i]
+        urow[i] = [(j, w[j]) for j in cols if j > i]
+        for j in cols:
+            if j < i:
+                lr.append(i); lc.append(j); lv.append(w[j])
+            else:
+                ur.append(i); uc.append(j); uv.append(w[j])
+    L = sp.csr_matrix((lv + [1.0] * n, (lr + list(range(n)), lc + list(range(n)))), shape=(n, n))
+    U = sp.csr_matrix((uv, (ur, uc)), shape=(n, n))
+    return L, U
+
+
 # --------------------------------------------------------------------------- flow (SteadyFlow)
 # Pinning of this section (tests/test_cpu_flow_oracle.py), independent of the restatement:
 #   grad_coeff()     sum_m c_m p_m = V grad(p) for linear p at every vertex (divergence

@@ -225,6 +225,36 @@ def test_ilu0_application_vs_oracle(ctx, prob, n):
     assert np.allclose(zj, r / A.diagonal(), rtol=1e-15, atol=0)
 
 
+@pytest.mark.parametrize("prob,n", [("mass", 30), ("heat", 40)])
+def test_ilu0_one_launch_sweeps_at_size(ctx, prob, n):
+    """The tile sweeps of one application run in one launch, each tile waiting for the completion
+    flags of the adjacent tiles it reads: on meshes of many tiles in four tile colours, M^-1 r
+    against the oracle's ILU(0) (the sparse-row IKJ in the device's row order) and triangular
+    solves, and bitwise equal over repeated applications."""
+    print(f"[ilu0 one-launch sweeps {prob} n={n}]", flush=True)
+    mesh = vfvm.Mesh.lattice(n)
+    p = heat_c1(mesh) if prob == "heat" else mass_c2(mesh, c_out=0.0, prod=0.3)
+    rp, col, vals, b = _assembled(ctx, p)
+    perm, cptr = ctx.ilu_order()
+    tptr, level = ctx.ilu_tiles()
+    assert len(tptr) - 1 >= 50 and len(cptr) - 1 >= 3, (len(tptr), len(cptr))
+    n = len(rp) - 1
+    A = sp.csr_matrix((vals, col, rp), shape=(n, n))
+    L, U = O.ilu0_rows(A, perm)
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), perm)), shape=(n, n))
+    r = np.random.default_rng(6).normal(size=n)
+    y = spla.spsolve_triangular(L.tocsr(), P @ r, lower=True, unit_diagonal=True)
+    z_ref = P.T @ spla.spsolve_triangular(U.tocsr(), y, lower=False)
+    zs = [ctx.precond_apply(r, precond=vfvm.PRECOND_ILU0, precision=vfvm.FP64) for _ in range(3)]
+    assert np.abs(zs[0] - z_ref).max() <= 1e-12 * np.abs(z_ref).max()
+    for z in zs[1:]:
+        assert np.array_equal(z, zs[0])
+    z32 = [ctx.precond_apply(r, precond=vfvm.PRECOND_ILU0, precision=vfvm.FP32_REFINED) for _ in range(3)]
+    assert np.abs(z32[0] - z_ref).max() <= 1e-4 * np.abs(z_ref).max()
+    for z in z32[1:]:
+        assert np.array_equal(z, z32[0])
+
+
 @pytest.mark.parametrize("prec,pc", [(vfvm.FP64, vfvm.PRECOND_ILU0), (vfvm.FP32_REFINED, vfvm.PRECOND_ILU0),
                                      (vfvm.FP64, vfvm.PRECOND_JACOBI)])
 def test_cycle_graphs_match_eager_launches(ctx, prec, pc):

@@ -151,14 +151,18 @@ struct IluData {
     DevBuf<int32_t> lptr, lcol, lpos, uptr, ucol, upos;
     int64_t nnz_l = 0, nnz_u = 0;
     std::vector<int32_t> h_color_ptr;
-    // tiled order (the rows of a tile contiguous, its levels inside): per tile 8 ints
-    // {r0, r1, L entries begin, end, U entries begin, end, 0, 0}, the first row of every level of
-    // every tile, the factorisation's row groups; the sweeps take one launch per tile colour
+    // tiled order (the rows of a tile contiguous, its levels inside): the first row of every level
+    // of every tile, the factorisation's row groups, and the sweeps' roles (all tile sweeps of
+    // one application in one launch): per role 16 ints {r0, r1, L entries begin, end, U entries
+    // begin, end, tile, kind, dependencies begin, end, 0 ...}, the dependency lists, and the
+    // launches' ticket counter, lapsed-wait count and per-tile completion flags
     bool tiled = false;
-    DevBuf<int32_t> tinfo, lev_row, fac_rows;
-    std::vector<int32_t> h_tc_ptr, h_fac_ptr, h_tile_row, h_lev_row;
+    DevBuf<int32_t> lev_row, fac_rows, trole, tdep;
+    DevBuf<unsigned long long> tsync;
+    std::vector<int32_t> h_fac_ptr, h_tile_row, h_lev_row;
     std::vector<int32_t> h_prp;  // host copy of the permuted CSR's row pointers
-    std::vector<double> h_tc_rows, h_tc_lent, h_tc_uent;  // per tile colour: rows, L and U entries
+    int32_t nrole = 0;
+    double sweep_rows = 0.0, sweep_lent = 0.0, sweep_uent = 0.0, fused_rows = 0.0;  // algorithmic byte counts
     int32_t nlev = 0, rmax = 0, emax_l = 0, emax_u = 0;
     bool ready = false;
     uint64_t generation = 0;  // bumped by every set-up (graphs bake the layout in)

@@ -61,6 +61,68 @@ static void scalar_pattern(const vfvm_mesh_info& info, const std::vector<int32_t
     }
 }
 
+// The one-launch sweeps' roles: every tile swept forward once and backward once; every
+// dependency an earlier role (so every wait ends); a role depends on every tile whose z it
+// reads; a tile's backward sweep waits for the backward sweep of every tile whose forward
+// sweep read its forward z; a tile doing both sweeps holds all its U entries.
+static void check_tile_roles(const vfvm::IluOrder& o) {
+    std::vector<int32_t> role, dep;
+    check(vfvm::tile_roles_build(o, 512, role, dep), "tile_roles_build");
+    const int64_t nt = (int64_t)o.tile_row.size() - 1, n = o.tile_row[nt];
+    const int64_t nrole = (int64_t)role.size() / vfvm::TILE_ROLE_INTS;
+    std::vector<int64_t> fwd(nt, -1), bwd(nt, -1);
+    std::vector<int32_t> tile_of(n);
+    for (int64_t k = 0; k < nt; ++k)
+        for (int32_t r = o.tile_row[k]; r < o.tile_row[k + 1]; ++r) tile_of[r] = (int32_t)k;
+    bool once = true, earlier = true, reads = true, war = true, inside = true;
+    for (int64_t i = 0; i < nrole; ++i) {
+        const int32_t* R = role.data() + i * vfvm::TILE_ROLE_INTS;
+        const int32_t k = R[6];
+        if (R[7] & 1) { once = once && fwd[k] < 0; fwd[k] = i; }
+        if (R[7] & 2) { once = once && bwd[k] < 0; bwd[k] = i; }
+        once = once && R[0] == o.tile_row[k] && R[1] == o.tile_row[k + 1];
+    }
+    for (int64_t k = 0; k < nt; ++k) once = once && fwd[k] >= 0 && bwd[k] >= 0;
+    check(once, "every tile is swept forward once and backward once");
+    if (!once) return;
+    auto has = [&](const int32_t* R, int32_t d) {
+        for (int32_t q = R[8]; q < R[9]; ++q)
+            if (dep[q] == d) return true;
+        return false;
+    };
+    for (int64_t i = 0; i < nrole; ++i) {
+        const int32_t* R = role.data() + i * vfvm::TILE_ROLE_INTS;
+        const int32_t k = R[6];
+        for (int32_t q = R[8]; q < R[9]; ++q) {
+            const int32_t j = dep[q] >> 1;
+            earlier = earlier && ((dep[q] & 1) ? fwd[j] : bwd[j]) < i;
+        }
+        for (int32_t r = R[0]; r < R[1]; ++r) {
+            if (R[7] & 1)
+                for (int32_t e = o.lptr[r]; e < o.lptr[r + 1]; ++e) {
+                    const int32_t j = tile_of[o.lcol[e]];
+                    if (j == k) continue;
+                    reads = reads && has(R, (j << 1) | 1);
+                    // tile j's forward z is read here: j's backward sweep waits for this tile's
+                    const int32_t* B = role.data() + bwd[j] * vfvm::TILE_ROLE_INTS;
+                    war = war && has(B, k << 1);
+                }
+            if (R[7] == 3)
+                for (int32_t e = o.uptr[r]; e < o.uptr[r + 1]; ++e) inside = inside && tile_of[o.ucol[e]] == k;
+            if (R[7] == 2)
+                for (int32_t e = o.uptr[r]; e < o.uptr[r + 1]; ++e) {
+                    const int32_t j = tile_of[o.ucol[e]];
+                    if (j != k) reads = reads && has(R, j << 1);
+                }
+        }
+        if (R[7] == 2) reads = reads && has(R, (k << 1) | 1);
+    }
+    check(earlier, "every dependency of a role is an earlier role");
+    check(reads, "a role waits for every tile whose z it reads");
+    check(war, "a backward sweep waits for the forward sweeps that read its tile's z");
+    check(inside, "a tile doing both sweeps holds all its U entries");
+}
+
 static void check_ilu_order(const std::vector<int32_t>& mrow, const std::vector<int32_t>& mcol, const std::vector<double>& pos,
                             int64_t nvert) {
     const int64_t n = (int64_t)mrow.size() - 1;
@@ -96,11 +158,12 @@ static void check_ilu_order(const std::vector<int32_t>& mrow, const std::vector<
         for (int64_t t = 0; t < n; ++t)
             for (int32_t k = o.lptr[t]; k < o.lptr[t + 1]; ++k) lower = lower && o.lcol[k] < t;
         check(lower, "L entries lie below the diagonal");
+        if (!o.tile_row.empty()) check_tile_roles(o);
     }
 }
 
 int main() {
-    for (int n : {3, 6, 9}) {
+    for (int n : {3, 6, 9, 22}) {
         for (uint64_t seed : {0ull, 5ull}) {
             vfvm_mesh* m = nullptr;
             check(vfvm_mesh_lattice(n, seed, 0.3, &m) == VFVM_OK, "vfvm_mesh_lattice");

@@ -325,4 +325,55 @@ int ilu_order_build(int64_t n, const std::vector<int32_t>& rp, const std::vector
     return 0;
 }
 
+bool tile_roles_build(const IluOrder& o, int max_deps, std::vector<int32_t>& role, std::vector<int32_t>& dep) {
+    role.clear();
+    dep.clear();
+    const int64_t nt = (int64_t)o.tile_row.size() - 1;
+    const int ntc = (int)o.tc_ptr.size() - 1;
+    if (nt <= 0 || ntc <= 0) return false;
+    const int64_t n = o.tile_row[nt];
+    std::vector<int32_t> tile_of(n), colour(nt);
+    for (int64_t k = 0; k < nt; ++k)
+        for (int32_t r = o.tile_row[k]; r < o.tile_row[k + 1]; ++r) tile_of[r] = (int32_t)k;
+    for (int q = 0; q < ntc; ++q)
+        for (int32_t k = o.tc_ptr[q]; k < o.tc_ptr[q + 1]; ++k) colour[k] = q;
+    // tile adjacency of the symmetrised pattern
+    std::vector<std::vector<int32_t>> adj(nt);
+    for (int64_t r = 0; r < n; ++r)
+        for (int32_t e = o.prp[r]; e < o.prp[r + 1]; ++e) {
+            const int32_t a = tile_of[r], b = tile_of[o.pcol[e]];
+            if (a != b) {
+                adj[a].push_back(b);
+                adj[b].push_back(a);
+            }
+        }
+    for (auto& v : adj) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    auto add = [&](int32_t k, int kind) -> bool {
+        const size_t d0 = dep.size();
+        const int q = colour[k];
+        if (kind == 2) dep.push_back((k << 1) | 1);
+        for (int32_t j : adj[k]) {
+            if (colour[j] == q) return false;  // adjacent tiles of one colour: not a tile colouring
+            if (kind != 2 && colour[j] < q) dep.push_back((j << 1) | 1);
+            if (kind == 2 && colour[j] > q) dep.push_back(j << 1);
+        }
+        if ((int64_t)(dep.size() - d0) > max_deps) return false;
+        const int32_t r0 = o.tile_row[k], r1 = o.tile_row[k + 1];
+        const int32_t rec[TILE_ROLE_INTS] = {r0, r1, o.lptr[r0], o.lptr[r1], o.uptr[r0], o.uptr[r1], k, kind,
+                                             (int32_t)d0, (int32_t)dep.size(), 0, 0, 0, 0, 0, 0};
+        role.insert(role.end(), rec, rec + TILE_ROLE_INTS);
+        return true;
+    };
+    for (int q = 0; q < ntc; ++q)
+        for (int32_t k = o.tc_ptr[q]; k < o.tc_ptr[q + 1]; ++k)
+            if (!add(k, q == ntc - 1 ? 3 : 1)) return false;
+    for (int q = ntc - 2; q >= 0; --q)
+        for (int32_t k = o.tc_ptr[q]; k < o.tc_ptr[q + 1]; ++k)
+            if (!add(k, 2)) return false;
+    return true;
+}
+
 }  // namespace vfvm

@@ -912,54 +912,84 @@ __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const
     if (lane == 0) z[t] = (zt - s) / dd;
 }
 
-// ---- tiled order: one workgroup per tile, the tile's rows in LDS
-// A tile's rows are contiguous (r0 .. r1) and sorted by level; the tiles of one tile colour do
-// not touch, so each launch sweeps a whole tile colour.  Phase 1 stages the tile's triangle
-// entries in LDS: an entry whose column lies outside the tile (a finished tile colour) is
-// multiplied by its z at once, an entry inside keeps its value and local column.  Phase 2 walks
-// the levels (barrier between levels; the rows of one level do not touch), each row summing its
-// staged entries in column order, so only the tile's own z round-trips, through LDS.  Phase 3
-// writes the tile's z.  tinfo: per tile {r0, r1, L begin, L end, U begin, U end, -, -};
-// lev: per tile the first row of each of the ILU_MAXLEV levels and the tile's end.
-// LDS: T sv[emax], T zl[rmax], int16 sc[emax], int16 lp[rmax + 1].  One thread per row in
-// phase 2 (a tile holds at most TBLK rows).
-// A workgroup of TBLK threads stages TSTAGE entries per thread and round: all of a round's
-// column / value loads are issued before its gathers, so a tile of up to TBLK * TSTAGE entries
-// costs one round trip for its entries and one for their z.
+// ---- tiled order: one workgroup per tile, the tile's rows in LDS, one launch per application
+// A tile's rows are contiguous (r0 .. r1) and sorted by level; tiles of one tile colour do not
+// touch.  A tile's sweep stages its triangle entries in LDS (an entry whose column lies outside
+// the tile, in a finished tile, is multiplied by that z at once; an entry inside keeps its value
+// and local column), then walks the levels with a barrier between them (the rows of one level
+// do not touch), each row summing its entries in column order, so only the tile's own z
+// round-trips, through LDS; then it writes the tile's z.  LDS: T sv[emax], T zl[rmax], int16
+// sc[emax], int16 lp[rmax + 1].  One thread per row (a tile holds at most TBLK rows); a thread
+// loads up to TSTAGE entries, all issued before their gathers.
+//
+// All tile sweeps of one application run in one launch of nrole workgroups.  A workgroup takes
+// a ticket from a counter that is never reset (ticket t = app * nrole + i: role i of the app-th
+// launch), so tickets follow the order in which workgroups start.  Role i is the forward sweep
+// of a tile (tile colours ascending), both sweeps of a tile of the last tile colour (its U
+// entries all lie inside the tile), or the backward sweep of a tile (tile colours descending).
+// A role waits for the completion flags of the roles it reads: the forward sweeps of adjacent
+// tiles of lower colours; the backward sweeps of adjacent tiles of higher colours and its own
+// tile's forward sweep (which also covers the reads of its z by the forward sweeps of those
+// higher tiles).  Every such role comes earlier in the list, so its workgroup started before:
+// the earliest unfinished role waits for nothing unfinished and every wait ends.  The flag of
+// tile k reads 2 app + 1 after its forward sweep and 2 app + 2 after its backward sweep; a
+// role polls a flag at most TILE_SPIN_MAX times, then writes NaN into its z (the Krylov solve
+// stops with a non-finite result) and counts the lapse.  The XCDs' L2 caches are not coherent
+// with each other, and device-scope release / acquire fences write back / invalidate a whole
+// L2 (which costs a thousand times the sweep), so the sweeps' z and the flags are written and
+// read as device-scope atomics instead, which reach memory past the L2: a tile's z stores are
+// complete (s_waitcnt before the workgroup barrier) before its flag is stored, and a role issues
+// its z loads only after it has seen its flags.
 constexpr int TBLK = 512;
 constexpr int TSTAGE = 10;
+constexpr int TILE_FWD = 1, TILE_BWD = 2;
+constexpr int TILE_SPIN_MAX = 1 << 17;
 static_assert(ILU_TILE_RMAX <= TBLK, "a tile's rows take one thread each");
 template <class T>
-__device__ inline void tile_stage(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                                  const T* __restrict__ val, const T* __restrict__ z, int32_t r0, int32_t r1,
-                                  int32_t base, int32_t ne, bool lower, T* sv, int16_t* sc, int16_t* lp) {
-    const int nr = r1 - r0;
-    for (int i = threadIdx.x; i < nr; i += TBLK) lp[i] = (int16_t)(ptr[r0 + i] - base);
-    if (threadIdx.x == 0) lp[nr] = (int16_t)ne;
-    for (int e0 = 0; e0 < ne; e0 += TBLK * TSTAGE) {
-        int32_t j[TSTAGE];
-        T v[TSTAGE];
+struct TileChain {
+    const int32_t* role;  // nrole records: r0, r1, L begin, L end, U begin, U end, tile, kind, deps begin, deps end
+    const int32_t* lev;   // per tile the first row of each of the ILU_MAXLEV levels and the tile's end
+    const int32_t* dep;   // (tile << 1) | 1 when only that tile's forward sweep is needed
+    const int32_t *lptr, *lcol, *uptr, *ucol;
+    const T *lval, *uval, *dval, *rhs;
+    const double* rscale;
+    T* z;
+    const int* ctrl;
+    unsigned long long* sync;  // [0] tickets taken, [1] lapsed waits, [2 + k] completion of tile k
+    int32_t nrole, nlev, emax, rmax;
+};
+// the entries [base, base + ne) of a triangle, TSTAGE per thread (ne <= TBLK * TSTAGE)
+template <class T>
+__device__ inline void tile_load(const int32_t* __restrict__ col, const T* __restrict__ val, int32_t base, int32_t ne,
+                                 int32_t r0, int32_t (&j)[TSTAGE], T (&v)[TSTAGE]) {
 #pragma unroll
-        for (int u = 0; u < TSTAGE; ++u) {
-            const int e = e0 + u * TBLK + threadIdx.x;
-            const bool in = e < ne;
-            j[u] = in ? col[base + e] : r0;
-            v[u] = in ? val[base + e] : (T)0;
-        }
-        T zx[TSTAGE];
+    for (int u = 0; u < TSTAGE; ++u) {
+        const int e = u * TBLK + threadIdx.x;
+        const bool in = e < ne;
+        j[u] = in ? col[base + e] : r0;
+        v[u] = in ? val[base + e] : (T)0;
+    }
+}
+// z of another workgroup's tile, read at device scope (past this XCD's L2)
+template <class T>
+__device__ inline T z_load(T* z) { return __hip_atomic_load(z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// the loaded entries into LDS: an entry outside the tile as its product with z (sc = -1)
+template <class T>
+__device__ inline void tile_place(T* z, int32_t r0, int32_t r1, int32_t ne, bool lower, const int32_t (&j)[TSTAGE],
+                                  const T (&v)[TSTAGE], T* sv, int16_t* sc) {
+    T zx[TSTAGE];
 #pragma unroll
-        for (int u = 0; u < TSTAGE; ++u) {
+    for (int u = 0; u < TSTAGE; ++u) {
+        const bool outside = lower ? j[u] < r0 : j[u] >= r1;
+        zx[u] = z_load<T>(z + (outside ? j[u] : r0));
+    }
+#pragma unroll
+    for (int u = 0; u < TSTAGE; ++u) {
+        const int e = u * TBLK + threadIdx.x;
+        if (e < ne) {
             const bool outside = lower ? j[u] < r0 : j[u] >= r1;
-            zx[u] = z[outside ? j[u] : r0];
-        }
-#pragma unroll
-        for (int u = 0; u < TSTAGE; ++u) {
-            const int e = e0 + u * TBLK + threadIdx.x;
-            if (e < ne) {
-                const bool outside = lower ? j[u] < r0 : j[u] >= r1;
-                sv[e] = outside ? v[u] * zx[u] : v[u];
-                sc[e] = outside ? (int16_t)-1 : (int16_t)(j[u] - r0);
-            }
+            sv[e] = outside ? v[u] * zx[u] : v[u];
+            sc[e] = outside ? (int16_t)-1 : (int16_t)(j[u] - r0);
         }
     }
 }
@@ -996,159 +1026,132 @@ __device__ inline T tile_row_inside(const T* sv, const int16_t* sc, const T* zl,
     if (q < q1) s += sv[q] * zl[sc[q]];
     return s;
 }
-// forward sweep of the tiles [k0, k0 + gridDim.x): z_t = c rhs_t - sum_L l_tj z_j
-template <class T>
-__global__ void __launch_bounds__(TBLK) k_ilu_tile_lower(int32_t k0, const int32_t* __restrict__ tinfo,
-                                                         const int32_t* __restrict__ lev, const int32_t* __restrict__ lptr,
-                                                         const int32_t* __restrict__ lcol, const T* __restrict__ lval,
-                                                         const T* __restrict__ rhs, const double* __restrict__ rscale,
-                                                         T* __restrict__ z, const int* __restrict__ ctrl, int32_t nlev,
-                                                         int32_t emax, int32_t rmax) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ int32_t lv[ILU_MAXLEV + 1];
-    const int64_t k = k0 + (int64_t)xcd_block();
-    const int go = ctrl_go(ctrl);
-    const int32_t r0 = tinfo[8 * k], r1 = tinfo[8 * k + 1], base = tinfo[8 * k + 2], end = tinfo[8 * k + 3];
-    if (ctrl_stop(go, r0 <= r1)) return;
-    T* sv = reinterpret_cast<T*>(lds);
-    T* zl = sv + emax;
-    int16_t* sc = reinterpret_cast<int16_t*>(zl + rmax);
-    int16_t* lp = sc + emax;
-    const int nr = r1 - r0;
-    const T c = (T)scale_of(rscale);
-    if (threadIdx.x <= ILU_MAXLEV) lv[threadIdx.x] = lev[k * (ILU_MAXLEV + 1) + threadIdx.x] - r0;
-    T rv = 0;
-    if ((int)threadIdx.x < nr) rv = (T)(rhs[r0 + threadIdx.x] * c);  // one row per thread (nr <= TBLK)
-    tile_stage<T>(lptr, lcol, lval, z, r0, r1, base, end - base, true, sv, sc, lp);
-    __syncthreads();
-    int range = 0;
-    if ((int)threadIdx.x < nr) {
-        T acc;
-        range = tile_row_outside<T>(sv, sc, lp, threadIdx.x, true, acc);
-        zl[threadIdx.x] = rv - acc;
-    }
-    __syncthreads();
-    for (int l = 0; l < nlev; ++l) {
-        const int i = threadIdx.x;
-        if (i >= lv[l] && i < lv[l + 1]) zl[i] -= tile_row_inside<T>(sv, sc, zl, range);
-        __syncthreads();
-    }
-    if ((int)threadIdx.x < nr) z[r0 + threadIdx.x] = zl[threadIdx.x];
-}
-// backward sweep of the tiles [k0, k0 + gridDim.x), in place: z_t = (z_t - sum_U u_tj z_j) / u_tt
-template <class T>
-__global__ void __launch_bounds__(TBLK) k_ilu_tile_upper(int32_t k0, const int32_t* __restrict__ tinfo,
-                                                         const int32_t* __restrict__ lev, const int32_t* __restrict__ uptr,
-                                                         const int32_t* __restrict__ ucol, const T* __restrict__ uval,
-                                                         const T* __restrict__ dval, T* __restrict__ z,
-                                                         const int* __restrict__ ctrl, int32_t nlev, int32_t emax, int32_t rmax) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ int32_t lv[ILU_MAXLEV + 1];
-    const int64_t k = k0 + (int64_t)xcd_block();
-    const int go = ctrl_go(ctrl);
-    const int32_t r0 = tinfo[8 * k], r1 = tinfo[8 * k + 1], base = tinfo[8 * k + 4], end = tinfo[8 * k + 5];
-    if (ctrl_stop(go, r0 <= r1)) return;
-    T* sv = reinterpret_cast<T*>(lds);
-    T* zl = sv + emax;
-    int16_t* sc = reinterpret_cast<int16_t*>(zl + rmax);
-    int16_t* lp = sc + emax;
-    const int nr = r1 - r0;
-    if (threadIdx.x <= ILU_MAXLEV) lv[threadIdx.x] = lev[k * (ILU_MAXLEV + 1) + threadIdx.x] - r0;
-    T zv = 0, dv = 1;
-    if ((int)threadIdx.x < nr) {  // one row per thread (nr <= TBLK)
-        zv = z[r0 + threadIdx.x];
-        dv = dval[r0 + threadIdx.x];
-    }
-    tile_stage<T>(uptr, ucol, uval, z, r0, r1, base, end - base, false, sv, sc, lp);
-    __syncthreads();
-    int range = 0;
-    if ((int)threadIdx.x < nr) {
-        T acc;
-        range = tile_row_outside<T>(sv, sc, lp, threadIdx.x, false, acc);
-        zl[threadIdx.x] = zv - acc;
-    }
-    __syncthreads();
-    for (int l = nlev - 1; l >= 0; --l) {
-        const int i = threadIdx.x;
-        if (i >= lv[l] && i < lv[l + 1]) zl[i] = (zl[i] - tile_row_inside<T>(sv, sc, zl, range)) / dv;
-        __syncthreads();
-    }
-    if ((int)threadIdx.x < nr) z[r0 + threadIdx.x] = zl[threadIdx.x];
-}
-// Both sweeps of the last tile colour in one launch: its U entries all lie inside the tile (no
-// later tile colour), so after the forward levels the same LDS holds the U entries, loaded into
-// registers while the forward sweep runs, and the levels are walked back.  LDS as the forward
-// sweep with emax = max(L, U entries of a tile).
-template <class T>
-__global__ void __launch_bounds__(TBLK) k_ilu_tile_last(int32_t k0, const int32_t* __restrict__ tinfo,
-                                                        const int32_t* __restrict__ lev, const int32_t* __restrict__ lptr,
-                                                        const int32_t* __restrict__ lcol, const T* __restrict__ lval,
-                                                        const int32_t* __restrict__ uptr, const int32_t* __restrict__ ucol,
-                                                        const T* __restrict__ uval, const T* __restrict__ dval,
-                                                        const T* __restrict__ rhs, const double* __restrict__ rscale,
-                                                        T* __restrict__ z, const int* __restrict__ ctrl, int32_t nlev,
-                                                        int32_t emax, int32_t rmax) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ int32_t lv[ILU_MAXLEV + 1];
-    const int64_t k = k0 + (int64_t)xcd_block();
-    const int go = ctrl_go(ctrl);
-    const int32_t r0 = tinfo[8 * k], r1 = tinfo[8 * k + 1], base = tinfo[8 * k + 2], end = tinfo[8 * k + 3];
-    const int32_t ubase = tinfo[8 * k + 4], uend = tinfo[8 * k + 5];
-    if (ctrl_stop(go, r0 <= r1)) return;
-    T* sv = reinterpret_cast<T*>(lds);
-    T* zl = sv + emax;
-    int16_t* sc = reinterpret_cast<int16_t*>(zl + rmax);
-    int16_t* lp = sc + emax;
-    const int nr = r1 - r0;
-    const T c = (T)scale_of(rscale);
-    if (threadIdx.x <= ILU_MAXLEV) lv[threadIdx.x] = lev[k * (ILU_MAXLEV + 1) + threadIdx.x] - r0;
-    T rv = 0, dv = 1;
-    int urange = 0;
-    if ((int)threadIdx.x < nr) {  // one row per thread (nr <= TBLK)
-        rv = (T)(rhs[r0 + threadIdx.x] * c);
-        dv = dval[r0 + threadIdx.x];
-        urange = ((uptr[r0 + threadIdx.x] - ubase) << 16) | (uptr[r0 + threadIdx.x + 1] - ubase);
-    }
-    // the tile's U entries, held in registers through the forward sweep
-    const int32_t nu = uend - ubase;
-    int32_t uj[TSTAGE];
-    T uv[TSTAGE];
-#pragma unroll
-    for (int u = 0; u < TSTAGE; ++u) {
-        const int e = u * TBLK + threadIdx.x;
-        uj[u] = e < nu ? ucol[ubase + e] : r0;
-        uv[u] = e < nu ? uval[ubase + e] : (T)0;
-    }
-    tile_stage<T>(lptr, lcol, lval, z, r0, r1, base, end - base, true, sv, sc, lp);
-    __syncthreads();
-    int range = 0;
-    if ((int)threadIdx.x < nr) {
-        T acc;
-        range = tile_row_outside<T>(sv, sc, lp, threadIdx.x, true, acc);
-        zl[threadIdx.x] = rv - acc;
-    }
-    __syncthreads();
-    for (int l = 0; l < nlev; ++l) {
-        const int i = threadIdx.x;
-        if (i >= lv[l] && i < lv[l + 1]) zl[i] -= tile_row_inside<T>(sv, sc, zl, range);
-        __syncthreads();
-    }
-    // backward: the U entries (all inside the tile) replace the L entries in LDS
-#pragma unroll
-    for (int u = 0; u < TSTAGE; ++u) {
-        const int e = u * TBLK + threadIdx.x;
-        if (e < nu) {
-            sv[e] = uv[u];
-            sc[e] = (int16_t)(uj[u] - r0);
+// wait for the completion flags of deps [d0, d1) (one thread each, d1 - d0 <= TBLK); true when
+// a wait lapsed
+__device__ inline bool tile_wait(const int32_t* __restrict__ dep, int32_t d0, int32_t d1, unsigned long long* sync,
+                                 unsigned long long app, int* fail) {
+    const int32_t i = d0 + (int32_t)threadIdx.x;
+    if (i < d1) {
+        const int32_t d = dep[i];
+        const unsigned long long want = 2 * app + 2 - (unsigned long long)(d & 1);
+        unsigned long long* f = sync + 2 + (d >> 1);
+        for (int spins = 0; __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want; ++spins) {
+            if (spins >= TILE_SPIN_MAX) {
+                *fail = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
         }
     }
     __syncthreads();
-    for (int l = nlev - 1; l >= 0; --l) {
-        const int i = threadIdx.x;
-        if (i >= lv[l] && i < lv[l + 1]) zl[i] = (zl[i] - tile_row_inside<T>(sv, sc, zl, urange)) / dv;
-        __syncthreads();
+    return *fail != 0;
+}
+template <class T>
+__global__ void __launch_bounds__(TBLK) k_ilu_tile_chain(TileChain<T> a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int32_t lv[ILU_MAXLEV + 1];
+    __shared__ unsigned long long s_ticket;
+    __shared__ int s_fail;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_ticket = __hip_atomic_fetch_add(a.sync, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_fail = 0;
     }
-    if ((int)threadIdx.x < nr) z[r0 + threadIdx.x] = zl[threadIdx.x];
+    const int go = ctrl_go(a.ctrl);
+    __syncthreads();
+    if (go == 0) return;  // after convergence: every workgroup of the launch still took its ticket
+    const unsigned long long t = s_ticket;
+    const unsigned long long app = t / (unsigned long long)a.nrole;
+    const int32_t i = __builtin_amdgcn_readfirstlane((int32_t)(t - app * (unsigned long long)a.nrole));
+    const int32_t* R = a.role + (int64_t)TILE_ROLE_INTS * i;
+    const int32_t r0 = R[0], r1 = R[1], lb = R[2], le = R[3], ub = R[4], ue = R[5], k = R[6], kind = R[7], d0 = R[8],
+                  d1 = R[9];
+    T* sv = reinterpret_cast<T*>(lds);
+    T* zl = sv + a.emax;
+    int16_t* sc = reinterpret_cast<int16_t*>(zl + a.rmax);
+    int16_t* lp = sc + a.emax;
+    const int nr = r1 - r0;
+    if (tid <= ILU_MAXLEV) lv[tid] = a.lev[(int64_t)k * (ILU_MAXLEV + 1) + tid] - r0;
+    int32_t j[TSTAGE];
+    T v[TSTAGE];
+    bool fail = false;
+    if (kind & TILE_FWD) {  // z_t = c rhs_t - sum_L l_tj z_j
+        const T c = (T)scale_of(a.rscale);
+        T rv = 0;
+        if (tid < nr) {
+            rv = (T)(a.rhs[r0 + tid] * c);
+            lp[tid] = (int16_t)(a.lptr[r0 + tid] - lb);
+        }
+        if (tid == 0) lp[nr] = (int16_t)(le - lb);
+        tile_load<T>(a.lcol, a.lval, lb, le - lb, r0, j, v);
+        fail = tile_wait(a.dep, d0, d1, a.sync, app, &s_fail);
+        tile_place<T>(a.z, r0, r1, le - lb, true, j, v, sv, sc);
+        __syncthreads();
+        int range = 0;
+        if (tid < nr) {
+            T acc;
+            range = tile_row_outside<T>(sv, sc, lp, tid, true, acc);
+            zl[tid] = rv - acc;
+        }
+        __syncthreads();
+        // both sweeps: the U entries load while the levels are walked
+        if (kind & TILE_BWD) tile_load<T>(a.ucol, a.uval, ub, ue - ub, r0, j, v);
+        for (int l = 0; l < a.nlev; ++l) {
+            if (tid >= lv[l] && tid < lv[l + 1]) zl[tid] -= tile_row_inside<T>(sv, sc, zl, range);
+            __syncthreads();
+        }
+    }
+    if (kind & TILE_BWD) {  // z_t = (z_t - sum_U u_tj z_j) / u_tt
+        T dv = 1;
+        int range = 0;
+        if (kind & TILE_FWD) {
+            // every U entry inside the tile: they replace the L entries in LDS
+            if (tid < nr) {
+                dv = a.dval[r0 + tid];
+                range = ((a.uptr[r0 + tid] - ub) << 16) | (a.uptr[r0 + tid + 1] - ub);
+            }
+#pragma unroll
+            for (int u = 0; u < TSTAGE; ++u) {
+                const int e = u * TBLK + tid;
+                if (e < ue - ub) {
+                    sv[e] = v[u];
+                    sc[e] = (int16_t)(j[u] - r0);
+                }
+            }
+            __syncthreads();
+        } else {
+            T zv = 0;
+            if (tid < nr) {
+                dv = a.dval[r0 + tid];
+                lp[tid] = (int16_t)(a.uptr[r0 + tid] - ub);
+            }
+            if (tid == 0) lp[nr] = (int16_t)(ue - ub);
+            tile_load<T>(a.ucol, a.uval, ub, ue - ub, r0, j, v);
+            fail = tile_wait(a.dep, d0, d1, a.sync, app, &s_fail);
+            if (tid < nr) zv = z_load<T>(a.z + r0 + tid);
+            tile_place<T>(a.z, r0, r1, ue - ub, false, j, v, sv, sc);
+            __syncthreads();
+            if (tid < nr) {
+                T acc;
+                range = tile_row_outside<T>(sv, sc, lp, tid, false, acc);
+                zl[tid] = zv - acc;
+            }
+            __syncthreads();
+        }
+        for (int l = a.nlev - 1; l >= 0; --l) {
+            if (tid >= lv[l] && tid < lv[l + 1]) zl[tid] = (zl[tid] - tile_row_inside<T>(sv, sc, zl, range)) / dv;
+            __syncthreads();
+        }
+    }
+    if (tid < nr)
+        __hip_atomic_store(a.z + r0 + tid, fail ? (T)__builtin_nan("") : zl[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (fail && tid == 0) __hip_atomic_fetch_add(a.sync + 1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // publish: every wave's z stores complete before the barrier, then the flag
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0)
+        __hip_atomic_store(a.sync + 2 + k, 2 * app + ((kind & TILE_BWD) ? 2ull : 1ull), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // IKJ ILU(0) rows of one (tile colour, level) group: the same update as k_ilu_factor
@@ -1533,12 +1536,9 @@ void mdot_launch(Ctx& c, Work& w, int nv, const T* V, const T* wv, double* part,
 }
 
 // LDS of a tile sweep workgroup (value size vb): staged entries and their 16-bit local columns,
-// the tile's z (and the diagonal for the backward sweep), 16-bit row offsets
+// the tile's z, 16-bit row offsets
 constexpr int64_t ILU_TILE_LDS = 64 * 1024 - 256;
-inline int64_t tile_lds_bytes(int vb, int64_t emax, int64_t rmax, bool upper) {
-    (void)upper;
-    return vb * (emax + rmax) + 2 * (emax + rmax + 1);
-}
+inline int64_t tile_lds_bytes(int vb, int64_t emax, int64_t rmax) { return vb * (emax + rmax) + 2 * (emax + rmax + 1); }
 
 // Row order of the matrix pattern (host, ilu_order.cpp: tiled, or multicolour), the permuted
 // CSR and the split triangle index arrays, uploaded once per pattern.
@@ -1549,37 +1549,35 @@ int ilu_setup(Ctx& c, const SysView& sy) {
     std::string err;
     const double* pos = c.g.h_pos.empty() ? nullptr : c.g.h_pos.data();
     if (ilu_order_build(sy.n, *sy.h_row, *sy.h_col, pos, c.g.N, o, err)) return set_error(VFVM_ERR_INVALID_ARGUMENT, err);
-    // the tiled order only when the tiles' FP64 entries fit the LDS of one workgroup (16-bit
-    // offsets inside a tile); else the multicolour order
-    if (!o.tile_row.empty() && (tile_lds_bytes(8, o.emax_l, o.rmax, false) > ILU_TILE_LDS ||
-                                tile_lds_bytes(8, o.emax_u, o.rmax, true) > ILU_TILE_LDS || o.emax_l > 32767 || o.emax_u > 32767 ||
-                                o.emax_u > TBLK * TSTAGE))
+    // the tiled order only when a tile's FP64 entries of both triangles fit the LDS of one
+    // workgroup (16-bit offsets inside a tile) and one load round of its threads, and no tile
+    // sweep waits for more tiles than it has threads; else the multicolour order
+    std::vector<int32_t> role, dep;
+    const int64_t emax = std::max(o.emax_l, o.emax_u);
+    if (!o.tile_row.empty() && (tile_lds_bytes(8, emax, o.rmax) > ILU_TILE_LDS || emax > TBLK * TSTAGE ||
+                                !tile_roles_build(o, TBLK, role, dep)))
         if (ilu_order_build(sy.n, *sy.h_row, *sy.h_col, pos, c.g.N, o, err, false)) return set_error(VFVM_ERR_INVALID_ARGUMENT, err);
     I.tiled = !o.tile_row.empty();
     if (I.tiled) {
         const int64_t nt = (int64_t)o.tile_row.size() - 1;
-        std::vector<int32_t> ti(8 * nt, 0);
-        for (int64_t k = 0; k < nt; ++k) {
-            const int32_t r0 = o.tile_row[k], r1 = o.tile_row[k + 1];
-            ti[8 * k] = r0; ti[8 * k + 1] = r1;
-            ti[8 * k + 2] = o.lptr[r0]; ti[8 * k + 3] = o.lptr[r1];
-            ti[8 * k + 4] = o.uptr[r0]; ti[8 * k + 5] = o.uptr[r1];
-        }
-        VFVM_HIP_TRY(I.tinfo.upload(ti.data(), ti.size(), c.stream));
         VFVM_HIP_TRY(I.lev_row.upload(o.lev_row.data(), o.lev_row.size(), c.stream));
         VFVM_HIP_TRY(I.fac_rows.upload(o.fac_rows.data(), o.fac_rows.size(), c.stream));
-        I.h_tc_ptr = o.tc_ptr;
+        VFVM_HIP_TRY(I.trole.upload(role.data(), role.size(), c.stream));
+        VFVM_HIP_TRY(I.tdep.upload(dep.data(), std::max<size_t>(dep.size(), 1), c.stream));
+        // the ticket counter starts at 0 and every flag below the first application's values
+        const std::vector<unsigned long long> zero(2 + nt, 0ull);
+        VFVM_HIP_TRY(I.tsync.upload(zero.data(), zero.size(), c.stream));
+        I.nrole = (int32_t)(role.size() / TILE_ROLE_INTS);
         I.h_fac_ptr = o.fac_ptr;
         I.h_tile_row = o.tile_row;
         I.h_lev_row = o.lev_row;
-        const int ntc = (int)o.tc_ptr.size() - 1;
-        I.h_tc_rows.assign(ntc, 0.0); I.h_tc_lent.assign(ntc, 0.0); I.h_tc_uent.assign(ntc, 0.0);
-        for (int q = 0; q < ntc; ++q)
-            for (int32_t k = o.tc_ptr[q]; k < o.tc_ptr[q + 1]; ++k) {
-                I.h_tc_rows[q] += ti[8 * k + 1] - ti[8 * k];
-                I.h_tc_lent[q] += ti[8 * k + 3] - ti[8 * k + 2];
-                I.h_tc_uent[q] += ti[8 * k + 5] - ti[8 * k + 4];
-            }
+        I.sweep_rows = I.sweep_lent = I.sweep_uent = I.fused_rows = 0.0;
+        for (int32_t i = 0; i < I.nrole; ++i) {
+            const int32_t* R = role.data() + (size_t)TILE_ROLE_INTS * i;
+            if (R[7] & 1) { I.sweep_rows += R[1] - R[0]; I.sweep_lent += R[3] - R[2]; }
+            if (R[7] & 2) I.sweep_uent += R[5] - R[4];
+            if (R[7] == 3) I.fused_rows += R[1] - R[0];
+        }
         I.nlev = o.nlev; I.rmax = o.rmax; I.emax_l = o.emax_l; I.emax_u = o.emax_u;
     }
     VFVM_HIP_TRY(I.perm.upload(o.perm.data(), o.perm.size(), c.stream));
@@ -1632,10 +1630,11 @@ int ilu_factor(Ctx& c, const SysView& sy, T* lu, T* full, const double* val) {
 // Lanes per row of the sweeps: four (a row holds ~8 entries of each triangle)
 constexpr int ILU_LPR = 4;
 
-// z = (LU)^-1 rhs in the colour-major space: forward sweep over the colours ascending (the
-// last one also finishing its backward sweep; colours 0 and 1 share a launch), backward
-// sweep over the other colours descending, in place in z (z must not alias rhs):
-// 2 colours - 2 launches.  lu is the split factor [L | U | diag].
+// z = (LU)^-1 rhs in the permuted space, in place in z (z must not alias rhs); lu is the split
+// factor [L | U | diag].  Tiled order: one launch (k_ilu_tile_chain).  Multicolour order: forward
+// sweep over the colours ascending (the last one also finishing its backward sweep; colours 0
+// and 1 share a launch), backward sweep over the other colours descending: 2 colours - 2
+// launches.
 template <class T>
 int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double* rscale, T* z, const int* ctrl) {
     constexpr int LPR = ILU_LPR;
@@ -1655,35 +1654,16 @@ int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double
         return (double)(e - s) * (per_row * (sizeof(T) + 4.0) + 8.0 + 2.0 * sizeof(T) + ((final || !lower) ? sizeof(T) : 0.0));
     };
     if (I.tiled) {
-        // one launch per tile colour and sweep (ascending forward, descending backward)
-        const int ntc = (int)I.h_tc_ptr.size() - 1;
-        const size_t lds_l = (size_t)tile_lds_bytes(sizeof(T), I.emax_l, I.rmax, false);
-        const size_t lds_u = (size_t)tile_lds_bytes(sizeof(T), I.emax_u, I.rmax, true);
-        // FP32: the last tile colour's two sweeps share a launch (FP64: the fused kernel's U entries
-        // held in registers cost more than the launch it saves, 35 against 14 + 16 us at 100^3)
-        const bool fuse = sizeof(T) == 4;
-        for (int q = 0; q + (fuse ? 1 : 0) < ntc; ++q) {
-            const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
-            const double bytes = I.h_tc_lent[q] * (sizeof(T) + 4.0) + I.h_tc_rows[q] * (4.0 + 2.0 * sizeof(T));
-            launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_lower<T>, dim3(nt), dim3(TBLK), lds_l, k0, I.tinfo.p,
-                             I.lev_row.p, I.lptr.p, I.lcol.p, lval, rhs, rscale, z, ctrl, I.nlev, I.emax_l, I.rmax);
-        }
-        if (fuse) {
-            // the last tile colour: both sweeps in one launch
-            const int q = ntc - 1;
-            const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
-            const int32_t emax = std::max(I.emax_l, I.emax_u);
-            const size_t lds = (size_t)tile_lds_bytes(sizeof(T), emax, I.rmax, false);
-            const double bytes = (I.h_tc_lent[q] + I.h_tc_uent[q]) * (sizeof(T) + 4.0) + I.h_tc_rows[q] * (8.0 + 3.0 * sizeof(T));
-            launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_last<T>, dim3(nt), dim3(TBLK), lds, k0, I.tinfo.p, I.lev_row.p,
-                             I.lptr.p, I.lcol.p, lval, I.uptr.p, I.ucol.p, uval, dval, rhs, rscale, z, ctrl, I.nlev, emax, I.rmax);
-        }
-        for (int q = ntc - (fuse ? 2 : 1); q >= 0; --q) {
-            const int32_t k0 = I.h_tc_ptr[q], nt = I.h_tc_ptr[q + 1] - k0;
-            const double bytes = I.h_tc_uent[q] * (sizeof(T) + 4.0) + I.h_tc_rows[q] * (4.0 + 3.0 * sizeof(T));
-            launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_upper<T>, dim3(nt), dim3(TBLK), lds_u, k0, I.tinfo.p,
-                             I.lev_row.p, I.uptr.p, I.ucol.p, uval, dval, z, ctrl, I.nlev, I.emax_u, I.rmax);
-        }
+        // every tile sweep of the application in one launch; algorithmic bytes: the triangles'
+        // entries (value + column), per row its L and U row pointers, rhs, diagonal and z written
+        // twice and read once between the sweeps (once, for the tiles doing both sweeps)
+        const double vb = (double)sizeof(T);
+        const double bytes = (I.sweep_lent + I.sweep_uent) * (vb + 4.0) + I.sweep_rows * (8.0 + 5.0 * vb) -
+                             I.fused_rows * 2.0 * vb;
+        TileChain<T> a{I.trole.p, I.lev_row.p, I.tdep.p, I.lptr.p, I.lcol.p, I.uptr.p, I.ucol.p, lval, uval, dval, rhs,
+                       rscale, z, ctrl, I.tsync.p, I.nrole, I.nlev, std::max(I.emax_l, I.emax_u), I.rmax};
+        const size_t lds = (size_t)tile_lds_bytes(sizeof(T), std::max(I.emax_l, I.emax_u), I.rmax);
+        launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_chain<T>, dim3(I.nrole), dim3(TBLK), lds, a);
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }

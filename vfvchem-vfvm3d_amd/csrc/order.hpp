@@ -51,6 +51,18 @@ constexpr int ILU_TILE_W = 2;
 int ilu_order_build(int64_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& cl, const double* pos,
                     int64_t nvert, IluOrder& out, std::string& err, bool tiled = true);
 
+// Roles of the one-launch sweeps of a tiled order: role records of TILE_ROLE_INTS ints {r0, r1,
+// L entries begin, end, U entries begin, end, tile, kind (1 forward, 2 backward, 3 both),
+// dependencies begin, end, 0 ...} and the dependency lists ((tile << 1) | 1 when only that
+// tile's forward sweep is needed, tile << 1 when its backward sweep is).  Roles: the forward
+// sweeps by tile colour ascending (the last tile colour's tiles do both sweeps), then the
+// backward sweeps by tile colour descending.  A forward sweep depends on the adjacent tiles
+// (an entry of A or A^T between them) of lower colours; a backward sweep on the adjacent tiles
+// of higher colours and on its own forward sweep; every dependency is an earlier role.
+// Returns false when a role has more than max_deps dependencies.
+constexpr int TILE_ROLE_INTS = 16;
+bool tile_roles_build(const IluOrder& o, int max_deps, std::vector<int32_t>& role, std::vector<int32_t>& dep);
+
 // 63-bit Morton code of a point of the box [lo, hi] (21 bits per axis).
 uint64_t morton3(const double* p, const double* lo, const double* hi);
 
