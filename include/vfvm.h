@@ -419,6 +419,13 @@ int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, int32_t* nc
  * = level of position t inside its tile (n entries); rows of one tile and one level are not
  * coupled.  For the parity tests; no reference counterpart. */
 int vfvm_ilu_tiles(vfvm_ctx* ctx, int32_t* tile_ptr, int32_t* ntiles, int32_t* level);
+/* The one-launch ILU(0) sweeps of the tiled order wait for the completion flags of adjacent
+ * tiles with at most `polls` polls each (2^17 by default; polls < 0: every wait for another tile
+ * gives up at once).  A sweep whose wait gives up writes NaN, so the solve stops with
+ * VFVM_ERR_LINEAR_NON_FINITE instead of hanging; negative limits exist to test that path.
+ * *lapsed (may be null) receives the number of waits that gave up so far.  No reference
+ * counterpart. */
+int vfvm_ilu_wait_limit(vfvm_ctx* ctx, int32_t polls, uint64_t* lapsed);
 
 /* ---- caller-assembled systems (LinearBase, src/linear/linear_base.rs:5-10) ----
  * For a solver whose operators are not on the device: it hands its triplet list, right-hand

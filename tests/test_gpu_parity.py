@@ -255,6 +255,31 @@ def test_ilu0_one_launch_sweeps_at_size(ctx, prob, n):
         assert np.array_equal(z, z32[0])
 
 
+def test_ilu0_sweep_wait_that_gives_up_stops_the_solve(ctx):
+    """A one-launch sweep whose wait for an adjacent tile gives up writes NaN (forced from the
+    host: every such wait gives up at once), so M^-1 r is not finite and the solve reports
+    LinearGmres non-finite instead of hanging; with the default limit restored the same system
+    solves again, bit for bit as before."""
+    print("[ilu0 wait limit]", flush=True)
+    p = mass_c2(vfvm.Mesh.lattice(24), c_out=0.0, prod=0.3)
+    rp, col, vals, b = _assembled(ctx, p)
+    n = len(rp) - 1
+    kw = dict(tol=1e-10, restart=50, max_outer=400, precond=vfvm.PRECOND_ILU0, precision=vfvm.FP64)
+    x0, _ = ctx.solve(None, **kw)
+    tptr, _ = ctx.ilu_tiles()
+    assert len(tptr) - 1 > 8
+    before = ctx.ilu_wait_limit(-1)
+    z = ctx.precond_apply(np.ones(n), precond=vfvm.PRECOND_ILU0, precision=vfvm.FP64)
+    assert np.isnan(z).any()
+    with pytest.raises(vfvm.VFVChemError) as e:
+        ctx.solve(None, **kw)
+    assert e.value.code == vfvm._lib.ERR_LINEAR_NON_FINITE
+    lapsed = ctx.ilu_wait_limit(1 << 17)
+    assert lapsed > before
+    x1, _ = ctx.solve(None, **kw)
+    assert np.array_equal(x1, x0)
+
+
 @pytest.mark.parametrize("prec,pc", [(vfvm.FP64, vfvm.PRECOND_ILU0), (vfvm.FP32_REFINED, vfvm.PRECOND_ILU0),
                                      (vfvm.FP64, vfvm.PRECOND_JACOBI)])
 def test_cycle_graphs_match_eager_launches(ctx, prec, pc):

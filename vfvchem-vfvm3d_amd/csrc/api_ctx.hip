@@ -954,6 +954,12 @@ extern "C" int vfvm_ilu_order(vfvm_ctx* ctx, int32_t* perm, int32_t* color_ptr, 
     return clear_error();
 }
 
+extern "C" int vfvm_ilu_wait_limit(vfvm_ctx* ctx, int32_t polls, uint64_t* lapsed) {
+    CTX_GUARD(ctx);
+    if (int st = ilu_wait_limit(c, polls, lapsed)) return st;
+    return clear_error();
+}
+
 extern "C" int vfvm_ilu_tiles(vfvm_ctx* ctx, int32_t* tile_ptr, int32_t* ntiles, int32_t* level) {
     CTX_GUARD(ctx);
     if (!c.has_pattern) return set_error(VFVM_ERR_INVALID_ARGUMENT, "vfvm_ilu_tiles: no pattern built");

@@ -166,6 +166,9 @@ struct IluData {
     int32_t nlev = 0, rmax = 0, emax_l = 0, emax_u = 0;
     bool ready = false;
     uint64_t generation = 0;  // bumped by every set-up (graphs bake the layout in)
+    // polls of a completion flag before a tile sweep's wait gives up (< 0: every wait gives up
+    // at once, for tests of that path)
+    int32_t wait_limit = 1 << 17;
 };
 
 // One linear system as the solver reads it (non-owning).
@@ -399,6 +402,7 @@ int solver_setup(Ctx& c, const SysView& s, const vfvm_solve_opts& o);
 int precond_apply(Ctx& c, const vfvm_solve_opts& o, const double* r_host, double* z_host);
 int ilu_order(Ctx& c, int32_t* perm, int32_t* color_ptr, int32_t* ncolors);
 int ilu_tiles(Ctx& c, int32_t* tile_ptr, int32_t* ntiles, int32_t* level);
+int ilu_wait_limit(Ctx& c, int32_t polls, uint64_t* lapsed);
 int residual_norm(Ctx& c, const SysView& s, const double* x_dev, double* rel, double* bnorm);
 
 // flow operators (flow.hip)

@@ -241,6 +241,15 @@ __device__ inline int find_slot(const int32_t* list, int len, int32_t m) {
 
 __device__ inline double comp3(int c, double x, double y, double z) { return c == 0 ? x : (c == 1 ? y : z); }
 
+// Ordering point of a one-wave workgroup: a wave's LDS accesses are carried out in the order it
+// issues them, so its lanes see each other's LDS writes without a barrier; this only keeps the
+// compiler from moving memory accesses across the point (global loads stay in flight)
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Continuity rows, one wave per vertex v: the divergence operator's velocity columns and the
 // pressure stabiliser's pressure columns and b, face by face in push order, then the
 // boundary sub-face terms of the divergence operator for nodes (opscl_dom_div.rs:86-150,
@@ -288,14 +297,14 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
         pc[i] = a.mcol[rs + 3 * glen + i] - (int32_t)(3 * N);
         ps[i] = 0.0;
     }
-    __syncthreads();
+    wave_sync();
     const int32_t es = g.rp[v], ee = g.rp[v + 1];
     const int deg_v = ee - es;
     for (int i = lane; i < glen; i += WAVE) gsl[i] = find_slot(pc, plen, gcv[i]);
     if (lane < deg_v) nsl[lane] = find_slot(gcv, glen, g.ngh[es + lane]);
     const int sv = find_slot(gcv, glen, (int32_t)v);
     const int psv = find_slot(pc, plen, (int32_t)v);
-    __syncthreads();
+    wave_sync();
 
     const double den_v = f.den[v];
     const double rc_v = f.rc[v];
@@ -373,7 +382,7 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
             us[lane * gm + sn] += t;
             us[lane * gm + sv] += t;
         }
-        __syncthreads();
+        wave_sync();
         // vertex gradient contribution: (vel_m - vel_v) terms over ngh(v)
         if (lane < deg_v) {
             const double w_skew = gwx * sx + gwy * sy + gwz * sz;
@@ -384,14 +393,14 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
                 tq[c * WAVE + lane] = val;
             }
         }
-        __syncthreads();
+        wave_sync();
         if (lane < 3) {
             // the chain into the v slot, summed in a register in push order
             double acc = us[lane * gm + sv];
             for (int q = 0; q < deg_v; ++q) acc += -tq[lane * WAVE + q];
             us[lane * gm + sv] = acc;
         }
-        __syncthreads();
+        wave_sync();
         // neighbour gradient contribution: (vel_m - vel_n) terms over ngh(n)
         if (lane < deg_n) {
             const double w_skew = nw0 * sx + nw1 * sy + nw2 * sz;
@@ -402,7 +411,7 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
                 tq[c * WAVE + lane] = val;
             }
         }
-        __syncthreads();
+        wave_sync();
         if (lane < 3) {
             double acc = us[lane * gm + sn];
             for (int q = 0; q < deg_n; ++q) acc += -tq[lane * WAVE + q];
@@ -414,13 +423,13 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
             ps[psn] += cpf / dist;
             ps[psv] += -cpf / dist;
         }
-        __syncthreads();
+        wave_sync();
         const double coeff_v = den_f * 0.5 * rc_v / vol_v * face;
         for (int i = lane; i < glen; i += WAVE) {
             const int64_t q = gs + i;
             ps[gsl[i]] += -coeff_v * (g.gc_c[q] * nx + g.gc_c[G + q] * ny + g.gc_c[2 * G + q] * nz);
         }
-        __syncthreads();
+        wave_sync();
         const double coeff_n = den_f * 0.5 * rc_n / vol_n * face;
         if (lane < glen_n) ps[find_slot(pc, plen, gv0)] += -coeff_n * (gx0 * nx + gy0 * ny + gz0 * nz);
         if (lane + WAVE < glen_n) ps[find_slot(pc, plen, gv1)] += -coeff_n * (gx1 * nx + gy1 * ny + gz1 * nz);
@@ -428,7 +437,7 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
             const int64_t q = gn + i;
             ps[find_slot(pc, plen, g.gc_vid[q])] += -coeff_n * (g.gc_c[q] * nx + g.gc_c[G + q] * ny + g.gc_c[2 * G + q] * nz);
         }
-        __syncthreads();
+        wave_sync();
         bacc += bterm;
     }
     // boundary sub-faces of a node: -(w_m rho_m A) n_c on vertices v, j, k
@@ -445,7 +454,7 @@ __global__ void __launch_bounds__(WAVE) k_flow_continuity_rows(FlowAsm a) {
             us[lane * gm + find_slot(gcv, glen, vk)] += -ck * n_c;
         }
     }
-    __syncthreads();
+    wave_sync();
     for (int i = lane; i < glen; i += WAVE) {
         a.vals[rs + i] = us[i];
         a.vals[rs + glen + i] = us[gm + i];

@@ -933,7 +933,7 @@ __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const
 // higher tiles).  Every such role comes earlier in the list, so its workgroup started before:
 // the earliest unfinished role waits for nothing unfinished and every wait ends.  The flag of
 // tile k reads 2 app + 1 after its forward sweep and 2 app + 2 after its backward sweep; a
-// role polls a flag at most TILE_SPIN_MAX times, then writes NaN into its z (the Krylov solve
+// role polls a flag at most wait_limit times (2^17 unless set), then writes NaN into its z (the Krylov solve
 // stops with a non-finite result) and counts the lapse.  The XCDs' L2 caches are not coherent
 // with each other, and device-scope release / acquire fences write back / invalidate a whole
 // L2 (which costs a thousand times the sweep), so the sweeps' z and the flags are written and
@@ -943,7 +943,6 @@ __global__ void __launch_bounds__(BLK) k_ilu_upper(int32_t t0, int32_t t1, const
 constexpr int TBLK = 512;
 constexpr int TSTAGE = 10;
 constexpr int TILE_FWD = 1, TILE_BWD = 2;
-constexpr int TILE_SPIN_MAX = 1 << 17;
 static_assert(ILU_TILE_RMAX <= TBLK, "a tile's rows take one thread each");
 template <class T>
 struct TileChain {
@@ -956,7 +955,7 @@ struct TileChain {
     T* z;
     const int* ctrl;
     unsigned long long* sync;  // [0] tickets taken, [1] lapsed waits, [2 + k] completion of tile k
-    int32_t nrole, nlev, emax, rmax;
+    int32_t nrole, nlev, emax, rmax, wait_limit;
 };
 // the entries [base, base + ne) of a triangle, TSTAGE per thread (ne <= TBLK * TSTAGE)
 template <class T>
@@ -1029,14 +1028,14 @@ __device__ inline T tile_row_inside(const T* sv, const int16_t* sc, const T* zl,
 // wait for the completion flags of deps [d0, d1) (one thread each, d1 - d0 <= TBLK); true when
 // a wait lapsed
 __device__ inline bool tile_wait(const int32_t* __restrict__ dep, int32_t d0, int32_t d1, unsigned long long* sync,
-                                 unsigned long long app, int* fail) {
+                                 unsigned long long app, int32_t limit, int* fail) {
     const int32_t i = d0 + (int32_t)threadIdx.x;
     if (i < d1) {
         const int32_t d = dep[i];
         const unsigned long long want = 2 * app + 2 - (unsigned long long)(d & 1);
         unsigned long long* f = sync + 2 + (d >> 1);
-        for (int spins = 0; __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want; ++spins) {
-            if (spins >= TILE_SPIN_MAX) {
+        for (int spins = 0; limit < 0 || __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want; ++spins) {
+            if (spins >= limit) {
                 *fail = 1;
                 break;
             }
@@ -1084,7 +1083,7 @@ __global__ void __launch_bounds__(TBLK) k_ilu_tile_chain(TileChain<T> a) {
         }
         if (tid == 0) lp[nr] = (int16_t)(le - lb);
         tile_load<T>(a.lcol, a.lval, lb, le - lb, r0, j, v);
-        fail = tile_wait(a.dep, d0, d1, a.sync, app, &s_fail);
+        fail = tile_wait(a.dep, d0, d1, a.sync, app, a.wait_limit, &s_fail);
         tile_place<T>(a.z, r0, r1, le - lb, true, j, v, sv, sc);
         __syncthreads();
         int range = 0;
@@ -1127,7 +1126,7 @@ __global__ void __launch_bounds__(TBLK) k_ilu_tile_chain(TileChain<T> a) {
             }
             if (tid == 0) lp[nr] = (int16_t)(ue - ub);
             tile_load<T>(a.ucol, a.uval, ub, ue - ub, r0, j, v);
-            fail = tile_wait(a.dep, d0, d1, a.sync, app, &s_fail);
+            fail = tile_wait(a.dep, d0, d1, a.sync, app, a.wait_limit, &s_fail);
             if (tid < nr) zv = z_load<T>(a.z + r0 + tid);
             tile_place<T>(a.z, r0, r1, ue - ub, false, j, v, sv, sc);
             __syncthreads();
@@ -1661,7 +1660,7 @@ int ilu_apply(Ctx& c, const SysView& sy, const T* lu, const T* rhs, const double
         const double bytes = (I.sweep_lent + I.sweep_uent) * (vb + 4.0) + I.sweep_rows * (8.0 + 5.0 * vb) -
                              I.fused_rows * 2.0 * vb;
         TileChain<T> a{I.trole.p, I.lev_row.p, I.tdep.p, I.lptr.p, I.lcol.p, I.uptr.p, I.ucol.p, lval, uval, dval, rhs,
-                       rscale, z, ctrl, I.tsync.p, I.nrole, I.nlev, std::max(I.emax_l, I.emax_u), I.rmax};
+                       rscale, z, ctrl, I.tsync.p, I.nrole, I.nlev, std::max(I.emax_l, I.emax_u), I.rmax, I.wait_limit};
         const size_t lds = (size_t)tile_lds_bytes(sizeof(T), std::max(I.emax_l, I.emax_u), I.rmax);
         launch_timed_lds(c, VFVM_K_PRECOND, bytes, k_ilu_tile_chain<T>, dim3(I.nrole), dim3(TBLK), lds, a);
         VFVM_HIP_TRY(hipGetLastError());
@@ -2008,6 +2007,22 @@ int ilu_order(Ctx& c, int32_t* perm, int32_t* color_ptr, int32_t* ncolors) {
     if (ncolors) *ncolors = nc;
     if (color_ptr) std::memcpy(color_ptr, I.h_color_ptr.data(), I.h_color_ptr.size() * sizeof(int32_t));
     if (perm) VFVM_HIP_TRY(hipMemcpy(perm, I.perm.p, sy.n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return VFVM_OK;
+}
+
+int ilu_wait_limit(Ctx& c, int32_t polls, uint64_t* lapsed) {
+    IluData& I = c.ilu;
+    if (lapsed) {
+        *lapsed = 0;
+        if (I.ready && I.tiled) {
+            VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
+            VFVM_HIP_TRY(hipMemcpy(lapsed, I.tsync.p + 1, sizeof(uint64_t), hipMemcpyDeviceToHost));
+        }
+    }
+    if (polls != I.wait_limit) {
+        I.wait_limit = polls;
+        ++I.generation;  // captured cycles bake the limit in
+    }
     return VFVM_OK;
 }
 

@@ -179,6 +179,7 @@ SIGNATURES = [
     ("vfvm_precond_apply", _I, [_P, C.POINTER(SolveOpts), _PD, _PD]),
     ("vfvm_ilu_order", _I, [_P, _PI, _PI, _PI]),
     ("vfvm_ilu_tiles", _I, [_P, _PI, _PI, _PI]),
+    ("vfvm_ilu_wait_limit", _I, [_P, C.c_int32, C.POINTER(C.c_uint64)]),
     ("vfvm_steady_solve", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), _I, _D, _D, C.POINTER(DriverStats)]),
     ("vfvm_adaptive_opts_default", None, [_D, _D, C.POINTER(AdaptiveOpts)]),
     ("vfvm_transient_adaptive", _I, [_P, C.POINTER(OpStruct), _I, C.POINTER(SolveOpts), C.POINTER(AdaptiveOpts), STEP_HOOK,

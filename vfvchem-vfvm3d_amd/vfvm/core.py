@@ -470,6 +470,13 @@ class Context:
         check(lib.vfvm_ilu_order(self._h, ptr(perm, C.c_int32), ptr(cptr, C.c_int32), C.byref(nc)))
         return perm, cptr[:nc.value + 1].copy()
 
+    def ilu_wait_limit(self, polls):
+        """Sets the polls of a one-launch ILU(0) sweep's wait for an adjacent tile (< 0: every such
+        wait gives up at once); returns the number of waits that gave up so far."""
+        lapsed = C.c_uint64(0)
+        check(lib.vfvm_ilu_wait_limit(self._h, int(polls), C.byref(lapsed)))
+        return int(lapsed.value)
+
     def ilu_tiles(self):
         """(tile_ptr, level): the tiles of the tiled ILU(0) order over the permuted rows and the
         level of each position inside its tile (tile_ptr empty for the multicolour order)."""
