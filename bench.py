@@ -20,7 +20,7 @@ And the flow line (lid-driven cavity at Re = 100 on an 80^3 jittered lattice, 2,
 DOF): the coupled velocity-pressure system after a few Picard iterations, solved FP64 and
 FP32-inner / FP64-refined with GMRES(100) + 4x4 block Jacobi.
 
-Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus N with N > 1 exits: the solver runs on one GPU.
 """
 import argparse
 import ctypes as C
@@ -620,7 +620,7 @@ def main():
         print(pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.copy_calibration or 256))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design only)", file=sys.stderr)
         sys.exit(2)
 
