@@ -547,6 +547,42 @@ def advdiff_1d(mesh, Pe):
     return p
 
 
+def random_cube(n_int, n_face, seed=0, device=None):
+    """A mesh of the unit cube on random points (no lattice): n_int uniform interior points,
+    n_face uniform points inside each face, 7 points along each edge and the 8 corners, joined by
+    the Delaunay tetrahedra of scipy (Qhull); boundary groups x0, x1, y0, y1, z0, z1 (an edge or
+    corner node in the first face it lies on)."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    inner = rng.uniform(0.02, 0.98, size=(n_int, 3))
+    nodes, grp = [], []
+    for ax in range(3):
+        for side in (0.0, 1.0):
+            q = rng.uniform(0.02, 0.98, size=(n_face, 3))
+            q[:, ax] = side
+            nodes.append(q)
+            grp += [2 * ax + int(side)] * n_face
+    t = np.linspace(0.0, 1.0, 9)[1:-1]
+    for a in range(3):
+        b, c = [x for x in range(3) if x != a]
+        for s1 in (0.0, 1.0):
+            for s2 in (0.0, 1.0):
+                q = np.zeros((len(t), 3))
+                q[:, a] = t
+                q[:, b] = s1
+                q[:, c] = s2
+                nodes.append(q)
+                grp += [2 * b + int(s1)] * len(t)
+    corners = np.array([[i, j, k] for i in (0.0, 1.0) for j in (0.0, 1.0) for k in (0.0, 1.0)])
+    nodes.append(corners)
+    grp += [int(q[0]) for q in corners]
+    grp = np.array(grp)
+    pos = np.concatenate([inner] + nodes)
+    tets = Delaunay(pos).simplices.astype(np.int32)
+    groups = [np.nonzero(grp == g)[0].astype(np.int32) for g in range(6)]
+    return vfvm.Mesh.from_tets(pos, tets, n_int, groups, device=device)
+
+
 def regroup_lattice(n, groups_of, seed=0, jitter=0.3, device=None):
     """The jittered lattice mesh with its nodes regrouped into boundaries: groups_of(pos of the
     nodes, lattice boundary of each node) returns a group index per node."""

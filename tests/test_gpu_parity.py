@@ -13,7 +13,7 @@ import scipy.sparse.linalg as spla
 
 import vfvm
 import vfvm_oracle as O
-from problems import heat_c1, mass_c2
+from problems import heat_c1, mass_c2, random_cube
 
 pytestmark = pytest.mark.gpu
 
@@ -253,6 +253,33 @@ def test_ilu0_one_launch_sweeps_at_size(ctx, prob, n):
     assert np.abs(z32[0] - z_ref).max() <= 1e-4 * np.abs(z_ref).max()
     for z in z32[1:]:
         assert np.array_equal(z, z32[0])
+
+
+def test_ilu0_one_launch_sweeps_random_points(ctx):
+    """The tiled order and the one-launch sweeps on a mesh of random points (irregular pencils,
+    tile sizes, tile graph and levels): M^-1 r against the oracle's ILU(0) in the device's order,
+    bitwise repeatable, and the ILU(0)-preconditioned solve against SuperLU."""
+    print("[ilu0 one-launch sweeps, random points]", flush=True)
+    mesh = random_cube(20000, 600, seed=3)
+    p = heat_c1(mesh)
+    rp, col, vals, b = _assembled(ctx, p)
+    perm, cptr = ctx.ilu_order()
+    tptr, level = ctx.ilu_tiles()
+    assert len(tptr) - 1 >= 50 and len(cptr) - 1 >= 3, (len(tptr), len(cptr))
+    n = len(rp) - 1
+    A = sp.csr_matrix((vals, col, rp), shape=(n, n))
+    L, U = O.ilu0_rows(A, perm)
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), perm)), shape=(n, n))
+    r = np.random.default_rng(8).normal(size=n)
+    y = spla.spsolve_triangular(L.tocsr(), P @ r, lower=True, unit_diagonal=True)
+    z_ref = P.T @ spla.spsolve_triangular(U.tocsr(), y, lower=False)
+    zs = [ctx.precond_apply(r, precond=vfvm.PRECOND_ILU0, precision=vfvm.FP64) for _ in range(3)]
+    assert np.abs(zs[0] - z_ref).max() <= 1e-12 * np.abs(z_ref).max()
+    for z in zs[1:]:
+        assert np.array_equal(z, zs[0])
+    x, st = ctx.solve(None, tol=1e-12, restart=50, max_outer=400, precond=vfvm.PRECOND_ILU0, precision=vfvm.FP64)
+    x_lu = _lu(rp, col, vals, b)
+    assert np.linalg.norm(x - x_lu) / np.linalg.norm(x_lu) <= 1e-10
 
 
 def test_ilu0_sweep_wait_that_gives_up_stops_the_solve(ctx):
