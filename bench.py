@@ -597,7 +597,8 @@ def main():
     ap.add_argument("--hmf-steps", type=int, default=1)
     ap.add_argument("--hmf-restart", type=int, default=100, help="GMRES restart of the heat-mass-flow solves")
     ap.add_argument("--hmf-inner-tol", type=float, default=1e-6, help="inner tolerance of the mixed heat-mass-flow solve")
-    ap.add_argument("--hmf-every", type=int, default=10, help="checkpoint interval of the heat-mass-flow adjoint")
+    ap.add_argument("--hmf-every", type=int, default=2, help="checkpoint interval of the heat-mass-flow adjoint "
+                    "(51 states of 48 MB at 100 steps)")
     ap.add_argument("--hmf-adj-tol", type=float, default=1e-8, help="outer tolerance of the heat-mass-flow adjoint solves")
     ap.add_argument("--hmf-adj-inner-tol", type=float, default=1e-2, help="tolerance of the transposed solves (outer preconditioner)")
     ap.add_argument("--hmf-adj-inner-restart", type=int, default=30, help="GMRES restart of the transposed solves")

@@ -1095,15 +1095,23 @@ extern "C" int vfvm_flow_adjoint_backward(vfvm_ctx* ctx, const vfvm_flow_adjoint
         const int32_t K = tp.every, NT = tp.num_ts;
         const int32_t s_last = (NT - 1) / K;
         const int64_t nck = NT / K + 1;
+        bool verified = false;  // one replayed segment is run to its end and checked against the next checkpoint
         for (int32_t sgi = s_last; sgi >= 0; --sgi) {
             const int32_t base = sgi * K;
             const int32_t count = std::min(K, NT - base);
             if (!(tp.seg_first == base && tp.seg_count == count)) {
-                // replay the segment from its checkpoint with the forward driver's steps
+                // replay the segment from its checkpoint with the forward driver's steps; its last
+                // state is the next checkpoint, copied instead of recomputed (except in the one
+                // segment replayed to the end, whose last state is compared with the checkpoint)
                 VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
                 auto r0 = clk::now();
+                const bool has_next = count == K && sgi + 1 < nck;
+                const bool verify = has_next && !verified;
+                const int32_t upto = has_next && !verify ? count - 1 : count;
                 if (int st = copy_d2d(c, tp.seg.p, tp.ckpt.p + (size_t)sgi * n, n)) return st;
-                for (int32_t k = 1; k <= count; ++k) {
+                if (upto < count)
+                    if (int st = copy_d2d(c, tp.seg.p + (size_t)count * n, tp.ckpt.p + (size_t)(sgi + 1) * n, n)) return st;
+                for (int32_t k = 1; k <= upto; ++k) {
                     if (int st = flow_unpack(c, tp.seg.p + (size_t)(k - 1) * n)) return st;
                     if (int st = update_prev_all(c)) return st;
                     vfvm_driver_stats ds;
@@ -1115,10 +1123,11 @@ extern "C" int vfvm_flow_adjoint_backward(vfvm_ctx* ctx, const vfvm_flow_adjoint
                     ++rec_steps;
                     if (int st = flow_pack(c, tp.seg.p + (size_t)k * n)) return st;
                 }
-                if (count == K && sgi + 1 < nck) {
+                if (verify) {
                     int64_t mm = 0;
                     if (int st = count_neq(c, tp.seg.p + (size_t)count * n, tp.ckpt.p + (size_t)(sgi + 1) * n, n, &mm)) return st;
                     mismatches += mm;
+                    verified = true;
                 }
                 VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
                 t_rec += secs(r0, clk::now());
