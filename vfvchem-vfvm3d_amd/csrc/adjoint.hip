@@ -521,15 +521,23 @@ extern "C" int vfvm_adjoint_backward(vfvm_ctx* ctx, const vfvm_solve_opts* opts,
         const int32_t K = tp.every, NT = tp.num_ts;
         const int32_t s_last = (NT - 1) / K;
         const int64_t nck = NT / K + 1;
+        bool verified = false;  // one replayed segment is run to its end and checked against the next checkpoint
         for (int32_t sgi = s_last; sgi >= 0; --sgi) {
             const int32_t base = sgi * K;
             const int32_t count = std::min(K, NT - base);
             if (!(tp.seg_first == base && tp.seg_count == count)) {
-                // replay the segment from its checkpoint, exactly as the forward pass ran it
+                // replay the segment from its checkpoint, exactly as the forward pass ran it; its
+                // last state is the next checkpoint, copied instead of recomputed (except in the
+                // one segment replayed to the end, whose last state is compared with the checkpoint)
                 VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
                 auto r0 = clk::now();
+                const bool has_next = count == K && sgi + 1 < nck;
+                const bool verify = has_next && !verified;
+                const int32_t upto = has_next && !verify ? count - 1 : count;
                 if (int st = copy_d2d(c, tp.seg.p, tp.ckpt.p + (size_t)sgi * n, n)) return st;
-                for (int32_t k = 1; k <= count; ++k) {
+                if (upto < count)
+                    if (int st = copy_d2d(c, tp.seg.p + (size_t)count * n, tp.ckpt.p + (size_t)(sgi + 1) * n, n)) return st;
+                for (int32_t k = 1; k <= upto; ++k) {
                     if (int st = copy_d2d(c, u.value_prev.p, tp.seg.p + (size_t)(k - 1) * n, n)) return st;
                     if (int st = copy_d2d(c, u.value.p, tp.seg.p + (size_t)(k - 1) * n, n)) return st;
                     if (int st = assemble_impl(c, ops.data(), nops, tp.dt)) return st;
@@ -540,10 +548,11 @@ extern "C" int vfvm_adjoint_backward(vfvm_ctx* ctx, const vfvm_solve_opts* opts,
                     ++rec_steps;
                     if (int st = copy_d2d(c, tp.seg.p + (size_t)k * n, c.x64.p, n)) return st;
                 }
-                if (count == K && sgi + 1 < nck) {
+                if (verify) {
                     int64_t mm = 0;
                     if (int st = count_neq(c, tp.seg.p + (size_t)count * n, tp.ckpt.p + (size_t)(sgi + 1) * n, n, &mm)) return st;
                     mismatches += mm;
+                    verified = true;
                 }
                 VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
                 t_rec += secs(r0, clk::now());
