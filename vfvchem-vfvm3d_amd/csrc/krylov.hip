@@ -232,6 +232,27 @@ __global__ void __launch_bounds__(BLK) k_spmv_vblock(int64_t nv, const int32_t* 
 #pragma unroll
     for (int c = 0; c < S; ++c) acc[c] = 0;
     int32_t k = b[0] + lane;
+    // eight entries in flight per lane (all column, value and x loads issued before the first
+    // product), then two; each lane still adds its entries in ascending order
+    constexpr int U = 8;
+    for (; k + (U - 1) * WAVE < b[S]; k += U * WAVE) {
+        int32_t cu[U];
+        T au[U], xu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            cu[u] = col[k + u * WAVE];
+            au[u] = val[k + u * WAVE];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xu[u] = x[cu[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const T pu = au[u] * xu[u];
+            const int32_t ku = k + u * WAVE;
+#pragma unroll
+            for (int c = 0; c < S; ++c) acc[c] += (ku >= b[c] && ku < b[c + 1]) ? pu : (T)0;
+        }
+    }
     for (; k + WAVE < b[S]; k += 2 * WAVE) {
         const int32_t c0 = col[k], c1 = col[k + WAVE];
         const T a0 = val[k], a1 = val[k + WAVE];
