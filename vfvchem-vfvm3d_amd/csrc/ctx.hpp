@@ -189,7 +189,7 @@ struct SysView {
     // generic CSR whose rows, columns and vectors are numbered vertex-interleaved (unknown c of
     // vertex v at (n / N) v + c): block Jacobi reads and writes its blocks there
     bool interleaved = false;
-    // most entries held by one block of SPMV_ROWS consecutive rows (0: the per-row SpMV kernel)
+    // most entries held by one chunk of SPMV_ROWS consecutive rows (0: the per-row SpMV kernel)
     int32_t spmv_emax = 0;
 };
 

@@ -123,14 +123,18 @@ __global__ void __launch_bounds__(BLK) k_spmv(int64_t n, const int32_t* __restri
     if (lane == 0) y[row] = (T)(s * sc);
 }
 
-// SpMV over blocks of SPMV_ROWS consecutive rows, one workgroup each: the block's entries are
-// streamed with SPMV_STAGE independent column / value loads and x gathers in flight per thread
-// (the per-row kernel keeps two per lane), their products kept in LDS, and each row then sums
-// its products in column order.  emax: most entries of a block (the LDS capacity).
+// SpMV over chunks of SPMV_ROWS consecutive rows, SPMV_CHUNKS consecutive chunks per workgroup.
+// A chunk's entries are read with STAGE independent column / value loads per thread (the
+// whole chunk in one trip: STAGE * SPMV_BLK >= emax), the x entries gathered, the products
+// kept in LDS and each row then sums its products in column order.  The next chunk's column
+// and value loads are issued before the current chunk's products are stored and summed, so
+// the HBM stream of the workgroup does not stop during the gathers, the LDS phase and the
+// barriers (one chunk per workgroup leaves it idle there: 22% slower FP32, 17% FP64 at
+// 16 entries per row).  emax: most entries of a chunk (the LDS capacity).
 constexpr int SPMV_ROWS = 128;
 constexpr int SPMV_BLK = 256;
-constexpr int SPMV_STAGE = 8;
-template <class T>
+constexpr int SPMV_CHUNKS = 2;
+template <class T, int STAGE>
 __global__ void __launch_bounds__(SPMV_BLK) k_spmv_chunk(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                          const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
                                                          const double* __restrict__ scale, const int* __restrict__ ctrl,
@@ -138,42 +142,58 @@ __global__ void __launch_bounds__(SPMV_BLK) k_spmv_chunk(int64_t n, const int32_
     extern __shared__ __align__(16) unsigned char lds[];
     T* prod = reinterpret_cast<T*>(lds);
     int32_t* lrp = reinterpret_cast<int32_t*>(prod + emax);
-    const int64_t r0 = (int64_t)xcd_block() * SPMV_ROWS;
-    const int64_t r1 = r0 + SPMV_ROWS < n ? r0 + SPMV_ROWS : n;
-    const int nr = (int)(r1 - r0);
+    const int64_t nchunk = (n + SPMV_ROWS - 1) / SPMV_ROWS;
+    const int64_t c0 = (int64_t)xcd_block() * SPMV_CHUNKS;
     const int go = ctrl_go(ctrl);
-    const int32_t base = rp[r0], ne = rp[r1] - base;
     const T sc = (T)scale_of(scale);
-    if (ctrl_stop(go, (base <= base + ne) | (sc != sc))) return;
-    for (int i = threadIdx.x; i <= nr; i += SPMV_BLK) lrp[i] = rp[r0 + i] - base;
-    for (int e0 = 0; e0 < ne; e0 += SPMV_BLK * SPMV_STAGE) {
-        int32_t j[SPMV_STAGE];
-        T a[SPMV_STAGE];
+    int32_t j[STAGE];
+    T a[STAGE];
+    int32_t base = 0, ne = 0, nr = 0, myrp = 0;  // myrp: rp of this thread's row of the chunk
+    auto load = [&](int64_t c) {
+        const int64_t r0 = c * SPMV_ROWS, r1 = r0 + SPMV_ROWS < n ? r0 + SPMV_ROWS : n;
+        nr = (int)(r1 - r0);
+        base = rp[r0];
+        ne = rp[r1] - base;
+        if ((int)threadIdx.x <= nr) myrp = rp[r0 + threadIdx.x];
 #pragma unroll
-        for (int u = 0; u < SPMV_STAGE; ++u) {
-            const int e = e0 + u * SPMV_BLK + threadIdx.x;
+        for (int u = 0; u < STAGE; ++u) {
+            const int e = u * SPMV_BLK + threadIdx.x;
             j[u] = e < ne ? col[base + e] : 0;
             a[u] = e < ne ? val[base + e] : (T)0;
         }
-        T xv[SPMV_STAGE];
+    };
+    load(c0);
+    if (ctrl_stop(go, (base <= base + ne) | (sc != sc))) return;
+    for (int k = 0; k < SPMV_CHUNKS; ++k) {
+        const int64_t c = c0 + k;
+        if (c >= nchunk) break;
+        T xv[STAGE];
 #pragma unroll
-        for (int u = 0; u < SPMV_STAGE; ++u) xv[u] = x[j[u]];
+        for (int u = 0; u < STAGE; ++u) xv[u] = x[j[u]];
+        T pr[STAGE];
 #pragma unroll
-        for (int u = 0; u < SPMV_STAGE; ++u) {
-            const int e = e0 + u * SPMV_BLK + threadIdx.x;
-            if (e < ne) prod[e] = a[u] * xv[u];
+        for (int u = 0; u < STAGE; ++u) pr[u] = a[u] * xv[u];
+        const int cne = ne, cnr = nr;
+        const int32_t crp = myrp - base;
+        if (k + 1 < SPMV_CHUNKS && c + 1 < nchunk) load(c + 1);
+        if ((int)threadIdx.x <= cnr) lrp[threadIdx.x] = crp;
+#pragma unroll
+        for (int u = 0; u < STAGE; ++u) {
+            const int e = u * SPMV_BLK + threadIdx.x;
+            if (e < cne) prod[e] = pr[u];
         }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nr; i += SPMV_BLK) {
-        const int q1 = lrp[i + 1];
-        T s = 0;
+        __syncthreads();
+        if ((int)threadIdx.x < cnr) {
+            const int q1 = lrp[threadIdx.x + 1];
+            T s = 0;
 #pragma unroll 4
-        for (int q = lrp[i]; q < q1; ++q) s += prod[q];
-        y[r0 + i] = (T)(s * sc);
+            for (int q = lrp[threadIdx.x]; q < q1; ++q) s += prod[q];
+            y[c * SPMV_ROWS + threadIdx.x] = (T)(s * sc);
+        }
+        __syncthreads();
     }
 }
-// most entries of a block of SPMV_ROWS rows of the CSR row pointers rp
+// most entries of a chunk of SPMV_ROWS consecutive rows of the CSR row pointers rp
 inline int32_t spmv_block_emax(const std::vector<int32_t>& rp) {
     const int64_t n = (int64_t)rp.size() - 1;
     int32_t m = 0;
@@ -183,9 +203,8 @@ inline int32_t spmv_block_emax(const std::vector<int32_t>& rp) {
     }
     return m;
 }
-// LDS of a k_spmv_chunk block, and the most it may take
+// LDS of a k_spmv_chunk block (at most 12 * SPMV_BLK products: 25 KB in FP64)
 inline size_t spmv_chunk_lds(int vb, int64_t emax) { return (size_t)(vb * emax + 4 * (SPMV_ROWS + 1)); }
-constexpr size_t SPMV_CHUNK_LDS = 48 * 1024;
 
 // SpMV of the coupled flow system [u, v, w, p] (rows and columns comp-major), one wave per
 // vertex v computing its four rows together.  The three velocity rows of a point share their
@@ -1491,9 +1510,20 @@ int spmv_launch(Ctx& c, const SysView& sy, int lpr, int64_t n, const T* val, con
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }
-    if (sy.spmv_emax > 0 && lpr <= 8 && spmv_chunk_lds(sizeof(T), sy.spmv_emax) <= SPMV_CHUNK_LDS) {
-        launch_timed_lds(c, VFVM_K_SPMV, bytes, k_spmv_chunk<T>, dim3((unsigned)((n + SPMV_ROWS - 1) / SPMV_ROWS)), dim3(SPMV_BLK),
-                         spmv_chunk_lds(sizeof(T), sy.spmv_emax), n, rp, cl, val, x, y, scale, ctrl, sy.spmv_emax);
+    if (sy.spmv_emax > 0 && lpr <= 8 && sy.spmv_emax <= SPMV_BLK * 12) {
+        // one trip per chunk: STAGE * SPMV_BLK >= emax
+        const int64_t nchunk = (n + SPMV_ROWS - 1) / SPMV_ROWS;
+        const dim3 g((unsigned)((nchunk + SPMV_CHUNKS - 1) / SPMV_CHUNKS));
+        const size_t lds = spmv_chunk_lds(sizeof(T), sy.spmv_emax);
+        const int32_t em = sy.spmv_emax;
+        if (em <= SPMV_BLK * 8)
+            launch_timed_lds(c, VFVM_K_SPMV, bytes, k_spmv_chunk<T, 8>, g, dim3(SPMV_BLK), lds, n, rp, cl, val, x, y, scale, ctrl, em);
+        else if (em <= SPMV_BLK * 9)
+            launch_timed_lds(c, VFVM_K_SPMV, bytes, k_spmv_chunk<T, 9>, g, dim3(SPMV_BLK), lds, n, rp, cl, val, x, y, scale, ctrl, em);
+        else if (em <= SPMV_BLK * 10)
+            launch_timed_lds(c, VFVM_K_SPMV, bytes, k_spmv_chunk<T, 10>, g, dim3(SPMV_BLK), lds, n, rp, cl, val, x, y, scale, ctrl, em);
+        else
+            launch_timed_lds(c, VFVM_K_SPMV, bytes, k_spmv_chunk<T, 12>, g, dim3(SPMV_BLK), lds, n, rp, cl, val, x, y, scale, ctrl, em);
         VFVM_HIP_TRY(hipGetLastError());
         return VFVM_OK;
     }
