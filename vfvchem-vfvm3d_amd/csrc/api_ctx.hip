@@ -337,6 +337,8 @@ extern "C" void vfvm_destroy(vfvm_ctx* ctx) {
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     for (auto e : c.ev_pool) (void)hipEventDestroy(e);
+    for (auto e : c.ctrl_ev)
+        if (e) (void)hipEventDestroy(e);
     for (auto& g : c.graphs)
         for (auto e : g.chunks) (void)hipGraphExecDestroy(e);
     if (c.h_pinned) (void)hipHostFree(c.h_pinned);

@@ -356,6 +356,8 @@ struct Ctx {
     bool graphs_enabled = true;
     uint64_t graph_clock = 0;
     std::vector<CycleGraphs> graphs;
+    // completion of the control-word readback of the two most recent cycle-graph chunks
+    hipEvent_t ctrl_ev[2] = {nullptr, nullptr};
 
     // timing
     uint32_t timing_mask = 0;  // bit k: time kernel class k

@@ -1927,16 +1927,40 @@ int arnoldi_cycle(Ctx& c, const SysView& sy, Work& w, T* V, T* wv, T* zv, const 
         cg = cycle_graphs<T>(c, sy, w, V, wv, zv, aval, pc, reorth, &status);
         if (status != VFVM_OK) return status;
     }
-    for (int j0 = 0; j0 < w.m; j0 += CHECK) {
-        if (cg) {
-            VFVM_HIP_TRY(hipGraphLaunch(cg->chunks[j0 / CHECK], c.stream));
-        } else {
-            for (int j = j0; j < std::min(w.m, j0 + CHECK); ++j) {
-                c.cur_step = j;
-                if (int st = issue_step<T>(c, sy, w, V, wv, zv, aval, pc, reorth, j)) return st;
-            }
-            c.cur_step = -1;
+    if (cg) {
+        // The next chunk is launched before the host waits for the control word of the
+        // current one, so the GPU does not idle while the host reads it back and launches;
+        // after convergence the queued chunk's kernels return at once.  Each chunk's control
+        // word is read back into its own pinned slot (slot q % 2, 256 bytes apart).
+        for (auto& e : c.ctrl_ev)
+            if (!e) VFVM_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        const int nchunk = (w.m + CHECK - 1) / CHECK;
+        auto issue = [&](int q) -> int {
+            VFVM_HIP_TRY(hipGraphLaunch(cg->chunks[q], c.stream));
+            VFVM_HIP_TRY(hipMemcpyAsync(w.h_ctrl + 64 * (q % 2), w.ctrl, CTRL_INTS * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+            VFVM_HIP_TRY(hipEventRecord(c.ctrl_ev[q % 2], c.stream));
+            return VFVM_OK;
+        };
+        if (int st = issue(0)) return st;
+        const int* hc = w.h_ctrl;
+        for (int q = 0; q < nchunk; ++q) {
+            if (q + 1 < nchunk)
+                if (int st = issue(q + 1)) return st;
+            VFVM_HIP_TRY(hipEventSynchronize(c.ctrl_ev[q % 2]));
+            hc = w.h_ctrl + 64 * (q % 2);
+            steps = hc[1];
+            if (hc[0] == 0) break;
         }
+        // the caller reads the last control word from slot 0 (a chunk still queued after
+        // convergence leaves the control word as it is, so its readback writes the same ints)
+        if (hc != w.h_ctrl) std::memcpy(w.h_ctrl, hc, CTRL_INTS * sizeof(int));
+    }
+    for (int j0 = 0; !cg && j0 < w.m; j0 += CHECK) {
+        for (int j = j0; j < std::min(w.m, j0 + CHECK); ++j) {
+            c.cur_step = j;
+            if (int st = issue_step<T>(c, sy, w, V, wv, zv, aval, pc, reorth, j)) return st;
+        }
+        c.cur_step = -1;
         VFVM_HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, CTRL_INTS * sizeof(int), hipMemcpyDeviceToHost, c.stream));
         VFVM_HIP_TRY(hipStreamSynchronize(c.stream));
         steps = w.h_ctrl[1];
