@@ -70,6 +70,33 @@ def test_solve_matrix_vs_lu(ctx, precision, precond, bar):
     assert np.linalg.norm(x2 - spla.spsolve(A, b2)) <= bar * np.linalg.norm(spla.spsolve(A, b2))
 
 
+@pytest.mark.parametrize("per_row", [6, 12, 17, 19, 22, 27])
+def test_solve_matrix_spmv_chunk_widths(ctx, per_row):
+    """Random diagonally dominant matrices whose 128-row chunks need 8, 9, 10 or 12 column /
+    value loads per thread of the block SpMV (6-22 entries per row; 27 per row takes the
+    per-row kernel), 41 full chunks and a ragged last one (an odd count: the last workgroup
+    takes one chunk): FP64 and refined mixed precision against SuperLU."""
+    print(f"[linear base spmv chunk widths per_row={per_row}]", flush=True)
+    n = 128 * 41 + 37
+    rng = np.random.default_rng(per_row)
+    k = per_row - 1
+    r = np.repeat(np.arange(n), k)
+    c = rng.integers(0, n, size=n * k)
+    v = rng.uniform(-1.0, 1.0, size=n * k)
+    d = np.bincount(r, weights=np.abs(v), minlength=n) + 1.0
+    rows, cols, vals = np.r_[r, np.arange(n)], np.r_[c, np.arange(n)], np.r_[v, d]
+    ctx.linear_update_sparse(n, rows, cols)
+    ctx.linear_build_matrix(vals)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n)).tocsc()
+    b = rng.standard_normal(n)
+    xl = spla.spsolve(A, b)
+    for precision, bar in ((vfvm.FP64, 1e-10), (vfvm.FP32_REFINED, 1e-8)):
+        x, st = ctx.linear_solve_matrix(b, None, tol=1e-13, restart=50, max_outer=200, precision=precision,
+                                        precond=vfvm.PRECOND_JACOBI)
+        assert st.rel_residual <= 1e-13
+        assert np.linalg.norm(x - xl) <= bar * np.linalg.norm(xl)
+
+
 def test_linear_base_errors(ctx):
     with pytest.raises(vfvm.VFVChemError) as e:
         ctx.linear_update_sparse(3, [0, 1, 3], [0, 1, 2])  # row 3 of a 3 x 3 matrix
