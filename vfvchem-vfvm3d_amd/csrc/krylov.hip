@@ -749,7 +749,7 @@ __global__ void __launch_bounds__(BLK) k_cycle_start(int nblk, const double* __r
 // product summed across the lanes in a fixed order
 __global__ void __launch_bounds__(WAVE) k_backsolve(double* __restrict__ S, StateLayout L, const int* __restrict__ ctrl) {
     __shared__ double Hs[64 * 64];
-    __shared__ double ys[MAX_RESTART];
+    __shared__ double ys[MAX_RESTART], gs[MAX_RESTART], ds[MAX_RESTART];
     const int k = ctrl[1];
     const int m = L.m;
     const int lane = threadIdx.x;
@@ -762,6 +762,11 @@ __global__ void __launch_bounds__(WAVE) k_backsolve(double* __restrict__ S, Stat
             const int l = idx / k, i = idx % k;
             Hs[l * 64 + i] = H[(int64_t)l * (m + 1) + i];
         }
+    // g and the diagonal staged too: a global load per row would sit on the serial chain
+    for (int i = lane; i < k; i += WAVE) {
+        gs[i] = g[i];
+        ds[i] = H[(int64_t)i * (m + 1) + i];
+    }
     __syncthreads();
     for (int i = k - 1; i >= 0; --i) {
         double p = 0.0;
@@ -769,8 +774,8 @@ __global__ void __launch_bounds__(WAVE) k_backsolve(double* __restrict__ S, Stat
             p += (staged ? Hs[l * 64 + i] : H[(int64_t)l * (m + 1) + i]) * ys[l];
         p = wave_sum(p);
         if (lane == 0) {
-            const double d = staged ? Hs[i * 64 + i] : H[(int64_t)i * (m + 1) + i];
-            ys[i] = d != 0.0 ? (g[i] - p) / d : 0.0;
+            const double d = ds[i];
+            ys[i] = d != 0.0 ? (gs[i] - p) / d : 0.0;
         }
         __syncthreads();
     }
