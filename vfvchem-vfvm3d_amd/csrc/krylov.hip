@@ -596,11 +596,12 @@ __global__ void __launch_bounds__(BLK) k_reduce(int nblk, const double* __restri
                                                 double* __restrict__ out, const int* __restrict__ ctrl) {
     __shared__ double sh[BLK / WAVE];
     const int k = blockIdx.x;
+    const double sk = sig[k];  // issued with the partials, not after the block sum
     double s = 0.0;
     for (int i = threadIdx.x; i < nblk; i += BLK) s += part[(int64_t)k * nblk + i];
     if (ctrl[0] == 0) return;  // tested after the partials' loads issue: one round trip, not two
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[k] = s * sig[k];
+    if (threadIdx.x == 0) out[k] = s * sk;
 }
 
 // Arnoldi step j (one block): h = h1 (+ h2), h_(j+1,j) = ||w|| from the update's partials,
@@ -618,6 +619,10 @@ __global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int reorth, int nbl
         hc[i] = reorth ? S[L.h1() + i] + S[L.h2() + i] : S[L.h1() + i];
         if (i < j) { cs_s[i] = S[L.cs() + i]; sn_s[i] = S[L.sn() + i]; }
     }
+    double* g = S + L.g();
+    // thread 0's scalars loaded with the partials: after the state stores below they could
+    // not be hoisted, and each would add a memory round trip to the serial chain
+    const double gj = g[j], target = S[L.misc() + 2];
     double s = 0.0;
     for (int i = threadIdx.x; i < nblk; i += BLK) s += part_nrm[i];
     if (ctrl[0] == 0) return;  // tested after the loads above issue (nothing is written before)
@@ -625,12 +630,15 @@ __global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int reorth, int nbl
     if (threadIdx.x == 0) {
         const double hn = sqrt(s);
         hc[j + 1] = hn;
+        // the rotated entry i + 1 is carried in a register to the next rotation (the same
+        // operations as rotating hc in place)
+        double a = hc[0];
         for (int i = 0; i < j; ++i) {
-            const double a = hc[i], b = hc[i + 1];
+            const double b = hc[i + 1];
             hc[i] = cs_s[i] * a + sn_s[i] * b;
-            hc[i + 1] = -sn_s[i] * a + cs_s[i] * b;
+            a = -sn_s[i] * a + cs_s[i] * b;
         }
-        const double a = hc[j], b = hc[j + 1];
+        const double b = hc[j + 1];
         const double r = hypot(a, b);
         const double c = r > 0.0 ? a / r : 1.0;
         const double sgn = r > 0.0 ? b / r : 0.0;
@@ -638,8 +646,6 @@ __global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int reorth, int nbl
         S[L.sn() + j] = sgn;
         hc[j] = r;
         hc[j + 1] = 0.0;
-        double* g = S + L.g();
-        const double gj = g[j];
         g[j + 1] = -sgn * gj;
         g[j] = c * gj;
         const double res = fabs(sgn * gj);
@@ -649,7 +655,7 @@ __global__ void __launch_bounds__(BLK) k_arnoldi_step(int j, int reorth, int nbl
         ctrl[1] = j + 1;
         const bool finite = isfinite(res) && isfinite(hn) && isfinite(r);
         if (!finite) ctrl[2] = 1;
-        if (!finite || res <= S[L.misc() + 2] || hn == 0.0 || r == 0.0) ctrl[0] = 0;
+        if (!finite || res <= target || hn == 0.0 || r == 0.0) ctrl[0] = 0;
     }
     __syncthreads();
     for (int i = threadIdx.x; i <= j + 1; i += BLK) H[cj + i] = hc[i];
